@@ -1,0 +1,213 @@
+/*
+ * voxphoton.h - C ABI of the MI355X-native FFT-VIE solve path.
+ *
+ * Replaces, for one GPU, the three function handles VoxPhoton's MATLAB
+ * scripts hand to GMRES and the GMRES call itself:
+ *   MVP_A  = @(J) mv_AN_const(J,fN,Mr,Mc,Gram,'notransp',idxS3,GPU_flag)
+ *   prec_A = @(J) chan_mvp_idx(...) / chan_2_mvp_for_parallel_idx(...)
+ *   [x,flag,relres,iter,resvec] = gmres(afun,b,restart,tol,maxit,mfun)
+ * (Bragg_MOR_script.m:78-90, resonator_MOR_scripts.m:197-208,
+ *  JVIE_Solver_const.m:282,301-333).
+ *
+ * Conventions
+ *  - Complex numbers are interleaved (re, im) pairs: double[2*n] for
+ *    complex128, float[2*n] for complex64 (MATLAB R2018a interleaved layout).
+ *  - Grid arrays are column-major (MATLAB order): voxel (l,m,n), 0-based,
+ *    sits at l + L*(m + M*n); component c of a 3-vector field at c*V + voxel.
+ *  - Index vectors are 1-based int64 (the MEX gateway converts MATLAB's
+ *    double indices); idx addresses the 3*L*M*N component-major array.
+ *  - Every entry point returns a status (VP_OK = 0, negative on error);
+ *    vp_last_error() returns a thread-local message for the last failure.
+ *    No C++ exception crosses the ABI.
+ *  - Handles are opaque and freed explicitly.  Calls on one handle are
+ *    serialised on its context's HIP stream; separate contexts are
+ *    independent.  mem = VP_MEM_HOST means the pointers are host memory
+ *    (always complex128 for vectors); VP_MEM_DEVICE means device memory in
+ *    the precision named by prec.
+ */
+#ifndef VOXPHOTON_H
+#define VOXPHOTON_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VP_OK 0
+#define VP_ERR_INVALID (-1)
+#define VP_ERR_OOM (-2)
+#define VP_ERR_DEVICE (-3)
+#define VP_ERR_UNSUPPORTED (-4)
+
+#define VP_MEM_HOST 0
+#define VP_MEM_DEVICE 1
+
+#define VP_FP64 1
+#define VP_FP32 2
+
+#define VP_NOTRANSP 0
+#define VP_TRANSP 1
+
+#define VP_PREC_LEFT 0
+#define VP_PREC_RIGHT 1
+
+typedef struct vp_ctx_s* vp_ctx;
+typedef struct vp_op_s* vp_op;
+typedef struct vp_prec_s* vp_prec;
+
+/* ---- library / context --------------------------------------------------- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* vp_last_error(void);
+/* Library version string. */
+const char* vp_version(void);
+/* Number of visible GPUs (0 if none). */
+int vp_device_count(int* count);
+/* Context: one device and one HIP stream (default priority). */
+int vp_ctx_create(int device, vp_ctx* out);
+int vp_ctx_destroy(vp_ctx ctx);
+int vp_ctx_synchronize(vp_ctx ctx);
+/* Device memory helpers for callers without their own allocator. */
+int vp_malloc(vp_ctx ctx, int64_t bytes, void** dptr);
+int vp_free(vp_ctx ctx, void* dptr);
+int vp_memcpy_h2d(vp_ctx ctx, void* dst, const void* src, int64_t bytes);
+int vp_memcpy_d2h(vp_ctx ctx, void* dst, const void* src, int64_t bytes);
+
+/* ---- N operator (replaces mv_AN_const, pc/src_vie/src_mvp/mv_AN_const.m:1) -- */
+
+/*
+ * Creates an operator from the reference's precomputed circulant Green
+ * tensor fN (getOPERATORS_SAM.m:1 first output; Lp x Mp x Np x 6 complex128,
+ * column-major, components xx,xy,xz,yy,yz,zz).  Mr, Mc: L x M x N complex128
+ * (Mr may be NULL for Mr == 1, the JVIE-II case).  gram: the real Gram factor
+ * dx^3.  idx: n one-based indices into the 3*L*M*N field (idxS3).
+ * precisions: VP_FP64, VP_FP32 or both OR-ed; FP32 copies are stored
+ * prescaled by 1/gram (see vp_op_fp32_scale).
+ */
+int vp_op_create(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_t Mp, int64_t Np,
+                 const double* fN, const double* Mr, const double* Mc, double gram, const int64_t* idx,
+                 int64_t n, int precisions, vp_op* out);
+
+/*
+ * Same operator from the Toeplitz tensor opToeplitz (getOPERATORS_SAM.m:1
+ * second output; L x M x N x 6 complex128).  The circulant embedding and
+ * its FFT (circulant_nop_const.m, fft_operator.m) run on the device.
+ * Lp, Mp, Np = 0 selects the reference sizes 2L, 2M, 2N; any sizes >= 2D-1
+ * give the same operator.
+ */
+int vp_op_create_toeplitz(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_t Mp, int64_t Np,
+                          const double* G, const double* Mr, const double* Mc, double gram, const int64_t* idx,
+                          int64_t n, int precisions, vp_op* out);
+
+/*
+ * Assembles the Toeplitz tensor on the device for a uniform grid of voxel
+ * size dx at wavenumber ko (assembly_nop_const.m:1-168 semantics, own
+ * cubature) and creates the operator from it.  gram is dx^3.
+ */
+int vp_op_create_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_t Mp, int64_t Np,
+                           double dx, double ko, const double* Mr, const double* Mc, const int64_t* idx,
+                           int64_t n, int precisions, vp_op* out);
+
+/* Writes the assembled L x M x N x 6 Toeplitz tensor (complex128) to G (mem). */
+int vp_assemble_toeplitz(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko, double* G, int mem);
+
+/*
+ * y = A x ('notransp') or A^H x ('transp'), A = Gram*Mr - Mc*N restricted to
+ * idx (mv_AN_const.m:66-146).  mem = VP_MEM_HOST: x, y are complex128 host
+ * vectors of length n and the result is A x for either prec.  mem =
+ * VP_MEM_DEVICE: x, y are device vectors in precision prec; for VP_FP32 the
+ * result is A x / vp_op_fp32_scale.  x and y must not alias.
+ */
+int vp_op_apply(vp_op op, int transp, const void* x, void* y, int mem, int prec);
+/* Batched apply: nvec device vectors (stride n elements) through ops[i]
+ * (same geometry), one launch per pass. */
+int vp_op_apply_batched(const vp_op* ops, int nvec, int transp, const void* x, void* y, int prec);
+int vp_op_fp32_scale(vp_op op, double* scale);
+int vp_op_size(vp_op op, int64_t* n);
+int vp_op_destroy(vp_op op);
+
+/* ---- preconditioners (pc/src_preconditioning) -------------------------------- */
+
+/*
+ * 2-level block-circulant preconditioner apply (chan_2_mvp_for_parallel_idx.m:2-23)
+ * from precomputed inverse blocks: blocks holds L*M complex128 matrices of
+ * size 3N x 3N, column-major, cell {i,j} (1-based) at offset
+ * ((i-1) + L*(j-1)) * 9*N*N.  idx indexes the 3*L*M*N box field.
+ */
+int vp_prec_create_chan2(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double* blocks, const int64_t* idx,
+                         int64_t n, int precisions, vp_prec* out);
+/* 1-level apply (chan_mvp_idx.m:2-16): L matrices of size 3MN x 3MN. */
+int vp_prec_create_chan1(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double* blocks, const int64_t* idx,
+                         int64_t n, int precisions, vp_prec* out);
+/*
+ * 2-level setup on the device from opToeplitz (level_2_parallel_func.m:1-192
+ * semantics, built without the 3MN x 3MN intermediate): Mc is a scalar
+ * (mc_re, mc_im) contrast, Mr(1) = mr0.  Blocks are inverted on the device.
+ */
+int vp_prec_create_level2(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double* G, double mc_re, double mc_im,
+                          double mr0, double gram, const int64_t* idx, int64_t n, int precisions, vp_prec* out);
+/* Same, from a device-assembled tensor (see vp_op_create_assembled). */
+int vp_prec_create_level2_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko, double mc_re,
+                                    double mc_im, double mr0, const int64_t* idx, int64_t n, int precisions,
+                                    vp_prec* out);
+/* Embeds a preconditioner built on a sub-box into the operator's unknowns:
+ * sub_of_op[i] (1-based, length nsub) is the position in the operator's
+ * unknown vector of the sub-box unknown i; unknowns not covered pass
+ * through unchanged (block-Jacobi, prec_fun_disk_idx.m:1-23). */
+int vp_prec_set_embedding(vp_prec p, int64_t n_outer, const int64_t* sub_of_op, int64_t nsub);
+/* y = M^{-1} x; same mem/prec conventions as vp_op_apply (the FP32 copy of a
+ * preconditioner built for the FP64 operator is scaled by the op's 1/gram so
+ * that it inverts the FP32 operator). */
+int vp_prec_apply(vp_prec p, const void* x, void* y, int mem, int prec);
+int vp_prec_set_fp32_scale(vp_prec p, double scale);
+int vp_prec_destroy(vp_prec p);
+
+/* ---- GMRES (pgmres.m:70-180, pgmres_iter.m:22-150) ----------------------- */
+
+typedef struct {
+  int flag;          /* 0 converged (relres <= tol), 1 not converged */
+  int iter_inner;    /* inner iterations of the last cycle (iter(1)) */
+  int iter_outer;    /* restart cycles (iter(2)) */
+  int total_iters;   /* total Arnoldi steps */
+  double relres;     /* ||b - A x|| / ||b||, FP64, unpreconditioned */
+  double* resvec;    /* caller buffer for the residual history or NULL */
+  int resvec_cap;    /* capacity of resvec */
+  int resvec_len;    /* entries written */
+  int ir_steps;      /* mixed precision: outer refinement steps */
+  double seconds;    /* wall time of the solve */
+} vp_gmres_info;
+
+/*
+ * Restarted GMRES(restart) on A x = b with optional preconditioner applied
+ * on the given side.  Arnoldi uses classical Gram-Schmidt with one
+ * reorthogonalisation pass; the Hessenberg least squares is updated with
+ * Givens rotations in FP64.  prec selects the arithmetic (VP_FP64 or VP_FP32
+ * for the whole Krylov loop).  b, x0 (nullable), x follow mem.
+ */
+int vp_gmres(vp_op op, vp_prec M, int side, const void* b, const void* x0, void* x, int mem, int prec, int restart,
+             double tol, int maxit, vp_gmres_info* info);
+
+/*
+ * Mixed precision: FP64 iterative refinement around FP32 GMRES.  Outer
+ * residual r = b - A x in FP64 on the device; inner FP32 GMRES(restart)
+ * with the FP32 preconditioner solves A d = r to inner_tol; stops when
+ * ||b - A x|| / ||b|| <= tol or after max_outer steps.
+ */
+int vp_solve_ir(vp_op op, vp_prec M, int side, const void* b, void* x, int mem, double tol, int restart,
+                double inner_tol, int inner_maxit, int max_outer, vp_gmres_info* info);
+
+/*
+ * Batched mixed-precision solves: nsolve right-hand sides, solve i through
+ * ops[i] and precs[i] (precs may be NULL or hold NULLs; all ops share one
+ * geometry).  b and x are device complex128 arrays of nsolve*n entries.
+ * Every Krylov step of all active solves runs as one launch per kernel.
+ */
+int vp_solve_ir_batched(const vp_op* ops, const vp_prec* precs, int nsolve, int side, const void* b, void* x,
+                        double tol, int restart, double inner_tol, int inner_maxit, int max_outer,
+                        vp_gmres_info* infos);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VOXPHOTON_H */

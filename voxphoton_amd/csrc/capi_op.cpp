@@ -1,0 +1,460 @@
+// C ABI: library, contexts, device memory helpers and the N operator.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "voxphoton.h"
+#include "op.hpp"
+
+namespace vp {
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace vp
+
+using namespace vp;
+
+#define VP_API_BEGIN try {
+#define VP_API_END                                   \
+  }                                                  \
+  catch (const VpError& e) {                         \
+    set_error(e.msg);                                \
+    return e.code;                                   \
+  }                                                  \
+  catch (const std::bad_alloc&) {                    \
+    set_error("host allocation failed");             \
+    return VP_ERR_OOM;                               \
+  }                                                  \
+  catch (const std::exception& e) {                  \
+    set_error(e.what());                             \
+    return VP_ERR_DEVICE;                            \
+  }                                                  \
+  catch (...) {                                      \
+    set_error("unknown error");                      \
+    return VP_ERR_DEVICE;                            \
+  }                                                  \
+  set_error("");                                     \
+  return VP_OK;
+
+extern "C" {
+
+const char* vp_last_error(void) { return g_last_error.c_str(); }
+const char* vp_version(void) { return "voxphoton-mi355x 0.1 (gfx950)"; }
+
+int vp_device_count(int* count) {
+  VP_API_BEGIN
+  VP_REQUIRE(count, ERR_INVALID, "count is NULL");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *count = c;
+  VP_API_END
+}
+
+int vp_ctx_create(int device, vp_ctx* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(out, ERR_INVALID, "out is NULL");
+  int c = 0;
+  VP_HIP(hipGetDeviceCount(&c));
+  VP_REQUIRE(device >= 0 && device < c, ERR_INVALID, "device index out of range");
+  VP_HIP(hipSetDevice(device));
+  auto* ctx = new vp_ctx_s();
+  ctx->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    VP_HIP(e);
+  }
+  *out = ctx;
+  VP_API_END
+}
+
+int vp_ctx_destroy(vp_ctx ctx) {
+  VP_API_BEGIN
+  if (!ctx) return VP_OK;
+  VP_HIP(hipSetDevice(ctx->device));
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  VP_API_END
+}
+
+int vp_ctx_synchronize(vp_ctx ctx) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx, ERR_INVALID, "ctx is NULL");
+  VP_HIP(hipSetDevice(ctx->device));
+  VP_HIP(hipStreamSynchronize(ctx->stream));
+  VP_API_END
+}
+
+int vp_malloc(vp_ctx ctx, int64_t bytes, void** dptr) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && dptr && bytes >= 0, ERR_INVALID, "vp_malloc: bad argument");
+  VP_HIP(hipSetDevice(ctx->device));
+  *dptr = nullptr;
+  if (bytes) VP_HIP(hipMalloc(dptr, size_t(bytes)));
+  VP_API_END
+}
+
+int vp_free(vp_ctx ctx, void* dptr) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx, ERR_INVALID, "ctx is NULL");
+  VP_HIP(hipSetDevice(ctx->device));
+  if (dptr) VP_HIP(hipFree(dptr));
+  VP_API_END
+}
+
+int vp_memcpy_h2d(vp_ctx ctx, void* dst, const void* src, int64_t bytes) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && dst && src && bytes >= 0, ERR_INVALID, "vp_memcpy_h2d: bad argument");
+  VP_HIP(hipSetDevice(ctx->device));
+  VP_HIP(hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyHostToDevice, ctx->stream));
+  VP_HIP(hipStreamSynchronize(ctx->stream));
+  VP_API_END
+}
+
+int vp_memcpy_d2h(vp_ctx ctx, void* dst, const void* src, int64_t bytes) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && dst && src && bytes >= 0, ERR_INVALID, "vp_memcpy_d2h: bad argument");
+  VP_HIP(hipSetDevice(ctx->device));
+  VP_HIP(hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyDeviceToHost, ctx->stream));
+  VP_HIP(hipStreamSynchronize(ctx->stream));
+  VP_API_END
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------------------
+// Operator construction
+// ----------------------------------------------------------------------------
+namespace vp {
+
+// Validates idx (1-based into the 3V field) and builds the voxel map: for the
+// [idxS; V+idxS; 2V+idxS] form with sorted idxS a V-entry voxel -> position
+// map (structured), otherwise a 3V-entry field -> position map.
+void index_map(int64_t V, const int64_t* idx, int64_t n, std::vector<int>& pos, int& structured_out,
+               int64_t& nS_out) {
+  VP_REQUIRE(idx && n > 0, ERR_INVALID, "idx must hold at least one index");
+  VP_REQUIRE(n <= 3 * V, ERR_INVALID, "more indices than field entries");
+  VP_REQUIRE(3 * V < (int64_t(1) << 31), ERR_UNSUPPORTED, "grid too large for 32-bit voxel maps");
+  for (int64_t i = 0; i < n; ++i)
+    VP_REQUIRE(idx[i] >= 1 && idx[i] <= 3 * V, ERR_INVALID,
+               "idx[" + std::to_string(i) + "] outside 1..3*L*M*N");
+  bool structured = (n % 3 == 0);
+  const int64_t nS = n / 3;
+  if (structured) {
+    for (int64_t i = 0; i < nS && structured; ++i) {
+      if (idx[i] > V) structured = false;
+      else if (i > 0 && idx[i] <= idx[i - 1]) structured = false;
+      else if (idx[nS + i] != idx[i] + V || idx[2 * nS + i] != idx[i] + 2 * V) structured = false;
+    }
+  }
+  if (structured) {
+    structured_out = 1;
+    nS_out = nS;
+    pos.assign(size_t(V), -1);
+    for (int64_t i = 0; i < nS; ++i) pos[size_t(idx[i] - 1)] = int(i);
+  } else {
+    structured_out = 0;
+    nS_out = n;
+    pos.assign(size_t(3 * V), -1);
+    for (int64_t i = 0; i < n; ++i) {
+      VP_REQUIRE(pos[size_t(idx[i] - 1)] < 0, ERR_INVALID, "duplicate index in idx");
+      pos[size_t(idx[i] - 1)] = int(i);
+    }
+  }
+}
+
+// Validates idx and fills the voxel map of an operator.
+static void index_setup(vp_op_s* op, const int64_t* idx, int64_t n, std::vector<int>& pos) {
+  index_map(op->V, idx, n, pos, op->structured, op->nS);
+  op->n = n;
+}
+
+// Gathers a complex128 grid array (L x M x N) to the unknown entries.
+static std::vector<double2> gather_material(const vp_op_s* op, const double* A, const int64_t* idx, double dflt) {
+  const int64_t cnt = op->structured ? op->nS : op->n;
+  std::vector<double2> out(static_cast<size_t>(cnt));
+  for (int64_t i = 0; i < cnt; ++i) {
+    const int64_t v = (idx[i] - 1) % op->V;
+    out[size_t(i)] = A ? make_double2(A[2 * v], A[2 * v + 1]) : make_double2(dflt, 0.0);
+  }
+  return out;
+}
+
+template <typename T>
+static void init_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const std::vector<double2>& mc,
+                      const std::vector<double2>* mr, double gram_t) {
+  P.present = true;
+  build_geometry(P.g, P.plan, *op, fp64, op->ctx->device);
+  upload_twiddles(P, op->ctx->stream);
+  const size_t cnt = mc.size();
+  std::vector<C<T>> h(cnt);
+  for (size_t i = 0; i < cnt; ++i) h[i] = mk<T>(T(mc[i].x), T(mc[i].y));
+  P.mc.alloc(cnt * sizeof(C<T>));
+  VP_HIP(hipMemcpy(P.mc.p, h.data(), cnt * sizeof(C<T>), hipMemcpyHostToDevice));
+  if (mr) {
+    for (size_t i = 0; i < cnt; ++i) h[i] = mk<T>(T((*mr)[i].x), T((*mr)[i].y));
+    P.mr.alloc(cnt * sizeof(C<T>));
+    VP_HIP(hipMemcpy(P.mr.p, h.data(), cnt * sizeof(C<T>), hipMemcpyHostToDevice));
+  }
+  P.gram = T(gram_t);
+  P.work.alloc(size_t(3) * op->Lp * op->M * op->N * sizeof(C<T>));
+  P.xs.alloc(size_t(op->n) * sizeof(C<T>));
+  P.ys.alloc(size_t(op->n) * sizeof(C<T>));
+}
+
+template <typename T>
+static void upload_probs(vp_op_s* op, OpPrec<T>& P) {
+  ProbPtrs<T> pp{};
+  pp.work = P.work.template as<C<T>>();
+  pp.fN = P.fN.template as<C<T>>();
+  pp.pos = op->pos.as<int>();
+  pp.mc = P.mc.template as<C<T>>();
+  pp.mr = P.mr.p ? P.mr.template as<C<T>>() : nullptr;
+  pp.gram = P.gram;
+  P.probs.alloc(sizeof(pp));
+  VP_HIP(hipMemcpy(P.probs.p, &pp, sizeof(pp), hipMemcpyHostToDevice));
+}
+
+// Common part of every vp_op_create*: everything except the tensor.
+static vp_op_s* op_begin(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_t Mp, int64_t Np,
+                         const double* Mr, const double* Mc, double gram, const int64_t* idx, int64_t n,
+                         int precisions) {
+  VP_REQUIRE(ctx, ERR_INVALID, "ctx is NULL");
+  VP_REQUIRE(L > 0 && M > 0 && N > 0, ERR_INVALID, "grid extents must be positive");
+  if (Lp == 0) Lp = 2 * L;
+  if (Mp == 0) Mp = 2 * M;
+  if (Np == 0) Np = 2 * N;
+  VP_REQUIRE(Lp >= 2 * L - 1 && Mp >= 2 * M - 1 && Np >= 2 * N - 1, ERR_INVALID,
+             "FFT sizes must be at least 2*D-1 in every dimension");
+  VP_REQUIRE(Lp < (1 << 20) && Mp < (1 << 20) && Np < (1 << 20), ERR_UNSUPPORTED, "FFT size too large");
+  VP_REQUIRE(Mc, ERR_INVALID, "Mc is NULL");
+  VP_REQUIRE(precisions & (VP_FP64 | VP_FP32), ERR_INVALID, "precisions must include VP_FP64 or VP_FP32");
+  VP_REQUIRE(std::isfinite(gram) && gram > 0, ERR_INVALID, "gram must be positive");
+  VP_HIP(hipSetDevice(ctx->device));
+  auto* op = new vp_op_s();
+  try {
+    op->ctx = ctx;
+    op->L = int(L); op->M = int(M); op->N = int(N);
+    op->Lp = int(Lp); op->Mp = int(Mp); op->Np = int(Np);
+    op->V = L * M * N;
+    op->gram = gram;
+    op->scale32 = gram;
+    op->has_mr = Mr != nullptr;
+    std::vector<int> pos;
+    index_setup(op, idx, n, pos);
+    op->pos.alloc(pos.size() * sizeof(int));
+    VP_HIP(hipMemcpy(op->pos.p, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice));
+    auto mc = gather_material(op, Mc, idx, 0.0);
+    std::vector<double2> mr;
+    if (Mr) mr = gather_material(op, Mr, idx, 1.0);
+    if (precisions & VP_FP64) init_prec(op, op->p64, true, mc, Mr ? &mr : nullptr, gram);
+    if (precisions & VP_FP32) init_prec(op, op->p32, false, mc, Mr ? &mr : nullptr, gram / op->scale32);
+  } catch (...) {
+    delete op;
+    throw;
+  }
+  return op;
+}
+
+static void op_finish(vp_op_s* op) {
+  if (op->p64.present) upload_probs(op, op->p64);
+  if (op->p32.present) upload_probs(op, op->p32);
+  VP_HIP(hipStreamSynchronize(op->ctx->stream));
+}
+
+template <typename T>
+void op_apply_dev(vp_op_s* op, int transp, const C<T>* x, C<T>* y) {
+  OpPrec<T>& P = prec_of<T>(op);
+  VP_REQUIRE(P.present, ERR_INVALID, "operator was not created for this precision");
+  VecArgs<T> va{x, y, 0, 0, nullptr};
+  launch_matvec<T>(P.g, P.plan, P.probs.template as<ProbPtrs<T>>(), va, 1, transp, P.twx(), P.twy(), P.twz(),
+                   op->ctx->stream);
+}
+template void op_apply_dev<double>(vp_op_s*, int, const double2*, double2*);
+template void op_apply_dev<float>(vp_op_s*, int, const float2*, float2*);
+
+}  // namespace vp
+
+extern "C" {
+
+int vp_op_create(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_t Mp, int64_t Np, const double* fN,
+                 const double* Mr, const double* Mc, double gram, const int64_t* idx, int64_t n, int precisions,
+                 vp_op* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(out && fN, ERR_INVALID, "out/fN is NULL");
+  VP_REQUIRE(Lp > 0 && Mp > 0 && Np > 0, ERR_INVALID, "fN extents must be given");
+  vp_op_s* op = op_begin(ctx, L, M, N, Lp, Mp, Np, Mr, Mc, gram, idx, n, precisions);
+  try {
+    const size_t bytes = size_t(op->Lp) * op->Mp * op->Np * 6 * sizeof(double2);
+    DevBuf F(bytes);
+    VP_HIP(hipMemcpy(F.p, fN, bytes, hipMemcpyHostToDevice));
+    layout_fN(op, F.as<double2>(), ctx->stream);
+    op_finish(op);
+  } catch (...) {
+    delete op;
+    throw;
+  }
+  *out = op;
+  VP_API_END
+}
+
+int vp_op_create_toeplitz(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_t Mp, int64_t Np,
+                          const double* G, const double* Mr, const double* Mc, double gram, const int64_t* idx,
+                          int64_t n, int precisions, vp_op* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(out && G, ERR_INVALID, "out/G is NULL");
+  vp_op_s* op = op_begin(ctx, L, M, N, Lp, Mp, Np, Mr, Mc, gram, idx, n, precisions);
+  try {
+    const size_t gb = size_t(op->V) * 6 * sizeof(double2);
+    DevBuf dG(gb);
+    VP_HIP(hipMemcpy(dG.p, G, gb, hipMemcpyHostToDevice));
+    DevBuf F(size_t(op->Lp) * op->Mp * op->Np * 6 * sizeof(double2));
+    toeplitz_to_fN(*op, dG.as<double2>(), F.as<double2>(), ctx->stream);
+    layout_fN(op, F.as<double2>(), ctx->stream);
+    op_finish(op);
+  } catch (...) {
+    delete op;
+    throw;
+  }
+  *out = op;
+  VP_API_END
+}
+
+int vp_op_create_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_t Mp, int64_t Np,
+                           double dx, double ko, const double* Mr, const double* Mc, const int64_t* idx, int64_t n,
+                           int precisions, vp_op* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(out, ERR_INVALID, "out is NULL");
+  VP_REQUIRE(dx > 0 && std::isfinite(ko) && ko >= 0, ERR_INVALID, "dx must be positive and ko finite");
+  vp_op_s* op = op_begin(ctx, L, M, N, Lp, Mp, Np, Mr, Mc, dx * dx * dx, idx, n, precisions);
+  try {
+    DevBuf dG(size_t(op->V) * 6 * sizeof(double2));
+    assemble_toeplitz_dev(op->L, op->M, op->N, dx, ko, dG.as<double2>(), ctx->stream);
+    DevBuf F(size_t(op->Lp) * op->Mp * op->Np * 6 * sizeof(double2));
+    toeplitz_to_fN(*op, dG.as<double2>(), F.as<double2>(), ctx->stream);
+    layout_fN(op, F.as<double2>(), ctx->stream);
+    op_finish(op);
+  } catch (...) {
+    delete op;
+    throw;
+  }
+  *out = op;
+  VP_API_END
+}
+
+int vp_assemble_toeplitz(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko, double* G, int mem) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && G, ERR_INVALID, "ctx/G is NULL");
+  VP_REQUIRE(L > 0 && M > 0 && N > 0 && dx > 0, ERR_INVALID, "bad grid");
+  VP_HIP(hipSetDevice(ctx->device));
+  const size_t bytes = size_t(L * M * N) * 6 * sizeof(double2);
+  if (mem == VP_MEM_DEVICE) {
+    assemble_toeplitz_dev(int(L), int(M), int(N), dx, ko, reinterpret_cast<double2*>(G), ctx->stream);
+  } else {
+    DevBuf dG(bytes);
+    assemble_toeplitz_dev(int(L), int(M), int(N), dx, ko, dG.as<double2>(), ctx->stream);
+    VP_HIP(hipMemcpy(G, dG.p, bytes, hipMemcpyDeviceToHost));
+  }
+  VP_API_END
+}
+
+int vp_op_apply(vp_op op, int transp, const void* x, void* y, int mem, int prec) {
+  VP_API_BEGIN
+  VP_REQUIRE(op && x && y, ERR_INVALID, "op/x/y is NULL");
+  VP_REQUIRE(x != y, ERR_INVALID, "x and y must not alias");
+  VP_REQUIRE(transp == VP_NOTRANSP || transp == VP_TRANSP, ERR_INVALID, "transp must be 0 or 1");
+  VP_REQUIRE(prec == VP_FP64 || prec == VP_FP32, ERR_INVALID, "prec must be VP_FP64 or VP_FP32");
+  VP_HIP(hipSetDevice(op->ctx->device));
+  hipStream_t s = op->ctx->stream;
+  const int64_t n = op->n;
+  if (mem == VP_MEM_DEVICE) {
+    if (prec == VP_FP64) op_apply_dev<double>(op, transp, static_cast<const double2*>(x), static_cast<double2*>(y));
+    else op_apply_dev<float>(op, transp, static_cast<const float2*>(x), static_cast<float2*>(y));
+  } else if (mem == VP_MEM_HOST) {
+    if (prec == VP_FP64) {
+      VP_REQUIRE(op->p64.present, ERR_INVALID, "operator has no FP64 copy");
+      VP_HIP(hipMemcpyAsync(op->p64.xs.p, x, n * sizeof(double2), hipMemcpyHostToDevice, s));
+      op_apply_dev<double>(op, transp, op->p64.xs.as<double2>(), op->p64.ys.as<double2>());
+      VP_HIP(hipMemcpyAsync(y, op->p64.ys.p, n * sizeof(double2), hipMemcpyDeviceToHost, s));
+    } else {
+      VP_REQUIRE(op->p32.present, ERR_INVALID, "operator has no FP32 copy");
+      // stage complex128 through the FP64 staging area of the FP32 copy
+      DevBuf t(n * sizeof(double2));
+      VP_HIP(hipMemcpyAsync(t.p, x, n * sizeof(double2), hipMemcpyHostToDevice, s));
+      convert_c128_to_c64(t.as<double2>(), op->p32.xs.as<float2>(), n, 1.0, s);
+      op_apply_dev<float>(op, transp, op->p32.xs.as<float2>(), op->p32.ys.as<float2>());
+      convert_c64_to_c128(op->p32.ys.as<float2>(), t.as<double2>(), n, op->scale32, s);
+      VP_HIP(hipMemcpyAsync(y, t.p, n * sizeof(double2), hipMemcpyDeviceToHost, s));
+      VP_HIP(hipStreamSynchronize(s));
+    }
+    VP_HIP(hipStreamSynchronize(s));
+  } else {
+    throw VpError{ERR_INVALID, "mem must be VP_MEM_HOST or VP_MEM_DEVICE"};
+  }
+  VP_API_END
+}
+
+int vp_op_apply_batched(const vp_op* ops, int nvec, int transp, const void* x, void* y, int prec) {
+  VP_API_BEGIN
+  VP_REQUIRE(ops && nvec > 0 && x && y, ERR_INVALID, "bad batched arguments");
+  VP_REQUIRE(prec == VP_FP64 || prec == VP_FP32, ERR_INVALID, "prec must be VP_FP64 or VP_FP32");
+  vp_op_s* o0 = ops[0];
+  VP_REQUIRE(o0, ERR_INVALID, "ops[0] is NULL");
+  for (int i = 1; i < nvec; ++i) {
+    VP_REQUIRE(ops[i] && ops[i]->L == o0->L && ops[i]->M == o0->M && ops[i]->N == o0->N &&
+                   ops[i]->Lp == o0->Lp && ops[i]->Mp == o0->Mp && ops[i]->Np == o0->Np && ops[i]->n == o0->n &&
+                   ops[i]->structured == o0->structured,
+               ERR_INVALID, "batched operators must share one geometry");
+  }
+  VP_HIP(hipSetDevice(o0->ctx->device));
+  hipStream_t s = o0->ctx->stream;
+  auto run = [&](auto tag) {
+    using T = decltype(tag);
+    std::vector<ProbPtrs<T>> h(nvec);
+    for (int i = 0; i < nvec; ++i) {
+      OpPrec<T>& P = prec_of<T>(ops[i]);
+      VP_REQUIRE(P.present, ERR_INVALID, "operator lacks the requested precision");
+      VP_HIP(hipMemcpy(&h[i], P.probs.p, sizeof(ProbPtrs<T>), hipMemcpyDeviceToHost));
+    }
+    OpPrec<T>& P0 = prec_of<T>(o0);
+    // all problems must have identical class splits; rebuild if a later op chose differently
+    for (int i = 1; i < nvec; ++i) {
+      OpPrec<T>& P = prec_of<T>(ops[i]);
+      VP_REQUIRE(P.g.y.R == P0.g.y.R && P.g.z.R == P0.g.z.R, ERR_INVALID, "batched operators differ in plan");
+    }
+    DevBuf d(sizeof(ProbPtrs<T>) * nvec);
+    VP_HIP(hipMemcpyAsync(d.p, h.data(), d.bytes, hipMemcpyHostToDevice, s));
+    VecArgs<T> va{static_cast<const C<T>*>(x), static_cast<C<T>*>(y), o0->n, o0->n, nullptr};
+    launch_matvec<T>(P0.g, P0.plan, d.as<ProbPtrs<T>>(), va, nvec, transp, P0.twx(), P0.twy(), P0.twz(), s);
+    VP_HIP(hipStreamSynchronize(s));
+  };
+  if (prec == VP_FP64) run(double{});
+  else run(float{});
+  VP_API_END
+}
+
+int vp_op_fp32_scale(vp_op op, double* scale) {
+  VP_API_BEGIN
+  VP_REQUIRE(op && scale, ERR_INVALID, "op/scale is NULL");
+  *scale = op->scale32;
+  VP_API_END
+}
+
+int vp_op_size(vp_op op, int64_t* n) {
+  VP_API_BEGIN
+  VP_REQUIRE(op && n, ERR_INVALID, "op/n is NULL");
+  *n = op->n;
+  VP_API_END
+}
+
+int vp_op_destroy(vp_op op) {
+  VP_API_BEGIN
+  if (!op) return VP_OK;
+  VP_HIP(hipSetDevice(op->ctx->device));
+  (void)hipStreamSynchronize(op->ctx->stream);
+  delete op;
+  VP_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,625 @@
+// C ABI: preconditioners, GMRES, mixed-precision iterative refinement and
+// batched solves.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "krylov.hpp"
+
+namespace vp {
+void set_error(const std::string& msg);
+}
+using namespace vp;
+
+#define VP_API_BEGIN try {
+#define VP_API_END                       \
+  }                                      \
+  catch (const VpError& e) {             \
+    set_error(e.msg);                    \
+    return e.code;                       \
+  }                                      \
+  catch (const std::bad_alloc&) {        \
+    set_error("host allocation failed"); \
+    return VP_ERR_OOM;                   \
+  }                                      \
+  catch (const std::exception& e) {      \
+    set_error(e.what());                 \
+    return VP_ERR_DEVICE;                \
+  }                                      \
+  catch (...) {                          \
+    set_error("unknown error");          \
+    return VP_ERR_DEVICE;                \
+  }                                      \
+  set_error("");                         \
+  return VP_OK;
+
+namespace vp {
+
+// ----------------------------------------------------------------------------
+// Preconditioner construction
+// ----------------------------------------------------------------------------
+__global__ void k_scale_c128(double2* a, int64_t n, double s);
+
+static void prec_geometry(vp_prec_s* p, int level) {
+  PcGeom& g = p->g;
+  g.level = level;
+  const int dev = p->ctx->device;
+  const int lds_max = max_lds_bytes(dev);
+  VP_REQUIRE(factorize(g.L, true, g.px, 1), ERR_UNSUPPORTED,
+             "preconditioner box length L has a prime factor above the supported radices");
+  if (level == 2)
+    VP_REQUIRE(factorize(g.M, true, g.py, 1), ERR_UNSUPPORTED,
+               "preconditioner box length M has a prime factor above the supported radices");
+  const size_t s = 16;   // size for the FP64 path bounds both precisions
+  g.B = 1;
+  for (int B : {16, 8, 4, 2, 1}) {
+    if (B > 1 && B > g.M) continue;
+    const int ld = g.L + (B > 1 ? 1 : 0);
+    if (2 * size_t(B) * ld * s <= size_t(lds_max)) {
+      g.B = B;
+      g.ld = ld;
+      break;
+    }
+  }
+  g.ld = g.L + (g.B > 1 ? 1 : 0);
+  VP_REQUIRE(2 * size_t(g.B) * g.ld * s <= size_t(lds_max), ERR_UNSUPPORTED, "preconditioner x pencil exceeds LDS");
+  g.lds1 = 2 * size_t(g.B) * g.ld * s;
+  g.tx = 256;
+  g.t2 = 256;
+  g.lds2 = size_t(2) * 3 * g.N * g.M * s;
+  if (level == 2)
+    VP_REQUIRE(g.lds2 <= size_t(lds_max), ERR_UNSUPPORTED, "preconditioner plane exceeds LDS");
+}
+
+static vp_prec_s* prec_begin(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const int64_t* idx, int64_t n,
+                             int level) {
+  VP_REQUIRE(ctx, ERR_INVALID, "ctx is NULL");
+  VP_REQUIRE(L > 0 && M > 0 && N > 0, ERR_INVALID, "box extents must be positive");
+  VP_HIP(hipSetDevice(ctx->device));
+  auto* p = new vp_prec_s();
+  try {
+    p->ctx = ctx;
+    p->g.L = int(L);
+    p->g.M = int(M);
+    p->g.N = int(N);
+    p->g.V = L * M * N;
+    std::vector<int> pos;
+    index_map(p->g.V, idx, n, pos, p->g.structured, p->g.nS);
+    p->nbox = n;
+    p->g.n_outer = 0;
+    p->pos.alloc(pos.size() * sizeof(int));
+    VP_HIP(hipMemcpy(p->pos.p, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice));
+    prec_geometry(p, level);
+  } catch (...) {
+    delete p;
+    throw;
+  }
+  return p;
+}
+
+template <typename T>
+static void prec_alloc_prec(vp_prec_s* p, PrecPrec<T>& P) {
+  P.present = true;
+  const PcGeom& g = p->g;
+  std::vector<C<T>> tw;
+  for (int dimP : {g.L, g.M}) {
+    for (int k = 0; k < dimP; ++k) {
+      SinCos v = sincos_turn(k, dimP);
+      tw.push_back(mk<T>(T(v.c), T(-v.s)));
+    }
+  }
+  P.tw.alloc(tw.size() * sizeof(C<T>));
+  VP_HIP(hipMemcpy(P.tw.p, tw.data(), P.tw.bytes, hipMemcpyHostToDevice));
+  P.work.alloc(size_t(3) * g.V * sizeof(C<T>));
+  if (g.level == 1) P.work2.alloc(size_t(3) * g.V * sizeof(C<T>));
+  P.xs.alloc(size_t(std::max<int64_t>(p->nbox, 1)) * sizeof(C<T>));
+  P.ys.alloc(size_t(std::max<int64_t>(p->nbox, 1)) * sizeof(C<T>));
+  const C<T>* bp = P.blocks.template as<C<T>>();
+  P.blockptr.alloc(sizeof(bp));
+  VP_HIP(hipMemcpy(P.blockptr.p, &bp, sizeof(bp), hipMemcpyHostToDevice));
+}
+
+// Builds the FP32 block copy from the FP64 blocks, scaled by scale32.
+static void prec_make_fp32(vp_prec_s* p, double scale32) {
+  const int64_t nb = int64_t(p->p64.blocks.bytes / sizeof(double2));
+  p->p32.blocks.alloc(size_t(nb) * sizeof(float2));
+  convert_c128_to_c64(p->p64.blocks.as<double2>(), p->p32.blocks.as<float2>(), nb, scale32, p->ctx->stream);
+  VP_HIP(hipStreamSynchronize(p->ctx->stream));
+  p->scale32 = scale32;
+  prec_alloc_prec<float>(p, p->p32);
+}
+
+// blocks_dev: FP64 blocks in (kx, ky)-major order, already inverted; scaled here by 1/(L*M)
+static void prec_finish(vp_prec_s* p, int precisions) {
+  const PcGeom& g = p->g;
+  const int64_t nb = int64_t(p->p64.blocks.bytes / sizeof(double2));
+  const double sc = 1.0 / (double(g.L) * (g.level == 2 ? g.M : 1));
+  hipLaunchKernelGGL(k_scale_c128, dim3(int(std::min<int64_t>((nb + 255) / 256, 8192))), dim3(256), 0,
+                     p->ctx->stream, p->p64.blocks.as<double2>(), nb, sc);
+  VP_HIP(hipGetLastError());
+  VP_HIP(hipStreamSynchronize(p->ctx->stream));
+  prec_alloc_prec<double>(p, p->p64);
+  if (precisions & VP_FP32) prec_make_fp32(p, 1.0);
+}
+
+__global__ void k_scale_c128(double2* a, int64_t n, double s) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    a[i] = make_double2(a[i].x * s, a[i].y * s);
+}
+
+template <typename T>
+static void prec_apply_dev(vp_prec_s* p, const C<T>* x, C<T>* y) {
+  PrecPrec<T>& P = pprec_of<T>(p);
+  VP_REQUIRE(P.present, ERR_INVALID, "preconditioner lacks this precision");
+  PcArgs<T> a = prec_args<T>(p, x, y);
+  launch_prec<T>(p->g, a, 1, p->ctx->stream);
+}
+
+// ----------------------------------------------------------------------------
+// Problem sets over operators (and preconditioners)
+// ----------------------------------------------------------------------------
+template <typename T>
+static void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* precs, int np) {
+  vp_op_s* o0 = ops[0];
+  OpPrec<T>& P0 = prec_of<T>(o0);
+  VP_REQUIRE(P0.present, ERR_INVALID, "operator lacks the precision this solve needs");
+  ps.nprob = np;
+  ps.n = o0->n;
+  ps.stream = o0->ctx->stream;
+  ps.g = P0.g;
+  ps.plan = P0.plan;
+  ps.twx = P0.twx();
+  ps.twy = P0.twy();
+  ps.twz = P0.twz();
+  std::vector<ProbPtrs<T>> h(np);
+  const size_t wbytes = size_t(3) * o0->Lp * o0->M * o0->N * sizeof(C<T>);
+  if (np > 1) ps.works.alloc(wbytes * np);
+  for (int p = 0; p < np; ++p) {
+    vp_op_s* op = ops[p];
+    VP_REQUIRE(op && op->L == o0->L && op->M == o0->M && op->N == o0->N && op->Lp == o0->Lp &&
+                   op->Mp == o0->Mp && op->Np == o0->Np && op->n == o0->n && op->structured == o0->structured,
+               ERR_INVALID, "batched operators must share one geometry");
+    OpPrec<T>& P = prec_of<T>(op);
+    VP_REQUIRE(P.present, ERR_INVALID, "operator lacks the precision this solve needs");
+    VP_REQUIRE(P.g.y.R == P0.g.y.R && P.g.z.R == P0.g.z.R, ERR_INVALID, "batched operators differ in plan");
+    ProbPtrs<T> pp{};
+    pp.work = np > 1 ? reinterpret_cast<C<T>*>(static_cast<char*>(ps.works.p) + wbytes * p)
+                     : P.work.template as<C<T>>();
+    pp.fN = P.fN.template as<C<T>>();
+    pp.pos = op->pos.as<int>();
+    pp.mc = P.mc.template as<C<T>>();
+    pp.mr = P.mr.p ? P.mr.template as<C<T>>() : nullptr;
+    pp.gram = P.gram;
+    h[p] = pp;
+  }
+  ps.probs.alloc(sizeof(ProbPtrs<T>) * np);
+  VP_HIP(hipMemcpy(ps.probs.p, h.data(), ps.probs.bytes, hipMemcpyHostToDevice));
+  ps.has_prec = precs && precs[0];
+  if (ps.has_prec) {
+    vp_prec_s* q0 = precs[0];
+    std::vector<const C<T>*> bps(np);
+    for (int p = 0; p < np; ++p) {
+      vp_prec_s* q = precs[p];
+      VP_REQUIRE(q && q->g.L == q0->g.L && q->g.M == q0->g.M && q->g.N == q0->g.N && q->nbox == q0->nbox &&
+                     q->g.level == q0->g.level && q->g.n_outer == q0->g.n_outer,
+                 ERR_INVALID, "batched preconditioners must share one geometry");
+      if (std::is_same<T, float>::value && (!q->p32.present || q->scale32 != ops[p]->scale32))
+        prec_make_fp32(q, ops[p]->scale32);
+      PrecPrec<T>& Q = pprec_of<T>(q);
+      VP_REQUIRE(Q.present, ERR_INVALID, "preconditioner lacks this precision");
+      bps[p] = Q.blocks.template as<C<T>>();
+    }
+    const int64_t outer = q0->g.n_outer ? q0->g.n_outer : q0->nbox;
+    VP_REQUIRE(outer == ps.n, ERR_INVALID, "preconditioner size does not match the operator");
+    ps.pg = q0->g;
+    ps.pblocks.alloc(sizeof(const C<T>*) * np);
+    VP_HIP(hipMemcpy(ps.pblocks.p, bps.data(), ps.pblocks.bytes, hipMemcpyHostToDevice));
+    const int64_t wst = 3 * q0->g.V;
+    ps.pwork.alloc(size_t(wst) * np * sizeof(C<T>));
+    if (q0->g.level == 1) ps.pwork2.alloc(size_t(wst) * np * sizeof(C<T>));
+    PcArgs<T> a = prec_args<T>(q0, nullptr, nullptr);
+    a.work = ps.pwork.template as<C<T>>();
+    a.work2 = q0->g.level == 1 ? ps.pwork2.template as<C<T>>() : nullptr;
+    a.wstride = wst;
+    a.blocks = ps.pblocks.template as<const C<T>*>();
+    ps.pa = a;
+  }
+}
+
+static void fill_info(vp_gmres_info* info, const GmresResult& r, int p) {
+  if (!info) return;
+  info->flag = r.flag[p];
+  info->iter_inner = r.iter_inner[p];
+  info->iter_outer = r.iter_outer[p];
+  info->total_iters = r.total[p];
+  info->relres = r.relres[p];
+  info->resvec_len = 0;
+  if (info->resvec && info->resvec_cap > 0) {
+    const int cnt = std::min<int>(info->resvec_cap, int(r.resvec[p].size()));
+    for (int i = 0; i < cnt; ++i) info->resvec[i] = r.resvec[p][i];
+    info->resvec_len = cnt;
+  }
+}
+
+// Mixed precision: FP64 refinement around FP32 GMRES for a problem set.
+struct IrResult {
+  std::vector<int> flag, steps, inner;
+  std::vector<double> relres;
+  std::vector<std::vector<double>> hist;
+};
+
+static void solve_ir_sets(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2* b, double2* x,
+                          double scale32, double tol, int restart, double inner_tol, int inner_maxit,
+                          int max_outer, int side, IrResult& out) {
+  const int np = p64.nprob;
+  const int64_t n = p64.n;
+  hipStream_t s = p64.stream;
+  DevBuf ax(size_t(n) * np * sizeof(double2)), r(size_t(n) * np * sizeof(double2));
+  DevBuf r32(size_t(n) * np * sizeof(float2)), d32(size_t(n) * np * sizeof(float2));
+  DevBuf bn(np * sizeof(double)), rn(np * sizeof(double)), act(np * sizeof(int));
+  VP_HIP(hipMemsetAsync(x, 0, size_t(n) * np * sizeof(double2), s));
+  dev_norms<double>(b, n, np, n, bn.as<double>(), s);
+  std::vector<double> hbn(np), hrn(np);
+  VP_HIP(hipMemcpy(hbn.data(), bn.p, np * sizeof(double), hipMemcpyDeviceToHost));
+  out.flag.assign(np, 1);
+  out.steps.assign(np, 0);
+  out.inner.assign(np, 0);
+  out.relres.assign(np, 1.0);
+  out.hist.assign(np, {});
+  std::vector<int> hact(np, 1);
+  for (int it = 0; it <= max_outer; ++it) {
+    // r = b - A x  (FP64)
+    VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    ps_apply_A<double>(p64, x, n, ax.as<double2>(), n, nullptr);
+    dev_residual_d(b, ax.as<double2>(), r.as<double2>(), n, np, s);
+    dev_norms<double>(r.as<double2>(), n, np, n, rn.as<double>(), s);
+    VP_HIP(hipMemcpy(hrn.data(), rn.p, np * sizeof(double), hipMemcpyDeviceToHost));
+    bool any = false;
+    for (int p = 0; p < np; ++p) {
+      if (!hact[p]) continue;
+      out.relres[p] = hbn[p] > 0 ? hrn[p] / hbn[p] : 0.0;
+      out.hist[p].push_back(out.relres[p]);
+      if (out.relres[p] <= tol || !std::isfinite(out.relres[p])) {
+        hact[p] = 0;
+        out.flag[p] = out.relres[p] <= tol ? 0 : 1;
+      } else if (it == max_outer) {
+        hact[p] = 0;
+      } else {
+        any = true;
+      }
+    }
+    if (!any) break;
+    VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    // inner FP32 solve of (A/s) d = r/||r||
+    dev_scale_to_f(r.as<double2>(), rn.as<double>(), r32.as<float2>(), n, np, 1.0, s);
+    GmresParams prm;
+    prm.restart = restart;
+    prm.tol = inner_tol;
+    prm.maxit = inner_maxit;
+    prm.side = side;
+    GmresResult gr;
+    gmres_device<float>(p32, r32.as<float2>(), d32.as<float2>(), prm, gr);
+    for (int p = 0; p < np; ++p)
+      if (hact[p]) {
+        out.inner[p] += gr.total[p];
+        out.steps[p] += 1;
+      }
+    // x += d * ||r|| / s
+    dev_update_from_f(x, d32.as<float2>(), rn.as<double>(), n, np, 1.0 / scale32, act.as<int>(), s);
+  }
+  VP_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace vp
+
+extern "C" {
+
+int vp_prec_create_chan2(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double* blocks, const int64_t* idx,
+                         int64_t n, int precisions, vp_prec* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(out && blocks, ERR_INVALID, "out/blocks is NULL");
+  vp_prec_s* p = prec_begin(ctx, L, M, N, idx, n, 2);
+  try {
+    const int64_t n3 = 3 * N;
+    const int64_t bs = n3 * n3;
+    std::vector<double2> h(size_t(L * M * bs));
+    for (int64_t j = 0; j < M; ++j)        // cell {i,j} at (i + L*j) -> (i*M + j)
+      for (int64_t i = 0; i < L; ++i) {
+        const double* src = blocks + 2 * (i + L * j) * bs;
+        double2* dst = h.data() + (i * M + j) * bs;
+        for (int64_t e = 0; e < bs; ++e) dst[e] = make_double2(src[2 * e], src[2 * e + 1]);
+      }
+    p->p64.blocks.alloc(h.size() * sizeof(double2));
+    VP_HIP(hipMemcpy(p->p64.blocks.p, h.data(), p->p64.blocks.bytes, hipMemcpyHostToDevice));
+    prec_finish(p, precisions);
+  } catch (...) {
+    delete p;
+    throw;
+  }
+  *out = p;
+  VP_API_END
+}
+
+int vp_prec_create_chan1(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double* blocks, const int64_t* idx,
+                         int64_t n, int precisions, vp_prec* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(out && blocks, ERR_INVALID, "out/blocks is NULL");
+  vp_prec_s* p = prec_begin(ctx, L, M, N, idx, n, 1);
+  try {
+    const int64_t n3 = 3 * M * N;
+    const size_t cnt = size_t(L * n3 * n3);
+    p->p64.blocks.alloc(cnt * sizeof(double2));
+    VP_HIP(hipMemcpy(p->p64.blocks.p, blocks, cnt * sizeof(double2), hipMemcpyHostToDevice));
+    prec_finish(p, precisions);
+  } catch (...) {
+    delete p;
+    throw;
+  }
+  *out = p;
+  VP_API_END
+}
+
+static int prec_level2_common(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double2* dG, double mc_re,
+                              double mc_im, double mr0, double gram, const int64_t* idx, int64_t n,
+                              int precisions, vp_prec* out) {
+  vp_prec_s* p = prec_begin(ctx, L, M, N, idx, n, 2);
+  try {
+    const int64_t n3 = 3 * N;
+    p->p64.blocks.alloc(size_t(L * M * n3 * n3) * sizeof(double2));
+    build_level2_blocks(dG, int(L), int(M), int(N), make_double2(mc_re, mc_im), gram * mr0,
+                        p->p64.blocks.as<double2>(), ctx->stream);
+    invert_blocks(p->p64.blocks.as<double2>(), L * M, int(n3), ctx->stream);
+    prec_finish(p, precisions);
+  } catch (...) {
+    delete p;
+    throw;
+  }
+  *out = p;
+  return VP_OK;
+}
+
+int vp_prec_create_level2(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double* G, double mc_re, double mc_im,
+                          double mr0, double gram, const int64_t* idx, int64_t n, int precisions, vp_prec* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && out && G, ERR_INVALID, "ctx/out/G is NULL");
+  VP_HIP(hipSetDevice(ctx->device));
+  DevBuf dG(size_t(L * M * N) * 6 * sizeof(double2));
+  VP_HIP(hipMemcpy(dG.p, G, dG.bytes, hipMemcpyHostToDevice));
+  prec_level2_common(ctx, L, M, N, dG.as<double2>(), mc_re, mc_im, mr0, gram, idx, n, precisions, out);
+  VP_API_END
+}
+
+int vp_prec_create_level2_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko, double mc_re,
+                                    double mc_im, double mr0, const int64_t* idx, int64_t n, int precisions,
+                                    vp_prec* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && out && dx > 0, ERR_INVALID, "bad arguments");
+  VP_HIP(hipSetDevice(ctx->device));
+  DevBuf dG(size_t(L * M * N) * 6 * sizeof(double2));
+  assemble_toeplitz_dev(int(L), int(M), int(N), dx, ko, dG.as<double2>(), ctx->stream);
+  prec_level2_common(ctx, L, M, N, dG.as<double2>(), mc_re, mc_im, mr0, dx * dx * dx, idx, n, precisions, out);
+  VP_API_END
+}
+
+int vp_prec_set_embedding(vp_prec p, int64_t n_outer, const int64_t* sub_of_op, int64_t nsub) {
+  VP_API_BEGIN
+  VP_REQUIRE(p && sub_of_op, ERR_INVALID, "p/sub is NULL");
+  VP_REQUIRE(nsub == p->nbox, ERR_INVALID, "embedding length must equal the number of box unknowns");
+  VP_REQUIRE(n_outer >= nsub, ERR_INVALID, "outer vector shorter than the box");
+  std::vector<int> h(static_cast<size_t>(nsub));
+  std::vector<char> seen(static_cast<size_t>(n_outer), 0);
+  for (int64_t i = 0; i < nsub; ++i) {
+    VP_REQUIRE(sub_of_op[i] >= 1 && sub_of_op[i] <= n_outer, ERR_INVALID, "embedding index out of range");
+    VP_REQUIRE(!seen[size_t(sub_of_op[i] - 1)], ERR_INVALID, "embedding index repeated");
+    seen[size_t(sub_of_op[i] - 1)] = 1;
+    h[size_t(i)] = int(sub_of_op[i] - 1);
+  }
+  VP_HIP(hipSetDevice(p->ctx->device));
+  p->sub.alloc(h.size() * sizeof(int));
+  VP_HIP(hipMemcpy(p->sub.p, h.data(), p->sub.bytes, hipMemcpyHostToDevice));
+  p->g.n_outer = n_outer;
+  p->p64.xs.alloc(size_t(n_outer) * sizeof(double2));
+  p->p64.ys.alloc(size_t(n_outer) * sizeof(double2));
+  if (p->p32.present) {
+    p->p32.xs.alloc(size_t(n_outer) * sizeof(float2));
+    p->p32.ys.alloc(size_t(n_outer) * sizeof(float2));
+  }
+  VP_API_END
+}
+
+int vp_prec_set_fp32_scale(vp_prec p, double scale) {
+  VP_API_BEGIN
+  VP_REQUIRE(p && std::isfinite(scale) && scale > 0, ERR_INVALID, "bad scale");
+  VP_HIP(hipSetDevice(p->ctx->device));
+  prec_make_fp32(p, scale);
+  if (p->g.n_outer) {
+    p->p32.xs.alloc(size_t(p->g.n_outer) * sizeof(float2));
+    p->p32.ys.alloc(size_t(p->g.n_outer) * sizeof(float2));
+  }
+  VP_API_END
+}
+
+int vp_prec_apply(vp_prec p, const void* x, void* y, int mem, int prec) {
+  VP_API_BEGIN
+  VP_REQUIRE(p && x && y && x != y, ERR_INVALID, "p/x/y invalid (x and y must not alias)");
+  VP_REQUIRE(prec == VP_FP64 || prec == VP_FP32, ERR_INVALID, "prec must be VP_FP64 or VP_FP32");
+  VP_HIP(hipSetDevice(p->ctx->device));
+  hipStream_t s = p->ctx->stream;
+  const int64_t n = p->g.n_outer ? p->g.n_outer : p->nbox;
+  if (mem == VP_MEM_DEVICE) {
+    if (prec == VP_FP64) prec_apply_dev<double>(p, static_cast<const double2*>(x), static_cast<double2*>(y));
+    else prec_apply_dev<float>(p, static_cast<const float2*>(x), static_cast<float2*>(y));
+  } else if (mem == VP_MEM_HOST) {
+    if (prec == VP_FP64) {
+      VP_HIP(hipMemcpyAsync(p->p64.xs.p, x, n * sizeof(double2), hipMemcpyHostToDevice, s));
+      prec_apply_dev<double>(p, p->p64.xs.as<double2>(), p->p64.ys.as<double2>());
+      VP_HIP(hipMemcpyAsync(y, p->p64.ys.p, n * sizeof(double2), hipMemcpyDeviceToHost, s));
+    } else {
+      VP_REQUIRE(p->p32.present, ERR_INVALID, "preconditioner has no FP32 copy");
+      DevBuf t(n * sizeof(double2));
+      VP_HIP(hipMemcpyAsync(t.p, x, n * sizeof(double2), hipMemcpyHostToDevice, s));
+      convert_c128_to_c64(t.as<double2>(), p->p32.xs.as<float2>(), n, 1.0, s);
+      prec_apply_dev<float>(p, p->p32.xs.as<float2>(), p->p32.ys.as<float2>());
+      convert_c64_to_c128(p->p32.ys.as<float2>(), t.as<double2>(), n, 1.0 / p->scale32, s);
+      VP_HIP(hipMemcpyAsync(y, t.p, n * sizeof(double2), hipMemcpyDeviceToHost, s));
+      VP_HIP(hipStreamSynchronize(s));
+    }
+    VP_HIP(hipStreamSynchronize(s));
+  } else {
+    throw VpError{ERR_INVALID, "mem must be VP_MEM_HOST or VP_MEM_DEVICE"};
+  }
+  VP_API_END
+}
+
+int vp_prec_destroy(vp_prec p) {
+  VP_API_BEGIN
+  if (!p) return VP_OK;
+  VP_HIP(hipSetDevice(p->ctx->device));
+  (void)hipStreamSynchronize(p->ctx->stream);
+  delete p;
+  VP_API_END
+}
+
+int vp_gmres(vp_op op, vp_prec M, int side, const void* b, const void* x0, void* x, int mem, int prec, int restart,
+             double tol, int maxit, vp_gmres_info* info) {
+  VP_API_BEGIN
+  VP_REQUIRE(op && b && x, ERR_INVALID, "op/b/x is NULL");
+  VP_REQUIRE(restart > 0 && restart <= 1000 && maxit > 0 && tol > 0, ERR_INVALID, "bad GMRES parameters");
+  VP_REQUIRE(side == VP_PREC_LEFT || side == VP_PREC_RIGHT, ERR_INVALID, "side must be LEFT or RIGHT");
+  VP_REQUIRE(prec == VP_FP64 || prec == VP_FP32, ERR_INVALID, "prec must be VP_FP64 or VP_FP32");
+  VP_HIP(hipSetDevice(op->ctx->device));
+  hipStream_t s = op->ctx->stream;
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t n = op->n;
+  GmresParams prm;
+  prm.restart = restart;
+  prm.tol = tol;
+  prm.maxit = maxit;
+  prm.side = side;
+  prm.x0_nonzero = x0 != nullptr;
+  GmresResult res;
+  vp_op_s* ops[1] = {op};
+  vp_prec_s* precs[1] = {M};
+  auto run = [&](auto tag) {
+    using T = decltype(tag);
+    ProblemSet<T> ps;
+    make_problem_set<T>(ps, ops, M ? precs : nullptr, 1);
+    const double sc = std::is_same<T, float>::value ? 1.0 / op->scale32 : 1.0;   // b for A/scale
+    if (mem == VP_MEM_DEVICE) {
+      if (x0) VP_HIP(hipMemcpyAsync(x, x0, n * sizeof(C<T>), hipMemcpyDeviceToDevice, s));
+      gmres_device<T>(ps, static_cast<const C<T>*>(b), static_cast<C<T>*>(x), prm, res);
+    } else {
+      DevBuf db(n * sizeof(C<T>)), dx(n * sizeof(C<T>)), t(n * sizeof(double2));
+      VP_HIP(hipMemcpyAsync(t.p, b, n * sizeof(double2), hipMemcpyHostToDevice, s));
+      if (std::is_same<T, float>::value) convert_c128_to_c64(t.as<double2>(), db.as<float2>(), n, sc, s);
+      else VP_HIP(hipMemcpyAsync(db.p, t.p, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+      if (x0) {
+        VP_HIP(hipMemcpyAsync(t.p, x0, n * sizeof(double2), hipMemcpyHostToDevice, s));
+        if (std::is_same<T, float>::value) convert_c128_to_c64(t.as<double2>(), dx.as<float2>(), n, 1.0, s);
+        else VP_HIP(hipMemcpyAsync(dx.p, t.p, n * sizeof(double2), hipMemcpyDeviceToDevice, s));
+      }
+      gmres_device<T>(ps, db.as<C<T>>(), dx.as<C<T>>(), prm, res);
+      if (std::is_same<T, float>::value) {
+        convert_c64_to_c128(dx.as<float2>(), t.as<double2>(), n, 1.0, s);
+        VP_HIP(hipMemcpyAsync(x, t.p, n * sizeof(double2), hipMemcpyDeviceToHost, s));
+      } else {
+        VP_HIP(hipMemcpyAsync(x, dx.p, n * sizeof(double2), hipMemcpyDeviceToHost, s));
+      }
+      VP_HIP(hipStreamSynchronize(s));
+    }
+  };
+  if (prec == VP_FP64) run(double{});
+  else run(float{});
+  fill_info(info, res, 0);
+  if (info) {
+    info->ir_steps = 0;
+    info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  VP_API_END
+}
+
+static void ir_common(vp_op_s* const* ops, vp_prec_s* const* precs, int np, int side, const double2* db,
+                      double2* dx, double tol, int restart, double inner_tol, int inner_maxit, int max_outer,
+                      vp_gmres_info* infos) {
+  VP_REQUIRE(restart > 0 && restart <= 1000 && inner_maxit > 0 && max_outer >= 0 && tol > 0 && inner_tol > 0,
+             ERR_INVALID, "bad solve parameters");
+  for (int p = 0; p < np; ++p)
+    VP_REQUIRE(ops[p]->p64.present && ops[p]->p32.present, ERR_INVALID,
+               "mixed precision needs operators created with VP_FP64|VP_FP32");
+  const auto t0 = std::chrono::steady_clock::now();
+  ProblemSet<double> p64;
+  make_problem_set<double>(p64, ops, nullptr, np);
+  ProblemSet<float> p32;
+  make_problem_set<float>(p32, ops, precs, np);
+  IrResult r;
+  solve_ir_sets(p64, p32, db, dx, ops[0]->scale32, tol, restart, inner_tol, inner_maxit, max_outer, side, r);
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (infos)
+    for (int p = 0; p < np; ++p) {
+      vp_gmres_info* info = infos + p;
+      info->flag = r.flag[p];
+      info->relres = r.relres[p];
+      info->ir_steps = r.steps[p];
+      info->total_iters = r.inner[p];
+      info->iter_inner = 0;
+      info->iter_outer = r.steps[p];
+      info->seconds = secs;
+      info->resvec_len = 0;
+      if (info->resvec && info->resvec_cap > 0) {
+        const int cnt = std::min<int>(info->resvec_cap, int(r.hist[p].size()));
+        for (int i = 0; i < cnt; ++i) info->resvec[i] = r.hist[p][i];
+        info->resvec_len = cnt;
+      }
+    }
+}
+
+int vp_solve_ir(vp_op op, vp_prec M, int side, const void* b, void* x, int mem, double tol, int restart,
+                double inner_tol, int inner_maxit, int max_outer, vp_gmres_info* info) {
+  VP_API_BEGIN
+  VP_REQUIRE(op && b && x, ERR_INVALID, "op/b/x is NULL");
+  VP_REQUIRE(side == VP_PREC_LEFT || side == VP_PREC_RIGHT, ERR_INVALID, "side must be LEFT or RIGHT");
+  VP_HIP(hipSetDevice(op->ctx->device));
+  hipStream_t s = op->ctx->stream;
+  const int64_t n = op->n;
+  vp_op_s* ops[1] = {op};
+  vp_prec_s* precs[1] = {M};
+  if (mem == VP_MEM_DEVICE) {
+    ir_common(ops, M ? precs : nullptr, 1, side, static_cast<const double2*>(b), static_cast<double2*>(x), tol,
+              restart, inner_tol, inner_maxit, max_outer, info);
+  } else {
+    DevBuf db(n * sizeof(double2)), dx(n * sizeof(double2));
+    VP_HIP(hipMemcpyAsync(db.p, b, n * sizeof(double2), hipMemcpyHostToDevice, s));
+    ir_common(ops, M ? precs : nullptr, 1, side, db.as<double2>(), dx.as<double2>(), tol, restart, inner_tol,
+              inner_maxit, max_outer, info);
+    VP_HIP(hipMemcpyAsync(x, dx.p, n * sizeof(double2), hipMemcpyDeviceToHost, s));
+    VP_HIP(hipStreamSynchronize(s));
+  }
+  VP_API_END
+}
+
+int vp_solve_ir_batched(const vp_op* ops, const vp_prec* precs, int nsolve, int side, const void* b, void* x,
+                        double tol, int restart, double inner_tol, int inner_maxit, int max_outer,
+                        vp_gmres_info* infos) {
+  VP_API_BEGIN
+  VP_REQUIRE(ops && nsolve > 0 && b && x, ERR_INVALID, "bad batched arguments");
+  VP_REQUIRE(side == VP_PREC_LEFT || side == VP_PREC_RIGHT, ERR_INVALID, "side must be LEFT or RIGHT");
+  VP_REQUIRE(ops[0], ERR_INVALID, "ops[0] is NULL");
+  bool anyp = false, allp = true;
+  if (precs)
+    for (int p = 0; p < nsolve; ++p) {
+      anyp |= precs[p] != nullptr;
+      allp &= precs[p] != nullptr;
+    }
+  VP_REQUIRE(!anyp || allp, ERR_INVALID, "either every solve has a preconditioner or none does");
+  VP_HIP(hipSetDevice(ops[0]->ctx->device));
+  std::vector<vp_op_s*> o(ops, ops + nsolve);
+  std::vector<vp_prec_s*> q;
+  if (anyp) q.assign(precs, precs + nsolve);
+  ir_common(o.data(), anyp ? q.data() : nullptr, nsolve, side, static_cast<const double2*>(b),
+            static_cast<double2*>(x), tol, restart, inner_tol, inner_maxit, max_outer, infos);
+  VP_API_END
+}
+
+}  // extern "C"

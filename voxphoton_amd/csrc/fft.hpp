@@ -1,0 +1,249 @@
+// Workgroup-level mixed-radix FFTs staged through LDS (gfx950).
+//
+// A batch of 1D DFTs lives in LDS; every stage of the Stockham autosort
+// algorithm (Govindaraju et al., "High performance discrete Fourier transforms
+// on graphics processors", SC'08) reads each butterfly's R inputs from one
+// LDS buffer, twiddles, runs a radix-R DFT in registers and writes the other
+// buffer.
+// Radices 2, 3, 4, 5, 7, 8, 11, 13 and 16 are supported.  Forward transforms
+// use exp(-2*pi*i*jk/n), inverse ones exp(+2*pi*i*jk/n); neither normalises.
+#pragma once
+#include "common.hpp"
+
+namespace vp {
+
+// ---------------------------------------------------------------------------
+// Compile-time sines and cosines of 2*pi*num/den: octant reduction to
+// [0, pi/4] followed by Taylor series (exact to a few ulp in double).
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double taylor_sin(double x) {
+  double term = x, sum = x;
+  for (int k = 1; k < 14; ++k) { term *= -x * x / double((2 * k) * (2 * k + 1)); sum += term; }
+  return sum;
+}
+constexpr double taylor_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 14; ++k) { term *= -x * x / double((2 * k - 1) * (2 * k)); sum += term; }
+  return sum;
+}
+struct SinCos { double s, c; };
+constexpr SinCos sincos_turn(long num, long den) {
+  long n = ((num % den) + den) % den;
+  long q = 8 * n;
+  long o = q / den;             // octant 0..7
+  long r = q - o * den;         // remainder in [0, den)
+  double sa = 0, ca = 0;
+  if ((o & 1) == 0) {
+    double phi = (kPi / 4.0) * double(r) / double(den);
+    sa = taylor_sin(phi); ca = taylor_cos(phi);
+  } else {
+    double psi = (kPi / 4.0) * double(den - r) / double(den);
+    sa = taylor_cos(psi); ca = taylor_sin(psi);
+  }
+  long Q = o / 2;
+  SinCos out{0, 0};
+  if (Q == 0) { out.s = sa; out.c = ca; }
+  else if (Q == 1) { out.s = ca; out.c = -sa; }
+  else if (Q == 2) { out.s = -sa; out.c = -ca; }
+  else { out.s = -ca; out.c = sa; }
+  return out;
+}
+
+template <int R> struct RootTable {
+  double c[R], s[R];
+  constexpr RootTable() : c(), s() {
+    for (int i = 0; i < R; ++i) { SinCos v = sincos_turn(i, R); c[i] = v.c; s[i] = v.s; }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Radix-R butterflies on registers (in place).
+// ---------------------------------------------------------------------------
+template <bool INV, typename V>
+__device__ __forceinline__ void bf2(V& a, V& b) { V t = a; a = cadd(t, b); b = csub(t, b); }
+
+template <bool INV, typename V>
+__device__ __forceinline__ void dft4(V* v) {
+  V s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+  V s13 = cadd(v[1], v[3]), d13 = mul_mi<INV>(csub(v[1], v[3]));
+  v[0] = cadd(s02, s13); v[2] = csub(s02, s13);
+  v[1] = cadd(d02, d13); v[3] = csub(d02, d13);
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ C<T> rot8(C<T> a, int k) {
+  // a * exp(-+ 2 pi i k / 8), k in 0..3
+  const T h = T(0.70710678118654752440084436210484903928);
+  if (k == 0) return a;
+  if (k == 2) return mul_mi<INV>(a);
+  C<T> w;
+  if (k == 1) { w.x = h; w.y = INV ? h : -h; }
+  else { w.x = -h; w.y = INV ? h : -h; }
+  return cmul(a, w);
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft8(C<T>* v) {
+  C<T> e[4] = {v[0], v[2], v[4], v[6]};
+  C<T> o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<INV>(e); dft4<INV>(o);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    C<T> t = rot8<T, INV>(o[k], k);
+    v[k] = cadd(e[k], t); v[k + 4] = csub(e[k], t);
+  }
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft16(C<T>* v) {
+  C<T> e[8], o[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) { e[k] = v[2 * k]; o[k] = v[2 * k + 1]; }
+  dft8<T, INV>(e); dft8<T, INV>(o);
+  constexpr RootTable<16> rt{};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    C<T> w; w.x = T(rt.c[k]); w.y = INV ? T(rt.s[k]) : T(-rt.s[k]);
+    C<T> t = cmul(o[k], w);
+    v[k] = cadd(e[k], t); v[k + 8] = csub(e[k], t);
+  }
+}
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft_odd(C<T>* v) {
+  constexpr int H = (R - 1) / 2;
+  constexpr RootTable<R> rt{};
+  C<T> a[H], b[H];
+#pragma unroll
+  for (int j = 1; j <= H; ++j) { a[j - 1] = cadd(v[j], v[R - j]); b[j - 1] = csub(v[j], v[R - j]); }
+  C<T> x0 = v[0];
+  C<T> y0 = x0;
+#pragma unroll
+  for (int j = 0; j < H; ++j) y0 = cadd(y0, a[j]);
+#pragma unroll
+  for (int k = 1; k <= H; ++k) {
+    T re = x0.x, im = x0.y, sr = T(0), si = T(0);
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+      const T c = T(rt.c[(j * k) % R]);
+      const T s = T(rt.s[(j * k) % R]);
+      re += a[j - 1].x * c; im += a[j - 1].y * c;
+      sr += s * b[j - 1].y; si -= s * b[j - 1].x;
+    }
+    if (INV) { sr = -sr; si = -si; }
+    v[k] = mk<T>(re + sr, im + si);
+    v[R - k] = mk<T>(re - sr, im - si);
+  }
+  v[0] = y0;
+}
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft(C<T>* v) {
+  if constexpr (R == 2) bf2<INV>(v[0], v[1]);
+  else if constexpr (R == 4) dft4<INV>(v);
+  else if constexpr (R == 8) dft8<T, INV>(v);
+  else if constexpr (R == 16) dft16<T, INV>(v);
+  else dft_odd<T, R, INV>(v);
+}
+
+// ---------------------------------------------------------------------------
+// Plans and LDS layouts
+// ---------------------------------------------------------------------------
+constexpr int kMaxStages = 12;
+
+// One 1D DFT length.  The twiddle table passed alongside holds
+// exp(-2*pi*i*k/P) for k < P with P = n * tws.  The radix of stage s sits in
+// bits [5s, 5s+5) of ``code`` (a scalar, so kernels never index an array).
+struct FftPlan {
+  int n;
+  int nst;
+  int tws;
+  int pad;
+  unsigned long long code;
+  __host__ __device__ __forceinline__ int radix(int s) const { return int((code >> (5 * s)) & 31ull); }
+};
+
+// Batch layout in LDS: transform beta starts at (beta / bi) * so + (beta % bi) * si
+// and its element j sits at start + j * st.
+struct LdsLayout {
+  int bi, so, si, st;
+  __device__ __forceinline__ int off(int beta) const {
+    int hi = beta / bi;
+    return hi * so + (beta - hi * bi) * si;
+  }
+};
+
+template <typename T, bool INV>
+__device__ __forceinline__ C<T> twiddle(const C<T>* __restrict__ tw, int idx) {
+  C<T> w = tw[idx];
+  if (INV) w.y = -w.y;
+  return w;
+}
+
+// One Stockham stage from ``src`` to ``dst`` (distinct LDS buffers with the
+// same layout).  Each thread loops over butterflies, so only R values are
+// live at a time; the caller synchronises between stages.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void stage(const C<T>* src, C<T>* dst, int n, int Ns, int nbatch, const LdsLayout& lay,
+                                      const C<T>* __restrict__ tw, int twmul) {
+  const int nb = n / R;
+  const int total = nb * nbatch;
+  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+    const int beta = b / nb;
+    const int j = b - beta * nb;
+    const int off = lay.off(beta);
+    C<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[off + (j + r * nb) * lay.st];
+    const int jm = j % Ns;
+    if (Ns > 1) {
+      // w = exp(-+ 2 pi i jm / (Ns R)) = table[jm * (n / (Ns R)) * tws]
+      const int step = jm * twmul;
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<T, INV>(tw, r * step));
+    }
+    dft<T, R, INV>(v);
+    const int base = (j - jm) * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[off + (base + r * Ns) * lay.st] = v[r];
+  }
+}
+
+// Runs all stages of ``pl`` on ``nbatch`` transforms laid out per ``lay``,
+// ping-ponging between ``a`` and ``b``; returns the buffer holding the
+// result.  Ends with a barrier.  Radices 13 and 16 are single precision only.
+template <typename T, bool INV>
+__device__ C<T>* lds_fft(C<T>* a, C<T>* b, const FftPlan& pl, int nbatch, const LdsLayout& lay,
+                         const C<T>* __restrict__ tw) {
+  int Ns = 1;
+  C<T>* src = a;
+  C<T>* dst = b;
+  for (int s = 0; s < pl.nst; ++s) {
+    const int R = pl.radix(s);
+    const int twmul = (pl.n / (Ns * R)) * pl.tws;
+    switch (R) {
+      case 2: stage<T, 2, INV>(src, dst, pl.n, Ns, nbatch, lay, tw, twmul); break;
+      case 3: stage<T, 3, INV>(src, dst, pl.n, Ns, nbatch, lay, tw, twmul); break;
+      case 4: stage<T, 4, INV>(src, dst, pl.n, Ns, nbatch, lay, tw, twmul); break;
+      case 5: stage<T, 5, INV>(src, dst, pl.n, Ns, nbatch, lay, tw, twmul); break;
+      case 7: stage<T, 7, INV>(src, dst, pl.n, Ns, nbatch, lay, tw, twmul); break;
+      case 8: stage<T, 8, INV>(src, dst, pl.n, Ns, nbatch, lay, tw, twmul); break;
+      case 11: stage<T, 11, INV>(src, dst, pl.n, Ns, nbatch, lay, tw, twmul); break;
+      case 13:
+        if constexpr (sizeof(T) == 4) stage<T, 13, INV>(src, dst, pl.n, Ns, nbatch, lay, tw, twmul);
+        break;
+      case 16:
+        if constexpr (sizeof(T) == 4) stage<T, 16, INV>(src, dst, pl.n, Ns, nbatch, lay, tw, twmul);
+        break;
+      default: break;
+    }
+    __syncthreads();
+    C<T>* t = src; src = dst; dst = t;
+    Ns *= R;
+  }
+  return src;
+}
+
+}  // namespace vp

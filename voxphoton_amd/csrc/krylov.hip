@@ -1,0 +1,627 @@
+// Restarted GMRES, mixed-precision helpers and batched problem sets.
+//
+// Arnoldi (pgmres_iter.m:60-127) runs as fused device kernels:
+//   k_dots       h1 = V^H w                       (FP64 partial sums per block)
+//   k_axpy_dots  w -= V h1 ; h2 = V^H w           (second Gram-Schmidt pass)
+//   k_axpy_norm  w -= V h2 ; ||w||^2
+//   k_arnoldi    H(:,k) = h1 + h2, ||w||, Givens rotations, residual estimate
+//   k_scale      v_{k+1} = w / ||w||
+// All reductions use fixed-order trees (no atomics), so a problem's result
+// does not depend on which other problems share the launch.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include "krylov.hpp"
+
+namespace vp {
+
+constexpr int KT = 256;    // threads per Krylov block
+constexpr int RPT = 4;     // rows per thread
+constexpr int RB = KT * RPT;
+constexpr int JC = 8;      // columns per register pass
+
+__device__ __forceinline__ double2 to_d(float2 v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ double2 to_d(double2 v) { return v; }
+template <typename T> __device__ __forceinline__ C<T> from_d(double2 v);
+template <> __device__ __forceinline__ float2 from_d<float>(double2 v) { return make_float2(float(v.x), float(v.y)); }
+template <> __device__ __forceinline__ double2 from_d<double>(double2 v) { return v; }
+
+template <int J>
+__device__ __forceinline__ void block_reduce_store(double2 (&acc)[J], double2* out, int64_t stride, int valid) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      acc[j].x += __shfl_xor(acc[j].x, off);
+      acc[j].y += __shfl_xor(acc[j].y, off);
+    }
+  }
+  __shared__ double2 sm[KT / 64][J];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < J; ++j) sm[wave][j] = acc[j];
+  __syncthreads();
+  if (threadIdx.x < valid) {
+    double2 s = make_double2(0.0, 0.0);
+    for (int w = 0; w < KT / 64; ++w) { s.x += sm[w][threadIdx.x].x; s.y += sm[w][threadIdx.x].y; }
+    out[threadIdx.x * stride] = s;
+  }
+  __syncthreads();
+}
+
+// partial dots of w (registers) with columns 0..k-1 of V
+template <typename T>
+__device__ __forceinline__ void dots_core(const C<T>* __restrict__ Vp, int64_t vcol, int k, const double2 (&wv)[RPT],
+                                          int64_t base, int64_t n, double2* part, int nblk) {
+  for (int j0 = 0; j0 < k; j0 += JC) {
+    double2 acc[JC];
+#pragma unroll
+    for (int jj = 0; jj < JC; ++jj) acc[jj] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int t = 0; t < RPT; ++t) {
+      const int64_t i = base + t * KT + threadIdx.x;
+      if (i < n) {
+#pragma unroll
+        for (int jj = 0; jj < JC; ++jj) {
+          if (j0 + jj < k) {
+            const double2 v = to_d(Vp[int64_t(j0 + jj) * vcol + i]);
+            acc[jj].x += v.x * wv[t].x + v.y * wv[t].y;     // conj(v) * w
+            acc[jj].y += v.x * wv[t].y - v.y * wv[t].x;
+          }
+        }
+      }
+    }
+    block_reduce_store<JC>(acc, part + int64_t(j0) * nblk, nblk, min(JC, k - j0));
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(KT) k_dots(const C<T>* __restrict__ V, int64_t vcol, int64_t vprob, int k,
+                                            const C<T>* __restrict__ w, int64_t wprob, int64_t n,
+                                            double2* __restrict__ partials, int maxcols,
+                                            const int* __restrict__ active) {
+  const int p = blockIdx.y;
+  if (active && !active[p]) return;
+  const C<T>* Vp = V + p * vprob;
+  const C<T>* wp = w + p * wprob;
+  const int nblk = gridDim.x;
+  const int64_t base = int64_t(blockIdx.x) * RB;
+  double2 wv[RPT];
+#pragma unroll
+  for (int t = 0; t < RPT; ++t) {
+    const int64_t i = base + t * KT + threadIdx.x;
+    wv[t] = i < n ? to_d(wp[i]) : make_double2(0.0, 0.0);
+  }
+  dots_core<T>(Vp, vcol, k, wv, base, n, partials + (int64_t(p) * maxcols) * nblk + blockIdx.x, nblk);
+}
+
+// w -= V h (h: FP64 per problem), then dots of the new w with V (mode 0) or
+// its squared norm (mode 1).
+template <typename T>
+__global__ void __launch_bounds__(KT) k_axpy_dots(const C<T>* __restrict__ V, int64_t vcol, int64_t vprob, int k,
+                                                 const double2* __restrict__ h, C<T>* __restrict__ w,
+                                                 int64_t wprob, int64_t n, double2* __restrict__ partials,
+                                                 int maxcols, int mode, const int* __restrict__ active) {
+  const int p = blockIdx.y;
+  if (active && !active[p]) return;
+  const C<T>* Vp = V + p * vprob;
+  C<T>* wp = w + p * wprob;
+  const double2* hp = h + int64_t(p) * maxcols;
+  const int nblk = gridDim.x;
+  const int64_t base = int64_t(blockIdx.x) * RB;
+  double2 wv[RPT];
+#pragma unroll
+  for (int t = 0; t < RPT; ++t) {
+    const int64_t i = base + t * KT + threadIdx.x;
+    if (i < n) {
+      double2 s = to_d(wp[i]);
+      for (int j = 0; j < k; ++j) {
+        const double2 v = to_d(Vp[int64_t(j) * vcol + i]);
+        const double2 hj = hp[j];
+        s.x -= v.x * hj.x - v.y * hj.y;
+        s.y -= v.x * hj.y + v.y * hj.x;
+      }
+      const C<T> st = from_d<T>(s);
+      wp[i] = st;
+      wv[t] = to_d(st);
+    } else {
+      wv[t] = make_double2(0.0, 0.0);
+    }
+  }
+  double2* part = partials + (int64_t(p) * maxcols) * nblk + blockIdx.x;
+  if (mode == 0) {
+    dots_core<T>(Vp, vcol, k, wv, base, n, part, nblk);
+  } else {
+    double2 acc[1] = {make_double2(0.0, 0.0)};
+#pragma unroll
+    for (int t = 0; t < RPT; ++t) acc[0].x += wv[t].x * wv[t].x + wv[t].y * wv[t].y;
+    block_reduce_store<1>(acc, part, nblk, 1);
+  }
+}
+
+// out[p][j] = sum over blocks of partials[p][j][blk] (fixed order)
+__global__ void k_reduce_cols(const double2* __restrict__ partials, int nblk, int ncol, int maxcols,
+                              double2* __restrict__ out, const int* __restrict__ active) {
+  const int p = blockIdx.x;
+  if (active && !active[p]) return;
+  for (int j = threadIdx.x; j < ncol; j += blockDim.x) {
+    const double2* src = partials + (int64_t(p) * maxcols + j) * nblk;
+    double2 s = make_double2(0.0, 0.0);
+    for (int b = 0; b < nblk; ++b) { s.x += src[b].x; s.y += src[b].y; }
+    out[int64_t(p) * maxcols + j] = s;
+  }
+}
+
+struct ArnoldiState {
+  double2* H;       // [p][(m+1) * m] column-major
+  double* cs;       // [p][m]
+  double2* sn;      // [p][m]
+  double2* g;       // [p][m+1]
+  double2* h1;      // [p][maxcols]
+  double2* h2;      // [p][maxcols]
+  double2* nrm;     // [p][maxcols] (entry 0: ||w||^2 in .x)
+  double* tolabs;   // [p]
+  double* invnorm;  // [p]
+  double* est;      // [p] |g(k+1)|
+  int* conv;        // [p] 1 when est < tolabs or breakdown
+  int m, maxcols;
+};
+
+__global__ void k_arnoldi(ArnoldiState st, int k, const int* __restrict__ active, int nprob) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob) return;
+  if (active && !active[p]) return;
+  const int m = st.m;
+  double2* col = st.H + int64_t(p) * (m + 1) * m + int64_t(k) * (m + 1);
+  const double2* h1 = st.h1 + int64_t(p) * st.maxcols;
+  const double2* h2 = st.h2 + int64_t(p) * st.maxcols;
+  for (int i = 0; i <= k; ++i) col[i] = make_double2(h1[i].x + h2[i].x, h1[i].y + h2[i].y);
+  const double hn = sqrt(fmax(st.nrm[int64_t(p) * st.maxcols].x, 0.0));
+  col[k + 1] = make_double2(hn, 0.0);
+  double* cs = st.cs + int64_t(p) * m;
+  double2* sn = st.sn + int64_t(p) * m;
+  for (int i = 0; i < k; ++i) {
+    const double2 a = col[i], b = col[i + 1];
+    const double c = cs[i];
+    const double2 s = sn[i];
+    // [a; b] <- [c a + s b ; -conj(s) a + c b]
+    const double2 t = make_double2(c * a.x + s.x * b.x - s.y * b.y, c * a.y + s.x * b.y + s.y * b.x);
+    const double2 u = make_double2(-(s.x * a.x + s.y * a.y) + c * b.x, -(s.x * a.y - s.y * a.x) + c * b.y);
+    col[i] = t;
+    col[i + 1] = u;
+  }
+  const double2 a = col[k];
+  const double bb = col[k + 1].x;   // real, >= 0
+  const double aa = sqrt(a.x * a.x + a.y * a.y);
+  double c;
+  double2 s, r;
+  if (bb == 0.0) {
+    c = 1.0; s = make_double2(0.0, 0.0); r = a;
+  } else if (aa == 0.0) {
+    c = 0.0; s = make_double2(1.0, 0.0); r = make_double2(bb, 0.0);
+  } else {
+    const double rho = sqrt(aa * aa + bb * bb);
+    c = aa / rho;
+    s = make_double2(a.x / aa * bb / rho, a.y / aa * bb / rho);   // (a/|a|) * conj(b) / rho, b real
+    r = make_double2(a.x / aa * rho, a.y / aa * rho);
+  }
+  cs[k] = c;
+  sn[k] = s;
+  col[k] = r;
+  col[k + 1] = make_double2(0.0, 0.0);
+  double2* g = st.g + int64_t(p) * (m + 1);
+  const double2 gk = g[k];
+  g[k] = make_double2(c * gk.x, c * gk.y);
+  g[k + 1] = make_double2(-(s.x * gk.x + s.y * gk.y), -(s.x * gk.y - s.y * gk.x));
+  const double est = sqrt(g[k + 1].x * g[k + 1].x + g[k + 1].y * g[k + 1].y);
+  st.est[p] = est;
+  st.invnorm[p] = hn > 0.0 ? 1.0 / hn : 0.0;
+  st.conv[p] = (est < st.tolabs[p]) || !(hn > 0.0) ? 1 : 0;
+}
+
+// y = R^{-1} g for the first kk[p] columns (back substitution, FP64)
+__global__ void k_backsolve(ArnoldiState st, const int* __restrict__ kk, double2* __restrict__ y, int nprob,
+                            const int* __restrict__ active) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob) return;
+  if (active && !active[p]) return;
+  const int m = st.m;
+  const int K = kk[p];
+  const double2* H = st.H + int64_t(p) * (m + 1) * m;
+  const double2* g = st.g + int64_t(p) * (m + 1);
+  double2* yp = y + int64_t(p) * st.maxcols;
+  for (int i = K - 1; i >= 0; --i) {
+    double2 s = g[i];
+    for (int j = i + 1; j < K; ++j) {
+      const double2 h = H[int64_t(j) * (m + 1) + i];
+      s.x -= h.x * yp[j].x - h.y * yp[j].y;
+      s.y -= h.x * yp[j].y + h.y * yp[j].x;
+    }
+    const double2 d = H[int64_t(i) * (m + 1) + i];
+    const double den = d.x * d.x + d.y * d.y;
+    yp[i] = den > 0 ? make_double2((s.x * d.x + s.y * d.y) / den, (s.y * d.x - s.x * d.y) / den)
+                    : make_double2(0.0, 0.0);
+  }
+}
+
+// out (+)= sum_{j < kk[p]} V_j y_j
+template <typename T>
+__global__ void __launch_bounds__(KT) k_lincomb(const C<T>* __restrict__ V, int64_t vcol, int64_t vprob,
+                                               const double2* __restrict__ y, const int* __restrict__ kk,
+                                               int maxcols, C<T>* __restrict__ out, int64_t oprob, int64_t n,
+                                               int accumulate, const int* __restrict__ active) {
+  const int p = blockIdx.y;
+  if (active && !active[p]) return;
+  const C<T>* Vp = V + p * vprob;
+  const double2* yp = y + int64_t(p) * maxcols;
+  C<T>* op = out + p * oprob;
+  const int K = kk[p];
+  for (int64_t i = blockIdx.x * int64_t(KT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * KT) {
+    double2 s = accumulate ? to_d(op[i]) : make_double2(0.0, 0.0);
+    for (int j = 0; j < K; ++j) {
+      const double2 v = to_d(Vp[int64_t(j) * vcol + i]);
+      s.x += v.x * yp[j].x - v.y * yp[j].y;
+      s.y += v.x * yp[j].y + v.y * yp[j].x;
+    }
+    op[i] = from_d<T>(s);
+  }
+}
+
+// x[p] *= scale[p] (scale FP64, inverted when invert != 0)
+template <typename T>
+__global__ void k_scale(C<T>* __restrict__ x, int64_t xprob, int64_t n, const double* __restrict__ scale,
+                        int invert, const int* __restrict__ active) {
+  const int p = blockIdx.y;
+  if (active && !active[p]) return;
+  double sc = scale[p];
+  if (invert) sc = sc > 0 ? 1.0 / sc : 0.0;
+  C<T>* xp = x + p * xprob;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double2 v = to_d(xp[i]);
+    xp[i] = from_d<T>(make_double2(v.x * sc, v.y * sc));
+  }
+}
+
+// r = b - y
+template <typename T>
+__global__ void k_sub(const C<T>* __restrict__ b, const C<T>* __restrict__ y, C<T>* __restrict__ r, int64_t n,
+                      int nprob, const int* __restrict__ active) {
+  const int64_t total = n * nprob;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x) {
+    if (active && !active[e / n]) continue;
+    r[e] = csub(b[e], y[e]);
+  }
+}
+
+template <typename T>
+__global__ void k_copy(const C<T>* __restrict__ a, C<T>* __restrict__ b, int64_t total) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x)
+    b[e] = a[e];
+}
+
+template <typename T>
+__global__ void k_zero(C<T>* __restrict__ a, int64_t total) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x)
+    a[e] = czero<T>();
+}
+
+__global__ void k_gmres_init(ArnoldiState st, const double* __restrict__ beta, int nprob,
+                             const int* __restrict__ active) {
+  const int p = blockIdx.x;
+  if (p >= nprob || (active && !active[p])) return;
+  const int m = st.m;
+  for (int i = threadIdx.x; i <= m; i += blockDim.x)
+    st.g[int64_t(p) * (m + 1) + i] = make_double2(i == 0 ? beta[p] : 0.0, 0.0);
+}
+
+// partial |x|^2 per block then per-problem total
+template <typename T>
+__global__ void __launch_bounds__(KT) k_norm_partial(const C<T>* __restrict__ x, int64_t xprob, int64_t n,
+                                                    double2* __restrict__ partials) {
+  const int p = blockIdx.y;
+  const C<T>* xp = x + p * xprob;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  for (int64_t i = blockIdx.x * int64_t(KT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * KT) {
+    const double2 v = to_d(xp[i]);
+    acc[0].x += v.x * v.x + v.y * v.y;
+  }
+  block_reduce_store<1>(acc, partials + int64_t(p) * gridDim.x + blockIdx.x, 1, 1);
+}
+
+__global__ void k_norm_final(const double2* __restrict__ partials, int nblk, double* __restrict__ out) {
+  const int p = blockIdx.x;
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += partials[int64_t(p) * nblk + b].x;
+    out[p] = sqrt(s);
+  }
+}
+
+template <typename T>
+void dev_norms(const C<T>* x, int64_t n, int nprob, int64_t vstride, double* d_out, hipStream_t s) {
+  const int nblk = int(std::min<int64_t>((n + KT - 1) / KT, 1024));
+  DevBuf part(size_t(nblk) * nprob * sizeof(double2));
+  hipLaunchKernelGGL(k_norm_partial<T>, dim3(nblk, nprob), dim3(KT), 0, s, x, vstride, n, part.as<double2>());
+  hipLaunchKernelGGL(k_norm_final, dim3(nprob), dim3(64), 0, s, part.as<double2>(), nblk, d_out);
+  VP_HIP(hipGetLastError());
+  VP_HIP(hipStreamSynchronize(s));   // part is freed on return
+}
+template void dev_norms<float>(const float2*, int64_t, int, int64_t, double*, hipStream_t);
+template void dev_norms<double>(const double2*, int64_t, int, int64_t, double*, hipStream_t);
+
+// ----- mixed precision helpers -------------------------------------------------
+__global__ void k_resid_d(const double2* __restrict__ b, const double2* __restrict__ ax, double2* __restrict__ r,
+                          int64_t total) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x)
+    r[e] = csub(b[e], ax[e]);
+}
+void dev_residual_d(const double2* b, const double2* Ax, double2* r, int64_t n, int nprob, hipStream_t s) {
+  const int64_t total = n * nprob;
+  hipLaunchKernelGGL(k_resid_d, dim3(int(std::min<int64_t>((total + 255) / 256, 8192))), dim3(256), 0, s, b, Ax, r,
+                     total);
+  VP_HIP(hipGetLastError());
+}
+__global__ void k_scale_to_f(const double2* __restrict__ r, const double* __restrict__ nrm, float2* __restrict__ o,
+                             int64_t n, int nprob, double extra) {
+  const int64_t total = n * nprob;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x) {
+    const double d = nrm[e / n];
+    const double sc = d > 0 ? extra / d : 0.0;
+    o[e] = make_float2(float(r[e].x * sc), float(r[e].y * sc));
+  }
+}
+void dev_scale_to_f(const double2* r, const double* d_norm, float2* out, int64_t n, int nprob, double extra,
+                    hipStream_t s) {
+  const int64_t total = n * nprob;
+  hipLaunchKernelGGL(k_scale_to_f, dim3(int(std::min<int64_t>((total + 255) / 256, 8192))), dim3(256), 0, s, r,
+                     d_norm, out, n, nprob, extra);
+  VP_HIP(hipGetLastError());
+}
+__global__ void k_update_from_f(double2* __restrict__ x, const float2* __restrict__ d, const double* __restrict__ nrm,
+                                int64_t n, int nprob, double extra, const int* __restrict__ active) {
+  const int64_t total = n * nprob;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total; e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t p = e / n;
+    if (active && !active[p]) continue;
+    const double sc = nrm[p] * extra;
+    x[e].x += double(d[e].x) * sc;
+    x[e].y += double(d[e].y) * sc;
+  }
+}
+void dev_update_from_f(double2* x, const float2* d, const double* d_norm, int64_t n, int nprob, double extra,
+                       const int* active, hipStream_t s) {
+  const int64_t total = n * nprob;
+  hipLaunchKernelGGL(k_update_from_f, dim3(int(std::min<int64_t>((total + 255) / 256, 8192))), dim3(256), 0, s, x,
+                     d, d_norm, n, nprob, extra, active);
+  VP_HIP(hipGetLastError());
+}
+
+// ----- operator / preconditioner application over a problem set ------------------
+template <typename T>
+void ps_apply_A(ProblemSet<T>& ps, const C<T>* x, int64_t xs, C<T>* y, int64_t ys, const int* active) {
+  VecArgs<T> va{x, y, xs, ys, active};
+  launch_matvec<T>(ps.g, ps.plan, ps.probs.template as<ProbPtrs<T>>(), va, ps.nprob, 0, ps.twx, ps.twy, ps.twz,
+                   ps.stream);
+}
+template <typename T>
+void ps_apply_M(ProblemSet<T>& ps, const C<T>* x, int64_t xs, C<T>* y, int64_t ys, const int* active) {
+  PcArgs<T> a = ps.pa;
+  a.x = x;
+  a.y = y;
+  a.xstride = xs;
+  a.ystride = ys;
+  a.active = active;
+  launch_prec<T>(ps.pg, a, ps.nprob, ps.stream);
+}
+template void ps_apply_A<float>(ProblemSet<float>&, const float2*, int64_t, float2*, int64_t, const int*);
+template void ps_apply_A<double>(ProblemSet<double>&, const double2*, int64_t, double2*, int64_t, const int*);
+template void ps_apply_M<float>(ProblemSet<float>&, const float2*, int64_t, float2*, int64_t, const int*);
+template void ps_apply_M<double>(ProblemSet<double>&, const double2*, int64_t, double2*, int64_t, const int*);
+
+// b[p] (+)= a[p] over problems, with separate strides
+template <typename T>
+__global__ void k_copy2(const C<T>* __restrict__ a, int64_t as, C<T>* __restrict__ b, int64_t bs, int64_t n,
+                        int accumulate, const int* __restrict__ active) {
+  const int p = blockIdx.y;
+  if (active && !active[p]) return;
+  const C<T>* ap = a + p * as;
+  C<T>* bp = b + p * bs;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    bp[i] = accumulate ? cadd(bp[i], ap[i]) : ap[i];
+}
+
+// ----- GMRES driver ---------------------------------------------------------------
+static int grid1(int64_t total) { return int(std::min<int64_t>((total + 255) / 256, 8192)); }
+
+template <typename T>
+void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& prm, GmresResult& res) {
+  const int np = ps.nprob;
+  const int64_t n = ps.n;
+  const int m = prm.restart;
+  const int maxcols = m + 2;
+  hipStream_t s = ps.stream;
+  const int nblk = int((n + RB - 1) / RB);
+  const bool left = ps.has_prec && prm.side == 0;
+  const bool right = ps.has_prec && prm.side == 1;
+
+  DevBuf V(size_t(m + 1) * n * np * sizeof(C<T>));
+  DevBuf t1(size_t(n) * np * sizeof(C<T>)), t2(size_t(n) * np * sizeof(C<T>));
+  DevBuf part(size_t(np) * maxcols * nblk * sizeof(double2));
+  DevBuf H(size_t(np) * (m + 1) * m * sizeof(double2)), cs(size_t(np) * m * sizeof(double)),
+      sn(size_t(np) * m * sizeof(double2)), g(size_t(np) * (m + 1) * sizeof(double2));
+  DevBuf h1(size_t(np) * maxcols * sizeof(double2)), h2(size_t(np) * maxcols * sizeof(double2)),
+      nr(size_t(np) * maxcols * sizeof(double2)), yv(size_t(np) * maxcols * sizeof(double2));
+  DevBuf tolabs(np * sizeof(double)), invn(np * sizeof(double)), est(np * sizeof(double)), conv(np * sizeof(int)),
+      kk(np * sizeof(int)), beta(np * sizeof(double)), bn(np * sizeof(double)), act(np * sizeof(int));
+  C<T>* Vb = V.as<C<T>>();
+  const int64_t vprob = int64_t(m + 1) * n;
+  ArnoldiState st{H.as<double2>(), cs.as<double>(), sn.as<double2>(), g.as<double2>(), h1.as<double2>(),
+                  h2.as<double2>(), nr.as<double2>(), tolabs.as<double>(), invn.as<double>(), est.as<double>(),
+                  conv.as<int>(), m, maxcols};
+  res.flag.assign(np, 1);
+  res.iter_inner.assign(np, 0);
+  res.iter_outer.assign(np, 0);
+  res.total.assign(np, 0);
+  res.relres.assign(np, 0.0);
+  res.resvec.assign(np, {});
+
+  std::vector<int> hact(np, 1);
+  auto upload_act = [&]() {
+    VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+  };
+  upload_act();
+  const int* dact = act.as<int>();
+  // bnorm: ||M^-1 b|| (left) or ||b||
+  if (left) {
+    ps_apply_M<T>(ps, b, n, t1.as<C<T>>(), n, nullptr);
+    dev_norms<T>(t1.as<C<T>>(), n, np, n, bn.as<double>(), s);
+  } else {
+    dev_norms<T>(b, n, np, n, bn.as<double>(), s);
+  }
+  std::vector<double> hbn(np), hbeta(np), hest(np);
+  std::vector<int> hconv(np);
+  VP_HIP(hipMemcpy(hbn.data(), bn.p, np * sizeof(double), hipMemcpyDeviceToHost));
+  // initial residual into V_0 slot of each problem
+  auto residual_into_V0 = [&](bool x_nonzero) {
+    C<T>* v0 = Vb;  // problem p at Vb + p * vprob
+    if (x_nonzero) {
+      ps_apply_A<T>(ps, x, n, t1.as<C<T>>(), n, dact);
+      hipLaunchKernelGGL(k_sub<T>, dim3(grid1(n * np)), dim3(256), 0, s, b, t1.as<C<T>>(), t2.as<C<T>>(), n, np,
+                         dact);
+    } else {
+      hipLaunchKernelGGL(k_copy<T>, dim3(grid1(n * np)), dim3(256), 0, s, b, t2.as<C<T>>(), n * np);
+    }
+    VP_HIP(hipGetLastError());
+    // precondition (left) or copy into the strided V_0 slots
+    if (left) {
+      ps_apply_M<T>(ps, t2.as<C<T>>(), n, v0, vprob, dact);
+    } else {
+      hipLaunchKernelGGL(k_copy2<T>, dim3(grid1(n), np), dim3(256), 0, s, t2.as<C<T>>(), n, v0, vprob, n, 0, dact);
+      VP_HIP(hipGetLastError());
+    }
+    dev_norms<T>(v0, n, np, vprob, beta.as<double>(), s);
+    VP_HIP(hipMemcpy(hbeta.data(), beta.p, np * sizeof(double), hipMemcpyDeviceToHost));
+  };
+  if (!prm.x0_nonzero) {
+    hipLaunchKernelGGL(k_zero<T>, dim3(grid1(n * np)), dim3(256), 0, s, x, n * np);
+    VP_HIP(hipGetLastError());
+  }
+  residual_into_V0(prm.x0_nonzero);
+  std::vector<double> last(np);
+  for (int p = 0; p < np; ++p) {
+    last[p] = hbn[p] > 0 ? hbeta[p] / hbn[p] : 0.0;
+    res.resvec[p].push_back(last[p]);
+    hact[p] = (last[p] > prm.tol) ? 1 : 0;
+  }
+  {
+    std::vector<double> ta(np);
+    for (int p = 0; p < np; ++p) ta[p] = prm.tol * hbn[p];
+    VP_HIP(hipMemcpy(tolabs.p, ta.data(), np * sizeof(double), hipMemcpyHostToDevice));
+  }
+  int outit = 0;
+  std::vector<int> hk(np, 0);
+  while (outit < prm.maxit) {
+    bool any = false;
+    for (int p = 0; p < np; ++p) any |= hact[p] != 0;
+    if (!any) break;
+    ++outit;
+    upload_act();
+    std::vector<int> inner(hact);   // active within this cycle
+    for (int p = 0; p < np; ++p) {
+      if (hact[p]) res.iter_outer[p] = outit;
+      hk[p] = 0;
+    }
+    DevBuf iact(np * sizeof(int));
+    VP_HIP(hipMemcpyAsync(iact.p, inner.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    const int* dia = iact.as<int>();
+    hipLaunchKernelGGL(k_scale<T>, dim3(grid1(n), np), dim3(256), 0, s, Vb, vprob, n, beta.as<double>(), 1, dia);
+    hipLaunchKernelGGL(k_gmres_init, dim3(np), dim3(64), 0, s, st, beta.as<double>(), np, dia);
+    VP_HIP(hipGetLastError());
+    for (int k = 0; k < m; ++k) {
+      C<T>* vk = Vb + int64_t(k) * n;
+      C<T>* w = Vb + int64_t(k + 1) * n;
+      if (right) {          // w = A M^-1 v_k
+        ps_apply_M<T>(ps, vk, vprob, t1.as<C<T>>(), n, dia);
+        ps_apply_A<T>(ps, t1.as<C<T>>(), n, w, vprob, dia);
+      } else if (left) {    // w = M^-1 A v_k
+        ps_apply_A<T>(ps, vk, vprob, t1.as<C<T>>(), n, dia);
+        ps_apply_M<T>(ps, t1.as<C<T>>(), n, w, vprob, dia);
+      } else {
+        ps_apply_A<T>(ps, vk, vprob, w, vprob, dia);
+      }
+      hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, w, vprob, n,
+                         part.as<double2>(), maxcols, dia);
+      hipLaunchKernelGGL(k_reduce_cols, dim3(np), dim3(64), 0, s, part.as<double2>(), nblk, k + 1, maxcols,
+                         h1.as<double2>(), dia);
+      hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, h1.as<double2>(), w,
+                         vprob, n, part.as<double2>(), maxcols, 0, dia);
+      hipLaunchKernelGGL(k_reduce_cols, dim3(np), dim3(64), 0, s, part.as<double2>(), nblk, k + 1, maxcols,
+                         h2.as<double2>(), dia);
+      hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, h2.as<double2>(), w,
+                         vprob, n, part.as<double2>(), maxcols, 1, dia);
+      hipLaunchKernelGGL(k_reduce_cols, dim3(np), dim3(64), 0, s, part.as<double2>(), nblk, 1, maxcols,
+                         nr.as<double2>(), dia);
+      hipLaunchKernelGGL(k_arnoldi, dim3((np + 63) / 64), dim3(64), 0, s, st, k, dia, np);
+      hipLaunchKernelGGL(k_scale<T>, dim3(grid1(n), np), dim3(256), 0, s, w, vprob, n, invn.as<double>(), 0, dia);
+      VP_HIP(hipGetLastError());
+      VP_HIP(hipMemcpyAsync(hest.data(), est.p, np * sizeof(double), hipMemcpyDeviceToHost, s));
+      VP_HIP(hipMemcpyAsync(hconv.data(), conv.p, np * sizeof(int), hipMemcpyDeviceToHost, s));
+      VP_HIP(hipStreamSynchronize(s));
+      bool still = false;
+      for (int p = 0; p < np; ++p) {
+        if (!inner[p]) continue;
+        hk[p] = k + 1;
+        res.total[p] += 1;
+        res.iter_inner[p] = k + 1;
+        last[p] = hbn[p] > 0 ? hest[p] / hbn[p] : 0.0;
+        res.resvec[p].push_back(last[p]);
+        if (hconv[p] || k == m - 1) inner[p] = 0;
+        else still = true;
+      }
+      VP_HIP(hipMemcpyAsync(iact.p, inner.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+      if (!still) break;
+    }
+    // solve the least squares and update x for every problem active in this cycle
+    VP_HIP(hipMemcpyAsync(kk.p, hk.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_backsolve, dim3((np + 63) / 64), dim3(64), 0, s, st, kk.as<int>(), yv.as<double2>(), np,
+                       dact);
+    if (right) {
+      // x += M^-1 (V y)
+      hipLaunchKernelGGL(k_lincomb<T>, dim3(grid1(n), np), dim3(KT), 0, s, Vb, n, vprob, yv.as<double2>(),
+                         kk.as<int>(), maxcols, t1.as<C<T>>(), n, n, 0, dact);
+      ps_apply_M<T>(ps, t1.as<C<T>>(), n, t2.as<C<T>>(), n, dact);
+      hipLaunchKernelGGL(k_copy2<T>, dim3(grid1(n), np), dim3(256), 0, s, t2.as<C<T>>(), n, x, n, n, 1, dact);
+    } else {
+      hipLaunchKernelGGL(k_lincomb<T>, dim3(grid1(n), np), dim3(KT), 0, s, Vb, n, vprob, yv.as<double2>(),
+                         kk.as<int>(), maxcols, x, n, n, 1, dact);
+    }
+    VP_HIP(hipGetLastError());
+    // restart: fresh residual (pgmres_iter.m:150 carries it by recurrence)
+    for (int p = 0; p < np; ++p) hact[p] = (hact[p] && last[p] > prm.tol) ? 1 : 0;
+    bool more = false;
+    for (int p = 0; p < np; ++p) more |= hact[p] != 0;
+    if (!more || outit >= prm.maxit) break;
+    upload_act();
+    residual_into_V0(true);
+  }
+  // final unpreconditioned relative residual in FP64 from the T data
+  std::vector<int> all(np, 1);
+  VP_HIP(hipMemcpyAsync(act.p, all.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+  ps_apply_A<T>(ps, x, n, t1.as<C<T>>(), n, act.as<int>());
+  hipLaunchKernelGGL(k_sub<T>, dim3(grid1(n * np)), dim3(256), 0, s, b, t1.as<C<T>>(), t2.as<C<T>>(), n, np,
+                     nullptr);
+  VP_HIP(hipGetLastError());
+  dev_norms<T>(t2.as<C<T>>(), n, np, n, beta.as<double>(), s);
+  dev_norms<T>(b, n, np, n, bn.as<double>(), s);
+  VP_HIP(hipMemcpy(hbeta.data(), beta.p, np * sizeof(double), hipMemcpyDeviceToHost));
+  VP_HIP(hipMemcpy(hbn.data(), bn.p, np * sizeof(double), hipMemcpyDeviceToHost));
+  for (int p = 0; p < np; ++p) {
+    res.relres[p] = hbn[p] > 0 ? hbeta[p] / hbn[p] : 0.0;
+    res.flag[p] = res.relres[p] <= prm.tol ? 0 : 1;
+  }
+}
+template void gmres_device<float>(ProblemSet<float>&, const float2*, float2*, const GmresParams&, GmresResult&);
+template void gmres_device<double>(ProblemSet<double>&, const double2*, double2*, const GmresParams&, GmresResult&);
+
+}  // namespace vp

@@ -1,0 +1,67 @@
+// Restarted GMRES on the device (pgmres.m:70-180 / pgmres_iter.m:22-150
+// semantics) with classical Gram-Schmidt + one reorthogonalisation pass and
+// FP64 Givens least squares, batched over independent problems.
+#pragma once
+#include <vector>
+#include "precond.hpp"
+
+namespace vp {
+
+// One set of independent problems of identical geometry solved together.
+// Vector p of a batch sits at base + p * n.
+template <typename T>
+struct ProblemSet {
+  int nprob = 0;
+  int64_t n = 0;
+  hipStream_t stream = nullptr;
+  // operator
+  Geometry g{};
+  MatvecPlan plan{};
+  DevBuf probs;                   // ProbPtrs<T>[nprob]
+  DevBuf works;                   // per-problem spectrum workspaces (may be empty: ops' own)
+  const C<T>* twx = nullptr;
+  const C<T>* twy = nullptr;
+  const C<T>* twz = nullptr;
+  // preconditioner (optional)
+  bool has_prec = false;
+  PcGeom pg{};
+  PcArgs<T> pa{};                 // x, y, active, strides filled at call time
+  DevBuf pblocks, pwork, pwork2;  // block pointer array, workspaces
+  DevBuf active;                  // int[nprob]
+};
+
+template <typename T>
+void ps_apply_A(ProblemSet<T>& ps, const C<T>* x, int64_t xs, C<T>* y, int64_t ys, const int* active);
+template <typename T>
+void ps_apply_M(ProblemSet<T>& ps, const C<T>* x, int64_t xs, C<T>* y, int64_t ys, const int* active);
+
+struct GmresParams {
+  int restart = 50;
+  double tol = 1e-6;
+  int maxit = 10;
+  int side = 0;        // 0 left, 1 right
+  bool x0_nonzero = false;
+  int resvec_cap = 0;  // per problem
+};
+
+struct GmresResult {
+  std::vector<int> flag, iter_inner, iter_outer, total;
+  std::vector<double> relres;
+  std::vector<std::vector<double>> resvec;
+};
+
+// b, x: device vectors (nprob * n) in precision T.  relres is computed in
+// FP64 from the T-precision data.
+template <typename T>
+void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& prm, GmresResult& res);
+
+// Device vector helpers (FP64 accumulation).
+template <typename T>
+void dev_norms(const C<T>* x, int64_t n, int nprob, int64_t vstride, double* d_out, hipStream_t s);
+void dev_residual_d(const double2* b, const double2* Ax, double2* r, int64_t n, int nprob, hipStream_t s);
+void dev_scale_to_f(const double2* r, const double* d_norm, float2* out, int64_t n, int nprob, double extra,
+                    hipStream_t s);
+void dev_update_from_f(double2* x, const float2* d, const double* d_norm, int64_t n, int nprob, double extra,
+                       const int* active, hipStream_t s);
+
+}  // namespace vp

@@ -1,0 +1,320 @@
+// JVIE N-operator matvec kernels: y = [Gram*Mr.*x - Mc.*N(x)](idx).
+//
+// Restates the arithmetic of pc/src_vie/src_mvp/mv_AN_const.m:102-132
+// ('notransp') and :66-99 ('transp') as three fused passes over the device
+// spectrum workspace X, laid out plane-major as X[kx][c][n][m]:
+//   K1 k_xfwd  : gather x through the voxel map, zero pad in x, DFT along x
+//                (two residue classes of length Lp/2), write X.
+//   K2 k_plane : per kx plane, per (ky, kz) residue class: fold + twiddle,
+//                2D DFT in LDS, 3x3 symmetric tensor multiply with fN,
+//                inverse 2D DFT, unfold into register accumulators; the
+//                cropped M x N plane is written back in place.
+//   K3 k_xinv  : inverse DFT along x per class, crop to L, epilogue
+//                Gram*Mr.*x - Mc.*N(x) and scatter to the unknown vector.
+// fN is stored prescaled by 1/(Lp*Mp*Np), so no pass normalises.
+#include "vp_internal.hpp"
+
+namespace vp {
+
+template <typename T>
+__device__ __forceinline__ C<T> load_unknown(const ProbPtrs<T>& P, const C<T>* __restrict__ X, const Geometry& g,
+                                             int64_t v, int c, int transp, int64_t& u_out) {
+  int64_t u;
+  int64_t pv;
+  if (g.structured) {
+    pv = P.pos[v];
+    u = pv < 0 ? -1 : pv + int64_t(c) * g.nS;
+  } else {
+    pv = P.pos[int64_t(c) * g.V + v];
+    u = pv;
+  }
+  u_out = u;
+  if (u < 0) return czero<T>();
+  C<T> xv = X[u];
+  if (transp) xv = cmul(P.mc[pv], cconj(xv));  // fftn(conj(conj(Mc).*J)) = fftn(Mc.*conj(J))
+  return xv;
+}
+
+// ----------------------------------------------------------------------------
+// K1: forward x pass.  Block = B consecutive m-pencils at fixed (n, c).
+// ----------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(1024) k_xfwd(const ProbPtrs<T>* __restrict__ probs, VecArgs<T> va, Geometry g,
+                                               MatvecPlan mp, int transp, const C<T>* __restrict__ twx) {
+  if (va.active && !va.active[blockIdx.y]) return;
+  const ProbPtrs<T> P = probs[blockIdx.y];
+  const C<T>* __restrict__ X = va.x + int64_t(blockIdx.y) * va.xstride;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W = reinterpret_cast<C<T>*>(smem_raw);
+  const int L = g.L, M = g.M, N = g.N;
+  const int Ls = g.x.S, Lp = g.x.P, B = mp.B, ld = mp.ldsx;
+  const int nmb = (M + B - 1) / B;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int mb = blk % nmb;
+  const int rest = blk / nmb;
+  const int n = rest % N;
+  const int c = rest / N;
+  const int m0 = mb * B;
+  const int64_t MN = int64_t(M) * N;
+  const LdsLayout lay{1, ld, 0, 1};
+  C<T>* W1 = W + B * ld;
+  for (int px = 0; px < g.x.R; ++px) {
+    for (int e = threadIdx.x; e < B * Ls; e += blockDim.x) {
+      const int bb = e / Ls;
+      const int j = e - bb * Ls;
+      const int m = m0 + bb;
+      C<T> acc = czero<T>();
+      if (m < M) {
+        for (int l = j; l < L; l += Ls) {
+          int64_t u;
+          C<T> xv = load_unknown(P, X, g, l + int64_t(L) * (m + int64_t(M) * n), c, transp, u);
+          if (u >= 0) acc = cadd(acc, px ? cmul(xv, twx[(int64_t(l) * px) % Lp]) : xv);
+        }
+      }
+      W[bb * ld + j] = acc;
+    }
+    __syncthreads();
+    const C<T>* R = lds_fft<T, false>(W, W1, g.x.plan, B, lay, twx);
+    for (int e = threadIdx.x; e < B * Ls; e += blockDim.x) {
+      const int a = e / B;
+      const int bb = e - a * B;
+      const int m = m0 + bb;
+      if (m < M) {
+        const int64_t kxi = int64_t(px) * Ls + a;
+        P.work[(kxi * 3 + c) * MN + int64_t(n) * M + m] = R[bb * ld + a];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ----------------------------------------------------------------------------
+// K2: one kx plane per workgroup.
+// ----------------------------------------------------------------------------
+// Launch bound for a kernel holding EPT complex accumulators per thread:
+// large accumulators get 256 VGPRs (512 threads) instead of 128.
+template <typename T, int EPT> constexpr int acc_bound() {
+  return (sizeof(T) == 8 ? EPT <= 1 : EPT <= 4) ? 1024 : 512;
+}
+
+template <typename T, int EPT>
+__global__ void __launch_bounds__((acc_bound<T, EPT>())) k_plane(const ProbPtrs<T>* __restrict__ probs, const int* __restrict__ active,
+                                                Geometry g, const C<T>* __restrict__ twy,
+                                                const C<T>* __restrict__ twz) {
+  if (active && !active[blockIdx.y]) return;
+  const ProbPtrs<T> P = probs[blockIdx.y];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W = reinterpret_cast<C<T>*>(smem_raw);
+  const int M = g.M, N = g.N;
+  const int Ms = g.y.S, Ns = g.z.S, Mp = g.y.P, Np = g.z.P, Ry = g.y.R, Rz = g.z.R;
+  const int MN = M * N;
+  const int MsNs = Ms * Ns;
+  const int kxi = xcd_remap(blockIdx.x, gridDim.x);
+  C<T>* Zp = P.work + int64_t(kxi) * 3 * MN;
+  C<T> acc[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) acc[k] = czero<T>();
+  const LdsLayout lay_y{1, Ms, 0, 1};
+  const LdsLayout lay_z{Ms, MsNs, 1, Ms};
+  C<T>* const W0 = W;
+  C<T>* const W1 = W + 3 * MsNs;
+  for (int py = 0; py < Ry; ++py) {
+    for (int pz = 0; pz < Rz; ++pz) {
+      // fold + twiddle: W[c][z][y] = sum Z[c][y + i Ms][z + j Ns] w_Mp^{m py} w_Np^{n pz}
+      for (int e = threadIdx.x; e < 3 * MsNs; e += blockDim.x) {
+        const int c = e / MsNs;
+        const int r = e - c * MsNs;
+        const int zz = r / Ms;
+        const int yy = r - zz * Ms;
+        C<T> v = czero<T>();
+        for (int n = zz; n < N; n += Ns) {
+          const C<T> wz = twz[(n * pz) % Np];
+          for (int m = yy; m < M; m += Ms) {
+            C<T> t = Zp[(c * N + n) * M + m];
+            if (py) t = cmul(t, twy[(m * py) % Mp]);
+            if (pz) t = cmul(t, wz);
+            v = cadd(v, t);
+          }
+        }
+        W0[e] = v;
+      }
+      __syncthreads();
+      C<T>* R = lds_fft<T, false>(W0, W1, g.y.plan, 3 * Ns, lay_y, twy);
+      R = lds_fft<T, false>(R, R == W0 ? W1 : W0, g.z.plan, 3 * Ms, lay_z, twz);
+      // 3x3 symmetric tensor multiply, components xx xy xz yy yz zz
+      const C<T>* F = P.fN + ((int64_t(kxi) * Ry + py) * Rz + pz) * int64_t(MsNs) * 6;
+      for (int e = threadIdx.x; e < MsNs; e += blockDim.x) {
+        const C<T>* f = F + int64_t(e) * 6;
+        const C<T> f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4], f5 = f[5];
+        const C<T> a = R[e], b = R[MsNs + e], cc = R[2 * MsNs + e];
+        R[e] = cadd(cadd(cmul(f0, a), cmul(f1, b)), cmul(f2, cc));
+        R[MsNs + e] = cadd(cadd(cmul(f1, a), cmul(f3, b)), cmul(f4, cc));
+        R[2 * MsNs + e] = cadd(cadd(cmul(f2, a), cmul(f4, b)), cmul(f5, cc));
+      }
+      __syncthreads();
+      R = lds_fft<T, true>(R, R == W0 ? W1 : W0, g.z.plan, 3 * Ms, lay_z, twz);
+      R = lds_fft<T, true>(R, R == W0 ? W1 : W0, g.y.plan, 3 * Ns, lay_y, twy);
+      // unfold: out[c][n][m] += W[c][n % Ns][m % Ms] conj(w_Mp^{m py} w_Np^{n pz})
+#pragma unroll
+      for (int k = 0; k < EPT; ++k) {
+        const int o = threadIdx.x + k * blockDim.x;
+        if (o < 3 * MN) {
+          const int c = o / MN;
+          const int r = o - c * MN;
+          const int n = r / M;
+          const int m = r - n * M;
+          C<T> t = R[(c * Ns + n % Ns) * Ms + m % Ms];
+          if (py) t = cmulc(t, twy[(m * py) % Mp]);
+          if (pz) t = cmulc(t, twz[(n * pz) % Np]);
+          acc[k] = cadd(acc[k], t);
+        }
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int o = threadIdx.x + k * blockDim.x;
+    if (o < 3 * MN) Zp[o] = acc[k];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// K3: inverse x pass + epilogue.
+// ----------------------------------------------------------------------------
+template <typename T, int EPT>
+__global__ void __launch_bounds__((acc_bound<T, EPT>())) k_xinv(const ProbPtrs<T>* __restrict__ probs, VecArgs<T> va, Geometry g,
+                                               MatvecPlan mp, int transp, const C<T>* __restrict__ twx) {
+  if (va.active && !va.active[blockIdx.y]) return;
+  const ProbPtrs<T> P = probs[blockIdx.y];
+  const C<T>* __restrict__ X = va.x + int64_t(blockIdx.y) * va.xstride;
+  C<T>* __restrict__ Y = va.y + int64_t(blockIdx.y) * va.ystride;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W = reinterpret_cast<C<T>*>(smem_raw);
+  const int L = g.L, M = g.M, N = g.N;
+  const int Ls = g.x.S, Lp = g.x.P, B = mp.B, ld = mp.ldsx;
+  const int nmb = (M + B - 1) / B;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int mb = blk % nmb;
+  const int rest = blk / nmb;
+  const int n = rest % N;
+  const int c = rest / N;
+  const int m0 = mb * B;
+  const int64_t MN = int64_t(M) * N;
+  const LdsLayout lay{1, ld, 0, 1};
+  C<T>* W1 = W + B * ld;
+  C<T> acc[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) acc[k] = czero<T>();
+  for (int px = 0; px < g.x.R; ++px) {
+    for (int e = threadIdx.x; e < B * Ls; e += blockDim.x) {
+      const int a = e / B;
+      const int bb = e - a * B;
+      const int m = m0 + bb;
+      C<T> v = czero<T>();
+      if (m < M) {
+        const int64_t kxi = int64_t(px) * Ls + a;
+        v = P.work[(kxi * 3 + c) * MN + int64_t(n) * M + m];
+      }
+      W[bb * ld + a] = v;
+    }
+    __syncthreads();
+    const C<T>* R = lds_fft<T, true>(W, W1, g.x.plan, B, lay, twx);
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int o = threadIdx.x + k * blockDim.x;
+      if (o < B * L) {
+        const int bb = o / L;
+        const int l = o - bb * L;
+        C<T> t = R[bb * ld + l % Ls];
+        if (px) t = cmulc(t, twx[(int64_t(l) * px) % Lp]);
+        acc[k] = cadd(acc[k], t);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int o = threadIdx.x + k * blockDim.x;
+    if (o < B * L) {
+      const int bb = o / L;
+      const int l = o - bb * L;
+      const int m = m0 + bb;
+      if (m < M) {
+        const int64_t v = l + int64_t(L) * (m + int64_t(M) * n);
+        int64_t pv, u;
+        if (g.structured) {
+          pv = P.pos[v];
+          u = pv < 0 ? -1 : pv + int64_t(c) * g.nS;
+        } else {
+          pv = P.pos[int64_t(c) * g.V + v];
+          u = pv;
+        }
+        if (u >= 0) {
+          const C<T> xv = X[u];
+          C<T> id = cscale(xv, P.gram);
+          if (P.mr) id = transp ? cmulc(id, P.mr[pv]) : cmul(id, P.mr[pv]);
+          const C<T> nv = acc[k];
+          Y[u] = transp ? csub(id, cconj(nv)) : csub(id, cmul(P.mc[pv], nv));
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Launch
+// ----------------------------------------------------------------------------
+#define VP_EPT_DISPATCH(EPTV, CALL) \
+  switch (EPTV) {                   \
+    case 1: CALL(1); break;         \
+    case 2: CALL(2); break;         \
+    case 4: CALL(4); break;         \
+    case 8: CALL(8); break;         \
+    case 16: CALL(16); break;       \
+    case 32: CALL(32); break;       \
+    default: throw VpError{ERR_UNSUPPORTED, "matvec: accumulator size out of range"}; \
+  }
+
+template <typename T>
+void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d_probs, VecArgs<T> va, int nprob,
+                   int transp, const C<T>* twx, const C<T>* twy, const C<T>* twz, hipStream_t s) {
+  const int nmb = (g.M + mp.B - 1) / mp.B;
+  dim3 gx(nmb * g.N * 3, nprob);
+  {
+    const void* fn = reinterpret_cast<const void*>(&k_xfwd<T>);
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds1));
+    hipLaunchKernelGGL(k_xfwd<T>, gx, dim3(mp.tx), mp.lds1, s, d_probs, va, g, mp, transp, twx);
+    VP_HIP(hipGetLastError());
+  }
+  {
+    dim3 gp(g.x.P, nprob);
+#define VP_K2(E)                                                                            \
+  do {                                                                                      \
+    const void* fn = reinterpret_cast<const void*>(&k_plane<T, E>);                         \
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds2)); \
+    hipLaunchKernelGGL((k_plane<T, E>), gp, dim3(mp.t2), mp.lds2, s, d_probs, va.active, g, twy, twz);  \
+  } while (0)
+    VP_EPT_DISPATCH(mp.ept2, VP_K2);
+#undef VP_K2
+    VP_HIP(hipGetLastError());
+  }
+  {
+#define VP_K3(E)                                                                            \
+  do {                                                                                      \
+    const void* fn = reinterpret_cast<const void*>(&k_xinv<T, E>);                          \
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds1)); \
+    hipLaunchKernelGGL((k_xinv<T, E>), gx, dim3(mp.tx), mp.lds1, s, d_probs, va, g, mp, transp, twx); \
+  } while (0)
+    VP_EPT_DISPATCH(mp.ept3, VP_K3);
+#undef VP_K3
+    VP_HIP(hipGetLastError());
+  }
+}
+
+template void launch_matvec<float>(const Geometry&, const MatvecPlan&, const ProbPtrs<float>*, VecArgs<float>, int,
+                                   int, const C<float>*, const C<float>*, const C<float>*, hipStream_t);
+template void launch_matvec<double>(const Geometry&, const MatvecPlan&, const ProbPtrs<double>*, VecArgs<double>,
+                                    int, int, const C<double>*, const C<double>*, const C<double>*, hipStream_t);
+
+}  // namespace vp

@@ -1,0 +1,100 @@
+// Operator / context objects behind the opaque C handles.
+#pragma once
+#include <memory>
+#include <vector>
+#include "voxphoton.h"
+#include "vp_internal.hpp"
+
+struct vp_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+namespace vp {
+
+// RAII device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t b) { alloc(b); }
+  void alloc(size_t b) {
+    release();
+    if (b) VP_HIP(hipMalloc(&p, b));
+    bytes = b;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  template <typename X> X* as() const { return reinterpret_cast<X*>(p); }
+};
+
+// Per-precision device state of an operator.
+template <typename T>
+struct OpPrec {
+  bool present = false;
+  Geometry g{};     // per precision: residue classes depend on the LDS budget
+  DevBuf fN;        // class layout, prescaled
+  DevBuf mc, mr;    // per unknown voxel (structured) or unknown (generic)
+  DevBuf tw;        // twiddles: Lp | Mp | Np entries
+  DevBuf work;      // spectrum workspace 3 * Lp * M * N
+  DevBuf xs, ys;    // vector staging for host-memory calls
+  DevBuf probs;     // one ProbPtrs<T>
+  T gram = T(1);
+  MatvecPlan plan{};
+  const C<T>* twx() const { return tw.as<C<T>>(); }
+  const C<T>* twy() const { return tw.as<C<T>>() + g.x.P; }
+  const C<T>* twz() const { return tw.as<C<T>>() + g.x.P + g.y.P; }
+};
+
+}  // namespace vp
+
+struct vp_op_s {
+  vp_ctx ctx = nullptr;
+  int L = 0, M = 0, N = 0, Lp = 0, Mp = 0, Np = 0;
+  int64_t V = 0, nS = 0;
+  int structured = 0;
+  int64_t n = 0;               // unknowns
+  double gram = 1.0;           // FP64 identity weight
+  double scale32 = 1.0;        // FP32 operator = A / scale32
+  bool has_mr = false;
+  vp::DevBuf pos;              // voxel map
+  vp::OpPrec<double> p64;
+  vp::OpPrec<float> p32;
+};
+
+namespace vp {
+
+template <typename T> OpPrec<T>& prec_of(vp_op_s* op);
+template <> inline OpPrec<double>& prec_of<double>(vp_op_s* op) { return op->p64; }
+template <> inline OpPrec<float>& prec_of<float>(vp_op_s* op) { return op->p32; }
+
+// Build helpers (setup.hip)
+// Chooses residue classes and launch shapes for one precision; throws
+// ERR_UNSUPPORTED when no shape fits.
+void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, int device);
+template <typename T> void upload_twiddles(OpPrec<T>& P, hipStream_t s);
+// Writes the class-layout tensor for both precisions from a true-frequency
+// device tensor F (Lp x Mp x Np x 6 complex128, column-major).
+void layout_fN(vp_op_s* op, const double2* F, hipStream_t s);
+// True-frequency tensor from a device Toeplitz tensor (L x M x N x 6).
+void toeplitz_to_fN(const vp_op_s& op, const double2* G, double2* F, hipStream_t s);
+// Device assembly of the Toeplitz tensor.
+void assemble_toeplitz_dev(int L, int M, int N, double dx, double ko, double2* G, hipStream_t s);
+
+template <typename T>
+void op_apply_dev(vp_op_s* op, int transp, const C<T>* x, C<T>* y);
+
+// idx (1-based, length n) -> voxel map; structured = [idxS; V+idxS; 2V+idxS] form
+void index_map(int64_t V, const int64_t* idx, int64_t n, std::vector<int>& pos, int& structured, int64_t& nS);
+
+// Generic device helpers (setup.hip)
+void convert_c128_to_c64(const double2* in, float2* out, int64_t n, double scale, hipStream_t s);
+void convert_c64_to_c128(const float2* in, double2* out, int64_t n, double scale, hipStream_t s);
+
+}  // namespace vp

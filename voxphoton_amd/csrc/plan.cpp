@@ -1,0 +1,106 @@
+// Host-side planning: radix factorisation and launch shapes for the matvec.
+#include <algorithm>
+#include <mutex>
+#include "vp_internal.hpp"
+
+namespace vp {
+
+bool factorize(int n, bool fp64, FftPlan& out, int tws) {
+  out.n = n;
+  out.tws = tws;
+  out.nst = 0;
+  out.pad = 0;
+  out.code = 0;
+  int m = n;
+  auto push = [&](int r) {
+    if (out.nst >= kMaxStages) return false;
+    out.code |= (unsigned long long)r << (5 * out.nst);
+    out.nst++;
+    m /= r;
+    return true;
+  };
+  // powers of two: 16 (single precision only), 8, 4, then a final 2
+  if (!fp64) while (m % 16 == 0) if (!push(16)) return false;
+  while (m % 8 == 0) if (!push(8)) return false;
+  while (m % 4 == 0) if (!push(4)) return false;
+  while (m % 2 == 0) if (!push(2)) return false;
+  const int odd32[] = {13, 11, 7, 5, 3};
+  const int odd64[] = {11, 7, 5, 3};
+  const int* odd = fp64 ? odd64 : odd32;
+  const int nodd = fp64 ? 4 : 5;
+  for (int i = 0; i < nodd; ++i)
+    while (m % odd[i] == 0) if (!push(odd[i])) return false;
+  return m == 1;
+}
+
+int max_lds_bytes(int device) {
+  static std::mutex mu;
+  static int cache[64] = {0};
+  std::lock_guard<std::mutex> lk(mu);
+  if (device < 0 || device >= 64) return 65536;
+  if (!cache[device]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || v <= 0)
+      v = 65536;
+    cache[device] = v;
+  }
+  return cache[device];
+}
+
+static int pow2ceil(int v) {
+  int p = 1;
+  while (p < v) p *= 2;
+  return p;
+}
+
+// Mirrors acc_bound<T, EPT>() in matvec.hip.
+static int acc_bound(bool fp64, int ept) { return (fp64 ? ept <= 1 : ept <= 4) ? 1024 : 512; }
+
+// Smallest accumulator count whose thread count fits its launch bound.
+static bool pick_threads(int points, bool fp64, int& t_out, int& e_out) {
+  const int ept_max = 32;
+  for (int e = 1; e <= ept_max; e *= 2) {
+    int t = (points + e - 1) / e;
+    t = std::max(64, ((t + 63) / 64) * 64);
+    if (t <= acc_bound(fp64, e)) { t_out = t; e_out = e; return true; }
+  }
+  return false;
+}
+
+bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out, std::string& why) {
+  const size_t s = fp64 ? 16 : 8;
+  const int lds_max = max_lds_bytes(device);
+  // ---- K2 (planes): classes fixed by the caller; pick threads/accumulators.
+  {
+    const int Ms = g.y.S, Ns = g.z.S;
+    const long long E = 3LL * Ms * Ns;
+    if (2 * E * (long long)s > lds_max) { why = "plane class buffers exceed LDS"; return false; }
+    int t = 0, e = 0;
+    if (!pick_threads(3 * g.M * g.N, fp64, t, e)) { why = "plane too large for register accumulators"; return false; }
+    out.t2 = t;
+    out.ept2 = e;
+    out.lds2 = 2 * size_t(E) * s;
+  }
+  // ---- K1/K3 (x pencils): largest pencil batch that fits LDS and registers.
+  {
+    const int Ls = g.x.S;
+    bool ok = false;
+    for (int B : {16, 8, 4, 2, 1}) {
+      if (B > 1 && B > g.M) continue;
+      const int ld = Ls + (B > 1 ? 1 : 0);
+      const size_t lds = 2 * size_t(B) * ld * s;
+      if (lds > size_t(lds_max)) continue;
+      int t = 0, e = 0;
+      if (!pick_threads(B * g.L, fp64, t, e)) continue;
+      if (e > 16) continue;
+      out.B = B; out.ldsx = ld; out.tx = std::max(t, 256); out.ept3 = e; out.lds1 = lds;
+      if (out.tx > acc_bound(fp64, e)) out.tx = acc_bound(fp64, e);
+      ok = true;
+      break;
+    }
+    if (!ok) { why = "no pencil tiling fits the x FFT"; return false; }
+  }
+  return true;
+}
+
+}  // namespace vp

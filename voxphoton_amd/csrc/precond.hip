@@ -1,0 +1,458 @@
+// Block-circulant (Chan) preconditioners on the device.
+//
+// Apply (pc/src_preconditioning/chan_2_mvp_for_parallel_idx.m:2-23):
+//   scatter -> DFT along x (length L) -> per kx: DFT along y (length M),
+//   3N x 3N block product per (kx, ky), inverse DFT along y -> inverse DFT
+//   along x -> gather.  The 1/(L*M) of MATLAB's ifft pair is folded into
+//   the stored blocks.
+// 1-level apply (chan_mvp_idx.m:2-16) is the same with M folded into the
+// block (3MN x 3MN blocks, no y transform).
+// Setup (level_2_parallel_func.m:1-192 semantics) builds every 3N x 3N
+// block directly from the Toeplitz tensor: Chan circulant + DFT along x,
+// signed y-Toeplitz first row/column per (c, n_t, d, n_s), Chan circulant +
+// DFT along y, then batched Gauss-Jordan inversion in FP64.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+#include "precond.hpp"
+
+namespace vp {
+
+// ----------------------------------------------------------------------------
+// Apply kernels
+// ----------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ int64_t pc_outer(const PcArgs<T>& a, int64_t j) {
+  return a.sub ? int64_t(a.sub[j]) : j;
+}
+
+// x pass forward: B pencils (consecutive m) at fixed (n, c), length L DFT.
+template <typename T>
+__global__ void __launch_bounds__(512) k_pc_xfwd(PcArgs<T> a, PcGeom g) {
+  if (a.active && !a.active[blockIdx.y]) return;
+  const C<T>* __restrict__ X = a.x + int64_t(blockIdx.y) * a.xstride;
+  C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
+  const int L = g.L, M = g.M, N = g.N, B = g.B, ld = g.ld;
+  C<T>* W1 = W0 + B * ld;
+  const int nmb = (M + B - 1) / B;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int mb = blk % nmb, rest = blk / nmb;
+  const int n = rest % N, c = rest / N;
+  const int m0 = mb * B;
+  const int64_t MN = int64_t(M) * N;
+  for (int e = threadIdx.x; e < B * L; e += blockDim.x) {
+    const int bb = e / L, l = e - bb * L, m = m0 + bb;
+    C<T> v = czero<T>();
+    if (m < M) {
+      const int64_t vox = l + int64_t(L) * (m + int64_t(M) * n);
+      int64_t j;
+      if (g.structured) {
+        const int p = a.pos[vox];
+        j = p < 0 ? -1 : p + int64_t(c) * g.nS;
+      } else {
+        j = a.pos[int64_t(c) * g.V + vox];
+      }
+      if (j >= 0) v = X[pc_outer(a, j)];
+    }
+    W0[bb * ld + l] = v;
+  }
+  __syncthreads();
+  const C<T>* R = lds_fft<T, false>(W0, W1, g.px, B, LdsLayout{1, ld, 0, 1}, a.twx);
+  for (int e = threadIdx.x; e < B * L; e += blockDim.x) {
+    const int kx = e / B, bb = e - kx * B, m = m0 + bb;
+    if (m < M) work[(int64_t(kx) * 3 + c) * MN + int64_t(n) * M + m] = R[bb * ld + kx];
+  }
+}
+
+// per kx plane: y DFT, 3N x 3N block products, inverse y DFT.
+template <typename T>
+__global__ void __launch_bounds__(512) k_pc_plane(PcArgs<T> a, PcGeom g) {
+  if (a.active && !a.active[blockIdx.y]) return;
+  C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
+  const C<T>* __restrict__ blocks = a.blocks[blockIdx.y];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
+  const int M = g.M, N = g.N, n3 = 3 * N;
+  const int plane = n3 * M;
+  C<T>* W1 = W0 + plane;
+  const int kx = blockIdx.x;
+  C<T>* Zp = work + int64_t(kx) * plane;
+  for (int e = threadIdx.x; e < plane; e += blockDim.x) W0[e] = Zp[e];
+  __syncthreads();
+  const LdsLayout lay{1, M, 0, 1};   // transform q = (c, n): elements q*M + m
+  C<T>* R = lds_fft<T, false>(W0, W1, g.py, n3, lay, a.twy);
+  C<T>* O = (R == W0) ? W1 : W0;
+  // one wavefront per ky; lanes over block rows
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  const C<T>* Bk = blocks + int64_t(kx) * M * n3 * n3;
+  for (int ky = wave; ky < M; ky += nw) {
+    const C<T>* Bb = Bk + int64_t(ky) * n3 * n3;
+    for (int r0 = 0; r0 < n3; r0 += 64) {
+      const int r = r0 + lane;
+      C<T> acc = czero<T>();
+      if (r < n3) {
+        for (int q = 0; q < n3; ++q) acc = cadd(acc, cmul(Bb[int64_t(q) * n3 + r], R[q * M + ky]));
+        O[r * M + ky] = acc;
+      }
+    }
+  }
+  __syncthreads();
+  const C<T>* F = lds_fft<T, true>(O, R, g.py, n3, lay, a.twy);
+  for (int e = threadIdx.x; e < plane; e += blockDim.x) Zp[e] = F[e];
+}
+
+// x pass inverse + gather into the output vector.
+template <typename T>
+__global__ void __launch_bounds__(512) k_pc_xinv(PcArgs<T> a, PcGeom g) {
+  if (a.active && !a.active[blockIdx.y]) return;
+  C<T>* __restrict__ Y = a.y + int64_t(blockIdx.y) * a.ystride;
+  const C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
+  const int L = g.L, M = g.M, N = g.N, B = g.B, ld = g.ld;
+  C<T>* W1 = W0 + B * ld;
+  const int nmb = (M + B - 1) / B;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int mb = blk % nmb, rest = blk / nmb;
+  const int n = rest % N, c = rest / N;
+  const int m0 = mb * B;
+  const int64_t MN = int64_t(M) * N;
+  for (int e = threadIdx.x; e < B * L; e += blockDim.x) {
+    const int kx = e / B, bb = e - kx * B, m = m0 + bb;
+    W0[bb * ld + kx] = (m < M) ? work[(int64_t(kx) * 3 + c) * MN + int64_t(n) * M + m] : czero<T>();
+  }
+  __syncthreads();
+  const C<T>* R = lds_fft<T, true>(W0, W1, g.px, B, LdsLayout{1, ld, 0, 1}, a.twx);
+  for (int e = threadIdx.x; e < B * L; e += blockDim.x) {
+    const int bb = e / L, l = e - bb * L, m = m0 + bb;
+    if (m < M) {
+      const int64_t vox = l + int64_t(L) * (m + int64_t(M) * n);
+      int64_t j;
+      if (g.structured) {
+        const int p = a.pos[vox];
+        j = p < 0 ? -1 : p + int64_t(c) * g.nS;
+      } else {
+        j = a.pos[int64_t(c) * g.V + vox];
+      }
+      if (j >= 0) Y[pc_outer(a, j)] = R[bb * ld + l];
+    }
+  }
+}
+
+// 1-level: block product over the whole 3MN x-frequency slice.
+template <typename T>
+__global__ void __launch_bounds__(256) k_pc1_block(PcArgs<T> a, PcGeom g) {
+  if (a.active && !a.active[blockIdx.y]) return;
+  C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
+  C<T>* __restrict__ out = a.work2 + int64_t(blockIdx.y) * a.wstride;
+  const C<T>* __restrict__ blocks = a.blocks[blockIdx.y];
+  const int64_t n3 = 3LL * g.M * g.N;
+  const int kx = blockIdx.x;
+  const C<T>* Bk = blocks + int64_t(kx) * n3 * n3;
+  const C<T>* v = work + int64_t(kx) * n3;
+  for (int64_t r = threadIdx.x + int64_t(blockIdx.z) * blockDim.x; r < n3; r += int64_t(gridDim.z) * blockDim.x) {
+    C<T> acc = czero<T>();
+    for (int64_t q = 0; q < n3; ++q) acc = cadd(acc, cmul(Bk[q * n3 + r], v[q]));
+    out[int64_t(kx) * n3 + r] = acc;
+  }
+}
+
+template <typename T>
+__global__ void k_copy_vec(const C<T>* __restrict__ x, C<T>* __restrict__ y, int64_t n, int64_t xstride,
+                           int64_t ystride, const int* __restrict__ active) {
+  if (active && !active[blockIdx.y]) return;
+  const C<T>* X = x + int64_t(blockIdx.y) * xstride;
+  C<T>* Y = y + int64_t(blockIdx.y) * ystride;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    Y[i] = X[i];
+}
+
+template <typename T>
+void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) {
+  if (a.sub && g.n_outer > 0) {
+    const int blocks = int(std::min<int64_t>((g.n_outer + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_copy_vec<T>, dim3(blocks, nprob), dim3(256), 0, s, a.x, a.y, g.n_outer, a.xstride,
+                       a.ystride, a.active);
+    VP_HIP(hipGetLastError());
+  }
+  const int nmb = (g.M + g.B - 1) / g.B;
+  const dim3 gx(nmb * g.N * 3, nprob);
+  const void* f1 = reinterpret_cast<const void*>(&k_pc_xfwd<T>);
+  (void)hipFuncSetAttribute(f1, hipFuncAttributeMaxDynamicSharedMemorySize, int(g.lds1));
+  hipLaunchKernelGGL(k_pc_xfwd<T>, gx, dim3(g.tx), g.lds1, s, a, g);
+  VP_HIP(hipGetLastError());
+  if (g.level == 2) {
+    const void* f2 = reinterpret_cast<const void*>(&k_pc_plane<T>);
+    (void)hipFuncSetAttribute(f2, hipFuncAttributeMaxDynamicSharedMemorySize, int(g.lds2));
+    hipLaunchKernelGGL(k_pc_plane<T>, dim3(g.L, nprob), dim3(g.t2), g.lds2, s, a, g);
+    VP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pc_xinv<T>, gx, dim3(g.tx), g.lds1, s, a, g);
+  } else {
+    const int64_t n3 = 3LL * g.M * g.N;
+    const int zs = int(std::min<int64_t>((n3 + 255) / 256, 64));
+    hipLaunchKernelGGL(k_pc1_block<T>, dim3(g.L, nprob, zs), dim3(256), 0, s, a, g);
+    VP_HIP(hipGetLastError());
+    PcArgs<T> b = a;
+    b.work = a.work2;
+    hipLaunchKernelGGL(k_pc_xinv<T>, gx, dim3(g.tx), g.lds1, s, b, g);
+  }
+  const void* f3 = reinterpret_cast<const void*>(&k_pc_xinv<T>);
+  (void)f3;
+  VP_HIP(hipGetLastError());
+}
+template void launch_prec<float>(const PcGeom&, const PcArgs<float>&, int, hipStream_t);
+template void launch_prec<double>(const PcGeom&, const PcArgs<double>&, int, hipStream_t);
+
+// ----------------------------------------------------------------------------
+// Setup kernels (FP64)
+// ----------------------------------------------------------------------------
+__constant__ int c_pc_even_x[6] = {1, 0, 0, 1, 1, 1};
+__constant__ int c_pc_comp[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+
+// ck[k][kx][m][n] = sum_i chan_x(A_k)(i) w_L^{i kx}, A_k = -Mc*G_k (+ Gram*Mr0 at origin
+// for xx, yy, zz); chan_x per level_1_parallel_func.m:12-21.
+__global__ void k_pc_chan_x(const double2* __restrict__ G, double2* __restrict__ ck, int L, int M, int N,
+                            double2 mc, double diag, const double2* __restrict__ twL) {
+  const int64_t total = int64_t(6) * L * M * N;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = e;
+    const int kx = int(r % L); r /= L;
+    const int m = int(r % M); r /= M;
+    const int n = int(r % N); r /= N;
+    const int k = int(r);
+    const int64_t V = int64_t(L) * M * N;
+    const int64_t base = int64_t(L) * (m + int64_t(M) * n) + V * k;
+    auto A = [&](int i) -> double2 {
+      double2 g = G[base + i];
+      double2 a = make_double2(-(mc.x * g.x - mc.y * g.y), -(mc.x * g.y + mc.y * g.x));
+      if (i == 0 && m == 0 && n == 0 && (k == 0 || k == 3 || k == 5)) a.x += diag;
+      return a;
+    };
+    const double sg = c_pc_even_x[k] ? 1.0 : -1.0;
+    double2 acc = A(0);
+    int64_t ph = kx;
+    for (int i = 1; i < L; ++i, ph += kx) {
+      if (ph >= L) ph -= L;
+      const double2 ai = A(i), aj = A(L - i);
+      const double wa = double(L - i) / L, wb = sg * double(i) / L;
+      const double2 cv = make_double2(wa * ai.x + wb * aj.x, wa * ai.y + wb * aj.y);
+      const double2 w = twL[ph];
+      acc.x += cv.x * w.x - cv.y * w.y;
+      acc.y += cv.x * w.y + cv.y * w.x;
+    }
+    ck[e] = acc;   // layout [k][n][m][kx]
+  }
+}
+
+// Block (kx, ky) entry [(c, nt), (d, ns)] = sum_t chan_y(t) w_M^{t ky}.  The
+// M x M y-Toeplitz block of component k at z-offset (nt, ns) has first
+// column col(t) and first row row(t) built from v(t) = ck[k][|nt-ns|][t][kx]
+// with the sign conventions of level_2_parallel_func.m:94-162 (toeplitz
+// arguments for xy and yz, the triu completions for xz and yz).
+__device__ __forceinline__ double2 neg2(double2 a) { return make_double2(-a.x, -a.y); }
+
+__device__ __forceinline__ double2 pc_col(int k, int nt, int ns, int t, double2 v) {
+  if (k == 0 || k == 3 || k == 5) return v;
+  if (k == 1) return t == 0 ? neg2(v) : v;
+  if (k == 2) {
+    if (nt < ns) return neg2(v);
+    if (nt > ns) return v;
+    return t == 0 ? neg2(v) : v;
+  }
+  if (nt < ns) return t == 0 ? v : neg2(v);   // k == 4
+  return v;
+}
+
+__device__ __forceinline__ double2 pc_row(int k, int nt, int ns, int t, double2 v) {
+  if (k == 0 || k == 3 || k == 5) return v;
+  if (k == 1) return neg2(v);
+  if (k == 2) return (nt > ns) ? v : neg2(v);
+  if (nt > ns) return t == 0 ? v : neg2(v);   // k == 4
+  return v;
+}
+
+__global__ void k_pc_blocks(const double2* __restrict__ ck, double2* __restrict__ blk, int L, int M, int N,
+                            const double2* __restrict__ twM) {
+  const int n3 = 3 * N;
+  const int64_t total = int64_t(L) * M * n3 * n3;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    // output layout: block (kx, ky) at (kx*M + ky) * n3^2, column-major (row fastest)
+    int64_t r = e;
+    const int row = int(r % n3); r /= n3;
+    const int colq = int(r % n3); r /= n3;
+    const int ky = int(r % M); r /= M;
+    const int kx = int(r);
+    const int c = row / N, nt = row - c * N;
+    const int d = colq / N, ns = colq - d * N;
+    const int k = c_pc_comp[c][d];
+    const int dz = nt > ns ? nt - ns : ns - nt;
+    const double2* v = ck + (int64_t(k) * N + dz) * int64_t(M) * L + kx;   // v[t * L]
+    double2 acc = pc_col(k, nt, ns, 0, v[0]);
+    int64_t ph = ky;
+    for (int t = 1; t < M; ++t, ph += ky) {
+      if (ph >= M) ph -= M;
+      const double2 ct = pc_col(k, nt, ns, t, v[int64_t(t) * L]);
+      const double2 rt = pc_row(k, nt, ns, M - t, v[int64_t(M - t) * L]);
+      const double wa = double(M - t) / M, wb = double(t) / M;
+      const double2 cv = make_double2(wa * ct.x + wb * rt.x, wa * ct.y + wb * rt.y);
+      const double2 w = twM[ph];
+      acc.x += cv.x * w.x - cv.y * w.y;
+      acc.y += cv.x * w.y + cv.y * w.x;
+    }
+    blk[e] = acc;
+  }
+}
+
+// In-place Gauss-Jordan inversion with partial pivoting of one n x n
+// column-major FP64 block per workgroup (in global memory).
+__global__ void __launch_bounds__(256) k_pc_invert(double2* __restrict__ blk, int n, int* __restrict__ singular) {
+  double2* A = blk + int64_t(blockIdx.x) * n * n;
+  __shared__ int s_piv;
+  __shared__ int s_perm[256];
+  __shared__ double2 s_rowk[256];
+  __shared__ double s_best[256];
+  __shared__ int s_bi[256];
+  for (int k = 0; k < n; ++k) {
+    // pivot search over rows k..n-1 of column k
+    double best = -1.0;
+    int bi = k;
+    for (int i = k + threadIdx.x; i < n; i += blockDim.x) {
+      const double2 v = A[int64_t(k) * n + i];
+      const double a = fabs(v.x) + fabs(v.y);
+      if (a > best) { best = a; bi = i; }
+    }
+    s_best[threadIdx.x] = best;
+    s_bi[threadIdx.x] = bi;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) {
+        const double b2 = s_best[threadIdx.x + off];
+        const int i2 = s_bi[threadIdx.x + off];
+        if (b2 > s_best[threadIdx.x] || (b2 == s_best[threadIdx.x] && i2 < s_bi[threadIdx.x])) {
+          s_best[threadIdx.x] = b2;
+          s_bi[threadIdx.x] = i2;
+        }
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      s_piv = s_bi[0];
+      s_perm[k] = s_bi[0];
+      if (!(s_best[0] > 0.0)) atomicExch(singular, 1);
+    }
+    __syncthreads();
+    const int p = s_piv;
+    // swap rows k and p
+    if (p != k)
+      for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const double2 t = A[int64_t(j) * n + k];
+        A[int64_t(j) * n + k] = A[int64_t(j) * n + p];
+        A[int64_t(j) * n + p] = t;
+      }
+    __syncthreads();
+    const double2 piv = A[int64_t(k) * n + k];
+    const double den = piv.x * piv.x + piv.y * piv.y;
+    const double2 ip = den > 0 ? make_double2(piv.x / den, -piv.y / den) : make_double2(0.0, 0.0);
+    // scale row k (pivot entry becomes 1/piv)
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      double2 v = (j == k) ? make_double2(1.0, 0.0) : A[int64_t(j) * n + k];
+      v = cmul(v, ip);
+      A[int64_t(j) * n + k] = v;
+      s_rowk[j] = v;
+    }
+    __syncthreads();
+    // eliminate column k from all other rows
+    for (int64_t e = threadIdx.x; e < int64_t(n) * n; e += blockDim.x) {
+      const int i = int(e % n);
+      const int j = int(e / n);
+      if (i == k) continue;
+      const double2 f = A[int64_t(k) * n + i];
+      if (j == k) {
+        // new column k entry: -f * (1/piv)
+        A[int64_t(j) * n + i] = make_double2(-(f.x * s_rowk[k].x - f.y * s_rowk[k].y),
+                                             -(f.x * s_rowk[k].y + f.y * s_rowk[k].x));
+      }
+    }
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < int64_t(n) * n; e += blockDim.x) {
+      const int i = int(e % n);
+      const int j = int(e / n);
+      if (i == k || j == k) continue;
+      // f was the old A[i][k]; recover it from the new column k entry: newAik = -f/piv  =>  f = -newAik*piv
+      const double2 na = A[int64_t(k) * n + i];
+      const double2 f = make_double2(-(na.x * piv.x - na.y * piv.y), -(na.x * piv.y + na.y * piv.x));
+      const double2 rk = s_rowk[j];
+      double2 v = A[int64_t(j) * n + i];
+      v.x -= f.x * rk.x - f.y * rk.y;
+      v.y -= f.x * rk.y + f.y * rk.x;
+      A[int64_t(j) * n + i] = v;
+    }
+    __syncthreads();
+  }
+  // undo the row interchanges as column interchanges, last first
+  for (int k = n - 1; k >= 0; --k) {
+    const int p = s_perm[k];
+    if (p != k)
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double2 t = A[int64_t(k) * n + i];
+        A[int64_t(k) * n + i] = A[int64_t(p) * n + i];
+        A[int64_t(p) * n + i] = t;
+      }
+    __syncthreads();
+  }
+}
+
+void build_level2_blocks(const double2* G, int L, int M, int N, double2 mc, double diag, double2* blk,
+                         hipStream_t s) {
+  auto table = [](int P) {
+    std::vector<double2> t(P);
+    for (int k = 0; k < P; ++k) {
+      SinCos v = sincos_turn(k, P);
+      t[k] = make_double2(v.c, -v.s);
+    }
+    return t;
+  };
+  auto tL = table(L), tM = table(M);
+  DevBuf dL(tL.size() * sizeof(double2)), dM(tM.size() * sizeof(double2));
+  VP_HIP(hipMemcpyAsync(dL.p, tL.data(), dL.bytes, hipMemcpyHostToDevice, s));
+  VP_HIP(hipMemcpyAsync(dM.p, tM.data(), dM.bytes, hipMemcpyHostToDevice, s));
+  DevBuf ck(size_t(6) * L * M * N * sizeof(double2));
+  {
+    const int64_t total = int64_t(6) * L * M * N;
+    const int blocks = int(std::min<int64_t>((total + 255) / 256, 1 << 20));
+    hipLaunchKernelGGL(k_pc_chan_x, dim3(blocks), dim3(256), 0, s, G, ck.as<double2>(), L, M, N, mc, diag,
+                       dL.as<double2>());
+    VP_HIP(hipGetLastError());
+  }
+  {
+    const int64_t total = int64_t(L) * M * 9 * N * N;
+    const int blocks = int(std::min<int64_t>((total + 255) / 256, 1 << 20));
+    hipLaunchKernelGGL(k_pc_blocks, dim3(blocks), dim3(256), 0, s, ck.as<double2>(), blk, L, M, N,
+                       dM.as<double2>());
+    VP_HIP(hipGetLastError());
+  }
+  VP_HIP(hipStreamSynchronize(s));
+}
+
+void invert_blocks(double2* blk, int64_t nblocks, int n, hipStream_t s) {
+  VP_REQUIRE(n <= 256, ERR_UNSUPPORTED, "block inversion supports blocks up to 256 x 256");
+  DevBuf flag(sizeof(int));
+  VP_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+  for (int64_t b0 = 0; b0 < nblocks; b0 += 65535) {
+    const int nb = int(std::min<int64_t>(65535, nblocks - b0));
+    hipLaunchKernelGGL(k_pc_invert, dim3(nb), dim3(256), 0, s, blk + b0 * n * n, n, flag.as<int>());
+    VP_HIP(hipGetLastError());
+  }
+  int h = 0;
+  VP_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  VP_HIP(hipStreamSynchronize(s));
+  VP_REQUIRE(h == 0, ERR_INVALID, "preconditioner block is singular");
+}
+
+}  // namespace vp

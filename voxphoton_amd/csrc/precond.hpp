@@ -1,0 +1,90 @@
+// Preconditioner objects and launch interfaces.
+#pragma once
+#include "op.hpp"
+
+namespace vp {
+
+struct PcGeom {
+  int L, M, N;
+  int64_t V, nS;        // voxels; unknown voxels (structured) or unknowns (generic)
+  int structured;
+  int level;            // 1 or 2
+  int64_t n_outer;      // length of the vector the preconditioner acts on (embedding)
+  int B, ld, tx, t2;    // pencil batch, LDS row stride, threads (x pass / plane)
+  size_t lds1, lds2;
+  FftPlan px, py;       // unpadded DFTs of length L and M
+};
+
+template <typename T>
+struct PcArgs {
+  const C<T>* x;
+  C<T>* y;
+  int64_t xstride, ystride;
+  C<T>* work;           // 3 * L * M * N per problem
+  C<T>* work2;          // 1-level only: block product output
+  int64_t wstride;
+  const C<T>* const* blocks;   // device array: blocks of problem p
+  const int* pos;
+  const int* sub;       // box unknown -> outer vector index (nullptr: identity)
+  const C<T>* twx;      // L entries
+  const C<T>* twy;      // M entries
+  const int* active;
+};
+
+template <typename T>
+struct PrecPrec {
+  bool present = false;
+  DevBuf blocks;        // (kx, ky) major, column-major blocks, scaled by 1/(L*M)
+  DevBuf blockptr;      // one device pointer to blocks
+  DevBuf tw;            // L | M twiddles
+  DevBuf work, work2;
+  DevBuf xs, ys;        // staging for host calls
+};
+
+template <typename T>
+void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s);
+
+// FP64 block construction (level_2_parallel_func.m semantics) and inversion.
+void build_level2_blocks(const double2* G, int L, int M, int N, double2 mc, double diag, double2* blk,
+                         hipStream_t s);
+void invert_blocks(double2* blk, int64_t nblocks, int n, hipStream_t s);
+
+}  // namespace vp
+
+struct vp_prec_s {
+  vp_ctx ctx = nullptr;
+  vp::PcGeom g{};
+  int64_t nbox = 0;          // unknowns inside the box (|idx|)
+  double scale32 = 1.0;      // FP32 copy inverts A / scale32
+  vp::DevBuf pos;
+  vp::DevBuf sub;            // embedding (optional)
+  vp::DevBuf blocks64_host_order;  // unused placeholder for symmetry
+  vp::PrecPrec<double> p64;
+  vp::PrecPrec<float> p32;
+};
+
+namespace vp {
+template <typename T> PrecPrec<T>& pprec_of(vp_prec_s* p);
+template <> inline PrecPrec<double>& pprec_of<double>(vp_prec_s* p) { return p->p64; }
+template <> inline PrecPrec<float>& pprec_of<float>(vp_prec_s* p) { return p->p32; }
+
+template <typename T>
+inline PcArgs<T> prec_args(vp_prec_s* p, const C<T>* x, C<T>* y) {
+  PrecPrec<T>& P = pprec_of<T>(p);
+  PcArgs<T> a{};
+  a.x = x;
+  a.y = y;
+  a.xstride = 0;
+  a.ystride = 0;
+  a.work = P.work.template as<C<T>>();
+  a.work2 = P.work2.template as<C<T>>();
+  a.wstride = 0;
+  a.blocks = P.blockptr.template as<const C<T>*>();
+  a.pos = p->pos.as<int>();
+  a.sub = p->sub.p ? p->sub.as<int>() : nullptr;
+  a.twx = P.tw.template as<C<T>>();
+  a.twy = P.tw.template as<C<T>>() + p->g.L;
+  a.active = nullptr;
+  return a;
+}
+}  // namespace vp

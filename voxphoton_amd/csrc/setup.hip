@@ -1,0 +1,468 @@
+// Operator setup on the device: geometry/plan selection, twiddle tables,
+// circulant tensor layout, Toeplitz -> circulant spectrum, and assembly of the
+// Toeplitz Green tensor.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+#include "op.hpp"
+
+namespace vp {
+
+// ----------------------------------------------------------------------------
+// Geometry / plan
+// ----------------------------------------------------------------------------
+static std::vector<int> divisors(int n) {
+  std::vector<int> d;
+  for (int i = 1; i <= n; ++i)
+    if (n % i == 0) d.push_back(i);
+  return d;
+}
+
+void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, int device) {
+  g = Geometry{};
+  g.L = op.L; g.M = op.M; g.N = op.N;
+  g.V = op.V;
+  g.nS = op.nS;
+  g.structured = op.structured;
+  g.x.D = op.L; g.x.P = op.Lp;
+  g.y.D = op.M; g.y.P = op.Mp;
+  g.z.D = op.N; g.z.P = op.Np;
+  g.x.R = (op.Lp % 2 == 0) ? 2 : 1;
+  g.x.S = op.Lp / g.x.R;
+  VP_REQUIRE(g.x.S >= op.L, ERR_INVALID, "Lp too small for the grid");
+  VP_REQUIRE(factorize(g.x.S, fp64, g.x.plan, g.x.R), ERR_UNSUPPORTED,
+             "x FFT length " + std::to_string(g.x.S) + " has a prime factor above the supported radices");
+  struct Cand { int ry, rz; };
+  std::vector<Cand> cands;
+  for (int ry : divisors(op.Mp))
+    for (int rz : divisors(op.Np)) cands.push_back({ry, rz});
+  std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) {
+    if (a.ry * a.rz != b.ry * b.rz) return a.ry * a.rz < b.ry * b.rz;
+    return std::abs(a.ry - a.rz) < std::abs(b.ry - b.rz);
+  });
+  std::string why = "no residue-class split of the yz plane fits";
+  for (const Cand& c : cands) {
+    Geometry t = g;
+    t.y.R = c.ry; t.y.S = op.Mp / c.ry;
+    t.z.R = c.rz; t.z.S = op.Np / c.rz;
+    if (!factorize(t.y.S, fp64, t.y.plan, t.y.R)) continue;
+    if (!factorize(t.z.S, fp64, t.z.plan, t.z.R)) continue;
+    MatvecPlan p{};
+    if (!make_matvec_plan(t, fp64, device, p, why)) continue;
+    g = t;
+    mp = p;
+    return;
+  }
+  throw VpError{ERR_UNSUPPORTED, "matvec plan: " + why};
+}
+
+// exp(-2 pi i k / P) for k < P, correctly rounded to a few ulp.
+static std::vector<double2> twiddle_table(int P) {
+  std::vector<double2> t(P);
+  for (int k = 0; k < P; ++k) {
+    SinCos v = sincos_turn(k, P);
+    t[k] = make_double2(v.c, -v.s);
+  }
+  return t;
+}
+
+template <typename T>
+void upload_twiddles(OpPrec<T>& P, hipStream_t s) {
+  const Geometry& g = P.g;
+  std::vector<C<T>> h;
+  h.reserve(g.x.P + g.y.P + g.z.P);
+  for (int dimP : {g.x.P, g.y.P, g.z.P}) {
+    auto t = twiddle_table(dimP);
+    for (auto& v : t) h.push_back(mk<T>(T(v.x), T(v.y)));
+  }
+  P.tw.alloc(h.size() * sizeof(C<T>));
+  VP_HIP(hipMemcpyAsync(P.tw.p, h.data(), h.size() * sizeof(C<T>), hipMemcpyHostToDevice, s));
+  VP_HIP(hipStreamSynchronize(s));
+}
+template void upload_twiddles<float>(OpPrec<float>&, hipStream_t);
+template void upload_twiddles<double>(OpPrec<double>&, hipStream_t);
+
+// ----------------------------------------------------------------------------
+// Class layout of the tensor
+// ----------------------------------------------------------------------------
+template <typename T>
+__global__ void k_layout_fN(const double2* __restrict__ F, C<T>* __restrict__ out, Geometry g, double scale) {
+  const int64_t total = int64_t(g.x.P) * g.y.P * g.z.P * 6;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = e;
+    const int k = int(r % 6); r /= 6;
+    const int yy = int(r % g.y.S); r /= g.y.S;
+    const int zz = int(r % g.z.S); r /= g.z.S;
+    const int pz = int(r % g.z.R); r /= g.z.R;
+    const int py = int(r % g.y.R); r /= g.y.R;
+    const int64_t kxi = r;
+    const int px = int(kxi / g.x.S);
+    const int a = int(kxi - int64_t(px) * g.x.S);
+    const int64_t kx = int64_t(g.x.R) * a + px;
+    const int64_t ky = int64_t(g.y.R) * yy + py;
+    const int64_t kz = int64_t(g.z.R) * zz + pz;
+    const double2 v = F[kx + g.x.P * (ky + int64_t(g.y.P) * (kz + int64_t(g.z.P) * k))];
+    out[e] = mk<T>(T(v.x * scale), T(v.y * scale));
+  }
+}
+
+void layout_fN(vp_op_s* op, const double2* F, hipStream_t s) {
+  const double P = double(op->Lp) * op->Mp * op->Np;
+  const int64_t total = int64_t(op->Lp) * op->Mp * op->Np * 6;
+  const int blocks = int(std::min<int64_t>((total + 255) / 256, 65536));
+  if (op->p64.present) {
+    op->p64.fN.alloc(total * sizeof(double2));
+    hipLaunchKernelGGL(k_layout_fN<double>, dim3(blocks), dim3(256), 0, s, F, op->p64.fN.as<double2>(), op->p64.g,
+                       1.0 / P);
+    VP_HIP(hipGetLastError());
+  }
+  if (op->p32.present) {
+    op->p32.fN.alloc(total * sizeof(float2));
+    hipLaunchKernelGGL(k_layout_fN<float>, dim3(blocks), dim3(256), 0, s, F, op->p32.fN.as<float2>(), op->p32.g,
+                       1.0 / (P * op->scale32));
+    VP_HIP(hipGetLastError());
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Toeplitz tensor -> circulant spectrum.  The circulant embedding of
+// circulant_nop_const.m (reflected copies with the component parity, zero
+// plane elsewhere) has, along each axis, the DFT
+//   sum_i C(i) w^{ik} = G(0) + sum_{i>=1} G(i) (w^{ik} + s w^{-ik}),
+// i.e. 2 cos for even and -2i sin for odd components; three separable
+// passes evaluate it directly in FP64.
+// ----------------------------------------------------------------------------
+__constant__ int c_parity[6][3] = {{1, 1, 1}, {-1, -1, 1}, {-1, 1, -1}, {1, 1, 1}, {1, -1, -1}, {1, 1, 1}};
+
+// in : extents (D0, D1, D2) x 6 along the transformed axis "ax" (0,1,2), column-major
+// out: same with extent P along ax.
+__global__ void k_parity_dft(const double2* __restrict__ in, double2* __restrict__ out, int ax, int D, int P,
+                             int e0, int e1, int e2, const double2* __restrict__ tw) {
+  // output extents: e0,e1,e2 with e_ax == P ; input extents identical except e_ax == D
+  const int64_t total = int64_t(e0) * e1 * e2 * 6;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = e;
+    const int i0 = int(r % e0); r /= e0;
+    const int i1 = int(r % e1); r /= e1;
+    const int i2 = int(r % e2); r /= e2;
+    const int k = int(r);
+    int idx[3] = {i0, i1, i2};
+    const int kk = idx[ax];
+    int ein[3] = {e0, e1, e2};
+    ein[ax] = D;
+    int64_t stride = 1;
+    for (int a = 0; a < ax; ++a) stride *= ein[a];
+    idx[ax] = 0;
+    const int64_t base = idx[0] + int64_t(ein[0]) * (idx[1] + int64_t(ein[1]) * (idx[2] + int64_t(ein[2]) * k));
+    const bool odd = c_parity[k][ax] < 0;
+    double2 acc = in[base];
+    if (odd) acc = make_double2(0.0, 0.0);
+    int64_t ph = kk;  // (i * kk) mod P, incremented
+    for (int i = 1; i < D; ++i, ph += kk) {
+      if (ph >= P) ph -= P;
+      const double2 w = tw[ph];          // cos, -sin
+      const double2 g = in[base + i * stride];
+      if (!odd) {
+        acc.x += 2.0 * w.x * g.x;
+        acc.y += 2.0 * w.x * g.y;
+      } else {
+        // g * (-2 i sin) = g * (2 i w.y)
+        acc.x += -2.0 * w.y * g.y;
+        acc.y += 2.0 * w.y * g.x;
+      }
+    }
+    out[e] = acc;
+  }
+}
+
+void toeplitz_to_fN(const vp_op_s& op, const double2* G, double2* F, hipStream_t s) {
+  auto twx = twiddle_table(op.Lp), twy = twiddle_table(op.Mp), twz = twiddle_table(op.Np);
+  DevBuf dtx(twx.size() * sizeof(double2)), dty(twy.size() * sizeof(double2)), dtz(twz.size() * sizeof(double2));
+  VP_HIP(hipMemcpyAsync(dtx.p, twx.data(), dtx.bytes, hipMemcpyHostToDevice, s));
+  VP_HIP(hipMemcpyAsync(dty.p, twy.data(), dty.bytes, hipMemcpyHostToDevice, s));
+  VP_HIP(hipMemcpyAsync(dtz.p, twz.data(), dtz.bytes, hipMemcpyHostToDevice, s));
+  DevBuf t1(size_t(op.Lp) * op.M * op.N * 6 * sizeof(double2));
+  DevBuf t2(size_t(op.Lp) * op.Mp * op.N * 6 * sizeof(double2));
+  auto launch = [&](const double2* in, double2* out, int ax, int D, int P, int e0, int e1, int e2,
+                    const double2* tw) {
+    const int64_t total = int64_t(e0) * e1 * e2 * 6;
+    const int blocks = int(std::min<int64_t>((total + 255) / 256, 1 << 20));
+    hipLaunchKernelGGL(k_parity_dft, dim3(blocks), dim3(256), 0, s, in, out, ax, D, P, e0, e1, e2, tw);
+    VP_HIP(hipGetLastError());
+  };
+  launch(G, t1.as<double2>(), 0, op.L, op.Lp, op.Lp, op.M, op.N, dtx.as<double2>());
+  launch(t1.as<double2>(), t2.as<double2>(), 1, op.M, op.Mp, op.Lp, op.Mp, op.N, dty.as<double2>());
+  launch(t2.as<double2>(), F, 2, op.N, op.Np, op.Lp, op.Mp, op.Np, dtz.as<double2>());
+  VP_HIP(hipStreamSynchronize(s));
+}
+
+// ----------------------------------------------------------------------------
+// Green tensor assembly.  For cell offset m (observation cell m, source cell
+// 0) and faces p, q (ordered -x,+x,-y,+y,-z,+z), the dimensionless face-pair
+// integral of e^{-j kappa rho}/rho is reduced through the difference
+// variables of the two unit squares to a 2D (parallel faces) or 3D
+// (perpendicular faces) integral with triangular weights Lam(s) = 1-|s|,
+// split at the kinks of Lam so that a 1/rho singularity can only sit at a
+// sub-box vertex, where a Duffy map removes it.
+//   G_k(m) = dx^3/(4 pi) * sum_{p,q} I_pq(m) * P(p,q,k)
+// with the projection P of vector_projection.m:7-22.
+// ----------------------------------------------------------------------------
+struct GaussRules {
+  double x[4][16];
+  double w[4][16];
+  int n[4];   // 0 singular, 1 near, 2 medium, 3 far
+};
+__constant__ GaussRules c_rules;
+__constant__ signed char c_proj[6][6][6];
+
+__device__ __forceinline__ double lam(double s) { return 1.0 - fabs(s); }
+
+__device__ __forceinline__ double2 kern(double kappa, double rho, double jac) {
+  double sn, cs;
+  sincos(kappa * rho, &sn, &cs);
+  const double f = jac / rho;
+  return make_double2(cs * f, -sn * f);
+}
+
+__device__ double2 pair_parallel(const int m[3], int a, int sp, int sq, double kappa, int rule) {
+  const int b = (a == 0) ? 1 : 0;
+  const int c = (a == 2) ? 1 : 2;
+  const double h = double(sq - (m[a] + sp));
+  const int mb = m[b], mc = m[c];
+  double2 tot = make_double2(0.0, 0.0);
+  for (int al = -1; al <= 0; ++al) {
+    for (int be = -1; be <= 0; ++be) {
+      const bool sing = (h == 0.0) && (mb == al || mb == al + 1) && (mc == be || mc == be + 1);
+      if (sing) {
+        const double v1 = mb, v2 = mc;
+        const double sg1 = (mb == al) ? 1.0 : -1.0;
+        const double sg2 = (mc == be) ? 1.0 : -1.0;
+        const int n = c_rules.n[0];
+        for (int i = 0; i < n; ++i) {
+          const double xi = c_rules.x[0][i];
+          for (int j = 0; j < n; ++j) {
+            const double eta = c_rules.x[0][j];
+            const double W = c_rules.w[0][i] * c_rules.w[0][j];
+            for (int tri = 0; tri < 2; ++tri) {
+              const double t1 = tri == 0 ? xi : xi * eta;
+              const double t2 = tri == 0 ? xi * eta : xi;
+              const double s1 = v1 + sg1 * t1, s2 = v2 + sg2 * t2;
+              const double rho = sqrt(t1 * t1 + t2 * t2);
+              const double2 g = kern(kappa, rho, xi);
+              const double wl = W * lam(s1) * lam(s2);
+              tot.x += wl * g.x;
+              tot.y += wl * g.y;
+            }
+          }
+        }
+      } else {
+        const int n = c_rules.n[rule];
+        for (int i = 0; i < n; ++i) {
+          const double s1 = al + c_rules.x[rule][i];
+          const double d1 = s1 - mb;
+          for (int j = 0; j < n; ++j) {
+            const double s2 = be + c_rules.x[rule][j];
+            const double d2 = s2 - mc;
+            const double rho = sqrt(h * h + d1 * d1 + d2 * d2);
+            const double2 g = kern(kappa, rho, 1.0);
+            const double wl = c_rules.w[rule][i] * c_rules.w[rule][j] * lam(s1) * lam(s2);
+            tot.x += wl * g.x;
+            tot.y += wl * g.y;
+          }
+        }
+      }
+    }
+  }
+  return tot;
+}
+
+__device__ double2 pair_perp(const int m[3], int ap, int sp, int aq, int sq, double kappa, int rule) {
+  const int c = 3 - ap - aq;
+  const int ustar = m[ap] + sp;
+  const int vstar = sq - m[aq];
+  const int sstar = m[c];
+  double2 tot = make_double2(0.0, 0.0);
+  for (int ga = -1; ga <= 0; ++ga) {
+    const bool sing = (ustar == 0 || ustar == 1) && (vstar == 0 || vstar == 1) && (sstar == ga || sstar == ga + 1);
+    if (sing) {
+      const double sgs = (sstar == ga) ? 1.0 : -1.0;
+      const int n = c_rules.n[0];
+      for (int i = 0; i < n; ++i) {
+        const double xi = c_rules.x[0][i];
+        for (int j = 0; j < n; ++j) {
+          const double eta = c_rules.x[0][j];
+          for (int k = 0; k < n; ++k) {
+            const double zeta = c_rules.x[0][k];
+            const double W = c_rules.w[0][i] * c_rules.w[0][j] * c_rules.w[0][k];
+            for (int pyr = 0; pyr < 3; ++pyr) {
+              double tu, tv, ts;
+              if (pyr == 0) { tu = xi; tv = xi * eta; ts = xi * zeta; }
+              else if (pyr == 1) { tu = xi * eta; tv = xi; ts = xi * zeta; }
+              else { tu = xi * eta; tv = xi * zeta; ts = xi; }
+              const double s = sstar + sgs * ts;
+              const double rho = sqrt(tu * tu + tv * tv + ts * ts);
+              const double2 g = kern(kappa, rho, xi * xi);
+              const double wl = W * lam(s);
+              tot.x += wl * g.x;
+              tot.y += wl * g.y;
+            }
+          }
+        }
+      }
+    } else {
+      const int n = c_rules.n[rule];
+      for (int i = 0; i < n; ++i) {
+        const double d1 = c_rules.x[rule][i] - m[ap] - sp;
+        for (int j = 0; j < n; ++j) {
+          const double d2 = sq - m[aq] - c_rules.x[rule][j];
+          for (int k = 0; k < n; ++k) {
+            const double s = ga + c_rules.x[rule][k];
+            const double d3 = s - m[c];
+            const double rho = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+            const double2 g = kern(kappa, rho, 1.0);
+            const double wl = c_rules.w[rule][i] * c_rules.w[rule][j] * c_rules.w[rule][k] * lam(s);
+            tot.x += wl * g.x;
+            tot.y += wl * g.y;
+          }
+        }
+      }
+    }
+  }
+  return tot;
+}
+
+__global__ void k_assemble(double2* __restrict__ G, int L, int M, int N, double kappa, double scale) {
+  const int64_t V = int64_t(L) * M * N;
+  for (int64_t v = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; v < V; v += int64_t(gridDim.x) * blockDim.x) {
+    const int i = int(v % L);
+    const int j = int((v / L) % M);
+    const int l = int(v / (int64_t(L) * M));
+    const int m[3] = {i, j, l};
+    const int mx = max(i, max(j, l));
+    const int rule = mx <= 1 ? 1 : (mx < 5 ? 2 : 3);
+    double2 acc[6];
+    for (int k = 0; k < 6; ++k) acc[k] = make_double2(0.0, 0.0);
+    for (int p = 0; p < 6; ++p) {
+      const int ap = p >> 1, sp = p & 1;
+      for (int q = 0; q < 6; ++q) {
+        const int aq = q >> 1, sq = q & 1;
+        bool any = false;
+        for (int k = 0; k < 6; ++k) any |= c_proj[p][q][k] != 0;
+        if (!any) continue;
+        const double2 I = (ap == aq) ? pair_parallel(m, ap, sp, sq, kappa, rule)
+                                     : pair_perp(m, ap, sp, aq, sq, kappa, rule);
+        for (int k = 0; k < 6; ++k) {
+          const double pr = c_proj[p][q][k];
+          acc[k].x += pr * I.x;
+          acc[k].y += pr * I.y;
+        }
+      }
+    }
+    for (int k = 0; k < 6; ++k) {
+      double2 o = make_double2(acc[k].x * scale, acc[k].y * scale);
+      bool zero = false;
+      for (int ax = 0; ax < 3; ++ax) zero |= (c_parity[k][ax] < 0) && (m[ax] == 0);
+      if (zero) o = make_double2(0.0, 0.0);
+      G[v + V * k] = o;
+    }
+  }
+}
+
+// Gauss-Legendre nodes and weights on [0, 1] (Newton iteration on the
+// three-term recurrence of the Legendre polynomials).
+static void gauss01(int n, double* x, double* w) {
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(kPi * (i + 0.75) / (n + 0.5));
+    double pp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      double z1 = z;
+      z = z1 - p1 / pp;
+      if (std::fabs(z - z1) < 1e-16) break;
+    }
+    // recompute derivative at converged root
+    double p1 = 1.0, p2 = 0.0;
+    for (int j = 1; j <= n; ++j) {
+      double p3 = p2;
+      p2 = p1;
+      p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+    }
+    pp = n * (z * p1 - p2) / (z * z - 1.0);
+    // node z in (-1,1), descending; map to [0,1]
+    x[i] = 0.5 * (1.0 - z);
+    w[i] = 1.0 / ((1.0 - z * z) * pp * pp);   // (2 / ((1-z^2) pp^2)) * 0.5
+  }
+}
+
+static void init_assembly_constants() {
+  static bool done[64] = {false};
+  int dev = 0;
+  VP_HIP(hipGetDevice(&dev));
+  if (dev < 64 && done[dev]) return;
+  GaussRules r{};
+  const int orders[4] = {12, 10, 6, 3};
+  for (int k = 0; k < 4; ++k) {
+    r.n[k] = orders[k];
+    gauss01(orders[k], r.x[k], r.w[k]);
+  }
+  VP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_rules), &r, sizeof(r)));
+  signed char P[6][6][6];
+  const int axes[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+  for (int p = 0; p < 6; ++p)
+    for (int q = 0; q < 6; ++q)
+      for (int k = 0; k < 6; ++k) {
+        // n_p x e_a . n_q x e_b ; n = sign * e_axis
+        const int ap = p >> 1, aq = q >> 1;
+        const int sgp = (p & 1) ? 1 : -1, sgq = (q & 1) ? 1 : -1;
+        const int a = axes[k][0], b = axes[k][1];
+        // (e_i x e_j) . (e_k x e_l) = d_ik d_jl - d_il d_jk
+        int val = (ap == aq && a == b) - (ap == b && a == aq);
+        P[p][q][k] = (signed char)(sgp * sgq * val);
+      }
+  VP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_proj), P, sizeof(P)));
+  if (dev < 64) done[dev] = true;
+}
+
+void assemble_toeplitz_dev(int L, int M, int N, double dx, double ko, double2* G, hipStream_t s) {
+  init_assembly_constants();
+  const int64_t V = int64_t(L) * M * N;
+  const int blocks = int(std::min<int64_t>((V + 127) / 128, 1 << 20));
+  hipLaunchKernelGGL(k_assemble, dim3(blocks), dim3(128), 0, s, G, L, M, N, ko * dx,
+                     dx * dx * dx / (4.0 * kPi));
+  VP_HIP(hipGetLastError());
+  VP_HIP(hipStreamSynchronize(s));
+}
+
+// ----------------------------------------------------------------------------
+// Precision conversion
+// ----------------------------------------------------------------------------
+__global__ void k_c128_to_c64(const double2* __restrict__ in, float2* __restrict__ out, int64_t n, double scale) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    out[i] = make_float2(float(in[i].x * scale), float(in[i].y * scale));
+}
+__global__ void k_c64_to_c128(const float2* __restrict__ in, double2* __restrict__ out, int64_t n, double scale) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    out[i] = make_double2(double(in[i].x) * scale, double(in[i].y) * scale);
+}
+void convert_c128_to_c64(const double2* in, float2* out, int64_t n, double scale, hipStream_t s) {
+  if (n <= 0) return;
+  const int blocks = int(std::min<int64_t>((n + 255) / 256, 8192));
+  hipLaunchKernelGGL(k_c128_to_c64, dim3(blocks), dim3(256), 0, s, in, out, n, scale);
+  VP_HIP(hipGetLastError());
+}
+void convert_c64_to_c128(const float2* in, double2* out, int64_t n, double scale, hipStream_t s) {
+  if (n <= 0) return;
+  const int blocks = int(std::min<int64_t>((n + 255) / 256, 8192));
+  hipLaunchKernelGGL(k_c64_to_c128, dim3(blocks), dim3(256), 0, s, in, out, n, scale);
+  VP_HIP(hipGetLastError());
+}
+
+}  // namespace vp

@@ -1,0 +1,95 @@
+// Internal structures shared by the VoxPhoton device library translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "common.hpp"
+#include "fft.hpp"
+
+namespace vp {
+
+// ----- error handling -------------------------------------------------------
+enum Status { OK = 0, ERR_INVALID = -1, ERR_OOM = -2, ERR_DEVICE = -3, ERR_UNSUPPORTED = -4 };
+void set_error(const std::string& msg);
+struct VpError {
+  int code;
+  std::string msg;
+};
+#define VP_HIP(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      throw ::vp::VpError{e_ == hipErrorOutOfMemory ? ::vp::ERR_OOM : ::vp::ERR_DEVICE, \
+                          std::string(#expr) + ": " + hipGetErrorString(e_)};          \
+  } while (0)
+#define VP_REQUIRE(cond, code, msg)                   \
+  do {                                                \
+    if (!(cond)) throw ::vp::VpError{(code), (msg)}; \
+  } while (0)
+
+// ----- geometry of one operator ----------------------------------------------
+// Grid L x M x N (column-major voxels, l fastest), FFT sizes Lp x Mp x Np.
+// Each FFT dimension is split into R residue classes k = R*a + p (p < R);
+// a class is a DFT of length S = P / R of the folded, twiddled input.
+struct Dim {
+  int D;      // grid extent
+  int P;      // padded FFT length
+  int R;      // number of residue classes
+  int S;      // class DFT length (P / R)
+  FftPlan plan;
+};
+
+struct MatvecPlan {
+  // K1/K3 (x pencils)
+  int B;         // pencils per workgroup
+  int ldsx;      // row stride of a pencil in LDS (elements)
+  int tx;        // threads per workgroup
+  int ept3;      // accumulator elements per thread in K3
+  // K2 (yz planes)
+  int t2;        // threads per workgroup
+  int ept2;      // accumulator elements per thread
+  size_t lds1, lds2;  // dynamic LDS bytes
+};
+
+struct Geometry {
+  int L, M, N;
+  int64_t V;
+  Dim x, y, z;
+  int64_t nS;        // unknown voxels (structured) or unknown entries (generic)
+  int structured;    // 1: idx = [idxS; V+idxS; 2V+idxS] with sorted idxS
+};
+
+// Static device pointers of one problem of a (possibly batched) launch.  The
+// vectors themselves are kernel arguments: problem p reads x + p * vstride
+// and writes y + p * vstride.
+template <typename T>
+struct ProbPtrs {
+  C<T>* work;        // 3 * Lp * M * N spectrum workspace
+  const C<T>* fN;    // tensor in class layout, 6 * Lp * Mp * Np, prescaled
+  const int* pos;    // voxel -> unknown map (V or 3V entries, -1 = not an unknown)
+  const C<T>* mc;    // contrast per unknown voxel (nS) / per unknown (generic n)
+  const C<T>* mr;    // Mr per unknown voxel; nullptr means Mr == 1
+  T gram;            // identity weight in this precision
+};
+
+template <typename T>
+struct VecArgs {
+  const C<T>* x;
+  C<T>* y;
+  int64_t xstride;      // elements between consecutive problems' input vectors
+  int64_t ystride;      // ... and output vectors
+  const int* active;    // per-problem flag (nullptr: all active)
+};
+
+// Host-side planning helpers (plan.cpp)
+bool factorize(int n, bool fp64, FftPlan& out, int tws);
+int max_lds_bytes(int device);
+
+template <typename T>
+void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d_probs, VecArgs<T> va,
+                   int nprob, int transp, const C<T>* twx, const C<T>* twy, const C<T>* twz, hipStream_t s);
+
+bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out, std::string& why);
+
+}  // namespace vp
