@@ -50,7 +50,8 @@ def test_matvec_generic_index_order(vp_ctx):
     assert rel(op.apply(x2), ref) < 1e-12
 
 
-@pytest.mark.parametrize('shape,fft', [((6, 5, 4), (0, 0, 0)), ((7, 5, 3), (16, 10, 6)), ((9, 4, 5), (17, 9, 9))])
+@pytest.mark.parametrize('shape,fft', [((6, 5, 4), (0, 0, 0)), ((7, 5, 3), (16, 10, 6)), ((9, 4, 5), (21, 9, 9)),
+                                       ((9, 4, 5), (18, 8, 12))])
 def test_matvec_from_toeplitz(vp_ctx, shape, fft):
     from voxphoton_amd.vie import NOperator
     G, Mr, Mc, Gram, idx, x = random_problem(*shape, seed=11)

@@ -1,0 +1,182 @@
+"""GPU parity of the preconditioners, GMRES and mixed-precision solves (C ABI)
+against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from oracle import vie as ovie, precond as opre, krylov as okry, greens as ogr
+from tests.helpers import random_problem, random_toeplitz, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _blocks(rng, L, M, N):
+    n3 = 3 * N
+    return (rng.standard_normal((L, M, n3, n3)) + 1j * rng.standard_normal((L, M, n3, n3))) / n3
+
+
+@pytest.mark.parametrize('shape', [(4, 3, 2), (10, 6, 3), (16, 5, 4), (8, 1, 2)])
+def test_chan2_apply_matches_oracle(vp_ctx, shape):
+    from voxphoton_amd.vie import Preconditioner
+    L, M, N = shape
+    rng = np.random.default_rng(L * 100 + M)
+    B = _blocks(rng, L, M, N)
+    mask = rng.random((L, M, N)) < 0.7
+    mask.flat[0] = True
+    idx = ovie.idx_from_mask(mask)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    ref = opre.chan_2_mvp_for_parallel_idx(B, x, L, M, N, idx)
+    P = Preconditioner.chan2(B, L, M, N, idx, ctx=vp_ctx)
+    y = P.apply(x)
+    print(shape, 'chan2 rel', rel(y, ref), flush=True)
+    assert rel(y, ref) < 1e-12
+    assert rel(P.apply(x, 'fp32'), ref) < 1e-5
+
+
+def test_chan1_apply_matches_oracle(vp_ctx):
+    from voxphoton_amd.vie import Preconditioner
+    L, M, N = 6, 3, 2
+    rng = np.random.default_rng(5)
+    n3 = 3 * M * N
+    B = (rng.standard_normal((L, n3, n3)) + 1j * rng.standard_normal((L, n3, n3))) / n3
+    idx = ovie.idx_from_mask(rng.random((L, M, N)) < 0.8)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    ref = opre.chan_mvp_idx(B, x, L, M, N, idx)
+    P = Preconditioner.chan1(B, L, M, N, idx, ctx=vp_ctx)
+    assert rel(P.apply(x), ref) < 1e-12
+
+
+@pytest.mark.parametrize('shape', [(6, 4, 3), (10, 5, 2)])
+def test_level2_device_setup_matches_oracle(vp_ctx, shape):
+    from voxphoton_amd.vie import Preconditioner
+    L, M, N = shape
+    dx, ko = 20e-9, 5.85e6
+    G = ogr.assemble_G(L, M, N, dx, ko)
+    mc, gram = 0.8278, dx ** 3
+    idx = ovie.idx_from_mask(np.ones((L, M, N), bool))
+    Binv = opre.level_2_parallel_func(G, mc, 1.0, L, M, N, gram)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
+    P = Preconditioner.level2(G, mc, 1.0, gram, idx, ctx=vp_ctx)
+    y = P.apply(x)
+    print(shape, 'level2 rel', rel(y, ref), flush=True)
+    assert rel(y, ref) < 1e-10
+
+
+def test_embedded_preconditioner_block_jacobi(vp_ctx):
+    # box preconditioner acting on part of a larger unknown vector (prec_fun_disk_idx.m semantics)
+    from voxphoton_amd.vie import Preconditioner
+    rng = np.random.default_rng(9)
+    L, M, N = 6, 4, 2
+    B = _blocks(rng, L, M, N)
+    idx = ovie.idx_from_mask(np.ones((L, M, N), bool))
+    n_outer = idx.size + 17
+    sub = rng.permutation(n_outer)[: idx.size] + 1
+    P = Preconditioner.chan2(B, L, M, N, idx, ctx=vp_ctx).set_embedding(n_outer, sub)
+    x = rng.standard_normal(n_outer) + 1j * rng.standard_normal(n_outer)
+    ref = x.copy()
+    ref[sub - 1] = opre.chan_2_mvp_for_parallel_idx(B, x[sub - 1], L, M, N, idx)
+    assert rel(P.apply(x), ref) < 1e-12
+
+
+def _physical(L, M, N, seed=0, frac=1.0):
+    dx = 20e-9
+    ko = 2 * np.pi * np.sqrt(2.085) / 1550e-9
+    G = ogr.assemble_G(L, M, N, dx, ko)
+    rng = np.random.default_rng(seed)
+    mask = rng.random((L, M, N)) < frac if frac < 1 else np.ones((L, M, N), bool)
+    mask.flat[0] = True
+    idx = ovie.idx_from_mask(mask)
+    er = 12.11 / 2.085
+    Mc = np.full((L, M, N), (er - 1) / er + 0j)
+    Gram = dx ** 3
+    b = (rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)) * Gram
+    return G, Mc, Gram, idx, b, dx, ko
+
+
+def test_gmres_fp64_matches_oracle(vp_ctx):
+    from voxphoton_amd.vie import NOperator, pgmres
+    G, Mc, Gram, idx, b, dx, ko = _physical(6, 5, 4, seed=1, frac=0.7)
+    op = NOperator.from_toeplitz(G, None, Mc, Gram, idx, ctx=vp_ctx)
+    fN = ovie.fN_from_toeplitz(G)
+    Mr = np.ones(Mc.shape, complex)
+    A = lambda v: ovie.mv_AN_const(v, fN, Mr, Mc, Gram, 'notransp', idx)   # noqa: E731
+    xo, flo, rro, ito, rvo = okry.pgmres(A, b, 10, 1e-10, 20, orth='cgs2')
+    x, flag, relres, it, rv = pgmres(op, b, 10, 1e-10, 20)
+    print('gmres iters', len(rv), len(rvo), 'relres', relres, rro, flush=True)
+    assert flag == 0 and flo == 0
+    k = min(len(rv), len(rvo), 8)
+    assert np.allclose(rv[:k], rvo[:k], rtol=1e-6, atol=1e-12)
+    Ad = ovie.dense_AN(G, Mr, Mc, Gram, idx)
+    assert rel(x, np.linalg.solve(Ad, b)) < 1e-8
+    assert np.linalg.norm(b - Ad @ x) / np.linalg.norm(b) <= 1e-10
+
+
+@pytest.mark.parametrize('side', ['left', 'right'])
+def test_gmres_preconditioned(vp_ctx, side):
+    from voxphoton_amd.vie import NOperator, Preconditioner, pgmres
+    L, M, N = 8, 4, 3
+    G, Mc, Gram, idx, b, dx, ko = _physical(L, M, N, seed=2)
+    op = NOperator.from_toeplitz(G, None, Mc, Gram, idx, ctx=vp_ctx)
+    P = Preconditioner.level2(G, Mc.flat[0], 1.0, Gram, idx, ctx=vp_ctx)
+    kw = {'L1': P} if side == 'left' else {'R1': P}
+    x, flag, relres, it, rv = pgmres(op, b, 20, 1e-9, 10, **kw)
+    x0, flag0, relres0, it0, rv0 = pgmres(op, b, 20, 1e-9, 10)
+    print(side, 'prec iters', len(rv), 'unprec iters', len(rv0), flush=True)
+    Ad = ovie.dense_AN(G, np.ones(Mc.shape, complex), Mc, Gram, idx)
+    assert flag == 0
+    assert np.linalg.norm(b - Ad @ x) / np.linalg.norm(b) <= 1e-9 * 1.01
+    assert len(rv) < len(rv0)
+
+
+def test_gmres_maxit_flag_and_breakdown(vp_ctx):
+    from voxphoton_amd.vie import NOperator, pgmres
+    G, Mc, Gram, idx, b, dx, ko = _physical(5, 4, 3, seed=4)
+    op = NOperator.from_toeplitz(G, None, Mc, Gram, idx, ctx=vp_ctx)
+    x, flag, relres, it, rv = pgmres(op, b, 2, 1e-14, 1)
+    assert flag == 1 and relres > 1e-14            # non-convergence is a flag, not an error
+    op0 = NOperator.from_toeplitz(G, None, 0 * Mc, Gram, idx, ctx=vp_ctx)   # A = Gram I
+    x, flag, relres, it, rv = pgmres(op0, b, 10, 1e-12, 3)
+    assert flag == 0 and len(rv) <= 3 and rel(x, b / Gram) < 1e-12
+
+
+def test_mixed_precision_ir_reaches_fp64_tolerance(vp_ctx):
+    from voxphoton_amd.vie import NOperator, Preconditioner, solve_ir
+    L, M, N = 12, 6, 4
+    G, Mc, Gram, idx, b, dx, ko = _physical(L, M, N, seed=3)
+    op = NOperator.from_toeplitz(G, None, Mc, Gram, idx, ctx=vp_ctx)
+    P = Preconditioner.level2(G, Mc.flat[0], 1.0, Gram, idx, ctx=vp_ctx)
+    x, info = solve_ir(op, b, tol=1e-10, restart=30, inner_tol=1e-4, inner_maxit=3, max_outer=20, M=P)
+    fN = ovie.fN_from_toeplitz(G)
+    r = b - ovie.mv_AN_const(x, fN, np.ones(Mc.shape, complex), Mc, Gram, 'notransp', idx)
+    true_rel = np.linalg.norm(r) / np.linalg.norm(b)
+    print('IR', info, 'true relres', true_rel, flush=True)
+    assert info['flag'] == 0 and true_rel <= 1e-10 * 1.01
+    assert abs(true_rel - info['relres']) <= 1e-3 * true_rel + 1e-15
+
+
+def test_batched_ir_alone_equals_in_batch(vp_ctx):
+    from voxphoton_amd.vie import NOperator, Preconditioner, solve_ir, solve_ir_batched
+    L, M, N = 10, 5, 3
+    dx = 20e-9
+    ops, precs, rhs = [], [], []
+    rng = np.random.default_rng(12)
+    idx = ovie.idx_from_mask(np.ones((L, M, N), bool))
+    er = 12.11 / 2.085
+    Mc = np.full((L, M, N), (er - 1) / er + 0j)
+    for lam in (1500e-9, 1550e-9, 1600e-9):
+        ko = 2 * np.pi * np.sqrt(2.085) / lam
+        ops.append(NOperator.assembled((L, M, N), dx, ko, None, Mc, idx, ctx=vp_ctx))
+        precs.append(Preconditioner.level2_assembled((L, M, N), dx, ko, Mc.flat[0], 1.0, idx, ctx=vp_ctx))
+    for i in range(4):   # two RHS share an operator
+        rhs.append((rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)) * dx ** 3)
+    bops = [ops[0], ops[0], ops[1], ops[2]]
+    bprecs = [precs[0], precs[0], precs[1], precs[2]]
+    X, infos = solve_ir_batched(bops, bprecs, np.array(rhs), tol=1e-8, restart=20, inner_tol=1e-3,
+                                inner_maxit=3, max_outer=15)
+    for i in range(4):
+        xa, ia = solve_ir(bops[i], rhs[i], tol=1e-8, restart=20, inner_tol=1e-3, inner_maxit=3, max_outer=15,
+                          M=bprecs[i])
+        print(i, infos[i]['relres'], ia['relres'], rel(X[i], xa), flush=True)
+        assert infos[i]['flag'] == 0 and ia['flag'] == 0
+        assert rel(X[i], xa) <= 1e-12      # same result alone and in the mix

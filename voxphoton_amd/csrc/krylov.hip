@@ -597,6 +597,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
                          kk.as<int>(), maxcols, x, n, n, 1, dact);
     }
     VP_HIP(hipGetLastError());
+    VP_HIP(hipStreamSynchronize(s));   // iact is released at the end of this cycle
     // restart: fresh residual (pgmres_iter.m:150 carries it by recurrence)
     for (int p = 0; p < np; ++p) hact[p] = (hact[p] && last[p] > prm.tol) ? 1 : 0;
     bool more = false;

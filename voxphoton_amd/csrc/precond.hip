@@ -344,7 +344,7 @@ __global__ void __launch_bounds__(256) k_pc_invert(double2* __restrict__ blk, in
     if (threadIdx.x == 0) {
       s_piv = s_bi[0];
       s_perm[k] = s_bi[0];
-      if (!(s_best[0] > 0.0)) atomicExch(singular, 1);
+      if (!(s_best[0] > 0.0)) singular[0] = 1;
     }
     __syncthreads();
     const int p = s_piv;

@@ -165,3 +165,189 @@ def mv_AN_const(JIn0, fN, Mr, Mc, Gram, transp_flag, idx, GPU_flag=1):
         op = NOperator.from_fN(fN, Mr, Mc, Gram, idx, precisions=('fp64',))
         _op_cache[key] = ent = (op, (fN, Mr, Mc, idx))
     return ent[0].apply(JIn0, transp_flag, 'fp64')
+
+
+class Preconditioner:
+    """Device-resident Chan block-circulant preconditioner (handle returns M^-1 x)."""
+
+    def __init__(self, handle, ctx, nbox):
+        self.handle = handle
+        self.ctx = ctx
+        self.nbox = int(nbox)
+        self.n = int(nbox)
+        self._fin = weakref.finalize(self, L.lib.vp_prec_destroy, handle)
+
+    @classmethod
+    def chan2(cls, blocks, L_, M_, N_, idx, precisions=('fp64', 'fp32'), ctx=None):
+        """From inverse blocks inv_blocks{i,j} given as an (L, M, 3N, 3N) array
+        (chan_2_mvp_for_parallel_idx.m:2-23)."""
+        ctx = ctx or default_context()
+        B = np.asarray(blocks)
+        # C ABI layout: cell {i,j} at (i + L*j), each column-major
+        flat = as_c128(np.transpose(B, (1, 0, 3, 2)).reshape(-1))
+        ix = as_idx(idx)
+        h = ctypes.c_void_p()
+        check(L.lib.vp_prec_create_chan2(ctx.handle, L_, M_, N_, ptr(flat), ptr(ix), ix.size,
+                                         _prec_mask(precisions), ctypes.byref(h)))
+        return cls(h, ctx, ix.size)
+
+    @classmethod
+    def chan1(cls, blocks, L_, M_, N_, idx, precisions=('fp64', 'fp32'), ctx=None):
+        """From inverse blocks inv_blocks{i} given as an (L, 3MN, 3MN) array (chan_mvp_idx.m:2-16)."""
+        ctx = ctx or default_context()
+        B = np.asarray(blocks)
+        flat = as_c128(np.transpose(B, (0, 2, 1)).reshape(-1))
+        ix = as_idx(idx)
+        h = ctypes.c_void_p()
+        check(L.lib.vp_prec_create_chan1(ctx.handle, L_, M_, N_, ptr(flat), ptr(ix), ix.size,
+                                         _prec_mask(precisions), ctypes.byref(h)))
+        return cls(h, ctx, ix.size)
+
+    @classmethod
+    def level2(cls, G, Mc, Mr0, Gram, idx, precisions=('fp64', 'fp32'), ctx=None):
+        """level_2_parallel_func(opToeplitz, Mc, Mr, L, M, N, Gram) with scalar Mc, built on the GPU."""
+        ctx = ctx or default_context()
+        G = np.asarray(G)
+        Lg, Mg, Ng = G.shape[:3]
+        g = as_c128(G.reshape(-1, order='F'))
+        ix = as_idx(idx)
+        h = ctypes.c_void_p()
+        mc = complex(Mc)
+        check(L.lib.vp_prec_create_level2(ctx.handle, Lg, Mg, Ng, ptr(g), mc.real, mc.imag, float(Mr0),
+                                          float(Gram), ptr(ix), ix.size, _prec_mask(precisions),
+                                          ctypes.byref(h)))
+        return cls(h, ctx, ix.size)
+
+    @classmethod
+    def level2_assembled(cls, shape, dx, ko, Mc, Mr0, idx, precisions=('fp64', 'fp32'), ctx=None):
+        """2-level preconditioner from a Green tensor assembled on the GPU."""
+        ctx = ctx or default_context()
+        Lg, Mg, Ng = shape
+        ix = as_idx(idx)
+        h = ctypes.c_void_p()
+        mc = complex(Mc)
+        check(L.lib.vp_prec_create_level2_assembled(ctx.handle, Lg, Mg, Ng, float(dx), float(ko), mc.real,
+                                                    mc.imag, float(Mr0), ptr(ix), ix.size,
+                                                    _prec_mask(precisions), ctypes.byref(h)))
+        return cls(h, ctx, ix.size)
+
+    def set_embedding(self, n_outer, sub_of_op):
+        """Act on an outer vector of length n_outer; box unknown i sits at
+        sub_of_op[i] (1-based); other entries pass through (block Jacobi)."""
+        s = as_idx(sub_of_op)
+        check(L.lib.vp_prec_set_embedding(self.handle, int(n_outer), ptr(s), s.size))
+        self.n = int(n_outer)
+        return self
+
+    def apply(self, x, prec='fp64'):
+        x = as_c128(np.asarray(x).reshape(-1))
+        if x.size != self.n:
+            raise ValueError(f'vector length {x.size} != preconditioner size {self.n}')
+        y = np.empty_like(x)
+        check(L.lib.vp_prec_apply(self.handle, ptr(x), ptr(y), L.VP_MEM_HOST,
+                                  L.VP_FP64 if prec == 'fp64' else L.VP_FP32))
+        return y
+
+    __call__ = apply
+
+
+def _info(cap):
+    buf = np.zeros(max(int(cap), 1), dtype=np.float64)
+    info = L.GmresInfo()
+    info.resvec = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    info.resvec_cap = buf.size
+    return info, buf
+
+
+def pgmres(A, b, restart=50, tol=1e-3, maxit=10, L1=None, L2=None, R1=None, R2=None, x0=None, prec='fp64'):
+    """[x, flag, relres, iter, resvec] = pgmres(A, b, restart, tol, maxit, L1, L2, R1, R2, x0).
+
+    Mirrors pc/src_numeric/src_iterative/pgmres.m:70.  ``A`` is an NOperator,
+    L1 or R1 a Preconditioner (one side; L2/R2 are not supported).  The
+    Krylov loop runs on the GPU in ``prec``; non-convergence gives flag = 1.
+    """
+    if L2 is not None or R2 is not None:
+        raise ValueError('a second preconditioner factor is not supported')
+    if L1 is not None and R1 is not None:
+        raise ValueError('use either a left or a right preconditioner')
+    Mh = (L1 or R1)
+    side = L.VP_PREC_RIGHT if R1 is not None else L.VP_PREC_LEFT
+    b = as_c128(np.asarray(b).reshape(-1))
+    x = np.empty_like(b)
+    x0a = None if x0 is None or not np.any(x0) else as_c128(np.asarray(x0).reshape(-1))
+    info, buf = _info(restart * maxit + 1)
+    check(L.lib.vp_gmres(A.handle, Mh.handle if Mh is not None else None, side, ptr(b), ptr(x0a), ptr(x),
+                         L.VP_MEM_HOST, L.VP_FP64 if prec == 'fp64' else L.VP_FP32, int(restart), float(tol),
+                         int(maxit), ctypes.byref(info)))
+    return x, info.flag, info.relres, [info.iter_inner, info.iter_outer], buf[:info.resvec_len].copy()
+
+
+def solve_ir(A, b, tol=1e-6, restart=50, inner_tol=1e-3, inner_maxit=4, max_outer=20, M=None, side='left'):
+    """Mixed precision: FP64 refinement around FP32 GMRES(restart) on the GPU.
+
+    Returns (x, info dict) with the true FP64 relative residual ``relres``.
+    """
+    b = as_c128(np.asarray(b).reshape(-1))
+    x = np.empty_like(b)
+    info, buf = _info(max_outer + 2)
+    check(L.lib.vp_solve_ir(A.handle, M.handle if M is not None else None,
+                            L.VP_PREC_RIGHT if side == 'right' else L.VP_PREC_LEFT, ptr(b), ptr(x), L.VP_MEM_HOST,
+                            float(tol), int(restart), float(inner_tol), int(inner_maxit), int(max_outer),
+                            ctypes.byref(info)))
+    return x, dict(flag=info.flag, relres=info.relres, ir_steps=info.ir_steps, inner_iters=info.total_iters,
+                   seconds=info.seconds, history=buf[:info.resvec_len].copy())
+
+
+class DeviceArray:
+    """Minimal owned device buffer (vp_malloc) for batched calls."""
+
+    def __init__(self, ctx, nbytes):
+        p = ctypes.c_void_p()
+        check(L.lib.vp_malloc(ctx.handle, int(nbytes), ctypes.byref(p)))
+        self.ptr = p
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self._fin = weakref.finalize(self, L.lib.vp_free, ctx.handle, p)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(L.lib.vp_memcpy_h2d(self.ctx.handle, self.ptr, ptr(a), a.nbytes))
+
+    def download(self, a):
+        assert a.flags['C_CONTIGUOUS'] and a.nbytes <= self.nbytes
+        check(L.lib.vp_memcpy_d2h(self.ctx.handle, ptr(a), self.ptr, a.nbytes))
+        return a
+
+
+def solve_ir_batched(ops, precs, B, tol=1e-6, restart=50, inner_tol=1e-3, inner_maxit=4, max_outer=20,
+                     side='left'):
+    """Batched mixed-precision solves: row i of B through ops[i] (and precs[i])."""
+    ctx = ops[0].ctx
+    B = as_c128(np.asarray(B))
+    ns, n = B.shape
+    db = DeviceArray(ctx, B.nbytes)
+    dx = DeviceArray(ctx, B.nbytes)
+    db.upload(B)
+    hops = (ctypes.c_void_p * ns)(*[o.handle.value for o in ops])
+    hpre = None
+    if precs is not None:
+        hpre = (ctypes.c_void_p * ns)(*[p.handle.value for p in precs])
+    infos = (L.GmresInfo * ns)()
+    bufs = []
+    for i in range(ns):
+        buf = np.zeros(max_outer + 2)
+        bufs.append(buf)
+        infos[i].resvec = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        infos[i].resvec_cap = buf.size
+    check(L.lib.vp_solve_ir_batched(ctypes.cast(hops, ctypes.c_void_p),
+                                    ctypes.cast(hpre, ctypes.c_void_p) if hpre is not None else None, ns,
+                                    L.VP_PREC_RIGHT if side == 'right' else L.VP_PREC_LEFT, db.ptr, dx.ptr,
+                                    float(tol), int(restart), float(inner_tol), int(inner_maxit), int(max_outer),
+                                    ctypes.cast(infos, ctypes.POINTER(L.GmresInfo))))
+    X = np.empty_like(B)
+    dx.download(X)
+    out = [dict(flag=infos[i].flag, relres=infos[i].relres, ir_steps=infos[i].ir_steps,
+                inner_iters=infos[i].total_iters, seconds=infos[i].seconds,
+                history=bufs[i][:infos[i].resvec_len].copy()) for i in range(ns)]
+    return X, out
