@@ -1,0 +1,284 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X FFT-VIE solve path (one GPU).
+
+A step is one mixed-precision solve (FP32 matvec + 2-level circulant
+preconditioner + GMRES(restart) inside FP64 iterative refinement) of the
+workload to a true FP64 relative residual <= tol, with the operator,
+preconditioner and right-hand side resident in HBM.  The JSON line also
+carries the plain FP64 GPU solve rate, FP64/FP32 matvec times with GB/s
+against the canonical byte model, p50/p99 latency, the roofline of the
+dominant kernel (HIP events on the library's stream) and a CPU baseline
+(the FP64 NumPy/SciPy oracle matvec timed on the host cores).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = 'solves/s at 1e-6 residual (mixed vs FP64); ms/matvec + GB/s; p50/p99 latency'
+HBM_PEAK = 8.0e12        # MI355X HBM3E, bytes/s
+
+WORKLOADS = {
+    'bragg_10nm': dict(
+        desc='Bragg grating, Si 500 nm +/-50 nm corrugation, 220 nm, 320 nm x 100 periods, quadratic '
+             'absorbers (Im eps 0->1 over 2 um) at both ends, SiO2, lambda 1550 nm, 10 nm voxels, '
+             'grid 3600x100x44 (FFT 7200x200x88), y-polarised Gaussian (w0 300 nm)',
+        tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=4, max_outer=12, damping=0.2),
+    'straight_wg_20nm': dict(
+        desc='straight Si waveguide 500x220 nm in SiO2, lambda 1550 nm, 20 nm voxels, grid 200x48x24 '
+             '(FFT 400x96x48), y-polarised Gaussian (w0 300 nm)',
+        tol=1e-6, restart=50, inner_tol=1e-2, inner_maxit=2, max_outer=30, damping=0.0),
+    'bragg_20nm': dict(
+        desc='Bragg grating (100 periods) at 20 nm voxels, grid 1800x50x22 (FFT 3600x100x44), lambda 1550 nm',
+        tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=4, max_outer=12, damping=0.2),
+}
+
+
+def mv_model_bytes(prob, s):
+    """Canonical algorithmic bytes of one matvec: s*(6P + 12Q + 3n + nS) + 8*nS."""
+    L, M, N = prob.shape
+    P = 8 * L * M * N
+    Q = 2 * L * M * N
+    nS = int(prob.mask.sum())
+    n = 3 * nS
+    return s * (6 * P + 12 * Q + 3 * n + nS) + 8 * nS
+
+
+def kernel_model_bytes(prob, kernel, s):
+    """Algorithmic bytes per launch of one matvec kernel (same terms as the model)."""
+    L, M, N = prob.shape
+    P = 8 * L * M * N
+    Q = 2 * L * M * N
+    nS = int(prob.mask.sum())
+    n = 3 * nS
+    if kernel == 'plane':     # fN once, half-spectra read and written
+        return s * (6 * P + 6 * Q)
+    if kernel == 'xfwd':      # J read, half-spectra written, idx read
+        return s * (n + 3 * Q) + 4 * nS
+    if kernel == 'xinv':      # half-spectra read, J read, Mc read, output written, idx read
+        return s * (3 * Q + 2 * n + nS) + 4 * nS
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--workload', default='bragg_10nm', choices=sorted(WORKLOADS))
+    ap.add_argument('--no-prec', action='store_true')
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-fp64-solve', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU only (asked for --gpus {args.gpus}); every solve of this path fits '
+              'one MI355X, and multi-GPU sharding is described, not run.', file=sys.stderr)
+        sys.exit(2)
+
+    from voxphoton_amd import vie, configs
+    from voxphoton_amd import _lib as L
+
+    wl = WORKLOADS[args.workload]
+    log = lambda *a: print('[bench]', *a, file=sys.stderr, flush=True)   # noqa: E731
+    prob = configs.config_problem(args.workload)
+    ctx = vie.default_context()
+    n = prob.n
+    log(f'workload {args.workload}: grid {prob.shape}, unknowns {n}')
+
+    t0 = time.perf_counter()
+    op = vie.NOperator.assembled(prob.shape, prob.dx, prob.ko, None, prob.Mc, prob.idx)
+    t_op = time.perf_counter() - t0
+    log(f'operator assembled + embedded in {t_op:.2f} s')
+    prec = None
+    t_pc = 0.0
+    if not args.no_prec:
+        t0 = time.perf_counter()
+        bshape, bidx, sub = prob.box_problem()
+        prec = vie.Preconditioner.level2_assembled(bshape, prob.dx, prob.ko, prob.core_contrast(), 1.0, bidx,
+                                                   damping=wl['damping'])
+        prec.set_embedding(n, sub)
+        t_pc = time.perf_counter() - t0
+        log(f'2-level preconditioner on box {bshape} (damping {wl["damping"]}) built in {t_pc:.2f} s')
+
+    b = prob.gaussian_rhs('y')
+    db = vie.DeviceArray(ctx, b.nbytes)
+    dx = vie.DeviceArray(ctx, b.nbytes)
+    db.upload(b)
+    hp = prec.handle if prec is not None else None
+
+    def solve_mixed():
+        info = L.GmresInfo()
+        L.check(L.lib.vp_solve_ir(op.handle, hp, L.VP_PREC_LEFT, db.ptr, dx.ptr, L.VP_MEM_DEVICE, wl['tol'],
+                                  wl['restart'], wl['inner_tol'], wl['inner_maxit'], wl['max_outer'],
+                                  ctypes.byref(info)))
+        return info
+
+    for _ in range(args.warmup):
+        info = solve_mixed()
+        log(f'warmup solve: relres {info.relres:.3e}, IR steps {info.ir_steps}, inner its {info.total_iters}')
+    L.check(L.lib.vp_ctx_set_profiling(ctx.handle, 1))
+    kernels = ['xfwd', 'plane', 'xinv', 'pc_xfwd', 'pc_plane', 'pc_xinv']
+    for k in kernels:
+        l_, m_ = ctypes.c_int64(), ctypes.c_double()
+        L.check(L.lib.vp_ctx_kernel_stats(ctx.handle, k.encode(), ctypes.byref(l_), ctypes.byref(m_), 1))
+    lat = []
+    infos = []
+    ctx.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        infos.append(solve_mixed())
+        ctx.synchronize()
+        lat.append(time.perf_counter() - ts)
+    ctx.synchronize()
+    t_total = time.perf_counter() - t_start
+    stats = {}
+    for k in kernels:
+        l_, m_ = ctypes.c_int64(), ctypes.c_double()
+        L.check(L.lib.vp_ctx_kernel_stats(ctx.handle, k.encode(), ctypes.byref(l_), ctypes.byref(m_), 1))
+        stats[k] = (l_.value, m_.value)
+    L.check(L.lib.vp_ctx_set_profiling(ctx.handle, 0))
+    solves_per_s = args.steps / t_total
+    worst = max(i.relres for i in infos)
+    log(f'mixed: {solves_per_s:.3f} solves/s, worst relres {worst:.3e}')
+
+    # ---- matvec microbenchmark (device vectors, both precisions)
+    x64 = vie.DeviceArray(ctx, n * 16)
+    y64 = vie.DeviceArray(ctx, n * 16)
+    rng = np.random.default_rng(1550)
+    xv = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x64.upload(xv)
+    x32 = vie.DeviceArray(ctx, n * 8)
+    y32 = vie.DeviceArray(ctx, n * 8)
+    x32.upload(xv.astype(np.complex64))
+
+    def time_mv(xd, yd, prec_id, reps=10):
+        for _ in range(2):
+            L.check(L.lib.vp_op_apply(op.handle, 0, xd.ptr, yd.ptr, L.VP_MEM_DEVICE, prec_id))
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            L.check(L.lib.vp_op_apply(op.handle, 0, xd.ptr, yd.ptr, L.VP_MEM_DEVICE, prec_id))
+        ctx.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    t_mv64 = time_mv(x64, y64, L.VP_FP64)
+    t_mv32 = time_mv(x32, y32, L.VP_FP32)
+    B64 = mv_model_bytes(prob, 16)
+    B32 = mv_model_bytes(prob, 8)
+    log(f'matvec fp64 {t_mv64 * 1e3:.3f} ms ({B64 / t_mv64 / 1e9:.0f} GB/s model), '
+        f'fp32 {t_mv32 * 1e3:.3f} ms ({B32 / t_mv32 / 1e9:.0f} GB/s model)')
+
+    # ---- plain FP64 GPU solve (serial)
+    fp64 = None
+    if not args.no_fp64_solve:
+        info = L.GmresInfo()
+        ctx.synchronize()
+        t = time.perf_counter()
+        L.check(L.lib.vp_gmres(op.handle, hp, L.VP_PREC_LEFT, db.ptr, None, dx.ptr, L.VP_MEM_DEVICE, L.VP_FP64,
+                               wl['restart'], wl['tol'], 400, ctypes.byref(info)))
+        ctx.synchronize()
+        t64 = time.perf_counter() - t
+        fp64 = dict(solves_per_s=1.0 / t64, seconds=t64, iterations=info.total_iters, relres=info.relres,
+                    flag=info.flag)
+        log(f'plain fp64: {t64:.3f} s/solve, {info.total_iters} its, relres {info.relres:.3e}')
+
+    # ---- roofline of the dominant kernel
+    dom = max(stats, key=lambda k: stats[k][1])
+    launches, ms = stats[dom]
+    avg_s = ms / 1e3 / max(launches, 1)
+    kb = kernel_model_bytes(prob, dom, 8)
+    roof = None
+    if kb is not None and launches:
+        achieved = kb / avg_s / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, 'profiles', f'traffic_{dom}_{args.workload}.json')
+        if os.path.exists(tf):
+            with open(tf) as f:
+                traffic = json.load(f).get('bytes_per_launch')
+        roof = dict(kernel=dom, bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK / 1e9, unit='GB/s',
+                    frac=round(achieved / (HBM_PEAK / 1e9), 4), traffic=traffic, avg_us=round(avg_s * 1e6, 2),
+                    launches=launches, model_bytes_per_launch=kb, precision='fp32')
+
+    # ---- CPU baseline: the FP64 oracle matvec on the host cores
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(prob, vie, fp64, log)
+
+    lat_ms = np.array(lat) * 1e3
+    line = {
+        'metric': METRIC,
+        'value': round(solves_per_s, 4),
+        'unit': 'solves/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(t_total / args.steps * 1e3, 3),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'c64 inner (FP32 matvec/preconditioner/GMRES), c128 outer residual',
+        'data': 'synthetic (device-assembled Green tensor, generated geometry)',
+        'config': {'workload': args.workload, 'description': wl['desc'], 'unknowns': n, 'tol': wl['tol'],
+                   'restart': wl['restart'], 'inner_tol': wl['inner_tol'], 'inner_maxit': wl['inner_maxit'],
+                   'preconditioner': None if prec is None else
+                   f'2-level circulant on the core box, complex-shift damping {wl["damping"]}',
+                   'parallelism': 'single GPU'},
+        'relres_worst': worst,
+        'ir_steps': [i.ir_steps for i in infos],
+        'inner_iterations': [i.total_iters for i in infos],
+        'latency_ms': {'p50': round(float(np.percentile(lat_ms, 50)), 3),
+                       'p99': round(float(np.percentile(lat_ms, 99)), 3)},
+        'fp64_plain': fp64,
+        'mixed_vs_fp64': None if fp64 is None else round(solves_per_s / fp64['solves_per_s'], 3),
+        'matvec': {'fp64_ms': round(t_mv64 * 1e3, 4), 'fp32_ms': round(t_mv32 * 1e3, 4),
+                   'model_bytes_fp64': B64, 'model_bytes_fp32': B32,
+                   'fp64_GBps': round(B64 / t_mv64 / 1e9, 1), 'fp32_GBps': round(B32 / t_mv32 / 1e9, 1),
+                   'fp64_frac_hbm': round(B64 / t_mv64 / HBM_PEAK, 4),
+                   'fp32_frac_hbm': round(B32 / t_mv32 / HBM_PEAK, 4)},
+        'kernel_ms': {k: dict(launches=v[0], avg_us=round(v[1] * 1e3 / max(v[0], 1), 2)) for k, v in stats.items()},
+        'setup_s': {'operator': round(t_op, 3), 'preconditioner': round(t_pc, 3)},
+        'roofline': roof,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(prob, vie, fp64, log):
+    """FP64 oracle matvec (NumPy/SciPy restatement of mv_AN_const) on the host."""
+    from oracle import vie as ovie
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    G = vie.assemble_toeplitz(prob.shape, prob.dx, prob.ko)
+    fN = ovie.fN_from_toeplitz(G, workers=cores)
+    del G
+    rng = np.random.default_rng(1550)
+    x = rng.standard_normal(prob.n) + 1j * rng.standard_normal(prob.n)
+    Mr = np.ones(prob.shape, complex)
+    Mc = prob.Mc
+    idx = prob.idx
+    t = time.perf_counter()
+    ovie.mv_AN_const(x, fN, Mr, Mc, prob.gram, 'notransp', idx, workers=cores)
+    t_mv = time.perf_counter() - t
+    log(f'cpu oracle matvec {t_mv:.2f} s on {cores} threads')
+    n_mv = None
+    if fp64 is not None:
+        n_mv = fp64['iterations'] + 2
+    value = None if n_mv is None else 1.0 / (t_mv * n_mv)
+    return dict(value=value, unit='solves/s', cores=cores, kind='port',
+                sample=f'one FP64 oracle matvec (scipy.fft, {cores} threads) = {t_mv:.2f} s, extrapolated to the '
+                       f'{n_mv} matvecs of one plain FP64 GMRES solve; preconditioner and Arnoldi work excluded',
+                matvec_s=t_mv)
+
+
+if __name__ == '__main__':
+    main()

@@ -68,6 +68,14 @@ int vp_device_count(int* count);
 int vp_ctx_create(int device, vp_ctx* out);
 int vp_ctx_destroy(vp_ctx ctx);
 int vp_ctx_synchronize(vp_ctx ctx);
+/* Per-kernel timing on the context's stream with HIP events: after
+ * vp_ctx_set_profiling(ctx, 1) every launch of the named kernels ("xfwd",
+ * "plane", "xinv", "pc_xfwd", "pc_plane", "pc_xinv") records an event pair;
+ * vp_ctx_kernel_stats returns the launch count and summed milliseconds
+ * (reset != 0 clears them).  No counterpart in the reference (which only
+ * prints tic/toc totals); used by the benchmark's roofline figures. */
+int vp_ctx_set_profiling(vp_ctx ctx, int on);
+int vp_ctx_kernel_stats(vp_ctx ctx, const char* kernel, int64_t* launches, double* total_ms, int reset);
 /* Device memory helpers for callers without their own allocator. */
 int vp_malloc(vp_ctx ctx, int64_t bytes, void** dptr);
 int vp_free(vp_ctx ctx, void* dptr);
@@ -147,10 +155,27 @@ int vp_prec_create_chan1(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const doub
  */
 int vp_prec_create_level2(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double* G, double mc_re, double mc_im,
                           double mr0, double gram, const int64_t* idx, int64_t n, int precisions, vp_prec* out);
-/* Same, from a device-assembled tensor (see vp_op_create_assembled). */
-int vp_prec_create_level2_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko, double mc_re,
+/* Same, from a device-assembled tensor (see vp_op_create_assembled).  damping >= 0
+ * assembles the preconditioner's kernel with the complex-shifted wavenumber
+ * ko*(1 - j*damping), which keeps every circulant block away from singularity
+ * (0 reproduces level_2_parallel_func exactly). */
+int vp_prec_create_level2_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko, double damping,
+                                    double mc_re,
                                     double mc_im, double mr0, const int64_t* idx, int64_t n, int precisions,
                                     vp_prec* out);
+/*
+ * Face-averaged 2-level preconditioner (the form Bragg_MOR_script.m:61-72
+ * builds with level_1_parallel_func_N: inv(Gram*I - diag(face3)*circ_N)),
+ * with the circulant approximation taken along the first two axes of the box
+ * and the third axis exact: blocks inv(Gram*I - diag(face(n)) * N2(kx, ky)),
+ * N2 the 2-level Chan circulant of the bare N operator (Green tensor
+ * assembled on the device for dx, ko), face: complex[N] contrast profile
+ * along the exact axis (e.g. (e-1)/e of the box-averaged permittivity).
+ * Callers permute axes so that the exact axis comes last.
+ */
+int vp_prec_create_level2_face(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko, double damping,
+                               const double* face,
+                               const int64_t* idx, int64_t n, int precisions, vp_prec* out);
 /* Embeds a preconditioner built on a sub-box into the operator's unknowns:
  * sub_of_op[i] (1-based, length nsub) is the position in the operator's
  * unknown vector of the sub-box unknown i; unknowns not covered pass

@@ -278,6 +278,21 @@ def level_2_blocks_direct(opToeplitz, Mc, Mr, L, M, N, Gram):
     return out
 
 
+def level_2_face_blocks(opToeplitz, face, Gram):
+    """Face-averaged 2-level inverse blocks inv(Gram*I - diag(face3)*N2).
+
+    The form of Bragg_MOR_script.m:61-72 (circ_N from level_1_parallel_func_N
+    without material, then Gram*I - diag(face3)*circ_N) with the circulant
+    approximation of level_2_parallel_func.m taken along the first two axes
+    and ``face`` (length N) the contrast along the exact third axis.
+    """
+    L, M, N, _ = opToeplitz.shape
+    N2 = level_2_blocks_direct(np.asarray(opToeplitz, dtype=complex), -1.0, 0.0, L, M, N, 0.0)
+    face3 = np.tile(np.asarray(face, dtype=complex), 3)
+    eye = np.eye(3 * N)
+    return np.linalg.inv(Gram * eye - face3[None, None, :, None] * N2)
+
+
 def level_2_parallel_func(opToeplitz, Mc, Mr, L, M, N, Gram):
     """Inverse 2-level blocks (level_2_parallel_func.m:1-192)."""
     return np.linalg.inv(level_2_blocks_direct(opToeplitz, Mc, Mr, L, M, N, Gram))

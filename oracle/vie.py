@@ -65,14 +65,17 @@ def circulant_nop_const(G):
     return Gp
 
 
-def fft_operator(A):
+def fft_operator(A, workers=None):
     """Per-component unnormalised 3D FFT (pc/src_vie/src_operators/fft_operator.m:1-8)."""
+    if workers and workers > 1:
+        import scipy.fft as sfft
+        return sfft.fftn(A, axes=(0, 1, 2), workers=workers)
     return np.fft.fftn(A, axes=(0, 1, 2))
 
 
-def fN_from_toeplitz(G):
+def fN_from_toeplitz(G, workers=None):
     """fN = fft_operator(circulant_nop_const(G)) (getOPERATORS_SAM.m:89-91)."""
-    return fft_operator(circulant_nop_const(G))
+    return fft_operator(circulant_nop_const(G), workers)
 
 
 def _scatter(JIn0, idx, L, M, N):
@@ -81,30 +84,38 @@ def _scatter(JIn0, idx, L, M, N):
     return JIn.reshape((L, M, N, 3), order='F')
 
 
-def mv_AN_const(JIn0, fN, Mr, Mc, Gram, transp_flag, idx):
+def mv_AN_const(JIn0, fN, Mr, Mc, Gram, transp_flag, idx, workers=None):
     """JOut = [Gram*Mr.*J - Mc.*N(J)](idx) via zero-padded FFT convolution.
 
     Restates pc/src_vie/src_mvp/mv_AN_const.m:1-146 (CPU branch, GPU_flag=0):
     scatter (:52-58), 'notransp' (:102-132), 'transp' (:66-99), gather (:145).
     ``Mr`` and ``Mc`` are L x M x N complex arrays; ``Gram`` is a real scalar.
+    ``workers`` > 1 runs the FFTs multi-threaded (scipy.fft; same pocketfft
+    algorithm as numpy.fft).
     """
+    if workers and workers > 1:
+        import scipy.fft as sfft
+        fftn = lambda a, s=None: sfft.fftn(a, s=s, axes=(0, 1, 2), workers=workers)   # noqa: E731
+        ifftn = lambda a: sfft.ifftn(a, axes=(0, 1, 2), workers=workers)               # noqa: E731
+    else:
+        fftn = lambda a, s=None: np.fft.fftn(a, s=s, axes=(0, 1, 2))                  # noqa: E731
+        ifftn = lambda a: np.fft.ifftn(a, axes=(0, 1, 2))                              # noqa: E731
     Lp, Mp, Np = fN.shape[:3]
     L, M, N = Mr.shape
     JIn = _scatter(JIn0, idx, L, M, N)
     JOut = np.zeros((L, M, N, 3), dtype=complex)
     if transp_flag == 'notransp':
-        fJ = [np.fft.fftn(JIn[..., d], s=(Lp, Mp, Np), axes=(0, 1, 2)) for d in range(3)]
+        fJ = [fftn(JIn[..., d], s=(Lp, Mp, Np)) for d in range(3)]
         for c in range(3):
             acc = sum(fN[..., COMP[c, d]] * fJ[d] for d in range(3))
-            Jc = np.fft.ifftn(acc)[:L, :M, :N]
+            Jc = ifftn(acc)[:L, :M, :N]
             JOut[..., c] = Gram * Mr * JIn[..., c] - Mc * Jc
     elif transp_flag == 'transp':
         Mcc = np.conj(Mc)
-        fJ = [np.fft.fftn(np.conj(Mcc * JIn[..., d]), s=(Lp, Mp, Np), axes=(0, 1, 2))
-              for d in range(3)]
+        fJ = [fftn(np.conj(Mcc * JIn[..., d]), s=(Lp, Mp, Np)) for d in range(3)]
         for c in range(3):
             acc = sum(fN[..., COMP[c, d]] * fJ[d] for d in range(3))
-            Jc = np.fft.ifftn(acc)[:L, :M, :N]
+            Jc = ifftn(acc)[:L, :M, :N]
             JOut[..., c] = Gram * np.conj(Mr) * JIn[..., c] - np.conj(Jc)
     else:
         raise ValueError("transp_flag must be 'transp' or 'notransp'")

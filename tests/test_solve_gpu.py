@@ -63,6 +63,37 @@ def test_level2_device_setup_matches_oracle(vp_ctx, shape):
     assert rel(y, ref) < 1e-10
 
 
+def test_level2_blocks_wider_than_a_wavefront(vp_ctx):
+    # 3N = 66 > 64: block inversion spans several wavefronts
+    from voxphoton_amd.vie import Preconditioner
+    L, M, N = 4, 3, 22
+    dx, ko = 20e-9, 5.85e6
+    G = ogr.assemble_G(L, M, N, dx, ko)
+    idx = ovie.idx_from_mask(np.ones((L, M, N), bool))
+    Binv = opre.level_2_parallel_func(G, 0.8278, 1.0, L, M, N, dx ** 3)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
+    P = Preconditioner.level2(G, 0.8278, 1.0, dx ** 3, idx, ctx=vp_ctx)
+    assert rel(P.apply(x), ref) < 1e-10
+    assert rel(P.apply(x, 'fp32'), ref) < 1e-4
+
+
+def test_level2_face_matches_oracle(vp_ctx):
+    from voxphoton_amd.vie import Preconditioner
+    L, M, N = 8, 5, 4
+    dx, ko = 20e-9, 5.85e6
+    G = ogr.assemble_G(L, M, N, dx, ko)
+    face = np.array([0.3, 0.8, 0.8, 0.5], complex)
+    idx = ovie.idx_from_mask(np.ones((L, M, N), bool))
+    Binv = opre.level_2_face_blocks(G, face, dx ** 3)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
+    P = Preconditioner.level2_face((L, M, N), dx, ko, face, idx, ctx=vp_ctx)
+    assert rel(P.apply(x), ref) < 1e-10
+
+
 def test_embedded_preconditioner_block_jacobi(vp_ctx):
     # box preconditioner acting on part of a larger unknown vector (prec_fun_disk_idx.m semantics)
     from voxphoton_amd.vie import Preconditioner

@@ -52,6 +52,8 @@ SIGNATURES = {
     'vp_ctx_create': (c_int, [c_int, P(c_vp)]),
     'vp_ctx_destroy': (c_int, [c_vp]),
     'vp_ctx_synchronize': (c_int, [c_vp]),
+    'vp_ctx_set_profiling': (c_int, [c_vp, c_int]),
+    'vp_ctx_kernel_stats': (c_int, [c_vp, ctypes.c_char_p, P(c_i64), P(c_dbl), c_int]),
     'vp_malloc': (c_int, [c_vp, c_i64, P(c_vp)]),
     'vp_free': (c_int, [c_vp, c_vp]),
     'vp_memcpy_h2d': (c_int, [c_vp, c_vp, c_vp, c_i64]),
@@ -72,8 +74,10 @@ SIGNATURES = {
     'vp_prec_create_chan1': (c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_int, P(c_vp)]),
     'vp_prec_create_level2': (c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_vp, c_i64,
                                       c_int, P(c_vp)]),
-    'vp_prec_create_level2_assembled': (c_int, [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl,
+    'vp_prec_create_level2_assembled': (c_int, [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl,
                                                 c_vp, c_i64, c_int, P(c_vp)]),
+    'vp_prec_create_level2_face': (c_int, [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_vp, c_vp, c_i64, c_int,
+                                           P(c_vp)]),
     'vp_prec_set_embedding': (c_int, [c_vp, c_i64, c_vp, c_i64]),
     'vp_prec_apply': (c_int, [c_vp, c_vp, c_vp, c_int, c_int]),
     'vp_prec_set_fp32_scale': (c_int, [c_vp, c_dbl]),

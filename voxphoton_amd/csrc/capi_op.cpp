@@ -86,6 +86,30 @@ int vp_ctx_synchronize(vp_ctx ctx) {
   VP_API_END
 }
 
+int vp_ctx_set_profiling(vp_ctx ctx, int on) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx, ERR_INVALID, "ctx is NULL");
+  VP_HIP(hipSetDevice(ctx->device));
+  prof_enable(ctx->stream, on != 0);
+  VP_API_END
+}
+
+int vp_ctx_kernel_stats(vp_ctx ctx, const char* kernel, int64_t* launches, double* total_ms, int reset) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && kernel && launches && total_ms, ERR_INVALID, "bad arguments");
+  int kid = -1;
+  for (int k = 0; k < kNumKernelIds; ++k)
+    if (std::string(kernel) == kernel_id_name(k)) kid = k;
+  VP_REQUIRE(kid >= 0, ERR_INVALID, std::string("unknown kernel name ") + kernel);
+  VP_HIP(hipSetDevice(ctx->device));
+  long long l = 0;
+  double ms = 0;
+  prof_stats(ctx->stream, kid, &l, &ms, reset != 0);
+  *launches = l;
+  *total_ms = ms;
+  VP_API_END
+}
+
 int vp_malloc(vp_ctx ctx, int64_t bytes, void** dptr) {
   VP_API_BEGIN
   VP_REQUIRE(ctx && dptr && bytes >= 0, ERR_INVALID, "vp_malloc: bad argument");

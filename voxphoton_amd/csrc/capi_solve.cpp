@@ -391,15 +391,41 @@ int vp_prec_create_level2(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const dou
   VP_API_END
 }
 
-int vp_prec_create_level2_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko, double mc_re,
+int vp_prec_create_level2_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko, double damping,
+                                    double mc_re,
                                     double mc_im, double mr0, const int64_t* idx, int64_t n, int precisions,
                                     vp_prec* out) {
   VP_API_BEGIN
-  VP_REQUIRE(ctx && out && dx > 0, ERR_INVALID, "bad arguments");
+  VP_REQUIRE(ctx && out && dx > 0 && damping >= 0, ERR_INVALID, "bad arguments");
   VP_HIP(hipSetDevice(ctx->device));
   DevBuf dG(size_t(L * M * N) * 6 * sizeof(double2));
-  assemble_toeplitz_dev(int(L), int(M), int(N), dx, ko, dG.as<double2>(), ctx->stream);
+  assemble_toeplitz_dev(int(L), int(M), int(N), dx, ko, dG.as<double2>(), ctx->stream, -damping * ko);
   prec_level2_common(ctx, L, M, N, dG.as<double2>(), mc_re, mc_im, mr0, dx * dx * dx, idx, n, precisions, out);
+  VP_API_END
+}
+
+int vp_prec_create_level2_face(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko, double damping,
+                               const double* face, const int64_t* idx, int64_t n, int precisions, vp_prec* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && out && face && dx > 0 && damping >= 0, ERR_INVALID, "bad arguments");
+  VP_HIP(hipSetDevice(ctx->device));
+  DevBuf dG(size_t(L * M * N) * 6 * sizeof(double2));
+  assemble_toeplitz_dev(int(L), int(M), int(N), dx, ko, dG.as<double2>(), ctx->stream, -damping * ko);
+  DevBuf dface(size_t(N) * sizeof(double2));
+  VP_HIP(hipMemcpy(dface.p, face, dface.bytes, hipMemcpyHostToDevice));
+  vp_prec_s* p = prec_begin(ctx, L, M, N, idx, n, 2);
+  try {
+    const int64_t n3 = 3 * N;
+    p->p64.blocks.alloc(size_t(L * M * n3 * n3) * sizeof(double2));
+    build_level2_blocks(dG.as<double2>(), int(L), int(M), int(N), make_double2(-1.0, 0.0), 0.0,
+                        p->p64.blocks.as<double2>(), ctx->stream, dface.as<double2>(), dx * dx * dx);
+    invert_blocks(p->p64.blocks.as<double2>(), L * M, int(n3), ctx->stream);
+    prec_finish(p, precisions);
+  } catch (...) {
+    delete p;
+    throw;
+  }
+  *out = p;
   VP_API_END
 }
 

@@ -284,6 +284,7 @@ void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d
   {
     const void* fn = reinterpret_cast<const void*>(&k_xfwd<T>);
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds1));
+    ProfScope ps(s, KID_XFWD);
     hipLaunchKernelGGL(k_xfwd<T>, gx, dim3(mp.tx), mp.lds1, s, d_probs, va, g, mp, transp, twx);
     VP_HIP(hipGetLastError());
   }
@@ -293,6 +294,7 @@ void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d
   do {                                                                                      \
     const void* fn = reinterpret_cast<const void*>(&k_plane<T, E>);                         \
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds2)); \
+    ProfScope ps(s, KID_PLANE);                                                             \
     hipLaunchKernelGGL((k_plane<T, E>), gp, dim3(mp.t2), mp.lds2, s, d_probs, va.active, g, twy, twz);  \
   } while (0)
     VP_EPT_DISPATCH(mp.ept2, VP_K2);
@@ -304,6 +306,7 @@ void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d
   do {                                                                                      \
     const void* fn = reinterpret_cast<const void*>(&k_xinv<T, E>);                          \
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds1)); \
+    ProfScope ps(s, KID_XINV);                                                              \
     hipLaunchKernelGGL((k_xinv<T, E>), gx, dim3(mp.tx), mp.lds1, s, d_probs, va, g, mp, transp, twx); \
   } while (0)
     VP_EPT_DISPATCH(mp.ept3, VP_K3);

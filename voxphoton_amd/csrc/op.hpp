@@ -85,7 +85,8 @@ void layout_fN(vp_op_s* op, const double2* F, hipStream_t s);
 // True-frequency tensor from a device Toeplitz tensor (L x M x N x 6).
 void toeplitz_to_fN(const vp_op_s& op, const double2* G, double2* F, hipStream_t s);
 // Device assembly of the Toeplitz tensor.
-void assemble_toeplitz_dev(int L, int M, int N, double dx, double ko, double2* G, hipStream_t s);
+// ko_im < 0 assembles a damped kernel e^{-j(ko + j ko_im)R}/R (preconditioners only).
+void assemble_toeplitz_dev(int L, int M, int N, double dx, double ko, double2* G, hipStream_t s, double ko_im = 0.0);
 
 template <typename T>
 void op_apply_dev(vp_op_s* op, int transp, const C<T>* x, C<T>* y);

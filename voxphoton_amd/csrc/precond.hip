@@ -183,13 +183,20 @@ void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) 
   const dim3 gx(nmb * g.N * 3, nprob);
   const void* f1 = reinterpret_cast<const void*>(&k_pc_xfwd<T>);
   (void)hipFuncSetAttribute(f1, hipFuncAttributeMaxDynamicSharedMemorySize, int(g.lds1));
-  hipLaunchKernelGGL(k_pc_xfwd<T>, gx, dim3(g.tx), g.lds1, s, a, g);
+  {
+    ProfScope ps(s, KID_PC_XFWD);
+    hipLaunchKernelGGL(k_pc_xfwd<T>, gx, dim3(g.tx), g.lds1, s, a, g);
+  }
   VP_HIP(hipGetLastError());
   if (g.level == 2) {
     const void* f2 = reinterpret_cast<const void*>(&k_pc_plane<T>);
     (void)hipFuncSetAttribute(f2, hipFuncAttributeMaxDynamicSharedMemorySize, int(g.lds2));
-    hipLaunchKernelGGL(k_pc_plane<T>, dim3(g.L, nprob), dim3(g.t2), g.lds2, s, a, g);
+    {
+      ProfScope ps(s, KID_PC_PLANE);
+      hipLaunchKernelGGL(k_pc_plane<T>, dim3(g.L, nprob), dim3(g.t2), g.lds2, s, a, g);
+    }
     VP_HIP(hipGetLastError());
+    ProfScope ps(s, KID_PC_XINV);
     hipLaunchKernelGGL(k_pc_xinv<T>, gx, dim3(g.tx), g.lds1, s, a, g);
   } else {
     const int64_t n3 = 3LL * g.M * g.N;
@@ -276,8 +283,11 @@ __device__ __forceinline__ double2 pc_row(int k, int nt, int ns, int t, double2 
   return v;
 }
 
+// With ``face`` set, the entry becomes Gram*delta - face(n_t) * N2 entry
+// (the face-averaged form inv(Gram*I - diag(face3)*circ_N) of
+// Bragg_MOR_script.m:64-72, here with the exact axis last).
 __global__ void k_pc_blocks(const double2* __restrict__ ck, double2* __restrict__ blk, int L, int M, int N,
-                            const double2* __restrict__ twM) {
+                            const double2* __restrict__ twM, const double2* __restrict__ face, double gram) {
   const int n3 = 3 * N;
   const int64_t total = int64_t(L) * M * n3 * n3;
   for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
@@ -304,6 +314,11 @@ __global__ void k_pc_blocks(const double2* __restrict__ ck, double2* __restrict_
       const double2 w = twM[ph];
       acc.x += cv.x * w.x - cv.y * w.y;
       acc.y += cv.x * w.y + cv.y * w.x;
+    }
+    if (face) {
+      const double2 f = face[nt];
+      acc = make_double2(-(f.x * acc.x - f.y * acc.y), -(f.x * acc.y + f.y * acc.x));
+      if (row == colq) acc.x += gram;
     }
     blk[e] = acc;
   }
@@ -359,6 +374,7 @@ __global__ void __launch_bounds__(256) k_pc_invert(double2* __restrict__ blk, in
     const double2 piv = A[int64_t(k) * n + k];
     const double den = piv.x * piv.x + piv.y * piv.y;
     const double2 ip = den > 0 ? make_double2(piv.x / den, -piv.y / den) : make_double2(0.0, 0.0);
+    __syncthreads();   // every wavefront has read the pivot before row k is overwritten
     // scale row k (pivot entry becomes 1/piv)
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
       double2 v = (j == k) ? make_double2(1.0, 0.0) : A[int64_t(j) * n + k];
@@ -409,7 +425,7 @@ __global__ void __launch_bounds__(256) k_pc_invert(double2* __restrict__ blk, in
 }
 
 void build_level2_blocks(const double2* G, int L, int M, int N, double2 mc, double diag, double2* blk,
-                         hipStream_t s) {
+                         hipStream_t s, const double2* face_dev, double gram) {
   auto table = [](int P) {
     std::vector<double2> t(P);
     for (int k = 0; k < P; ++k) {
@@ -434,7 +450,7 @@ void build_level2_blocks(const double2* G, int L, int M, int N, double2 mc, doub
     const int64_t total = int64_t(L) * M * 9 * N * N;
     const int blocks = int(std::min<int64_t>((total + 255) / 256, 1 << 20));
     hipLaunchKernelGGL(k_pc_blocks, dim3(blocks), dim3(256), 0, s, ck.as<double2>(), blk, L, M, N,
-                       dM.as<double2>());
+                       dM.as<double2>(), face_dev, gram);
     VP_HIP(hipGetLastError());
   }
   VP_HIP(hipStreamSynchronize(s));

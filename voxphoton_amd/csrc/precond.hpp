@@ -45,8 +45,12 @@ template <typename T>
 void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s);
 
 // FP64 block construction (level_2_parallel_func.m semantics) and inversion.
+// face_dev == nullptr: level_2_parallel_func semantics (A = -Mc*G, Gram*Mr0 =
+// diag folded at the origin).  face_dev = device complex[N]: blocks are
+// Gram*I - diag(face(n))*N2 with N2 the bare 2-level circulant of G (pass
+// mc = -1, diag = 0).
 void build_level2_blocks(const double2* G, int L, int M, int N, double2 mc, double diag, double2* blk,
-                         hipStream_t s);
+                         hipStream_t s, const double2* face_dev = nullptr, double gram = 0.0);
 void invert_blocks(double2* blk, int64_t nblocks, int n, hipStream_t s);
 
 }  // namespace vp

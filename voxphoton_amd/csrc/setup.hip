@@ -219,14 +219,15 @@ __constant__ signed char c_proj[6][6][6];
 
 __device__ __forceinline__ double lam(double s) { return 1.0 - fabs(s); }
 
-__device__ __forceinline__ double2 kern(double kappa, double rho, double jac) {
+// e^{-j kappa rho} / rho * jac for complex kappa = (re, im); im <= 0 damps.
+__device__ __forceinline__ double2 kern(double2 kappa, double rho, double jac) {
   double sn, cs;
-  sincos(kappa * rho, &sn, &cs);
-  const double f = jac / rho;
+  sincos(kappa.x * rho, &sn, &cs);
+  const double f = (kappa.y != 0.0 ? exp(kappa.y * rho) : 1.0) * jac / rho;
   return make_double2(cs * f, -sn * f);
 }
 
-__device__ double2 pair_parallel(const int m[3], int a, int sp, int sq, double kappa, int rule) {
+__device__ double2 pair_parallel(const int m[3], int a, int sp, int sq, double2 kappa, int rule) {
   const int b = (a == 0) ? 1 : 0;
   const int c = (a == 2) ? 1 : 2;
   const double h = double(sq - (m[a] + sp));
@@ -278,7 +279,7 @@ __device__ double2 pair_parallel(const int m[3], int a, int sp, int sq, double k
   return tot;
 }
 
-__device__ double2 pair_perp(const int m[3], int ap, int sp, int aq, int sq, double kappa, int rule) {
+__device__ double2 pair_perp(const int m[3], int ap, int sp, int aq, int sq, double2 kappa, int rule) {
   const int c = 3 - ap - aq;
   const int ustar = m[ap] + sp;
   const int vstar = sq - m[aq];
@@ -333,7 +334,7 @@ __device__ double2 pair_perp(const int m[3], int ap, int sp, int aq, int sq, dou
   return tot;
 }
 
-__global__ void k_assemble(double2* __restrict__ G, int L, int M, int N, double kappa, double scale) {
+__global__ void k_assemble(double2* __restrict__ G, int L, int M, int N, double2 kappa, double scale) {
   const int64_t V = int64_t(L) * M * N;
   for (int64_t v = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; v < V; v += int64_t(gridDim.x) * blockDim.x) {
     const int i = int(v % L);
@@ -431,11 +432,11 @@ static void init_assembly_constants() {
   if (dev < 64) done[dev] = true;
 }
 
-void assemble_toeplitz_dev(int L, int M, int N, double dx, double ko, double2* G, hipStream_t s) {
+void assemble_toeplitz_dev(int L, int M, int N, double dx, double ko, double2* G, hipStream_t s, double ko_im) {
   init_assembly_constants();
   const int64_t V = int64_t(L) * M * N;
   const int blocks = int(std::min<int64_t>((V + 127) / 128, 1 << 20));
-  hipLaunchKernelGGL(k_assemble, dim3(blocks), dim3(128), 0, s, G, L, M, N, ko * dx,
+  hipLaunchKernelGGL(k_assemble, dim3(blocks), dim3(128), 0, s, G, L, M, N, make_double2(ko * dx, ko_im * dx),
                      dx * dx * dx / (4.0 * kPi));
   VP_HIP(hipGetLastError());
   VP_HIP(hipStreamSynchronize(s));

@@ -82,6 +82,20 @@ struct VecArgs {
   const int* active;    // per-problem flag (nullptr: all active)
 };
 
+// Per-kernel event timing (prof.cpp)
+enum KernelId { KID_XFWD = 0, KID_PLANE, KID_XINV, KID_PC_XFWD, KID_PC_PLANE, KID_PC_XINV, kNumKernelIds };
+const char* kernel_id_name(int kid);
+void prof_begin(hipStream_t s, int kid, void** token);
+void prof_end(hipStream_t s, void* token);
+void prof_enable(hipStream_t s, bool on);
+void prof_stats(hipStream_t s, int kid, long long* launches, double* ms, bool reset);
+struct ProfScope {
+  hipStream_t s;
+  void* tok;
+  ProfScope(hipStream_t s_, int kid) : s(s_), tok(nullptr) { prof_begin(s, kid, &tok); }
+  ~ProfScope() { prof_end(s, tok); }
+};
+
 // Host-side planning helpers (plan.cpp)
 bool factorize(int n, bool fp64, FftPlan& out, int tws);
 int max_lds_bytes(int device);

@@ -219,16 +219,31 @@ class Preconditioner:
         return cls(h, ctx, ix.size)
 
     @classmethod
-    def level2_assembled(cls, shape, dx, ko, Mc, Mr0, idx, precisions=('fp64', 'fp32'), ctx=None):
+    def level2_assembled(cls, shape, dx, ko, Mc, Mr0, idx, precisions=('fp64', 'fp32'), ctx=None, damping=0.0):
         """2-level preconditioner from a Green tensor assembled on the GPU."""
         ctx = ctx or default_context()
         Lg, Mg, Ng = shape
         ix = as_idx(idx)
         h = ctypes.c_void_p()
         mc = complex(Mc)
-        check(L.lib.vp_prec_create_level2_assembled(ctx.handle, Lg, Mg, Ng, float(dx), float(ko), mc.real,
+        check(L.lib.vp_prec_create_level2_assembled(ctx.handle, Lg, Mg, Ng, float(dx), float(ko), float(damping), mc.real,
                                                     mc.imag, float(Mr0), ptr(ix), ix.size,
                                                     _prec_mask(precisions), ctypes.byref(h)))
+        return cls(h, ctx, ix.size)
+
+    @classmethod
+    def level2_face(cls, shape, dx, ko, face, idx, precisions=('fp64', 'fp32'), ctx=None, damping=0.0):
+        """Face-averaged 2-level preconditioner, exact along the last axis of
+        ``shape`` with contrast profile ``face`` (complex, length shape[2])."""
+        ctx = ctx or default_context()
+        Lg, Mg, Ng = shape
+        f = as_c128(np.asarray(face).reshape(-1))
+        if f.size != Ng:
+            raise ValueError('face profile length must equal the exact-axis extent')
+        ix = as_idx(idx)
+        h = ctypes.c_void_p()
+        check(L.lib.vp_prec_create_level2_face(ctx.handle, Lg, Mg, Ng, float(dx), float(ko), float(damping), ptr(f), ptr(ix),
+                                               ix.size, _prec_mask(precisions), ctypes.byref(h)))
         return cls(h, ctx, ix.size)
 
     def set_embedding(self, n_outer, sub_of_op):
