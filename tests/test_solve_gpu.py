@@ -186,8 +186,10 @@ def test_mixed_precision_ir_reaches_fp64_tolerance(vp_ctx):
     assert abs(true_rel - info['relres']) <= 1e-3 * true_rel + 1e-15
 
 
-def test_batched_ir_alone_equals_in_batch(vp_ctx):
+@pytest.mark.parametrize('variant', ['fused', 'split'])
+def test_batched_ir_alone_equals_in_batch(vp_ctx, monkeypatch, variant):
     from voxphoton_amd.vie import NOperator, Preconditioner, solve_ir, solve_ir_batched
+    monkeypatch.setenv('VP_MATVEC_VARIANT', variant)
     L, M, N = 10, 5, 3
     dx = 20e-9
     ops, precs, rhs = [], [], []

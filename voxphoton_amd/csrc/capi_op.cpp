@@ -1,8 +1,12 @@
 // C ABI: library, contexts, device memory helpers and the N operator.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 #include "voxphoton.h"
 #include "op.hpp"
@@ -208,9 +212,8 @@ static std::vector<double2> gather_material(const vp_op_s* op, const double* A, 
 template <typename T>
 static void init_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const std::vector<double2>& mc,
                       const std::vector<double2>* mr, double gram_t) {
+  (void)fp64;
   P.present = true;
-  build_geometry(P.g, P.plan, *op, fp64, op->ctx->device);
-  upload_twiddles(P, op->ctx->stream);
   const size_t cnt = mc.size();
   std::vector<C<T>> h(cnt);
   for (size_t i = 0; i < cnt; ++i) h[i] = mk<T>(T(mc[i].x), T(mc[i].y));
@@ -231,6 +234,7 @@ template <typename T>
 static void upload_probs(vp_op_s* op, OpPrec<T>& P) {
   ProbPtrs<T> pp{};
   pp.work = P.work.template as<C<T>>();
+  pp.work2 = P.work2.p ? P.work2.template as<C<T>>() : nullptr;
   pp.fN = P.fN.template as<C<T>>();
   pp.pos = op->pos.as<int>();
   pp.mc = P.mc.template as<C<T>>();
@@ -281,9 +285,107 @@ static vp_op_s* op_begin(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp
   return op;
 }
 
-static void op_finish(vp_op_s* op) {
-  if (op->p64.present) upload_probs(op, op->p64);
-  if (op->p32.present) upload_probs(op, op->p32);
+// Matvec variant: VP_MATVEC_VARIANT = fused | split | auto (default).
+// -1 means "time both".
+static int variant_request() {
+  const char* e = std::getenv("VP_MATVEC_VARIANT");
+  if (!e || !*e || !std::strcmp(e, "auto")) return -1;
+  if (!std::strcmp(e, "fused")) return 0;
+  if (!std::strcmp(e, "split")) return 1;
+  throw VpError{ERR_INVALID, "VP_MATVEC_VARIANT must be fused, split or auto"};
+}
+
+// Geometry, twiddles, tensor layout and workspaces of one precision for one
+// matvec variant.
+template <typename T>
+static void configure_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const double2* F, double scale, int variant) {
+  hipStream_t s = op->ctx->stream;
+  build_geometry(P.g, P.plan, *op, fp64, op->ctx->device, variant);
+  upload_twiddles(P, s);
+  layout_fN<T>(P.g, F, P.fN, scale, s);
+  if (variant == 1)
+    P.work2.alloc(size_t(3) * op->Lp * op->Mp * op->N * sizeof(C<T>));
+  else
+    P.work2.release();
+  upload_probs(op, P);
+}
+
+// Mean time of a few applies on zero input, in ms.
+template <typename T>
+static double time_applies(vp_op_s* op, OpPrec<T>& P) {
+  hipStream_t s = op->ctx->stream;
+  VP_HIP(hipMemsetAsync(P.xs.p, 0, P.xs.bytes, s));
+  C<T>* x = P.xs.template as<C<T>>();
+  C<T>* y = P.ys.template as<C<T>>();
+  op_apply_dev<T>(op, 0, x, y);
+  hipEvent_t a, b;
+  VP_HIP(hipEventCreate(&a));
+  VP_HIP(hipEventCreate(&b));
+  const int reps = 3;
+  float ms = 0;
+  hipError_t err = hipEventRecord(a, s);
+  for (int i = 0; i < reps && err == hipSuccess; ++i) op_apply_dev<T>(op, 0, x, y);
+  if (err == hipSuccess) err = hipEventRecord(b, s);
+  if (err == hipSuccess) err = hipEventSynchronize(b);
+  if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  VP_HIP(err);
+  return double(ms) / reps;
+}
+
+// Timed choice between the fused and split matvec, made once per
+// (device, grid, FFT sizes, precision) in a process so that operators of one
+// shape always share a plan (batched solves rely on it).
+struct VariantKey {
+  int device, L, M, N, Lp, Mp, Np, fp64;
+  bool operator<(const VariantKey& o) const {
+    return std::tie(device, L, M, N, Lp, Mp, Np, fp64) < std::tie(o.device, o.L, o.M, o.N, o.Lp, o.Mp, o.Np, o.fp64);
+  }
+};
+static std::mutex g_variant_mu;
+static std::map<VariantKey, int> g_variant;
+
+template <typename T>
+static void finalize_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const double2* F, double scale) {
+  const VariantKey key{op->ctx->device, op->L, op->M, op->N, op->Lp, op->Mp, op->Np, int(fp64)};
+  int v = variant_request();
+  if (v < 0) {
+    std::lock_guard<std::mutex> lk(g_variant_mu);
+    auto it = g_variant.find(key);
+    if (it != g_variant.end()) v = it->second;
+  }
+  if (v >= 0) {
+    configure_prec(op, P, fp64, F, scale, v);
+    return;
+  }
+  double best = 0;
+  int best_v = -1, configured = -1;
+  std::string why;
+  for (int cand : {0, 1}) {
+    try {
+      configure_prec(op, P, fp64, F, scale, cand);
+    } catch (const VpError& e) {
+      if (e.code != ERR_UNSUPPORTED) throw;
+      why = e.msg;
+      configured = -1;
+      continue;
+    }
+    configured = cand;
+    const double t = time_applies(op, P);
+    if (best_v < 0 || t < best) { best = t; best_v = cand; }
+  }
+  VP_REQUIRE(best_v >= 0, ERR_UNSUPPORTED, why);
+  if (configured != best_v) configure_prec(op, P, fp64, F, scale, best_v);
+  std::lock_guard<std::mutex> lk(g_variant_mu);
+  g_variant.emplace(key, best_v);
+}
+
+// Common tail of every vp_op_create*: F is the true-frequency tensor.
+static void op_finish(vp_op_s* op, const double2* F) {
+  const double P = double(op->Lp) * op->Mp * op->Np;
+  if (op->p64.present) finalize_prec(op, op->p64, true, F, 1.0 / P);
+  if (op->p32.present) finalize_prec(op, op->p32, false, F, 1.0 / (P * op->scale32));
   VP_HIP(hipStreamSynchronize(op->ctx->stream));
 }
 
@@ -313,8 +415,7 @@ int vp_op_create(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_
     const size_t bytes = size_t(op->Lp) * op->Mp * op->Np * 6 * sizeof(double2);
     DevBuf F(bytes);
     VP_HIP(hipMemcpy(F.p, fN, bytes, hipMemcpyHostToDevice));
-    layout_fN(op, F.as<double2>(), ctx->stream);
-    op_finish(op);
+    op_finish(op, F.as<double2>());
   } catch (...) {
     delete op;
     throw;
@@ -335,8 +436,7 @@ int vp_op_create_toeplitz(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t L
     VP_HIP(hipMemcpy(dG.p, G, gb, hipMemcpyHostToDevice));
     DevBuf F(size_t(op->Lp) * op->Mp * op->Np * 6 * sizeof(double2));
     toeplitz_to_fN(*op, dG.as<double2>(), F.as<double2>(), ctx->stream);
-    layout_fN(op, F.as<double2>(), ctx->stream);
-    op_finish(op);
+    op_finish(op, F.as<double2>());
   } catch (...) {
     delete op;
     throw;
@@ -357,8 +457,7 @@ int vp_op_create_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t 
     assemble_toeplitz_dev(op->L, op->M, op->N, dx, ko, dG.as<double2>(), ctx->stream);
     DevBuf F(size_t(op->Lp) * op->Mp * op->Np * 6 * sizeof(double2));
     toeplitz_to_fN(*op, dG.as<double2>(), F.as<double2>(), ctx->stream);
-    layout_fN(op, F.as<double2>(), ctx->stream);
-    op_finish(op);
+    op_finish(op, F.as<double2>());
   } catch (...) {
     delete op;
     throw;

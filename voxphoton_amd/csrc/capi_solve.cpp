@@ -67,7 +67,7 @@ static void prec_geometry(vp_prec_s* p, int level) {
   VP_REQUIRE(2 * size_t(g.B) * g.ld * s <= size_t(lds_max), ERR_UNSUPPORTED, "preconditioner x pencil exceeds LDS");
   g.lds1 = 2 * size_t(g.B) * g.ld * s;
   g.tx = 256;
-  g.t2 = 256;
+  g.t2 = 512;
   g.lds2 = size_t(2) * 3 * g.N * g.M * s;
   if (level == 2)
     VP_REQUIRE(g.lds2 <= size_t(lds_max), ERR_UNSUPPORTED, "preconditioner plane exceeds LDS");
@@ -175,7 +175,11 @@ static void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* 
   ps.twz = P0.twz();
   std::vector<ProbPtrs<T>> h(np);
   const size_t wbytes = size_t(3) * o0->Lp * o0->M * o0->N * sizeof(C<T>);
-  if (np > 1) ps.works.alloc(wbytes * np);
+  const size_t w2bytes = P0.plan.variant == 1 ? size_t(3) * o0->Lp * o0->Mp * o0->N * sizeof(C<T>) : 0;
+  if (np > 1) {
+    ps.works.alloc(wbytes * np);
+    ps.works2.alloc(w2bytes * np);
+  }
   for (int p = 0; p < np; ++p) {
     vp_op_s* op = ops[p];
     VP_REQUIRE(op && op->L == o0->L && op->M == o0->M && op->N == o0->N && op->Lp == o0->Lp &&
@@ -183,10 +187,15 @@ static void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* 
                ERR_INVALID, "batched operators must share one geometry");
     OpPrec<T>& P = prec_of<T>(op);
     VP_REQUIRE(P.present, ERR_INVALID, "operator lacks the precision this solve needs");
-    VP_REQUIRE(P.g.y.R == P0.g.y.R && P.g.z.R == P0.g.z.R, ERR_INVALID, "batched operators differ in plan");
+    // The plan is a function of (grid, FFT sizes, precision, variant, device).
+    VP_REQUIRE(P.plan.variant == P0.plan.variant && P.g.y.R == P0.g.y.R && P.g.z.R == P0.g.z.R, ERR_INVALID,
+               "batched operators differ in plan");
     ProbPtrs<T> pp{};
     pp.work = np > 1 ? reinterpret_cast<C<T>*>(static_cast<char*>(ps.works.p) + wbytes * p)
                      : P.work.template as<C<T>>();
+    if (w2bytes)
+      pp.work2 = np > 1 ? reinterpret_cast<C<T>*>(static_cast<char*>(ps.works2.p) + w2bytes * p)
+                        : P.work2.template as<C<T>>();
     pp.fN = P.fN.template as<C<T>>();
     pp.pos = op->pos.as<int>();
     pp.mc = P.mc.template as<C<T>>();

@@ -149,6 +149,43 @@ __device__ __forceinline__ void dft(C<T>* v) {
 }
 
 // ---------------------------------------------------------------------------
+// Division by a runtime-invariant divisor d (1 <= d < 2^31) for numerators
+// n < 2^31: q = (umulhi(n, m) + n) >> s with s = ceil(log2 d) and
+// m = floor(2^32 (2^s - d) / d) + 1 (Granlund & Montgomery, PLDI'94).
+// ---------------------------------------------------------------------------
+struct FastDiv {
+  uint32_t d, m, s;
+  __host__ __device__ FastDiv() : d(1), m(1), s(0) {}
+  __host__ __device__ explicit FastDiv(uint32_t dd) { init(dd); }
+  __host__ __device__ void init(uint32_t dd) {
+    d = dd;
+    s = 0;
+    while ((uint64_t(1) << s) < d) ++s;
+    m = uint32_t(((uint64_t(1) << 32) * ((uint64_t(1) << s) - d)) / d + 1);
+  }
+  __device__ __forceinline__ uint32_t div(uint32_t n) const {
+    return uint32_t((uint64_t(__umulhi(n, m)) + n) >> s);
+  }
+  __device__ __forceinline__ uint32_t mod(uint32_t n) const { return n - div(n) * d; }
+};
+
+// Cheap division for index arithmetic inside kernels: numerators 0 <= n < 2^20.
+// n * rcp(d) in FP32 is within 1/8 of n/d there, so one correction step
+// gives the exact quotient.
+struct IDiv {
+  int d;
+  float inv;
+  __device__ __forceinline__ explicit IDiv(int dd) : d(dd), inv(__builtin_amdgcn_rcpf(float(dd))) {}
+  __device__ __forceinline__ int div(int n) const {
+    int q = int(float(n) * inv);
+    const int r = n - q * d;
+    q += (r >= d) - (r < 0);
+    return q;
+  }
+  __device__ __forceinline__ int mod(int n) const { return n - div(n) * d; }
+};
+
+// ---------------------------------------------------------------------------
 // Plans and LDS layouts
 // ---------------------------------------------------------------------------
 constexpr int kMaxStages = 12;
@@ -190,14 +227,16 @@ __device__ __forceinline__ void stage(const C<T>* src, C<T>* dst, int n, int Ns,
                                       const C<T>* __restrict__ tw, int twmul) {
   const int nb = n / R;
   const int total = nb * nbatch;
+  const IDiv dnb(nb), dns(Ns), dbi(lay.bi);
   for (int b = threadIdx.x; b < total; b += blockDim.x) {
-    const int beta = b / nb;
+    const int beta = dnb.div(b);
     const int j = b - beta * nb;
-    const int off = lay.off(beta);
+    const int hi = dbi.div(beta);
+    const int off = hi * lay.so + (beta - hi * lay.bi) * lay.si;
     C<T> v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = src[off + (j + r * nb) * lay.st];
-    const int jm = j % Ns;
+    const int jm = dns.mod(j);
     if (Ns > 1) {
       // w = exp(-+ 2 pi i jm / (Ns R)) = table[jm * (n / (Ns R)) * tws]
       const int step = jm * twmul;

@@ -140,17 +140,19 @@ __global__ void __launch_bounds__(KT) k_axpy_dots(const C<T>* __restrict__ V, in
   }
 }
 
-// out[p][j] = sum over blocks of partials[p][j][blk] (fixed order)
-__global__ void k_reduce_cols(const double2* __restrict__ partials, int nblk, int ncol, int maxcols,
-                              double2* __restrict__ out, const int* __restrict__ active) {
-  const int p = blockIdx.x;
-  if (active && !active[p]) return;
-  for (int j = threadIdx.x; j < ncol; j += blockDim.x) {
-    const double2* src = partials + (int64_t(p) * maxcols + j) * nblk;
-    double2 s = make_double2(0.0, 0.0);
-    for (int b = 0; b < nblk; ++b) { s.x += src[b].x; s.y += src[b].y; }
-    out[int64_t(p) * maxcols + j] = s;
-  }
+// out[p][j] = sum over blocks of partials[p][j][blk]: one workgroup per
+// (column, problem); thread t sums blocks t, t+KT, ... and a fixed tree
+// combines the threads, so the order never depends on timing.
+__global__ void __launch_bounds__(KT) k_reduce_cols(const double2* __restrict__ partials, int nblk, int ncol,
+                                                   int maxcols, double2* __restrict__ out,
+                                                   const int* __restrict__ active) {
+  const int j = blockIdx.x;
+  const int p = blockIdx.y;
+  if (j >= ncol || (active && !active[p])) return;
+  const double2* src = partials + (int64_t(p) * maxcols + j) * nblk;
+  double2 acc[1] = {make_double2(0.0, 0.0)};
+  for (int b = threadIdx.x; b < nblk; b += KT) { acc[0].x += src[b].x; acc[0].y += src[b].y; }
+  block_reduce_store<1>(acc, out + int64_t(p) * maxcols + j, 1, 1);
 }
 
 struct ArnoldiState {
@@ -552,15 +554,15 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       }
       hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, w, vprob, n,
                          part.as<double2>(), maxcols, dia);
-      hipLaunchKernelGGL(k_reduce_cols, dim3(np), dim3(64), 0, s, part.as<double2>(), nblk, k + 1, maxcols,
+      hipLaunchKernelGGL(k_reduce_cols, dim3(k + 1, np), dim3(KT), 0, s, part.as<double2>(), nblk, k + 1, maxcols,
                          h1.as<double2>(), dia);
       hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, h1.as<double2>(), w,
                          vprob, n, part.as<double2>(), maxcols, 0, dia);
-      hipLaunchKernelGGL(k_reduce_cols, dim3(np), dim3(64), 0, s, part.as<double2>(), nblk, k + 1, maxcols,
+      hipLaunchKernelGGL(k_reduce_cols, dim3(k + 1, np), dim3(KT), 0, s, part.as<double2>(), nblk, k + 1, maxcols,
                          h2.as<double2>(), dia);
       hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, h2.as<double2>(), w,
                          vprob, n, part.as<double2>(), maxcols, 1, dia);
-      hipLaunchKernelGGL(k_reduce_cols, dim3(np), dim3(64), 0, s, part.as<double2>(), nblk, 1, maxcols,
+      hipLaunchKernelGGL(k_reduce_cols, dim3(1, np), dim3(KT), 0, s, part.as<double2>(), nblk, 1, maxcols,
                          nr.as<double2>(), dia);
       hipLaunchKernelGGL(k_arnoldi, dim3((np + 63) / 64), dim3(64), 0, s, st, k, dia, np);
       hipLaunchKernelGGL(k_scale<T>, dim3(grid1(n), np), dim3(256), 0, s, w, vprob, n, invn.as<double>(), 0, dia);

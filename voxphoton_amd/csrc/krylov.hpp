@@ -18,7 +18,7 @@ struct ProblemSet {
   Geometry g{};
   MatvecPlan plan{};
   DevBuf probs;                   // ProbPtrs<T>[nprob]
-  DevBuf works;                   // per-problem spectrum workspaces (may be empty: ops' own)
+  DevBuf works, works2;           // per-problem spectrum workspaces (may be empty: ops' own)
   const C<T>* twx = nullptr;
   const C<T>* twy = nullptr;
   const C<T>* twz = nullptr;

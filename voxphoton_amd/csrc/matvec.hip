@@ -118,20 +118,27 @@ __global__ void __launch_bounds__((acc_bound<T, EPT>())) k_plane(const ProbPtrs<
   const LdsLayout lay_z{Ms, MsNs, 1, Ms};
   C<T>* const W0 = W;
   C<T>* const W1 = W + 3 * MsNs;
+  // twiddle tables live in LDS behind the two work buffers
+  C<T>* const ty = W + 6 * MsNs;
+  C<T>* const tz = ty + Mp;
+  for (int e = threadIdx.x; e < Mp; e += blockDim.x) ty[e] = twy[e];
+  for (int e = threadIdx.x; e < Np; e += blockDim.x) tz[e] = twz[e];
+  const IDiv dMsNs(MsNs), dMs(Ms), dMN(MN), dM(M), dNs(Ns), dMp(Mp), dNp(Np);
+  __syncthreads();
   for (int py = 0; py < Ry; ++py) {
     for (int pz = 0; pz < Rz; ++pz) {
       // fold + twiddle: W[c][z][y] = sum Z[c][y + i Ms][z + j Ns] w_Mp^{m py} w_Np^{n pz}
       for (int e = threadIdx.x; e < 3 * MsNs; e += blockDim.x) {
-        const int c = e / MsNs;
+        const int c = dMsNs.div(e);
         const int r = e - c * MsNs;
-        const int zz = r / Ms;
+        const int zz = dMs.div(r);
         const int yy = r - zz * Ms;
         C<T> v = czero<T>();
         for (int n = zz; n < N; n += Ns) {
-          const C<T> wz = twz[(n * pz) % Np];
+          const C<T> wz = tz[dNp.mod(n * pz)];
           for (int m = yy; m < M; m += Ms) {
             C<T> t = Zp[(c * N + n) * M + m];
-            if (py) t = cmul(t, twy[(m * py) % Mp]);
+            if (py) t = cmul(t, ty[dMp.mod(m * py)]);
             if (pz) t = cmul(t, wz);
             v = cadd(v, t);
           }
@@ -139,8 +146,8 @@ __global__ void __launch_bounds__((acc_bound<T, EPT>())) k_plane(const ProbPtrs<
         W0[e] = v;
       }
       __syncthreads();
-      C<T>* R = lds_fft<T, false>(W0, W1, g.y.plan, 3 * Ns, lay_y, twy);
-      R = lds_fft<T, false>(R, R == W0 ? W1 : W0, g.z.plan, 3 * Ms, lay_z, twz);
+      C<T>* R = lds_fft<T, false>(W0, W1, g.y.plan, 3 * Ns, lay_y, ty);
+      R = lds_fft<T, false>(R, R == W0 ? W1 : W0, g.z.plan, 3 * Ms, lay_z, tz);
       // 3x3 symmetric tensor multiply, components xx xy xz yy yz zz
       const C<T>* F = P.fN + ((int64_t(kxi) * Ry + py) * Rz + pz) * int64_t(MsNs) * 6;
       for (int e = threadIdx.x; e < MsNs; e += blockDim.x) {
@@ -152,20 +159,20 @@ __global__ void __launch_bounds__((acc_bound<T, EPT>())) k_plane(const ProbPtrs<
         R[2 * MsNs + e] = cadd(cadd(cmul(f2, a), cmul(f4, b)), cmul(f5, cc));
       }
       __syncthreads();
-      R = lds_fft<T, true>(R, R == W0 ? W1 : W0, g.z.plan, 3 * Ms, lay_z, twz);
-      R = lds_fft<T, true>(R, R == W0 ? W1 : W0, g.y.plan, 3 * Ns, lay_y, twy);
+      R = lds_fft<T, true>(R, R == W0 ? W1 : W0, g.z.plan, 3 * Ms, lay_z, tz);
+      R = lds_fft<T, true>(R, R == W0 ? W1 : W0, g.y.plan, 3 * Ns, lay_y, ty);
       // unfold: out[c][n][m] += W[c][n % Ns][m % Ms] conj(w_Mp^{m py} w_Np^{n pz})
 #pragma unroll
       for (int k = 0; k < EPT; ++k) {
         const int o = threadIdx.x + k * blockDim.x;
         if (o < 3 * MN) {
-          const int c = o / MN;
+          const int c = dMN.div(o);
           const int r = o - c * MN;
-          const int n = r / M;
+          const int n = dM.div(r);
           const int m = r - n * M;
-          C<T> t = R[(c * Ns + n % Ns) * Ms + m % Ms];
-          if (py) t = cmulc(t, twy[(m * py) % Mp]);
-          if (pz) t = cmulc(t, twz[(n * pz) % Np]);
+          C<T> t = R[(c * Ns + dNs.mod(n)) * Ms + dMs.mod(m)];
+          if (py) t = cmulc(t, ty[dMp.mod(m * py)]);
+          if (pz) t = cmulc(t, tz[dNp.mod(n * pz)]);
           acc[k] = cadd(acc[k], t);
         }
       }
@@ -176,6 +183,111 @@ __global__ void __launch_bounds__((acc_bound<T, EPT>())) k_plane(const ProbPtrs<
   for (int k = 0; k < EPT; ++k) {
     const int o = threadIdx.x + k * blockDim.x;
     if (o < 3 * MN) Zp[o] = acc[k];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Split variant of the plane pass (y, z, y^-1 as separate sweeps).  The x
+// spectra X[kx][c][n][m] are y-transformed into Y[kx][c][n][ky] (zero padded
+// from M to Mp), the z pass works on tiles of Tz consecutive ky columns of one
+// kx plane (zero pad N -> Np, DFT, tensor multiply, inverse, crop), and the
+// inverse y sweep crops back to M into X.
+// ----------------------------------------------------------------------------
+template <typename T, bool INV>
+__global__ void __launch_bounds__(256) k_ypass(const ProbPtrs<T>* __restrict__ probs, const int* __restrict__ active,
+                                               Geometry g, MatvecPlan mp, const C<T>* __restrict__ twy) {
+  if (active && !active[blockIdx.y]) return;
+  const ProbPtrs<T> P = probs[blockIdx.y];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
+  const int M = g.M, Mp = g.y.P, B = mp.By, ld = mp.ldy;
+  C<T>* W1 = W0 + B * ld;
+  C<T>* tw = W1 + B * ld;
+  for (int e = threadIdx.x; e < Mp; e += blockDim.x) tw[e] = twy[e];
+  const int64_t npen = int64_t(g.x.P) * 3 * g.N;
+  const int64_t p0 = int64_t(xcd_remap(blockIdx.x, gridDim.x)) * B;
+  const int nb = int(min<int64_t>(B, npen - p0));
+  const IDiv dMp(Mp), dM(M);
+  if (!INV) {
+    for (int e = threadIdx.x; e < B * Mp; e += blockDim.x) {
+      const int bb = dMp.div(e), k = e - bb * Mp;
+      W0[bb * ld + k] = (bb < nb && k < M) ? P.work[(p0 + bb) * M + k] : czero<T>();
+    }
+  } else {
+    for (int e = threadIdx.x; e < B * Mp; e += blockDim.x) {
+      const int bb = dMp.div(e), k = e - bb * Mp;
+      W0[bb * ld + k] = (bb < nb) ? P.work2[(p0 + bb) * Mp + k] : czero<T>();
+    }
+  }
+  __syncthreads();
+  const C<T>* R = lds_fft<T, INV>(W0, W1, g.y.plan, B, LdsLayout{1, ld, 0, 1}, tw);
+  if (!INV) {
+    for (int e = threadIdx.x; e < nb * Mp; e += blockDim.x) {
+      const int bb = dMp.div(e), k = e - bb * Mp;
+      P.work2[(p0 + bb) * Mp + k] = R[bb * ld + k];
+    }
+  } else {
+    for (int e = threadIdx.x; e < nb * M; e += blockDim.x) {
+      const int bb = dM.div(e), k = e - bb * M;
+      P.work[(p0 + bb) * M + k] = R[bb * ld + k];
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_zpass(const ProbPtrs<T>* __restrict__ probs, const int* __restrict__ active,
+                                               Geometry g, MatvecPlan mp, const C<T>* __restrict__ twz) {
+  if (active && !active[blockIdx.y]) return;
+  const ProbPtrs<T> P = probs[blockIdx.y];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
+  const int N = g.N, Np = g.z.P, Mp = g.y.P, Tz = mp.Tz;
+  const int nt = (Mp + Tz - 1) / Tz;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int kxi = blk / nt;
+  const int ky0 = (blk - kxi * nt) * Tz;
+  const int E = 3 * Tz * Np;
+  C<T>* W1 = W0 + E;
+  C<T>* tw = W1 + E;
+  for (int e = threadIdx.x; e < Np; e += blockDim.x) tw[e] = twz[e];
+  const IDiv dTz(Tz), dN(N);
+  const int64_t base = int64_t(kxi) * 3 * N * Mp;   // Y[kx][c][n][ky]
+  // load: W0[(c*Tz + t)*Np + n] = Y[kx][c][n][ky0 + t], zero for n >= N
+  for (int e = threadIdx.x; e < 3 * Tz * N; e += blockDim.x) {
+    const int cn = dTz.div(e), t = e - cn * Tz;
+    const int c = dN.div(cn), n = cn - c * N;
+    const int ky = ky0 + t;
+    W0[(c * Tz + t) * Np + n] = ky < Mp ? P.work2[base + (int64_t(c) * N + n) * Mp + ky] : czero<T>();
+  }
+  for (int e = threadIdx.x; e < 3 * Tz * (Np - N); e += blockDim.x) {
+    const int q = e / (Np - N), n = N + (e - q * (Np - N));
+    W0[q * Np + n] = czero<T>();
+  }
+  __syncthreads();
+  const LdsLayout lay{1, Np, 0, 1};
+  C<T>* R = lds_fft<T, false>(W0, W1, g.z.plan, 3 * Tz, lay, tw);
+  // tensor multiply; fN class layout with Ry = Rz = 1 is [kx][kz][ky][6]
+  const C<T>* F = P.fN + int64_t(kxi) * Np * Mp * 6;
+  for (int e = threadIdx.x; e < Tz * Np; e += blockDim.x) {
+    const int kz = dTz.div(e), t = e - kz * Tz;
+    const int ky = ky0 + t;
+    if (ky < Mp) {
+      const C<T>* f = F + (int64_t(kz) * Mp + ky) * 6;
+      const C<T> f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4], f5 = f[5];
+      const int i0 = (0 * Tz + t) * Np + kz, i1 = (1 * Tz + t) * Np + kz, i2 = (2 * Tz + t) * Np + kz;
+      const C<T> a = R[i0], b = R[i1], cc = R[i2];
+      R[i0] = cadd(cadd(cmul(f0, a), cmul(f1, b)), cmul(f2, cc));
+      R[i1] = cadd(cadd(cmul(f1, a), cmul(f3, b)), cmul(f4, cc));
+      R[i2] = cadd(cadd(cmul(f2, a), cmul(f4, b)), cmul(f5, cc));
+    }
+  }
+  __syncthreads();
+  const C<T>* Q = lds_fft<T, true>(R, R == W0 ? W1 : W0, g.z.plan, 3 * Tz, lay, tw);
+  for (int e = threadIdx.x; e < 3 * Tz * N; e += blockDim.x) {
+    const int cn = dTz.div(e), t = e - cn * Tz;
+    const int c = dN.div(cn), n = cn - c * N;
+    const int ky = ky0 + t;
+    if (ky < Mp) P.work2[base + (int64_t(c) * N + n) * Mp + ky] = Q[(c * Tz + t) * Np + n];
   }
 }
 
@@ -288,7 +400,22 @@ void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d
     hipLaunchKernelGGL(k_xfwd<T>, gx, dim3(mp.tx), mp.lds1, s, d_probs, va, g, mp, transp, twx);
     VP_HIP(hipGetLastError());
   }
-  {
+  if (mp.variant == 1) {
+    const int64_t npen = int64_t(g.x.P) * 3 * g.N;
+    const dim3 gy(unsigned((npen + mp.By - 1) / mp.By), nprob);
+    const void* fy0 = reinterpret_cast<const void*>(&k_ypass<T, false>);
+    const void* fy1 = reinterpret_cast<const void*>(&k_ypass<T, true>);
+    const void* fz = reinterpret_cast<const void*>(&k_zpass<T>);
+    (void)hipFuncSetAttribute(fy0, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds_y));
+    (void)hipFuncSetAttribute(fy1, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds_y));
+    (void)hipFuncSetAttribute(fz, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds_z));
+    ProfScope ps(s, KID_PLANE);
+    hipLaunchKernelGGL((k_ypass<T, false>), gy, dim3(mp.ty), mp.lds_y, s, d_probs, va.active, g, mp, twy);
+    const dim3 gz(unsigned(g.x.P * ((g.y.P + mp.Tz - 1) / mp.Tz)), nprob);
+    hipLaunchKernelGGL(k_zpass<T>, gz, dim3(mp.tz), mp.lds_z, s, d_probs, va.active, g, mp, twz);
+    hipLaunchKernelGGL((k_ypass<T, true>), gy, dim3(mp.ty), mp.lds_y, s, d_probs, va.active, g, mp, twy);
+    VP_HIP(hipGetLastError());
+  } else {
     dim3 gp(g.x.P, nprob);
 #define VP_K2(E)                                                                            \
   do {                                                                                      \

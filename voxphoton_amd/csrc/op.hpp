@@ -43,6 +43,7 @@ struct OpPrec {
   DevBuf mc, mr;    // per unknown voxel (structured) or unknown (generic)
   DevBuf tw;        // twiddles: Lp | Mp | Np entries
   DevBuf work;      // spectrum workspace 3 * Lp * M * N
+  DevBuf work2;     // split variant: y-transformed workspace 3 * Lp * Mp * N
   DevBuf xs, ys;    // vector staging for host-memory calls
   DevBuf probs;     // one ProbPtrs<T>
   T gram = T(1);
@@ -77,11 +78,11 @@ template <> inline OpPrec<float>& prec_of<float>(vp_op_s* op) { return op->p32; 
 // Build helpers (setup.hip)
 // Chooses residue classes and launch shapes for one precision; throws
 // ERR_UNSUPPORTED when no shape fits.
-void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, int device);
+void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, int device, int variant);
 template <typename T> void upload_twiddles(OpPrec<T>& P, hipStream_t s);
-// Writes the class-layout tensor for both precisions from a true-frequency
-// device tensor F (Lp x Mp x Np x 6 complex128, column-major).
-void layout_fN(vp_op_s* op, const double2* F, hipStream_t s);
+// Writes the class-layout tensor of geometry g from a true-frequency device
+// tensor F (Lp x Mp x Np x 6 complex128, column-major), times scale.
+template <typename T> void layout_fN(const Geometry& g, const double2* F, DevBuf& out, double scale, hipStream_t s);
 // True-frequency tensor from a device Toeplitz tensor (L x M x N x 6).
 void toeplitz_to_fN(const vp_op_s& op, const double2* G, double2* F, hipStream_t s);
 // Device assembly of the Toeplitz tensor.

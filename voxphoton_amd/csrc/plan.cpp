@@ -1,5 +1,6 @@
 // Host-side planning: radix factorisation and launch shapes for the matvec.
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
 #include "vp_internal.hpp"
 
@@ -67,19 +68,46 @@ static bool pick_threads(int points, bool fp64, int& t_out, int& e_out) {
   return false;
 }
 
-bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out, std::string& why) {
+bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out, std::string& why, int variant) {
   const size_t s = fp64 ? 16 : 8;
   const int lds_max = max_lds_bytes(device);
+  out.variant = variant;
+  if (variant == 1) {
+    // ---- split passes: y sweeps over By pencils, z sweep over Tz ky columns
+    const int Mp = g.y.P, Np = g.z.P;
+    const size_t budget = 80 * 1024;   // two workgroups per CU
+    out.By = 0;
+    for (int B : {16, 8, 4, 2, 1}) {
+      const int ld = Mp + (B > 1 ? 1 : 0);
+      const size_t lds = (2 * size_t(B) * ld + Mp) * s;
+      if (lds <= budget || B == 1) { out.By = B; out.ldy = ld; out.lds_y = lds; break; }
+    }
+    out.Tz = 0;
+    for (int Tz : {32, 16, 8, 4, 2, 1}) {
+      if (Tz > Mp) continue;
+      const size_t lds = (2 * size_t(3) * Tz * Np + Np) * s;
+      if (lds <= budget || Tz == 1) { out.Tz = Tz; out.lds_z = lds; break; }
+    }
+    if (out.lds_y > size_t(lds_max) || out.lds_z > size_t(lds_max)) { why = "split passes exceed LDS"; return false; }
+    out.ty = 256;
+    out.tz = 256;
+    out.t2 = 0;
+    out.ept2 = 1;
+    out.lds2 = 0;
+  }
   // ---- K2 (planes): classes fixed by the caller; pick threads/accumulators.
-  {
+  if (variant == 0) {
     const int Ms = g.y.S, Ns = g.z.S;
     const long long E = 3LL * Ms * Ns;
-    if (2 * E * (long long)s > lds_max) { why = "plane class buffers exceed LDS"; return false; }
+    const long long lds = (2 * E + g.y.P + g.z.P) * (long long)s;   // work buffers + twiddles
+    long long cap = lds_max;
+    if (const char* e = std::getenv("VP_K2_LDS_CAP")) cap = std::min<long long>(cap, std::atoll(e));
+    if (lds > cap) { why = "plane class buffers exceed LDS"; return false; }
     int t = 0, e = 0;
     if (!pick_threads(3 * g.M * g.N, fp64, t, e)) { why = "plane too large for register accumulators"; return false; }
     out.t2 = t;
     out.ept2 = e;
-    out.lds2 = 2 * size_t(E) * s;
+    out.lds2 = size_t(lds);
   }
   // ---- K1/K3 (x pencils): largest pencil batch that fits LDS and registers.
   {

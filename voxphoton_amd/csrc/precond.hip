@@ -84,21 +84,21 @@ __global__ void __launch_bounds__(512) k_pc_plane(PcArgs<T> a, PcGeom g) {
   const LdsLayout lay{1, M, 0, 1};   // transform q = (c, n): elements q*M + m
   C<T>* R = lds_fft<T, false>(W0, W1, g.py, n3, lay, a.twy);
   C<T>* O = (R == W0) ? W1 : W0;
-  // one wavefront per ky; lanes over block rows
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int nw = blockDim.x >> 6;
+  // one thread per (block row, ky): consecutive threads read consecutive rows
+  // of a column-major block, so every load of the block stream is coalesced.
   const C<T>* Bk = blocks + int64_t(kx) * M * n3 * n3;
-  for (int ky = wave; ky < M; ky += nw) {
-    const C<T>* Bb = Bk + int64_t(ky) * n3 * n3;
-    for (int r0 = 0; r0 < n3; r0 += 64) {
-      const int r = r0 + lane;
-      C<T> acc = czero<T>();
-      if (r < n3) {
-        for (int q = 0; q < n3; ++q) acc = cadd(acc, cmul(Bb[int64_t(q) * n3 + r], R[q * M + ky]));
-        O[r * M + ky] = acc;
-      }
+  for (int e = threadIdx.x; e < plane; e += blockDim.x) {
+    const int ky = e / n3;
+    const int r = e - ky * n3;
+    const C<T>* Bb = Bk + int64_t(ky) * n3 * n3 + r;
+    C<T> a0 = czero<T>(), a1 = czero<T>();
+    int q = 0;
+    for (; q + 1 < n3; q += 2) {
+      a0 = cadd(a0, cmul(Bb[int64_t(q) * n3], R[q * M + ky]));
+      a1 = cadd(a1, cmul(Bb[int64_t(q + 1) * n3], R[(q + 1) * M + ky]));
     }
+    if (q < n3) a0 = cadd(a0, cmul(Bb[int64_t(q) * n3], R[q * M + ky]));
+    O[r * M + ky] = cadd(a0, a1);
   }
   __syncthreads();
   const C<T>* F = lds_fft<T, true>(O, R, g.py, n3, lay, a.twy);
@@ -181,23 +181,27 @@ void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) 
   }
   const int nmb = (g.M + g.B - 1) / g.B;
   const dim3 gx(nmb * g.N * 3, nprob);
+  // the geometry sizes LDS for complex128; complex64 launches need half
+  const size_t lds1 = g.lds1 * sizeof(C<T>) / 16, lds2 = g.lds2 * sizeof(C<T>) / 16;
   const void* f1 = reinterpret_cast<const void*>(&k_pc_xfwd<T>);
-  (void)hipFuncSetAttribute(f1, hipFuncAttributeMaxDynamicSharedMemorySize, int(g.lds1));
+  const void* f3 = reinterpret_cast<const void*>(&k_pc_xinv<T>);
+  (void)hipFuncSetAttribute(f1, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds1));
+  (void)hipFuncSetAttribute(f3, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds1));
   {
     ProfScope ps(s, KID_PC_XFWD);
-    hipLaunchKernelGGL(k_pc_xfwd<T>, gx, dim3(g.tx), g.lds1, s, a, g);
+    hipLaunchKernelGGL(k_pc_xfwd<T>, gx, dim3(g.tx), lds1, s, a, g);
   }
   VP_HIP(hipGetLastError());
   if (g.level == 2) {
     const void* f2 = reinterpret_cast<const void*>(&k_pc_plane<T>);
-    (void)hipFuncSetAttribute(f2, hipFuncAttributeMaxDynamicSharedMemorySize, int(g.lds2));
+    (void)hipFuncSetAttribute(f2, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2));
     {
       ProfScope ps(s, KID_PC_PLANE);
-      hipLaunchKernelGGL(k_pc_plane<T>, dim3(g.L, nprob), dim3(g.t2), g.lds2, s, a, g);
+      hipLaunchKernelGGL(k_pc_plane<T>, dim3(g.L, nprob), dim3(g.t2), lds2, s, a, g);
     }
     VP_HIP(hipGetLastError());
     ProfScope ps(s, KID_PC_XINV);
-    hipLaunchKernelGGL(k_pc_xinv<T>, gx, dim3(g.tx), g.lds1, s, a, g);
+    hipLaunchKernelGGL(k_pc_xinv<T>, gx, dim3(g.tx), lds1, s, a, g);
   } else {
     const int64_t n3 = 3LL * g.M * g.N;
     const int zs = int(std::min<int64_t>((n3 + 255) / 256, 64));
@@ -205,10 +209,8 @@ void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) 
     VP_HIP(hipGetLastError());
     PcArgs<T> b = a;
     b.work = a.work2;
-    hipLaunchKernelGGL(k_pc_xinv<T>, gx, dim3(g.tx), g.lds1, s, b, g);
+    hipLaunchKernelGGL(k_pc_xinv<T>, gx, dim3(g.tx), lds1, s, b, g);
   }
-  const void* f3 = reinterpret_cast<const void*>(&k_pc_xinv<T>);
-  (void)f3;
   VP_HIP(hipGetLastError());
 }
 template void launch_prec<float>(const PcGeom&, const PcArgs<float>&, int, hipStream_t);

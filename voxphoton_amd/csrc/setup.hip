@@ -18,7 +18,7 @@ static std::vector<int> divisors(int n) {
   return d;
 }
 
-void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, int device) {
+void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, int device, int variant) {
   g = Geometry{};
   g.L = op.L; g.M = op.M; g.N = op.N;
   g.V = op.V;
@@ -34,6 +34,20 @@ void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, i
              "x FFT length " + std::to_string(g.x.S) + " has a prime factor above the supported radices");
   struct Cand { int ry, rz; };
   std::vector<Cand> cands;
+  if (variant == 1) {
+    // split passes: full padded lengths, no residue classes
+    Geometry t = g;
+    t.y.R = 1; t.y.S = op.Mp;
+    t.z.R = 1; t.z.S = op.Np;
+    std::string why2;
+    VP_REQUIRE(factorize(t.y.S, fp64, t.y.plan, 1) && factorize(t.z.S, fp64, t.z.plan, 1), ERR_UNSUPPORTED,
+               "padded y/z FFT lengths have a prime factor above the supported radices");
+    MatvecPlan p{};
+    VP_REQUIRE(make_matvec_plan(t, fp64, device, p, why2, 1), ERR_UNSUPPORTED, "matvec plan: " + why2);
+    g = t;
+    mp = p;
+    return;
+  }
   for (int ry : divisors(op.Mp))
     for (int rz : divisors(op.Np)) cands.push_back({ry, rz});
   std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) {
@@ -48,7 +62,7 @@ void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, i
     if (!factorize(t.y.S, fp64, t.y.plan, t.y.R)) continue;
     if (!factorize(t.z.S, fp64, t.z.plan, t.z.R)) continue;
     MatvecPlan p{};
-    if (!make_matvec_plan(t, fp64, device, p, why)) continue;
+    if (!make_matvec_plan(t, fp64, device, p, why, 0)) continue;
     g = t;
     mp = p;
     return;
@@ -107,23 +121,16 @@ __global__ void k_layout_fN(const double2* __restrict__ F, C<T>* __restrict__ ou
   }
 }
 
-void layout_fN(vp_op_s* op, const double2* F, hipStream_t s) {
-  const double P = double(op->Lp) * op->Mp * op->Np;
-  const int64_t total = int64_t(op->Lp) * op->Mp * op->Np * 6;
+template <typename T>
+void layout_fN(const Geometry& g, const double2* F, DevBuf& out, double scale, hipStream_t s) {
+  const int64_t total = int64_t(g.x.P) * g.y.P * g.z.P * 6;
   const int blocks = int(std::min<int64_t>((total + 255) / 256, 65536));
-  if (op->p64.present) {
-    op->p64.fN.alloc(total * sizeof(double2));
-    hipLaunchKernelGGL(k_layout_fN<double>, dim3(blocks), dim3(256), 0, s, F, op->p64.fN.as<double2>(), op->p64.g,
-                       1.0 / P);
-    VP_HIP(hipGetLastError());
-  }
-  if (op->p32.present) {
-    op->p32.fN.alloc(total * sizeof(float2));
-    hipLaunchKernelGGL(k_layout_fN<float>, dim3(blocks), dim3(256), 0, s, F, op->p32.fN.as<float2>(), op->p32.g,
-                       1.0 / (P * op->scale32));
-    VP_HIP(hipGetLastError());
-  }
+  if (out.bytes != size_t(total) * sizeof(C<T>)) out.alloc(size_t(total) * sizeof(C<T>));
+  hipLaunchKernelGGL(k_layout_fN<T>, dim3(blocks), dim3(256), 0, s, F, out.as<C<T>>(), g, scale);
+  VP_HIP(hipGetLastError());
 }
+template void layout_fN<double>(const Geometry&, const double2*, DevBuf&, double, hipStream_t);
+template void layout_fN<float>(const Geometry&, const double2*, DevBuf&, double, hipStream_t);
 
 // ----------------------------------------------------------------------------
 // Toeplitz tensor -> circulant spectrum.  The circulant embedding of

@@ -41,6 +41,11 @@ struct Dim {
 };
 
 struct MatvecPlan {
+  // 0: fused plane kernel over residue classes; 1: separate y / z / y^-1 passes
+  int variant;
+  // split variant: y passes (By pencils of Mp per workgroup), z pass (Tz ky columns)
+  int By, ldy, ty, Tz, tz;
+  size_t lds_y, lds_z;
   // K1/K3 (x pencils)
   int B;         // pencils per workgroup
   int ldsx;      // row stride of a pencil in LDS (elements)
@@ -66,6 +71,7 @@ struct Geometry {
 template <typename T>
 struct ProbPtrs {
   C<T>* work;        // 3 * Lp * M * N spectrum workspace
+  C<T>* work2;       // 3 * Lp * Mp * N (split variant only)
   const C<T>* fN;    // tensor in class layout, 6 * Lp * Mp * Np, prescaled
   const int* pos;    // voxel -> unknown map (V or 3V entries, -1 = not an unknown)
   const C<T>* mc;    // contrast per unknown voxel (nS) / per unknown (generic n)
@@ -104,6 +110,6 @@ template <typename T>
 void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d_probs, VecArgs<T> va,
                    int nprob, int transp, const C<T>* twx, const C<T>* twy, const C<T>* twz, hipStream_t s);
 
-bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out, std::string& why);
+bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out, std::string& why, int variant);
 
 }  // namespace vp
