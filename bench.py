@@ -2,7 +2,7 @@
 """Benchmark of the MI355X FFT-VIE solve path (one GPU).
 
 A step is one mixed-precision solve (FP32 matvec + 2-level circulant
-preconditioner + GMRES(restart) inside FP64 iterative refinement) of the
+right preconditioner + GMRES(restart) inside FP64 iterative refinement) of the
 workload to a true FP64 relative residual <= tol, with the operator,
 preconditioner and right-hand side resident in HBM.  The JSON line also
 carries the plain FP64 GPU solve rate, FP64/FP32 matvec times with GB/s
@@ -33,14 +33,14 @@ WORKLOADS = {
         desc='Bragg grating, Si 500 nm +/-50 nm corrugation, 220 nm, 320 nm x 100 periods, quadratic '
              'absorbers (Im eps 0->1 over 2 um) at both ends, SiO2, lambda 1550 nm, 10 nm voxels, '
              'grid 3600x100x44 (FFT 7200x200x88), y-polarised Gaussian (w0 300 nm)',
-        tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=4, max_outer=12, damping=0.2),
+        tol=1e-6, restart=100, inner_tol=1e-3, inner_maxit=6, max_outer=12, prec='face', axis=1, damping=0.0),
     'straight_wg_20nm': dict(
         desc='straight Si waveguide 500x220 nm in SiO2, lambda 1550 nm, 20 nm voxels, grid 200x48x24 '
              '(FFT 400x96x48), y-polarised Gaussian (w0 300 nm)',
-        tol=1e-6, restart=50, inner_tol=1e-2, inner_maxit=2, max_outer=30, damping=0.0),
+        tol=1e-6, restart=50, inner_tol=1e-2, inner_maxit=2, max_outer=30, prec='scalar', damping=0.0),
     'bragg_20nm': dict(
         desc='Bragg grating (100 periods) at 20 nm voxels, grid 1800x50x22 (FFT 3600x100x44), lambda 1550 nm',
-        tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=4, max_outer=12, damping=0.2),
+        tol=1e-6, restart=100, inner_tol=1e-3, inner_maxit=6, max_outer=12, prec='face', axis=1, damping=0.0),
 }
 
 
@@ -55,17 +55,28 @@ def mv_model_bytes(prob, s):
 
 
 def kernel_model_bytes(prob, kernel, s):
-    """Algorithmic bytes per launch of one matvec kernel (same terms as the model)."""
+    """Algorithmic bytes per launch of one matvec kernel.
+
+    Q = Lp*M*N x-spectra per component, Y = Lp*Mp*N y-spectra per component
+    (split variant's intermediate), P = Lp*Mp*Np tensor entries per component.
+    """
     L, M, N = prob.shape
     P = 8 * L * M * N
     Q = 2 * L * M * N
+    Y = 4 * L * M * N
     nS = int(prob.mask.sum())
     n = 3 * nS
-    if kernel == 'plane':     # fN once, half-spectra read and written
+    if kernel == 'plane':     # fused: fN once, x-spectra read and written
         return s * (6 * P + 6 * Q)
-    if kernel == 'xfwd':      # J read, half-spectra written, idx read
+    if kernel == 'yfwd':      # x-spectra read, y-spectra written
+        return s * (3 * Q + 3 * Y)
+    if kernel == 'zpass':     # y-spectra read and written, fN once
+        return s * (6 * P + 6 * Y)
+    if kernel == 'yinv':      # y-spectra read, x-spectra written
+        return s * (3 * Y + 3 * Q)
+    if kernel == 'xfwd':      # J read, x-spectra written, voxel map read
         return s * (n + 3 * Q) + 4 * nS
-    if kernel == 'xinv':      # half-spectra read, J read, Mc read, output written, idx read
+    if kernel == 'xinv':      # x-spectra read, J and Mc read, output written, voxel map read
         return s * (3 * Q + 2 * n + nS) + 4 * nS
     return None
 
@@ -100,15 +111,13 @@ def main():
     t_op = time.perf_counter() - t0
     log(f'operator assembled + embedded in {t_op:.2f} s')
     prec = None
+    pdesc = None
     t_pc = 0.0
     if not args.no_prec:
         t0 = time.perf_counter()
-        bshape, bidx, sub = prob.box_problem()
-        prec = vie.Preconditioner.level2_assembled(bshape, prob.dx, prob.ko, prob.core_contrast(), 1.0, bidx,
-                                                   damping=wl['damping'])
-        prec.set_embedding(n, sub)
+        prec, pdesc = make_preconditioner(prob, wl, vie, configs)
         t_pc = time.perf_counter() - t0
-        log(f'2-level preconditioner on box {bshape} (damping {wl["damping"]}) built in {t_pc:.2f} s')
+        log(f'{pdesc} built in {t_pc:.2f} s')
 
     b = prob.gaussian_rhs('y')
     db = vie.DeviceArray(ctx, b.nbytes)
@@ -118,7 +127,7 @@ def main():
 
     def solve_mixed():
         info = L.GmresInfo()
-        L.check(L.lib.vp_solve_ir(op.handle, hp, L.VP_PREC_LEFT, db.ptr, dx.ptr, L.VP_MEM_DEVICE, wl['tol'],
+        L.check(L.lib.vp_solve_ir(op.handle, hp, L.VP_PREC_RIGHT, db.ptr, dx.ptr, L.VP_MEM_DEVICE, wl['tol'],
                                   wl['restart'], wl['inner_tol'], wl['inner_maxit'], wl['max_outer'],
                                   ctypes.byref(info)))
         return info
@@ -127,7 +136,7 @@ def main():
         info = solve_mixed()
         log(f'warmup solve: relres {info.relres:.3e}, IR steps {info.ir_steps}, inner its {info.total_iters}')
     L.check(L.lib.vp_ctx_set_profiling(ctx.handle, 1))
-    kernels = ['xfwd', 'plane', 'xinv', 'pc_xfwd', 'pc_plane', 'pc_xinv']
+    kernels = ['xfwd', 'plane', 'yfwd', 'zpass', 'yinv', 'xinv', 'pc_xfwd', 'pc_plane', 'pc_xinv']
     for k in kernels:
         l_, m_ = ctypes.c_int64(), ctypes.c_double()
         L.check(L.lib.vp_ctx_kernel_stats(ctx.handle, k.encode(), ctypes.byref(l_), ctypes.byref(m_), 1))
@@ -185,7 +194,7 @@ def main():
         info = L.GmresInfo()
         ctx.synchronize()
         t = time.perf_counter()
-        L.check(L.lib.vp_gmres(op.handle, hp, L.VP_PREC_LEFT, db.ptr, None, dx.ptr, L.VP_MEM_DEVICE, L.VP_FP64,
+        L.check(L.lib.vp_gmres(op.handle, hp, L.VP_PREC_RIGHT, db.ptr, None, dx.ptr, L.VP_MEM_DEVICE, L.VP_FP64,
                                wl['restart'], wl['tol'], 400, ctypes.byref(info)))
         ctx.synchronize()
         t64 = time.perf_counter() - t
@@ -231,8 +240,7 @@ def main():
         'data': 'synthetic (device-assembled Green tensor, generated geometry)',
         'config': {'workload': args.workload, 'description': wl['desc'], 'unknowns': n, 'tol': wl['tol'],
                    'restart': wl['restart'], 'inner_tol': wl['inner_tol'], 'inner_maxit': wl['inner_maxit'],
-                   'preconditioner': None if prec is None else
-                   f'2-level circulant on the core box, complex-shift damping {wl["damping"]}',
+                   'preconditioner': None if prec is None else pdesc, 'preconditioning': 'right',
                    'parallelism': 'single GPU'},
         'relres_worst': worst,
         'ir_steps': [i.ir_steps for i in infos],
@@ -252,6 +260,27 @@ def main():
         'cpu_baseline': cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def make_preconditioner(prob, wl, vie, configs):
+    """2-level circulant preconditioner of the workload on its core box.
+
+    'face': contrast averaged over the two circulant axes, exact along
+    wl['axis'] (1 = y, 2 = z); 'scalar': the core contrast on the box.
+    """
+    n = prob.n
+    if wl['prec'] == 'face':
+        shape, pidx, sub, face = configs.face_box(prob, wl['axis'])
+        P = vie.Preconditioner.level2_face(shape, prob.dx, prob.ko, face, pidx, damping=wl['damping'])
+        desc = (f'2-level circulant, face-averaged contrast, exact along {"yz"[wl["axis"] - 1]}, box {shape}, '
+                f'damping {wl["damping"]}')
+    else:
+        shape, bidx, sub = prob.box_problem()
+        P = vie.Preconditioner.level2_assembled(shape, prob.dx, prob.ko, prob.core_contrast(), 1.0, bidx,
+                                                damping=wl['damping'])
+        desc = f'2-level circulant, core contrast, box {shape}, damping {wl["damping"]}'
+    P.set_embedding(n, sub)
+    return P, desc
 
 
 def cpu_baseline(prob, vie, fp64, log):

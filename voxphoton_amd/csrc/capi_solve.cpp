@@ -303,11 +303,18 @@ static void solve_ir_sets(ProblemSet<double>& p64, ProblemSet<float>& p32, const
     VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
     // inner FP32 solve of (A/s) d = r/||r||
     dev_scale_to_f(r.as<double2>(), rn.as<double>(), r32.as<float2>(), n, np, 1.0, s);
+    // The inner solve reduces its residual by inner_tol, or by no more than
+    // the last step needs to reach tol (with a factor 2 margin for the FP32
+    // residual gap); the outer loop recomputes the residual in FP64.
     GmresParams prm;
     prm.restart = restart;
     prm.tol = inner_tol;
     prm.maxit = inner_maxit;
     prm.side = side;
+    prm.final_residual = false;
+    prm.tol_per.assign(np, inner_tol);
+    for (int p = 0; p < np; ++p)
+      if (hact[p] && out.relres[p] > 0) prm.tol_per[p] = std::max(inner_tol, std::min(0.5, 0.5 * tol / out.relres[p]));
     GmresResult gr;
     gmres_device<float>(p32, r32.as<float2>(), d32.as<float2>(), prm, gr);
     for (int p = 0; p < np; ++p)

@@ -467,6 +467,8 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   res.total.assign(np, 0);
   res.relres.assign(np, 0.0);
   res.resvec.assign(np, {});
+  VP_REQUIRE(prm.tol_per.empty() || int(prm.tol_per.size()) == np, ERR_INVALID, "per-problem tolerances");
+  auto tol_of = [&](int p) { return prm.tol_per.empty() ? prm.tol : prm.tol_per[size_t(p)]; };
 
   std::vector<int> hact(np, 1);
   auto upload_act = [&]() {
@@ -514,11 +516,11 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   for (int p = 0; p < np; ++p) {
     last[p] = hbn[p] > 0 ? hbeta[p] / hbn[p] : 0.0;
     res.resvec[p].push_back(last[p]);
-    hact[p] = (last[p] > prm.tol) ? 1 : 0;
+    hact[p] = (last[p] > tol_of(p)) ? 1 : 0;
   }
   {
     std::vector<double> ta(np);
-    for (int p = 0; p < np; ++p) ta[p] = prm.tol * hbn[p];
+    for (int p = 0; p < np; ++p) ta[p] = tol_of(p) * hbn[p];
     VP_HIP(hipMemcpy(tolabs.p, ta.data(), np * sizeof(double), hipMemcpyHostToDevice));
   }
   int outit = 0;
@@ -601,12 +603,20 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     VP_HIP(hipGetLastError());
     VP_HIP(hipStreamSynchronize(s));   // iact is released at the end of this cycle
     // restart: fresh residual (pgmres_iter.m:150 carries it by recurrence)
-    for (int p = 0; p < np; ++p) hact[p] = (hact[p] && last[p] > prm.tol) ? 1 : 0;
+    for (int p = 0; p < np; ++p) hact[p] = (hact[p] && last[p] > tol_of(p)) ? 1 : 0;
     bool more = false;
     for (int p = 0; p < np; ++p) more |= hact[p] != 0;
     if (!more || outit >= prm.maxit) break;
     upload_act();
     residual_into_V0(true);
+  }
+  if (!prm.final_residual) {
+    // caller recomputes the residual itself: report the recurrence estimate
+    for (int p = 0; p < np; ++p) {
+      res.relres[p] = last[p];
+      res.flag[p] = last[p] <= tol_of(p) ? 0 : 1;
+    }
+    return;
   }
   // final unpreconditioned relative residual in FP64 from the T data
   std::vector<int> all(np, 1);
@@ -621,7 +631,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   VP_HIP(hipMemcpy(hbn.data(), bn.p, np * sizeof(double), hipMemcpyDeviceToHost));
   for (int p = 0; p < np; ++p) {
     res.relres[p] = hbn[p] > 0 ? hbeta[p] / hbn[p] : 0.0;
-    res.flag[p] = res.relres[p] <= prm.tol ? 0 : 1;
+    res.flag[p] = res.relres[p] <= tol_of(p) ? 0 : 1;
   }
 }
 template void gmres_device<float>(ProblemSet<float>&, const float2*, float2*, const GmresParams&, GmresResult&);

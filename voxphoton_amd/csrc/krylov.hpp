@@ -42,6 +42,8 @@ struct GmresParams {
   int side = 0;        // 0 left, 1 right
   bool x0_nonzero = false;
   int resvec_cap = 0;  // per problem
+  std::vector<double> tol_per;   // per-problem tolerance (empty: tol for all)
+  bool final_residual = true;    // recompute the true residual after the last cycle
 };
 
 struct GmresResult {

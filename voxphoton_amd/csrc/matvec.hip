@@ -409,11 +409,19 @@ void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d
     (void)hipFuncSetAttribute(fy0, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds_y));
     (void)hipFuncSetAttribute(fy1, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds_y));
     (void)hipFuncSetAttribute(fz, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds_z));
-    ProfScope ps(s, KID_PLANE);
-    hipLaunchKernelGGL((k_ypass<T, false>), gy, dim3(mp.ty), mp.lds_y, s, d_probs, va.active, g, mp, twy);
-    const dim3 gz(unsigned(g.x.P * ((g.y.P + mp.Tz - 1) / mp.Tz)), nprob);
-    hipLaunchKernelGGL(k_zpass<T>, gz, dim3(mp.tz), mp.lds_z, s, d_probs, va.active, g, mp, twz);
-    hipLaunchKernelGGL((k_ypass<T, true>), gy, dim3(mp.ty), mp.lds_y, s, d_probs, va.active, g, mp, twy);
+    {
+      ProfScope ps(s, KID_YFWD);
+      hipLaunchKernelGGL((k_ypass<T, false>), gy, dim3(mp.ty), mp.lds_y, s, d_probs, va.active, g, mp, twy);
+    }
+    {
+      ProfScope ps(s, KID_ZPASS);
+      const dim3 gz(unsigned(g.x.P * ((g.y.P + mp.Tz - 1) / mp.Tz)), nprob);
+      hipLaunchKernelGGL(k_zpass<T>, gz, dim3(mp.tz), mp.lds_z, s, d_probs, va.active, g, mp, twz);
+    }
+    {
+      ProfScope ps(s, KID_YINV);
+      hipLaunchKernelGGL((k_ypass<T, true>), gy, dim3(mp.ty), mp.lds_y, s, d_probs, va.active, g, mp, twy);
+    }
     VP_HIP(hipGetLastError());
   } else {
     dim3 gp(g.x.P, nprob);

@@ -50,7 +50,8 @@ void resolve(ProfState& st) {
 }  // namespace
 
 const char* kernel_id_name(int kid) {
-  static const char* names[kNumKernelIds] = {"xfwd", "plane", "xinv", "pc_xfwd", "pc_plane", "pc_xinv"};
+  static const char* names[kNumKernelIds] = {"xfwd", "plane", "xinv", "pc_xfwd", "pc_plane", "pc_xinv",
+                                                 "yfwd", "zpass", "yinv"};
   return (kid >= 0 && kid < kNumKernelIds) ? names[kid] : "";
 }
 

@@ -89,7 +89,10 @@ struct VecArgs {
 };
 
 // Per-kernel event timing (prof.cpp)
-enum KernelId { KID_XFWD = 0, KID_PLANE, KID_XINV, KID_PC_XFWD, KID_PC_PLANE, KID_PC_XINV, kNumKernelIds };
+enum KernelId {
+  KID_XFWD = 0, KID_PLANE, KID_XINV, KID_PC_XFWD, KID_PC_PLANE, KID_PC_XINV, KID_YFWD, KID_ZPASS, KID_YINV,
+  kNumKernelIds
+};
 const char* kernel_id_name(int kid);
 void prof_begin(hipStream_t s, int kid, void** token);
 void prof_end(hipStream_t s, void* token);
