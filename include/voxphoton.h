@@ -70,7 +70,8 @@ int vp_ctx_destroy(vp_ctx ctx);
 int vp_ctx_synchronize(vp_ctx ctx);
 /* Per-kernel timing on the context's stream with HIP events: after
  * vp_ctx_set_profiling(ctx, 1) every launch of the named kernels ("xfwd",
- * "plane", "xinv", "pc_xfwd", "pc_plane", "pc_xinv") records an event pair;
+ * "plane" (fused yz pass) or "yfwd"/"zpass"/"yinv" (split yz passes), "xinv",
+ * "pc_xfwd", "pc_plane", "pc_xinv") records an event pair;
  * vp_ctx_kernel_stats returns the launch count and summed milliseconds
  * (reset != 0 clears them).  No counterpart in the reference (which only
  * prints tic/toc totals); used by the benchmark's roofline figures. */
@@ -216,8 +217,10 @@ int vp_gmres(vp_op op, vp_prec M, int side, const void* b, const void* x0, void*
 /*
  * Mixed precision: FP64 iterative refinement around FP32 GMRES.  Outer
  * residual r = b - A x in FP64 on the device; inner FP32 GMRES(restart)
- * with the FP32 preconditioner solves A d = r to inner_tol; stops when
- * ||b - A x|| / ||b|| <= tol or after max_outer steps.
+ * with the FP32 preconditioner solves A d = r to inner_tol, loosened to
+ * 0.5 * tol / relres (at most 0.5) when that suffices to reach tol; stops
+ * when ||b - A x|| / ||b|| <= tol or after max_outer steps.  No counterpart
+ * in the reference (which solves in FP64 only).
  */
 int vp_solve_ir(vp_op op, vp_prec M, int side, const void* b, void* x, int mem, double tol, int restart,
                 double inner_tol, int inner_maxit, int max_outer, vp_gmres_info* info);
