@@ -99,3 +99,55 @@ def test_matvec_variants(vp_ctx, monkeypatch, variant, shape, fft):
         e32 = rel(op.apply(x, flag, 'fp32'), ref)
         print(variant, shape, fft, flag, 'rel', e64, e32, flush=True)
         assert e64 < 1e-12 and e32 < 1e-5
+
+
+@pytest.mark.parametrize('variant', ['fused', 'split'])
+@pytest.mark.parametrize('prune', ['1', '0'])
+@pytest.mark.parametrize('form', ['structured', 'generic'])
+def test_matvec_unknowns_in_sub_box(vp_ctx, monkeypatch, variant, prune, form):
+    # Unknowns confined to an offset sub-box with a hole: the passes run on
+    # the bounding box only (prune=1) or on the full grid (prune=0).
+    from voxphoton_amd.vie import NOperator
+    monkeypatch.setenv('VP_MATVEC_VARIANT', variant)
+    monkeypatch.setenv('VP_MATVEC_PRUNE', prune)
+    L, M, N = 9, 8, 7
+    rng = np.random.default_rng(33)
+    G, Mr, Mc, Gram, _, _ = random_problem(L, M, N, seed=33, with_mr=True)
+    mask = np.zeros((L, M, N), bool)
+    mask[2:7, 3:7, 1:4] = True
+    mask[4, 5, 2] = False
+    idx = ovie.idx_from_mask(mask)
+    if form == 'generic':
+        idx = rng.permutation(idx)[: idx.size - 7]
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    A = ovie.dense_AN(G, Mr, Mc, Gram, idx)
+    op = NOperator.from_toeplitz(G, Mr, Mc, Gram, idx, ctx=vp_ctx)
+    for flag, ref in (('notransp', A @ x), ('transp', A.conj().T @ x)):
+        e64 = rel(op.apply(x, flag, 'fp64'), ref)
+        e32 = rel(op.apply(x, flag, 'fp32'), ref)
+        print(variant, prune, form, flag, 'rel', e64, e32, flush=True)
+        assert e64 < 1e-12 and e32 < 1e-5
+
+
+@pytest.mark.parametrize('variant', ['fused', 'split'])
+@pytest.mark.parametrize('sym', ['1', '0'])
+def test_matvec_fN_with_and_without_parity_structure(vp_ctx, monkeypatch, variant, sym):
+    # A caller's fN whose odd components do not vanish on their zero planes
+    # lacks the octant symmetry: it must be stored in full.  A symmetric fN
+    # is stored as its octant unless VP_FN_SYMMETRY=0.
+    from voxphoton_amd.vie import NOperator
+    monkeypatch.setenv('VP_MATVEC_VARIANT', variant)
+    monkeypatch.setenv('VP_FN_SYMMETRY', sym)
+    rng = np.random.default_rng(44)
+    L, M, N = 7, 6, 5
+    G, Mr, Mc, Gram, idx, x = random_problem(L, M, N, seed=44)
+    Gasym = G + 0.5 * (rng.standard_normal(G.shape) + 1j * rng.standard_normal(G.shape))
+    for GG in (G, Gasym):
+        fN = ovie.fN_from_toeplitz(GG)
+        op = NOperator.from_fN(fN, None, Mc, Gram, idx, ctx=vp_ctx)
+        for flag in ('notransp', 'transp'):
+            ref = ovie.mv_AN_const(x, fN, Mr, Mc, Gram, flag, idx)
+            e64 = rel(op.apply(x, flag, 'fp64'), ref)
+            e32 = rel(op.apply(x, flag, 'fp32'), ref)
+            print(variant, sym, GG is G, flag, e64, e32, flush=True)
+            assert e64 < 1e-12 and e32 < 1e-5

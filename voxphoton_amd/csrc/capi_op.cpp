@@ -198,6 +198,31 @@ static void index_setup(vp_op_s* op, const int64_t* idx, int64_t n, std::vector<
   op->n = n;
 }
 
+// Bounding box of the voxels that carry unknowns.  The passes run on the box
+// only; VP_MATVEC_PRUNE=0 keeps the full grid.
+static void unknown_box(vp_op_s* op, const std::vector<int>& pos) {
+  const char* e = std::getenv("VP_MATVEC_PRUNE");
+  const bool prune = !(e && std::strcmp(e, "0") == 0);
+  int lo[3] = {op->L, op->M, op->N}, hi[3] = {-1, -1, -1};
+  if (prune) {
+    for (size_t i = 0; i < pos.size(); ++i) {
+      if (pos[i] < 0) continue;
+      int64_t v = int64_t(i) % op->V;
+      const int c[3] = {int(v % op->L), int((v / op->L) % op->M), int(v / (int64_t(op->L) * op->M))};
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], c[a]);
+        hi[a] = std::max(hi[a], c[a]);
+      }
+    }
+  }
+  if (!prune || hi[0] < 0) {
+    lo[0] = lo[1] = lo[2] = 0;
+    hi[0] = op->L - 1; hi[1] = op->M - 1; hi[2] = op->N - 1;
+  }
+  op->l0 = lo[0]; op->m0 = lo[1]; op->n0 = lo[2];
+  op->Lb = hi[0] - lo[0] + 1; op->Mb = hi[1] - lo[1] + 1; op->Nb = hi[2] - lo[2] + 1;
+}
+
 // Gathers a complex128 grid array (L x M x N) to the unknown entries.
 static std::vector<double2> gather_material(const vp_op_s* op, const double* A, const int64_t* idx, double dflt) {
   const int64_t cnt = op->structured ? op->nS : op->n;
@@ -225,7 +250,7 @@ static void init_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const std::vector<do
     VP_HIP(hipMemcpy(P.mr.p, h.data(), cnt * sizeof(C<T>), hipMemcpyHostToDevice));
   }
   P.gram = T(gram_t);
-  P.work.alloc(size_t(3) * op->Lp * op->M * op->N * sizeof(C<T>));
+  P.work.alloc(size_t(3) * op->Lp * op->Mb * op->Nb * sizeof(C<T>));
   P.xs.alloc(size_t(op->n) * sizeof(C<T>));
   P.ys.alloc(size_t(op->n) * sizeof(C<T>));
 }
@@ -271,6 +296,7 @@ static vp_op_s* op_begin(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp
     op->has_mr = Mr != nullptr;
     std::vector<int> pos;
     index_setup(op, idx, n, pos);
+    unknown_box(op, pos);
     op->pos.alloc(pos.size() * sizeof(int));
     VP_HIP(hipMemcpy(op->pos.p, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice));
     auto mc = gather_material(op, Mc, idx, 0.0);
@@ -302,9 +328,13 @@ static void configure_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const double2* 
   hipStream_t s = op->ctx->stream;
   build_geometry(P.g, P.plan, *op, fp64, op->ctx->device, variant);
   upload_twiddles(P, s);
-  layout_fN<T>(P.g, F, P.fN, scale, s);
+  P.plan.fsym = (variant == 1 && op->fsym) ? 1 : 0;
+  if (P.plan.fsym)
+    layout_fN_sym<T>(P.g, F, P.fN, scale, s);
+  else
+    layout_fN<T>(P.g, F, P.fN, scale, s);
   if (variant == 1)
-    P.work2.alloc(size_t(3) * op->Lp * op->Mp * op->N * sizeof(C<T>));
+    P.work2.alloc(size_t(3) * op->Lp * op->Mp * op->Nb * sizeof(C<T>));
   else
     P.work2.release();
   upload_probs(op, P);
@@ -335,12 +365,13 @@ static double time_applies(vp_op_s* op, OpPrec<T>& P) {
 }
 
 // Timed choice between the fused and split matvec, made once per
-// (device, grid, FFT sizes, precision) in a process so that operators of one
+// (device, unknowns' box, FFT sizes, precision) in a process so that operators of one
 // shape always share a plan (batched solves rely on it).
 struct VariantKey {
-  int device, L, M, N, Lp, Mp, Np, fp64;
+  int device, Lb, Mb, Nb, Lp, Mp, Np, fp64;
   bool operator<(const VariantKey& o) const {
-    return std::tie(device, L, M, N, Lp, Mp, Np, fp64) < std::tie(o.device, o.L, o.M, o.N, o.Lp, o.Mp, o.Np, o.fp64);
+    return std::tie(device, Lb, Mb, Nb, Lp, Mp, Np, fp64) <
+           std::tie(o.device, o.Lb, o.Mb, o.Nb, o.Lp, o.Mp, o.Np, o.fp64);
   }
 };
 static std::mutex g_variant_mu;
@@ -348,7 +379,7 @@ static std::map<VariantKey, int> g_variant;
 
 template <typename T>
 static void finalize_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const double2* F, double scale) {
-  const VariantKey key{op->ctx->device, op->L, op->M, op->N, op->Lp, op->Mp, op->Np, int(fp64)};
+  const VariantKey key{op->ctx->device, op->Lb, op->Mb, op->Nb, op->Lp, op->Mp, op->Np, int(fp64)};
   int v = variant_request();
   if (v < 0) {
     std::lock_guard<std::mutex> lk(g_variant_mu);
@@ -384,6 +415,9 @@ static void finalize_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const double2* F
 // Common tail of every vp_op_create*: F is the true-frequency tensor.
 static void op_finish(vp_op_s* op, const double2* F) {
   const double P = double(op->Lp) * op->Mp * op->Np;
+  // VP_FN_SYMMETRY=0 keeps the full tensor in every variant.
+  const char* e = std::getenv("VP_FN_SYMMETRY");
+  op->fsym = !(e && std::strcmp(e, "0") == 0) && fN_parity_symmetric(*op, F, op->ctx->stream);
   if (op->p64.present) finalize_prec(op, op->p64, true, F, 1.0 / P);
   if (op->p32.present) finalize_prec(op, op->p32, false, F, 1.0 / (P * op->scale32));
   VP_HIP(hipStreamSynchronize(op->ctx->stream));

@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(1024) k_xfwd(const ProbPtrs<T>* __restrict__ p
       if (m < M) {
         for (int l = j; l < L; l += Ls) {
           int64_t u;
-          C<T> xv = load_unknown(P, X, g, l + int64_t(L) * (m + int64_t(M) * n), c, transp, u);
+          C<T> xv = load_unknown(P, X, g, g.voxel(l, m, n), c, transp, u);
           if (u >= 0) acc = cadd(acc, px ? cmul(xv, twx[(int64_t(l) * px) % Lp]) : xv);
         }
       }
@@ -243,9 +243,17 @@ __global__ void __launch_bounds__(256) k_zpass(const ProbPtrs<T>* __restrict__ p
   C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
   const int N = g.N, Np = g.z.P, Mp = g.y.P, Tz = mp.Tz;
   const int nt = (Mp + Tz - 1) / Tz;
+  const int Lp = g.x.P;
+  // Logical order pairs kx with Lp - kx (0, 1, Lp-1, 2, Lp-2, ...): the two
+  // planes read the same octant of a parity-compressed tensor, and with the
+  // XCD remap their tiles run close together on one XCD (shared L2).  The
+  // x spectra are stored by residue class: kx = Rx*a + px sits at px*Sx + a.
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
-  const int kxi = blk / nt;
-  const int ky0 = (blk - kxi * nt) * Tz;
+  const int io = blk / nt;
+  const int jo = (io + 1) >> 1;
+  const int kx = io == 0 ? 0 : ((io & 1) ? jo : Lp - jo);
+  const int kxi = (kx % g.x.R) * g.x.S + kx / g.x.R;
+  const int ky0 = (blk - io * nt) * Tz;
   const int E = 3 * Tz * Np;
   C<T>* W1 = W0 + E;
   C<T>* tw = W1 + E;
@@ -266,14 +274,30 @@ __global__ void __launch_bounds__(256) k_zpass(const ProbPtrs<T>* __restrict__ p
   __syncthreads();
   const LdsLayout lay{1, Np, 0, 1};
   C<T>* R = lds_fft<T, false>(W0, W1, g.z.plan, 3 * Tz, lay, tw);
-  // tensor multiply; fN class layout with Ry = Rz = 1 is [kx][kz][ky][6]
-  const C<T>* F = P.fN + int64_t(kxi) * Np * Mp * 6;
+  // tensor multiply: fN is [kx][kz][ky][6] (class layout with Ry = Rz = 1) or,
+  // parity compressed, the octant [kxc][kzc][kyc][6] with the signs of the
+  // odd components (xy: x,y; xz: x,z; yz: y,z) applied on mirrored axes.
+  const bool mx = 2 * kx > Lp;
+  const int kxc = mx ? Lp - kx : kx;
+  const int Hy = Mp / 2 + 1, Hz = Np / 2 + 1;
+  const C<T>* F = mp.fsym ? P.fN + int64_t(kxc) * Hz * Hy * 6 : P.fN + int64_t(kxi) * Np * Mp * 6;
   for (int e = threadIdx.x; e < Tz * Np; e += blockDim.x) {
     const int kz = dTz.div(e), t = e - kz * Tz;
     const int ky = ky0 + t;
     if (ky < Mp) {
-      const C<T>* f = F + (int64_t(kz) * Mp + ky) * 6;
-      const C<T> f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4], f5 = f[5];
+      C<T> f0, f1, f2, f3, f4, f5;
+      if (mp.fsym) {
+        const bool my = 2 * ky > Mp, mz = 2 * kz > Np;
+        const int kyc = my ? Mp - ky : ky, kzc = mz ? Np - kz : kz;
+        const C<T>* f = F + (int64_t(kzc) * Hy + kyc) * 6;
+        f0 = f[0]; f1 = f[1]; f2 = f[2]; f3 = f[3]; f4 = f[4]; f5 = f[5];
+        if (mx != my) f1 = cneg(f1);
+        if (mx != mz) f2 = cneg(f2);
+        if (my != mz) f4 = cneg(f4);
+      } else {
+        const C<T>* f = F + (int64_t(kz) * Mp + ky) * 6;
+        f0 = f[0]; f1 = f[1]; f2 = f[2]; f3 = f[3]; f4 = f[4]; f5 = f[5];
+      }
       const int i0 = (0 * Tz + t) * Np + kz, i1 = (1 * Tz + t) * Np + kz, i2 = (2 * Tz + t) * Np + kz;
       const C<T> a = R[i0], b = R[i1], cc = R[i2];
       R[i0] = cadd(cadd(cmul(f0, a), cmul(f1, b)), cmul(f2, cc));
@@ -353,7 +377,7 @@ __global__ void __launch_bounds__((acc_bound<T, EPT>())) k_xinv(const ProbPtrs<T
       const int l = o - bb * L;
       const int m = m0 + bb;
       if (m < M) {
-        const int64_t v = l + int64_t(L) * (m + int64_t(M) * n);
+        const int64_t v = g.voxel(l, m, n);
         int64_t pv, u;
         if (g.structured) {
           pv = P.pos[v];

@@ -58,12 +58,14 @@ struct OpPrec {
 struct vp_op_s {
   vp_ctx ctx = nullptr;
   int L = 0, M = 0, N = 0, Lp = 0, Mp = 0, Np = 0;
+  int l0 = 0, m0 = 0, n0 = 0, Lb = 0, Mb = 0, Nb = 0;   // bounding box of the unknowns' voxels
   int64_t V = 0, nS = 0;
   int structured = 0;
   int64_t n = 0;               // unknowns
   double gram = 1.0;           // FP64 identity weight
   double scale32 = 1.0;        // FP32 operator = A / scale32
   bool has_mr = false;
+  bool fsym = false;           // tensor is parity symmetric: split variant stores its octant
   vp::DevBuf pos;              // voxel map
   vp::OpPrec<double> p64;
   vp::OpPrec<float> p32;
@@ -83,6 +85,11 @@ template <typename T> void upload_twiddles(OpPrec<T>& P, hipStream_t s);
 // Writes the class-layout tensor of geometry g from a true-frequency device
 // tensor F (Lp x Mp x Np x 6 complex128, column-major), times scale.
 template <typename T> void layout_fN(const Geometry& g, const double2* F, DevBuf& out, double scale, hipStream_t s);
+// Octant layout [kxc][kzc][kyc][6] of a parity-symmetric tensor (split variant).
+template <typename T>
+void layout_fN_sym(const Geometry& g, const double2* F, DevBuf& out, double scale, hipStream_t s);
+// True when F(P - k) = parity * F(k) along every axis (to 1e-12 of max |F|).
+bool fN_parity_symmetric(const vp_op_s& op, const double2* F, hipStream_t s);
 // True-frequency tensor from a device Toeplitz tensor (L x M x N x 6).
 void toeplitz_to_fN(const vp_op_s& op, const double2* G, double2* F, hipStream_t s);
 // Device assembly of the Toeplitz tensor.

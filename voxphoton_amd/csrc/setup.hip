@@ -20,7 +20,9 @@ static std::vector<int> divisors(int n) {
 
 void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, int device, int variant) {
   g = Geometry{};
-  g.L = op.L; g.M = op.M; g.N = op.N;
+  g.L = op.Lb; g.M = op.Mb; g.N = op.Nb;
+  g.l0 = op.l0; g.m0 = op.m0; g.n0 = op.n0;
+  g.Lg = op.L; g.Mg = op.M;
   g.V = op.V;
   g.nS = op.nS;
   g.structured = op.structured;
@@ -204,6 +206,99 @@ void toeplitz_to_fN(const vp_op_s& op, const double2* G, double2* F, hipStream_t
   launch(t2.as<double2>(), F, 2, op.N, op.Np, op.Lp, op.Mp, op.Np, dtz.as<double2>());
   VP_HIP(hipStreamSynchronize(s));
 }
+
+// ----------------------------------------------------------------------------
+// Parity symmetry of the circulant spectrum.  The embedding of an even (odd)
+// Toeplitz component is even (odd) along each axis, so
+//   F_k(.., P - k_a, ..) = s_{k,a} F_k(.., k_a, ..)
+// and the octant 0 <= k_a <= P_a/2 holds the whole tensor.  The check below
+// measures max |F(e) - s F(mirror_a(e))| over all entries and axes against
+// max |F|; a tensor given by the caller is compressed only if it passes.
+// ----------------------------------------------------------------------------
+__global__ void k_sym_check(const double2* __restrict__ F, int Lp, int Mp, int Np, double* __restrict__ part) {
+  const int64_t total = int64_t(Lp) * Mp * Np * 6;
+  double dmax = 0.0, amax = 0.0;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = e;
+    const int kx = int(r % Lp); r /= Lp;
+    const int ky = int(r % Mp); r /= Mp;
+    const int kz = int(r % Np); r /= Np;
+    const int k = int(r);
+    const double2 v = F[e];
+    amax = fmax(amax, fabs(v.x) + fabs(v.y));
+    const int mir[3] = {kx ? Lp - kx : 0, ky ? Mp - ky : 0, kz ? Np - kz : 0};
+    for (int a = 0; a < 3; ++a) {
+      int q[3] = {kx, ky, kz};
+      q[a] = mir[a];
+      const double2 w = F[q[0] + int64_t(Lp) * (q[1] + int64_t(Mp) * (q[2] + int64_t(Np) * k))];
+      const double sgn = c_parity[k][a];
+      dmax = fmax(dmax, fabs(v.x - sgn * w.x) + fabs(v.y - sgn * w.y));
+    }
+  }
+  __shared__ double sd[256], sa[256];
+  sd[threadIdx.x] = dmax;
+  sa[threadIdx.x] = amax;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) {
+      sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + o]);
+      sa[threadIdx.x] = fmax(sa[threadIdx.x], sa[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sd[0];
+    part[2 * blockIdx.x + 1] = sa[0];
+  }
+}
+
+bool fN_parity_symmetric(const vp_op_s& op, const double2* F, hipStream_t s) {
+  const int64_t total = int64_t(op.Lp) * op.Mp * op.Np * 6;
+  const int blocks = int(std::min<int64_t>((total + 255) / 256, 4096));
+  DevBuf part(size_t(2) * blocks * sizeof(double));
+  hipLaunchKernelGGL(k_sym_check, dim3(blocks), dim3(256), 0, s, F, op.Lp, op.Mp, op.Np, part.as<double>());
+  VP_HIP(hipGetLastError());
+  std::vector<double> h(size_t(2) * blocks);
+  VP_HIP(hipMemcpyAsync(h.data(), part.p, part.bytes, hipMemcpyDeviceToHost, s));
+  VP_HIP(hipStreamSynchronize(s));
+  double dmax = 0.0, amax = 0.0;
+  for (int b = 0; b < blocks; ++b) {
+    dmax = std::max(dmax, h[2 * b]);
+    amax = std::max(amax, h[2 * b + 1]);
+  }
+  return std::isfinite(dmax) && dmax <= 1e-12 * amax;
+}
+
+// Octant layout [kxc][kzc][kyc][6], kxc <= Lp/2, kyc <= Mp/2, kzc <= Np/2.
+template <typename T>
+__global__ void k_layout_fN_sym(const double2* __restrict__ F, C<T>* __restrict__ out, int Lp, int Mp, int Np,
+                                double scale) {
+  const int Hx = Lp / 2 + 1, Hy = Mp / 2 + 1, Hz = Np / 2 + 1;
+  const int64_t total = int64_t(Hx) * Hy * Hz * 6;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = e;
+    const int k = int(r % 6); r /= 6;
+    const int ky = int(r % Hy); r /= Hy;
+    const int kz = int(r % Hz); r /= Hz;
+    const int kx = int(r);
+    const double2 v = F[kx + int64_t(Lp) * (ky + int64_t(Mp) * (kz + int64_t(Np) * k))];
+    out[e] = mk<T>(T(v.x * scale), T(v.y * scale));
+  }
+}
+
+template <typename T>
+void layout_fN_sym(const Geometry& g, const double2* F, DevBuf& out, double scale, hipStream_t s) {
+  const int64_t total = int64_t(g.x.P / 2 + 1) * (g.y.P / 2 + 1) * (g.z.P / 2 + 1) * 6;
+  const int blocks = int(std::min<int64_t>((total + 255) / 256, 65536));
+  if (out.bytes != size_t(total) * sizeof(C<T>)) out.alloc(size_t(total) * sizeof(C<T>));
+  hipLaunchKernelGGL(k_layout_fN_sym<T>, dim3(blocks), dim3(256), 0, s, F, out.as<C<T>>(), g.x.P, g.y.P, g.z.P,
+                     scale);
+  VP_HIP(hipGetLastError());
+}
+template void layout_fN_sym<double>(const Geometry&, const double2*, DevBuf&, double, hipStream_t);
+template void layout_fN_sym<float>(const Geometry&, const double2*, DevBuf&, double, hipStream_t);
 
 // ----------------------------------------------------------------------------
 // Green tensor assembly.  For cell offset m (observation cell m, source cell

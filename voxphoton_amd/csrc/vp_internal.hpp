@@ -45,6 +45,7 @@ struct MatvecPlan {
   int variant;
   // split variant: y passes (By pencils of Mp per workgroup), z pass (Tz ky columns)
   int By, ldy, ty, Tz, tz;
+  int fsym;      // split variant: fN stored as its parity octant
   size_t lds_y, lds_z;
   // K1/K3 (x pencils)
   int B;         // pencils per workgroup
@@ -57,12 +58,21 @@ struct MatvecPlan {
   size_t lds1, lds2;  // dynamic LDS bytes
 };
 
+// L, M, N are the extents of the unknowns' bounding box (l0, m0, n0 its
+// offset in the Lg x Mg x . grid): the circulant convolution is shift
+// invariant, so every pass runs on the box and only the voxel index uses the
+// full grid.  V is the full grid's voxel count.
 struct Geometry {
   int L, M, N;
+  int l0, m0, n0;
+  int64_t Lg, Mg;
   int64_t V;
   Dim x, y, z;
   int64_t nS;        // unknown voxels (structured) or unknown entries (generic)
   int structured;    // 1: idx = [idxS; V+idxS; 2V+idxS] with sorted idxS
+  __host__ __device__ int64_t voxel(int l, int m, int n) const {
+    return (l0 + l) + Lg * ((m0 + m) + Mg * int64_t(n0 + n));
+  }
 };
 
 // Static device pointers of one problem of a (possibly batched) launch.  The
