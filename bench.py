@@ -54,30 +54,46 @@ def mv_model_bytes(prob, s):
     return s * (6 * P + 12 * Q + 3 * n + nS) + 8 * nS
 
 
-def kernel_model_bytes(prob, kernel, s):
-    """Algorithmic bytes per launch of one matvec kernel.
+def unknown_box(prob):
+    """Extents (Lb, Mb, Nb) of the bounding box of the unknown voxels."""
+    ext = []
+    for ax in range(3):
+        other = tuple(a for a in range(3) if a != ax)
+        nz = np.flatnonzero(prob.mask.any(axis=other))
+        ext.append(int(nz[-1] - nz[0] + 1))
+    return tuple(ext)
 
-    Q = Lp*M*N x-spectra per component, Y = Lp*Mp*N y-spectra per component
-    (split variant's intermediate), P = Lp*Mp*Np tensor entries per component.
+
+def kernel_model_bytes(prob, kernel, s, fsym=True):
+    """Algorithmic bytes per launch of one matvec kernel as built.
+
+    The passes run on the unknowns' bounding box Lb x Mb x Nb inside the
+    Lp x Mp x Np FFT grid: Qb = Lp*Mb*Nb x-spectra and Yb = Lp*Mp*Nb
+    y-spectra per component.  The split z pass reads the parity octant of
+    fN, Pc = (Lp/2+1)(Mp/2+1)(Np/2+1) entries per component (fsym), the
+    fused plane kernel the full P = Lp*Mp*Np.
     """
     L, M, N = prob.shape
-    P = 8 * L * M * N
-    Q = 2 * L * M * N
-    Y = 4 * L * M * N
+    Lp, Mp, Np = 2 * L, 2 * M, 2 * N
+    Lb, Mb, Nb = unknown_box(prob)
+    P = Lp * Mp * Np
+    Pc = (Lp // 2 + 1) * (Mp // 2 + 1) * (Np // 2 + 1) if fsym else P
+    Qb = Lp * Mb * Nb
+    Yb = Lp * Mp * Nb
     nS = int(prob.mask.sum())
     n = 3 * nS
     if kernel == 'plane':     # fused: fN once, x-spectra read and written
-        return s * (6 * P + 6 * Q)
+        return s * (6 * P + 6 * Qb)
     if kernel == 'yfwd':      # x-spectra read, y-spectra written
-        return s * (3 * Q + 3 * Y)
-    if kernel == 'zpass':     # y-spectra read and written, fN once
-        return s * (6 * P + 6 * Y)
+        return s * (3 * Qb + 3 * Yb)
+    if kernel == 'zpass':     # y-spectra read and written, fN octant once
+        return s * (6 * Pc + 6 * Yb)
     if kernel == 'yinv':      # y-spectra read, x-spectra written
-        return s * (3 * Y + 3 * Q)
+        return s * (3 * Yb + 3 * Qb)
     if kernel == 'xfwd':      # J read, x-spectra written, voxel map read
-        return s * (n + 3 * Q) + 4 * nS
+        return s * (n + 3 * Qb) + 4 * nS
     if kernel == 'xinv':      # x-spectra read, J and Mc read, output written, voxel map read
-        return s * (3 * Q + 2 * n + nS) + 4 * nS
+        return s * (3 * Qb + 2 * n + nS) + 4 * nS
     return None
 
 
@@ -90,6 +106,8 @@ def main():
     ap.add_argument('--no-prec', action='store_true')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-fp64-solve', action='store_true')
+    ap.add_argument('--traffic-probe', action='store_true',
+                    help='for rocprofv3 --pmc runs: a 1 GiB device copy (calibration), then 3 FP32 matvecs')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU only (asked for --gpus {args.gpus}); every solve of this path fits '
@@ -110,6 +128,9 @@ def main():
     op = vie.NOperator.assembled(prob.shape, prob.dx, prob.ko, None, prob.Mc, prob.idx)
     t_op = time.perf_counter() - t0
     log(f'operator assembled + embedded in {t_op:.2f} s')
+    if args.traffic_probe:
+        traffic_probe(op, ctx, n, vie, L, log)
+        return
     prec = None
     pdesc = None
     t_pc = 0.0
@@ -135,6 +156,14 @@ def main():
     for _ in range(args.warmup):
         info = solve_mixed()
         log(f'warmup solve: relres {info.relres:.3e}, IR steps {info.ir_steps}, inner its {info.total_iters}')
+    def read_stats(names):
+        out = {}
+        for k in names:
+            l_, m_ = ctypes.c_int64(), ctypes.c_double()
+            L.check(L.lib.vp_ctx_kernel_stats(ctx.handle, k.encode(), ctypes.byref(l_), ctypes.byref(m_), 1))
+            out[k] = (l_.value, m_.value)
+        return out
+
     L.check(L.lib.vp_ctx_set_profiling(ctx.handle, 1))
     kernels = ['xfwd', 'plane', 'yfwd', 'zpass', 'yinv', 'xinv', 'pc_xfwd', 'pc_plane', 'pc_xinv']
     for k in kernels:
@@ -181,12 +210,26 @@ def main():
         ctx.synchronize()
         return (time.perf_counter() - t) / reps
 
+    def built_bytes(xd, yd, prec_id, s):
+        # one profiled pass: which matvec kernels ran, summed as-built bytes
+        L.check(L.lib.vp_ctx_set_profiling(ctx.handle, 1))
+        mv_kernels = ['xfwd', 'plane', 'yfwd', 'zpass', 'yinv', 'xinv']
+        read_stats(mv_kernels)
+        L.check(L.lib.vp_op_apply(op.handle, 0, xd.ptr, yd.ptr, L.VP_MEM_DEVICE, prec_id))
+        ctx.synchronize()
+        ran = read_stats(mv_kernels)
+        L.check(L.lib.vp_ctx_set_profiling(ctx.handle, 0))
+        return sum(kernel_model_bytes(prob, k, s) for k, (cnt, _) in ran.items() if cnt)
+
     t_mv64 = time_mv(x64, y64, L.VP_FP64)
     t_mv32 = time_mv(x32, y32, L.VP_FP32)
     B64 = mv_model_bytes(prob, 16)
     B32 = mv_model_bytes(prob, 8)
-    log(f'matvec fp64 {t_mv64 * 1e3:.3f} ms ({B64 / t_mv64 / 1e9:.0f} GB/s model), '
-        f'fp32 {t_mv32 * 1e3:.3f} ms ({B32 / t_mv32 / 1e9:.0f} GB/s model)')
+    Bb64 = built_bytes(x64, y64, L.VP_FP64, 16)
+    Bb32 = built_bytes(x32, y32, L.VP_FP32, 8)
+    log(f'matvec fp64 {t_mv64 * 1e3:.3f} ms ({B64 / t_mv64 / 1e9:.0f} GB/s canonical model, '
+        f'{Bb64 / t_mv64 / 1e9:.0f} GB/s as built), fp32 {t_mv32 * 1e3:.3f} ms ({B32 / t_mv32 / 1e9:.0f} GB/s '
+        f'canonical, {Bb32 / t_mv32 / 1e9:.0f} GB/s as built)')
 
     # ---- plain FP64 GPU solve (serial)
     fp64 = None
@@ -251,6 +294,8 @@ def main():
         'mixed_vs_fp64': None if fp64 is None else round(solves_per_s / fp64['solves_per_s'], 3),
         'matvec': {'fp64_ms': round(t_mv64 * 1e3, 4), 'fp32_ms': round(t_mv32 * 1e3, 4),
                    'model_bytes_fp64': B64, 'model_bytes_fp32': B32,
+                   'built_bytes_fp64': Bb64, 'built_bytes_fp32': Bb32,
+                   'fp64_GBps_built': round(Bb64 / t_mv64 / 1e9, 1), 'fp32_GBps_built': round(Bb32 / t_mv32 / 1e9, 1),
                    'fp64_GBps': round(B64 / t_mv64 / 1e9, 1), 'fp32_GBps': round(B32 / t_mv32 / 1e9, 1),
                    'fp64_frac_hbm': round(B64 / t_mv64 / HBM_PEAK, 4),
                    'fp32_frac_hbm': round(B32 / t_mv32 / HBM_PEAK, 4)},
@@ -260,6 +305,25 @@ def main():
         'cpu_baseline': cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def traffic_probe(op, ctx, n, vie, L, log):
+    """Kernels for the HBM-traffic counters: torch's device copy of 1 GiB
+    (read 1 GiB, write 1 GiB: the calibration of FETCH_SIZE / WRITE_SIZE),
+    then three FP32 matvecs."""
+    import torch
+    src = torch.ones(1 << 28, dtype=torch.float32, device='cuda')
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    x32 = vie.DeviceArray(ctx, n * 8)
+    y32 = vie.DeviceArray(ctx, n * 8)
+    x32.upload((np.random.default_rng(1550).standard_normal(n) + 0j).astype(np.complex64))
+    for _ in range(3):
+        L.check(L.lib.vp_op_apply(op.handle, 0, x32.ptr, y32.ptr, L.VP_MEM_DEVICE, L.VP_FP32))
+    ctx.synchronize()
+    log('traffic probe done: 3 x 1 GiB copies, 3 FP32 matvecs')
 
 
 def make_preconditioner(prob, wl, vie, configs):

@@ -45,9 +45,12 @@ def test_chan1_apply_matches_oracle(vp_ctx):
     assert rel(P.apply(x), ref) < 1e-12
 
 
-@pytest.mark.parametrize('shape', [(6, 4, 3), (10, 5, 2)])
-def test_level2_device_setup_matches_oracle(vp_ctx, shape):
+@pytest.mark.parametrize('sym', ['1', '0'])
+@pytest.mark.parametrize('shape', [(6, 4, 3), (10, 5, 2), (7, 5, 3)])
+def test_level2_device_setup_matches_oracle(vp_ctx, monkeypatch, shape, sym):
+    # sym=1 stores the (kx <= L/2, ky <= M/2) block quadrant, sym=0 all blocks
     from voxphoton_amd.vie import Preconditioner
+    monkeypatch.setenv('VP_PC_SYMMETRY', sym)
     L, M, N = shape
     dx, ko = 20e-9, 5.85e6
     G = ogr.assemble_G(L, M, N, dx, ko)
@@ -79,8 +82,10 @@ def test_level2_blocks_wider_than_a_wavefront(vp_ctx):
     assert rel(P.apply(x, 'fp32'), ref) < 1e-4
 
 
-def test_level2_face_matches_oracle(vp_ctx):
+@pytest.mark.parametrize('sym', ['1', '0'])
+def test_level2_face_matches_oracle(vp_ctx, monkeypatch, sym):
     from voxphoton_amd.vie import Preconditioner
+    monkeypatch.setenv('VP_PC_SYMMETRY', sym)
     L, M, N = 8, 5, 4
     dx, ko = 20e-9, 5.85e6
     G = ogr.assemble_G(L, M, N, dx, ko)
@@ -91,6 +96,22 @@ def test_level2_face_matches_oracle(vp_ctx):
     x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
     ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
     P = Preconditioner.level2_face((L, M, N), dx, ko, face, idx, ctx=vp_ctx)
+    assert rel(P.apply(x), ref) < 1e-10
+
+
+def test_level2_without_parity_zeros_keeps_all_blocks(vp_ctx):
+    # Odd components that do not vanish on their zero planes break the
+    # mirror symmetry of the Chan blocks: every block must be stored.
+    from voxphoton_amd.vie import Preconditioner
+    L, M, N = 6, 5, 3
+    rng = np.random.default_rng(12)
+    G = random_toeplitz(L, M, N, rng) * 1e-3
+    G[0, :, :, 1] += 2e-4 * (1 + 1j)
+    idx = ovie.idx_from_mask(np.ones((L, M, N), bool))
+    Binv = opre.level_2_parallel_func(G, 0.5, 1.0, L, M, N, 1.0)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
+    P = Preconditioner.level2(G, 0.5, 1.0, 1.0, idx, ctx=vp_ctx)
     assert rel(P.apply(x), ref) < 1e-10
 
 

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -85,6 +86,9 @@ static vp_prec_s* prec_begin(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const 
     p->g.M = int(M);
     p->g.N = int(N);
     p->g.V = L * M * N;
+    p->g.bsym = 0;
+    p->g.Hx = int(L);
+    p->g.Hy = int(M);
     std::vector<int> pos;
     index_map(p->g.V, idx, n, pos, p->g.structured, p->g.nS);
     p->nbox = n;
@@ -215,7 +219,7 @@ static void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* 
     for (int p = 0; p < np; ++p) {
       vp_prec_s* q = precs[p];
       VP_REQUIRE(q && q->g.L == q0->g.L && q->g.M == q0->g.M && q->g.N == q0->g.N && q->nbox == q0->nbox &&
-                     q->g.level == q0->g.level && q->g.n_outer == q0->g.n_outer,
+                     q->g.level == q0->g.level && q->g.n_outer == q0->g.n_outer && q->g.bsym == q0->g.bsym,
                  ERR_INVALID, "batched preconditioners must share one geometry");
       if (std::is_same<T, float>::value && (!q->p32.present || q->scale32 != ops[p]->scale32))
         prec_make_fp32(q, ops[p]->scale32);
@@ -380,16 +384,29 @@ int vp_prec_create_chan1(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const doub
   VP_API_END
 }
 
+// Device-built 2-level blocks are kept as the (kx <= L/2, ky <= M/2)
+// quadrant when the tensor has the parity zeros that make the Chan
+// circulants symmetric (VP_PC_SYMMETRY=0 keeps all L*M blocks).
+static void level2_block_range(vp_prec_s* p, const double2* dG) {
+  const char* e = std::getenv("VP_PC_SYMMETRY");
+  PcGeom& g = p->g;
+  g.bsym = (!(e && std::strcmp(e, "0") == 0) && toeplitz_parity_zero(dG, g.L, g.M, g.N, p->ctx->stream)) ? 1 : 0;
+  g.Hx = g.bsym ? g.L / 2 + 1 : g.L;
+  g.Hy = g.bsym ? g.M / 2 + 1 : g.M;
+}
+
 static int prec_level2_common(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double2* dG, double mc_re,
                               double mc_im, double mr0, double gram, const int64_t* idx, int64_t n,
                               int precisions, vp_prec* out) {
   vp_prec_s* p = prec_begin(ctx, L, M, N, idx, n, 2);
   try {
     const int64_t n3 = 3 * N;
-    p->p64.blocks.alloc(size_t(L * M * n3 * n3) * sizeof(double2));
+    level2_block_range(p, dG);
+    const int64_t nb = int64_t(p->g.Hx) * p->g.Hy;
+    p->p64.blocks.alloc(size_t(nb * n3 * n3) * sizeof(double2));
     build_level2_blocks(dG, int(L), int(M), int(N), make_double2(mc_re, mc_im), gram * mr0,
-                        p->p64.blocks.as<double2>(), ctx->stream);
-    invert_blocks(p->p64.blocks.as<double2>(), L * M, int(n3), ctx->stream);
+                        p->p64.blocks.as<double2>(), p->g.Hx, p->g.Hy, ctx->stream);
+    invert_blocks(p->p64.blocks.as<double2>(), nb, int(n3), ctx->stream);
     prec_finish(p, precisions);
   } catch (...) {
     delete p;
@@ -435,10 +452,13 @@ int vp_prec_create_level2_face(vp_ctx ctx, int64_t L, int64_t M, int64_t N, doub
   vp_prec_s* p = prec_begin(ctx, L, M, N, idx, n, 2);
   try {
     const int64_t n3 = 3 * N;
-    p->p64.blocks.alloc(size_t(L * M * n3 * n3) * sizeof(double2));
+    level2_block_range(p, dG.as<double2>());
+    const int64_t nb = int64_t(p->g.Hx) * p->g.Hy;
+    p->p64.blocks.alloc(size_t(nb * n3 * n3) * sizeof(double2));
     build_level2_blocks(dG.as<double2>(), int(L), int(M), int(N), make_double2(-1.0, 0.0), 0.0,
-                        p->p64.blocks.as<double2>(), ctx->stream, dface.as<double2>(), dx * dx * dx);
-    invert_blocks(p->p64.blocks.as<double2>(), L * M, int(n3), ctx->stream);
+                        p->p64.blocks.as<double2>(), p->g.Hx, p->g.Hy, ctx->stream, dface.as<double2>(),
+                        dx * dx * dx);
+    invert_blocks(p->p64.blocks.as<double2>(), nb, int(n3), ctx->stream);
     prec_finish(p, precisions);
   } catch (...) {
     delete p;

@@ -77,28 +77,44 @@ __global__ void __launch_bounds__(512) k_pc_plane(PcArgs<T> a, PcGeom g) {
   const int M = g.M, N = g.N, n3 = 3 * N;
   const int plane = n3 * M;
   C<T>* W1 = W0 + plane;
-  const int kx = blockIdx.x;
+  // launch order pairs kx with L - kx (0, 1, L-1, 2, L-2, ...), which share
+  // their stored blocks when the blocks are kept as a quadrant
+  const int L = g.L;
+  const int io = xcd_remap(blockIdx.x, gridDim.x);
+  const int jo = (io + 1) >> 1;
+  const int kx = io == 0 ? 0 : ((io & 1) ? jo : L - jo);
   C<T>* Zp = work + int64_t(kx) * plane;
   for (int e = threadIdx.x; e < plane; e += blockDim.x) W0[e] = Zp[e];
   __syncthreads();
   const LdsLayout lay{1, M, 0, 1};   // transform q = (c, n): elements q*M + m
   C<T>* R = lds_fft<T, false>(W0, W1, g.py, n3, lay, a.twy);
   C<T>* O = (R == W0) ? W1 : W0;
-  // one thread per (block row, ky): consecutive threads read consecutive rows
+  // One thread per (block row, ky): consecutive threads read consecutive rows
   // of a column-major block, so every load of the block stream is coalesced.
-  const C<T>* Bk = blocks + int64_t(kx) * M * n3 * n3;
+  // ky runs in mirror-pair order too, so B(kx, ky) and B(kx, M-ky) are read
+  // by neighbouring wavefronts.  Mirrored blocks come from the stored one as
+  // S B S: the x (y) component's row and column sums change sign.
+  const bool mx = g.bsym && 2 * kx > L;
+  const C<T>* Bk = blocks + int64_t(mx ? L - kx : kx) * g.Hy * n3 * n3;
   for (int e = threadIdx.x; e < plane; e += blockDim.x) {
-    const int ky = e / n3;
-    const int r = e - ky * n3;
-    const C<T>* Bb = Bk + int64_t(ky) * n3 * n3 + r;
-    C<T> a0 = czero<T>(), a1 = czero<T>();
-    int q = 0;
-    for (; q + 1 < n3; q += 2) {
-      a0 = cadd(a0, cmul(Bb[int64_t(q) * n3], R[q * M + ky]));
-      a1 = cadd(a1, cmul(Bb[int64_t(q + 1) * n3], R[(q + 1) * M + ky]));
+    const int i = e / n3;
+    const int r = e - i * n3;
+    const int jy = (i + 1) >> 1;
+    const int ky = i == 0 ? 0 : ((i & 1) ? jy : M - jy);
+    const bool my = g.bsym && 2 * ky > M;
+    const C<T>* Bb = Bk + int64_t(my ? M - ky : ky) * n3 * n3 + r;
+    C<T> sx = czero<T>(), sy = czero<T>(), sz = czero<T>();
+    for (int q = 0; q < N; ++q) {
+      sx = cadd(sx, cmul(Bb[int64_t(q) * n3], R[q * M + ky]));
+      sy = cadd(sy, cmul(Bb[int64_t(q + N) * n3], R[(q + N) * M + ky]));
+      sz = cadd(sz, cmul(Bb[int64_t(q + 2 * N) * n3], R[(q + 2 * N) * M + ky]));
     }
-    if (q < n3) a0 = cadd(a0, cmul(Bb[int64_t(q) * n3], R[q * M + ky]));
-    O[r * M + ky] = cadd(a0, a1);
+    if (mx) sx = cneg(sx);
+    if (my) sy = cneg(sy);
+    C<T> acc = cadd(cadd(sx, sy), sz);
+    const int cr = r / N;
+    if ((cr == 0 && mx) || (cr == 1 && my)) acc = cneg(acc);
+    O[r * M + ky] = acc;
   }
   __syncthreads();
   const C<T>* F = lds_fft<T, true>(O, R, g.py, n3, lay, a.twy);
@@ -289,16 +305,17 @@ __device__ __forceinline__ double2 pc_row(int k, int nt, int ns, int t, double2 
 // (the face-averaged form inv(Gram*I - diag(face3)*circ_N) of
 // Bragg_MOR_script.m:64-72, here with the exact axis last).
 __global__ void k_pc_blocks(const double2* __restrict__ ck, double2* __restrict__ blk, int L, int M, int N,
-                            const double2* __restrict__ twM, const double2* __restrict__ face, double gram) {
+                            int kxn, int kyn, const double2* __restrict__ twM, const double2* __restrict__ face,
+                            double gram) {
   const int n3 = 3 * N;
-  const int64_t total = int64_t(L) * M * n3 * n3;
+  const int64_t total = int64_t(kxn) * kyn * n3 * n3;
   for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
        e += int64_t(gridDim.x) * blockDim.x) {
-    // output layout: block (kx, ky) at (kx*M + ky) * n3^2, column-major (row fastest)
+    // output layout: block (kx, ky) at (kx*kyn + ky) * n3^2, column-major (row fastest)
     int64_t r = e;
     const int row = int(r % n3); r /= n3;
     const int colq = int(r % n3); r /= n3;
-    const int ky = int(r % M); r /= M;
+    const int ky = int(r % kyn); r /= kyn;
     const int kx = int(r);
     const int c = row / N, nt = row - c * N;
     const int d = colq / N, ns = colq - d * N;
@@ -427,7 +444,7 @@ __global__ void __launch_bounds__(256) k_pc_invert(double2* __restrict__ blk, in
 }
 
 void build_level2_blocks(const double2* G, int L, int M, int N, double2 mc, double diag, double2* blk,
-                         hipStream_t s, const double2* face_dev, double gram) {
+                         int kxn, int kyn, hipStream_t s, const double2* face_dev, double gram) {
   auto table = [](int P) {
     std::vector<double2> t(P);
     for (int k = 0; k < P; ++k) {
@@ -449,13 +466,60 @@ void build_level2_blocks(const double2* G, int L, int M, int N, double2 mc, doub
     VP_HIP(hipGetLastError());
   }
   {
-    const int64_t total = int64_t(L) * M * 9 * N * N;
+    const int64_t total = int64_t(kxn) * kyn * 9 * N * N;
     const int blocks = int(std::min<int64_t>((total + 255) / 256, 1 << 20));
-    hipLaunchKernelGGL(k_pc_blocks, dim3(blocks), dim3(256), 0, s, ck.as<double2>(), blk, L, M, N,
+    hipLaunchKernelGGL(k_pc_blocks, dim3(blocks), dim3(256), 0, s, ck.as<double2>(), blk, L, M, N, kxn, kyn,
                        dM.as<double2>(), face_dev, gram);
     VP_HIP(hipGetLastError());
   }
   VP_HIP(hipStreamSynchronize(s));
+}
+
+__global__ void k_parity_zero(const double2* __restrict__ G, int L, int M, int N, double* __restrict__ part) {
+  const int64_t V = int64_t(L) * M * N;
+  double zmax = 0.0, amax = 0.0;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < 6 * V;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int k = int(e / V);
+    const int64_t v = e - k * V;
+    const int l = int(v % L), m = int((v / L) % M), n = int(v / (int64_t(L) * M));
+    const double a = fabs(G[e].x) + fabs(G[e].y);
+    amax = fmax(amax, a);
+    const bool on = (k == 1 && (l == 0 || m == 0)) || (k == 2 && (l == 0 || n == 0)) || (k == 4 && (m == 0 || n == 0));
+    if (on) zmax = fmax(zmax, a);
+  }
+  __shared__ double sz[256], sa[256];
+  sz[threadIdx.x] = zmax;
+  sa[threadIdx.x] = amax;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) {
+      sz[threadIdx.x] = fmax(sz[threadIdx.x], sz[threadIdx.x + o]);
+      sa[threadIdx.x] = fmax(sa[threadIdx.x], sa[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sz[0];
+    part[2 * blockIdx.x + 1] = sa[0];
+  }
+}
+
+bool toeplitz_parity_zero(const double2* G, int L, int M, int N, hipStream_t s) {
+  const int64_t total = int64_t(6) * L * M * N;
+  const int blocks = int(std::min<int64_t>((total + 255) / 256, 2048));
+  DevBuf part(size_t(2) * blocks * sizeof(double));
+  hipLaunchKernelGGL(k_parity_zero, dim3(blocks), dim3(256), 0, s, G, L, M, N, part.as<double>());
+  VP_HIP(hipGetLastError());
+  std::vector<double> h(size_t(2) * blocks);
+  VP_HIP(hipMemcpyAsync(h.data(), part.p, part.bytes, hipMemcpyDeviceToHost, s));
+  VP_HIP(hipStreamSynchronize(s));
+  double z = 0.0, a = 0.0;
+  for (int b = 0; b < blocks; ++b) {
+    z = std::max(z, h[2 * b]);
+    a = std::max(a, h[2 * b + 1]);
+  }
+  return std::isfinite(a) && z <= 1e-12 * a;
 }
 
 void invert_blocks(double2* blk, int64_t nblocks, int n, hipStream_t s) {

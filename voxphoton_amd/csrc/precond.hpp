@@ -11,6 +11,10 @@ struct PcGeom {
   int level;            // 1 or 2
   int64_t n_outer;      // length of the vector the preconditioner acts on (embedding)
   int B, ld, tx, t2;    // pencil batch, LDS row stride, threads (x pass / plane)
+  // 2-level blocks stored for kx < Hx, ky < Hy.  bsym = 1: Hx = L/2+1,
+  // Hy = M/2+1 and B(L-kx, ky) = Sx B(kx, ky) Sx, B(kx, M-ky) = Sy B Sy with
+  // Sx (Sy) negating the x (y) component; bsym = 0: Hx = L, Hy = M.
+  int bsym, Hx, Hy;
   size_t lds1, lds2;
   FftPlan px, py;       // unpadded DFTs of length L and M
 };
@@ -49,8 +53,13 @@ void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s);
 // diag folded at the origin).  face_dev = device complex[N]: blocks are
 // Gram*I - diag(face(n))*N2 with N2 the bare 2-level circulant of G (pass
 // mc = -1, diag = 0).
+// Blocks are built for kx < kxn, ky < kyn at (kx * kyn + ky).
 void build_level2_blocks(const double2* G, int L, int M, int N, double2 mc, double diag, double2* blk,
-                         hipStream_t s, const double2* face_dev = nullptr, double gram = 0.0);
+                         int kxn, int kyn, hipStream_t s, const double2* face_dev = nullptr, double gram = 0.0);
+// True when the odd components of a device Toeplitz tensor (xy, xz, yz)
+// vanish on their zero-offset planes (to 1e-12 of max |G|), which makes the
+// Chan circulants parity symmetric.
+bool toeplitz_parity_zero(const double2* G, int L, int M, int N, hipStream_t s);
 void invert_blocks(double2* blk, int64_t nblocks, int n, hipStream_t s);
 
 }  // namespace vp
