@@ -250,6 +250,65 @@ __device__ __forceinline__ void stage(const C<T>* src, C<T>* dst, int n, int Ns,
   }
 }
 
+// Batch-fastest variant: element j of transform beta sits at beta + j * ld
+// and consecutive threads take consecutive transforms, so every LDS access
+// of a wavefront (same j, consecutive beta) is a contiguous run of words.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void stage_bf(const C<T>* src, C<T>* dst, int n, int Ns, int nbatch, int ld,
+                                         const C<T>* __restrict__ tw, int twmul) {
+  const int nb = n / R;
+  const int total = nb * nbatch;
+  const IDiv dnbatch(nbatch), dns(Ns);
+  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+    const int j = dnbatch.div(b);
+    const int beta = b - j * nbatch;
+    C<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[beta + (j + r * nb) * ld];
+    const int jm = dns.mod(j);
+    if (Ns > 1) {
+      const int step = jm * twmul;
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<T, INV>(tw, r * step));
+    }
+    dft<T, R, INV>(v);
+    const int base = (j - jm) * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[beta + (base + r * Ns) * ld] = v[r];
+  }
+}
+
+template <typename T, bool INV>
+__device__ C<T>* lds_fft_bf(C<T>* a, C<T>* b, const FftPlan& pl, int nbatch, int ld, const C<T>* __restrict__ tw) {
+  int Ns = 1;
+  C<T>* src = a;
+  C<T>* dst = b;
+  for (int s = 0; s < pl.nst; ++s) {
+    const int R = pl.radix(s);
+    const int twmul = (pl.n / (Ns * R)) * pl.tws;
+    switch (R) {
+      case 2: stage_bf<T, 2, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
+      case 3: stage_bf<T, 3, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
+      case 4: stage_bf<T, 4, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
+      case 5: stage_bf<T, 5, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
+      case 7: stage_bf<T, 7, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
+      case 8: stage_bf<T, 8, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
+      case 11: stage_bf<T, 11, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
+      case 13:
+        if constexpr (sizeof(T) == 4) stage_bf<T, 13, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul);
+        break;
+      case 16:
+        if constexpr (sizeof(T) == 4) stage_bf<T, 16, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul);
+        break;
+      default: break;
+    }
+    __syncthreads();
+    C<T>* t = src; src = dst; dst = t;
+    Ns *= R;
+  }
+  return src;
+}
+
 // Runs all stages of ``pl`` on ``nbatch`` transforms laid out per ``lay``,
 // ping-ponging between ``a`` and ``b``; returns the buffer holding the
 // result.  Ends with a barrier.  Radices 13 and 16 are single precision only.

@@ -260,20 +260,18 @@ __global__ void __launch_bounds__(256) k_zpass(const ProbPtrs<T>* __restrict__ p
   for (int e = threadIdx.x; e < Np; e += blockDim.x) tw[e] = twz[e];
   const IDiv dTz(Tz), dN(N);
   const int64_t base = int64_t(kxi) * 3 * N * Mp;   // Y[kx][c][n][ky]
-  // load: W0[(c*Tz + t)*Np + n] = Y[kx][c][n][ky0 + t], zero for n >= N
+  // LDS holds the 3*Tz columns batch-fastest: element n of column q = c*Tz + t
+  // at n*ld + q, so loads, FFT stages and the multiply touch contiguous words.
+  const int ld = 3 * Tz;
   for (int e = threadIdx.x; e < 3 * Tz * N; e += blockDim.x) {
     const int cn = dTz.div(e), t = e - cn * Tz;
     const int c = dN.div(cn), n = cn - c * N;
     const int ky = ky0 + t;
-    W0[(c * Tz + t) * Np + n] = ky < Mp ? P.work2[base + (int64_t(c) * N + n) * Mp + ky] : czero<T>();
+    W0[n * ld + c * Tz + t] = ky < Mp ? P.work2[base + (int64_t(c) * N + n) * Mp + ky] : czero<T>();
   }
-  for (int e = threadIdx.x; e < 3 * Tz * (Np - N); e += blockDim.x) {
-    const int q = e / (Np - N), n = N + (e - q * (Np - N));
-    W0[q * Np + n] = czero<T>();
-  }
+  for (int e = threadIdx.x; e < ld * (Np - N); e += blockDim.x) W0[N * ld + e] = czero<T>();
   __syncthreads();
-  const LdsLayout lay{1, Np, 0, 1};
-  C<T>* R = lds_fft<T, false>(W0, W1, g.z.plan, 3 * Tz, lay, tw);
+  C<T>* R = lds_fft_bf<T, false>(W0, W1, g.z.plan, ld, ld, tw);
   // tensor multiply: fN is [kx][kz][ky][6] (class layout with Ry = Rz = 1) or,
   // parity compressed, the octant [kxc][kzc][kyc][6] with the signs of the
   // odd components (xy: x,y; xz: x,z; yz: y,z) applied on mirrored axes.
@@ -298,7 +296,7 @@ __global__ void __launch_bounds__(256) k_zpass(const ProbPtrs<T>* __restrict__ p
         const C<T>* f = F + (int64_t(kz) * Mp + ky) * 6;
         f0 = f[0]; f1 = f[1]; f2 = f[2]; f3 = f[3]; f4 = f[4]; f5 = f[5];
       }
-      const int i0 = (0 * Tz + t) * Np + kz, i1 = (1 * Tz + t) * Np + kz, i2 = (2 * Tz + t) * Np + kz;
+      const int i0 = kz * ld + t, i1 = i0 + Tz, i2 = i0 + 2 * Tz;
       const C<T> a = R[i0], b = R[i1], cc = R[i2];
       R[i0] = cadd(cadd(cmul(f0, a), cmul(f1, b)), cmul(f2, cc));
       R[i1] = cadd(cadd(cmul(f1, a), cmul(f3, b)), cmul(f4, cc));
@@ -306,12 +304,161 @@ __global__ void __launch_bounds__(256) k_zpass(const ProbPtrs<T>* __restrict__ p
     }
   }
   __syncthreads();
-  const C<T>* Q = lds_fft<T, true>(R, R == W0 ? W1 : W0, g.z.plan, 3 * Tz, lay, tw);
+  const C<T>* Q = lds_fft_bf<T, true>(R, R == W0 ? W1 : W0, g.z.plan, ld, ld, tw);
   for (int e = threadIdx.x; e < 3 * Tz * N; e += blockDim.x) {
     const int cn = dTz.div(e), t = e - cn * Tz;
     const int c = dN.div(cn), n = cn - c * N;
     const int ky = ky0 + t;
-    if (ky < Mp) P.work2[base + (int64_t(c) * N + n) * Mp + ky] = Q[(c * Tz + t) * Np + n];
+    if (ky < Mp) P.work2[base + (int64_t(c) * N + n) * Mp + ky] = Q[n * ld + c * Tz + t];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Register-resident z pass for Np = R1 * R2.  One wavefront per component,
+// TC = 64 / R1 ky columns per wavefront, column spread over R1 lanes: lane
+// (l, t) holds x[R1*a + l], a < R2, of column ky0 + t.  Forward transform:
+// R2-point DFT in registers, twiddle w^{l k1}, then a radix-2 decimation-in-
+// frequency network across the R1 lanes (shuffles), which leaves
+// X[k1 + R2*brev(l)] in lane l.  The three components meet in LDS for the
+// tensor multiply; the inverse runs the mirror image (decimation-in-time
+// network from bit-reversed order, conjugate twiddles, inverse R2 DFT) and
+// stores rows n < N.  Twiddles come from the Np-entry table (LDS copy).
+// ----------------------------------------------------------------------------
+template <int R1> __device__ __forceinline__ int brev_lane(int l) {
+  int r = 0;
+#pragma unroll
+  for (int b = 1; b < R1; b <<= 1) r = (r << 1) | ((l & b) ? 1 : 0);
+  return r;
+}
+
+template <typename T> __device__ __forceinline__ C<T> shfl_xor_c(C<T> v, int mask) {
+  C<T> o;
+  o.x = __shfl_xor(v.x, mask);
+  o.y = __shfl_xor(v.y, mask);
+  return o;
+}
+
+// f_{c,d}(kx, ky, kz) for d = 0..2 (row c of the symmetric tensor).
+template <typename T>
+__device__ __forceinline__ void tensor_row(const ProbPtrs<T>& P, const MatvecPlan& mp, int c, int kxi, bool mx,
+                                           int kxc, int ky, int kz, int Mp, int Np, C<T>& r0, C<T>& r1, C<T>& r2) {
+  const C<T>* f;
+  bool sxy = false, sxz = false, syz = false;
+  if (mp.fsym) {
+    const bool my = 2 * ky > Mp, mz = 2 * kz > Np;
+    const int Hy = Mp / 2 + 1, Hz = Np / 2 + 1;
+    f = P.fN + ((int64_t(kxc) * Hz + (mz ? Np - kz : kz)) * Hy + (my ? Mp - ky : ky)) * 6;
+    sxy = mx != my;
+    sxz = mx != mz;
+    syz = my != mz;
+  } else {
+    f = P.fN + ((int64_t(kxi) * Np + kz) * Mp + ky) * 6;
+  }
+  if (c == 0) {
+    r0 = f[0]; r1 = f[1]; r2 = f[2];
+    if (sxy) r1 = cneg(r1);
+    if (sxz) r2 = cneg(r2);
+  } else if (c == 1) {
+    r0 = f[1]; r1 = f[3]; r2 = f[4];
+    if (sxy) r0 = cneg(r0);
+    if (syz) r2 = cneg(r2);
+  } else {
+    r0 = f[2]; r1 = f[4]; r2 = f[5];
+    if (sxz) r0 = cneg(r0);
+    if (syz) r1 = cneg(r1);
+  }
+}
+
+template <typename T, int R1, int R2>
+__global__ void __launch_bounds__(192) k_zcol(const ProbPtrs<T>* __restrict__ probs, const int* __restrict__ active,
+                                              Geometry g, MatvecPlan mp, const C<T>* __restrict__ twz) {
+  constexpr int TC = 64 / R1;
+  constexpr int Np = R1 * R2;
+  constexpr bool PREFETCH = sizeof(T) == 4;   // FP32 keeps the tensor row in registers across the FFT
+  if (active && !active[blockIdx.y]) return;
+  const ProbPtrs<T> P = probs[blockIdx.y];
+  __shared__ C<T> S[3][Np][TC];
+  __shared__ C<T> tw[Np];
+  for (int e = threadIdx.x; e < Np; e += blockDim.x) tw[e] = twz[e];
+  const int N = g.N, Mp = g.y.P, Lp = g.x.P;
+  const int nt = (Mp + TC - 1) / TC;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int io = blk / nt;
+  const int jo = (io + 1) >> 1;
+  const int kx = io == 0 ? 0 : ((io & 1) ? jo : Lp - jo);   // kx paired with Lp - kx (shared tensor octant)
+  const int kxi = (kx % g.x.R) * g.x.S + kx / g.x.R;        // residue-class storage index of kx
+  const bool mx = 2 * kx > Lp;
+  const int kxc = mx ? Lp - kx : kx;
+  const int ky0 = (blk - io * nt) * TC;
+  const int c = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int l = lane / TC, t = lane - l * TC;
+  const int ky = ky0 + t;
+  const bool valid = ky < Mp;
+  const int k2 = brev_lane<R1>(l);
+  C<T>* Y = P.work2 + (int64_t(kxi) * 3 + c) * N * int64_t(Mp);
+  C<T> fr[PREFETCH ? R2 : 1][3];
+  if constexpr (PREFETCH) {
+#pragma unroll
+    for (int k1 = 0; k1 < R2; ++k1) {
+      if (valid) tensor_row(P, mp, c, kxi, mx, kxc, ky, k1 + R2 * k2, Mp, Np, fr[k1][0], fr[k1][1], fr[k1][2]);
+      else fr[k1][0] = fr[k1][1] = fr[k1][2] = czero<T>();
+    }
+  }
+  C<T> v[R2];
+#pragma unroll
+  for (int a = 0; a < R2; ++a) {
+    const int n = R1 * a + l;
+    v[a] = (valid && n < N) ? Y[int64_t(n) * Mp + ky] : czero<T>();
+  }
+  __syncthreads();   // twiddle table
+  dft<T, R2, false>(v);
+#pragma unroll
+  for (int k1 = 1; k1 < R2; ++k1) v[k1] = cmul(v[k1], tw[(l * k1) % Np]);
+#pragma unroll
+  for (int h = R1 / 2; h >= 1; h >>= 1) {
+    const bool upper = (l & h) != 0;
+    const C<T> w = tw[(l & (h - 1)) * (Np / (2 * h))];
+#pragma unroll
+    for (int k1 = 0; k1 < R2; ++k1) {
+      const C<T> o = shfl_xor_c<T>(v[k1], h * TC);
+      v[k1] = upper ? cmul(csub(o, v[k1]), w) : cadd(v[k1], o);
+    }
+  }
+  // lane l holds X[k1 + R2 * k2]; components meet in LDS
+#pragma unroll
+  for (int k1 = 0; k1 < R2; ++k1) S[c][k1 + R2 * k2][t] = v[k1];
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < R2; ++k1) {
+    const int kz = k1 + R2 * k2;
+    C<T> f0, f1, f2;
+    if constexpr (PREFETCH) {
+      f0 = fr[k1][0]; f1 = fr[k1][1]; f2 = fr[k1][2];
+    } else {
+      if (valid) tensor_row(P, mp, c, kxi, mx, kxc, ky, kz, Mp, Np, f0, f1, f2);
+      else f0 = f1 = f2 = czero<T>();
+    }
+    v[k1] = cadd(cadd(cmul(f0, S[0][kz][t]), cmul(f1, S[1][kz][t])), cmul(f2, S[2][kz][t]));
+  }
+  // inverse: DIT network across lanes (bit-reversed in, natural out)
+#pragma unroll
+  for (int h = 1; h < R1; h <<= 1) {
+    const bool upper = (l & h) != 0;
+    const C<T> w = cconj(tw[(l & (h - 1)) * (Np / (2 * h))]);
+#pragma unroll
+    for (int k1 = 0; k1 < R2; ++k1) {
+      const C<T> o = shfl_xor_c<T>(v[k1], h * TC);
+      v[k1] = upper ? csub(o, cmul(w, v[k1])) : cadd(v[k1], cmul(w, o));
+    }
+  }
+#pragma unroll
+  for (int k1 = 1; k1 < R2; ++k1) v[k1] = cmulc(v[k1], tw[(l * k1) % Np]);
+  dft<T, R2, true>(v);
+#pragma unroll
+  for (int a = 0; a < R2; ++a) {
+    const int n = R1 * a + l;
+    if (valid && n < N) Y[int64_t(n) * Mp + ky] = v[a];
   }
 }
 
@@ -439,8 +586,23 @@ void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d
     }
     {
       ProfScope ps(s, KID_ZPASS);
-      const dim3 gz(unsigned(g.x.P * ((g.y.P + mp.Tz - 1) / mp.Tz)), nprob);
-      hipLaunchKernelGGL(k_zpass<T>, gz, dim3(mp.tz), mp.lds_z, s, d_probs, va.active, g, mp, twz);
+      if (mp.zr1 > 0) {
+        const dim3 gz(unsigned(g.x.P * ((g.y.P + 64 / mp.zr1 - 1) / (64 / mp.zr1))), nprob);
+#define VP_ZCOL(A, B)                                                                                   \
+  if (mp.zr1 == A && mp.zr2 == B) {                                                                     \
+    hipLaunchKernelGGL((k_zcol<T, A, B>), gz, dim3(192), 0, s, d_probs, va.active, g, mp, twz);           \
+    zdone = true;                                                                                       \
+  }
+        bool zdone = false;
+        VP_ZCOL(8, 11) VP_ZCOL(8, 7) VP_ZCOL(8, 5) VP_ZCOL(8, 3) VP_ZCOL(8, 8) VP_ZCOL(8, 4)
+        VP_ZCOL(4, 11) VP_ZCOL(4, 7) VP_ZCOL(4, 5) VP_ZCOL(4, 3) VP_ZCOL(4, 8) VP_ZCOL(4, 4)
+        VP_ZCOL(16, 11) VP_ZCOL(16, 7) VP_ZCOL(16, 5) VP_ZCOL(16, 3) VP_ZCOL(16, 8) VP_ZCOL(16, 4)
+#undef VP_ZCOL
+        if (!zdone) throw VpError{ERR_UNSUPPORTED, "matvec: no register z-column kernel for this length"};
+      } else {
+        const dim3 gz(unsigned(g.x.P * ((g.y.P + mp.Tz - 1) / mp.Tz)), nprob);
+        hipLaunchKernelGGL(k_zpass<T>, gz, dim3(mp.tz), mp.lds_z, s, d_probs, va.active, g, mp, twz);
+      }
     }
     {
       ProfScope ps(s, KID_YINV);

@@ -91,6 +91,20 @@ bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out,
     if (out.lds_y > size_t(lds_max) || out.lds_z > size_t(lds_max)) { why = "split passes exceed LDS"; return false; }
     out.ty = 256;
     out.tz = 256;
+    // register z-column kernel (VP_ZCOL=1): Np = lanes x in-register radix; the LDS pass is the default
+    out.zr1 = out.zr2 = 0;
+    const char* ez = std::getenv("VP_ZCOL");
+    if (ez && ez[0] == '1') {
+      for (int r1 : {8, 4, 16}) {
+        if (Np % r1) continue;
+        const int r2 = Np / r1;
+        if (r2 == 3 || r2 == 4 || r2 == 5 || r2 == 7 || r2 == 8 || r2 == 11) {
+          out.zr1 = r1;
+          out.zr2 = r2;
+          break;
+        }
+      }
+    }
     out.t2 = 0;
     out.ept2 = 1;
     out.lds2 = 0;

@@ -46,6 +46,7 @@ struct MatvecPlan {
   // split variant: y passes (By pencils of Mp per workgroup), z pass (Tz ky columns)
   int By, ldy, ty, Tz, tz;
   int fsym;      // split variant: fN stored as its parity octant
+  int zr1, zr2;  // split variant: register z-column kernel Np = zr1 * zr2 (zr1 = 0: LDS z pass)
   size_t lds_y, lds_z;
   // K1/K3 (x pencils)
   int B;         // pencils per workgroup
