@@ -235,6 +235,47 @@ int vp_solve_ir_batched(const vp_op* ops, const vp_prec* precs, int nsolve, int 
                         double tol, int restart, double inner_tol, int inner_maxit, int max_outer,
                         vp_gmres_info* infos);
 
+/* ---- request server (continuous batching) ----------------------------------
+ * No counterpart in the reference, whose scripts solve one wavelength after
+ * another (Bragg_MOR_script.m:88-95, example_Bragg_1500nm_to_1600nm.m:85-114).
+ * A server owns one HIP stream on the context's device and a worker thread.
+ * Submitted mixed-precision solves (vp_solve_ir semantics) wait in a FIFO
+ * queue and are admitted while the server's HBM budget (solver state of the
+ * admitted requests: Krylov basis, refinement vectors, batch workspaces) and
+ * its slot count allow.  Every round, the worker groups the admitted
+ * requests by operator geometry and solve parameters and runs one
+ * refinement step for each group as one batched launch sequence; requests
+ * admitted while others are running join their group at the next round, and
+ * finished requests free their memory at once.  A request's arithmetic does
+ * not depend on its group: results are identical alone and in the mix.
+ * b and x are device complex128 vectors of the operator's length; the caller
+ * keeps b, x, the operator and the preconditioner alive and does not use
+ * them until the request is done.
+ */
+typedef struct vp_server_s* vp_server;
+typedef struct {
+  double tol;         /* outer FP64 relative residual target */
+  int restart;        /* inner GMRES restart length */
+  double inner_tol;   /* inner relative tolerance */
+  int inner_maxit;    /* inner restart cycles per refinement step */
+  int max_outer;      /* refinement steps */
+  int side;           /* VP_PREC_LEFT or VP_PREC_RIGHT */
+} vp_solve_params;
+int vp_server_create(vp_ctx ctx, int64_t hbm_budget_bytes, int max_active, vp_server* out);
+/* Queues a solve; returns at once with a ticket (> 0). */
+int vp_server_submit(vp_server s, vp_op op, vp_prec M, const void* b, void* x, const vp_solve_params* prm,
+                     int64_t* ticket);
+/* done = 1 when the request has finished; info (nullable) then holds flag,
+ * relres, ir_steps, total_iters and seconds = submit-to-finish latency
+ * (resvec: relative residual at the start of every round). */
+int vp_server_poll(vp_server s, int64_t ticket, int* done, vp_gmres_info* info);
+/* Blocks until the request has finished; releases the ticket. */
+int vp_server_wait(vp_server s, int64_t ticket, vp_gmres_info* info);
+/* Queue length, admitted requests, admitted bytes and rounds run so far. */
+int vp_server_stats(vp_server s, int64_t* pending, int64_t* active, int64_t* bytes_in_use, int64_t* rounds);
+/* Finishes every queued request, then stops the worker. */
+int vp_server_destroy(vp_server s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,126 @@
+"""Request server (continuous batching): every result equals the solo solve,
+whatever was running beside it; the HBM budget limits admission."""
+import time
+
+import numpy as np
+import pytest
+
+from tests.helpers import rel
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = dict(tol=1e-8, restart=20, inner_tol=1e-3, inner_maxit=3, max_outer=15, side='right')
+
+
+def _requests(vp_ctx):
+    """Three geometries (two waveguide lengths, one short grating), two
+    wavelengths each, y- and z-polarised right-hand sides."""
+    import dataclasses
+    from voxphoton_amd import configs, vie
+    bases = [configs.straight_waveguide(L=24, M=14, N=10, dx=40e-9),
+             configs.straight_waveguide(L=30, M=14, N=10, dx=40e-9),
+             configs.bragg_grating(periods=3, dx=40e-9, len_abs=0.4e-6, M=16, N=10)]
+    reqs = []
+    for base in bases:
+        for lam in (1520e-9, 1580e-9):
+            prob = dataclasses.replace(base, lam=lam)
+            op = vie.NOperator.assembled(prob.shape, prob.dx, prob.ko, None, prob.Mc, prob.idx, ctx=vp_ctx)
+            shape, bidx, sub = prob.box_problem()
+            P = vie.Preconditioner.level2_assembled(shape, prob.dx, prob.ko, prob.core_contrast(), 1.0, bidx,
+                                                    ctx=vp_ctx).set_embedding(prob.n, sub)
+            for pol in ('y', 'z'):
+                b = prob.gaussian_rhs(pol)
+                db = vie.DeviceArray(vp_ctx, b.nbytes)
+                db.upload(b)
+                reqs.append((op, P, db, b.size))
+    return reqs
+
+
+def _solo(vp_ctx, reqs):
+    import ctypes
+    from voxphoton_amd import _lib as L, vie
+    out = []
+    for op, P, db, n in reqs:
+        dx = vie.DeviceArray(vp_ctx, n * 16)
+        info = L.GmresInfo()
+        L.check(L.lib.vp_solve_ir(op.handle, P.handle, L.VP_PREC_RIGHT, db.ptr, dx.ptr, L.VP_MEM_DEVICE,
+                                  PARAMS['tol'], PARAMS['restart'], PARAMS['inner_tol'], PARAMS['inner_maxit'],
+                                  PARAMS['max_outer'], ctypes.byref(info)))
+        x = np.empty(n, complex)
+        dx.download(x)
+        out.append((x, info.flag, info.relres))
+    return out
+
+
+def test_server_results_equal_solo_solves(vp_ctx):
+    from voxphoton_amd import vie
+    reqs = _requests(vp_ctx)
+    solo = _solo(vp_ctx, reqs)
+    srv = vie.Server(hbm_budget_bytes=8 << 30, max_active=16, ctx=vp_ctx)
+    try:
+        xs = [vie.DeviceArray(vp_ctx, n * 16) for (_, _, _, n) in reqs]
+        tickets = []
+        # the first half starts alone; the second half arrives while it runs
+        half = len(reqs) // 2
+        for i in range(half):
+            op, P, db, n = reqs[i]
+            tickets.append(srv.submit(op, P, db, xs[i], **PARAMS))
+        t0 = time.time()
+        while srv.stats()['rounds'] < 2 and time.time() - t0 < 30:
+            time.sleep(0.002)
+        for i in range(half, len(reqs)):
+            op, P, db, n = reqs[i]
+            tickets.append(srv.submit(op, P, db, xs[i], **PARAMS))
+        infos = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+        print('server stats', st, flush=True)
+        assert st['pending'] == 0 and st['active'] == 0 and st['bytes_in_use'] == 0
+        for i, (info, (x_solo, flag, relres)) in enumerate(zip(infos, solo)):
+            x = np.empty_like(x_solo)
+            xs[i].download(x)
+            print(i, info['relres'], relres, info['ir_steps'], rel(x, x_solo), flush=True)
+            assert info['flag'] == 0 and flag == 0
+            assert rel(x, x_solo) <= 1e-12          # same result alone and in the mix
+            assert info['relres'] == relres
+    finally:
+        srv.close()
+
+
+def test_server_budget_admits_one_at_a_time(vp_ctx):
+    from voxphoton_amd import vie
+    reqs = _requests(vp_ctx)[:4]
+    solo = _solo(vp_ctx, reqs)
+    srv = vie.Server(hbm_budget_bytes=1, max_active=16, ctx=vp_ctx)   # every request is oversize: one at a time
+    try:
+        xs = [vie.DeviceArray(vp_ctx, n * 16) for (_, _, _, n) in reqs]
+        tickets = [srv.submit(op, P, db, xs[i], **PARAMS) for i, (op, P, db, n) in enumerate(reqs)]
+        seen_active = 0
+        while True:
+            st = srv.stats()
+            seen_active = max(seen_active, st['active'])
+            if st['pending'] == 0 and st['active'] == 0:
+                break
+            time.sleep(0.001)
+        assert seen_active <= 1
+        for i, t in enumerate(tickets):
+            info = srv.wait(t)
+            x = np.empty(reqs[i][3], complex)
+            xs[i].download(x)
+            assert info['flag'] == 0 and rel(x, solo[i][0]) <= 1e-12
+    finally:
+        srv.close()
+
+
+def test_server_rejects_mismatched_preconditioner(vp_ctx):
+    from voxphoton_amd import vie
+    from voxphoton_amd._lib import VoxPhotonError
+    reqs = _requests(vp_ctx)
+    op_a = reqs[0][0]
+    P_b = reqs[4][1]          # preconditioner of another geometry
+    srv = vie.Server(hbm_budget_bytes=1 << 30, ctx=vp_ctx)
+    try:
+        x = vie.DeviceArray(vp_ctx, reqs[0][3] * 16)
+        with pytest.raises(VoxPhotonError):
+            srv.submit(op_a, P_b, reqs[0][2], x, **PARAMS)
+    finally:
+        srv.close()

@@ -44,6 +44,11 @@ class GmresInfo(ctypes.Structure):
                 ('ir_steps', c_int), ('seconds', c_dbl)]
 
 
+class SolveParams(ctypes.Structure):
+    _fields_ = [('tol', c_dbl), ('restart', c_int), ('inner_tol', c_dbl), ('inner_maxit', c_int),
+                ('max_outer', c_int), ('side', c_int)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     'vp_last_error': (ctypes.c_char_p, []),
@@ -87,6 +92,12 @@ SIGNATURES = {
                             P(GmresInfo)]),
     'vp_solve_ir_batched': (c_int, [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_dbl, c_int, c_dbl, c_int, c_int,
                                     P(GmresInfo)]),
+    'vp_server_create': (c_int, [c_vp, c_i64, c_int, P(c_vp)]),
+    'vp_server_submit': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, P(SolveParams), P(c_i64)]),
+    'vp_server_poll': (c_int, [c_vp, c_i64, P(c_int), P(GmresInfo)]),
+    'vp_server_wait': (c_int, [c_vp, c_i64, P(GmresInfo)]),
+    'vp_server_stats': (c_int, [c_vp, P(c_i64), P(c_i64), P(c_i64), P(c_i64)]),
+    'vp_server_destroy': (c_int, [c_vp]),
 }
 
 missing = []

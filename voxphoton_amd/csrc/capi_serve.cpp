@@ -1,0 +1,359 @@
+// C ABI: request server with continuous batching of mixed-precision solves.
+//
+// A worker thread owns one HIP stream.  Requests wait in a FIFO queue and are
+// admitted while the HBM budget (their solver state) and the slot count
+// allow; an oversize request is admitted alone.  Each round the admitted
+// requests are grouped by operator geometry and solve parameters, and every
+// group runs one refinement step (ir_step_set: FP64 residual, inner FP32
+// GMRES, update) as one batched launch sequence on gathered b/x vectors.
+// Requests admitted later join their group at the next round; finished
+// requests leave and free their budget at once.  Per-problem arithmetic in
+// the batched kernels never depends on the other problems of a launch, so a
+// request's result is the same alone and in any mix.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+#include "krylov.hpp"
+
+namespace vp {
+void set_error(const std::string& msg);
+}
+using namespace vp;
+
+#define VP_API_BEGIN try {
+#define VP_API_END                       \
+  }                                      \
+  catch (const VpError& e) {             \
+    set_error(e.msg);                    \
+    return e.code;                       \
+  }                                      \
+  catch (const std::bad_alloc&) {        \
+    set_error("host allocation failed"); \
+    return VP_ERR_OOM;                   \
+  }                                      \
+  catch (const std::exception& e) {      \
+    set_error(e.what());                 \
+    return VP_ERR_DEVICE;                \
+  }                                      \
+  catch (...) {                          \
+    set_error("unknown error");          \
+    return VP_ERR_DEVICE;                \
+  }                                      \
+  set_error("");                         \
+  return VP_OK;
+
+namespace {
+using Clock = std::chrono::steady_clock;
+
+struct ServeReq {
+  int64_t ticket = 0;
+  vp_op_s* op = nullptr;
+  vp_prec_s* prec = nullptr;
+  const double2* b = nullptr;
+  double2* x = nullptr;
+  vp_solve_params prm{};
+  int64_t bytes = 0;
+  std::string key;
+  IrProblem st;
+  bool finished = false;
+  int error = 0;
+  std::string errmsg;
+  Clock::time_point t_submit, t_done;
+};
+
+// Solver-state bytes of one request when it runs in a batch: Krylov basis
+// and GMRES vectors (FP32), refinement and gathered vectors (FP64), and the
+// per-problem matvec / preconditioner workspaces of a batched launch.
+int64_t request_bytes(const vp_op_s* op, const vp_prec_s* pc, const vp_solve_params& prm) {
+  const int64_t n = op->n;
+  int64_t b = int64_t(prm.restart + 1 + 6) * n * 8 + int64_t(6) * n * 16;
+  const int64_t w = int64_t(3) * op->Lp * op->Mb * op->Nb;
+  const int64_t w2 = op->p32.plan.variant == 1 ? int64_t(3) * op->Lp * op->Mp * op->Nb : 0;
+  b += (w + w2) * (16 + 8);
+  if (pc) b += int64_t(3) * pc->g.V * (16 + 8) * (pc->g.level == 1 ? 2 : 1);
+  return b;
+}
+
+std::string group_key(const vp_op_s* o, const vp_prec_s* q, const vp_solve_params& p) {
+  char buf[512];
+  std::snprintf(buf, sizeof(buf), "%d %d %d|%d %d %d %d %d %d|%lld %d %d %d %d %.17g|", o->Lp, o->Mp, o->Np, o->l0,
+                o->m0, o->n0, o->Lb, o->Mb, o->Nb, static_cast<long long>(o->n), o->structured,
+                o->p64.plan.variant, o->p32.plan.variant, o->p32.plan.fsym, o->scale32);
+  std::string k(buf);
+  if (q)
+    std::snprintf(buf, sizeof(buf), "P%d %d %d %lld %d %lld %d|", q->g.L, q->g.M, q->g.N,
+                  static_cast<long long>(q->nbox), q->g.level, static_cast<long long>(q->g.n_outer), q->g.bsym);
+  else
+    std::snprintf(buf, sizeof(buf), "P-|");
+  k += buf;
+  std::snprintf(buf, sizeof(buf), "%.17g %d %.17g %d %d %d", p.tol, p.restart, p.inner_tol, p.inner_maxit,
+                p.max_outer, p.side);
+  return k + buf;
+}
+}  // namespace
+
+struct vp_server_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t budget = 0;
+  int max_active = 1;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<ServeReq*> pending;
+  std::vector<ServeReq*> active;
+  std::map<int64_t, std::unique_ptr<ServeReq>> reqs;
+  int64_t used = 0, next_ticket = 1, rounds = 0;
+  bool stop = false;
+  std::thread worker;
+};
+
+namespace {
+
+// One refinement step for a group of admitted requests of one geometry.
+void run_round(vp_server_s* S, const std::vector<ServeReq*>& g) {
+  const int np = int(g.size());
+  vp_op_s* o0 = g[0]->op;
+  const int64_t n = o0->n;
+  const vp_solve_params& prm = g[0]->prm;
+  std::vector<vp_op_s*> ops(np);
+  std::vector<vp_prec_s*> precs(np);
+  for (int p = 0; p < np; ++p) {
+    ops[p] = g[p]->op;
+    precs[p] = g[p]->prec;
+  }
+  ProblemSet<double> p64;
+  make_problem_set<double>(p64, ops.data(), nullptr, np);
+  p64.stream = S->stream;
+  ProblemSet<float> p32;
+  make_problem_set<float>(p32, ops.data(), precs[0] ? precs.data() : nullptr, np);
+  p32.stream = S->stream;
+  DevBuf B(size_t(n) * np * sizeof(double2)), X(size_t(n) * np * sizeof(double2));
+  for (int p = 0; p < np; ++p) {
+    VP_HIP(hipMemcpyAsync(B.as<double2>() + p * n, g[p]->b, n * sizeof(double2), hipMemcpyDeviceToDevice,
+                          S->stream));
+    VP_HIP(hipMemcpyAsync(X.as<double2>() + p * n, g[p]->x, n * sizeof(double2), hipMemcpyDeviceToDevice,
+                          S->stream));
+  }
+  std::vector<IrProblem> st(np);
+  for (int p = 0; p < np; ++p) st[p] = g[p]->st;
+  ir_step_set(p64, p32, B.as<double2>(), X.as<double2>(), o0->scale32, prm.tol, prm.restart, prm.inner_tol,
+              prm.inner_maxit, prm.max_outer, prm.side, st);
+  for (int p = 0; p < np; ++p) {
+    VP_HIP(hipMemcpyAsync(g[p]->x, X.as<double2>() + p * n, n * sizeof(double2), hipMemcpyDeviceToDevice,
+                          S->stream));
+    g[p]->st = st[p];
+  }
+  VP_HIP(hipStreamSynchronize(S->stream));
+}
+
+void serve_loop(vp_server_s* S) {
+  (void)hipSetDevice(S->device);
+  for (;;) {
+    std::vector<ServeReq*> run, fresh;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->active.empty(); });
+      if (S->pending.empty() && S->active.empty()) return;   // stop requested and drained
+      while (!S->pending.empty() && int(S->active.size()) < S->max_active) {
+        ServeReq* r = S->pending.front();
+        if (!S->active.empty() && S->used + r->bytes > S->budget) break;
+        S->pending.pop_front();
+        S->used += r->bytes;
+        S->active.push_back(r);
+        fresh.push_back(r);
+      }
+      run = S->active;
+      S->rounds += 1;
+    }
+    for (ServeReq* r : fresh) {
+      const hipError_t e = hipMemsetAsync(r->x, 0, size_t(r->op->n) * sizeof(double2), S->stream);
+      if (e != hipSuccess) {
+        r->error = VP_ERR_DEVICE;
+        r->errmsg = hipGetErrorString(e);
+        r->st.done = true;
+      }
+    }
+    std::map<std::string, std::vector<ServeReq*>> groups;
+    for (ServeReq* r : run)
+      if (!r->st.done) groups[r->key].push_back(r);
+    for (auto& kv : groups) {
+      try {
+        run_round(S, kv.second);
+      } catch (const VpError& e) {
+        for (ServeReq* r : kv.second) {
+          r->error = e.code;
+          r->errmsg = e.msg;
+          r->st.done = true;
+        }
+      } catch (const std::exception& e) {
+        for (ServeReq* r : kv.second) {
+          r->error = VP_ERR_DEVICE;
+          r->errmsg = e.what();
+          r->st.done = true;
+        }
+      }
+    }
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      const auto now = Clock::now();
+      std::vector<ServeReq*> keep;
+      for (ServeReq* r : S->active) {
+        if (r->st.done) {
+          r->finished = true;
+          r->t_done = now;
+          S->used -= r->bytes;
+        } else {
+          keep.push_back(r);
+        }
+      }
+      S->active.swap(keep);
+    }
+    S->cv_done.notify_all();
+  }
+}
+
+void fill_serve_info(const ServeReq* r, vp_gmres_info* info) {
+  if (!info) return;
+  info->flag = r->error ? 1 : r->st.flag;
+  info->relres = r->st.relres;
+  info->ir_steps = r->st.steps;
+  info->total_iters = r->st.inner;
+  info->iter_inner = 0;
+  info->iter_outer = r->st.steps;
+  info->seconds = std::chrono::duration<double>(r->t_done - r->t_submit).count();
+  info->resvec_len = 0;
+  if (info->resvec && info->resvec_cap > 0) {
+    const int cnt = std::min<int>(info->resvec_cap, int(r->st.hist.size()));
+    for (int i = 0; i < cnt; ++i) info->resvec[i] = r->st.hist[i];
+    info->resvec_len = cnt;
+  }
+}
+
+ServeReq* find_req(vp_server_s* S, int64_t ticket) {
+  auto it = S->reqs.find(ticket);
+  VP_REQUIRE(it != S->reqs.end(), ERR_INVALID, "unknown server ticket");
+  return it->second.get();
+}
+}  // namespace
+
+extern "C" {
+
+int vp_server_create(vp_ctx ctx, int64_t hbm_budget_bytes, int max_active, vp_server* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && out && hbm_budget_bytes > 0 && max_active > 0, ERR_INVALID, "bad server arguments");
+  VP_HIP(hipSetDevice(ctx->device));
+  auto S = std::make_unique<vp_server_s>();
+  S->device = ctx->device;
+  S->budget = hbm_budget_bytes;
+  S->max_active = max_active;
+  VP_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+  vp_server_s* raw = S.get();
+  try {
+    S->worker = std::thread(serve_loop, raw);
+  } catch (...) {
+    (void)hipStreamDestroy(S->stream);
+    throw;
+  }
+  *out = S.release();
+  VP_API_END
+}
+
+int vp_server_submit(vp_server s, vp_op op, vp_prec M, const void* b, void* x, const vp_solve_params* prm,
+                     int64_t* ticket) {
+  VP_API_BEGIN
+  VP_REQUIRE(s && op && b && x && prm && ticket, ERR_INVALID, "bad submit arguments");
+  VP_REQUIRE(prm->restart > 0 && prm->restart <= 1000 && prm->inner_maxit > 0 && prm->max_outer >= 0 &&
+                 prm->tol > 0 && prm->inner_tol > 0 && (prm->side == VP_PREC_LEFT || prm->side == VP_PREC_RIGHT),
+             ERR_INVALID, "bad solve parameters");
+  VP_REQUIRE(op->p64.present && op->p32.present, ERR_INVALID,
+             "mixed precision needs operators created with VP_FP64|VP_FP32");
+  VP_REQUIRE(op->ctx->device == s->device, ERR_INVALID, "operator lives on another device");
+  if (M) {
+    const int64_t outer = M->g.n_outer ? M->g.n_outer : M->nbox;
+    VP_REQUIRE(outer == op->n, ERR_INVALID, "preconditioner size does not match the operator");
+  }
+  auto r = std::make_unique<ServeReq>();
+  r->op = op;
+  r->prec = M;
+  r->b = static_cast<const double2*>(b);
+  r->x = static_cast<double2*>(x);
+  r->prm = *prm;
+  r->bytes = request_bytes(op, M, *prm);
+  r->key = group_key(op, M, *prm);
+  r->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    VP_REQUIRE(!s->stop, ERR_INVALID, "server is shutting down");
+    r->ticket = s->next_ticket++;
+    *ticket = r->ticket;
+    s->pending.push_back(r.get());
+    s->reqs.emplace(r->ticket, std::move(r));
+  }
+  s->cv_work.notify_one();
+  VP_API_END
+}
+
+int vp_server_poll(vp_server s, int64_t ticket, int* done, vp_gmres_info* info) {
+  VP_API_BEGIN
+  VP_REQUIRE(s && done, ERR_INVALID, "bad poll arguments");
+  std::lock_guard<std::mutex> lk(s->mu);
+  ServeReq* r = find_req(s, ticket);
+  *done = r->finished ? 1 : 0;
+  if (r->finished) {
+    fill_serve_info(r, info);
+    VP_REQUIRE(r->error == 0, r->error, "request failed: " + r->errmsg);
+  }
+  VP_API_END
+}
+
+int vp_server_wait(vp_server s, int64_t ticket, vp_gmres_info* info) {
+  VP_API_BEGIN
+  VP_REQUIRE(s, ERR_INVALID, "server is NULL");
+  std::unique_lock<std::mutex> lk(s->mu);
+  ServeReq* r = find_req(s, ticket);
+  s->cv_done.wait(lk, [&] { return r->finished; });
+  fill_serve_info(r, info);
+  const int err = r->error;
+  const std::string msg = r->errmsg;
+  s->reqs.erase(ticket);
+  VP_REQUIRE(err == 0, err, "request failed: " + msg);
+  VP_API_END
+}
+
+int vp_server_stats(vp_server s, int64_t* pending, int64_t* active, int64_t* bytes_in_use, int64_t* rounds) {
+  VP_API_BEGIN
+  VP_REQUIRE(s, ERR_INVALID, "server is NULL");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (pending) *pending = int64_t(s->pending.size());
+  if (active) *active = int64_t(s->active.size());
+  if (bytes_in_use) *bytes_in_use = s->used;
+  if (rounds) *rounds = s->rounds;
+  VP_API_END
+}
+
+int vp_server_destroy(vp_server s) {
+  VP_API_BEGIN
+  if (!s) return VP_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  (void)hipSetDevice(s->device);
+  (void)hipStreamSynchronize(s->stream);
+  (void)hipStreamDestroy(s->stream);
+  delete s;
+  VP_API_END
+}
+
+}  // extern "C"

@@ -165,7 +165,7 @@ static void prec_apply_dev(vp_prec_s* p, const C<T>* x, C<T>* y) {
 // Problem sets over operators (and preconditioners)
 // ----------------------------------------------------------------------------
 template <typename T>
-static void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* precs, int np) {
+void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* precs, int np) {
   vp_op_s* o0 = ops[0];
   OpPrec<T>& P0 = prec_of<T>(o0);
   VP_REQUIRE(P0.present, ERR_INVALID, "operator lacks the precision this solve needs");
@@ -244,6 +244,9 @@ static void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* 
   }
 }
 
+template void make_problem_set<double>(ProblemSet<double>&, vp_op_s* const*, vp_prec_s* const*, int);
+template void make_problem_set<float>(ProblemSet<float>&, vp_op_s* const*, vp_prec_s* const*, int);
+
 static void fill_info(vp_gmres_info* info, const GmresResult& r, int p) {
   if (!info) return;
   info->flag = r.flag[p];
@@ -266,73 +269,100 @@ struct IrResult {
   std::vector<std::vector<double>> hist;
 };
 
-static void solve_ir_sets(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2* b, double2* x,
-                          double scale32, double tol, int restart, double inner_tol, int inner_maxit,
-                          int max_outer, int side, IrResult& out) {
+// One refinement round over a problem set (b, x: np device vectors of n).
+// For every problem not yet done: r = b - A x in FP64, relres = ||r||/||b||;
+// the problem is done when relres <= tol (flag 0), when relres is not
+// finite or after max_outer steps (flag 1); otherwise one inner FP32 GMRES
+// solve of (A/s) d = r/||r|| and x += d*||r||/s.  The inner solve reduces
+// its residual by inner_tol, or by no more than this step needs to reach tol
+// (factor 2 margin for the FP32 residual gap).  Problems already done do no
+// work.  Returns true while a problem is still running.
+bool ir_step_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2* b, double2* x, double scale32,
+                 double tol, int restart, double inner_tol, int inner_maxit, int max_outer, int side,
+                 std::vector<IrProblem>& st) {
   const int np = p64.nprob;
   const int64_t n = p64.n;
   hipStream_t s = p64.stream;
   DevBuf ax(size_t(n) * np * sizeof(double2)), r(size_t(n) * np * sizeof(double2));
-  DevBuf r32(size_t(n) * np * sizeof(float2)), d32(size_t(n) * np * sizeof(float2));
   DevBuf bn(np * sizeof(double)), rn(np * sizeof(double)), act(np * sizeof(int));
-  VP_HIP(hipMemsetAsync(x, 0, size_t(n) * np * sizeof(double2), s));
+  std::vector<int> hact(np);
+  for (int p = 0; p < np; ++p) hact[p] = st[p].done ? 0 : 1;
+  VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+  // r = b - A x  (FP64)
+  ps_apply_A<double>(p64, x, n, ax.as<double2>(), n, act.as<int>());
+  dev_residual_d(b, ax.as<double2>(), r.as<double2>(), n, np, s);
+  dev_norms<double>(r.as<double2>(), n, np, n, rn.as<double>(), s);
   dev_norms<double>(b, n, np, n, bn.as<double>(), s);
   std::vector<double> hbn(np), hrn(np);
-  VP_HIP(hipMemcpy(hbn.data(), bn.p, np * sizeof(double), hipMemcpyDeviceToHost));
-  out.flag.assign(np, 1);
-  out.steps.assign(np, 0);
-  out.inner.assign(np, 0);
-  out.relres.assign(np, 1.0);
-  out.hist.assign(np, {});
-  std::vector<int> hact(np, 1);
-  for (int it = 0; it <= max_outer; ++it) {
-    // r = b - A x  (FP64)
-    VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
-    ps_apply_A<double>(p64, x, n, ax.as<double2>(), n, nullptr);
-    dev_residual_d(b, ax.as<double2>(), r.as<double2>(), n, np, s);
-    dev_norms<double>(r.as<double2>(), n, np, n, rn.as<double>(), s);
-    VP_HIP(hipMemcpy(hrn.data(), rn.p, np * sizeof(double), hipMemcpyDeviceToHost));
-    bool any = false;
-    for (int p = 0; p < np; ++p) {
-      if (!hact[p]) continue;
-      out.relres[p] = hbn[p] > 0 ? hrn[p] / hbn[p] : 0.0;
-      out.hist[p].push_back(out.relres[p]);
-      if (out.relres[p] <= tol || !std::isfinite(out.relres[p])) {
-        hact[p] = 0;
-        out.flag[p] = out.relres[p] <= tol ? 0 : 1;
-      } else if (it == max_outer) {
-        hact[p] = 0;
-      } else {
-        any = true;
-      }
-    }
-    if (!any) break;
-    VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
-    // inner FP32 solve of (A/s) d = r/||r||
-    dev_scale_to_f(r.as<double2>(), rn.as<double>(), r32.as<float2>(), n, np, 1.0, s);
-    // The inner solve reduces its residual by inner_tol, or by no more than
-    // the last step needs to reach tol (with a factor 2 margin for the FP32
-    // residual gap); the outer loop recomputes the residual in FP64.
-    GmresParams prm;
-    prm.restart = restart;
-    prm.tol = inner_tol;
-    prm.maxit = inner_maxit;
-    prm.side = side;
-    prm.final_residual = false;
-    prm.tol_per.assign(np, inner_tol);
-    for (int p = 0; p < np; ++p)
-      if (hact[p] && out.relres[p] > 0) prm.tol_per[p] = std::max(inner_tol, std::min(0.5, 0.5 * tol / out.relres[p]));
-    GmresResult gr;
-    gmres_device<float>(p32, r32.as<float2>(), d32.as<float2>(), prm, gr);
-    for (int p = 0; p < np; ++p)
-      if (hact[p]) {
-        out.inner[p] += gr.total[p];
-        out.steps[p] += 1;
-      }
-    // x += d * ||r|| / s
-    dev_update_from_f(x, d32.as<float2>(), rn.as<double>(), n, np, 1.0 / scale32, act.as<int>(), s);
-  }
+  VP_HIP(hipMemcpyAsync(hrn.data(), rn.p, np * sizeof(double), hipMemcpyDeviceToHost, s));
+  VP_HIP(hipMemcpyAsync(hbn.data(), bn.p, np * sizeof(double), hipMemcpyDeviceToHost, s));
   VP_HIP(hipStreamSynchronize(s));
+  bool any = false;
+  for (int p = 0; p < np; ++p) {
+    if (!hact[p]) continue;
+    IrProblem& q = st[p];
+    q.relres = hbn[p] > 0 ? hrn[p] / hbn[p] : 0.0;
+    q.hist.push_back(q.relres);
+    if (q.relres <= tol || !std::isfinite(q.relres)) {
+      q.done = true;
+      q.flag = q.relres <= tol ? 0 : 1;
+      hact[p] = 0;
+    } else if (q.steps >= max_outer) {
+      q.done = true;
+      q.flag = 1;
+      hact[p] = 0;
+    } else {
+      any = true;
+    }
+  }
+  if (!any) return false;
+  VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+  DevBuf r32(size_t(n) * np * sizeof(float2)), d32(size_t(n) * np * sizeof(float2));
+  dev_scale_to_f(r.as<double2>(), rn.as<double>(), r32.as<float2>(), n, np, 1.0, s);
+  GmresParams prm;
+  prm.restart = restart;
+  prm.tol = inner_tol;
+  prm.maxit = inner_maxit;
+  prm.side = side;
+  prm.final_residual = false;
+  prm.active_in = hact;
+  prm.tol_per.assign(np, inner_tol);
+  for (int p = 0; p < np; ++p)
+    if (hact[p] && st[p].relres > 0)
+      prm.tol_per[p] = std::max(inner_tol, std::min(0.5, 0.5 * tol / st[p].relres));
+  GmresResult gr;
+  gmres_device<float>(p32, r32.as<float2>(), d32.as<float2>(), prm, gr);
+  for (int p = 0; p < np; ++p)
+    if (hact[p]) {
+      st[p].inner += gr.total[p];
+      st[p].steps += 1;
+    }
+  // x += d * ||r|| / s
+  dev_update_from_f(x, d32.as<float2>(), rn.as<double>(), n, np, 1.0 / scale32, act.as<int>(), s);
+  VP_HIP(hipStreamSynchronize(s));
+  return true;
+}
+
+static void solve_ir_sets(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2* b, double2* x,
+                          double scale32, double tol, int restart, double inner_tol, int inner_maxit,
+                          int max_outer, int side, IrResult& out) {
+  const int np = p64.nprob;
+  VP_HIP(hipMemsetAsync(x, 0, size_t(p64.n) * np * sizeof(double2), p64.stream));
+  std::vector<IrProblem> st(np);
+  while (ir_step_set(p64, p32, b, x, scale32, tol, restart, inner_tol, inner_maxit, max_outer, side, st)) {
+  }
+  out.flag.resize(np);
+  out.steps.resize(np);
+  out.inner.resize(np);
+  out.relres.resize(np);
+  out.hist.resize(np);
+  for (int p = 0; p < np; ++p) {
+    out.flag[p] = st[p].flag;
+    out.steps[p] = st[p].steps;
+    out.inner[p] = st[p].inner;
+    out.relres[p] = st[p].relres;
+    out.hist[p] = st[p].hist;
+  }
 }
 
 }  // namespace vp

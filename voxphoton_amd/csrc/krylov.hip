@@ -468,9 +468,12 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   res.relres.assign(np, 0.0);
   res.resvec.assign(np, {});
   VP_REQUIRE(prm.tol_per.empty() || int(prm.tol_per.size()) == np, ERR_INVALID, "per-problem tolerances");
+  VP_REQUIRE(prm.active_in.empty() || int(prm.active_in.size()) == np, ERR_INVALID, "per-problem active flags");
+  auto joins = [&](int p) { return prm.active_in.empty() || prm.active_in[size_t(p)] != 0; };
   auto tol_of = [&](int p) { return prm.tol_per.empty() ? prm.tol : prm.tol_per[size_t(p)]; };
 
   std::vector<int> hact(np, 1);
+  for (int p = 0; p < np; ++p) hact[p] = joins(p) ? 1 : 0;
   auto upload_act = [&]() {
     VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
   };
@@ -478,7 +481,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   const int* dact = act.as<int>();
   // bnorm: ||M^-1 b|| (left) or ||b||
   if (left) {
-    ps_apply_M<T>(ps, b, n, t1.as<C<T>>(), n, nullptr);
+    ps_apply_M<T>(ps, b, n, t1.as<C<T>>(), n, dact);
     dev_norms<T>(t1.as<C<T>>(), n, np, n, bn.as<double>(), s);
   } else {
     dev_norms<T>(b, n, np, n, bn.as<double>(), s);
@@ -516,7 +519,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   for (int p = 0; p < np; ++p) {
     last[p] = hbn[p] > 0 ? hbeta[p] / hbn[p] : 0.0;
     res.resvec[p].push_back(last[p]);
-    hact[p] = (last[p] > tol_of(p)) ? 1 : 0;
+    hact[p] = (joins(p) && last[p] > tol_of(p)) ? 1 : 0;
   }
   {
     std::vector<double> ta(np);

@@ -32,6 +32,10 @@ struct ProblemSet {
 
 template <typename T>
 void ps_apply_A(ProblemSet<T>& ps, const C<T>* x, int64_t xs, C<T>* y, int64_t ys, const int* active);
+// Problem set over operators (and preconditioners, may be NULL) of one
+// geometry; throws ERR_INVALID when they differ.
+template <typename T>
+void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* precs, int np);
 template <typename T>
 void ps_apply_M(ProblemSet<T>& ps, const C<T>* x, int64_t xs, C<T>* y, int64_t ys, const int* active);
 
@@ -43,6 +47,7 @@ struct GmresParams {
   bool x0_nonzero = false;
   int resvec_cap = 0;  // per problem
   std::vector<double> tol_per;   // per-problem tolerance (empty: tol for all)
+  std::vector<int> active_in;    // per-problem participation (empty: all); inactive problems do no work
   bool final_residual = true;    // recompute the true residual after the last cycle
 };
 
@@ -56,6 +61,17 @@ struct GmresResult {
 // FP64 from the T-precision data.
 template <typename T>
 void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& prm, GmresResult& res);
+
+// Per-problem state of a mixed-precision refinement run.
+struct IrProblem {
+  int steps = 0, inner = 0, flag = 1;
+  double relres = 1.0;
+  bool done = false;
+  std::vector<double> hist;   // relres at the start of every round
+};
+bool ir_step_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2* b, double2* x, double scale32,
+                 double tol, int restart, double inner_tol, int inner_maxit, int max_outer, int side,
+                 std::vector<IrProblem>& st);
 
 // Device vector helpers (FP64 accumulation).
 template <typename T>

@@ -366,3 +366,58 @@ def solve_ir_batched(ops, precs, B, tol=1e-6, restart=50, inner_tol=1e-3, inner_
                 inner_iters=infos[i].total_iters, seconds=infos[i].seconds,
                 history=bufs[i][:infos[i].resvec_len].copy()) for i in range(ns)]
     return X, out
+
+
+class Server:
+    """Request server with continuous batching (vp_server_*).
+
+    ``submit`` queues one mixed-precision solve of device vectors (b, x are
+    DeviceArray objects of n complex128) and returns a ticket; ``wait``
+    returns its info dict, with ``seconds`` = submit-to-finish latency.
+    Operators, preconditioners and the vectors must stay alive and unused by
+    the caller until the request has finished.
+    """
+
+    def __init__(self, hbm_budget_bytes, max_active=64, ctx=None):
+        self.ctx = ctx or default_context()
+        h = ctypes.c_void_p()
+        check(L.lib.vp_server_create(self.ctx.handle, int(hbm_budget_bytes), int(max_active), ctypes.byref(h)))
+        self.handle = h
+        self._keep = {}
+
+    def submit(self, op, M, b, x, tol=1e-6, restart=50, inner_tol=1e-3, inner_maxit=4, max_outer=20, side='right'):
+        prm = L.SolveParams(float(tol), int(restart), float(inner_tol), int(inner_maxit), int(max_outer),
+                            L.VP_PREC_RIGHT if side == 'right' else L.VP_PREC_LEFT)
+        t = ctypes.c_int64()
+        check(L.lib.vp_server_submit(self.handle, op.handle, M.handle if M is not None else None, b.ptr, x.ptr,
+                                     ctypes.byref(prm), ctypes.byref(t)))
+        self._keep[t.value] = (op, M, b, x, max_outer)
+        return t.value
+
+    def _info(self, ticket, fn, *args):
+        cap = self._keep[ticket][4] + 2
+        info, buf = _info(cap)
+        check(fn(self.handle, ticket, *args, ctypes.byref(info)))
+        return dict(flag=info.flag, relres=info.relres, ir_steps=info.ir_steps, inner_iters=info.total_iters,
+                    seconds=info.seconds, history=buf[:info.resvec_len].copy())
+
+    def poll(self, ticket):
+        done = ctypes.c_int()
+        out = self._info(ticket, L.lib.vp_server_poll, ctypes.byref(done))
+        return bool(done.value), (out if done.value else None)
+
+    def wait(self, ticket):
+        out = self._info(ticket, L.lib.vp_server_wait)
+        self._keep.pop(ticket, None)
+        return out
+
+    def stats(self):
+        v = [ctypes.c_int64() for _ in range(4)]
+        check(L.lib.vp_server_stats(self.handle, *[ctypes.byref(a) for a in v]))
+        return dict(pending=v[0].value, active=v[1].value, bytes_in_use=v[2].value, rounds=v[3].value)
+
+    def close(self):
+        if self.handle:
+            check(L.lib.vp_server_destroy(self.handle))
+            self.handle = None
+            self._keep.clear()
