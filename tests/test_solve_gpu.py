@@ -45,12 +45,15 @@ def test_chan1_apply_matches_oracle(vp_ctx):
     assert rel(P.apply(x), ref) < 1e-12
 
 
+@pytest.mark.parametrize('split', ['0', '1'])
 @pytest.mark.parametrize('sym', ['1', '0'])
 @pytest.mark.parametrize('shape', [(6, 4, 3), (10, 5, 2), (7, 5, 3)])
-def test_level2_device_setup_matches_oracle(vp_ctx, monkeypatch, shape, sym):
-    # sym=1 stores the (kx <= L/2, ky <= M/2) block quadrant, sym=0 all blocks
+def test_level2_device_setup_matches_oracle(vp_ctx, monkeypatch, shape, sym, split):
+    # sym=1 stores the (kx <= L/2, ky <= M/2) block quadrant, sym=0 all blocks;
+    # split=1 runs the plane as y-row DFT / block-product / inverse passes
     from voxphoton_amd.vie import Preconditioner
     monkeypatch.setenv('VP_PC_SYMMETRY', sym)
+    monkeypatch.setenv('VP_PC_SPLIT', split)
     L, M, N = shape
     dx, ko = 20e-9, 5.85e6
     G = ogr.assemble_G(L, M, N, dx, ko)
@@ -97,6 +100,21 @@ def test_level2_face_matches_oracle(vp_ctx, monkeypatch, sym):
     ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
     P = Preconditioner.level2_face((L, M, N), dx, ko, face, idx, ctx=vp_ctx)
     assert rel(P.apply(x), ref) < 1e-10
+
+
+def test_level2_plane_larger_than_lds(vp_ctx):
+    # 3N x M plane of 90 x 60 complex128 exceeds LDS: the split passes run
+    from voxphoton_amd.vie import Preconditioner
+    L, M, N = 4, 60, 30
+    rng = np.random.default_rng(8)
+    G = random_toeplitz(L, M, N, rng) * 1e-3
+    idx = ovie.idx_from_mask(np.ones((L, M, N), bool))
+    Binv = opre.level_2_parallel_func(G, 0.5, 1.0, L, M, N, 1.0)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
+    P = Preconditioner.level2(G, 0.5, 1.0, 1.0, idx, ctx=vp_ctx)
+    assert rel(P.apply(x), ref) < 1e-10
+    assert rel(P.apply(x, 'fp32'), ref) < 1e-4
 
 
 def test_level2_without_parity_zeros_keeps_all_blocks(vp_ctx):

@@ -221,6 +221,60 @@ def bragg_grating(periods=100, period=320e-9, dx=10e-9, lam=1550e-9, width=500e-
     return p
 
 
+def disk_resonator(radius=2e-6, dx=20e-9, lam=1550e-9, thick=220e-9, margin=2):
+    """Si microdisk (220 nm thick) in SiO2: a cylinder of ``radius`` centred
+    in an (2R/dx + 2*margin)^2 x (thick/dx + 2*margin) grid."""
+    d = int(round(2 * radius / dx))
+    tz = int(round(thick / dx))
+    L = M = d + 2 * margin
+    N = tz + 2 * margin
+    c = (L - 1) / 2.0
+    ii, jj = np.meshgrid(np.arange(L), np.arange(M), indexing='ij')
+    disk = ((ii - c) ** 2 + (jj - c) ** 2) <= (radius / dx) ** 2
+    mask = np.zeros((L, M, N), bool)
+    mask[:, :, margin:margin + tz] = disk[:, :, None]
+    eps = np.where(mask, EPS_SI, 1.0).astype(np.complex128)
+    p = Problem('disk', L, M, N, dx, lam, mask, eps)
+    p.box = _bbox(mask)
+    p.extra = dict(radius=radius)
+    return p
+
+
+def smooth_length(n, primes=(2, 3, 5, 7, 11)):
+    """Smallest m >= n whose prime factors are all in ``primes`` (lengths the
+    device FFT plans in every precision)."""
+    m = max(int(n), 1)
+    while True:
+        r = m
+        for q in primes:
+            while r % q == 0:
+                r //= q
+        if r == 1:
+            return m
+        m += 1
+
+
+def smoothen(prob):
+    """Pads the grid with empty voxels (high end of every axis) to lengths
+    with small prime factors, so that the 2x FFT sizes plan, and widens the
+    preconditioner box likewise inside the padded grid.  The unknowns and
+    their values are unchanged."""
+    L, M, N = (smooth_length(v) for v in prob.shape)
+    if (L, M, N) != prob.shape:
+        mask = np.zeros((L, M, N), bool)
+        eps = np.ones((L, M, N), np.complex128)
+        mask[:prob.L, :prob.M, :prob.N] = prob.mask
+        eps[:prob.L, :prob.M, :prob.N] = prob.eps
+        prob = Problem(prob.name, L, M, N, prob.dx, prob.lam, mask, eps, prob.n_ext, prob.box, dict(prob.extra))
+    box = []
+    for sl, ext in zip(prob.box, prob.shape):
+        w = smooth_length(sl.stop - sl.start)
+        start = min(sl.start, ext - w)
+        box.append(slice(start, start + w))
+    prob.box = tuple(box)
+    return prob
+
+
 def config_problem(name):
     """The workloads named by the benchmark configurations."""
     if name == 'straight_wg_20nm':

@@ -70,8 +70,30 @@ static void prec_geometry(vp_prec_s* p, int level) {
   g.tx = 256;
   g.t2 = 512;
   g.lds2 = size_t(2) * 3 * g.N * g.M * s;
-  if (level == 2)
-    VP_REQUIRE(g.lds2 <= size_t(lds_max), ERR_UNSUPPORTED, "preconditioner plane exceeds LDS");
+  g.psplit = 0;
+  const char* e = std::getenv("VP_PC_SPLIT");
+  if (level == 2 && (g.lds2 > size_t(lds_max) || (e && e[0] == '1'))) {
+    g.psplit = 1;
+    g.Brow = 0;
+    for (int B : {16, 8, 4, 2, 1}) {
+      const int ld = g.M + (B > 1 ? 1 : 0);
+      if (2 * size_t(B) * ld * s <= size_t(std::min(lds_max, 96 * 1024)) || B == 1) {
+        g.Brow = B;
+        g.ldrow = ld;
+        g.lds_row = 2 * size_t(B) * ld * s;
+        break;
+      }
+    }
+    VP_REQUIRE(g.lds_row <= size_t(lds_max), ERR_UNSUPPORTED, "preconditioner y rows exceed LDS");
+    g.TK = 1;
+    for (int tk : {64, 32, 16, 8, 4, 2, 1})
+      if ((tk <= g.M || tk == 1) && size_t(3) * g.N * tk * s <= 48 * 1024) {
+        g.TK = tk;
+        break;
+      }
+    g.lds_tile = size_t(3) * g.N * g.TK * s;
+    VP_REQUIRE(g.lds_tile <= size_t(lds_max), ERR_UNSUPPORTED, "preconditioner block column exceeds LDS");
+  }
 }
 
 static vp_prec_s* prec_begin(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const int64_t* idx, int64_t n,

@@ -121,6 +121,80 @@ __global__ void __launch_bounds__(512) k_pc_plane(PcArgs<T> a, PcGeom g) {
   for (int e = threadIdx.x; e < plane; e += blockDim.x) Zp[e] = F[e];
 }
 
+// Split plane, pass 1/3: y DFT (length M) of Brow consecutive rows of the
+// box spectrum (row (kx, c, n) holds M contiguous entries), in place.
+template <typename T, bool INV>
+__global__ void __launch_bounds__(256) k_pc_yrows(PcArgs<T> a, PcGeom g) {
+  if (a.active && !a.active[blockIdx.y]) return;
+  C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
+  const int M = g.M, B = g.Brow, ld = g.ldrow;
+  C<T>* W1 = W0 + B * ld;
+  const int64_t nrows = int64_t(g.L) * 3 * g.N;
+  const int64_t r0 = int64_t(xcd_remap(blockIdx.x, gridDim.x)) * B;
+  const int nb = int(min<int64_t>(B, nrows - r0));
+  const IDiv dM(M);
+  for (int e = threadIdx.x; e < B * M; e += blockDim.x) {
+    const int bb = dM.div(e), m = e - bb * M;
+    W0[bb * ld + m] = bb < nb ? work[(r0 + bb) * M + m] : czero<T>();
+  }
+  __syncthreads();
+  const C<T>* R = lds_fft<T, INV>(W0, W1, g.py, B, LdsLayout{1, ld, 0, 1}, a.twy);
+  for (int e = threadIdx.x; e < nb * M; e += blockDim.x) {
+    const int bb = dM.div(e), m = e - bb * M;
+    work[(r0 + bb) * M + m] = R[bb * ld + m];
+  }
+}
+
+// Split plane, pass 2/3: 3N x 3N block products for TK consecutive ky of one
+// kx (kx paired with L - kx in launch order, as the stored quadrant of the
+// blocks is shared).  The 3N x TK column tile is staged in LDS; thread
+// (t, r) reads row r of block (kx, ky0 + t) column by column (coalesced).
+template <typename T>
+__global__ void __launch_bounds__(256) k_pc_blockmul(PcArgs<T> a, PcGeom g) {
+  if (a.active && !a.active[blockIdx.y]) return;
+  C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
+  const C<T>* __restrict__ blocks = a.blocks[blockIdx.y];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* X = reinterpret_cast<C<T>*>(smem_raw);   // X[q * TK + t]
+  const int L = g.L, M = g.M, N = g.N, n3 = 3 * N, TK = g.TK;
+  const int nt = (M + TK - 1) / TK;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int io = blk / nt;
+  const int jo = (io + 1) >> 1;
+  const int kx = io == 0 ? 0 : ((io & 1) ? jo : L - jo);
+  const int ky0 = (blk - io * nt) * TK;
+  C<T>* plane = work + int64_t(kx) * n3 * M;       // row q = c*N + n at plane + q*M
+  const IDiv dTK(TK), dn3(n3);
+  for (int e = threadIdx.x; e < n3 * TK; e += blockDim.x) {
+    const int q = dTK.div(e), t = e - q * TK;
+    X[e] = (ky0 + t < M) ? plane[int64_t(q) * M + ky0 + t] : czero<T>();
+  }
+  __syncthreads();
+  const bool mx = g.bsym && 2 * kx > L;
+  const C<T>* Bk = blocks + int64_t(mx ? L - kx : kx) * g.Hy * n3 * n3;
+  for (int e = threadIdx.x; e < n3 * TK; e += blockDim.x) {
+    const int t = dn3.div(e), r = e - t * n3;
+    const int ky = ky0 + t;
+    if (ky >= M) continue;
+    const bool my = g.bsym && 2 * ky > M;
+    const C<T>* Bb = Bk + int64_t(my ? M - ky : ky) * n3 * n3 + r;
+    C<T> sx = czero<T>(), sy = czero<T>(), sz = czero<T>();
+    for (int q = 0; q < N; ++q) {
+      sx = cadd(sx, cmul(Bb[int64_t(q) * n3], X[q * TK + t]));
+      sy = cadd(sy, cmul(Bb[int64_t(q + N) * n3], X[(q + N) * TK + t]));
+      sz = cadd(sz, cmul(Bb[int64_t(q + 2 * N) * n3], X[(q + 2 * N) * TK + t]));
+    }
+    if (mx) sx = cneg(sx);
+    if (my) sy = cneg(sy);
+    C<T> acc = cadd(cadd(sx, sy), sz);
+    const int cr = r / N;
+    if ((cr == 0 && mx) || (cr == 1 && my)) acc = cneg(acc);
+    plane[int64_t(r) * M + ky] = acc;
+  }
+}
+
 // x pass inverse + gather into the output vector.
 template <typename T>
 __global__ void __launch_bounds__(512) k_pc_xinv(PcArgs<T> a, PcGeom g) {
@@ -208,7 +282,27 @@ void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) 
     hipLaunchKernelGGL(k_pc_xfwd<T>, gx, dim3(g.tx), lds1, s, a, g);
   }
   VP_HIP(hipGetLastError());
-  if (g.level == 2) {
+  if (g.level == 2 && g.psplit) {
+    const size_t ldr = g.lds_row * sizeof(C<T>) / 16, ldt = g.lds_tile * sizeof(C<T>) / 16;
+    const void* fr0 = reinterpret_cast<const void*>(&k_pc_yrows<T, false>);
+    const void* fr1 = reinterpret_cast<const void*>(&k_pc_yrows<T, true>);
+    const void* fb = reinterpret_cast<const void*>(&k_pc_blockmul<T>);
+    (void)hipFuncSetAttribute(fr0, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldr));
+    (void)hipFuncSetAttribute(fr1, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldr));
+    (void)hipFuncSetAttribute(fb, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldt));
+    const int64_t nrows = int64_t(g.L) * 3 * g.N;
+    const dim3 gr(unsigned((nrows + g.Brow - 1) / g.Brow), nprob);
+    const dim3 gb(unsigned(int64_t(g.L) * ((g.M + g.TK - 1) / g.TK)), nprob);
+    {
+      ProfScope ps(s, KID_PC_PLANE);
+      hipLaunchKernelGGL((k_pc_yrows<T, false>), gr, dim3(256), ldr, s, a, g);
+      hipLaunchKernelGGL(k_pc_blockmul<T>, gb, dim3(256), ldt, s, a, g);
+      hipLaunchKernelGGL((k_pc_yrows<T, true>), gr, dim3(256), ldr, s, a, g);
+    }
+    VP_HIP(hipGetLastError());
+    ProfScope ps(s, KID_PC_XINV);
+    hipLaunchKernelGGL(k_pc_xinv<T>, gx, dim3(g.tx), lds1, s, a, g);
+  } else if (g.level == 2) {
     const void* f2 = reinterpret_cast<const void*>(&k_pc_plane<T>);
     (void)hipFuncSetAttribute(f2, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2));
     {

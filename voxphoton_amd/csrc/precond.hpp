@@ -15,6 +15,11 @@ struct PcGeom {
   // Hy = M/2+1 and B(L-kx, ky) = Sx B(kx, ky) Sx, B(kx, M-ky) = Sy B Sy with
   // Sx (Sy) negating the x (y) component; bsym = 0: Hx = L, Hy = M.
   int bsym, Hx, Hy;
+  // 2-level apply split into passes when the fused (3N x M) plane exceeds
+  // LDS: y DFT of Brow rows per workgroup (row stride ldrow), block products
+  // over (kx, TK-column tiles), inverse y DFT.
+  int psplit, Brow, ldrow, TK;
+  size_t lds_row, lds_tile;
   size_t lds1, lds2;
   FftPlan px, py;       // unpadded DFTs of length L and M
 };
