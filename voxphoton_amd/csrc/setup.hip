@@ -186,6 +186,49 @@ __global__ void k_parity_dft(const double2* __restrict__ in, double2* __restrict
   }
 }
 
+// The x pass (the longest axis of the configurations) as an FFT: one
+// workgroup per (m, n, component) pencil builds the embedded sequence C of
+// length P = R*S (C(0) = G(0), zero for odd components; C(i) = G(i) and
+// C(P-i) = s G(i) for 0 < i < D), folds it into R residue classes and runs
+// an LDS FFT of length S per class: F(R a + p) = DFT_S(fold_p)(a).
+__global__ void __launch_bounds__(256) k_embed_dft_x(const double2* __restrict__ G, double2* __restrict__ F, int D,
+                                                     int P, int R, int S, FftPlan plan, int64_t npencil,
+                                                     const double2* __restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double2* W0 = reinterpret_cast<double2*>(smem_raw);
+  double2* W1 = W0 + S;
+  const int64_t pen = blockIdx.x;
+  if (pen >= npencil) return;
+  const int k = int(pen / (npencil / 6));
+  const bool odd = c_parity[k][0] < 0;
+  const double sg = odd ? -1.0 : 1.0;
+  const double2* g = G + pen * D;
+  double2* f = F + pen * P;
+  auto C = [&](int i) -> double2 {
+    if (i == 0) return odd ? make_double2(0.0, 0.0) : g[0];
+    if (i < D) return g[i];
+    if (i > P - D) { const double2 v = g[P - i]; return make_double2(sg * v.x, sg * v.y); }
+    return make_double2(0.0, 0.0);
+  };
+  const LdsLayout lay{1, S, 0, 1};
+  for (int px = 0; px < R; ++px) {
+    for (int j = threadIdx.x; j < S; j += blockDim.x) {
+      double2 acc = make_double2(0.0, 0.0);
+      for (int r = 0; r < R; ++r) {
+        const int i = j + r * S;
+        double2 v = C(i);
+        if (px) v = cmul(v, tw[(int64_t(i) * px) % P]);
+        acc = cadd(acc, v);
+      }
+      W0[j] = acc;
+    }
+    __syncthreads();
+    const double2* Rr = lds_fft<double, false>(W0, W1, plan, 1, lay, tw);
+    for (int a = threadIdx.x; a < S; a += blockDim.x) f[int64_t(R) * a + px] = Rr[a];
+    __syncthreads();
+  }
+}
+
 void toeplitz_to_fN(const vp_op_s& op, const double2* G, double2* F, hipStream_t s) {
   auto twx = twiddle_table(op.Lp), twy = twiddle_table(op.Mp), twz = twiddle_table(op.Np);
   DevBuf dtx(twx.size() * sizeof(double2)), dty(twy.size() * sizeof(double2)), dtz(twz.size() * sizeof(double2));
@@ -201,7 +244,24 @@ void toeplitz_to_fN(const vp_op_s& op, const double2* G, double2* F, hipStream_t
     hipLaunchKernelGGL(k_parity_dft, dim3(blocks), dim3(256), 0, s, in, out, ax, D, P, e0, e1, e2, tw);
     VP_HIP(hipGetLastError());
   };
-  launch(G, t1.as<double2>(), 0, op.L, op.Lp, op.Lp, op.M, op.N, dtx.as<double2>());
+  {
+    // x pass by FFT when the residue-class length plans and fits LDS; the
+    // direct sum otherwise
+    const int R = (op.Lp % 2 == 0) ? 2 : 1;
+    const int S = op.Lp / R;
+    FftPlan plan{};
+    const size_t lds = size_t(2) * S * sizeof(double2);
+    if (factorize(S, true, plan, R) && lds <= size_t(max_lds_bytes(op.ctx->device)) && S >= op.L) {
+      const int64_t npen = int64_t(op.M) * op.N * 6;
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_embed_dft_x),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+      hipLaunchKernelGGL(k_embed_dft_x, dim3(unsigned(npen)), dim3(256), lds, s, G, t1.as<double2>(), op.L, op.Lp,
+                         R, S, plan, npen, dtx.as<double2>());
+      VP_HIP(hipGetLastError());
+    } else {
+      launch(G, t1.as<double2>(), 0, op.L, op.Lp, op.Lp, op.M, op.N, dtx.as<double2>());
+    }
+  }
   launch(t1.as<double2>(), t2.as<double2>(), 1, op.M, op.Mp, op.Lp, op.Mp, op.N, dty.as<double2>());
   launch(t2.as<double2>(), F, 2, op.N, op.Np, op.Lp, op.Mp, op.Np, dtz.as<double2>());
   VP_HIP(hipStreamSynchronize(s));
