@@ -206,8 +206,10 @@ typedef struct {
 
 /*
  * Restarted GMRES(restart) on A x = b with optional preconditioner applied
- * on the given side.  Arnoldi uses classical Gram-Schmidt with one
- * reorthogonalisation pass; the Hessenberg least squares is updated with
+ * on the given side.  Arnoldi uses classical Gram-Schmidt with a second
+ * pass where the first one cancelled more than half of ||w||^2 (DGKS
+ * criterion; VP_GMRES_CGS2=1 forces it every step); the Hessenberg least
+ * squares is updated with
  * Givens rotations in FP64.  prec selects the arithmetic (VP_FP64 or VP_FP32
  * for the whole Krylov loop).  b, x0 (nullable), x follow mem.
  */

@@ -1,16 +1,18 @@
 // Restarted GMRES, mixed-precision helpers and batched problem sets.
 //
 // Arnoldi (pgmres_iter.m:60-127) runs as fused device kernels:
-//   k_dots       h1 = V^H w                       (FP64 partial sums per block)
-//   k_axpy_dots  w -= V h1 ; h2 = V^H w           (second Gram-Schmidt pass)
-//   k_axpy_norm  w -= V h2 ; ||w||^2
-//   k_arnoldi    H(:,k) = h1 + h2, ||w||, Givens rotations, residual estimate
+//   k_dots       h1 = V^H w, ||w||^2               (FP64 partial sums per block)
+//   k_axpy_dots  w -= V h1 ; ||w||^2               (mode 1)
+//   k_dgks       second pass wanted where ||w'||^2 < ||w||^2 / 2
+//   k_dots, k_axpy_dots  h2 = V^H w, w -= V h2     (only those problems)
+//   k_arnoldi    H(:,k) = h1 (+ h2), ||w||, Givens rotations, residual estimate
 //   k_scale      v_{k+1} = w / ||w||
 // All reductions use fixed-order trees (no atomics), so a problem's result
 // does not depend on which other problems share the launch.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include "krylov.hpp"
 
 namespace vp {
@@ -76,11 +78,12 @@ __device__ __forceinline__ void dots_core(const C<T>* __restrict__ Vp, int64_t v
   }
 }
 
+// with_norm: column k of the partials also gets ||w||^2 (in .x).
 template <typename T>
 __global__ void __launch_bounds__(KT) k_dots(const C<T>* __restrict__ V, int64_t vcol, int64_t vprob, int k,
                                             const C<T>* __restrict__ w, int64_t wprob, int64_t n,
                                             double2* __restrict__ partials, int maxcols,
-                                            const int* __restrict__ active) {
+                                            const int* __restrict__ active, int with_norm) {
   const int p = blockIdx.y;
   if (active && !active[p]) return;
   const C<T>* Vp = V + p * vprob;
@@ -93,7 +96,27 @@ __global__ void __launch_bounds__(KT) k_dots(const C<T>* __restrict__ V, int64_t
     const int64_t i = base + t * KT + threadIdx.x;
     wv[t] = i < n ? to_d(wp[i]) : make_double2(0.0, 0.0);
   }
-  dots_core<T>(Vp, vcol, k, wv, base, n, partials + (int64_t(p) * maxcols) * nblk + blockIdx.x, nblk);
+  double2* part = partials + (int64_t(p) * maxcols) * nblk + blockIdx.x;
+  dots_core<T>(Vp, vcol, k, wv, base, n, part, nblk);
+  if (with_norm) {
+    double2 acc[1] = {make_double2(0.0, 0.0)};
+#pragma unroll
+    for (int t = 0; t < RPT; ++t) acc[0].x += wv[t].x * wv[t].x + wv[t].y * wv[t].y;
+    block_reduce_store<1>(acc, part + int64_t(k) * nblk, nblk, 1);
+  }
+}
+
+// Selective reorthogonalisation (Daniel-Gragg-Kaufman-Stewart): a second
+// Gram-Schmidt pass for problem p only when the first pass cancelled more
+// than half of ||w||^2, i.e. ||w'||^2 < ||w||^2 / 2 (always = 1: every step).
+__global__ void k_dgks(const double2* __restrict__ h1, const double2* __restrict__ nr, int maxcols, int k,
+                       const int* __restrict__ active, int* __restrict__ reorth, int nprob, int always) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob) return;
+  const bool on = !active || active[p];
+  const double w2 = h1[int64_t(p) * maxcols + k + 1].x;
+  const double r2 = nr[int64_t(p) * maxcols].x;
+  reorth[p] = (on && (always || r2 < 0.5 * w2)) ? 1 : 0;
 }
 
 // w -= V h (h: FP64 per problem), then dots of the new w with V (mode 0) or
@@ -170,7 +193,8 @@ struct ArnoldiState {
   int m, maxcols;
 };
 
-__global__ void k_arnoldi(ArnoldiState st, int k, const int* __restrict__ active, int nprob) {
+__global__ void k_arnoldi(ArnoldiState st, int k, const int* __restrict__ active, int nprob,
+                          const int* __restrict__ reorth) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nprob) return;
   if (active && !active[p]) return;
@@ -178,7 +202,10 @@ __global__ void k_arnoldi(ArnoldiState st, int k, const int* __restrict__ active
   double2* col = st.H + int64_t(p) * (m + 1) * m + int64_t(k) * (m + 1);
   const double2* h1 = st.h1 + int64_t(p) * st.maxcols;
   const double2* h2 = st.h2 + int64_t(p) * st.maxcols;
-  for (int i = 0; i <= k; ++i) col[i] = make_double2(h1[i].x + h2[i].x, h1[i].y + h2[i].y);
+  if (reorth[p])
+    for (int i = 0; i <= k; ++i) col[i] = make_double2(h1[i].x + h2[i].x, h1[i].y + h2[i].y);
+  else
+    for (int i = 0; i <= k; ++i) col[i] = h1[i];
   const double hn = sqrt(fmax(st.nrm[int64_t(p) * st.maxcols].x, 0.0));
   col[k + 1] = make_double2(hn, 0.0);
   double* cs = st.cs + int64_t(p) * m;
@@ -474,6 +501,9 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
 
   std::vector<int> hact(np, 1);
   for (int p = 0; p < np; ++p) hact[p] = joins(p) ? 1 : 0;
+  DevBuf ract(np * sizeof(int));                 // per-step second-pass flags
+  const char* ecg = std::getenv("VP_GMRES_CGS2");  // 1: second pass at every step
+  const bool cgs2_always = ecg && ecg[0] == '1';
   auto upload_act = [&]() {
     VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
   };
@@ -557,19 +587,27 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       } else {
         ps_apply_A<T>(ps, vk, vprob, w, vprob, dia);
       }
+      // first classical Gram-Schmidt pass: h1 = V^H w (and ||w||^2), w' = w - V h1, ||w'||^2
       hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, w, vprob, n,
-                         part.as<double2>(), maxcols, dia);
-      hipLaunchKernelGGL(k_reduce_cols, dim3(k + 1, np), dim3(KT), 0, s, part.as<double2>(), nblk, k + 1, maxcols,
+                         part.as<double2>(), maxcols, dia, 1);
+      hipLaunchKernelGGL(k_reduce_cols, dim3(k + 2, np), dim3(KT), 0, s, part.as<double2>(), nblk, k + 2, maxcols,
                          h1.as<double2>(), dia);
       hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, h1.as<double2>(), w,
-                         vprob, n, part.as<double2>(), maxcols, 0, dia);
-      hipLaunchKernelGGL(k_reduce_cols, dim3(k + 1, np), dim3(KT), 0, s, part.as<double2>(), nblk, k + 1, maxcols,
-                         h2.as<double2>(), dia);
-      hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, h2.as<double2>(), w,
                          vprob, n, part.as<double2>(), maxcols, 1, dia);
       hipLaunchKernelGGL(k_reduce_cols, dim3(1, np), dim3(KT), 0, s, part.as<double2>(), nblk, 1, maxcols,
                          nr.as<double2>(), dia);
-      hipLaunchKernelGGL(k_arnoldi, dim3((np + 63) / 64), dim3(64), 0, s, st, k, dia, np);
+      // second pass only where the first cancelled (blocks of other problems exit at once)
+      hipLaunchKernelGGL(k_dgks, dim3((np + 63) / 64), dim3(64), 0, s, h1.as<double2>(), nr.as<double2>(), maxcols,
+                         k, dia, ract.as<int>(), np, cgs2_always ? 1 : 0);
+      hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, w, vprob, n,
+                         part.as<double2>(), maxcols, ract.as<int>(), 0);
+      hipLaunchKernelGGL(k_reduce_cols, dim3(k + 1, np), dim3(KT), 0, s, part.as<double2>(), nblk, k + 1, maxcols,
+                         h2.as<double2>(), ract.as<int>());
+      hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, h2.as<double2>(), w,
+                         vprob, n, part.as<double2>(), maxcols, 1, ract.as<int>());
+      hipLaunchKernelGGL(k_reduce_cols, dim3(1, np), dim3(KT), 0, s, part.as<double2>(), nblk, 1, maxcols,
+                         nr.as<double2>(), ract.as<int>());
+      hipLaunchKernelGGL(k_arnoldi, dim3((np + 63) / 64), dim3(64), 0, s, st, k, dia, np, ract.as<int>());
       hipLaunchKernelGGL(k_scale<T>, dim3(grid1(n), np), dim3(256), 0, s, w, vprob, n, invn.as<double>(), 0, dia);
       VP_HIP(hipGetLastError());
       VP_HIP(hipMemcpyAsync(hest.data(), est.p, np * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -1,6 +1,6 @@
 // Restarted GMRES on the device (pgmres.m:70-180 / pgmres_iter.m:22-150
-// semantics) with classical Gram-Schmidt + one reorthogonalisation pass and
-// FP64 Givens least squares, batched over independent problems.
+// semantics) with classical Gram-Schmidt, a selective second pass (DGKS
+// criterion) and FP64 Givens least squares, batched over independent problems.
 #pragma once
 #include <vector>
 #include "precond.hpp"
