@@ -120,7 +120,8 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-fp64-solve', action='store_true')
     ap.add_argument('--traffic-probe', action='store_true',
-                    help='for rocprofv3 --pmc runs: a 1 GiB device copy (calibration), then 3 FP32 matvecs')
+                    help='for rocprofv3 --pmc runs: a 1 GiB device copy (calibration), then 3 FP32 matvecs '
+                         'and 3 FP32 preconditioner applies')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU only (asked for --gpus {args.gpus}); every solve of this path fits '
@@ -148,7 +149,8 @@ def main():
     t_op = time.perf_counter() - t0
     log(f'operator assembled + embedded in {t_op:.2f} s')
     if args.traffic_probe:
-        traffic_probe(op, ctx, n, vie, L, log)
+        P, _ = make_preconditioner(prob, wl, vie, configs)
+        traffic_probe(op, P, ctx, n, vie, L, log)
         return
     prec = None
     pdesc = None
@@ -549,10 +551,10 @@ def run_serve(args, wl, log):
     print(json.dumps(line), flush=True)
 
 
-def traffic_probe(op, ctx, n, vie, L, log):
+def traffic_probe(op, P, ctx, n, vie, L, log):
     """Kernels for the HBM-traffic counters: torch's device copy of 1 GiB
     (read 1 GiB, write 1 GiB: the calibration of FETCH_SIZE / WRITE_SIZE),
-    then three FP32 matvecs."""
+    then three FP32 matvecs and three FP32 preconditioner applies."""
     import torch
     src = torch.ones(1 << 28, dtype=torch.float32, device='cuda')
     dst = torch.empty_like(src)
@@ -564,8 +566,11 @@ def traffic_probe(op, ctx, n, vie, L, log):
     x32.upload((np.random.default_rng(1550).standard_normal(n) + 0j).astype(np.complex64))
     for _ in range(3):
         L.check(L.lib.vp_op_apply(op.handle, 0, x32.ptr, y32.ptr, L.VP_MEM_DEVICE, L.VP_FP32))
+    L.check(L.lib.vp_prec_set_fp32_scale(P.handle, 1.0))
+    for _ in range(3):
+        L.check(L.lib.vp_prec_apply(P.handle, x32.ptr, y32.ptr, L.VP_MEM_DEVICE, L.VP_FP32))
     ctx.synchronize()
-    log('traffic probe done: 3 x 1 GiB copies, 3 FP32 matvecs')
+    log('traffic probe done: 3 x 1 GiB copies, 3 FP32 matvecs, 3 FP32 preconditioner applies')
 
 
 def make_preconditioner(prob, wl, vie, configs):

@@ -278,30 +278,121 @@ __device__ __forceinline__ void stage_bf(const C<T>* src, C<T>* dst, int n, int 
   }
 }
 
+// Pruned stage of a zero-padded transform (batch-fastest layout), with a
+// compile-time count RC:
+//   first stage (Ns = 1): inputs j + r*nb with r >= RC are zero padding
+//     (the caller guarantees it), so every output sums RC terms;
+//   last stage (Ns * R = n): only outputs j + r*Ns with r < RC are wanted.
+template <typename T, int R, int RC, bool INV, bool FIRST>
+__device__ __forceinline__ void stage_bf_pruned(const C<T>* src, C<T>* dst, int n, int Ns, int nbatch, int ld,
+                                                const C<T>* __restrict__ tw, int twmul) {
+  constexpr RootTable<R> rt{};
+  const int nb = n / R;
+  const int total = nb * nbatch;
+  const IDiv dnbatch(nbatch), dns(Ns);
+  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+    const int j = dnbatch.div(b);
+    const int beta = b - j * nbatch;
+    if (FIRST) {
+      C<T> v[RC];
+#pragma unroll
+      for (int r = 0; r < RC; ++r) v[r] = src[beta + (j + r * nb) * ld];
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        C<T> acc = v[0];
+#pragma unroll
+        for (int r = 1; r < RC; ++r) {
+          const int e = (r * k) % R;
+          C<T> w;
+          w.x = T(rt.c[e]);
+          w.y = INV ? T(rt.s[e]) : T(-rt.s[e]);
+          acc = cadd(acc, cmul(v[r], w));
+        }
+        dst[beta + (j * R + k) * ld] = acc;   // Ns = 1: outputs j*R + k
+      }
+    } else {
+      C<T> v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = src[beta + (j + r * nb) * ld];
+      const int jm = dns.mod(j);
+      if (Ns > 1) {
+        const int step = jm * twmul;
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<T, INV>(tw, r * step));
+      }
+      const int base = (j - jm) * R + jm;
+#pragma unroll
+      for (int r = 0; r < RC; ++r) {
+        C<T> acc = v[0];
+#pragma unroll
+        for (int k = 1; k < R; ++k) {
+          const int e = (r * k) % R;
+          C<T> w;
+          w.x = T(rt.c[e]);
+          w.y = INV ? T(rt.s[e]) : T(-rt.s[e]);
+          acc = cadd(acc, cmul(v[k], w));
+        }
+        dst[beta + (base + r * Ns) * ld] = acc;
+      }
+    }
+  }
+}
+
+// Runs one stage, pruned when mode != 0 and (R, rc) has an instantiation:
+// mode 1 first stage with rc nonzero inputs per butterfly, mode 2 last stage
+// with rc wanted outputs per butterfly.
 template <typename T, bool INV>
-__device__ C<T>* lds_fft_bf(C<T>* a, C<T>* b, const FftPlan& pl, int nbatch, int ld, const C<T>* __restrict__ tw) {
+__device__ __forceinline__ void stage_bf_any(const C<T>* src, C<T>* dst, int R, int n, int Ns, int nbatch, int ld,
+                                             const C<T>* __restrict__ tw, int twmul, int mode, int rc) {
+#define VP_PR(RR, K)                                                                                    \
+  if (R == RR && rc == K) {                                                                             \
+    if (mode == 1) stage_bf_pruned<T, RR, K, INV, true>(src, dst, n, Ns, nbatch, ld, tw, twmul);          \
+    else stage_bf_pruned<T, RR, K, INV, false>(src, dst, n, Ns, nbatch, ld, tw, twmul);                  \
+    return;                                                                                             \
+  }
+  if (mode != 0) {
+    VP_PR(11, 1) VP_PR(11, 2) VP_PR(11, 3) VP_PR(11, 4) VP_PR(11, 5)
+    VP_PR(7, 1) VP_PR(7, 2) VP_PR(7, 3)
+    VP_PR(5, 1) VP_PR(5, 2)
+    VP_PR(8, 1) VP_PR(8, 2) VP_PR(8, 3) VP_PR(8, 4)
+  }
+#undef VP_PR
+  switch (R) {
+    case 2: stage_bf<T, 2, INV>(src, dst, n, Ns, nbatch, ld, tw, twmul); break;
+    case 3: stage_bf<T, 3, INV>(src, dst, n, Ns, nbatch, ld, tw, twmul); break;
+    case 4: stage_bf<T, 4, INV>(src, dst, n, Ns, nbatch, ld, tw, twmul); break;
+    case 5: stage_bf<T, 5, INV>(src, dst, n, Ns, nbatch, ld, tw, twmul); break;
+    case 7: stage_bf<T, 7, INV>(src, dst, n, Ns, nbatch, ld, tw, twmul); break;
+    case 8: stage_bf<T, 8, INV>(src, dst, n, Ns, nbatch, ld, tw, twmul); break;
+    case 11: stage_bf<T, 11, INV>(src, dst, n, Ns, nbatch, ld, tw, twmul); break;
+    case 13:
+      if constexpr (sizeof(T) == 4) stage_bf<T, 13, INV>(src, dst, n, Ns, nbatch, ld, tw, twmul);
+      break;
+    case 16:
+      if constexpr (sizeof(T) == 4) stage_bf<T, 16, INV>(src, dst, n, Ns, nbatch, ld, tw, twmul);
+      break;
+    default: break;
+  }
+}
+
+// nin: leading inputs that may be nonzero (the rest are zero padding);
+// nout: leading outputs needed.  Values >= n disable the pruning.
+template <typename T, bool INV>
+__device__ C<T>* lds_fft_bf(C<T>* a, C<T>* b, const FftPlan& pl, int nbatch, int ld, const C<T>* __restrict__ tw,
+                            int nin = 1 << 30, int nout = 1 << 30) {
   int Ns = 1;
   C<T>* src = a;
   C<T>* dst = b;
   for (int s = 0; s < pl.nst; ++s) {
     const int R = pl.radix(s);
     const int twmul = (pl.n / (Ns * R)) * pl.tws;
-    switch (R) {
-      case 2: stage_bf<T, 2, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
-      case 3: stage_bf<T, 3, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
-      case 4: stage_bf<T, 4, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
-      case 5: stage_bf<T, 5, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
-      case 7: stage_bf<T, 7, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
-      case 8: stage_bf<T, 8, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
-      case 11: stage_bf<T, 11, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul); break;
-      case 13:
-        if constexpr (sizeof(T) == 4) stage_bf<T, 13, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul);
-        break;
-      case 16:
-        if constexpr (sizeof(T) == 4) stage_bf<T, 16, INV>(src, dst, pl.n, Ns, nbatch, ld, tw, twmul);
-        break;
-      default: break;
-    }
+    // pruned counts are uniform over the stage: ceil(lim / span) terms (first
+    // stage, span nb = n/R) or outputs (last stage, span Ns)
+    int mode = 0, rc = 0;
+    if (s == 0 && nin < pl.n && pl.nst > 1) { mode = 1; const int nb = pl.n / R; rc = (nin + nb - 1) / nb; }
+    else if (s == pl.nst - 1 && nout < pl.n && pl.nst > 1) { mode = 2; rc = (nout + Ns - 1) / Ns; }
+    if (rc >= R) mode = 0;
+    stage_bf_any<T, INV>(src, dst, R, pl.n, Ns, nbatch, ld, tw, twmul, mode, rc);
     __syncthreads();
     C<T>* t = src; src = dst; dst = t;
     Ns *= R;

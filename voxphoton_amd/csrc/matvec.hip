@@ -271,9 +271,9 @@ __global__ void __launch_bounds__(256) k_zpass(const ProbPtrs<T>* __restrict__ p
   }
   for (int e = threadIdx.x; e < ld * (Np - N); e += blockDim.x) W0[N * ld + e] = czero<T>();
   __syncthreads();
-  C<T>* R = lds_fft_bf<T, false>(W0, W1, g.z.plan, ld, ld, tw);
+  C<T>* R = lds_fft_bf<T, false>(W0, W1, mp.zf, ld, ld, tw, N);
   // tensor multiply: fN is [kx][kz][ky][6] (class layout with Ry = Rz = 1) or,
-  // parity compressed, the octant [kxc][kzc][kyc][6] with the signs of the
+  // parity compressed, the octant [kxc][kzc][comp][kyc] with the signs of the
   // odd components (xy: x,y; xz: x,z; yz: y,z) applied on mirrored axes.
   const bool mx = 2 * kx > Lp;
   const int kxc = mx ? Lp - kx : kx;
@@ -287,8 +287,8 @@ __global__ void __launch_bounds__(256) k_zpass(const ProbPtrs<T>* __restrict__ p
       if (mp.fsym) {
         const bool my = 2 * ky > Mp, mz = 2 * kz > Np;
         const int kyc = my ? Mp - ky : ky, kzc = mz ? Np - kz : kz;
-        const C<T>* f = F + (int64_t(kzc) * Hy + kyc) * 6;
-        f0 = f[0]; f1 = f[1]; f2 = f[2]; f3 = f[3]; f4 = f[4]; f5 = f[5];
+        const C<T>* f = F + int64_t(kzc) * 6 * Hy + kyc;     // component k at f[k * Hy]
+        f0 = f[0]; f1 = f[Hy]; f2 = f[2 * Hy]; f3 = f[3 * Hy]; f4 = f[4 * Hy]; f5 = f[5 * Hy];
         if (mx != my) f1 = cneg(f1);
         if (mx != mz) f2 = cneg(f2);
         if (my != mz) f4 = cneg(f4);
@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(256) k_zpass(const ProbPtrs<T>* __restrict__ p
     }
   }
   __syncthreads();
-  const C<T>* Q = lds_fft_bf<T, true>(R, R == W0 ? W1 : W0, g.z.plan, ld, ld, tw);
+  const C<T>* Q = lds_fft_bf<T, true>(R, R == W0 ? W1 : W0, mp.zi, ld, ld, tw, 1 << 30, N);
   for (int e = threadIdx.x; e < 3 * Tz * N; e += blockDim.x) {
     const int cn = dTz.div(e), t = e - cn * Tz;
     const int c = dN.div(cn), n = cn - c * N;
@@ -344,10 +344,12 @@ __device__ __forceinline__ void tensor_row(const ProbPtrs<T>& P, const MatvecPla
                                            int kxc, int ky, int kz, int Mp, int Np, C<T>& r0, C<T>& r1, C<T>& r2) {
   const C<T>* f;
   bool sxy = false, sxz = false, syz = false;
+  int cs = 1;   // stride between components
   if (mp.fsym) {
     const bool my = 2 * ky > Mp, mz = 2 * kz > Np;
     const int Hy = Mp / 2 + 1, Hz = Np / 2 + 1;
-    f = P.fN + ((int64_t(kxc) * Hz + (mz ? Np - kz : kz)) * Hy + (my ? Mp - ky : ky)) * 6;
+    f = P.fN + (int64_t(kxc) * Hz + (mz ? Np - kz : kz)) * 6 * Hy + (my ? Mp - ky : ky);
+    cs = Hy;
     sxy = mx != my;
     sxz = mx != mz;
     syz = my != mz;
@@ -355,15 +357,15 @@ __device__ __forceinline__ void tensor_row(const ProbPtrs<T>& P, const MatvecPla
     f = P.fN + ((int64_t(kxi) * Np + kz) * Mp + ky) * 6;
   }
   if (c == 0) {
-    r0 = f[0]; r1 = f[1]; r2 = f[2];
+    r0 = f[0]; r1 = f[cs]; r2 = f[2 * cs];
     if (sxy) r1 = cneg(r1);
     if (sxz) r2 = cneg(r2);
   } else if (c == 1) {
-    r0 = f[1]; r1 = f[3]; r2 = f[4];
+    r0 = f[cs]; r1 = f[3 * cs]; r2 = f[4 * cs];
     if (sxy) r0 = cneg(r0);
     if (syz) r2 = cneg(r2);
   } else {
-    r0 = f[2]; r1 = f[4]; r2 = f[5];
+    r0 = f[2 * cs]; r1 = f[4 * cs]; r2 = f[5 * cs];
     if (sxz) r0 = cneg(r0);
     if (syz) r1 = cneg(r1);
   }

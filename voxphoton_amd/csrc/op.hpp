@@ -85,7 +85,7 @@ template <typename T> void upload_twiddles(OpPrec<T>& P, hipStream_t s);
 // Writes the class-layout tensor of geometry g from a true-frequency device
 // tensor F (Lp x Mp x Np x 6 complex128, column-major), times scale.
 template <typename T> void layout_fN(const Geometry& g, const double2* F, DevBuf& out, double scale, hipStream_t s);
-// Octant layout [kxc][kzc][kyc][6] of a parity-symmetric tensor (split variant).
+// Octant layout [kxc][kzc][comp][kyc] of a parity-symmetric tensor (split variant).
 template <typename T>
 void layout_fN_sym(const Geometry& g, const double2* F, DevBuf& out, double scale, hipStream_t s);
 // True when F(P - k) = parity * F(k) along every axis (to 1e-12 of max |F|).

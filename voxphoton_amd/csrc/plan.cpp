@@ -6,7 +6,7 @@
 
 namespace vp {
 
-bool factorize(int n, bool fp64, FftPlan& out, int tws) {
+bool factorize(int n, bool fp64, FftPlan& out, int tws, bool odd_first) {
   out.n = n;
   out.tws = tws;
   out.nst = 0;
@@ -20,17 +20,24 @@ bool factorize(int n, bool fp64, FftPlan& out, int tws) {
     m /= r;
     return true;
   };
+  const int odd32[] = {13, 11, 7, 5, 3};
+  const int odd64[] = {11, 7, 5, 3};
+  const int* odd = fp64 ? odd64 : odd32;
+  const int nodd = fp64 ? 4 : 5;
+  auto odds = [&]() {
+    for (int i = 0; i < nodd; ++i)
+      while (m % odd[i] == 0) if (!push(odd[i])) return false;
+    return true;
+  };
+  // largest odd radices first when asked (a zero-padded input then prunes
+  // the most expensive stage), else powers of two first
+  if (odd_first && !odds()) return false;
   // powers of two: 16 (single precision only), 8, 4, then a final 2
   if (!fp64) while (m % 16 == 0) if (!push(16)) return false;
   while (m % 8 == 0) if (!push(8)) return false;
   while (m % 4 == 0) if (!push(4)) return false;
   while (m % 2 == 0) if (!push(2)) return false;
-  const int odd32[] = {13, 11, 7, 5, 3};
-  const int odd64[] = {11, 7, 5, 3};
-  const int* odd = fp64 ? odd64 : odd32;
-  const int nodd = fp64 ? 4 : 5;
-  for (int i = 0; i < nodd; ++i)
-    while (m % odd[i] == 0) if (!push(odd[i])) return false;
+  if (!odd_first && !odds()) return false;
   return m == 1;
 }
 
@@ -75,7 +82,10 @@ bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out,
   if (variant == 1) {
     // ---- split passes: y sweeps over By pencils, z sweep over Tz ky columns
     const int Mp = g.y.P, Np = g.z.P;
-    const size_t budget = 80 * 1024;   // two workgroups per CU
+    // LDS per workgroup: about 40 KB (four workgroups per CU) measured best
+    // for the z pass in both precisions; VP_SPLIT_LDS overrides (bytes)
+    size_t budget = 40 * 1024;
+    if (const char* e = std::getenv("VP_SPLIT_LDS")) budget = size_t(std::max(4096, std::atoi(e)));
     out.By = 0;
     for (int B : {16, 8, 4, 2, 1}) {
       const int ld = Mp + (B > 1 ? 1 : 0);
@@ -83,14 +93,20 @@ bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out,
       if (lds <= budget || B == 1) { out.By = B; out.ldy = ld; out.lds_y = lds; break; }
     }
     out.Tz = 0;
+    int tz_cap = 32;
+    if (const char* e = std::getenv("VP_ZTILE")) tz_cap = std::max(1, std::atoi(e));
     for (int Tz : {32, 16, 8, 4, 2, 1}) {
-      if (Tz > Mp) continue;
+      if (Tz > Mp || Tz > tz_cap) continue;
       const size_t lds = (2 * size_t(3) * Tz * Np + Np) * s;
       if (lds <= budget || Tz == 1) { out.Tz = Tz; out.lds_z = lds; break; }
     }
     if (out.lds_y > size_t(lds_max) || out.lds_z > size_t(lds_max)) { why = "split passes exceed LDS"; return false; }
     out.ty = 256;
     out.tz = 256;
+    // z transforms of the split pass: forward with the odd radices first
+    // (pruned zero-padded input), inverse with them last (pruned outputs)
+    out.zi = g.z.plan;
+    if (!factorize(Np, fp64, out.zf, 1, true)) { why = "z FFT length does not factor"; return false; }
     // register z-column kernel (VP_ZCOL=1): Np = lanes x in-register radix; the LDS pass is the default
     out.zr1 = out.zr2 = 0;
     const char* ez = std::getenv("VP_ZCOL");
@@ -127,8 +143,12 @@ bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out,
   {
     const int Ls = g.x.S;
     bool ok = false;
+    // at most two pencils per workgroup: more workgroups in flight measured
+    // faster than wider ones (VP_XPENCILS overrides)
+    int bcap = 2;
+    if (const char* e = std::getenv("VP_XPENCILS")) bcap = std::max(1, std::atoi(e));
     for (int B : {16, 8, 4, 2, 1}) {
-      if (B > 1 && B > g.M) continue;
+      if (B > 1 && (B > g.M || B > bcap)) continue;
       const int ld = Ls + (B > 1 ? 1 : 0);
       const size_t lds = 2 * size_t(B) * ld * s;
       if (lds > size_t(lds_max)) continue;

@@ -330,7 +330,9 @@ bool fN_parity_symmetric(const vp_op_s& op, const double2* F, hipStream_t s) {
   return std::isfinite(dmax) && dmax <= 1e-12 * amax;
 }
 
-// Octant layout [kxc][kzc][kyc][6], kxc <= Lp/2, kyc <= Mp/2, kzc <= Np/2.
+// Octant layout [kxc][kzc][comp][kyc], kxc <= Lp/2, kyc <= Mp/2, kzc <= Np/2:
+// component-major inside a (kx, kz) slab, so work items on consecutive ky read
+// consecutive words.
 template <typename T>
 __global__ void k_layout_fN_sym(const double2* __restrict__ F, C<T>* __restrict__ out, int Lp, int Mp, int Np,
                                 double scale) {
@@ -339,8 +341,8 @@ __global__ void k_layout_fN_sym(const double2* __restrict__ F, C<T>* __restrict_
   for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
        e += int64_t(gridDim.x) * blockDim.x) {
     int64_t r = e;
-    const int k = int(r % 6); r /= 6;
     const int ky = int(r % Hy); r /= Hy;
+    const int k = int(r % 6); r /= 6;
     const int kz = int(r % Hz); r /= Hz;
     const int kx = int(r);
     const double2 v = F[kx + int64_t(Lp) * (ky + int64_t(Mp) * (kz + int64_t(Np) * k))];

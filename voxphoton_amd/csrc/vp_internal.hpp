@@ -47,6 +47,7 @@ struct MatvecPlan {
   int By, ldy, ty, Tz, tz;
   int fsym;      // split variant: fN stored as its parity octant
   int zr1, zr2;  // split variant: register z-column kernel Np = zr1 * zr2 (zr1 = 0: LDS z pass)
+  FftPlan zf, zi; // split variant: z plans of the forward (odd radices first) and inverse transforms
   size_t lds_y, lds_z;
   // K1/K3 (x pencils)
   int B;         // pencils per workgroup
@@ -117,7 +118,7 @@ struct ProfScope {
 };
 
 // Host-side planning helpers (plan.cpp)
-bool factorize(int n, bool fp64, FftPlan& out, int tws);
+bool factorize(int n, bool fp64, FftPlan& out, int tws, bool odd_first = false);
 int max_lds_bytes(int device);
 
 template <typename T>
