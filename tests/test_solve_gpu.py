@@ -45,15 +45,17 @@ def test_chan1_apply_matches_oracle(vp_ctx):
     assert rel(P.apply(x), ref) < 1e-12
 
 
-@pytest.mark.parametrize('split', ['0', '1'])
+@pytest.mark.parametrize('mode', ['fused', 'split', 'quad'])
 @pytest.mark.parametrize('sym', ['1', '0'])
-@pytest.mark.parametrize('shape', [(6, 4, 3), (10, 5, 2), (7, 5, 3)])
-def test_level2_device_setup_matches_oracle(vp_ctx, monkeypatch, shape, sym, split):
+@pytest.mark.parametrize('shape', [(6, 4, 3), (10, 5, 2), (7, 5, 3), (9, 8, 4)])
+def test_level2_device_setup_matches_oracle(vp_ctx, monkeypatch, shape, sym, mode):
     # sym=1 stores the (kx <= L/2, ky <= M/2) block quadrant, sym=0 all blocks;
-    # split=1 runs the plane as y-row DFT / block-product / inverse passes
+    # the plane runs fused, as y-row DFT / block-product / inverse passes
+    # (split), or with one workgroup per stored block (quad, sym=1 only)
     from voxphoton_amd.vie import Preconditioner
     monkeypatch.setenv('VP_PC_SYMMETRY', sym)
-    monkeypatch.setenv('VP_PC_SPLIT', split)
+    monkeypatch.setenv('VP_PC_SPLIT', '1' if mode == 'split' else '0')
+    monkeypatch.setenv('VP_PC_QUAD', '1' if mode == 'quad' else '0')
     L, M, N = shape
     dx, ko = 20e-9, 5.85e6
     G = ogr.assemble_G(L, M, N, dx, ko)
@@ -85,10 +87,12 @@ def test_level2_blocks_wider_than_a_wavefront(vp_ctx):
     assert rel(P.apply(x, 'fp32'), ref) < 1e-4
 
 
+@pytest.mark.parametrize('quad', ['1', '0'])
 @pytest.mark.parametrize('sym', ['1', '0'])
-def test_level2_face_matches_oracle(vp_ctx, monkeypatch, sym):
+def test_level2_face_matches_oracle(vp_ctx, monkeypatch, sym, quad):
     from voxphoton_amd.vie import Preconditioner
     monkeypatch.setenv('VP_PC_SYMMETRY', sym)
+    monkeypatch.setenv('VP_PC_QUAD', quad)
     L, M, N = 8, 5, 4
     dx, ko = 20e-9, 5.85e6
     G = ogr.assemble_G(L, M, N, dx, ko)

@@ -71,20 +71,25 @@ static void prec_geometry(vp_prec_s* p, int level) {
   g.t2 = 512;
   g.lds2 = size_t(2) * 3 * g.N * g.M * s;
   g.psplit = 0;
-  const char* e = std::getenv("VP_PC_SPLIT");
-  if (level == 2 && (g.lds2 > size_t(lds_max) || (e && e[0] == '1'))) {
-    g.psplit = 1;
-    g.Brow = 0;
+  g.pquad = 0;
+  g.Brow = 0;
+  if (level == 2) {
+    // y-row DFT passes (split and quadrant paths): Brow rows per workgroup
     for (int B : {16, 8, 4, 2, 1}) {
       const int ld = g.M + (B > 1 ? 1 : 0);
-      if (2 * size_t(B) * ld * s <= size_t(std::min(lds_max, 96 * 1024)) || B == 1) {
+      if (2 * size_t(B) * ld * s <= size_t(std::min(lds_max, 40 * 1024)) || B == 1) {
         g.Brow = B;
         g.ldrow = ld;
         g.lds_row = 2 * size_t(B) * ld * s;
         break;
       }
     }
-    VP_REQUIRE(g.lds_row <= size_t(lds_max), ERR_UNSUPPORTED, "preconditioner y rows exceed LDS");
+    if (g.lds_row > size_t(lds_max)) g.Brow = 0;
+  }
+  const char* e = std::getenv("VP_PC_SPLIT");
+  if (level == 2 && (g.lds2 > size_t(lds_max) || (e && e[0] == '1'))) {
+    g.psplit = 1;
+    VP_REQUIRE(g.Brow > 0, ERR_UNSUPPORTED, "preconditioner y rows exceed LDS");
     g.TK = 1;
     for (int tk : {64, 32, 16, 8, 4, 2, 1})
       if ((tk <= g.M || tk == 1) && size_t(3) * g.N * tk * s <= 48 * 1024) {
@@ -445,6 +450,10 @@ static void level2_block_range(vp_prec_s* p, const double2* dG) {
   g.bsym = (!(e && std::strcmp(e, "0") == 0) && toeplitz_parity_zero(dG, g.L, g.M, g.N, p->ctx->stream)) ? 1 : 0;
   g.Hx = g.bsym ? g.L / 2 + 1 : g.L;
   g.Hy = g.bsym ? g.M / 2 + 1 : g.M;
+  // quadrant blocks: one read per stored block for its four mirrored
+  // columns (VP_PC_QUAD=0 keeps the per-plane kernel)
+  const char* eq = std::getenv("VP_PC_QUAD");
+  g.pquad = (g.bsym && 3 * g.N <= 255 && g.Brow > 0 && !(eq && eq[0] == '0')) ? 1 : 0;
 }
 
 static int prec_level2_common(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double2* dG, double mc_re,

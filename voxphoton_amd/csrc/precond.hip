@@ -195,6 +195,65 @@ __global__ void __launch_bounds__(256) k_pc_blockmul(PcArgs<T> a, PcGeom g) {
   }
 }
 
+// Quadrant block products (blocks stored for kx <= L/2, ky <= M/2): one
+// workgroup per stored block B(kxc, kyc) applies it to the up to four
+// columns (kx, ky) in {kxc, L-kxc} x {kyc, M-kyc} that share it, so every
+// block entry is read once and feeds four products.  The mirrored blocks
+// are S B S (S negates the x or y component): the sign of the column's
+// component is folded into the staged vector, the row sign into the result.
+template <typename T>
+__global__ void __launch_bounds__(256) k_pc_quadmul(PcArgs<T> a, PcGeom g) {
+  if (a.active && !a.active[blockIdx.y]) return;
+  C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
+  const C<T>* __restrict__ blocks = a.blocks[blockIdx.y];
+  const int L = g.L, M = g.M, N = g.N, n3 = 3 * N, Hy = g.Hy;
+  __shared__ C<T> Xs[4][3 * 85];
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int kxc = blk / Hy, kyc = blk - kxc * Hy;
+  int kxs[4], kys[4];
+  bool mxs[4], mys[4], use[4];
+  const bool twox = kxc != 0 && 2 * kxc != L, twoy = kyc != 0 && 2 * kyc != M;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    mxs[c] = (c & 1) != 0;
+    mys[c] = (c & 2) != 0;
+    use[c] = (!mxs[c] || twox) && (!mys[c] || twoy);
+    kxs[c] = mxs[c] ? L - kxc : kxc;
+    kys[c] = mys[c] ? M - kyc : kyc;
+  }
+  for (int e = threadIdx.x; e < 4 * n3; e += blockDim.x) {
+    const int c = e / n3, q = e - c * n3;
+    C<T> v = czero<T>();
+    if (use[c]) {
+      v = work[(int64_t(kxs[c]) * n3 + q) * M + kys[c]];
+      const int cq = q / N;
+      if ((cq == 0 && mxs[c]) || (cq == 1 && mys[c])) v = cneg(v);
+    }
+    Xs[c][q] = v;
+  }
+  __syncthreads();
+  const C<T>* Bb = blocks + (int64_t(kxc) * Hy + kyc) * n3 * n3;
+  for (int r = threadIdx.x; r < n3; r += blockDim.x) {
+    C<T> acc0 = czero<T>(), acc1 = czero<T>(), acc2 = czero<T>(), acc3 = czero<T>();
+    for (int q = 0; q < n3; ++q) {
+      const C<T> b = Bb[int64_t(q) * n3 + r];
+      acc0 = cadd(acc0, cmul(b, Xs[0][q]));
+      acc1 = cadd(acc1, cmul(b, Xs[1][q]));
+      acc2 = cadd(acc2, cmul(b, Xs[2][q]));
+      acc3 = cadd(acc3, cmul(b, Xs[3][q]));
+    }
+    const int cr = r / N;
+    const C<T> accs[4] = {acc0, acc1, acc2, acc3};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (!use[c]) continue;
+      C<T> v = accs[c];
+      if ((cr == 0 && mxs[c]) || (cr == 1 && mys[c])) v = cneg(v);
+      work[(int64_t(kxs[c]) * n3 + r) * M + kys[c]] = v;
+    }
+  }
+}
+
 // x pass inverse + gather into the output vector.
 template <typename T>
 __global__ void __launch_bounds__(512) k_pc_xinv(PcArgs<T> a, PcGeom g) {
@@ -282,7 +341,26 @@ void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) 
     hipLaunchKernelGGL(k_pc_xfwd<T>, gx, dim3(g.tx), lds1, s, a, g);
   }
   VP_HIP(hipGetLastError());
-  if (g.level == 2 && g.psplit) {
+  if (g.level == 2 && g.pquad) {
+    const size_t ldr = g.lds_row * sizeof(C<T>) / 16;
+    const void* fr0 = reinterpret_cast<const void*>(&k_pc_yrows<T, false>);
+    const void* fr1 = reinterpret_cast<const void*>(&k_pc_yrows<T, true>);
+    (void)hipFuncSetAttribute(fr0, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldr));
+    (void)hipFuncSetAttribute(fr1, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldr));
+    const int64_t nrows = int64_t(g.L) * 3 * g.N;
+    const dim3 gr(unsigned((nrows + g.Brow - 1) / g.Brow), nprob);
+    const dim3 gq(unsigned(int64_t(g.Hx) * g.Hy), nprob);
+    const int tq = std::min(256, ((3 * g.N + 63) / 64) * 64);
+    {
+      ProfScope ps(s, KID_PC_PLANE);
+      hipLaunchKernelGGL((k_pc_yrows<T, false>), gr, dim3(256), ldr, s, a, g);
+      hipLaunchKernelGGL(k_pc_quadmul<T>, gq, dim3(tq), 0, s, a, g);
+      hipLaunchKernelGGL((k_pc_yrows<T, true>), gr, dim3(256), ldr, s, a, g);
+    }
+    VP_HIP(hipGetLastError());
+    ProfScope ps(s, KID_PC_XINV);
+    hipLaunchKernelGGL(k_pc_xinv<T>, gx, dim3(g.tx), lds1, s, a, g);
+  } else if (g.level == 2 && g.psplit) {
     const size_t ldr = g.lds_row * sizeof(C<T>) / 16, ldt = g.lds_tile * sizeof(C<T>) / 16;
     const void* fr0 = reinterpret_cast<const void*>(&k_pc_yrows<T, false>);
     const void* fr1 = reinterpret_cast<const void*>(&k_pc_yrows<T, true>);

@@ -19,6 +19,9 @@ struct PcGeom {
   // LDS: y DFT of Brow rows per workgroup (row stride ldrow), block products
   // over (kx, TK-column tiles), inverse y DFT.
   int psplit, Brow, ldrow, TK;
+  // quadrant blocks (bsym) with 3N <= 255: y-row DFT / one workgroup per
+  // stored block applying it to its four mirrored columns / inverse y-row DFT
+  int pquad;
   size_t lds_row, lds_tile;
   size_t lds1, lds2;
   FftPlan px, py;       // unpadded DFTs of length L and M
