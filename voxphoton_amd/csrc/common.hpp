@@ -46,6 +46,56 @@ __host__ __device__ __forceinline__ V mul_mi(V a) {
 
 template <typename T> __host__ __device__ __forceinline__ C<T> czero() { C<T> r; r.x = T(0); r.y = T(0); return r; }
 
+// Buffer-resource access to a device range of at most 4 GiB: 32-bit element
+// offsets (no 64-bit address arithmetic per lane), loads counted on vmcnt
+// only (LDS waits never wait on them), and an offset past the range reads 0.
+#if defined(__HIP_DEVICE_COMPILE__)
+using BufRes = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ BufRes buf_res(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, int(bytes), 0x00020000);
+}
+// (the raw intrinsics' vector results are bit-cast whole: element access on
+// them narrows the load to its first dword)
+typedef float vp_f2v __attribute__((ext_vector_type(2)));
+typedef double vp_d2v __attribute__((ext_vector_type(2)));
+template <typename T> __device__ __forceinline__ C<T> buf_load(BufRes r, uint32_t elem);
+template <> __device__ __forceinline__ float2 buf_load<float>(BufRes r, uint32_t elem) {
+  const vp_f2v v = __builtin_bit_cast(vp_f2v, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 8u, 0, 0));
+  float2 o;
+  o.x = v.x;
+  o.y = v.y;
+  return o;
+}
+template <> __device__ __forceinline__ double2 buf_load<double>(BufRes r, uint32_t elem) {
+  const vp_d2v v = __builtin_bit_cast(vp_d2v, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 16u, 0, 0));
+  double2 o;
+  o.x = v.x;
+  o.y = v.y;
+  return o;
+}
+__device__ __forceinline__ void buf_store(BufRes r, uint32_t elem, float2 a) {
+  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+  vp_f2v v;
+  v.x = a.x;
+  v.y = a.y;
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, elem * 8u, 0, 0);
+}
+__device__ __forceinline__ void buf_store(BufRes r, uint32_t elem, double2 a) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  vp_d2v v;
+  v.x = a.x;
+  v.y = a.y;
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, elem * 16u, 0, 0);
+}
+#else
+// host pass: declarations only (device code is never run on the host)
+struct BufRes {};
+__device__ BufRes buf_res(const void* base, uint32_t bytes);
+template <typename T> __device__ C<T> buf_load(BufRes r, uint32_t elem);
+__device__ void buf_store(BufRes r, uint32_t elem, float2 a);
+__device__ void buf_store(BufRes r, uint32_t elem, double2 a);
+#endif
+
 // Maps the hardware's round-robin placement of workgroups over the 8 XCDs
 // to a logical block order in which consecutive logical blocks run on the
 // same XCD (and share its L2).  Valid for any total; the tail beyond the

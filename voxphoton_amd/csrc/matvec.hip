@@ -314,6 +314,160 @@ __global__ void __launch_bounds__(256) k_zpass(const ProbPtrs<T>* __restrict__ p
 }
 
 // ----------------------------------------------------------------------------
+// Two-stage z pass for Np = RA * RB, fused around the tensor multiply.
+// Forward plan [RA, RB] and inverse plan [RB, RA] (Stockham, natural order):
+//   A : radix-RA butterflies j < RB on inputs n = j + RB*r (zero for n >= N),
+//       read straight from the y spectra; outputs to LDS slots j*RA + k;
+//   B : radix-RB butterflies j < RA on slots j + RA*r with twiddles w^{jr},
+//       which leaves X[kz = j + RA*r] in registers for all three
+//       components; the tensor multiply at those kz; then the first inverse
+//       stage (radix RB, Ns = 1) runs on the same registers, outputs to
+//       slots j*RB + k;
+//   B': radix-RA butterflies j < RB on slots j + RB*r with twiddles w^{-jr};
+//       outputs n = j + RB*r < N go straight back to the spectra.
+// A workgroup takes TZ ky columns of one kx plane with tpk >= RA threads per
+// column (thread = w*TZ + t): A and B' loop over the 3*RB (component,
+// butterfly) items, B gives butterfly j = w to one thread for all three
+// components.  One LDS buffer of Np x 3 x TZ values, element (slot, c, t) at
+// slot*3*TZ + c*TZ + t; a wavefront's accesses are runs of TZ words.
+// ----------------------------------------------------------------------------
+template <typename T, int RA, int RB, int TZ>
+__global__ void __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) k_zfused(const ProbPtrs<T>* __restrict__ probs, const int* __restrict__ active,
+                                                Geometry g, MatvecPlan mp, const C<T>* __restrict__ twz) {
+  constexpr int LD = 3 * TZ;
+  if (active && !active[blockIdx.y]) return;
+  const ProbPtrs<T> P = probs[blockIdx.y];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W = reinterpret_cast<C<T>*>(smem_raw);
+  const int N = g.N, Mp = g.y.P, Lp = g.x.P, tpk = mp.ztpk;
+  const int nt = (Mp + TZ - 1) / TZ;
+  // kx order and residue-class storage as in k_zpass
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int io = blk / nt;
+  const int jo = (io + 1) >> 1;
+  const int kx = io == 0 ? 0 : ((io & 1) ? jo : Lp - jo);
+  const int kxi = (kx % g.x.R) * g.x.S + kx / g.x.R;
+  const int t = threadIdx.x % TZ, w = threadIdx.x / TZ;
+  const int ky = (blk - io * nt) * TZ + t;
+  const bool kyok = ky < Mp;
+  // the kx plane of the y spectra, Y[c][n][ky] at (c*N + n)*Mp + ky
+  const uint32_t MpN = uint32_t(Mp) * N;
+  const BufRes Y = buf_res(P.work2 + int64_t(kxi) * 3 * MpN, 3 * MpN * uint32_t(sizeof(C<T>)));
+  const int rc = (N + RB - 1) / RB;                              // A inputs that can be nonzero
+
+  // ---- A: forward radix-RA stage from the spectra
+  for (int it = w; it < 3 * RB; it += tpk) {
+    const int c = it / RB, j = it - c * RB;
+    C<T> v[RA];
+#pragma unroll
+    for (int r = 0; r < RA; ++r) {
+      const int n = j + RB * r;
+      v[r] = czero<T>();
+      if (r < rc && kyok && n < N) v[r] = buf_load<T>(Y, c * MpN + uint32_t(n) * Mp + ky);
+    }
+    dft<T, RA, false>(v);
+#pragma unroll
+    for (int k = 0; k < RA; ++k) W[(j * RA + k) * LD + c * TZ + t] = v[k];
+  }
+  __syncthreads();
+
+  // ---- B: second forward stage, tensor multiply, first inverse stage
+  C<T> x0[RB], x1[RB], x2[RB];
+  const bool inb = w < RA;
+  if (inb) {
+    const int j = w;
+    C<T> tw[RB];
+#pragma unroll
+    for (int r = 1; r < RB; ++r) tw[r] = twz[j * r];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int o = (j + RA * r) * LD + t;
+      x0[r] = W[o];
+      x1[r] = W[o + TZ];
+      x2[r] = W[o + 2 * TZ];
+      if (r > 0) {
+        x0[r] = cmul(x0[r], tw[r]);
+        x1[r] = cmul(x1[r], tw[r]);
+        x2[r] = cmul(x2[r], tw[r]);
+      }
+    }
+    dft<T, RB, false>(x0);
+    dft<T, RB, false>(x1);
+    dft<T, RB, false>(x2);
+    if (kyok) {
+      constexpr int Np = RA * RB;
+      const bool mx = 2 * kx > Lp, my = 2 * ky > Mp;
+      const int kxc = mx ? Lp - kx : kx, kyc = my ? Mp - ky : ky;
+      const uint32_t Hy = Mp / 2 + 1, Hz = Np / 2 + 1;
+      // tensor slab of this kx: octant [kzc][comp][kyc] or full [kz][ky][comp]
+      const BufRes F = mp.fsym ? buf_res(P.fN + int64_t(kxc) * Hz * 6 * Hy, Hz * 6 * Hy * uint32_t(sizeof(C<T>)))
+                               : buf_res(P.fN + int64_t(kxi) * Np * Mp * 6, uint32_t(Np) * Mp * 6 * sizeof(C<T>));
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int kz = j + RA * r;
+        C<T> f0, f1, f2, f3, f4, f5;
+        if (mp.fsym) {
+          const bool mz = 2 * kz > Np;
+          const uint32_t o = uint32_t(mz ? Np - kz : kz) * 6 * Hy + kyc;
+          f0 = buf_load<T>(F, o); f1 = buf_load<T>(F, o + Hy); f2 = buf_load<T>(F, o + 2 * Hy);
+          f3 = buf_load<T>(F, o + 3 * Hy); f4 = buf_load<T>(F, o + 4 * Hy); f5 = buf_load<T>(F, o + 5 * Hy);
+          if (mx != my) f1 = cneg(f1);
+          if (mx != mz) f2 = cneg(f2);
+          if (my != mz) f4 = cneg(f4);
+        } else {
+          const uint32_t o = (uint32_t(kz) * Mp + ky) * 6;
+          f0 = buf_load<T>(F, o); f1 = buf_load<T>(F, o + 1); f2 = buf_load<T>(F, o + 2);
+          f3 = buf_load<T>(F, o + 3); f4 = buf_load<T>(F, o + 4); f5 = buf_load<T>(F, o + 5);
+        }
+        const C<T> a = x0[r], b = x1[r], cc = x2[r];
+        x0[r] = cadd(cadd(cmul(f0, a), cmul(f1, b)), cmul(f2, cc));
+        x1[r] = cadd(cadd(cmul(f1, a), cmul(f3, b)), cmul(f4, cc));
+        x2[r] = cadd(cadd(cmul(f2, a), cmul(f4, b)), cmul(f5, cc));
+        // double precision: keep at most a quarter of the tensor loads in
+        // flight so the registers fit two waves per SIMD without spilling
+        if constexpr (sizeof(T) == 8 && RB >= 8) {
+          if ((r + 1) % (RB / 4) == 0) __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    }
+    dft<T, RB, true>(x0);
+    dft<T, RB, true>(x1);
+    dft<T, RB, true>(x2);
+  }
+  __syncthreads();   // every slot read before the buffer is reused
+  if (inb) {
+    const int j = w;
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+      const int o = (j * RB + k) * LD + t;
+      W[o] = x0[k];
+      W[o + TZ] = x1[k];
+      W[o + 2 * TZ] = x2[k];
+    }
+  }
+  __syncthreads();
+
+  // ---- B': last inverse stage (radix RA), cropped to n < N
+  for (int it = w; it < 3 * RB; it += tpk) {
+    const int c = it / RB, j = it - c * RB;
+    C<T> v[RA];
+#pragma unroll
+    for (int r = 0; r < RA; ++r) {
+      v[r] = W[(j + RB * r) * LD + c * TZ + t];
+      if (r > 0) v[r] = cmul(v[r], twiddle<T, true>(twz, j * r));
+    }
+    dft<T, RA, true>(v);
+    if (kyok) {
+#pragma unroll
+      for (int r = 0; r < RA; ++r) {
+        const int n = j + RB * r;
+        if (n < N) buf_store(Y, c * MpN + uint32_t(n) * Mp + ky, v[r]);
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
 // Register-resident z pass for Np = R1 * R2.  One wavefront per component,
 // TC = 64 / R1 ky columns per wavefront, column spread over R1 lanes: lane
 // (l, t) holds x[R1*a + l], a < R2, of column ky0 + t.  Forward transform:
@@ -588,7 +742,26 @@ void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d
     }
     {
       ProfScope ps(s, KID_ZPASS);
-      if (mp.zr1 > 0) {
+      if (mp.zra > 0) {
+        const dim3 gz(unsigned(g.x.P * ((g.y.P + kZfTile - 1) / kZfTile)), nprob);
+        const dim3 bz(unsigned(kZfTile * mp.ztpk));
+        bool zdone = false;
+#define VP_ZF(A, B)                                                                                     \
+  if (!zdone && mp.zra == A && mp.zrb == B) {                                                           \
+    const void* fn = reinterpret_cast<const void*>(&k_zfused<T, A, B, kZfTile>);                        \
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds_zf));          \
+    hipLaunchKernelGGL((k_zfused<T, A, B, kZfTile>), gz, bz, mp.lds_zf, s, d_probs, va.active, g, mp, twz); \
+    zdone = true;                                                                                       \
+  }
+        VP_ZF(11, 8) VP_ZF(11, 4) VP_ZF(11, 2) VP_ZF(7, 8) VP_ZF(7, 4) VP_ZF(7, 2) VP_ZF(5, 8) VP_ZF(5, 4)
+        VP_ZF(5, 2) VP_ZF(3, 8) VP_ZF(3, 4) VP_ZF(3, 2) VP_ZF(3, 3) VP_ZF(8, 8) VP_ZF(8, 4) VP_ZF(4, 4)
+        VP_ZF(4, 2)
+        if constexpr (sizeof(T) == 4) {
+          VP_ZF(11, 16) VP_ZF(7, 16) VP_ZF(5, 16) VP_ZF(3, 16) VP_ZF(16, 8)
+        }
+#undef VP_ZF
+        if (!zdone) throw VpError{ERR_UNSUPPORTED, "matvec: no fused z kernel for this radix pair"};
+      } else if (mp.zr1 > 0) {
         const dim3 gz(unsigned(g.x.P * ((g.y.P + 64 / mp.zr1 - 1) / (64 / mp.zr1))), nprob);
 #define VP_ZCOL(A, B)                                                                                   \
   if (mp.zr1 == A && mp.zr2 == B) {                                                                     \

@@ -61,6 +61,45 @@ static int pow2ceil(int v) {
   return p;
 }
 
+// Radix pairs (RA, RB) of the fused z kernel, instantiated in matvec.hip in
+// the same order; the first pair with RA * RB = Np is used.
+static const int kZfPairs[][3] = {   // RA, RB, single precision only
+    {11, 8, 0}, {11, 4, 0}, {11, 2, 0}, {7, 8, 0}, {7, 4, 0}, {7, 2, 0}, {5, 8, 0}, {5, 4, 0}, {5, 2, 0},
+    {3, 8, 0},  {3, 4, 0},  {3, 2, 0},  {3, 3, 0}, {8, 8, 0}, {8, 4, 0}, {4, 4, 0}, {4, 2, 0},
+    {11, 16, 1}, {7, 16, 1}, {5, 16, 1}, {3, 16, 1}, {16, 8, 1}};
+
+bool zfused_pair(int Np, bool fp64, int& ra, int& rb) {
+  for (const auto& p : kZfPairs) {
+    if (p[0] * p[1] != Np || (fp64 && p[2])) continue;
+    ra = p[0];
+    rb = p[1];
+    return true;
+  }
+  return false;
+}
+
+// Threads per ky column: at least RA (stage B gives one butterfly per
+// thread), at most 256 / kZfTile; chosen by a VALU cost model of the three
+// stages (A and B' loop over 3*RB items, B runs once).
+int zfused_threads(int ra, int rb) {
+  auto dftc = [](int r) {
+    switch (r) {
+      case 2: return 4; case 3: return 12; case 4: return 16; case 5: return 32; case 7: return 64;
+      case 8: return 56; case 11: return 130; default: return 144;
+    }
+  };
+  const int items = 3 * rb;
+  const long long ca = dftc(ra) + 3 * ra, cb = 3 * (2 * dftc(rb) + 5 * rb) + 40 * rb;
+  int best = ra;
+  long long bestc = -1;
+  for (int tpk = ra; tpk <= 256 / kZfTile; ++tpk) {
+    const long long passes = (items + tpk - 1) / tpk;
+    const long long c = tpk * (2 * passes * ca + cb);
+    if (bestc < 0 || c < bestc) { bestc = c; best = tpk; }
+  }
+  return best;
+}
+
 // Mirrors acc_bound<T, EPT>() in matvec.hip.
 static int acc_bound(bool fp64, int ept) { return (fp64 ? ept <= 1 : ept <= 4) ? 1024 : 512; }
 
@@ -118,6 +157,22 @@ bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out,
           out.zr1 = r1;
           out.zr2 = r2;
           break;
+        }
+      }
+    }
+    // fused two-stage z kernel (default; VP_ZFUSED=0 keeps the LDS z pass)
+    out.zra = out.zrb = out.ztpk = 0;
+    out.lds_zf = 0;
+    const char* ef = std::getenv("VP_ZFUSED");
+    if (out.zr1 == 0 && !(ef && ef[0] == '0')) {
+      int ra = 0, rb = 0;
+      if (zfused_pair(Np, fp64, ra, rb)) {
+        const size_t lds = size_t(Np) * 3 * kZfTile * s;
+        if (lds <= size_t(lds_max)) {
+          out.zra = ra;
+          out.zrb = rb;
+          out.ztpk = zfused_threads(ra, rb);
+          out.lds_zf = lds;
         }
       }
     }

@@ -47,6 +47,8 @@ struct MatvecPlan {
   int By, ldy, ty, Tz, tz;
   int fsym;      // split variant: fN stored as its parity octant
   int zr1, zr2;  // split variant: register z-column kernel Np = zr1 * zr2 (zr1 = 0: LDS z pass)
+  int zra, zrb, ztpk;  // split variant: fused two-stage z kernel Np = zra * zrb, ztpk threads per ky column (0: off)
+  size_t lds_zf;       // its LDS bytes
   FftPlan zf, zi; // split variant: z plans of the forward (odd radices first) and inverse transforms
   size_t lds_y, lds_z;
   // K1/K3 (x pencils)
@@ -119,6 +121,11 @@ struct ProfScope {
 
 // Host-side planning helpers (plan.cpp)
 bool factorize(int n, bool fp64, FftPlan& out, int tws, bool odd_first = false);
+// Fused two-stage z kernel: ky columns per workgroup, radix pair for Np and
+// threads per column (plan.cpp).
+constexpr int kZfTile = 16;
+bool zfused_pair(int Np, bool fp64, int& ra, int& rb);
+int zfused_threads(int ra, int rb);
 int max_lds_bytes(int device);
 
 template <typename T>
