@@ -77,15 +77,35 @@ def unknown_box(prob):
     return tuple(ext)
 
 
-def kernel_model_bytes(prob, kernel, s, fsym=True):
-    """Algorithmic bytes per launch of one matvec kernel as built.
+def kernel_model_bytes(prob, kernel, s, fsym=True, pc_box=None):
+    """Algorithmic bytes per launch of one matvec or preconditioner kernel.
 
     The passes run on the unknowns' bounding box Lb x Mb x Nb inside the
     Lp x Mp x Np FFT grid: Qb = Lp*Mb*Nb x-spectra and Yb = Lp*Mp*Nb
     y-spectra per component.  The split z pass reads the parity octant of
     fN, Pc = (Lp/2+1)(Mp/2+1)(Np/2+1) entries per component (fsym), the
     fused plane kernel the full P = Lp*Mp*Np.
+
+    Preconditioner kernels (pc_box = (Lc, Mc, Nc): two circulant axes and
+    the dense one): the quadrant form stores (Lc/2+1)(Mc/2+1) blocks of
+    (3 Nc)^2 entries, read once per apply ('pc_block', which also reads and
+    writes the 3 Lc Mc Nc spectra); 'pc_yrows' reads and writes the spectra.
     """
+    if kernel.startswith('pc_'):
+        if pc_box is None:
+            return None
+        Lc, Mc, Nc = pc_box
+        V3 = 3 * Lc * Mc * Nc
+        n = prob.n
+        if kernel == 'pc_block':
+            return s * ((Lc // 2 + 1) * (Mc // 2 + 1) * (3 * Nc) ** 2 + 2 * V3)
+        if kernel == 'pc_yrows':
+            return s * 2 * V3
+        if kernel == 'pc_xfwd':
+            return s * (n + V3)
+        if kernel == 'pc_xinv':
+            return s * (V3 + 2 * n)
+        return None
     L, M, N = prob.shape
     Lp, Mp, Np = 2 * L, 2 * M, 2 * N
     Lb, Mb, Nb = unknown_box(prob)
@@ -108,6 +128,20 @@ def kernel_model_bytes(prob, kernel, s, fsym=True):
     if kernel == 'xinv':      # x-spectra read, J and Mc read, output written, voxel map read
         return s * (3 * Qb + 2 * n + nS) + 4 * nS
     return None
+
+
+def kernel_table(prob, stats, pc_box):
+    """Per-kernel launches, mean duration (HIP events) and, where the byte
+    model covers the kernel, its FP32 algorithmic GB/s."""
+    out = {}
+    for k, (cnt, ms) in stats.items():
+        avg = ms / 1e3 / max(cnt, 1)
+        row = dict(launches=cnt, avg_us=round(avg * 1e6, 2))
+        kb = kernel_model_bytes(prob, k, 8, pc_box=pc_box)
+        if kb is not None and cnt:
+            row['GBps'] = round(kb / avg / 1e9, 1)
+        out[k] = row
+    return out
 
 
 def main():
@@ -186,7 +220,8 @@ def main():
         return out
 
     L.check(L.lib.vp_ctx_set_profiling(ctx.handle, 1))
-    kernels = ['xfwd', 'plane', 'yfwd', 'zpass', 'yinv', 'xinv', 'pc_xfwd', 'pc_plane', 'pc_xinv']
+    kernels = ['xfwd', 'plane', 'yfwd', 'zpass', 'yinv', 'xinv', 'pc_xfwd', 'pc_plane', 'pc_yrows', 'pc_block',
+               'pc_xinv']
     for k in kernels:
         l_, m_ = ctypes.c_int64(), ctypes.c_double()
         L.check(L.lib.vp_ctx_kernel_stats(ctx.handle, k.encode(), ctypes.byref(l_), ctypes.byref(m_), 1))
@@ -270,7 +305,8 @@ def main():
     dom = max(stats, key=lambda k: stats[k][1])
     launches, ms = stats[dom]
     avg_s = ms / 1e3 / max(launches, 1)
-    kb = kernel_model_bytes(prob, dom, 8)
+    pc_box = getattr(prec, 'box_shape', None)
+    kb = kernel_model_bytes(prob, dom, 8, pc_box=pc_box)
     roof = None
     if kb is not None and launches:
         achieved = kb / avg_s / 1e9
@@ -320,7 +356,7 @@ def main():
                    'fp64_GBps': round(B64 / t_mv64 / 1e9, 1), 'fp32_GBps': round(B32 / t_mv32 / 1e9, 1),
                    'fp64_frac_hbm': round(B64 / t_mv64 / HBM_PEAK, 4),
                    'fp32_frac_hbm': round(B32 / t_mv32 / HBM_PEAK, 4)},
-        'kernel_ms': {k: dict(launches=v[0], avg_us=round(v[1] * 1e3 / max(v[0], 1), 2)) for k, v in stats.items()},
+        'kernel_ms': kernel_table(prob, stats, pc_box),
         'setup_s': {'operator': round(t_op, 3), 'preconditioner': round(t_pc, 3)},
         'roofline': roof,
         'cpu_baseline': cpu,
@@ -591,6 +627,7 @@ def make_preconditioner(prob, wl, vie, configs):
                                                 damping=wl['damping'])
         desc = f'2-level circulant, core contrast, box {shape}, damping {wl["damping"]}'
     P.set_embedding(n, sub)
+    P.box_shape = tuple(int(v) for v in shape)
     return P, desc
 
 

@@ -71,7 +71,9 @@ int vp_ctx_synchronize(vp_ctx ctx);
 /* Per-kernel timing on the context's stream with HIP events: after
  * vp_ctx_set_profiling(ctx, 1) every launch of the named kernels ("xfwd",
  * "plane" (fused yz pass) or "yfwd"/"zpass"/"yinv" (split yz passes), "xinv",
- * "pc_xfwd", "pc_plane", "pc_xinv") records an event pair;
+ * "pc_xfwd", "pc_plane" (fused preconditioner plane) or "pc_yrows"/"pc_block"
+ * (row transforms and block products of the split and quadrant forms),
+ * "pc_xinv") records an event pair;
  * vp_ctx_kernel_stats returns the launch count and summed milliseconds
  * (reset != 0 clears them).  No counterpart in the reference (which only
  * prints tic/toc totals); used by the benchmark's roofline figures. */

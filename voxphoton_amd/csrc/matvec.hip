@@ -234,6 +234,44 @@ __global__ void __launch_bounds__(256) k_ypass(const ProbPtrs<T>* __restrict__ p
   }
 }
 
+// Compile-time y pass for Mp = product of Rs: NB pencils (kx, c, n) per
+// workgroup, the box rows m < M in the x spectra X[p][m] and the full rows
+// of the y spectra Y[p][ky].  Forward: X -> Y (rows zero padded past M);
+// inverse: Y -> X (cropped to M).  Buffer resources cover the workgroup's
+// pencils only, so a partial last group reads zeros and drops its stores.
+template <typename T, bool INV, int NB, int... Rs>
+__global__ void __launch_bounds__(256) k_ypass_ct(const ProbPtrs<T>* __restrict__ probs,
+                                                  const int* __restrict__ active, Geometry g,
+                                                  const C<T>* __restrict__ twy) {
+  constexpr int NLEN = (Rs * ...);
+  constexpr int LD = NB + 1;
+  if (active && !active[blockIdx.y]) return;
+  const ProbPtrs<T> P = probs[blockIdx.y];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
+  C<T>* W1 = W0 + NLEN * LD;
+  C<T>* tw = W1 + NLEN * LD;
+  for (int e = threadIdx.x; e < NLEN; e += 256) tw[e] = twy[e];
+  const int M = g.M;
+  const int64_t npen = int64_t(g.x.P) * 3 * g.N;
+  const int64_t p0 = int64_t(xcd_remap(blockIdx.x, gridDim.x)) * NB;
+  const uint32_t nb = uint32_t(min<int64_t>(NB, npen - p0));
+  const BufRes X = buf_res(P.work + p0 * M, nb * uint32_t(M) * uint32_t(sizeof(C<T>)));
+  const BufRes Y = buf_res(P.work2 + p0 * NLEN, nb * uint32_t(NLEN) * uint32_t(sizeof(C<T>)));
+  __syncthreads();
+  if constexpr (!INV) {
+    auto ld = [&](int b, int k) { return k < M ? buf_load<T>(X, uint32_t(b * M + k)) : czero<T>(); };
+    auto st = [&](int b, int k, C<T> v) { buf_store(Y, uint32_t(b * NLEN + k), v); };
+    ct_fft<T, false, NLEN, NB, LD, 256, 1, true, Rs...>(W0, W1, tw, ld, st);
+  } else {
+    auto ld = [&](int b, int k) { return buf_load<T>(Y, uint32_t(b * NLEN + k)); };
+    auto st = [&](int b, int k, C<T> v) {
+      if (k < M) buf_store(X, uint32_t(b * M + k), v);
+    };
+    ct_fft<T, true, NLEN, NB, LD, 256, 1, true, Rs...>(W0, W1, tw, ld, st);
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_zpass(const ProbPtrs<T>* __restrict__ probs, const int* __restrict__ active,
                                                Geometry g, MatvecPlan mp, const C<T>* __restrict__ twz) {
@@ -715,6 +753,27 @@ __global__ void __launch_bounds__((acc_bound<T, EPT>())) k_xinv(const ProbPtrs<T
     default: throw VpError{ERR_UNSUPPORTED, "matvec: accumulator size out of range"}; \
   }
 
+template <typename T, bool INV>
+static void launch_ypass_ct(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d_probs, const int* active,
+                            int nprob, const C<T>* twy, hipStream_t s) {
+  constexpr int E = int(sizeof(C<T>));
+  const int64_t npen = int64_t(g.x.P) * 3 * g.N;
+  switch (mp.yct) {
+#define VP_YCT_LAUNCH(NL, ...)                                                                           \
+  case NL: {                                                                                            \
+    constexpr int NB = yct_pencils(NL, E);                                                              \
+    const void* fn = reinterpret_cast<const void*>(&k_ypass_ct<T, INV, NB, __VA_ARGS__>);               \
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, yct_lds(NL, E));          \
+    hipLaunchKernelGGL((k_ypass_ct<T, INV, NB, __VA_ARGS__>), dim3(unsigned((npen + NB - 1) / NB), nprob), \
+                       dim3(256), yct_lds(NL, E), s, d_probs, active, g, twy);                          \
+    break;                                                                                              \
+  }
+    VP_YCT_LIST(VP_YCT_LAUNCH)
+#undef VP_YCT_LAUNCH
+    default: throw VpError{ERR_UNSUPPORTED, "matvec: no compile-time y pass for this length"};
+  }
+}
+
 template <typename T>
 void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d_probs, VecArgs<T> va, int nprob,
                    int transp, const C<T>* twx, const C<T>* twy, const C<T>* twz, hipStream_t s) {
@@ -738,7 +797,8 @@ void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d
     (void)hipFuncSetAttribute(fz, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds_z));
     {
       ProfScope ps(s, KID_YFWD);
-      hipLaunchKernelGGL((k_ypass<T, false>), gy, dim3(mp.ty), mp.lds_y, s, d_probs, va.active, g, mp, twy);
+      if (mp.yct) launch_ypass_ct<T, false>(g, mp, d_probs, va.active, nprob, twy, s);
+      else hipLaunchKernelGGL((k_ypass<T, false>), gy, dim3(mp.ty), mp.lds_y, s, d_probs, va.active, g, mp, twy);
     }
     {
       ProfScope ps(s, KID_ZPASS);
@@ -781,7 +841,8 @@ void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d
     }
     {
       ProfScope ps(s, KID_YINV);
-      hipLaunchKernelGGL((k_ypass<T, true>), gy, dim3(mp.ty), mp.lds_y, s, d_probs, va.active, g, mp, twy);
+      if (mp.yct) launch_ypass_ct<T, true>(g, mp, d_probs, va.active, nprob, twy, s);
+      else hipLaunchKernelGGL((k_ypass<T, true>), gy, dim3(mp.ty), mp.lds_y, s, d_probs, va.active, g, mp, twy);
     }
     VP_HIP(hipGetLastError());
   } else {

@@ -100,6 +100,16 @@ int zfused_threads(int ra, int rb) {
   return best;
 }
 
+bool yct_supported(int n) {
+#define VP_YCT_CASE(NL, ...) case NL:
+  switch (n) {
+    VP_YCT_LIST(VP_YCT_CASE)
+    return true;
+    default: return false;
+  }
+#undef VP_YCT_CASE
+}
+
 // Mirrors acc_bound<T, EPT>() in matvec.hip.
 static int acc_bound(bool fp64, int ept) { return (fp64 ? ept <= 1 : ept <= 4) ? 1024 : 512; }
 
@@ -160,6 +170,9 @@ bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out,
         }
       }
     }
+    // compile-time y pass where Mp has one (VP_YCT=0 keeps the generic pass)
+    const char* ey = std::getenv("VP_YCT");
+    out.yct = (!(ey && ey[0] == '0') && yct_supported(Mp) && yct_lds(Mp, int(s)) <= lds_max) ? Mp : 0;
     // fused two-stage z kernel (default; VP_ZFUSED=0 keeps the LDS z pass)
     out.zra = out.zrb = out.ztpk = 0;
     out.lds_zf = 0;

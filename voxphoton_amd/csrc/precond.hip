@@ -352,9 +352,16 @@ void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) 
     const dim3 gq(unsigned(int64_t(g.Hx) * g.Hy), nprob);
     const int tq = std::min(256, ((3 * g.N + 63) / 64) * 64);
     {
-      ProfScope ps(s, KID_PC_PLANE);
-      hipLaunchKernelGGL((k_pc_yrows<T, false>), gr, dim3(256), ldr, s, a, g);
-      hipLaunchKernelGGL(k_pc_quadmul<T>, gq, dim3(tq), 0, s, a, g);
+      // row transforms and the block products timed separately
+      {
+        ProfScope ps(s, KID_PC_YROWS);
+        hipLaunchKernelGGL((k_pc_yrows<T, false>), gr, dim3(256), ldr, s, a, g);
+      }
+      {
+        ProfScope ps(s, KID_PC_BLOCK);
+        hipLaunchKernelGGL(k_pc_quadmul<T>, gq, dim3(tq), 0, s, a, g);
+      }
+      ProfScope ps(s, KID_PC_YROWS);
       hipLaunchKernelGGL((k_pc_yrows<T, true>), gr, dim3(256), ldr, s, a, g);
     }
     VP_HIP(hipGetLastError());
@@ -372,9 +379,15 @@ void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) 
     const dim3 gr(unsigned((nrows + g.Brow - 1) / g.Brow), nprob);
     const dim3 gb(unsigned(int64_t(g.L) * ((g.M + g.TK - 1) / g.TK)), nprob);
     {
-      ProfScope ps(s, KID_PC_PLANE);
-      hipLaunchKernelGGL((k_pc_yrows<T, false>), gr, dim3(256), ldr, s, a, g);
-      hipLaunchKernelGGL(k_pc_blockmul<T>, gb, dim3(256), ldt, s, a, g);
+      {
+        ProfScope ps(s, KID_PC_YROWS);
+        hipLaunchKernelGGL((k_pc_yrows<T, false>), gr, dim3(256), ldr, s, a, g);
+      }
+      {
+        ProfScope ps(s, KID_PC_BLOCK);
+        hipLaunchKernelGGL(k_pc_blockmul<T>, gb, dim3(256), ldt, s, a, g);
+      }
+      ProfScope ps(s, KID_PC_YROWS);
       hipLaunchKernelGGL((k_pc_yrows<T, true>), gr, dim3(256), ldr, s, a, g);
     }
     VP_HIP(hipGetLastError());

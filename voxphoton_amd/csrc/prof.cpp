@@ -51,7 +51,7 @@ void resolve(ProfState& st) {
 
 const char* kernel_id_name(int kid) {
   static const char* names[kNumKernelIds] = {"xfwd", "plane", "xinv", "pc_xfwd", "pc_plane", "pc_xinv",
-                                                 "yfwd", "zpass", "yinv"};
+                                                 "yfwd", "zpass", "yinv", "pc_yrows", "pc_block"};
   return (kid >= 0 && kid < kNumKernelIds) ? names[kid] : "";
 }
 

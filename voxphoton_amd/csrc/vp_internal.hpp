@@ -49,6 +49,7 @@ struct MatvecPlan {
   int zr1, zr2;  // split variant: register z-column kernel Np = zr1 * zr2 (zr1 = 0: LDS z pass)
   int zra, zrb, ztpk;  // split variant: fused two-stage z kernel Np = zra * zrb, ztpk threads per ky column (0: off)
   size_t lds_zf;       // its LDS bytes
+  int yct;             // split variant: compile-time y pass of length Mp (0: generic y pass)
   FftPlan zf, zi; // split variant: z plans of the forward (odd radices first) and inverse transforms
   size_t lds_y, lds_z;
   // K1/K3 (x pencils)
@@ -105,6 +106,7 @@ struct VecArgs {
 // Per-kernel event timing (prof.cpp)
 enum KernelId {
   KID_XFWD = 0, KID_PLANE, KID_XINV, KID_PC_XFWD, KID_PC_PLANE, KID_PC_XINV, KID_YFWD, KID_ZPASS, KID_YINV,
+  KID_PC_YROWS, KID_PC_BLOCK,
   kNumKernelIds
 };
 const char* kernel_id_name(int kid);
@@ -126,6 +128,20 @@ bool factorize(int n, bool fp64, FftPlan& out, int tws, bool odd_first = false);
 constexpr int kZfTile = 16;
 bool zfused_pair(int Np, bool fp64, int& ra, int& rb);
 int zfused_threads(int ra, int rb);
+// Compile-time y pass: pencils per workgroup for length n and element size
+// (two ping-pong buffers of n x (NB + 1) plus the twiddle table within
+// 40 KB), and the radix list instantiated for n (plan.cpp; false: none).
+__host__ __device__ constexpr int yct_pencils(int n, int elem) {
+  int nb = 16;
+  while (nb > 2 && (2 * n * (nb + 1) + n) * elem > 40 * 1024) nb /= 2;
+  return nb;
+}
+__host__ __device__ constexpr int yct_lds(int n, int elem) { return (2 * n * (yct_pencils(n, elem) + 1) + n) * elem; }
+// Lengths with a compile-time y pass and their radix lists (both precisions).
+#define VP_YCT_LIST(X)                                                                                  \
+  X(12, 3, 4) X(16, 4, 4) X(20, 4, 5) X(24, 3, 8) X(32, 4, 8) X(40, 5, 8) X(48, 3, 4, 4) X(64, 8, 8)     \
+  X(96, 3, 4, 8) X(100, 4, 5, 5) X(120, 3, 5, 8) X(128, 2, 8, 8) X(200, 5, 5, 8) X(256, 4, 8, 8)
+bool yct_supported(int n);
 int max_lds_bytes(int device);
 
 template <typename T>
