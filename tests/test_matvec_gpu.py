@@ -87,11 +87,13 @@ def test_assembly_matches_oracle(vp_ctx):
 @pytest.mark.parametrize('shape,fft', [((8, 6, 4), (0, 0, 0)), ((9, 4, 5), (21, 9, 9)), ((20, 12, 6), (0, 0, 0)),
                                        ((4, 11, 9), (0, 0, 0)), ((6, 5, 22), (0, 0, 44)), ((5, 7, 24), (0, 0, 48)),
                                        ((4, 5, 22), (0, 0, 88)), ((3, 4, 40), (0, 0, 88)), ((5, 10, 6), (0, 0, 64)),
-                                       ((2, 100, 3), (0, 200, 0)), ((3, 48, 2), (0, 96, 0))])
+                                       ((2, 100, 3), (0, 200, 0)), ((3, 48, 2), (0, 96, 0)),
+                                       ((30, 6, 4), (200, 0, 0)), ((25, 6, 3), (400, 0, 0))])
 def test_matvec_variants(vp_ctx, monkeypatch, variant, shape, fft):
-    # The yz-plane strategies (fused residue-class kernel; split y/z passes
-    # with the compile-time y pass and two-stage fused z kernel, with the
-    # generic LDS y and z passes, or with the register z-column kernel)
+    # The pass strategies (fused residue-class yz kernel; split y/z passes
+    # with the compile-time x and y passes (pencil-major x spectra where the
+    # class length has them) and the two-stage fused z kernel, with the
+    # generic LDS x, y and z passes, or with the register z-column kernel)
     # against the dense Toeplitz matrix; 'transp' is its conjugate transpose.
     from voxphoton_amd.vie import NOperator
     monkeypatch.setenv('VP_MATVEC_VARIANT', variant.split('_')[0])

@@ -439,23 +439,27 @@ __device__ C<T>* lds_fft(C<T>* a, C<T>* b, const FftPlan& pl, int nbatch, const 
 // Compile-time Stockham FFT over a batch of NB transforms of length NLEN in
 // LDS, element k of transform b at k*LD + b.  Every size is a template
 // constant, so index arithmetic folds into shifts and multiplies and the
-// butterfly loops unroll.  Thread it takes butterfly j = it % (NLEN/R) of
-// transform b = it / (NLEN/R): a wavefront's accesses step through
-// consecutive elements of one transform, LD words apart (LD odd: no bank
-// conflicts for 8-byte elements).  The first stage reads through ld(b, k)
-// and the last one writes through st(b, k, v) (global memory), so a
-// transform passes LDS only between stages.  tw: exp(-2 pi i k / NLEN).
+// butterfly loops unroll.  A stage maps thread it to butterfly j of
+// transform b either j-fastest (it = b*(NLEN/R) + j: a wavefront walks
+// consecutive elements of one transform) or b-fastest (it = j*NB + b:
+// consecutive transforms at one element); LD odd keeps both free of LDS
+// bank conflicts for 8-byte elements.  The first stage reads through
+// ld(b, k) and the last one writes through st(b, k, v), so global data can
+// enter and leave in whichever order is contiguous in memory (FB / LB pick
+// b-fastest there).  tw[k * twm] = exp(-2 pi i k / NLEN).
 // ---------------------------------------------------------------------------
-template <typename T, bool INV, int NLEN, int NB, int LD, int NT, int NS, int R, bool FIRST, bool LAST, class Ld,
-          class St>
-__device__ __forceinline__ void ct_stage(const C<T>* src, C<T>* dst, const C<T>* tw, Ld& ld, St& st) {
+template <typename T, bool INV, int NLEN, int NB, int LD, int NT, int NS, int R, bool FIRST, bool LAST, bool BF,
+          class Ld, class St>
+__device__ __forceinline__ void ct_stage(const C<T>* src, C<T>* dst, const C<T>* __restrict__ tw, int twm, Ld& ld,
+                                         St& st) {
   constexpr int NBF = NLEN / R;
   constexpr int TOTAL = NBF * NB;
 #pragma unroll
   for (int it0 = 0; it0 < TOTAL; it0 += NT) {
     const int it = it0 + int(threadIdx.x);
     if (TOTAL % NT == 0 || it < TOTAL) {
-      const int b = it / NBF, j = it - b * NBF;
+      const int b = BF ? it % NB : it / NBF;
+      const int j = BF ? it / NB : it - b * NBF;
       C<T> v[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) v[r] = FIRST ? ld(b, j + NBF * r) : src[(j + NBF * r) * LD + b];
@@ -463,7 +467,7 @@ __device__ __forceinline__ void ct_stage(const C<T>* src, C<T>* dst, const C<T>*
       if constexpr (NS > 1) {
         constexpr int TM = NLEN / (NS * R);
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<T, INV>(tw, r * jm * TM));
+        for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<T, INV>(tw, r * jm * TM * twm));
       }
       dft<T, R, INV>(v);
       const int base = (j - jm) * R + jm;
@@ -476,16 +480,24 @@ __device__ __forceinline__ void ct_stage(const C<T>* src, C<T>* dst, const C<T>*
   }
 }
 
-// Runs the radix list R0, Rs... (product NLEN) from ``ld`` to ``st``,
-// ping-ponging between LDS buffers a and b; synchronises between stages.
-template <typename T, bool INV, int NLEN, int NB, int LD, int NT, int NS, bool FIRST, int R0, int... Rs, class Ld,
-          class St>
-__device__ __forceinline__ void ct_fft(C<T>* a, C<T>* b, const C<T>* tw, Ld& ld, St& st) {
-  constexpr bool LAST = sizeof...(Rs) == 0;
-  ct_stage<T, INV, NLEN, NB, LD, NT, NS, R0, FIRST, LAST>(a, b, tw, ld, st);
-  if constexpr (!LAST) {
+// Runs the radix list R0, Rs... (product NLEN) from ``ld`` to ``st`` (or,
+// LDSOUT, to LDS), ping-ponging between buffers a and b with a barrier
+// between stages.  Returns the LDS buffer holding the result when LDSOUT
+// (after a barrier), else nullptr.
+template <typename T, bool INV, int NLEN, int NB, int LD, int NT, int NS, bool FIRST, bool FB, bool LB, bool LDSOUT,
+          int R0, int... Rs, class Ld, class St>
+__device__ __forceinline__ C<T>* ct_fft(C<T>* a, C<T>* b, const C<T>* __restrict__ tw, int twm, Ld& ld, St& st) {
+  constexpr bool END = sizeof...(Rs) == 0;
+  constexpr bool BF = FIRST ? FB : (END && !LDSOUT ? LB : false);
+  ct_stage<T, INV, NLEN, NB, LD, NT, NS, R0, FIRST, END && !LDSOUT, BF>(a, b, tw, twm, ld, st);
+  if constexpr (!END) {
     __syncthreads();
-    ct_fft<T, INV, NLEN, NB, LD, NT, NS * R0, false, Rs...>(b, a, tw, ld, st);
+    return ct_fft<T, INV, NLEN, NB, LD, NT, NS * R0, false, FB, LB, LDSOUT, Rs...>(b, a, tw, twm, ld, st);
+  } else if constexpr (LDSOUT) {
+    __syncthreads();
+    return b;
+  } else {
+    return nullptr;
   }
 }
 

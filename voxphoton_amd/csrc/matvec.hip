@@ -234,17 +234,120 @@ __global__ void __launch_bounds__(256) k_ypass(const ProbPtrs<T>* __restrict__ p
   }
 }
 
-// Compile-time y pass for Mp = product of Rs: NB pencils (kx, c, n) per
-// workgroup, the box rows m < M in the x spectra X[p][m] and the full rows
-// of the y spectra Y[p][ky].  Forward: X -> Y (rows zero padded past M);
-// inverse: Y -> X (cropped to M).  Buffer resources cover the workgroup's
-// pencils only, so a partial last group reads zeros and drops its stores.
-template <typename T, bool INV, int NB, int... Rs>
+// Compile-time x passes over pencil-major x spectra X[c][n][m][kx], the
+// class-px transform of pencil (c, n, m) at X[pencil][px*S + a] for
+// kx = R*a + px (S = Lp/R = product of Rs, one pencil per workgroup, the
+// transform contiguous in LDS).
+// Forward: one workgroup per (pencil, px) gathers the L box voxels of the
+// pencil through the voxel map, multiplies by w_Lp^{l px} (the first
+// decimation-in-frequency step of the zero-padded length-Lp transform) and
+// writes the S-point DFT.
+template <typename T, int NT, int... Rs>
+__global__ void __launch_bounds__(NT) k_xfwd_ct(const ProbPtrs<T>* __restrict__ probs, VecArgs<T> va, Geometry g,
+                                                int transp, const C<T>* __restrict__ twx) {
+  constexpr int S = (Rs * ...);
+  if (va.active && !va.active[blockIdx.y]) return;
+  const ProbPtrs<T> P = probs[blockIdx.y];
+  const C<T>* __restrict__ X = va.x + int64_t(blockIdx.y) * va.xstride;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
+  C<T>* W1 = W0 + S;
+  const int R = g.x.R, L = g.L, M = g.M, N = g.N;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int pencil = blk / R, px = blk - pencil * R;
+  const int cn = pencil / M, m = pencil - cn * M;
+  const int c = cn / N, n = cn - c * N;
+  auto ld = [&](int, int l) {
+    C<T> v = czero<T>();
+    if (l < L) {
+      int64_t u;
+      v = load_unknown(P, X, g, g.voxel(l, m, n), c, transp, u);
+      if (px) v = cmul(v, twx[l * px]);
+    }
+    return v;
+  };
+  const BufRes Wo = buf_res(P.work + int64_t(pencil) * g.x.P + int64_t(px) * S, uint32_t(S) * uint32_t(sizeof(C<T>)));
+  auto st = [&](int, int k, C<T> v) { buf_store(Wo, uint32_t(k), v); };
+  ct_fft<T, false, S, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, W1, twx, R, ld, st);
+}
+
+// Inverse: one workgroup per pencil runs the R class transforms, folds
+// them back with conj(w_Lp^{l px}) in registers, crops to l < L and applies
+// the epilogue Gram*Mr.*x - Mc.*N(x) (or its 'transp' form) per unknown.
+template <typename T, int NT, int... Rs>
+__global__ void __launch_bounds__(NT) k_xinv_ct(const ProbPtrs<T>* __restrict__ probs, VecArgs<T> va, Geometry g,
+                                                int transp, const C<T>* __restrict__ twx) {
+  constexpr int S = (Rs * ...);
+  constexpr int EPT = (S + NT - 1) / NT;
+  if (va.active && !va.active[blockIdx.y]) return;
+  const ProbPtrs<T> P = probs[blockIdx.y];
+  const C<T>* __restrict__ X = va.x + int64_t(blockIdx.y) * va.xstride;
+  C<T>* __restrict__ Y = va.y + int64_t(blockIdx.y) * va.ystride;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
+  C<T>* W1 = W0 + S;
+  const int R = g.x.R, L = g.L, M = g.M, N = g.N;
+  const int pencil = xcd_remap(blockIdx.x, gridDim.x);
+  const int cn = pencil / M, m = pencil - cn * M;
+  const int c = cn / N, n = cn - c * N;
+  C<T> acc[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) acc[e] = czero<T>();
+  auto none = [](int, int, C<T>) {};
+  for (int px = 0; px < R; ++px) {
+    const BufRes Wi = buf_res(P.work + int64_t(pencil) * g.x.P + int64_t(px) * S, uint32_t(S) * uint32_t(sizeof(C<T>)));
+    auto ld = [&](int, int k) { return buf_load<T>(Wi, uint32_t(k)); };
+    const C<T>* Rb = ct_fft<T, true, S, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, W1, twx, R, ld, none);
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int l = int(threadIdx.x) + e * NT;
+      if (l < L) {
+        C<T> t = Rb[l];
+        if (px) t = cmulc(t, twx[l * px]);
+        acc[e] = cadd(acc[e], t);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int l = int(threadIdx.x) + e * NT;
+    if (l < L) {
+      const int64_t v = g.voxel(l, m, n);
+      int64_t pv, u;
+      if (g.structured) {
+        pv = P.pos[v];
+        u = pv < 0 ? -1 : pv + int64_t(c) * g.nS;
+      } else {
+        pv = P.pos[int64_t(c) * g.V + v];
+        u = pv;
+      }
+      if (u >= 0) {
+        const C<T> xv = X[u];
+        C<T> id = cscale(xv, P.gram);
+        if (P.mr) id = transp ? cmulc(id, P.mr[pv]) : cmul(id, P.mr[pv]);
+        Y[u] = transp ? csub(id, cconj(acc[e])) : csub(id, cmul(P.mc[pv], acc[e]));
+      }
+    }
+  }
+}
+
+// Compile-time y pass for Mp = product of Rs over NB pencils (kx, c, n):
+// the box rows m < M of the x spectra and the full rows of the y spectra
+// Y[kx][c][n][ky].  Forward: X -> Y (rows zero padded past M); inverse:
+// Y -> X (cropped to M).  The x spectra sit either plane-major,
+// X[kx][c][n][m] (XB false: a workgroup takes NB consecutive pencils), or
+// pencil-major along x, X[c][n][m][kx] (XB true: NB consecutive kx of one
+// (c, n), read and written b-fastest so a wavefront touches runs of NB
+// consecutive kx).  Buffer resources cover the workgroup's data only, so a
+// partial last group reads zeros and drops its stores.
+template <typename T, bool INV, bool XB, int NB, int... Rs>
 __global__ void __launch_bounds__(256) k_ypass_ct(const ProbPtrs<T>* __restrict__ probs,
                                                   const int* __restrict__ active, Geometry g,
                                                   const C<T>* __restrict__ twy) {
   constexpr int NLEN = (Rs * ...);
   constexpr int LD = NB + 1;
+  constexpr uint32_t E = sizeof(C<T>);
   if (active && !active[blockIdx.y]) return;
   const ProbPtrs<T> P = probs[blockIdx.y];
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -252,23 +355,45 @@ __global__ void __launch_bounds__(256) k_ypass_ct(const ProbPtrs<T>* __restrict_
   C<T>* W1 = W0 + NLEN * LD;
   C<T>* tw = W1 + NLEN * LD;
   for (int e = threadIdx.x; e < NLEN; e += 256) tw[e] = twy[e];
-  const int M = g.M;
-  const int64_t npen = int64_t(g.x.P) * 3 * g.N;
-  const int64_t p0 = int64_t(xcd_remap(blockIdx.x, gridDim.x)) * NB;
-  const uint32_t nb = uint32_t(min<int64_t>(NB, npen - p0));
-  const BufRes X = buf_res(P.work + p0 * M, nb * uint32_t(M) * uint32_t(sizeof(C<T>)));
-  const BufRes Y = buf_res(P.work2 + p0 * NLEN, nb * uint32_t(NLEN) * uint32_t(sizeof(C<T>)));
+  const int M = g.M, N = g.N, Lp = g.x.P;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  // X element (b, m) at xo(b) + m*xs; Y element (b, ky) at b*ys + ky
+  BufRes X, Y;
+  uint32_t xs, ys;
+  int nb;
+  if constexpr (XB) {
+    const int nkb = (Lp + NB - 1) / NB;
+    const int cn = blk / nkb, kb = blk - cn * nkb;   // cn = c*N + n
+    const int c = cn / N, n = cn - c * N;
+    const int kx0 = kb * NB;
+    nb = min(NB, Lp - kx0);
+    X = buf_res(P.work + int64_t(cn) * M * Lp + kx0, uint32_t((M - 1) * Lp + nb) * E);
+    xs = uint32_t(Lp);
+    ys = uint32_t(3 * N * NLEN);
+    Y = buf_res(P.work2 + (int64_t(kx0) * 3 * N + int64_t(c) * N + n) * NLEN, uint32_t((nb - 1) * ys + NLEN) * E);
+  } else {
+    const int64_t npen = int64_t(Lp) * 3 * N;
+    const int64_t p0 = int64_t(blk) * NB;
+    nb = int(min<int64_t>(NB, npen - p0));
+    X = buf_res(P.work + p0 * M, uint32_t(nb) * uint32_t(M) * E);
+    xs = 1;
+    ys = uint32_t(NLEN);
+    Y = buf_res(P.work2 + p0 * NLEN, uint32_t(nb) * uint32_t(NLEN) * E);
+  }
+  const uint32_t xb = XB ? 1u : uint32_t(M);   // X offset of pencil b: b * xb
   __syncthreads();
   if constexpr (!INV) {
-    auto ld = [&](int b, int k) { return k < M ? buf_load<T>(X, uint32_t(b * M + k)) : czero<T>(); };
-    auto st = [&](int b, int k, C<T> v) { buf_store(Y, uint32_t(b * NLEN + k), v); };
-    ct_fft<T, false, NLEN, NB, LD, 256, 1, true, Rs...>(W0, W1, tw, ld, st);
-  } else {
-    auto ld = [&](int b, int k) { return buf_load<T>(Y, uint32_t(b * NLEN + k)); };
-    auto st = [&](int b, int k, C<T> v) {
-      if (k < M) buf_store(X, uint32_t(b * M + k), v);
+    auto ld = [&](int b, int k) {
+      return (k < M && b < nb) ? buf_load<T>(X, uint32_t(b) * xb + uint32_t(k) * xs) : czero<T>();
     };
-    ct_fft<T, true, NLEN, NB, LD, 256, 1, true, Rs...>(W0, W1, tw, ld, st);
+    auto st = [&](int b, int k, C<T> v) { buf_store(Y, uint32_t(b) * ys + uint32_t(k), v); };
+    ct_fft<T, false, NLEN, NB, LD, 256, 1, true, XB, false, false, Rs...>(W0, W1, tw, 1, ld, st);
+  } else {
+    auto ld = [&](int b, int k) { return buf_load<T>(Y, uint32_t(b) * ys + uint32_t(k)); };
+    auto st = [&](int b, int k, C<T> v) {
+      if (k < M && b < nb) buf_store(X, uint32_t(b) * xb + uint32_t(k) * xs, v);
+    };
+    ct_fft<T, true, NLEN, NB, LD, 256, 1, true, false, XB, false, Rs...>(W0, W1, tw, 1, ld, st);
   }
 }
 
@@ -758,14 +883,23 @@ static void launch_ypass_ct(const Geometry& g, const MatvecPlan& mp, const ProbP
                             int nprob, const C<T>* twy, hipStream_t s) {
   constexpr int E = int(sizeof(C<T>));
   const int64_t npen = int64_t(g.x.P) * 3 * g.N;
+  const bool xb = mp.xct != 0;
   switch (mp.yct) {
 #define VP_YCT_LAUNCH(NL, ...)                                                                           \
   case NL: {                                                                                            \
     constexpr int NB = yct_pencils(NL, E);                                                              \
-    const void* fn = reinterpret_cast<const void*>(&k_ypass_ct<T, INV, NB, __VA_ARGS__>);               \
-    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, yct_lds(NL, E));          \
-    hipLaunchKernelGGL((k_ypass_ct<T, INV, NB, __VA_ARGS__>), dim3(unsigned((npen + NB - 1) / NB), nprob), \
-                       dim3(256), yct_lds(NL, E), s, d_probs, active, g, twy);                          \
+    const unsigned nblk = xb ? unsigned(3 * g.N * ((g.x.P + NB - 1) / NB)) : unsigned((npen + NB - 1) / NB); \
+    if (xb) {                                                                                           \
+      const void* fn = reinterpret_cast<const void*>(&k_ypass_ct<T, INV, true, NB, __VA_ARGS__>);       \
+      (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, yct_lds(NL, E));        \
+      hipLaunchKernelGGL((k_ypass_ct<T, INV, true, NB, __VA_ARGS__>), dim3(nblk, nprob), dim3(256),     \
+                         yct_lds(NL, E), s, d_probs, active, g, twy);                                   \
+    } else {                                                                                            \
+      const void* fn = reinterpret_cast<const void*>(&k_ypass_ct<T, INV, false, NB, __VA_ARGS__>);      \
+      (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, yct_lds(NL, E));        \
+      hipLaunchKernelGGL((k_ypass_ct<T, INV, false, NB, __VA_ARGS__>), dim3(nblk, nprob), dim3(256),    \
+                         yct_lds(NL, E), s, d_probs, active, g, twy);                                   \
+    }                                                                                                   \
     break;                                                                                              \
   }
     VP_YCT_LIST(VP_YCT_LAUNCH)
@@ -774,12 +908,44 @@ static void launch_ypass_ct(const Geometry& g, const MatvecPlan& mp, const ProbP
   }
 }
 
+// Compile-time x passes (class length mp.xct): forward one workgroup per
+// (pencil, class), inverse one per pencil.
+template <typename T, bool INV>
+static void launch_xpass_ct(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d_probs, VecArgs<T> va,
+                            int nprob, int transp, const C<T>* twx, hipStream_t s) {
+  const unsigned npen = unsigned(3 * g.N * g.M);
+  const dim3 grid(INV ? npen : npen * unsigned(g.x.R), nprob);
+  bool done = false;
+#define VP_XCT_LAUNCH(SL, NT, ...)                                                                      \
+  if (!done && mp.xct == SL) {                                                                          \
+    const size_t lds = 2 * size_t(SL) * sizeof(C<T>);                                                   \
+    const void* fn = INV ? reinterpret_cast<const void*>(&k_xinv_ct<T, NT, __VA_ARGS__>)                 \
+                         : reinterpret_cast<const void*>(&k_xfwd_ct<T, NT, __VA_ARGS__>);                \
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));                \
+    if (INV) hipLaunchKernelGGL((k_xinv_ct<T, NT, __VA_ARGS__>), grid, dim3(NT), lds, s, d_probs, va, g, transp, twx); \
+    else hipLaunchKernelGGL((k_xfwd_ct<T, NT, __VA_ARGS__>), grid, dim3(NT), lds, s, d_probs, va, g, transp, twx);     \
+    done = true;                                                                                        \
+  }
+  if constexpr (sizeof(T) == 8) {
+    VP_XCT64_LIST(VP_XCT_LAUNCH)
+  } else {
+    VP_XCT32_LIST(VP_XCT_LAUNCH)
+  }
+#undef VP_XCT_LAUNCH
+  if (!done) throw VpError{ERR_UNSUPPORTED, "matvec: no compile-time x pass for this length"};
+}
+
 template <typename T>
 void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d_probs, VecArgs<T> va, int nprob,
                    int transp, const C<T>* twx, const C<T>* twy, const C<T>* twz, hipStream_t s) {
   const int nmb = (g.M + mp.B - 1) / mp.B;
   dim3 gx(nmb * g.N * 3, nprob);
-  {
+  const bool xct = mp.variant == 1 && mp.xct != 0;
+  if (xct) {
+    ProfScope ps(s, KID_XFWD);
+    launch_xpass_ct<T, false>(g, mp, d_probs, va, nprob, transp, twx, s);
+    VP_HIP(hipGetLastError());
+  } else {
     const void* fn = reinterpret_cast<const void*>(&k_xfwd<T>);
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(mp.lds1));
     ProfScope ps(s, KID_XFWD);
@@ -858,7 +1024,11 @@ void launch_matvec(const Geometry& g, const MatvecPlan& mp, const ProbPtrs<T>* d
 #undef VP_K2
     VP_HIP(hipGetLastError());
   }
-  {
+  if (xct) {
+    ProfScope ps(s, KID_XINV);
+    launch_xpass_ct<T, true>(g, mp, d_probs, va, nprob, transp, twx, s);
+    VP_HIP(hipGetLastError());
+  } else {
 #define VP_K3(E)                                                                            \
   do {                                                                                      \
     const void* fn = reinterpret_cast<const void*>(&k_xinv<T, E>);                          \

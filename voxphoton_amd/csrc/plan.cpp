@@ -110,6 +110,22 @@ bool yct_supported(int n) {
 #undef VP_YCT_CASE
 }
 
+int xct_threads(int S, bool fp64) {
+#define VP_XCT_CASE(SL, NT, ...) \
+  case SL: return NT;
+  if (fp64) {
+    switch (S) {
+      VP_XCT64_LIST(VP_XCT_CASE)
+      default: return 0;
+    }
+  }
+  switch (S) {
+    VP_XCT32_LIST(VP_XCT_CASE)
+    default: return 0;
+  }
+#undef VP_XCT_CASE
+}
+
 // Mirrors acc_bound<T, EPT>() in matvec.hip.
 static int acc_bound(bool fp64, int ept) { return (fp64 ? ept <= 1 : ept <= 4) ? 1024 : 512; }
 
@@ -173,6 +189,13 @@ bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out,
     // compile-time y pass where Mp has one (VP_YCT=0 keeps the generic pass)
     const char* ey = std::getenv("VP_YCT");
     out.yct = (!(ey && ey[0] == '0') && yct_supported(Mp) && yct_lds(Mp, int(s)) <= lds_max) ? Mp : 0;
+    // compile-time x passes over pencil-major x spectra where the class
+    // length has them and the y pass is compile-time too (VP_XCT=0 keeps
+    // the generic x passes and plane-major x spectra)
+    const char* ex = std::getenv("VP_XCT");
+    out.xct = 0;
+    if (out.yct && !(ex && ex[0] == '0') && xct_threads(g.x.S, fp64) > 0 && 2 * size_t(g.x.S) * s <= size_t(lds_max))
+      out.xct = g.x.S;
     // fused two-stage z kernel (default; VP_ZFUSED=0 keeps the LDS z pass)
     out.zra = out.zrb = out.ztpk = 0;
     out.lds_zf = 0;

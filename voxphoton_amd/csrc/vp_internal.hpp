@@ -50,6 +50,8 @@ struct MatvecPlan {
   int zra, zrb, ztpk;  // split variant: fused two-stage z kernel Np = zra * zrb, ztpk threads per ky column (0: off)
   size_t lds_zf;       // its LDS bytes
   int yct;             // split variant: compile-time y pass of length Mp (0: generic y pass)
+  int xct;             // split variant: compile-time x passes of class length S over pencil-major
+                       // x spectra X[c][n][m][kx] (0: generic x passes, plane-major X[kx][c][n][m])
   FftPlan zf, zi; // split variant: z plans of the forward (odd radices first) and inverse transforms
   size_t lds_y, lds_z;
   // K1/K3 (x pencils)
@@ -142,6 +144,15 @@ __host__ __device__ constexpr int yct_lds(int n, int elem) { return (2 * n * (yc
   X(12, 3, 4) X(16, 4, 4) X(20, 4, 5) X(24, 3, 8) X(32, 4, 8) X(40, 5, 8) X(48, 3, 4, 4) X(64, 8, 8)     \
   X(96, 3, 4, 8) X(100, 4, 5, 5) X(120, 3, 5, 8) X(128, 2, 8, 8) X(200, 5, 5, 8) X(256, 4, 8, 8)
 bool yct_supported(int n);
+// Class lengths S = Lp/R with compile-time x passes: (S, threads, radices).
+#define VP_XCT32_LIST(X)                                                                                \
+  X(3600, 512, 16, 9, 5, 5) X(1800, 512, 8, 9, 5, 5) X(1200, 512, 16, 3, 5, 5) X(900, 256, 4, 9, 5, 5)    \
+  X(450, 256, 2, 9, 5, 5) X(400, 256, 16, 5, 5) X(360, 256, 8, 9, 5) X(200, 256, 8, 5, 5) X(100, 256, 4, 5, 5)
+#define VP_XCT64_LIST(X)                                                                                \
+  X(3600, 512, 8, 2, 9, 5, 5) X(1800, 512, 8, 9, 5, 5) X(1200, 512, 8, 2, 3, 5, 5) X(900, 256, 4, 9, 5, 5) \
+  X(450, 256, 2, 9, 5, 5) X(400, 256, 4, 4, 5, 5) X(360, 256, 8, 9, 5) X(200, 256, 8, 5, 5) X(100, 256, 4, 5, 5)
+// Threads of the compile-time x pass for class length S (0: none).
+int xct_threads(int S, bool fp64);
 int max_lds_bytes(int device);
 
 template <typename T>
