@@ -153,6 +153,8 @@ def main():
     ap.add_argument('--no-prec', action='store_true')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-fp64-solve', action='store_true')
+    ap.add_argument('--inner-tol', type=float, default=None, help='override the workload inner tolerance')
+    ap.add_argument('--inner-maxit', type=int, default=None, help='override the workload inner restart cycles')
     ap.add_argument('--traffic-probe', action='store_true',
                     help='for rocprofv3 --pmc runs: a 1 GiB device copy (calibration), then 3 FP32 matvecs '
                          'and 3 FP32 preconditioner applies')
@@ -165,7 +167,11 @@ def main():
     from voxphoton_amd import vie, configs
     from voxphoton_amd import _lib as L
 
-    wl = WORKLOADS[args.workload]
+    wl = dict(WORKLOADS[args.workload])
+    if args.inner_tol is not None:
+        wl['inner_tol'] = args.inner_tol
+    if args.inner_maxit is not None:
+        wl['inner_maxit'] = args.inner_maxit
     log = lambda *a: print('[bench]', *a, file=sys.stderr, flush=True)   # noqa: E731
     if 'nlam' in wl:
         run_sweep(args, wl, log)

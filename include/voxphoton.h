@@ -208,10 +208,10 @@ typedef struct {
 
 /*
  * Restarted GMRES(restart) on A x = b with optional preconditioner applied
- * on the given side.  Arnoldi uses classical Gram-Schmidt with a second
- * pass where the first one cancelled more than half of ||w||^2 (DGKS
- * criterion; VP_GMRES_CGS2=1 forces it every step); the Hessenberg least
- * squares is updated with
+ * on the given side.  Arnoldi uses one classical Gram-Schmidt pass per
+ * step, as pgmres_iter.m:106-107 (VP_DGKS_ETA=eta adds a second pass where
+ * ||w'|| < eta ||w||; VP_GMRES_CGS2=1 adds it every step); the Hessenberg
+ * least squares is updated with
  * Givens rotations in FP64.  prec selects the arithmetic (VP_FP64 or VP_FP32
  * for the whole Krylov loop).  b, x0 (nullable), x follow mem.
  */
@@ -221,9 +221,11 @@ int vp_gmres(vp_op op, vp_prec M, int side, const void* b, const void* x0, void*
 /*
  * Mixed precision: FP64 iterative refinement around FP32 GMRES.  Outer
  * residual r = b - A x in FP64 on the device; inner FP32 GMRES(restart)
- * with the FP32 preconditioner solves A d = r to inner_tol, loosened to
- * 0.5 * tol / relres (at most 0.5) when that suffices to reach tol; stops
- * when ||b - A x|| / ||b|| <= tol or after max_outer steps.  No counterpart
+ * with the FP32 preconditioner solves A d = r to inner_tol, or to
+ * need = min(0.5, 0.5 * tol / relres) when need >= inner_tol / 8 (a step
+ * that can reach tol goes as deep as it needs instead of leaving a short
+ * extra step); stops when ||b - A x|| / ||b|| <= tol or after max_outer
+ * steps.  No counterpart
  * in the reference (which solves in FP64 only).
  */
 int vp_solve_ir(vp_op op, vp_prec M, int side, const void* b, void* x, int mem, double tol, int restart,

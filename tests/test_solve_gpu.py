@@ -168,14 +168,20 @@ def _physical(L, M, N, seed=0, frac=1.0):
     return G, Mc, Gram, idx, b, dx, ko
 
 
-def test_gmres_fp64_matches_oracle(vp_ctx):
+@pytest.mark.parametrize('orth', ['cgs', 'dgks', 'cgs2'])
+def test_gmres_fp64_matches_oracle(vp_ctx, monkeypatch, orth):
+    # one Gram-Schmidt pass per step (default, as pgmres_iter.m), the
+    # DGKS-selected second pass and the second pass at every step: the
+    # residual histories of the oracle's single and double pass GMRES
     from voxphoton_amd.vie import NOperator, pgmres
+    monkeypatch.setenv('VP_DGKS_ETA', '0.7071' if orth == 'dgks' else '0')
+    monkeypatch.setenv('VP_GMRES_CGS2', '1' if orth == 'cgs2' else '0')
     G, Mc, Gram, idx, b, dx, ko = _physical(6, 5, 4, seed=1, frac=0.7)
     op = NOperator.from_toeplitz(G, None, Mc, Gram, idx, ctx=vp_ctx)
     fN = ovie.fN_from_toeplitz(G)
     Mr = np.ones(Mc.shape, complex)
     A = lambda v: ovie.mv_AN_const(v, fN, Mr, Mc, Gram, 'notransp', idx)   # noqa: E731
-    xo, flo, rro, ito, rvo = okry.pgmres(A, b, 10, 1e-10, 20, orth='cgs2')
+    xo, flo, rro, ito, rvo = okry.pgmres(A, b, 10, 1e-10, 20, orth='cgs' if orth == 'cgs' else 'cgs2')
     x, flag, relres, it, rv = pgmres(op, b, 10, 1e-10, 20)
     print('gmres iters', len(rv), len(rvo), 'relres', relres, rro, flush=True)
     assert flag == 0 and flo == 0

@@ -301,9 +301,11 @@ struct IrResult {
 // the problem is done when relres <= tol (flag 0), when relres is not
 // finite or after max_outer steps (flag 1); otherwise one inner FP32 GMRES
 // solve of (A/s) d = r/||r|| and x += d*||r||/s.  The inner solve reduces
-// its residual by inner_tol, or by no more than this step needs to reach tol
-// (factor 2 margin for the FP32 residual gap).  Problems already done do no
-// work.  Returns true while a problem is still running.
+// its residual by what this step needs to reach tol (need = tol/relres/2,
+// factor 2 margin for the FP32 residual gap) when that is looser than
+// inner_tol or within a factor 8 below it (one slightly deeper inner solve
+// instead of another refinement step), else by inner_tol.  Problems already
+// done do no work.  Returns true while a problem is still running.
 bool ir_step_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2* b, double2* x, double scale32,
                  double tol, int restart, double inner_tol, int inner_maxit, int max_outer, int side,
                  std::vector<IrProblem>& st) {
@@ -355,8 +357,10 @@ bool ir_step_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2*
   prm.active_in = hact;
   prm.tol_per.assign(np, inner_tol);
   for (int p = 0; p < np; ++p)
-    if (hact[p] && st[p].relres > 0)
-      prm.tol_per[p] = std::max(inner_tol, std::min(0.5, 0.5 * tol / st[p].relres));
+    if (hact[p] && st[p].relres > 0) {
+      const double need = std::min(0.5, 0.5 * tol / st[p].relres);
+      prm.tol_per[p] = need >= inner_tol / 8 ? need : inner_tol;
+    }
   GmresResult gr;
   gmres_device<float>(p32, r32.as<float2>(), d32.as<float2>(), prm, gr);
   for (int p = 0; p < np; ++p)

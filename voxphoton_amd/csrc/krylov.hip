@@ -3,8 +3,9 @@
 // Arnoldi (pgmres_iter.m:60-127) runs as fused device kernels:
 //   k_dots       h1 = V^H w, ||w||^2               (FP64 partial sums per block)
 //   k_axpy_dots  w -= V h1 ; ||w||^2               (mode 1)
-//   k_dgks       second pass wanted where ||w'||^2 < ||w||^2 / 2
-//   k_dots, k_axpy_dots  h2 = V^H w, w -= V h2     (only those problems)
+//   k_dgks       optional second pass wanted where ||w'||^2 < eta^2 ||w||^2
+//   k_dots, k_axpy_dots  h2 = V^H w, w -= V h2     (only those problems; off by
+//                default: one pass per step as pgmres_iter.m:106-107)
 //   k_arnoldi    H(:,k) = h1 (+ h2), ||w||, Givens rotations, residual estimate
 //   k_scale      v_{k+1} = w / ||w||
 // All reductions use fixed-order trees (no atomics), so a problem's result
@@ -52,26 +53,36 @@ __device__ __forceinline__ void block_reduce_store(double2 (&acc)[J], double2* o
   __syncthreads();
 }
 
+// Loads of the Krylov kernels are never under a branch: a row past n reads
+// row n-1 (its w entry is zero, or its result is not stored) and a column
+// past k reads column k-1 (its coefficient is zero, or its sum is not
+// stored), so each chunk's JC x RPT loads are in flight together.
+__device__ __forceinline__ int64_t krow(int64_t base, int t, int64_t n) {
+  return min<int64_t>(base + t * KT + threadIdx.x, n - 1);
+}
+
 // partial dots of w (registers) with columns 0..k-1 of V
 template <typename T>
 __device__ __forceinline__ void dots_core(const C<T>* __restrict__ Vp, int64_t vcol, int k, const double2 (&wv)[RPT],
                                           int64_t base, int64_t n, double2* part, int nblk) {
   for (int j0 = 0; j0 < k; j0 += JC) {
+    C<T> v[JC][RPT];
+#pragma unroll
+    for (int jj = 0; jj < JC; ++jj) {
+      const int64_t col = min(j0 + jj, k - 1);
+#pragma unroll
+      for (int t = 0; t < RPT; ++t) v[jj][t] = Vp[col * vcol + krow(base, t, n)];
+    }
     double2 acc[JC];
 #pragma unroll
     for (int jj = 0; jj < JC; ++jj) acc[jj] = make_double2(0.0, 0.0);
 #pragma unroll
     for (int t = 0; t < RPT; ++t) {
-      const int64_t i = base + t * KT + threadIdx.x;
-      if (i < n) {
 #pragma unroll
-        for (int jj = 0; jj < JC; ++jj) {
-          if (j0 + jj < k) {
-            const double2 v = to_d(Vp[int64_t(j0 + jj) * vcol + i]);
-            acc[jj].x += v.x * wv[t].x + v.y * wv[t].y;     // conj(v) * w
-            acc[jj].y += v.x * wv[t].y - v.y * wv[t].x;
-          }
-        }
+      for (int jj = 0; jj < JC; ++jj) {
+        const double2 d = to_d(v[jj][t]);
+        acc[jj].x += d.x * wv[t].x + d.y * wv[t].y;     // conj(v) * w
+        acc[jj].y += d.x * wv[t].y - d.y * wv[t].x;
       }
     }
     block_reduce_store<JC>(acc, part + int64_t(j0) * nblk, nblk, min(JC, k - j0));
@@ -93,8 +104,8 @@ __global__ void __launch_bounds__(KT) k_dots(const C<T>* __restrict__ V, int64_t
   double2 wv[RPT];
 #pragma unroll
   for (int t = 0; t < RPT; ++t) {
-    const int64_t i = base + t * KT + threadIdx.x;
-    wv[t] = i < n ? to_d(wp[i]) : make_double2(0.0, 0.0);
+    const C<T> x = wp[krow(base, t, n)];
+    wv[t] = base + t * KT + threadIdx.x < n ? to_d(x) : make_double2(0.0, 0.0);
   }
   double2* part = partials + (int64_t(p) * maxcols) * nblk + blockIdx.x;
   dots_core<T>(Vp, vcol, k, wv, base, n, part, nblk);
@@ -107,16 +118,18 @@ __global__ void __launch_bounds__(KT) k_dots(const C<T>* __restrict__ V, int64_t
 }
 
 // Selective reorthogonalisation (Daniel-Gragg-Kaufman-Stewart): a second
-// Gram-Schmidt pass for problem p only when the first pass cancelled more
-// than half of ||w||^2, i.e. ||w'||^2 < ||w||^2 / 2 (always = 1: every step).
+// Gram-Schmidt pass for problem p only when the first pass cancelled enough
+// of w, ||w'||^2 < eta2 ||w||^2 (eta2 = 1/2 in the classical criterion;
+// always = 1: every step).
 __global__ void k_dgks(const double2* __restrict__ h1, const double2* __restrict__ nr, int maxcols, int k,
-                       const int* __restrict__ active, int* __restrict__ reorth, int nprob, int always) {
+                       const int* __restrict__ active, int* __restrict__ reorth, int nprob, int always,
+                       double eta2) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nprob) return;
   const bool on = !active || active[p];
   const double w2 = h1[int64_t(p) * maxcols + k + 1].x;
   const double r2 = nr[int64_t(p) * maxcols].x;
-  reorth[p] = (on && (always || r2 < 0.5 * w2)) ? 1 : 0;
+  reorth[p] = (on && (always || r2 < eta2 * w2)) ? 1 : 0;
 }
 
 // w -= V h (h: FP64 per problem), then dots of the new w with V (mode 0) or
@@ -135,21 +148,39 @@ __global__ void __launch_bounds__(KT) k_axpy_dots(const C<T>* __restrict__ V, in
   const int64_t base = int64_t(blockIdx.x) * RB;
   double2 wv[RPT];
 #pragma unroll
+  for (int t = 0; t < RPT; ++t) wv[t] = to_d(wp[krow(base, t, n)]);
+  // columns in chunks of JC: the chunk's JC x RPT loads are issued before
+  // any is used; each row still subtracts its terms in column order
+  for (int j0 = 0; j0 < k; j0 += JC) {
+    C<T> v[JC][RPT];
+#pragma unroll
+    for (int jj = 0; jj < JC; ++jj) {
+      const int64_t col = min(j0 + jj, k - 1);
+#pragma unroll
+      for (int t = 0; t < RPT; ++t) v[jj][t] = Vp[col * vcol + krow(base, t, n)];
+    }
+#pragma unroll
+    for (int jj = 0; jj < JC; ++jj) {
+      if (j0 + jj < k) {
+        const double2 hj = hp[j0 + jj];
+#pragma unroll
+        for (int t = 0; t < RPT; ++t) {
+          const double2 vv = to_d(v[jj][t]);
+          wv[t].x -= vv.x * hj.x - vv.y * hj.y;
+          wv[t].y -= vv.x * hj.y + vv.y * hj.x;
+        }
+      }
+    }
+  }
+#pragma unroll
   for (int t = 0; t < RPT; ++t) {
     const int64_t i = base + t * KT + threadIdx.x;
     if (i < n) {
-      double2 s = to_d(wp[i]);
-      for (int j = 0; j < k; ++j) {
-        const double2 v = to_d(Vp[int64_t(j) * vcol + i]);
-        const double2 hj = hp[j];
-        s.x -= v.x * hj.x - v.y * hj.y;
-        s.y -= v.x * hj.y + v.y * hj.x;
-      }
-      const C<T> st = from_d<T>(s);
+      const C<T> st = from_d<T>(wv[t]);
       wp[i] = st;
       wv[t] = to_d(st);
     } else {
-      wv[t] = make_double2(0.0, 0.0);
+      wv[t] = make_double2(0.0, 0.0);   // rows past n stay out of the sums below
     }
   }
   double2* part = partials + (int64_t(p) * maxcols) * nblk + blockIdx.x;
@@ -504,6 +535,12 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   DevBuf ract(np * sizeof(int));                 // per-step second-pass flags
   const char* ecg = std::getenv("VP_GMRES_CGS2");  // 1: second pass at every step
   const bool cgs2_always = ecg && ecg[0] == '1';
+  // One classical Gram-Schmidt pass per step, as pgmres_iter.m:106-107.
+  // VP_DGKS_ETA = eta > 0 adds the selective second pass where
+  // ||w'|| < eta ||w|| (eta = 1/sqrt(2) is the classical DGKS criterion).
+  double eta2 = 0.0;
+  if (const char* ee = std::getenv("VP_DGKS_ETA")) eta2 = std::atof(ee) * std::atof(ee);
+  const bool second_pass = cgs2_always || eta2 > 0.0;
   auto upload_act = [&]() {
     VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
   };
@@ -596,17 +633,19 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
                          vprob, n, part.as<double2>(), maxcols, 1, dia);
       hipLaunchKernelGGL(k_reduce_cols, dim3(1, np), dim3(KT), 0, s, part.as<double2>(), nblk, 1, maxcols,
                          nr.as<double2>(), dia);
-      // second pass only where the first cancelled (blocks of other problems exit at once)
+      // optional second pass only where the first cancelled (blocks of other problems exit at once)
       hipLaunchKernelGGL(k_dgks, dim3((np + 63) / 64), dim3(64), 0, s, h1.as<double2>(), nr.as<double2>(), maxcols,
-                         k, dia, ract.as<int>(), np, cgs2_always ? 1 : 0);
-      hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, w, vprob, n,
-                         part.as<double2>(), maxcols, ract.as<int>(), 0);
-      hipLaunchKernelGGL(k_reduce_cols, dim3(k + 1, np), dim3(KT), 0, s, part.as<double2>(), nblk, k + 1, maxcols,
-                         h2.as<double2>(), ract.as<int>());
-      hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, h2.as<double2>(), w,
-                         vprob, n, part.as<double2>(), maxcols, 1, ract.as<int>());
-      hipLaunchKernelGGL(k_reduce_cols, dim3(1, np), dim3(KT), 0, s, part.as<double2>(), nblk, 1, maxcols,
-                         nr.as<double2>(), ract.as<int>());
+                         k, dia, ract.as<int>(), np, cgs2_always ? 1 : 0, eta2);
+      if (second_pass) {
+        hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, w, vprob, n,
+                           part.as<double2>(), maxcols, ract.as<int>(), 0);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(k + 1, np), dim3(KT), 0, s, part.as<double2>(), nblk, k + 1,
+                           maxcols, h2.as<double2>(), ract.as<int>());
+        hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, h2.as<double2>(),
+                           w, vprob, n, part.as<double2>(), maxcols, 1, ract.as<int>());
+        hipLaunchKernelGGL(k_reduce_cols, dim3(1, np), dim3(KT), 0, s, part.as<double2>(), nblk, 1, maxcols,
+                           nr.as<double2>(), ract.as<int>());
+      }
       hipLaunchKernelGGL(k_arnoldi, dim3((np + 63) / 64), dim3(64), 0, s, st, k, dia, np, ract.as<int>());
       hipLaunchKernelGGL(k_scale<T>, dim3(grid1(n), np), dim3(256), 0, s, w, vprob, n, invn.as<double>(), 0, dia);
       VP_HIP(hipGetLastError());
