@@ -153,6 +153,7 @@ def main():
     ap.add_argument('--no-prec', action='store_true')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-fp64-solve', action='store_true')
+    ap.add_argument('--no-fields', action='store_true', help='skip the excitation / total-field timing')
     ap.add_argument('--inner-tol', type=float, default=None, help='override the workload inner tolerance')
     ap.add_argument('--inner-maxit', type=int, default=None, help='override the workload inner restart cycles')
     ap.add_argument('--traffic-probe', action='store_true',
@@ -325,6 +326,11 @@ def main():
                     frac=round(achieved / (HBM_PEAK / 1e9), 4), traffic=traffic, avg_us=round(avg_s * 1e6, 2),
                     launches=launches, model_bytes_per_launch=kb, precision='fp32')
 
+    # ---- excitation and post-processing on the same grid
+    fields = None
+    if not args.no_fields:
+        fields = field_block(prob, vie, L, ctx, dx, log)
+
     # ---- CPU baseline: the FP64 oracle matvec on the host cores
     cpu = None
     if not args.no_cpu_baseline:
@@ -365,6 +371,7 @@ def main():
         'kernel_ms': kernel_table(prob, stats, pc_box),
         'setup_s': {'operator': round(t_op, 3), 'preconditioner': round(t_pc, 3)},
         'roofline': roof,
+        'fields': fields,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line), flush=True)
@@ -613,6 +620,72 @@ def traffic_probe(op, P, ctx, n, vie, L, log):
         L.check(L.lib.vp_prec_apply(P.handle, x32.ptr, y32.ptr, L.VP_MEM_DEVICE, L.VP_FP32))
     ctx.synchronize()
     log('traffic probe done: 3 x 1 GiB copies, 3 FP32 matvecs, 3 FP32 preconditioner applies')
+
+
+def field_block(prob, vie, L, ctx, xsol, log):
+    """Steps either side of the solve on the workload grid (FP64, device
+    vectors, HIP-synchronised wall time):
+
+    * the complex-point-source beam of example_Bragg_1500nm_to_1600nm.m:62-65
+      (440 nm, source lambda/4 past the absorber, imaginary offset -lambda,
+      cutoff lambda/2, y-polarised) at every voxel centre;
+    * the total field Einc + (N(J) - Gram J)/(Gram j w eps0) of the solution
+      (E_field_Nop_const) through a field operator over the whole grid, with
+      the workload's Gaussian incident field;
+    * the transmitted power through the cross-section one wavelength inside
+      the far absorber (example_Bragg_1500nm_to_1600nm.m:108-113).
+    """
+    Lg, Mg, Ng = prob.shape
+    V = Lg * Mg * Ng
+    dx = prob.dx
+    lam_t = 440e-9
+    len_abs = prob.extra.get('len_abs', 2e-6)
+    src = np.array([lam_t / 4 + len_abs - 1j * lam_t, Mg * dx / 2, Ng * dx / 2], dtype=complex)
+    Jd = np.array([0, 1, 0], dtype=complex)
+    origin = np.array([0.5 * dx, 0.5 * dx, 0.5 * dx])
+    Ed = vie.DeviceArray(ctx, 3 * V * 16)
+
+    def dgf():
+        L.check(L.lib.vp_field_dgf_grid(ctx.handle, L.ptr(Jd), L.ptr(src), 1, L.ptr(origin), dx, Lg, Mg, Ng,
+                                        float(prob.ko), 0.5 * lam_t, Ed.ptr, L.VP_MEM_DEVICE))
+    dgf()
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(5):
+        dgf()
+    ctx.synchronize()
+    t_dgf = (time.perf_counter() - t) / 5
+    t = time.perf_counter()
+    fop = vie.NOperator.for_fields(prob.shape, dx=dx, ko=prob.ko, ctx=ctx)
+    t_setup = time.perf_counter() - t
+    xs = xsol.download(np.empty(prob.n, dtype=complex))
+    J = np.zeros(3 * V, dtype=complex)
+    J[prob.idx - 1] = xs
+    Jdev = vie.DeviceArray(ctx, J.nbytes)
+    Jdev.upload(J)
+    Einc = vie.DeviceArray(ctx, J.nbytes)
+    Einc.upload(np.ascontiguousarray(prob.gaussian_field('y').reshape(-1, order='F')))
+    Et = vie.DeviceArray(ctx, J.nbytes)
+
+    def total():
+        L.check(L.lib.vp_field_total(fop.handle, Jdev.ptr, Einc.ptr, float(prob.freq), Et.ptr, L.VP_MEM_DEVICE))
+    total()
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(3):
+        total()
+    ctx.synchronize()
+    t_tot = (time.perf_counter() - t) / 3
+    E = Et.download(np.empty(3 * V, dtype=complex)).reshape(Lg, Mg, Ng, 3, order='F')
+    x_thru = Lg - int(round((len_abs + 450e-9) / dx))
+    p_thru = vie.power_through(E, x_thru, 0)
+    p_in = vie.power_through(E, int(round((len_abs + 450e-9) / dx)), 0)
+    fop.close()
+    log(f'fields: source beam {t_dgf * 1e3:.2f} ms over {V} voxels, field operator {t_setup:.2f} s, '
+        f'total field {t_tot * 1e3:.2f} ms, power through x={x_thru}: {p_thru:.4e} (x={Lg - x_thru}: {p_in:.4e})')
+    return dict(points=V, source_beam_ms=round(t_dgf * 1e3, 3), field_operator_setup_s=round(t_setup, 3),
+                total_field_ms=round(t_tot * 1e3, 3), power_thru=p_thru, power_near_source_side=p_in,
+                x_thru=x_thru)
 
 
 def make_preconditioner(prob, wl, vie, configs):

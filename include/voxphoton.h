@@ -282,6 +282,36 @@ int vp_server_stats(vp_server s, int64_t* pending, int64_t* active, int64_t* byt
 /* Finishes every queued request, then stops the worker. */
 int vp_server_destroy(vp_server s);
 
+/*
+ * Excitation and post-processing either side of the solve (FP64).
+ *
+ * vp_field_dgf replaces E_field_DGF_mollified.m:1-111: the field of Ni point
+ * currents J (3*Ni complex128, column-major Ni x 3, host) at source points
+ * Icoord (Ni x 3 complex128, column-major, host; complex coordinates give
+ * the complex-point-source beam of example_Bragg_1500nm_to_1600nm.m:64-67)
+ * observed at No points Ocoord (No x 3 double, column-major, follows mem):
+ * E (No x 3 complex128, column-major, follows mem) = dyadic Green's function
+ * times J, weighted by the mollifier exp(-cutoff / (cutoff - R^2)) where
+ * Re(R) < cutoff and 0 elsewhere (R the principal root of R.R).  The
+ * reference weighs the summed field and is defined for Ni = 1; here each
+ * source's term is weighed, which is the same for Ni = 1.
+ * vp_field_dgf_grid observes at voxel centres origin + (l, m, n) dx of an
+ * L x M x N grid (E: L*M*N x 3 in column-major voxel order), the Ocoord of
+ * the configs, without materialising the coordinates.
+ */
+int vp_field_dgf(vp_ctx ctx, const void* J, const void* Icoord, int64_t Ni, const double* Ocoord, int64_t No,
+                 double ko, double cutoff, void* E, int mem);
+int vp_field_dgf_grid(vp_ctx ctx, const void* J, const void* Icoord, int64_t Ni, const double* origin, double dx,
+                      int64_t L, int64_t M, int64_t N, double ko, double cutoff, void* E, int mem);
+/*
+ * Total field (E_field_Nop_const.m:76-84 over mv_L_const.m:61-85):
+ * E = Einc + (N(J) - Gram J) / (Gram j w eps0) at every voxel, w = 2 pi freq.
+ * op must be a field operator: idx = 1..3V (every voxel, in order), Mc = 1,
+ * no Mr, an FP64 copy (VP_ERR_INVALID otherwise); then A(J) = Gram J - N(J).
+ * J, Einc (nullable: zero), E: 3V complex128 following mem.
+ */
+int vp_field_total(vp_op op, const void* J, const void* Einc, double freq, void* E, int mem);
+
 #ifdef __cplusplus
 }
 #endif

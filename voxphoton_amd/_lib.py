@@ -98,6 +98,10 @@ SIGNATURES = {
     'vp_server_wait': (c_int, [c_vp, c_i64, P(GmresInfo)]),
     'vp_server_stats': (c_int, [c_vp, P(c_i64), P(c_i64), P(c_i64), P(c_i64)]),
     'vp_server_destroy': (c_int, [c_vp]),
+    'vp_field_dgf': (c_int, [c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_int]),
+    'vp_field_dgf_grid': (c_int, [c_vp, c_vp, c_vp, c_i64, c_vp, c_dbl, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp,
+                                  c_int]),
+    'vp_field_total': (c_int, [c_vp, c_vp, c_vp, c_dbl, c_vp, c_int]),
 }
 
 missing = []

@@ -74,8 +74,9 @@ class Problem:
         er = EPS_SI / EPS_SIO2
         return (er - 1.0) / er
 
-    def gaussian_rhs(self, pol='y', w0=300e-9):
-        """Vrhs for an x-propagating Gaussian-profile wave polarised along ``pol``."""
+    def gaussian_field(self, pol='y', w0=300e-9):
+        """Incident field (L x M x N x 3) of an x-propagating Gaussian-profile
+        wave polarised along ``pol``, at the voxel centres (l + 1/2) dx."""
         dx = self.dx
         y = (np.arange(self.M) + 0.5) * dx
         z = (np.arange(self.N) + 0.5) * dx
@@ -86,6 +87,11 @@ class Problem:
         comp = {'x': 0, 'y': 1, 'z': 2}[pol]
         Einc = np.zeros((self.L, self.M, self.N, 3), dtype=np.complex128)
         Einc[..., comp] = E
+        return Einc
+
+    def gaussian_rhs(self, pol='y', w0=300e-9):
+        """Vrhs for an x-propagating Gaussian-profile wave polarised along ``pol``."""
+        Einc = self.gaussian_field(pol, w0)
         tau = 1j * self.omega * EPS0 * self.Mc
         tau3 = np.concatenate([tau.reshape(-1, order='F')] * 3)
         i0 = self.idx - 1

@@ -9,6 +9,7 @@
 #include <tuple>
 #include <vector>
 #include "voxphoton.h"
+#include "capi_common.hpp"
 #include "op.hpp"
 
 namespace vp {
@@ -18,27 +19,6 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 using namespace vp;
 
-#define VP_API_BEGIN try {
-#define VP_API_END                                   \
-  }                                                  \
-  catch (const VpError& e) {                         \
-    set_error(e.msg);                                \
-    return e.code;                                   \
-  }                                                  \
-  catch (const std::bad_alloc&) {                    \
-    set_error("host allocation failed");             \
-    return VP_ERR_OOM;                               \
-  }                                                  \
-  catch (const std::exception& e) {                  \
-    set_error(e.what());                             \
-    return VP_ERR_DEVICE;                            \
-  }                                                  \
-  catch (...) {                                      \
-    set_error("unknown error");                      \
-    return VP_ERR_DEVICE;                            \
-  }                                                  \
-  set_error("");                                     \
-  return VP_OK;
 
 extern "C" {
 
@@ -302,6 +282,9 @@ static vp_op_s* op_begin(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp
     auto mc = gather_material(op, Mc, idx, 0.0);
     std::vector<double2> mr;
     if (Mr) mr = gather_material(op, Mr, idx, 1.0);
+    op->unit_contrast = !Mr && std::all_of(mc.begin(), mc.end(), [](double2 v) { return v.x == 1.0 && v.y == 0.0; });
+    op->all_voxels = n == 3 * op->V;
+    for (int64_t i = 0; op->all_voxels && i < n; ++i) op->all_voxels = idx[i] == i + 1;
     if (precisions & VP_FP64) init_prec(op, op->p64, true, mc, Mr ? &mr : nullptr, gram);
     if (precisions & VP_FP32) init_prec(op, op->p32, false, mc, Mr ? &mr : nullptr, gram / op->scale32);
   } catch (...) {

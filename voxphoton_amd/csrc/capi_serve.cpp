@@ -23,33 +23,10 @@
 #include <thread>
 #include <vector>
 #include "krylov.hpp"
+#include "capi_common.hpp"
 
-namespace vp {
-void set_error(const std::string& msg);
-}
 using namespace vp;
 
-#define VP_API_BEGIN try {
-#define VP_API_END                       \
-  }                                      \
-  catch (const VpError& e) {             \
-    set_error(e.msg);                    \
-    return e.code;                       \
-  }                                      \
-  catch (const std::bad_alloc&) {        \
-    set_error("host allocation failed"); \
-    return VP_ERR_OOM;                   \
-  }                                      \
-  catch (const std::exception& e) {      \
-    set_error(e.what());                 \
-    return VP_ERR_DEVICE;                \
-  }                                      \
-  catch (...) {                          \
-    set_error("unknown error");          \
-    return VP_ERR_DEVICE;                \
-  }                                      \
-  set_error("");                         \
-  return VP_OK;
 
 namespace {
 using Clock = std::chrono::steady_clock;

@@ -66,6 +66,8 @@ struct vp_op_s {
   double scale32 = 1.0;        // FP32 operator = A / scale32
   bool has_mr = false;
   bool fsym = false;           // tensor is parity symmetric: split variant stores its octant
+  bool unit_contrast = false;  // Mc = 1 at every unknown and no Mr: A = Gram I - N (field evaluation)
+  bool all_voxels = false;     // idx = 1..3V in order: unknown vector = the whole L x M x N x 3 field
   vp::DevBuf pos;              // voxel map
   vp::OpPrec<double> p64;
   vp::OpPrec<float> p32;

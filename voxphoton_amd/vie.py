@@ -5,6 +5,8 @@ Names, argument meaning and error behaviour follow the reference:
   * ``chan_2_mvp_for_parallel_idx``  pc/src_preconditioning/chan_2_mvp_for_parallel_idx.m:2
   * ``chan_mvp_idx``                 pc/src_preconditioning/chan_mvp_idx.m:2
   * ``pgmres``                       pc/src_numeric/src_iterative/pgmres.m:70
+  * ``E_field_DGF_mollified``        pc/src_vie/src_fields/src_dgf/E_field_DGF_mollified.m:1
+  * ``E_field_Nop_const``            pc/src_vie/src_fields/src_op/E_field_Nop_const.m:1
 All arithmetic runs in libvoxphoton.so on the GPU; this module only moves
 host arrays across the C ABI and keeps device-resident handles alive.
 """
@@ -134,6 +136,21 @@ class NOperator:
         """Device pointers (ints) in the given precision; no host traffic."""
         check(L.lib.vp_op_apply(self.handle, int(transp), ctypes.c_void_p(xdev), ctypes.c_void_p(ydev),
                                 L.VP_MEM_DEVICE, L.VP_FP64 if prec == 'fp64' else L.VP_FP32))
+
+    @classmethod
+    def for_fields(cls, shape, Gram=None, fN=None, G=None, dx=None, ko=None, precisions=('fp64',), ctx=None):
+        """Field operator over every voxel with unit contrast (A = Gram I - N),
+        the form E_field_Nop_const uses through mv_L_const: from fN or
+        opToeplitz G (with Gram), or assembled on the GPU for dx and ko
+        (Gram = dx^3)."""
+        Lg, Mg, Ng = shape
+        idx = np.arange(1, 3 * Lg * Mg * Ng + 1)
+        ones = np.ones(shape, complex)
+        if fN is not None:
+            return cls.from_fN(fN, None, ones, Gram, idx, precisions=precisions, ctx=ctx)
+        if G is not None:
+            return cls.from_toeplitz(G, None, ones, Gram, idx, precisions=precisions, ctx=ctx)
+        return cls.assembled(shape, dx, ko, None, ones, idx, precisions=precisions, ctx=ctx)
 
     def close(self):
         self._fin()
@@ -421,3 +438,70 @@ class Server:
             check(L.lib.vp_server_destroy(self.handle))
             self.handle = None
             self._keep.clear()
+
+
+def E_field_DGF_mollified(J, Icoord, Ocoord, ko, cutoff, ctx=None):
+    """Field of point currents J (3*Ni, column-major Ni x 3) at Icoord
+    (Ni x 3, complex coordinates allowed) observed at Ocoord (No x 3);
+    returns No x 3 (E_field_DGF_mollified.m:1-111)."""
+    ctx = ctx or default_context()
+    ic = np.atleast_2d(np.asarray(Icoord, dtype=complex))
+    ni = ic.shape[0]
+    j = as_c128(np.asarray(J).reshape(-1))
+    if j.size != 3 * ni:
+        raise ValueError('J must hold 3 components per source')
+    icf = as_c128(ic.reshape(-1, order='F'))
+    oc = np.atleast_2d(np.asarray(Ocoord, dtype=float))
+    no = oc.shape[0]
+    ocf = np.ascontiguousarray(oc.reshape(-1, order='F'))
+    E = np.empty(3 * no, dtype=complex)
+    check(L.lib.vp_field_dgf(ctx.handle, ptr(j), ptr(icf), ni, ptr(ocf), no, float(ko), float(cutoff), ptr(E),
+                             L.VP_MEM_HOST))
+    return E.reshape(no, 3, order='F')
+
+
+def E_field_DGF_grid(J, Icoord, origin, dx, shape, ko, cutoff, ctx=None):
+    """E_field_DGF_mollified at the voxel centres origin + (l, m, n) dx of a
+    grid (the configs' Ocoord = reshape(r, L*M*N, 3)); returns L x M x N x 3."""
+    ctx = ctx or default_context()
+    ic = np.atleast_2d(np.asarray(Icoord, dtype=complex))
+    ni = ic.shape[0]
+    j = as_c128(np.asarray(J).reshape(-1))
+    if j.size != 3 * ni:
+        raise ValueError('J must hold 3 components per source')
+    icf = as_c128(ic.reshape(-1, order='F'))
+    o = np.ascontiguousarray(np.asarray(origin, dtype=float).reshape(3))
+    Lg, Mg, Ng = shape
+    E = np.empty(3 * Lg * Mg * Ng, dtype=complex)
+    check(L.lib.vp_field_dgf_grid(ctx.handle, ptr(j), ptr(icf), ni, ptr(o), float(dx), Lg, Mg, Ng, float(ko),
+                                  float(cutoff), ptr(E), L.VP_MEM_HOST))
+    return E.reshape(Lg, Mg, Ng, 3, order='F')
+
+
+def E_field_Nop_const(J, op, freq, Einc=None):
+    """Total field Einc + (N(J) - Gram J) / (Gram j w eps0) (E_field_Nop_const.m:1-84).
+
+    ``op`` is a field operator (``NOperator.for_fields``) of the grid, the
+    counterpart of the reference's (fN, Gram) pair; J, Einc: L x M x N x 3.
+    A zero J returns Einc, as the reference."""
+    J = np.asarray(J)
+    if Einc is None:
+        Einc = np.zeros(J.shape, dtype=complex)
+    if not np.any(J):
+        return np.array(Einc, dtype=complex)
+    j = as_c128(J.reshape(-1, order='F'))
+    if j.size != op.n:
+        raise ValueError(f'field of {j.size} entries for an operator over {op.n}')
+    e0 = as_c128(np.asarray(Einc).reshape(-1, order='F'))
+    E = np.empty_like(j)
+    check(L.lib.vp_field_total(op.handle, ptr(j), ptr(e0), float(freq), ptr(E), L.VP_MEM_HOST))
+    return E.reshape(J.shape, order='F')
+
+
+def power_through(E, x_thru, m_edge):
+    """Transmitted power: sum of |E|^2 over the cross-section x = x_thru
+    (1-based), m_edge < m <= M - m_edge, all n and components
+    (example_Bragg_1500nm_to_1600nm.m:108-113), on the host field."""
+    M = E.shape[1]
+    s = E[x_thru - 1, m_edge:M - m_edge, :, :]
+    return float(np.sum((s * np.conj(s)).real))
