@@ -89,7 +89,8 @@ def kernel_model_bytes(prob, kernel, s, fsym=True, pc_box=None):
     Preconditioner kernels (pc_box = (Lc, Mc, Nc): two circulant axes and
     the dense one): the quadrant form stores (Lc/2+1)(Mc/2+1) blocks of
     (3 Nc)^2 entries, read once per apply ('pc_block', which also reads and
-    writes the 3 Lc Mc Nc spectra); 'pc_yrows' reads and writes the spectra.
+    writes the 3 Lc Mc Nc spectra; FP32 applies read binary16 blocks unless
+    VP_PC_HALF=0); 'pc_yrows' reads and writes the spectra.
     """
     if kernel.startswith('pc_'):
         if pc_box is None:
@@ -97,8 +98,9 @@ def kernel_model_bytes(prob, kernel, s, fsym=True, pc_box=None):
         Lc, Mc, Nc = pc_box
         V3 = 3 * Lc * Mc * Nc
         n = prob.n
-        if kernel == 'pc_block':
-            return s * ((Lc // 2 + 1) * (Mc // 2 + 1) * (3 * Nc) ** 2 + 2 * V3)
+        if kernel == 'pc_block':   # FP32 applies stream the blocks as binary16 pairs (4 bytes)
+            bs = 4 if s == 8 and os.environ.get('VP_PC_HALF', '1') != '0' else s
+            return bs * (Lc // 2 + 1) * (Mc // 2 + 1) * (3 * Nc) ** 2 + s * 2 * V3
         if kernel == 'pc_yrows':
             return s * 2 * V3
         if kernel == 'pc_xfwd':

@@ -81,6 +81,9 @@ def test_byte_models():
     z = bench.kernel_model_bytes(prob, 'zpass', 8)
     Pc = 3601 * 101 * 45
     assert z == 8 * (6 * Pc + 6 * 7200 * 200 * 22)
+    # quadrant blocks of the face-averaged preconditioner, binary16 in FP32 applies
+    pb = bench.kernel_model_bytes(prob, 'pc_block', 8, pc_box=(3600, 22, 60))
+    assert pb == 4 * 1801 * 12 * 180 ** 2 + 8 * 2 * 3 * 3600 * 22 * 60
 
 
 def test_server_arguments_fail_cleanly():

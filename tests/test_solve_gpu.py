@@ -71,9 +71,13 @@ def test_level2_device_setup_matches_oracle(vp_ctx, monkeypatch, shape, sym, mod
     assert rel(y, ref) < 1e-10
 
 
-def test_level2_blocks_wider_than_a_wavefront(vp_ctx):
-    # 3N = 66 > 64: block inversion spans several wavefronts
+@pytest.mark.parametrize('half', ['1', '0'])
+def test_level2_blocks_wider_than_a_wavefront(vp_ctx, monkeypatch, half):
+    # 3N = 66 > 64: block inversion spans several wavefronts.  The FP32 apply
+    # of the quadrant form streams binary16 blocks by default (11-bit
+    # significands: ~5e-4 per entry) or FP32 blocks with VP_PC_HALF=0.
     from voxphoton_amd.vie import Preconditioner
+    monkeypatch.setenv('VP_PC_HALF', half)
     L, M, N = 4, 3, 22
     dx, ko = 20e-9, 5.85e6
     G = ogr.assemble_G(L, M, N, dx, ko)
@@ -84,7 +88,11 @@ def test_level2_blocks_wider_than_a_wavefront(vp_ctx):
     ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
     P = Preconditioner.level2(G, 0.8278, 1.0, dx ** 3, idx, ctx=vp_ctx)
     assert rel(P.apply(x), ref) < 1e-10
-    assert rel(P.apply(x, 'fp32'), ref) < 1e-4
+    e32 = rel(P.apply(x, 'fp32'), ref)
+    print('fp32 apply, binary16 blocks' if half == '1' else 'fp32 apply', e32, flush=True)
+    assert e32 < (2e-3 if half == '1' else 1e-4)
+    if half == '1':
+        assert e32 > 1e-6      # the binary16 copy is what ran
 
 
 @pytest.mark.parametrize('quad', ['1', '0'])
@@ -104,6 +112,7 @@ def test_level2_face_matches_oracle(vp_ctx, monkeypatch, sym, quad):
     ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
     P = Preconditioner.level2_face((L, M, N), dx, ko, face, idx, ctx=vp_ctx)
     assert rel(P.apply(x), ref) < 1e-10
+    assert rel(P.apply(x, 'fp32'), ref) < (2e-3 if sym == '1' and quad == '1' else 1e-4)
 
 
 def test_level2_plane_larger_than_lds(vp_ctx):
@@ -118,7 +127,7 @@ def test_level2_plane_larger_than_lds(vp_ctx):
     ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
     P = Preconditioner.level2(G, 0.5, 1.0, 1.0, idx, ctx=vp_ctx)
     assert rel(P.apply(x), ref) < 1e-10
-    assert rel(P.apply(x, 'fp32'), ref) < 1e-4
+    assert rel(P.apply(x, 'fp32'), ref) < 2e-3      # binary16 blocks of the quadrant form
 
 
 def test_level2_without_parity_zeros_keeps_all_blocks(vp_ctx):

@@ -137,6 +137,10 @@ static void prec_make_fp32(vp_prec_s* p, double scale32) {
   VP_HIP(hipStreamSynchronize(p->ctx->stream));
   p->scale32 = scale32;
   prec_alloc_prec<float>(p, p->p32);
+  // quadrant form: the FP32 apply streams binary16 blocks (VP_PC_HALF=0 keeps FP32)
+  const char* eh = std::getenv("VP_PC_HALF");
+  p->p32.hblocks.release();
+  if (p->g.level == 2 && p->g.pquad && !(eh && eh[0] == '0')) make_half_blocks(p->p32, p->ctx->stream);
 }
 
 // blocks_dev: FP64 blocks in (kx, ky)-major order, already inverted; scaled here by 1/(L*M)
@@ -220,6 +224,9 @@ void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* 
   if (ps.has_prec) {
     vp_prec_s* q0 = precs[0];
     std::vector<const C<T>*> bps(np);
+    std::vector<const uint32_t*> hps(np, nullptr);
+    std::vector<float> hsc(np, 1.f);
+    bool half = true;
     for (int p = 0; p < np; ++p) {
       vp_prec_s* q = precs[p];
       VP_REQUIRE(q && q->g.L == q0->g.L && q->g.M == q0->g.M && q->g.N == q0->g.N && q->nbox == q0->nbox &&
@@ -230,6 +237,11 @@ void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* 
       PrecPrec<T>& Q = pprec_of<T>(q);
       VP_REQUIRE(Q.present, ERR_INVALID, "preconditioner lacks this precision");
       bps[p] = Q.blocks.template as<C<T>>();
+      half = half && Q.hblocks.p != nullptr;
+      if (Q.hblocks.p) {
+        hps[p] = Q.hblocks.template as<uint32_t>();
+        hsc[p] = Q.hscale;
+      }
     }
     const int64_t outer = q0->g.n_outer ? q0->g.n_outer : q0->nbox;
     VP_REQUIRE(outer == ps.n, ERR_INVALID, "preconditioner size does not match the operator");
@@ -244,6 +256,16 @@ void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* 
     a.work2 = q0->g.level == 1 ? ps.pwork2.template as<C<T>>() : nullptr;
     a.wstride = wst;
     a.blocks = ps.pblocks.template as<const C<T>*>();
+    a.hblocks = nullptr;
+    a.hscale = nullptr;
+    if (half) {   // every problem's preconditioner has the binary16 copy
+      ps.phblocks.alloc(sizeof(const uint32_t*) * np);
+      VP_HIP(hipMemcpy(ps.phblocks.p, hps.data(), ps.phblocks.bytes, hipMemcpyHostToDevice));
+      ps.phscale.alloc(sizeof(float) * np);
+      VP_HIP(hipMemcpy(ps.phscale.p, hsc.data(), ps.phscale.bytes, hipMemcpyHostToDevice));
+      a.hblocks = ps.phblocks.template as<const uint32_t*>();
+      a.hscale = ps.phscale.template as<float>();
+    }
     ps.pa = a;
   }
 }

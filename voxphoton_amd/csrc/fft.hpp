@@ -148,6 +148,66 @@ __device__ __forceinline__ void dft(C<T>* v) {
   else dft_odd<T, R, INV>(v);
 }
 
+// Pruned radix-R DFTs for zero-padded transforms, rc uniform over the
+// workgroup (1 <= rc <= R):
+//   dft_in_pruned : only inputs v[r], r < rc, are nonzero; all R outputs.
+//   dft_out_pruned: all R inputs; only outputs q < rc are wanted (odd R by
+//                   the symmetric pairs of dft_odd; the rest are left as is).
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft_in_pruned(C<T>* v, int rc) {
+  constexpr RootTable<R> rt{};
+  C<T> o[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) o[k] = v[0];
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    if (r < rc) {
+      o[0] = cadd(o[0], v[r]);
+#pragma unroll
+      for (int k = 1; k < R; ++k) {
+        const int e = (r * k) % R;
+        o[k] = cadd(o[k], cmul(v[r], mk<T>(T(rt.c[e]), INV ? T(rt.s[e]) : T(-rt.s[e]))));
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) v[k] = o[k];
+}
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft_out_pruned(C<T>* v, int rc) {
+  if constexpr (R % 2 == 0) {
+    dft<T, R, INV>(v);
+  } else {
+    constexpr int H = (R - 1) / 2;
+    constexpr RootTable<R> rt{};
+    C<T> a[H], b[H];
+#pragma unroll
+    for (int j = 1; j <= H; ++j) { a[j - 1] = cadd(v[j], v[R - j]); b[j - 1] = csub(v[j], v[R - j]); }
+    const C<T> x0 = v[0];
+    C<T> y0 = x0;
+#pragma unroll
+    for (int j = 0; j < H; ++j) y0 = cadd(y0, a[j]);
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+      if (k < rc || R - k < rc) {
+        T re = x0.x, im = x0.y, sr = T(0), si = T(0);
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+          const T c = T(rt.c[(j * k) % R]);
+          const T sn = T(rt.s[(j * k) % R]);
+          re += a[j - 1].x * c; im += a[j - 1].y * c;
+          sr += sn * b[j - 1].y; si -= sn * b[j - 1].x;
+        }
+        if (INV) { sr = -sr; si = -si; }
+        v[k] = mk<T>(re + sr, im + si);
+        v[R - k] = mk<T>(re - sr, im - si);
+      }
+    }
+    v[0] = y0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Division by a runtime-invariant divisor d (1 <= d < 2^31) for numerators
 // n < 2^31: q = (umulhi(n, m) + n) >> s with s = ceil(log2 d) and

@@ -27,6 +27,7 @@ struct ProblemSet {
   PcGeom pg{};
   PcArgs<T> pa{};                 // x, y, active, strides filled at call time
   DevBuf pblocks, pwork, pwork2;  // block pointer array, workspaces
+  DevBuf phblocks, phscale;       // binary16 block pointers and scales (FP32 quadrant form)
   DevBuf active;                  // int[nprob]
 };
 

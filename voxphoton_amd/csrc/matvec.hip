@@ -502,7 +502,8 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) k_zfused(const 
   const ProbPtrs<T> P = probs[blockIdx.y];
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C<T>* W = reinterpret_cast<C<T>*>(smem_raw);
-  const int N = g.N, Mp = g.y.P, Lp = g.x.P, tpk = mp.ztpk;
+  constexpr int TPK = zfused_threads(RA, RB);   // threads per ky column (launch width / TZ)
+  const int N = g.N, Mp = g.y.P, Lp = g.x.P;
   const int nt = (Mp + TZ - 1) / TZ;
   // kx order and residue-class storage as in k_zpass
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -518,8 +519,12 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) k_zfused(const 
   const BufRes Y = buf_res(P.work2 + int64_t(kxi) * 3 * MpN, 3 * MpN * uint32_t(sizeof(C<T>)));
   const int rc = (N + RB - 1) / RB;                              // A inputs that can be nonzero
 
+  // butterflies of A see at most rc nonzero inputs and B' keeps at most rc
+  // outputs: pruned DFTs when that is at most half the radix
+  const bool prune = 2 * rc <= RA;
+
   // ---- A: forward radix-RA stage from the spectra
-  for (int it = w; it < 3 * RB; it += tpk) {
+  for (int it = w; it < 3 * RB; it += TPK) {
     const int c = it / RB, j = it - c * RB;
     C<T> v[RA];
 #pragma unroll
@@ -528,7 +533,8 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) k_zfused(const 
       v[r] = czero<T>();
       if (r < rc && kyok && n < N) v[r] = buf_load<T>(Y, c * MpN + uint32_t(n) * Mp + ky);
     }
-    dft<T, RA, false>(v);
+    if (prune) dft_in_pruned<T, RA, false>(v, rc);
+    else dft<T, RA, false>(v);
 #pragma unroll
     for (int k = 0; k < RA; ++k) W[(j * RA + k) * LD + c * TZ + t] = v[k];
   }
@@ -536,7 +542,7 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) k_zfused(const 
 
   // ---- B: second forward stage, tensor multiply, first inverse stage
   C<T> x0[RB], x1[RB], x2[RB];
-  const bool inb = w < RA;
+  const bool inb = w < RA;   // TPK >= RA: one stage-B butterfly per thread at most
   if (inb) {
     const int j = w;
     C<T> tw[RB];
@@ -611,7 +617,7 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) k_zfused(const 
   __syncthreads();
 
   // ---- B': last inverse stage (radix RA), cropped to n < N
-  for (int it = w; it < 3 * RB; it += tpk) {
+  for (int it = w; it < 3 * RB; it += TPK) {
     const int c = it / RB, j = it - c * RB;
     C<T> v[RA];
 #pragma unroll
@@ -619,12 +625,13 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) k_zfused(const 
       v[r] = W[(j + RB * r) * LD + c * TZ + t];
       if (r > 0) v[r] = cmul(v[r], twiddle<T, true>(twz, j * r));
     }
-    dft<T, RA, true>(v);
+    if (prune) dft_out_pruned<T, RA, true>(v, rc);
+    else dft<T, RA, true>(v);
     if (kyok) {
 #pragma unroll
       for (int r = 0; r < RA; ++r) {
         const int n = j + RB * r;
-        if (n < N) buf_store(Y, c * MpN + uint32_t(n) * Mp + ky, v[r]);
+        if (r < rc && n < N) buf_store(Y, c * MpN + uint32_t(n) * Mp + ky, v[r]);
       }
     }
   }
@@ -916,14 +923,18 @@ static void launch_xpass_ct(const Geometry& g, const MatvecPlan& mp, const ProbP
   const unsigned npen = unsigned(3 * g.N * g.M);
   const dim3 grid(INV ? npen : npen * unsigned(g.x.R), nprob);
   bool done = false;
-#define VP_XCT_LAUNCH(SL, NT, ...)                                                                      \
+#define VP_XCT_LAUNCH(SL, NTF, FR, NTI, IR)                                                             \
   if (!done && mp.xct == SL) {                                                                          \
     const size_t lds = 2 * size_t(SL) * sizeof(C<T>);                                                   \
-    const void* fn = INV ? reinterpret_cast<const void*>(&k_xinv_ct<T, NT, __VA_ARGS__>)                 \
-                         : reinterpret_cast<const void*>(&k_xfwd_ct<T, NT, __VA_ARGS__>);                \
-    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));                \
-    if (INV) hipLaunchKernelGGL((k_xinv_ct<T, NT, __VA_ARGS__>), grid, dim3(NT), lds, s, d_probs, va, g, transp, twx); \
-    else hipLaunchKernelGGL((k_xfwd_ct<T, NT, __VA_ARGS__>), grid, dim3(NT), lds, s, d_probs, va, g, transp, twx);     \
+    if (INV) {                                                                                          \
+      const void* fn = reinterpret_cast<const void*>(&k_xinv_ct<T, NTI, VP_UNPAREN IR>);                \
+      (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));              \
+      hipLaunchKernelGGL((k_xinv_ct<T, NTI, VP_UNPAREN IR>), grid, dim3(NTI), lds, s, d_probs, va, g, transp, twx); \
+    } else {                                                                                            \
+      const void* fn = reinterpret_cast<const void*>(&k_xfwd_ct<T, NTF, VP_UNPAREN FR>);                \
+      (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));              \
+      hipLaunchKernelGGL((k_xfwd_ct<T, NTF, VP_UNPAREN FR>), grid, dim3(NTF), lds, s, d_probs, va, g, transp, twx); \
+    }                                                                                                   \
     done = true;                                                                                        \
   }
   if constexpr (sizeof(T) == 8) {

@@ -78,28 +78,6 @@ bool zfused_pair(int Np, bool fp64, int& ra, int& rb) {
   return false;
 }
 
-// Threads per ky column: at least RA (stage B gives one butterfly per
-// thread), at most 256 / kZfTile; chosen by a VALU cost model of the three
-// stages (A and B' loop over 3*RB items, B runs once).
-int zfused_threads(int ra, int rb) {
-  auto dftc = [](int r) {
-    switch (r) {
-      case 2: return 4; case 3: return 12; case 4: return 16; case 5: return 32; case 7: return 64;
-      case 8: return 56; case 11: return 130; default: return 144;
-    }
-  };
-  const int items = 3 * rb;
-  const long long ca = dftc(ra) + 3 * ra, cb = 3 * (2 * dftc(rb) + 5 * rb) + 40 * rb;
-  int best = ra;
-  long long bestc = -1;
-  for (int tpk = ra; tpk <= 256 / kZfTile; ++tpk) {
-    const long long passes = (items + tpk - 1) / tpk;
-    const long long c = tpk * (2 * passes * ca + cb);
-    if (bestc < 0 || c < bestc) { bestc = c; best = tpk; }
-  }
-  return best;
-}
-
 bool yct_supported(int n) {
 #define VP_YCT_CASE(NL, ...) case NL:
   switch (n) {
@@ -111,8 +89,8 @@ bool yct_supported(int n) {
 }
 
 int xct_threads(int S, bool fp64) {
-#define VP_XCT_CASE(SL, NT, ...) \
-  case SL: return NT;
+#define VP_XCT_CASE(SL, NTF, FR, NTI, IR) \
+  case SL: return NTF;
   if (fp64) {
     switch (S) {
       VP_XCT64_LIST(VP_XCT_CASE)

@@ -13,6 +13,7 @@
 // DFT along y, then batched Gauss-Jordan inversion in FP64.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 #include "precond.hpp"
 
@@ -232,15 +233,38 @@ __global__ void __launch_bounds__(256) k_pc_quadmul(PcArgs<T> a, PcGeom g) {
     Xs[c][q] = v;
   }
   __syncthreads();
-  const C<T>* Bb = blocks + (int64_t(kxc) * Hy + kyc) * n3 * n3;
+  const int64_t boff = (int64_t(kxc) * Hy + kyc) * n3 * n3;
+  const C<T>* Bb = blocks + boff;
+  const uint32_t* Hb = a.hblocks ? a.hblocks[blockIdx.y] + boff : nullptr;
   for (int r = threadIdx.x; r < n3; r += blockDim.x) {
     C<T> acc0 = czero<T>(), acc1 = czero<T>(), acc2 = czero<T>(), acc3 = czero<T>();
-    for (int q = 0; q < n3; ++q) {
-      const C<T> b = Bb[int64_t(q) * n3 + r];
-      acc0 = cadd(acc0, cmul(b, Xs[0][q]));
-      acc1 = cadd(acc1, cmul(b, Xs[1][q]));
-      acc2 = cadd(acc2, cmul(b, Xs[2][q]));
-      acc3 = cadd(acc3, cmul(b, Xs[3][q]));
+    if constexpr (sizeof(T) == 4) {
+      if (Hb) {
+        // binary16 blocks: half the bytes of the block stream
+        for (int q = 0; q < n3; ++q) {
+          const uint32_t hw = Hb[int64_t(q) * n3 + r];
+          const C<T> b = mk<T>(T(__builtin_bit_cast(_Float16, uint16_t(hw & 0xffffu))),
+                               T(__builtin_bit_cast(_Float16, uint16_t(hw >> 16))));
+          acc0 = cadd(acc0, cmul(b, Xs[0][q]));
+          acc1 = cadd(acc1, cmul(b, Xs[1][q]));
+          acc2 = cadd(acc2, cmul(b, Xs[2][q]));
+          acc3 = cadd(acc3, cmul(b, Xs[3][q]));
+        }
+        const T hs = T(a.hscale[blockIdx.y]);
+        acc0 = cscale(acc0, hs);
+        acc1 = cscale(acc1, hs);
+        acc2 = cscale(acc2, hs);
+        acc3 = cscale(acc3, hs);
+      }
+    }
+    if (!Hb) {
+      for (int q = 0; q < n3; ++q) {
+        const C<T> b = Bb[int64_t(q) * n3 + r];
+        acc0 = cadd(acc0, cmul(b, Xs[0][q]));
+        acc1 = cadd(acc1, cmul(b, Xs[1][q]));
+        acc2 = cadd(acc2, cmul(b, Xs[2][q]));
+        acc3 = cadd(acc3, cmul(b, Xs[3][q]));
+      }
     }
     const int cr = r / N;
     const C<T> accs[4] = {acc0, acc1, acc2, acc3};
@@ -318,6 +342,47 @@ __global__ void k_copy_vec(const C<T>* __restrict__ x, C<T>* __restrict__ y, int
   C<T>* Y = y + int64_t(blockIdx.y) * ystride;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     Y[i] = X[i];
+}
+
+// max over entries of max(|re|, |im|) (non-negative floats order as their bits)
+__global__ void k_absmax_c64(const float2* __restrict__ a, int64_t n, unsigned* __restrict__ out) {
+  float m = 0.f;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    m = fmaxf(m, fmaxf(fabsf(a[i].x), fabsf(a[i].y)));
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+__global__ void k_to_half_pairs(const float2* __restrict__ a, int64_t n, float inv, uint32_t* __restrict__ out) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const _Float16 re = _Float16(a[i].x * inv), im = _Float16(a[i].y * inv);
+    out[i] = uint32_t(__builtin_bit_cast(uint16_t, re)) | (uint32_t(__builtin_bit_cast(uint16_t, im)) << 16);
+  }
+}
+
+void make_half_blocks(PrecPrec<float>& P, hipStream_t s) {
+  const int64_t n = int64_t(P.blocks.bytes / sizeof(float2));
+  DevBuf mx(sizeof(unsigned));
+  VP_HIP(hipMemsetAsync(mx.p, 0, sizeof(unsigned), s));
+  const unsigned nb = unsigned(std::min<int64_t>((n + 255) / 256, 16384));
+  hipLaunchKernelGGL(k_absmax_c64, dim3(nb), dim3(256), 0, s, P.blocks.as<float2>(), n, mx.as<unsigned>());
+  unsigned bits = 0;
+  VP_HIP(hipMemcpyAsync(&bits, mx.p, sizeof(bits), hipMemcpyDeviceToHost, s));
+  VP_HIP(hipStreamSynchronize(s));
+  float m;
+  std::memcpy(&m, &bits, sizeof(m));
+  // entries land in [-1, 1] x 2^15: well inside binary16's normal range
+  P.hscale = m > 0.f ? m / 32768.f : 1.f;
+  P.hblocks.alloc(size_t(n) * sizeof(uint32_t));
+  hipLaunchKernelGGL(k_to_half_pairs, dim3(nb), dim3(256), 0, s, P.blocks.as<float2>(), n, 1.f / P.hscale,
+                     P.hblocks.as<uint32_t>());
+  VP_HIP(hipGetLastError());
+  const uint32_t* hp = P.hblocks.as<uint32_t>();
+  P.hblockptr.alloc(sizeof(hp));
+  VP_HIP(hipMemcpyAsync(P.hblockptr.p, &hp, sizeof(hp), hipMemcpyHostToDevice, s));
+  P.hscaleptr.alloc(sizeof(float));
+  VP_HIP(hipMemcpyAsync(P.hscaleptr.p, &P.hscale, sizeof(float), hipMemcpyHostToDevice, s));
+  VP_HIP(hipStreamSynchronize(s));
 }
 
 template <typename T>

@@ -41,6 +41,10 @@ struct PcArgs {
   const C<T>* twx;      // L entries
   const C<T>* twy;      // M entries
   const int* active;
+  // FP32 apply of the quadrant form only: blocks stored as binary16 pairs
+  // scaled by 1/hscale (nullptr: the blocks above)
+  const uint32_t* const* hblocks;
+  const float* hscale;
 };
 
 template <typename T>
@@ -51,6 +55,11 @@ struct PrecPrec {
   DevBuf tw;            // L | M twiddles
   DevBuf work, work2;
   DevBuf xs, ys;        // staging for host calls
+  // FP32 quadrant form: binary16 copy of the blocks (real, imaginary halves
+  // of one 32-bit word), scaled by 1/hscale; pointer and scale as device
+  // arrays of one entry for single-problem launches
+  DevBuf hblocks, hblockptr, hscaleptr;
+  float hscale = 1.f;
 };
 
 template <typename T>
@@ -69,6 +78,8 @@ void build_level2_blocks(const double2* G, int L, int M, int N, double2 mc, doub
 // Chan circulants parity symmetric.
 bool toeplitz_parity_zero(const double2* G, int L, int M, int N, hipStream_t s);
 void invert_blocks(double2* blk, int64_t nblocks, int n, hipStream_t s);
+// binary16 copy of the FP32 blocks (P.hblocks, P.hscale) for the quadrant form
+void make_half_blocks(PrecPrec<float>& P, hipStream_t s);
 
 }  // namespace vp
 
@@ -106,6 +117,8 @@ inline PcArgs<T> prec_args(vp_prec_s* p, const C<T>* x, C<T>* y) {
   a.twx = P.tw.template as<C<T>>();
   a.twy = P.tw.template as<C<T>>() + p->g.L;
   a.active = nullptr;
+  a.hblocks = P.hblocks.p ? P.hblockptr.template as<const uint32_t*>() : nullptr;
+  a.hscale = P.hblocks.p ? P.hscaleptr.template as<float>() : nullptr;
   return a;
 }
 }  // namespace vp

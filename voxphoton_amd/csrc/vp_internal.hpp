@@ -129,7 +129,25 @@ bool factorize(int n, bool fp64, FftPlan& out, int tws, bool odd_first = false);
 // threads per column (plan.cpp).
 constexpr int kZfTile = 16;
 bool zfused_pair(int Np, bool fp64, int& ra, int& rb);
-int zfused_threads(int ra, int rb);
+// Threads per ky column: at least RA (stage B gives one butterfly per
+// thread), at most 256 / kZfTile; chosen by a VALU cost model of the three
+// stages (A and B' loop over 3*RB items, B runs once).  A compile-time
+// constant of the kernel and the launch width on the host.
+__host__ __device__ constexpr int zfused_threads(int ra, int rb) {
+  auto dftc = [](int r) {
+    return r == 2 ? 4 : r == 3 ? 12 : r == 4 ? 16 : r == 5 ? 32 : r == 7 ? 64 : r == 8 ? 56 : r == 11 ? 130 : 144;
+  };
+  const int items = 3 * rb;
+  const long long ca = dftc(ra) + 3 * ra, cb = 3 * (2 * dftc(rb) + 5 * rb) + 40 * rb;
+  int best = ra;
+  long long bestc = -1;
+  for (int tpk = ra; tpk <= 256 / kZfTile; ++tpk) {
+    const long long passes = (items + tpk - 1) / tpk;
+    const long long c = tpk * (2 * passes * ca + cb);
+    if (bestc < 0 || c < bestc) { bestc = c; best = tpk; }
+  }
+  return best;
+}
 // Compile-time y pass: pencils per workgroup for length n and element size
 // (two ping-pong buffers of n x (NB + 1) plus the twiddle table within
 // 40 KB), and the radix list instantiated for n (plan.cpp; false: none).
@@ -144,13 +162,22 @@ __host__ __device__ constexpr int yct_lds(int n, int elem) { return (2 * n * (yc
   X(12, 3, 4) X(16, 4, 4) X(20, 4, 5) X(24, 3, 8) X(32, 4, 8) X(40, 5, 8) X(48, 3, 4, 4) X(64, 8, 8)     \
   X(96, 3, 4, 8) X(100, 4, 5, 5) X(120, 3, 5, 8) X(128, 2, 8, 8) X(200, 5, 5, 8) X(256, 4, 8, 8)
 bool yct_supported(int n);
-// Class lengths S = Lp/R with compile-time x passes: (S, threads, radices).
+// Class lengths S = Lp/R with compile-time x passes:
+// X(S, forward threads, (forward radices), inverse threads, (inverse radices)).
+// The forward pass gathers through the voxel map in its first stage, so a
+// first stage with many butterflies (small radix) keeps more lanes loading;
+// the inverse reads contiguous data and ends in LDS.
+#define VP_UNPAREN(...) __VA_ARGS__
 #define VP_XCT32_LIST(X)                                                                                \
-  X(3600, 512, 16, 9, 5, 5) X(1800, 512, 8, 9, 5, 5) X(1200, 512, 16, 3, 5, 5) X(900, 256, 4, 9, 5, 5)    \
-  X(450, 256, 2, 9, 5, 5) X(400, 256, 16, 5, 5) X(360, 256, 8, 9, 5) X(200, 256, 8, 5, 5) X(100, 256, 4, 5, 5)
+  X(3600, 768, (5, 5, 9, 16), 512, (16, 9, 5, 5)) X(1800, 512, (8, 9, 5, 5), 512, (8, 9, 5, 5))           \
+  X(1200, 512, (16, 3, 5, 5), 512, (16, 3, 5, 5)) X(900, 256, (4, 9, 5, 5), 256, (4, 9, 5, 5))             \
+  X(450, 256, (2, 9, 5, 5), 256, (2, 9, 5, 5)) X(400, 256, (16, 5, 5), 256, (16, 5, 5))                   \
+  X(360, 256, (8, 9, 5), 256, (8, 9, 5)) X(200, 256, (8, 5, 5), 256, (8, 5, 5)) X(100, 256, (4, 5, 5), 256, (4, 5, 5))
 #define VP_XCT64_LIST(X)                                                                                \
-  X(3600, 512, 8, 2, 9, 5, 5) X(1800, 512, 8, 9, 5, 5) X(1200, 512, 8, 2, 3, 5, 5) X(900, 256, 4, 9, 5, 5) \
-  X(450, 256, 2, 9, 5, 5) X(400, 256, 4, 4, 5, 5) X(360, 256, 8, 9, 5) X(200, 256, 8, 5, 5) X(100, 256, 4, 5, 5)
+  X(3600, 512, (8, 2, 9, 5, 5), 512, (8, 2, 9, 5, 5)) X(1800, 512, (8, 9, 5, 5), 512, (8, 9, 5, 5))       \
+  X(1200, 512, (8, 2, 3, 5, 5), 512, (8, 2, 3, 5, 5)) X(900, 256, (4, 9, 5, 5), 256, (4, 9, 5, 5))         \
+  X(450, 256, (2, 9, 5, 5), 256, (2, 9, 5, 5)) X(400, 256, (4, 4, 5, 5), 256, (4, 4, 5, 5))               \
+  X(360, 256, (8, 9, 5), 256, (8, 9, 5)) X(200, 256, (8, 5, 5), 256, (8, 5, 5)) X(100, 256, (4, 5, 5), 256, (4, 5, 5))
 // Threads of the compile-time x pass for class length S (0: none).
 int xct_threads(int S, bool fp64);
 int max_lds_bytes(int device);
