@@ -115,6 +115,31 @@ def test_level2_face_matches_oracle(vp_ctx, monkeypatch, sym, quad):
     assert rel(P.apply(x, 'fp32'), ref) < (2e-3 if sym == '1' and quad == '1' else 1e-4)
 
 
+@pytest.mark.parametrize('ct', ['1', '0'])
+@pytest.mark.parametrize('shape', [(100, 10, 3), (200, 11, 2)])
+def test_level2_pencil_major_quadrant(vp_ctx, monkeypatch, shape, ct):
+    # quadrant form over pencil-major spectra (compile-time x passes of length
+    # L and row passes of length M) against the plane-major form and the
+    # oracle; the Toeplitz tensor is the device assembly (pinned elsewhere)
+    from voxphoton_amd.vie import Preconditioner, assemble_toeplitz
+    monkeypatch.setenv('VP_PC_CT', ct)
+    L, M, N = shape
+    dx, ko = 20e-9, 5.85e6
+    G = assemble_toeplitz(shape, dx, ko, ctx=vp_ctx)
+    mc, gram = 0.8278, dx ** 3
+    mask = np.ones(shape, bool)
+    mask[::7, 1, :] = False
+    idx = ovie.idx_from_mask(mask)
+    Binv = opre.level_2_parallel_func(G, mc, 1.0, L, M, N, gram)
+    rng = np.random.default_rng(L + M)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
+    P = Preconditioner.level2(G, mc, 1.0, gram, idx, ctx=vp_ctx)
+    e64, e32 = rel(P.apply(x), ref), rel(P.apply(x, 'fp32'), ref)
+    print(shape, 'ct', ct, 'rel', e64, e32, flush=True)
+    assert e64 < 1e-10 and e32 < 2e-3
+
+
 def test_level2_plane_larger_than_lds(vp_ctx):
     # 3N x M plane of 90 x 60 complex128 exceeds LDS: the split passes run
     from voxphoton_amd.vie import Preconditioner

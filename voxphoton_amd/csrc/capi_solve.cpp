@@ -49,6 +49,7 @@ static void prec_geometry(vp_prec_s* p, int level) {
   g.lds2 = size_t(2) * 3 * g.N * g.M * s;
   g.psplit = 0;
   g.pquad = 0;
+  g.pct = 0;
   g.Brow = 0;
   if (level == 2) {
     // y-row DFT passes (split and quadrant paths): Brow rows per workgroup
@@ -457,6 +458,14 @@ static void level2_block_range(vp_prec_s* p, const double2* dG) {
   // columns (VP_PC_QUAD=0 keeps the per-plane kernel)
   const char* eq = std::getenv("VP_PC_QUAD");
   g.pquad = (g.bsym && 3 * g.N <= 255 && g.Brow > 0 && !(eq && eq[0] == '0')) ? 1 : 0;
+  // pencil-major spectra with compile-time x and row passes where both
+  // lengths have them in both precisions (VP_PC_CT=0 keeps plane-major)
+  const char* ec = std::getenv("VP_PC_CT");
+  const int lds_max = max_lds_bytes(p->ctx->device);
+  g.pct = (g.pquad && !(ec && ec[0] == '0') && xct_threads(g.L, false) > 0 && xct_threads(g.L, true) > 0 &&
+           pcrow_supported(g.M) && 2 * size_t(g.L) * 16 <= size_t(lds_max))
+              ? 1
+              : 0;
 }
 
 static int prec_level2_common(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double2* dG, double mc_re,

@@ -553,8 +553,12 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   } else {
     dev_norms<T>(b, n, np, n, bn.as<double>(), s);
   }
-  std::vector<double> hbn(np), hbeta(np), hest(np);
-  std::vector<int> hconv(np);
+  std::vector<double> hbn(np), hbeta(np);
+  // per-step estimates and convergence flags come back through pinned host
+  // memory (a DMA per step instead of a staged copy)
+  PinnedBuf pin(np * (sizeof(double) + sizeof(int)));
+  double* hest = static_cast<double*>(pin.p);
+  int* hconv = reinterpret_cast<int*>(hest + np);
   VP_HIP(hipMemcpy(hbn.data(), bn.p, np * sizeof(double), hipMemcpyDeviceToHost));
   // initial residual into V_0 slot of each problem
   auto residual_into_V0 = [&](bool x_nonzero) {
@@ -649,10 +653,10 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       hipLaunchKernelGGL(k_arnoldi, dim3((np + 63) / 64), dim3(64), 0, s, st, k, dia, np, ract.as<int>());
       hipLaunchKernelGGL(k_scale<T>, dim3(grid1(n), np), dim3(256), 0, s, w, vprob, n, invn.as<double>(), 0, dia);
       VP_HIP(hipGetLastError());
-      VP_HIP(hipMemcpyAsync(hest.data(), est.p, np * sizeof(double), hipMemcpyDeviceToHost, s));
-      VP_HIP(hipMemcpyAsync(hconv.data(), conv.p, np * sizeof(int), hipMemcpyDeviceToHost, s));
+      VP_HIP(hipMemcpyAsync(hest, est.p, np * sizeof(double), hipMemcpyDeviceToHost, s));
+      VP_HIP(hipMemcpyAsync(hconv, conv.p, np * sizeof(int), hipMemcpyDeviceToHost, s));
       VP_HIP(hipStreamSynchronize(s));
-      bool still = false;
+      bool still = false, changed = false;
       for (int p = 0; p < np; ++p) {
         if (!inner[p]) continue;
         hk[p] = k + 1;
@@ -660,11 +664,16 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
         res.iter_inner[p] = k + 1;
         last[p] = hbn[p] > 0 ? hest[p] / hbn[p] : 0.0;
         res.resvec[p].push_back(last[p]);
-        if (hconv[p] || k == m - 1) inner[p] = 0;
-        else still = true;
+        if (hconv[p] || k == m - 1) {
+          inner[p] = 0;
+          changed = true;
+        } else {
+          still = true;
+        }
       }
-      VP_HIP(hipMemcpyAsync(iact.p, inner.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
       if (!still) break;
+      // the device copy of the active flags changes only when a problem stops
+      if (changed) VP_HIP(hipMemcpyAsync(iact.p, inner.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
     }
     // solve the least squares and update x for every problem active in this cycle
     VP_HIP(hipMemcpyAsync(kk.p, hk.data(), np * sizeof(int), hipMemcpyHostToDevice, s));

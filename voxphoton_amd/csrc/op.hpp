@@ -34,6 +34,17 @@ struct DevBuf {
   template <typename X> X* as() const { return reinterpret_cast<X*>(p); }
 };
 
+// RAII pinned host buffer (hipHostMalloc).
+struct PinnedBuf {
+  void* p = nullptr;
+  explicit PinnedBuf(size_t b) { VP_HIP(hipHostMalloc(&p, b, hipHostMallocDefault)); }
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+};
+
 // Per-precision device state of an operator.
 template <typename T>
 struct OpPrec {

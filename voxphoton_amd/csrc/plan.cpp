@@ -88,6 +88,16 @@ bool yct_supported(int n) {
 #undef VP_YCT_CASE
 }
 
+bool pcrow_supported(int n) {
+#define VP_PCROW_CASE(NL, ...) case NL:
+  switch (n) {
+    VP_PCROW_LIST(VP_PCROW_CASE)
+    return true;
+    default: return false;
+  }
+#undef VP_PCROW_CASE
+}
+
 int xct_threads(int S, bool fp64) {
 #define VP_XCT_CASE(SL, NTF, FR, NTI, IR) \
   case SL: return NTF;

@@ -210,7 +210,12 @@ __global__ void __launch_bounds__(256) k_pc_quadmul(PcArgs<T> a, PcGeom g) {
   const int L = g.L, M = g.M, N = g.N, n3 = 3 * N, Hy = g.Hy;
   __shared__ C<T> Xs[4][3 * 85];
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
-  const int kxc = blk / Hy, kyc = blk - kxc * Hy;
+  // consecutive workgroups (one XCD) take neighbouring columns: along ky for
+  // plane-major spectra (ky contiguous), along kx for pencil-major (kx
+  // contiguous), so their column gathers share cache lines in L2
+  const int Hx = g.Hx;
+  const int kxc = g.pct ? blk % Hx : blk / Hy;
+  const int kyc = g.pct ? blk / Hx : blk - kxc * Hy;
   int kxs[4], kys[4];
   bool mxs[4], mys[4], use[4];
   const bool twox = kxc != 0 && 2 * kxc != L, twoy = kyc != 0 && 2 * kyc != M;
@@ -226,7 +231,7 @@ __global__ void __launch_bounds__(256) k_pc_quadmul(PcArgs<T> a, PcGeom g) {
     const int c = e / n3, q = e - c * n3;
     C<T> v = czero<T>();
     if (use[c]) {
-      v = work[(int64_t(kxs[c]) * n3 + q) * M + kys[c]];
+      v = work[g.pct ? (int64_t(q) * M + kys[c]) * L + kxs[c] : (int64_t(kxs[c]) * n3 + q) * M + kys[c]];
       const int cq = q / N;
       if ((cq == 0 && mxs[c]) || (cq == 1 && mys[c])) v = cneg(v);
     }
@@ -273,7 +278,7 @@ __global__ void __launch_bounds__(256) k_pc_quadmul(PcArgs<T> a, PcGeom g) {
       if (!use[c]) continue;
       C<T> v = accs[c];
       if ((cr == 0 && mxs[c]) || (cr == 1 && mys[c])) v = cneg(v);
-      work[(int64_t(kxs[c]) * n3 + r) * M + kys[c]] = v;
+      work[g.pct ? (int64_t(r) * M + kys[c]) * L + kxs[c] : (int64_t(kxs[c]) * n3 + r) * M + kys[c]] = v;
     }
   }
 }
@@ -344,6 +349,94 @@ __global__ void k_copy_vec(const C<T>* __restrict__ x, C<T>* __restrict__ y, int
     Y[i] = X[i];
 }
 
+// ---------------------------------------------------------------------------
+// Quadrant form over pencil-major spectra X[c][n][m][kx] (g.pct): the x
+// passes take one pencil (c, n, m) per workgroup with a compile-time
+// length-L FFT contiguous in memory; the row passes transform NB
+// consecutive kx of one (c, n) along m in place, read and written
+// b-fastest; k_pc_quadmul addresses the same layout.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ int64_t pc_unknown(const PcArgs<T>& a, const PcGeom& g, int64_t vox, int c) {
+  if (g.structured) {
+    const int p = a.pos[vox];
+    return p < 0 ? -1 : p + int64_t(c) * g.nS;
+  }
+  return a.pos[int64_t(c) * g.V + vox];
+}
+
+template <typename T, int NT, int... Rs>
+__global__ void __launch_bounds__(NT) k_pc_xfwd_ct(PcArgs<T> a, PcGeom g) {
+  constexpr int L = (Rs * ...);
+  if (a.active && !a.active[blockIdx.y]) return;
+  const C<T>* __restrict__ X = a.x + int64_t(blockIdx.y) * a.xstride;
+  C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
+  C<T>* W1 = W0 + L;
+  const int M = g.M, N = g.N;
+  const int p = xcd_remap(blockIdx.x, gridDim.x);   // pencil (c, n, m), m fastest
+  const int cn = p / M, m = p - cn * M;
+  const int c = cn / N, n = cn - c * N;
+  auto ld = [&](int, int l) {
+    const int64_t j = pc_unknown(a, g, l + int64_t(L) * (m + int64_t(M) * n), c);
+    return j >= 0 ? X[pc_outer(a, j)] : czero<T>();
+  };
+  const BufRes Wo = buf_res(work + int64_t(p) * L, uint32_t(L) * uint32_t(sizeof(C<T>)));
+  auto st = [&](int, int k, C<T> v) { buf_store(Wo, uint32_t(k), v); };
+  ct_fft<T, false, L, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, W1, a.twx, 1, ld, st);
+}
+
+template <typename T, int NT, int... Rs>
+__global__ void __launch_bounds__(NT) k_pc_xinv_ct(PcArgs<T> a, PcGeom g) {
+  constexpr int L = (Rs * ...);
+  if (a.active && !a.active[blockIdx.y]) return;
+  C<T>* __restrict__ Y = a.y + int64_t(blockIdx.y) * a.ystride;
+  const C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
+  C<T>* W1 = W0 + L;
+  const int M = g.M, N = g.N;
+  const int p = xcd_remap(blockIdx.x, gridDim.x);
+  const int cn = p / M, m = p - cn * M;
+  const int c = cn / N, n = cn - c * N;
+  const BufRes Wi = buf_res(work + int64_t(p) * L, uint32_t(L) * uint32_t(sizeof(C<T>)));
+  auto ld = [&](int, int k) { return buf_load<T>(Wi, uint32_t(k)); };
+  auto none = [](int, int, C<T>) {};
+  const C<T>* R = ct_fft<T, true, L, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, W1, a.twx, 1, ld, none);
+  for (int l = threadIdx.x; l < L; l += NT) {
+    const int64_t j = pc_unknown(a, g, l + int64_t(L) * (m + int64_t(M) * n), c);
+    if (j >= 0) Y[pc_outer(a, j)] = R[l];
+  }
+}
+
+template <typename T, bool INV, int NB, int... Rs>
+__global__ void __launch_bounds__(256) k_pc_rows_ct(PcArgs<T> a, PcGeom g) {
+  constexpr int MM = (Rs * ...);
+  constexpr int LD = NB + 1;
+  if (a.active && !a.active[blockIdx.y]) return;
+  C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
+  C<T>* W1 = W0 + MM * LD;
+  C<T>* tw = W1 + MM * LD;
+  for (int e = threadIdx.x; e < MM; e += 256) tw[e] = a.twy[e];
+  const int L = g.L;
+  const int nkb = (L + NB - 1) / NB;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int cn = blk / nkb, kb = blk - cn * nkb;
+  const int kx0 = kb * NB;
+  const int nb = min(NB, L - kx0);
+  // (m, kx0 + b) at m*L + b from the row block's origin
+  const BufRes X = buf_res(work + int64_t(cn) * MM * L + kx0, uint32_t((MM - 1) * L + nb) * uint32_t(sizeof(C<T>)));
+  __syncthreads();
+  auto ld = [&](int b, int m) { return b < nb ? buf_load<T>(X, uint32_t(m * L + b)) : czero<T>(); };
+  auto st = [&](int b, int m, C<T> v) {
+    if (b < nb) buf_store(X, uint32_t(m * L + b), v);
+  };
+  ct_fft<T, INV, MM, NB, LD, 256, 1, true, true, true, false, Rs...>(W0, W1, tw, 1, ld, st);
+}
+
 // max over entries of max(|re|, |im|) (non-negative floats order as their bits)
 __global__ void k_absmax_c64(const float2* __restrict__ a, int64_t n, unsigned* __restrict__ out) {
   float m = 0.f;
@@ -385,6 +478,77 @@ void make_half_blocks(PrecPrec<float>& P, hipStream_t s) {
   VP_HIP(hipStreamSynchronize(s));
 }
 
+// Quadrant form over pencil-major spectra (g.pct).
+template <typename T>
+static void launch_prec_pct(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) {
+  constexpr bool F64 = sizeof(T) == 8;
+  const unsigned npen = unsigned(3 * g.N * g.M);
+  const size_t ldsx = 2 * size_t(g.L) * sizeof(C<T>);
+  auto xpass = [&](bool inv) {
+    bool done = false;
+#define VP_PCX(SL, NTF, FR, NTI, IR)                                                                      \
+  if (!done && g.L == SL) {                                                                             \
+    if (inv) {                                                                                          \
+      const void* fn = reinterpret_cast<const void*>(&k_pc_xinv_ct<T, NTI, VP_UNPAREN IR>);             \
+      (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsx));             \
+      hipLaunchKernelGGL((k_pc_xinv_ct<T, NTI, VP_UNPAREN IR>), dim3(npen, nprob), dim3(NTI), ldsx, s, a, g); \
+    } else {                                                                                            \
+      const void* fn = reinterpret_cast<const void*>(&k_pc_xfwd_ct<T, NTF, VP_UNPAREN FR>);             \
+      (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsx));             \
+      hipLaunchKernelGGL((k_pc_xfwd_ct<T, NTF, VP_UNPAREN FR>), dim3(npen, nprob), dim3(NTF), ldsx, s, a, g); \
+    }                                                                                                   \
+    done = true;                                                                                        \
+  }
+    if constexpr (F64) {
+      VP_XCT64_LIST(VP_PCX)
+    } else {
+      VP_XCT32_LIST(VP_PCX)
+    }
+#undef VP_PCX
+    if (!done) throw VpError{ERR_UNSUPPORTED, "preconditioner: no compile-time x pass for this length"};
+  };
+  constexpr int NB = 16;
+  const unsigned nrowblk = unsigned(3 * g.N * ((g.L + NB - 1) / NB));
+  auto rows = [&](bool inv) {
+    bool done = false;
+#define VP_PCR(NL, ...)                                                                                 \
+  if (!done && g.M == NL) {                                                                             \
+    const size_t lds = (2 * size_t(NL) * (NB + 1) + NL) * sizeof(C<T>);                                 \
+    if (inv)                                                                                            \
+      hipLaunchKernelGGL((k_pc_rows_ct<T, true, NB, __VA_ARGS__>), dim3(nrowblk, nprob), dim3(256), lds, s, a, g); \
+    else                                                                                                \
+      hipLaunchKernelGGL((k_pc_rows_ct<T, false, NB, __VA_ARGS__>), dim3(nrowblk, nprob), dim3(256), lds, s, a, g); \
+    done = true;                                                                                        \
+  }
+    VP_PCROW_LIST(VP_PCR)
+#undef VP_PCR
+    if (!done) throw VpError{ERR_UNSUPPORTED, "preconditioner: no compile-time row pass for this length"};
+  };
+  const dim3 gq(unsigned(int64_t(g.Hx) * g.Hy), nprob);
+  const int tq = std::min(256, ((3 * g.N + 63) / 64) * 64);
+  {
+    ProfScope ps(s, KID_PC_XFWD);
+    xpass(false);
+  }
+  {
+    ProfScope ps(s, KID_PC_YROWS);
+    rows(false);
+  }
+  {
+    ProfScope ps(s, KID_PC_BLOCK);
+    hipLaunchKernelGGL(k_pc_quadmul<T>, gq, dim3(tq), 0, s, a, g);
+  }
+  {
+    ProfScope ps(s, KID_PC_YROWS);
+    rows(true);
+  }
+  {
+    ProfScope ps(s, KID_PC_XINV);
+    xpass(true);
+  }
+  VP_HIP(hipGetLastError());
+}
+
 template <typename T>
 void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) {
   if (a.sub && g.n_outer > 0) {
@@ -401,6 +565,10 @@ void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) 
   const void* f3 = reinterpret_cast<const void*>(&k_pc_xinv<T>);
   (void)hipFuncSetAttribute(f1, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds1));
   (void)hipFuncSetAttribute(f3, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds1));
+  if (g.level == 2 && g.pquad && g.pct) {
+    launch_prec_pct<T>(g, a, nprob, s);
+    return;
+  }
   {
     ProfScope ps(s, KID_PC_XFWD);
     hipLaunchKernelGGL(k_pc_xfwd<T>, gx, dim3(g.tx), lds1, s, a, g);

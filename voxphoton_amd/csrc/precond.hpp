@@ -22,6 +22,9 @@ struct PcGeom {
   // quadrant blocks (bsym) with 3N <= 255: y-row DFT / one workgroup per
   // stored block applying it to its four mirrored columns / inverse y-row DFT
   int pquad;
+  // quadrant form over pencil-major spectra X[c][n][m][kx] with compile-time
+  // x passes (length L) and row passes (length M); 0: plane-major X[kx][c][n][m]
+  int pct;
   size_t lds_row, lds_tile;
   size_t lds1, lds2;
   FftPlan px, py;       // unpadded DFTs of length L and M

@@ -178,6 +178,12 @@ bool yct_supported(int n);
   X(1200, 512, (8, 2, 3, 5, 5), 512, (8, 2, 3, 5, 5)) X(900, 256, (4, 9, 5, 5), 256, (4, 9, 5, 5))         \
   X(450, 256, (2, 9, 5, 5), 256, (2, 9, 5, 5)) X(400, 256, (4, 4, 5, 5), 256, (4, 4, 5, 5))               \
   X(360, 256, (8, 9, 5), 256, (8, 9, 5)) X(200, 256, (8, 5, 5), 256, (8, 5, 5)) X(100, 256, (4, 5, 5), 256, (4, 5, 5))
+// Preconditioner row lengths (unpadded circulant axis) with compile-time
+// row passes, and their radix lists.
+#define VP_PCROW_LIST(X)                                                                                \
+  X(11, 11) X(22, 2, 11) X(10, 2, 5) X(12, 3, 4) X(20, 4, 5) X(24, 3, 8) X(44, 4, 11) X(48, 3, 4, 4)       \
+  X(16, 4, 4) X(32, 4, 8)
+bool pcrow_supported(int n);
 // Threads of the compile-time x pass for class length S (0: none).
 int xct_threads(int S, bool fp64);
 int max_lds_bytes(int device);
