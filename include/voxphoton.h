@@ -312,6 +312,25 @@ int vp_field_dgf_grid(vp_ctx ctx, const void* J, const void* Icoord, int64_t Ni,
  */
 int vp_field_total(vp_op op, const void* J, const void* Einc, double freq, void* E, int mem);
 
+/*
+ * Wavelength sweep by model order reduction (Bragg_MOR_script.m:1-143 and
+ * example_Bragg_1500nm_to_1600nm.m:85-114).
+ *
+ * vp_op_create_lincomb: operator whose tensor is sum_i w[i] fN_i (w: k
+ * interleaved complex128 weights, host), with the geometry, contrast and
+ * plan of ops[0]; every source must share them (VP_ERR_INVALID otherwise).
+ * With Lagrange weights this is lagrange_int(fN, S, s) (lagrange_int.m:1-13)
+ * or lagrange_int_deriv (lagrange_int_deriv.m:1-20) applied to the tensors.
+ *
+ * vp_vec_dots: out (k x m complex128, column-major, host) = X^H Y for k and m
+ * device complex128 vectors of length n (the projections P'*A*P and P'*b).
+ * vp_vec_lincomb: Y_j = sum_i X_i C(i, j) for C (k x m complex128,
+ * column-major, host); Y must not alias X (the lift P*x).
+ */
+int vp_op_create_lincomb(const vp_op* ops, const double* w, int k, vp_op* out);
+int vp_vec_dots(vp_ctx ctx, const void* const* X, int k, const void* const* Y, int m, int64_t n, double* out);
+int vp_vec_lincomb(vp_ctx ctx, const void* const* X, int k, const double* C, int m, int64_t n, void* const* Y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -102,6 +102,9 @@ SIGNATURES = {
     'vp_field_dgf_grid': (c_int, [c_vp, c_vp, c_vp, c_i64, c_vp, c_dbl, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp,
                                   c_int]),
     'vp_field_total': (c_int, [c_vp, c_vp, c_vp, c_dbl, c_vp, c_int]),
+    'vp_op_create_lincomb': (c_int, [c_vp, c_vp, c_int, P(c_vp)]),
+    'vp_vec_dots': (c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_i64, c_vp]),
+    'vp_vec_lincomb': (c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_i64, c_vp]),
 }
 
 missing = []

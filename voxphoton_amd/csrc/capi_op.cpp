@@ -574,6 +574,76 @@ int vp_op_apply_batched(const vp_op* ops, int nvec, int transp, const void* x, v
   VP_API_END
 }
 
+// New operator with the sources' geometry, contrast and plan whose tensor is
+// sum_i w[i] fN_i, combined per precision in the stored layout (the layout is
+// linear in the tensor and every source shares its prescale).
+int vp_op_create_lincomb(const vp_op* ops, const double* w, int k, vp_op* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(ops && w && out && k > 0 && k <= 64, ERR_INVALID, "bad lincomb arguments");
+  const vp_op_s* o0 = ops[0];
+  VP_REQUIRE(o0, ERR_INVALID, "ops[0] is NULL");
+  for (int i = 1; i < k; ++i) {
+    const vp_op_s* o = ops[i];
+    VP_REQUIRE(o && o->ctx == o0->ctx && o->L == o0->L && o->M == o0->M && o->N == o0->N && o->Lp == o0->Lp &&
+                   o->Mp == o0->Mp && o->Np == o0->Np && o->n == o0->n && o->gram == o0->gram &&
+                   o->scale32 == o0->scale32 && o->fsym == o0->fsym && o->p64.present == o0->p64.present &&
+                   o->p32.present == o0->p32.present && o->p64.fN.bytes == o0->p64.fN.bytes &&
+                   o->p32.fN.bytes == o0->p32.fN.bytes && o->p64.plan.variant == o0->p64.plan.variant &&
+                   o->p32.plan.variant == o0->p32.plan.variant,
+               ERR_INVALID, "combined operators must share geometry, scaling and plan");
+  }
+  VP_HIP(hipSetDevice(o0->ctx->device));
+  hipStream_t s = o0->ctx->stream;
+  auto* op = new vp_op_s();
+  try {
+    op->ctx = o0->ctx;
+    op->L = o0->L; op->M = o0->M; op->N = o0->N;
+    op->Lp = o0->Lp; op->Mp = o0->Mp; op->Np = o0->Np;
+    op->l0 = o0->l0; op->m0 = o0->m0; op->n0 = o0->n0;
+    op->Lb = o0->Lb; op->Mb = o0->Mb; op->Nb = o0->Nb;
+    op->V = o0->V; op->nS = o0->nS; op->structured = o0->structured; op->n = o0->n;
+    op->gram = o0->gram; op->scale32 = o0->scale32; op->has_mr = o0->has_mr; op->fsym = o0->fsym;
+    op->unit_contrast = o0->unit_contrast; op->all_voxels = o0->all_voxels;
+    auto dup = [&](DevBuf& dst, const DevBuf& src) {
+      dst.alloc(src.bytes);
+      if (src.bytes) VP_HIP(hipMemcpyAsync(dst.p, src.p, src.bytes, hipMemcpyDeviceToDevice, s));
+    };
+    dup(op->pos, o0->pos);
+    std::vector<double2> wv(k);
+    for (int i = 0; i < k; ++i) wv[i] = make_double2(w[2 * i], w[2 * i + 1]);
+    auto one = [&](auto tag) {
+      using T = decltype(tag);
+      const OpPrec<T>& S = prec_of<T>(const_cast<vp_op_s*>(o0));
+      if (!S.present) return;
+      OpPrec<T>& P = prec_of<T>(op);
+      P.present = true;
+      P.g = S.g;
+      P.plan = S.plan;
+      P.gram = S.gram;
+      dup(P.mc, S.mc);
+      dup(P.mr, S.mr);
+      dup(P.tw, S.tw);
+      P.work.alloc(S.work.bytes);
+      P.work2.alloc(S.work2.bytes);
+      P.xs.alloc(S.xs.bytes);
+      P.ys.alloc(S.ys.bytes);
+      P.fN.alloc(S.fN.bytes);
+      std::vector<const C<T>*> src(k);
+      for (int i = 0; i < k; ++i) src[i] = prec_of<T>(const_cast<vp_op_s*>(ops[i])).fN.template as<C<T>>();
+      tensor_lincomb<T>(P.fN.template as<C<T>>(), src.data(), wv.data(), k, int64_t(S.fN.bytes / sizeof(C<T>)), s);
+      upload_probs(op, P);
+    };
+    one(double{});
+    one(float{});
+    VP_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    delete op;
+    throw;
+  }
+  *out = op;
+  VP_API_END
+}
+
 int vp_op_fp32_scale(vp_op op, double* scale) {
   VP_API_BEGIN
   VP_REQUIRE(op && scale, ERR_INVALID, "op/scale is NULL");

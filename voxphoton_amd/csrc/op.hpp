@@ -115,6 +115,10 @@ void op_apply_dev(vp_op_s* op, int transp, const C<T>* x, C<T>* y);
 // idx (1-based, length n) -> voxel map; structured = [idxS; V+idxS; 2V+idxS] form
 void index_map(int64_t V, const int64_t* idx, int64_t n, std::vector<int>& pos, int& structured, int64_t& nS);
 
+// out = sum_i w[i] in[i] over n entries (tensor layouts of one geometry; mor.hip)
+template <typename T>
+void tensor_lincomb(C<T>* out, const C<T>* const* in, const double2* w, int k, int64_t n, hipStream_t s);
+
 // Generic device helpers (setup.hip)
 void convert_c128_to_c64(const double2* in, float2* out, int64_t n, double scale, hipStream_t s);
 void convert_c64_to_c128(const float2* in, double2* out, int64_t n, double scale, hipStream_t s);
