@@ -322,14 +322,34 @@ int vp_field_total(vp_op op, const void* J, const void* Einc, double freq, void*
  * With Lagrange weights this is lagrange_int(fN, S, s) (lagrange_int.m:1-13)
  * or lagrange_int_deriv (lagrange_int_deriv.m:1-20) applied to the tensors.
  *
+ * vp_op_lincomb_into: the same combination written over the tensor of an
+ * existing operator out (not one of ops; same geometry), allocating nothing:
+ * fN_temp = lagrange_int(fN, S, s) of example_Bragg_1500nm_to_1600nm.m:104
+ * once per swept wavelength.
+ *
  * vp_vec_dots: out (k x m complex128, column-major, host) = X^H Y for k and m
  * device complex128 vectors of length n (the projections P'*A*P and P'*b).
  * vp_vec_lincomb: Y_j = sum_i X_i C(i, j) for C (k x m complex128,
  * column-major, host); Y must not alias X (the lift P*x).
+ * vp_vec_absmax: out[i] = max_r |X_i[r]| (host doubles) for k device vectors:
+ * the per-component max(max(max(abs(Einc)))) normalising the incident field
+ * (example:90-91).
+ * vp_vec_gather: Y[i] = alpha w[i] X[idx[i]] for i < n, idx 0-based device
+ * int64 into X (length nx), w nullable (device complex128, 1 when NULL):
+ * Vrhs = Gram tau3 Einc(idxS3) (example:93-95).  vp_vec_scatter: Y (length
+ * ny) = 0, then Y[idx[i]] = X[i] (distinct indices): Je(idxS3) = x
+ * (example:101-102).  An index out of range is VP_ERR_INVALID.
+ * All vector entry points run on ctx's stream and return after it drains;
+ * they share a context scratch buffer, so calls on one context serialise.
  */
 int vp_op_create_lincomb(const vp_op* ops, const double* w, int k, vp_op* out);
+int vp_op_lincomb_into(vp_op out, const vp_op* ops, const double* w, int k);
 int vp_vec_dots(vp_ctx ctx, const void* const* X, int k, const void* const* Y, int m, int64_t n, double* out);
 int vp_vec_lincomb(vp_ctx ctx, const void* const* X, int k, const double* C, int m, int64_t n, void* const* Y);
+int vp_vec_absmax(vp_ctx ctx, const void* const* X, int k, int64_t n, double* out);
+int vp_vec_gather(vp_ctx ctx, const void* X, int64_t nx, const int64_t* idx, const void* w, double alpha_re,
+                  double alpha_im, int64_t n, void* Y);
+int vp_vec_scatter(vp_ctx ctx, const void* X, const int64_t* idx, int64_t n, void* Y, int64_t ny);
 
 #ifdef __cplusplus
 }

@@ -72,3 +72,30 @@ def mor_project(A, U):
 def mor_solve(Q, s, S, U, b):
     """x = U (lagrange_int(Q, S, s) \\ U' b) (example:97-100)."""
     return U @ np.linalg.solve(lagrange_int(Q, S, s), U.conj().T @ b)
+
+
+def spectrum_point(Q, s, U, S, fN, Mc, idx, Gram, origin, dx, Eo, src, cutoff, x_thru, m_edge, n_ext=1.0,
+                   lam_scale=1.0):
+    """One wavelength of the reduced sweep (example_Bragg_1500nm_to_1600nm.m:85-114).
+
+    S: wavelength in the units of s (lam_scale metres each); fN: the sample
+    tensors (Lagrange-interpolated to S for the total field); Mc: L x M x N
+    contrast; idx: 1-based unknowns; the incident field is the point current
+    Eo at the complex point src observed at the voxel centres, each
+    component divided by its largest modulus (max(max(max(abs(.)))) leaves
+    one maximum per component).  Returns (power through x = x_thru,
+    reduced solution x, total field)."""
+    from .fields import E_field_DGF_mollified, E_field_Nop_const, grid_coords, power_through
+    from .vie import C0, em_constants, jvie2_rhs
+    L, M, N = Mc.shape
+    freq = C0 * n_ext / (S * lam_scale)
+    _, ko, _ = em_constants(freq)
+    E = E_field_DGF_mollified(np.asarray(Eo, complex), np.atleast_2d(src), grid_coords(origin, dx, L, M, N), ko,
+                              cutoff).reshape(L, M, N, 3, order='F')
+    Einc = E / np.abs(E).max(axis=(0, 1, 2))
+    b = jvie2_rhs(Einc, Mc, Gram, freq, idx)
+    x = mor_solve(Q, s, S, U, b)
+    J = np.zeros(3 * L * M * N, dtype=complex)
+    J[np.asarray(idx, dtype=np.int64) - 1] = x
+    Eout = E_field_Nop_const(J.reshape(L, M, N, 3, order='F'), lagrange_int(fN, S, s), Gram, freq, Einc)
+    return power_through(Eout, x_thru, m_edge), x, Eout

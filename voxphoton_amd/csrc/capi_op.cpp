@@ -20,6 +20,21 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 using namespace vp;
 
 
+namespace vp {
+void* ctx_scratch(vp_ctx_s* ctx, size_t bytes) {
+  if (bytes > ctx->scratch_bytes) {
+    if (ctx->scratch) VP_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    const size_t b = std::max<size_t>(bytes, size_t(1) << 20);
+    VP_HIP(hipMalloc(&ctx->scratch, b));
+    ctx->scratch_bytes = b;
+  }
+  return ctx->scratch;
+}
+}  // namespace vp
+
 extern "C" {
 
 const char* vp_last_error(void) { return g_last_error.c_str(); }
@@ -58,6 +73,7 @@ int vp_ctx_destroy(vp_ctx ctx) {
   VP_HIP(hipSetDevice(ctx->device));
   (void)hipStreamSynchronize(ctx->stream);
   (void)hipStreamDestroy(ctx->stream);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
   delete ctx;
   VP_API_END
 }
@@ -419,6 +435,36 @@ template void op_apply_dev<float>(vp_op_s*, int, const float2*, float2*);
 
 }  // namespace vp
 
+namespace {
+// every operator of a combination shares the geometry, scaling, precisions
+// and plan of ops[0], so that their stored tensors combine entry by entry
+void require_combinable(const vp_op_s* o0, const vp_op_s* o) {
+  VP_REQUIRE(o && o->ctx == o0->ctx && o->L == o0->L && o->M == o0->M && o->N == o0->N && o->Lp == o0->Lp &&
+                 o->Mp == o0->Mp && o->Np == o0->Np && o->n == o0->n && o->gram == o0->gram &&
+                 o->scale32 == o0->scale32 && o->fsym == o0->fsym && o->p64.present == o0->p64.present &&
+                 o->p32.present == o0->p32.present && o->p64.fN.bytes == o0->p64.fN.bytes &&
+                 o->p32.fN.bytes == o0->p32.fN.bytes && o->p64.plan.variant == o0->p64.plan.variant &&
+                 o->p32.plan.variant == o0->p32.plan.variant,
+             ERR_INVALID, "combined operators must share geometry, scaling and plan");
+}
+
+// out.fN = sum_i w[i] ops[i].fN per present precision (stream-ordered)
+void combine_tensors(vp_op_s* out, const vp_op* ops, const double* w, int k, hipStream_t s) {
+  std::vector<double2> wv(k);
+  for (int i = 0; i < k; ++i) wv[i] = make_double2(w[2 * i], w[2 * i + 1]);
+  auto one = [&](auto tag) {
+    using T = decltype(tag);
+    OpPrec<T>& P = prec_of<T>(out);
+    if (!P.present) return;
+    std::vector<const C<T>*> src(k);
+    for (int i = 0; i < k; ++i) src[i] = prec_of<T>(const_cast<vp_op_s*>(ops[i])).fN.template as<C<T>>();
+    tensor_lincomb<T>(P.fN.template as<C<T>>(), src.data(), wv.data(), k, int64_t(P.fN.bytes / sizeof(C<T>)), s);
+  };
+  one(double{});
+  one(float{});
+}
+}  // namespace
+
 extern "C" {
 
 int vp_op_create(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_t Mp, int64_t Np, const double* fN,
@@ -582,16 +628,7 @@ int vp_op_create_lincomb(const vp_op* ops, const double* w, int k, vp_op* out) {
   VP_REQUIRE(ops && w && out && k > 0 && k <= 64, ERR_INVALID, "bad lincomb arguments");
   const vp_op_s* o0 = ops[0];
   VP_REQUIRE(o0, ERR_INVALID, "ops[0] is NULL");
-  for (int i = 1; i < k; ++i) {
-    const vp_op_s* o = ops[i];
-    VP_REQUIRE(o && o->ctx == o0->ctx && o->L == o0->L && o->M == o0->M && o->N == o0->N && o->Lp == o0->Lp &&
-                   o->Mp == o0->Mp && o->Np == o0->Np && o->n == o0->n && o->gram == o0->gram &&
-                   o->scale32 == o0->scale32 && o->fsym == o0->fsym && o->p64.present == o0->p64.present &&
-                   o->p32.present == o0->p32.present && o->p64.fN.bytes == o0->p64.fN.bytes &&
-                   o->p32.fN.bytes == o0->p32.fN.bytes && o->p64.plan.variant == o0->p64.plan.variant &&
-                   o->p32.plan.variant == o0->p32.plan.variant,
-               ERR_INVALID, "combined operators must share geometry, scaling and plan");
-  }
+  for (int i = 1; i < k; ++i) require_combinable(o0, ops[i]);
   VP_HIP(hipSetDevice(o0->ctx->device));
   hipStream_t s = o0->ctx->stream;
   auto* op = new vp_op_s();
@@ -609,8 +646,6 @@ int vp_op_create_lincomb(const vp_op* ops, const double* w, int k, vp_op* out) {
       if (src.bytes) VP_HIP(hipMemcpyAsync(dst.p, src.p, src.bytes, hipMemcpyDeviceToDevice, s));
     };
     dup(op->pos, o0->pos);
-    std::vector<double2> wv(k);
-    for (int i = 0; i < k; ++i) wv[i] = make_double2(w[2 * i], w[2 * i + 1]);
     auto one = [&](auto tag) {
       using T = decltype(tag);
       const OpPrec<T>& S = prec_of<T>(const_cast<vp_op_s*>(o0));
@@ -628,19 +663,32 @@ int vp_op_create_lincomb(const vp_op* ops, const double* w, int k, vp_op* out) {
       P.xs.alloc(S.xs.bytes);
       P.ys.alloc(S.ys.bytes);
       P.fN.alloc(S.fN.bytes);
-      std::vector<const C<T>*> src(k);
-      for (int i = 0; i < k; ++i) src[i] = prec_of<T>(const_cast<vp_op_s*>(ops[i])).fN.template as<C<T>>();
-      tensor_lincomb<T>(P.fN.template as<C<T>>(), src.data(), wv.data(), k, int64_t(S.fN.bytes / sizeof(C<T>)), s);
       upload_probs(op, P);
     };
     one(double{});
     one(float{});
+    combine_tensors(op, ops, w, k, s);
     VP_HIP(hipStreamSynchronize(s));
   } catch (...) {
     delete op;
     throw;
   }
   *out = op;
+  VP_API_END
+}
+
+// Same combination written over the tensor of an existing operator: no
+// allocation, so a sweep re-interpolates one operator per wavelength.
+int vp_op_lincomb_into(vp_op out, const vp_op* ops, const double* w, int k) {
+  VP_API_BEGIN
+  VP_REQUIRE(out && ops && w && k > 0 && k <= 64, ERR_INVALID, "bad lincomb arguments");
+  for (int i = 0; i < k; ++i) {
+    VP_REQUIRE(ops[i] != out, ERR_INVALID, "the target must not be one of the sources");
+    require_combinable(out, ops[i]);
+  }
+  VP_HIP(hipSetDevice(out->ctx->device));
+  combine_tensors(out, ops, w, k, out->ctx->stream);
+  VP_HIP(hipStreamSynchronize(out->ctx->stream));
   VP_API_END
 }
 

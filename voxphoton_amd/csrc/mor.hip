@@ -3,8 +3,15 @@
 //   - weighted sums of operator tensors (lagrange_int / lagrange_int_deriv
 //     applied to fN) for vp_op_create_lincomb;
 //   - Gram matrices X^H Y and linear combinations X C of device vectors
-//     (the projections P' A P, P' b and the lift P x of the reduced model).
-// FP64 complex vectors; fixed-order reductions (no atomics).
+//     (the projections P' A P, P' b and the lift P x of the reduced model);
+//   - per-vector maxima, gathers and scatters of the per-wavelength
+//     excitation and field (example_Bragg_1500nm_to_1600nm.m:89-106).
+// FP64 complex vectors; sums in a fixed order (no atomics); maxima through
+// atomicMax on the bits of non-negative doubles, which is exact.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
 #include <vector>
 #include "op.hpp"
 #include "capi_common.hpp"
@@ -105,6 +112,88 @@ __global__ void __launch_bounds__(256) k_vlincomb(const double2* const* __restri
   }
 }
 
+// out[i] = bits of max_r |X_i[r]|^2 (non-negative doubles order as their bits)
+__global__ void __launch_bounds__(KT) k_absmax2(const double2* const* __restrict__ X, int64_t n,
+                                               unsigned long long* __restrict__ out) {
+  const double2* x = X[blockIdx.y];
+  double m = 0.0;
+  for (int64_t r = blockIdx.x * int64_t(KT) + threadIdx.x; r < n; r += int64_t(gridDim.x) * KT) {
+    const double2 v = x[r];
+    m = fmax(m, v.x * v.x + v.y * v.y);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  __shared__ double sm[KT / 64];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < KT / 64; ++w) m = fmax(m, sm[w]);
+    atomicMax(out + blockIdx.y, static_cast<unsigned long long>(__double_as_longlong(m)));
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// Y[i] = alpha w[i] X[idx[i]] (w may be NULL); an index outside [0, nx) gives
+// 0 and raises *bad
+__global__ void __launch_bounds__(256) k_gather(const double2* __restrict__ X, int64_t nx,
+                                               const int64_t* __restrict__ idx, const double2* __restrict__ w,
+                                               double2 alpha, int64_t n, double2* __restrict__ Y,
+                                               unsigned int* __restrict__ bad) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = idx[i];
+    double2 v = make_double2(0.0, 0.0);
+    if (r >= 0 && r < nx) {
+      const double2 x = X[r];
+      double2 a = alpha;
+      if (w) {
+        const double2 c = w[i];
+        a = make_double2(alpha.x * c.x - alpha.y * c.y, alpha.x * c.y + alpha.y * c.x);
+      }
+      v = make_double2(a.x * x.x - a.y * x.y, a.x * x.y + a.y * x.x);
+    } else {
+      atomicOr(bad, 1u);
+    }
+    Y[i] = v;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_zero_c128(double2* __restrict__ Y, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    Y[i] = make_double2(0.0, 0.0);
+}
+
+// Y[idx[i]] = X[i] over a zeroed Y; an index outside [0, ny) raises *bad
+__global__ void __launch_bounds__(256) k_scatter(const double2* __restrict__ X, const int64_t* __restrict__ idx,
+                                                int64_t n, double2* __restrict__ Y, int64_t ny,
+                                                unsigned int* __restrict__ bad) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = idx[i];
+    if (r >= 0 && r < ny) Y[r] = X[i];
+    else atomicOr(bad, 1u);
+  }
+}
+
+__global__ void k_fill_u32(unsigned int* __restrict__ p) { *p = 0u; }
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// carves consecutive 256-byte aligned pieces out of the context scratch
+struct Carve {
+  char* base;
+  size_t off = 0;
+  template <typename X>
+  X* take(size_t count) {
+    X* p = reinterpret_cast<X*>(base + off);
+    off += align256(count * sizeof(X));
+    return p;
+  }
+};
+
+unsigned grid_for(int64_t n, unsigned cap) { return unsigned(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, cap))); }
+
 }  // namespace
 
 template <typename T>
@@ -131,17 +220,22 @@ int vp_vec_dots(vp_ctx ctx, const void* const* X, int k, const void* const* Y, i
   for (int j = 0; j < m; ++j) VP_REQUIRE(Y[j], ERR_INVALID, "Y has a NULL vector");
   VP_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  DevBuf dx(sizeof(void*) * k), dy(sizeof(void*) * m);
-  VP_HIP(hipMemcpyAsync(dx.p, X, dx.bytes, hipMemcpyHostToDevice, s));
-  VP_HIP(hipMemcpyAsync(dy.p, Y, dy.bytes, hipMemcpyHostToDevice, s));
   const int nblk = int((n + int64_t(KT) * RPT - 1) / (int64_t(KT) * RPT));
-  DevBuf part(size_t(k) * m * nblk * sizeof(double2)), res(size_t(k) * m * sizeof(double2));
-  hipLaunchKernelGGL(k_vdots, dim3(unsigned(nblk), unsigned(k), unsigned((m + JT - 1) / JT)), dim3(KT), 0, s,
-                     dx.as<const double2*>(), dy.as<const double2*>(), k, m, n, part.as<double2>());
-  hipLaunchKernelGGL(k_vdots_reduce, dim3(unsigned(k * m)), dim3(KT), 0, s, part.as<double2>(), nblk,
-                     res.as<double2>());
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  const size_t need = align256(sizeof(void*) * k) + align256(sizeof(void*) * m) +
+                      align256(size_t(k) * m * nblk * sizeof(double2)) + align256(size_t(k) * m * sizeof(double2));
+  Carve cv{static_cast<char*>(ctx_scratch(ctx, need))};
+  auto* dx = cv.take<const double2*>(k);
+  auto* dy = cv.take<const double2*>(m);
+  auto* part = cv.take<double2>(size_t(k) * m * nblk);
+  auto* res = cv.take<double2>(size_t(k) * m);
+  VP_HIP(hipMemcpyAsync(dx, X, sizeof(void*) * k, hipMemcpyHostToDevice, s));
+  VP_HIP(hipMemcpyAsync(dy, Y, sizeof(void*) * m, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_vdots, dim3(unsigned(nblk), unsigned(k), unsigned((m + JT - 1) / JT)), dim3(KT), 0, s, dx, dy,
+                     k, m, n, part);
+  hipLaunchKernelGGL(k_vdots_reduce, dim3(unsigned(k * m)), dim3(KT), 0, s, part, nblk, res);
   VP_HIP(hipGetLastError());
-  VP_HIP(hipMemcpyAsync(out, res.p, res.bytes, hipMemcpyDeviceToHost, s));
+  VP_HIP(hipMemcpyAsync(out, res, size_t(k) * m * sizeof(double2), hipMemcpyDeviceToHost, s));
   VP_HIP(hipStreamSynchronize(s));
   VP_API_END
 }
@@ -157,15 +251,88 @@ int vp_vec_lincomb(vp_ctx ctx, const void* const* X, int k, const double* Cm, in
   }
   VP_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  DevBuf dx(sizeof(void*) * k), dy(sizeof(void*) * m), dc(size_t(k) * m * sizeof(double2));
-  VP_HIP(hipMemcpyAsync(dx.p, X, dx.bytes, hipMemcpyHostToDevice, s));
-  VP_HIP(hipMemcpyAsync(dy.p, Y, dy.bytes, hipMemcpyHostToDevice, s));
-  VP_HIP(hipMemcpyAsync(dc.p, Cm, dc.bytes, hipMemcpyHostToDevice, s));
-  const unsigned blocks = unsigned(std::min<int64_t>((n + 255) / 256, 4096));
-  hipLaunchKernelGGL(k_vlincomb, dim3(blocks, unsigned(m)), dim3(256), 0, s, dx.as<const double2*>(),
-                     dc.as<double2>(), k, m, n, dy.as<double2*>());
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  const size_t need =
+      align256(sizeof(void*) * k) + align256(sizeof(void*) * m) + align256(size_t(k) * m * sizeof(double2));
+  Carve cv{static_cast<char*>(ctx_scratch(ctx, need))};
+  auto* dx = cv.take<const double2*>(k);
+  auto* dy = cv.take<double2*>(m);
+  auto* dc = cv.take<double2>(size_t(k) * m);
+  VP_HIP(hipMemcpyAsync(dx, X, sizeof(void*) * k, hipMemcpyHostToDevice, s));
+  VP_HIP(hipMemcpyAsync(dy, Y, sizeof(void*) * m, hipMemcpyHostToDevice, s));
+  VP_HIP(hipMemcpyAsync(dc, Cm, size_t(k) * m * sizeof(double2), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_vlincomb, dim3(grid_for(n, 4096), unsigned(m)), dim3(256), 0, s, dx, dc, k, m, n, dy);
   VP_HIP(hipGetLastError());
   VP_HIP(hipStreamSynchronize(s));
+  VP_API_END
+}
+
+int vp_vec_absmax(vp_ctx ctx, const void* const* X, int k, int64_t n, double* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && X && out, ERR_INVALID, "ctx/X/out is NULL");
+  VP_REQUIRE(k > 0 && k <= 4096 && n > 0, ERR_INVALID, "bad vector counts");
+  for (int i = 0; i < k; ++i) VP_REQUIRE(X[i], ERR_INVALID, "X has a NULL vector");
+  VP_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  Carve cv{static_cast<char*>(ctx_scratch(ctx, align256(sizeof(void*) * k) + align256(8 * size_t(k))))};
+  auto* dx = cv.take<const double2*>(k);
+  auto* res = cv.take<unsigned long long>(k);
+  VP_HIP(hipMemcpyAsync(dx, X, sizeof(void*) * k, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_fill_u64, dim3(unsigned((k + 255) / 256)), dim3(256), 0, s, res, k);
+  hipLaunchKernelGGL(k_absmax2, dim3(grid_for(n, 2048), unsigned(k)), dim3(KT), 0, s, dx, n, res);
+  VP_HIP(hipGetLastError());
+  std::vector<unsigned long long> bits(k);
+  VP_HIP(hipMemcpyAsync(bits.data(), res, 8 * size_t(k), hipMemcpyDeviceToHost, s));
+  VP_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < k; ++i) {
+    double m2;
+    std::memcpy(&m2, &bits[i], 8);
+    out[i] = std::sqrt(m2);
+  }
+  VP_API_END
+}
+
+int vp_vec_gather(vp_ctx ctx, const void* X, int64_t nx, const int64_t* idx, const void* w, double alpha_re,
+                  double alpha_im, int64_t n, void* Y) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && X && idx && Y, ERR_INVALID, "ctx/X/idx/Y is NULL");
+  VP_REQUIRE(n > 0 && nx > 0, ERR_INVALID, "bad vector lengths");
+  VP_REQUIRE(Y != X, ERR_INVALID, "Y must not alias X");
+  VP_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  auto* bad = static_cast<unsigned int*>(ctx_scratch(ctx, 256));
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, s, bad);
+  hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 8192)), dim3(256), 0, s, static_cast<const double2*>(X), nx, idx,
+                     static_cast<const double2*>(w), make_double2(alpha_re, alpha_im), n, static_cast<double2*>(Y),
+                     bad);
+  VP_HIP(hipGetLastError());
+  unsigned int hbad = 0;
+  VP_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
+  VP_HIP(hipStreamSynchronize(s));
+  VP_REQUIRE(!hbad, ERR_INVALID, "gather index out of range");
+  VP_API_END
+}
+
+int vp_vec_scatter(vp_ctx ctx, const void* X, const int64_t* idx, int64_t n, void* Y, int64_t ny) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && X && idx && Y, ERR_INVALID, "ctx/X/idx/Y is NULL");
+  VP_REQUIRE(n > 0 && ny > 0, ERR_INVALID, "bad vector lengths");
+  VP_REQUIRE(Y != X, ERR_INVALID, "Y must not alias X");
+  VP_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  auto* bad = static_cast<unsigned int*>(ctx_scratch(ctx, 256));
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(1), 0, s, bad);
+  hipLaunchKernelGGL(k_zero_c128, dim3(grid_for(ny, 8192)), dim3(256), 0, s, static_cast<double2*>(Y), ny);
+  hipLaunchKernelGGL(k_scatter, dim3(grid_for(n, 8192)), dim3(256), 0, s, static_cast<const double2*>(X), idx, n,
+                     static_cast<double2*>(Y), ny, bad);
+  VP_HIP(hipGetLastError());
+  unsigned int hbad = 0;
+  VP_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, s));
+  VP_HIP(hipStreamSynchronize(s));
+  VP_REQUIRE(!hbad, ERR_INVALID, "scatter index out of range");
   VP_API_END
 }
 

@@ -1,6 +1,7 @@
 // Operator / context objects behind the opaque C handles.
 #pragma once
 #include <memory>
+#include <mutex>
 #include <vector>
 #include "voxphoton.h"
 #include "vp_internal.hpp"
@@ -8,6 +9,11 @@
 struct vp_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  // grow-only device scratch of the context's vector entry points (pointer
+  // tables, partial sums, flags); held under mu for a whole call
+  std::mutex mu;
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
 };
 
 namespace vp {
@@ -116,6 +122,9 @@ void op_apply_dev(vp_op_s* op, int transp, const C<T>* x, C<T>* y);
 void index_map(int64_t V, const int64_t* idx, int64_t n, std::vector<int>& pos, int& structured, int64_t& nS);
 
 // out = sum_i w[i] in[i] over n entries (tensor layouts of one geometry; mor.hip)
+// ctx->scratch grown to at least bytes (caller holds ctx->mu)
+void* ctx_scratch(vp_ctx_s* ctx, size_t bytes);
+
 template <typename T>
 void tensor_lincomb(C<T>* out, const C<T>* const* in, const double2* w, int k, int64_t n, hipStream_t s);
 

@@ -44,36 +44,72 @@ def lagrange_deriv_weights(S, s):
     return w
 
 
+def _addr(a):
+    """Device address of a DeviceArray or an int (offset pointers)."""
+    return a.ptr.value if isinstance(a, DeviceArray) else int(a)
+
+
+def _ops(ops):
+    return (ctypes.c_void_p * len(ops))(*[o.handle.value for o in ops])
+
+
 def op_lincomb(ops, w):
     """Operator whose tensor is sum_i w[i] fN_i (vp_op_create_lincomb)."""
-    k = len(ops)
-    hs = (ctypes.c_void_p * k)(*[o.handle.value for o in ops])
     wv = np.ascontiguousarray(np.asarray(w, dtype=np.complex128))
     h = ctypes.c_void_p()
-    check(L.lib.vp_op_create_lincomb(hs, ptr(wv), k, ctypes.byref(h)))
+    check(L.lib.vp_op_create_lincomb(_ops(ops), ptr(wv), len(ops), ctypes.byref(h)))
     return NOperator(h, ops[0].ctx, ops[0].n, (ops[0].ctx,))
+
+
+def op_lincomb_into(out, ops, w):
+    """Overwrite out's tensor with sum_i w[i] fN_i (vp_op_lincomb_into)."""
+    wv = np.ascontiguousarray(np.asarray(w, dtype=np.complex128))
+    check(L.lib.vp_op_lincomb_into(out.handle, _ops(ops), ptr(wv), len(ops)))
 
 
 def vec_dots(ctx, X, Y, n):
     """X^H Y (len(X) x len(Y)) of device complex128 vectors (vp_vec_dots)."""
     k, m = len(X), len(Y)
-    xs = (ctypes.c_void_p * k)(*[a.ptr.value for a in X])
-    ys = (ctypes.c_void_p * m)(*[a.ptr.value for a in Y])
+    xs = (ctypes.c_void_p * k)(*[_addr(a) for a in X])
+    ys = (ctypes.c_void_p * m)(*[_addr(a) for a in Y])
     out = np.empty(k * m, dtype=np.complex128)
     check(L.lib.vp_vec_dots(ctx.handle, xs, k, ys, m, int(n), ptr(out)))
     return out.reshape(k, m, order='F')
 
 
-def vec_lincomb(ctx, X, C, n):
-    """New device vectors Y_j = sum_i X_i C[i, j] (vp_vec_lincomb)."""
+def vec_lincomb(ctx, X, C, n, out=None):
+    """Device vectors Y_j = sum_i X_i C[i, j] (vp_vec_lincomb), written to
+    ``out`` (a list of device vectors) or to new DeviceArrays."""
     C = np.asarray(C, dtype=np.complex128).reshape(len(X), -1)
     k, m = C.shape
-    Y = [DeviceArray(ctx, n * 16) for _ in range(m)]
-    xs = (ctypes.c_void_p * k)(*[a.ptr.value for a in X])
-    ys = (ctypes.c_void_p * m)(*[a.ptr.value for a in Y])
+    Y = out if out is not None else [DeviceArray(ctx, n * 16) for _ in range(m)]
+    xs = (ctypes.c_void_p * k)(*[_addr(a) for a in X])
+    ys = (ctypes.c_void_p * m)(*[_addr(a) for a in Y])
     cf = np.ascontiguousarray(C.reshape(-1, order='F'))
     check(L.lib.vp_vec_lincomb(ctx.handle, xs, k, ptr(cf), m, int(n), ys))
     return Y
+
+
+def vec_absmax(ctx, X, n):
+    """max |X_i| of each device vector (vp_vec_absmax)."""
+    xs = (ctypes.c_void_p * len(X))(*[_addr(a) for a in X])
+    out = np.empty(len(X))
+    check(L.lib.vp_vec_absmax(ctx.handle, xs, len(X), int(n), ptr(out)))
+    return out
+
+
+def vec_gather(ctx, X, nx, idx, n, Y, w=None, alpha=1.0):
+    """Y[i] = alpha w[i] X[idx[i]] (idx: 0-based device int64; vp_vec_gather)."""
+    a = complex(alpha)
+    check(L.lib.vp_vec_gather(ctx.handle, ctypes.c_void_p(_addr(X)), int(nx), ctypes.c_void_p(_addr(idx)),
+                              None if w is None else ctypes.c_void_p(_addr(w)), a.real, a.imag, int(n),
+                              ctypes.c_void_p(_addr(Y))))
+
+
+def vec_scatter(ctx, X, idx, n, Y, ny):
+    """Y = 0; Y[idx[i]] = X[i] (vp_vec_scatter)."""
+    check(L.lib.vp_vec_scatter(ctx.handle, ctypes.c_void_p(_addr(X)), ctypes.c_void_p(_addr(idx)), int(n),
+                               ctypes.c_void_p(_addr(Y)), int(ny)))
 
 
 class ReducedModel:
@@ -173,13 +209,112 @@ class ReducedModel:
         """lagrange_int(Q, S, s): the reduced matrix at S."""
         return sum(w * Q for w, Q in zip(lagrange_weights(S, self.s), self.Q))
 
-    def solve(self, S, b):
+    def solve(self, S, b, out=None):
         """Reduced solution at S for a device or host right-hand side:
-        returns (device x, reduced coefficients)."""
+        returns (device x, reduced coefficients); x goes to ``out`` if given."""
         if not isinstance(b, DeviceArray):
             bd = DeviceArray(self.ctx, self.n * 16)
             bd.upload(np.ascontiguousarray(np.asarray(b, dtype=np.complex128)))
             b = bd
         c = vec_dots(self.ctx, self.U, [b], self.n)[:, 0]
         y = np.linalg.solve(self.reduced(S), c)
-        return vec_lincomb(self.ctx, self.U, y[:, None], self.n)[0], y
+        return vec_lincomb(self.ctx, self.U, y[:, None], self.n, out=None if out is None else [out])[0], y
+
+
+class SpectrumSweep:
+    """Per-wavelength evaluation of a reduced model on the device
+    (example_Bragg_1500nm_to_1600nm.m:85-114).
+
+    At wavelength lam (the units of rm.s): the field of the point current Eo
+    at the complex source point src observed at the voxel centres
+    origin + (l, m, n) dx, each component divided by its largest modulus
+    (max(max(max(abs(Einc)))) is per component); Vrhs = Gram tau3
+    Einc(idxS3) with tau = j w eps0 Mc; the reduced solve and lift; J
+    scattered onto the grid; the total field through the field operator
+    interpolated to lam (fN_temp = lagrange_int(fN, S, s)); and the power
+    through the cross-section x = x_thru (1-based), m_edge <= m < M - m_edge.
+
+    fops[i]: field operators (NOperator.for_fields) at rm.s[i]; Mc: L x M x N
+    contrast; idx: 1-based unknown indices of the reduced model's operators;
+    lam_scale: metres per unit of rm.s; n_ext: background index.
+    """
+
+    def __init__(self, rm, fops, Mc, idx, gram, origin, dx, Eo, src, cutoff, x_thru, m_edge=0, n_ext=1.0,
+                 lam_scale=1.0):
+        from .configs import C0, EPS0
+        self.c0, self.eps0 = C0, EPS0
+        self.rm, self.fops = rm, list(fops)
+        self.ctx = rm.ctx
+        Lg, Mg, Ng = np.shape(Mc)
+        self.shape = (Lg, Mg, Ng)
+        self.V = Lg * Mg * Ng
+        self.n = rm.n
+        self.gram, self.dx, self.cutoff = float(gram), float(dx), float(cutoff)
+        self.origin = np.ascontiguousarray(np.asarray(origin, dtype=float).reshape(3))
+        self.Eo = np.ascontiguousarray(np.asarray(Eo, dtype=np.complex128).reshape(3))
+        self.src = np.ascontiguousarray(np.asarray(src, dtype=np.complex128).reshape(3))
+        self.n_ext, self.lam_scale = float(n_ext), float(lam_scale)
+        i0 = np.ascontiguousarray(np.asarray(idx, dtype=np.int64).reshape(-1) - 1)
+        if i0.size != self.n:
+            raise ValueError('idx does not match the reduced model size')
+        mc3 = np.ascontiguousarray(np.tile(np.asarray(Mc, dtype=np.complex128).reshape(-1, order='F'), 3)[i0])
+        ls = np.arange(m_edge, Mg - m_edge)
+        ns = np.arange(Ng)
+        cs = np.arange(3)
+        l0 = x_thru - 1
+        sl = (l0 + Lg * (ls[:, None, None] + Mg * (ns[None, :, None] + Ng * cs[None, None, :]))).reshape(-1)
+        self.nsl = sl.size
+        dev = lambda a: self._upload(a)      # noqa: E731
+        self.d_idx, self.d_mc3, self.d_sl = dev(i0), dev(mc3), dev(np.ascontiguousarray(sl.astype(np.int64)))
+        nb = 3 * self.V * 16
+        self.d_E0, self.d_Einc, self.d_J, self.d_E = (DeviceArray(self.ctx, nb) for _ in range(4))
+        self.d_b, self.d_x = DeviceArray(self.ctx, self.n * 16), DeviceArray(self.ctx, self.n * 16)
+        self.d_slice = DeviceArray(self.ctx, self.nsl * 16)
+        self.fop = op_lincomb(self.fops, lagrange_weights(rm.s[0], rm.s))
+
+    def _upload(self, a):
+        d = DeviceArray(self.ctx, a.nbytes)
+        d.upload(a)
+        return d
+
+    def _comp(self, d, c):
+        return d.ptr.value + c * self.V * 16
+
+    def incident(self, lam):
+        """Normalised incident field on the device (d_Einc); returns the
+        per-component maxima it was divided by."""
+        ko = 2 * np.pi * self.n_ext / (lam * self.lam_scale)
+        Lg, Mg, Ng = self.shape
+        check(L.lib.vp_field_dgf_grid(self.ctx.handle, ptr(self.Eo), ptr(self.src), 1, ptr(self.origin), self.dx,
+                                      Lg, Mg, Ng, float(ko), self.cutoff, self.d_E0.ptr, L.VP_MEM_DEVICE))
+        amax = vec_absmax(self.ctx, [self._comp(self.d_E0, c) for c in range(3)], self.V)
+        for c in range(3):
+            vec_lincomb(self.ctx, [self._comp(self.d_E0, c)], [[1.0 / amax[c]]], self.V,
+                        out=[self._comp(self.d_Einc, c)])
+        return amax
+
+    def rhs(self, lam):
+        """Vrhs = Gram tau3 Einc(idxS3) on the device (d_b)."""
+        freq = self.c0 * self.n_ext / (lam * self.lam_scale)
+        self.incident(lam)
+        vec_gather(self.ctx, self.d_Einc, 3 * self.V, self.d_idx, self.n, self.d_b, w=self.d_mc3,
+                   alpha=self.gram * 1j * 2 * np.pi * freq * self.eps0)
+        return self.d_b
+
+    def point(self, lam):
+        """Transmitted power at lam from the reduced model; the total field
+        is left in d_E and the solution in d_x."""
+        freq = self.c0 * self.n_ext / (lam * self.lam_scale)
+        self.rhs(lam)
+        self.rm.solve(lam, self.d_b, out=self.d_x)
+        vec_scatter(self.ctx, self.d_x, self.d_idx, self.n, self.d_J, 3 * self.V)
+        op_lincomb_into(self.fop, self.fops, lagrange_weights(lam, self.rm.s))
+        check(L.lib.vp_field_total(self.fop.handle, self.d_J.ptr, self.d_Einc.ptr, float(freq), self.d_E.ptr,
+                                   L.VP_MEM_DEVICE))
+        return self.power()
+
+    def power(self):
+        """Sum of |E|^2 over the cross-section of the field in d_E."""
+        vec_gather(self.ctx, self.d_E, 3 * self.V, self.d_sl, self.nsl, self.d_slice)
+        s = self.d_slice.download(np.empty(self.nsl, dtype=np.complex128))
+        return float(np.sum(s.real ** 2 + s.imag ** 2))
