@@ -182,10 +182,12 @@ def test_spectrum_point_matches_oracle(vp_ctx):
     E = [sum(w * Ai for w, Ai in zip(omor.lagrange_deriv_weights(t, s), A)) for t in s]
     U = omor.mor_basis(A, E, rhs, order=3)
     Q = omor.mor_project(A, U)
-    sw = SpectrumSweep(rm, fops, Mc, idx, Gram, origin, dx, [0, 1, 0], src, 0.5 * lam_t, x_thru=10, m_edge=1,
+    sw = SpectrumSweep(s, fops, Mc, idx, Gram, origin, dx, [0, 1, 0], src, 0.5 * lam_t, x_thru=10, m_edge=1,
                        n_ext=n_ext)
+    # the device right-hand side at a sample is the host one
+    assert rel(sw.rhs_copy(s[1]).download(np.empty(idx.size, complex)), rhs[1]) < 1e-13
     for S in (1.55e-6, 1.521e-6, 1.597e-6):
-        p = sw.point(S)
+        p = sw.point(S, rm)
         po, xo, Eo = omor.spectrum_point(Q, s, U, S, fN, Mc, idx, Gram, origin, dx, [0, 1, 0], src, 0.5 * lam_t,
                                          10, 1, n_ext)
         xd = sw.d_x.download(np.empty(idx.size, complex))

@@ -9,6 +9,7 @@ libvoxphoton.so on device vectors; only the nG*order-sized reduced systems
 are formed on the host, as the reference does in MATLAB.
 """
 import ctypes
+import time
 
 import numpy as np
 
@@ -115,8 +116,9 @@ def vec_scatter(ctx, X, idx, n, Y, ny):
 class ReducedModel:
     """Reduced model of A(s) x = b(s) over sample parameters s (wavelengths).
 
-    ops[i]: the operator at s[i]; precs[i]: its preconditioner (or None);
-    rhs[i]: host right-hand side at s[i].  Per sample the basis holds the
+    ops[i]: the operator at s[i] (no Mr, as MVP_E assumes); precs[i]: its
+    preconditioner (or None); rhs[i]: right-hand side at s[i] (host array
+    or DeviceArray).  Per sample the basis holds the
     solution and order-1 moments A^-1 (-+ E V_n), E = A'(s_i) the operator
     of the Lagrange-differentiated tensor minus Gram I (MVP_E), each
     orthogonalised against the sample's earlier columns by one classical
@@ -137,38 +139,53 @@ class ReducedModel:
         self.solve_kw = dict(tol=tol, restart=restart, inner_tol=inner_tol, inner_maxit=inner_maxit,
                              max_outer=max_outer, side=side)
         self.solve_stats = []
+        self.times = dict(solves=0.0, moments=0.0, orthonormalise=0.0, project=0.0)
         n, ctx = self.n, self.ctx
         gram = self._gram(ops[0])
         cols = []
         for i, (op, P) in enumerate(zip(self.ops, self.precs)):
+            t0 = time.perf_counter()
             E = op_lincomb(self.ops, lagrange_deriv_weights(self.s[i], self.s))
-            b = DeviceArray(ctx, n * 16)
-            b.upload(np.ascontiguousarray(np.asarray(rhs[i], dtype=np.complex128)))
+            if isinstance(rhs[i], DeviceArray):
+                b = rhs[i]
+            else:
+                b = DeviceArray(ctx, n * 16)
+                b.upload(np.ascontiguousarray(np.asarray(rhs[i], dtype=np.complex128)))
+            self.times['moments'] += time.perf_counter() - t0
             V = [self._normalise(self._solve(op, P, b))]
             for nn in range(1, order):
+                t0 = time.perf_counter()
                 y = DeviceArray(ctx, n * 16)
                 E.apply_device(V[-1].ptr.value, y.ptr.value)
                 sg = -1.0 if nn % 2 == 1 else 1.0     # v = -+ (A'(V_n) - Gram V_n)
                 v = vec_lincomb(ctx, [y, V[-1]], [[sg], [-sg * gram]], n)[0]
+                self.times['moments'] += time.perf_counter() - t0
                 w = self._solve(op, P, v)
+                t0 = time.perf_counter()
                 h = vec_dots(ctx, V, [w], n)[:, 0]
                 w = vec_lincomb(ctx, V + [w], np.concatenate([-h, [1.0]])[:, None], n)[0]
                 V.append(self._normalise(w))
+                self.times['moments'] += time.perf_counter() - t0
             cols.extend(V)
             E.close()
             if log:
-                log(f'  MOR sample {i}: {order} columns, solves {self.solve_stats[-order:]}')
+                log(f'  MOR sample {i}: {order} columns, solves (flag, inner its, relres) '
+                    f'{self.solve_stats[-order:]}')
+        t0 = time.perf_counter()
         self.U = self._orthonormalise(cols, drop)
         self.K = len(self.U)
+        del cols
+        self.times['orthonormalise'] = time.perf_counter() - t0
+        t0 = time.perf_counter()
         self.Q = []
+        y = DeviceArray(ctx, n * 16)
         for op in self.ops:
-            AU = []
-            for u in self.U:
-                y = DeviceArray(ctx, n * 16)
+            Qi = np.empty((self.K, self.K), dtype=np.complex128)
+            for j, u in enumerate(self.U):
                 op.apply_device(u.ptr.value, y.ptr.value)
-                AU.append(y)
-            self.Q.append(vec_dots(ctx, self.U, AU, n))
-            del AU
+                Qi[:, j] = vec_dots(ctx, self.U, [y], n)[:, 0]
+            self.Q.append(Qi)
+        self.times['project'] = time.perf_counter() - t0
 
     @staticmethod
     def _gram(op):
@@ -176,6 +193,7 @@ class ReducedModel:
         return op.fp32_scale
 
     def _solve(self, op, P, b):
+        t0 = time.perf_counter()
         x = DeviceArray(self.ctx, self.n * 16)
         kw = self.solve_kw
         info, _ = _info(kw['max_outer'] + 2)
@@ -184,6 +202,7 @@ class ReducedModel:
                                 L.VP_MEM_DEVICE, float(kw['tol']), int(kw['restart']), float(kw['inner_tol']),
                                 int(kw['inner_maxit']), int(kw['max_outer']), ctypes.byref(info)))
         self.solve_stats.append((info.flag, info.total_iters, float(info.relres)))
+        self.times['solves'] += time.perf_counter() - t0
         return x
 
     def _normalise(self, v):
@@ -225,7 +244,7 @@ class SpectrumSweep:
     """Per-wavelength evaluation of a reduced model on the device
     (example_Bragg_1500nm_to_1600nm.m:85-114).
 
-    At wavelength lam (the units of rm.s): the field of the point current Eo
+    At wavelength lam (the units of s): the field of the point current Eo
     at the complex source point src observed at the voxel centres
     origin + (l, m, n) dx, each component divided by its largest modulus
     (max(max(max(abs(Einc)))) is per component); Vrhs = Gram tau3
@@ -234,29 +253,29 @@ class SpectrumSweep:
     interpolated to lam (fN_temp = lagrange_int(fN, S, s)); and the power
     through the cross-section x = x_thru (1-based), m_edge <= m < M - m_edge.
 
-    fops[i]: field operators (NOperator.for_fields) at rm.s[i]; Mc: L x M x N
-    contrast; idx: 1-based unknown indices of the reduced model's operators;
-    lam_scale: metres per unit of rm.s; n_ext: background index.
+    s: sample wavelengths (those of the reduced models evaluated); fops[i]:
+    field operators (NOperator.for_fields) at s[i]; Mc: L x M x N contrast;
+    idx: 1-based unknown indices of the reduced models' operators;
+    lam_scale: metres per unit of s; n_ext: background index.  rhs() forms
+    the right-hand side the reduced models are built from at the samples.
     """
 
-    def __init__(self, rm, fops, Mc, idx, gram, origin, dx, Eo, src, cutoff, x_thru, m_edge=0, n_ext=1.0,
+    def __init__(self, s, fops, Mc, idx, gram, origin, dx, Eo, src, cutoff, x_thru, m_edge=0, n_ext=1.0,
                  lam_scale=1.0):
         from .configs import C0, EPS0
         self.c0, self.eps0 = C0, EPS0
-        self.rm, self.fops = rm, list(fops)
-        self.ctx = rm.ctx
+        self.s, self.fops = np.asarray(s, dtype=float), list(fops)
+        self.ctx = self.fops[0].ctx
         Lg, Mg, Ng = np.shape(Mc)
         self.shape = (Lg, Mg, Ng)
         self.V = Lg * Mg * Ng
-        self.n = rm.n
+        self.n = int(np.size(idx))
         self.gram, self.dx, self.cutoff = float(gram), float(dx), float(cutoff)
         self.origin = np.ascontiguousarray(np.asarray(origin, dtype=float).reshape(3))
         self.Eo = np.ascontiguousarray(np.asarray(Eo, dtype=np.complex128).reshape(3))
         self.src = np.ascontiguousarray(np.asarray(src, dtype=np.complex128).reshape(3))
         self.n_ext, self.lam_scale = float(n_ext), float(lam_scale)
         i0 = np.ascontiguousarray(np.asarray(idx, dtype=np.int64).reshape(-1) - 1)
-        if i0.size != self.n:
-            raise ValueError('idx does not match the reduced model size')
         mc3 = np.ascontiguousarray(np.tile(np.asarray(Mc, dtype=np.complex128).reshape(-1, order='F'), 3)[i0])
         ls = np.arange(m_edge, Mg - m_edge)
         ns = np.arange(Ng)
@@ -270,7 +289,7 @@ class SpectrumSweep:
         self.d_E0, self.d_Einc, self.d_J, self.d_E = (DeviceArray(self.ctx, nb) for _ in range(4))
         self.d_b, self.d_x = DeviceArray(self.ctx, self.n * 16), DeviceArray(self.ctx, self.n * 16)
         self.d_slice = DeviceArray(self.ctx, self.nsl * 16)
-        self.fop = op_lincomb(self.fops, lagrange_weights(rm.s[0], rm.s))
+        self.fop = op_lincomb(self.fops, lagrange_weights(self.s[0], self.s))
 
     def _upload(self, a):
         d = DeviceArray(self.ctx, a.nbytes)
@@ -301,17 +320,27 @@ class SpectrumSweep:
                    alpha=self.gram * 1j * 2 * np.pi * freq * self.eps0)
         return self.d_b
 
-    def point(self, lam):
-        """Transmitted power at lam from the reduced model; the total field
-        is left in d_E and the solution in d_x."""
-        freq = self.c0 * self.n_ext / (lam * self.lam_scale)
+    def rhs_copy(self, lam):
+        """rhs(lam) in a new DeviceArray (the sample right-hand sides)."""
+        return vec_lincomb(self.ctx, [self.rhs(lam)], [[1.0]], self.n)[0]
+
+    def point(self, lam, rm):
+        """Transmitted power at lam from the reduced model rm (built over the
+        same samples); the total field is left in d_E, the solution in d_x."""
+        if rm.n != self.n or not np.array_equal(rm.s, self.s):
+            raise ValueError('reduced model of another problem or other samples')
         self.rhs(lam)
-        self.rm.solve(lam, self.d_b, out=self.d_x)
+        rm.solve(lam, self.d_b, out=self.d_x)
+        self.field(lam)
+        return self.power()
+
+    def field(self, lam):
+        """Total field at lam (d_E) of the current solution d_x."""
+        freq = self.c0 * self.n_ext / (lam * self.lam_scale)
         vec_scatter(self.ctx, self.d_x, self.d_idx, self.n, self.d_J, 3 * self.V)
-        op_lincomb_into(self.fop, self.fops, lagrange_weights(lam, self.rm.s))
+        op_lincomb_into(self.fop, self.fops, lagrange_weights(lam, self.s))
         check(L.lib.vp_field_total(self.fop.handle, self.d_J.ptr, self.d_Einc.ptr, float(freq), self.d_E.ptr,
                                    L.VP_MEM_DEVICE))
-        return self.power()
 
     def power(self):
         """Sum of |E|^2 over the cross-section of the field in d_E."""
