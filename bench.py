@@ -53,6 +53,11 @@ WORKLOADS = {
         tol=1e-4, restart=100, inner_tol=1e-2, inner_maxit=4, max_outer=12, prec='face', axis=1, damping=0.0,
         base='bragg_20nm', nG=5, order=5, lam0=1500e-9, lam1=1600e-9, dlam=0.1e-9, check_tol=1e-6,
         check=(1512.3e-9, 1526.9e-9, 1587.1e-9)),
+    'dr_bragg_20nm': dict(
+        desc='GMRES with deflated restarts (pgmresDR, GCRO-DR) against restarted GMRES (pgmres) with the same '
+             'short restart, FP64, on the 20 nm Bragg grating at 1550 nm (1800x50x22) with the face-averaged '
+             '2-level right preconditioner',
+        tol=1e-6, restart=40, ritz=15, maxit=75, prec='face', axis=1, damping=0.0, base='bragg_20nm'),
     'serve_mix': dict(
         desc='serving stream: seeded Poisson arrivals (mean 50 req/s) of mixed solves at 20 nm voxels, '
              'lambda U[1500,1600] nm: 40% straight waveguides of length U[2,8] um, 40% Bragg gratings of '
@@ -190,6 +195,9 @@ def main():
         return
     if 'nG' in wl:
         run_mor(args, wl, log)
+        return
+    if 'ritz' in wl:
+        run_dr(args, wl, log)
         return
     if 'nreq' in wl:
         run_serve(args, wl, log)
@@ -609,6 +617,45 @@ def run_mor(args, wl, log):
         'speedup_vs_direct_full_solves': round(direct / t_total, 2),
         'normalised_transmission': {'min': float(tn.min()), 'argmin_nm': round(float(lams[np.argmin(tn)]) * 1e9, 2),
                                     'sampled_every_10nm': [round(float(v), 4) for v in tn[::100]]},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_dr(args, wl, log):
+    """pgmresDR against pgmres at the same restart length (FP64, right
+    preconditioner): iterations, time and true residual of both."""
+    from voxphoton_amd import vie, configs
+    from voxphoton_amd import _lib as L
+    from voxphoton_amd.krylov_dr import pgmresDR
+    ctx = vie.default_context()
+    prob = configs.config_problem(wl['base'])
+    op = vie.NOperator.assembled(prob.shape, prob.dx, prob.ko, None, prob.Mc, prob.idx, precisions=('fp64',))
+    P, pdesc = make_preconditioner(prob, wl, vie, configs)
+    b = prob.gaussian_rhs('y')
+    log(f'deflated restarts: grid {prob.shape}, unknowns {prob.n}, restart {wl["restart"]}, ritz {wl["ritz"]}')
+    res = {}
+    for name in ('pgmres', 'pgmresDR'):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        if name == 'pgmres':
+            x, flag, relres, it, rv = vie.pgmres(op, b, wl['restart'], wl['tol'], wl['maxit'], R1=P)
+        else:
+            x, flag, relres, it, rv = pgmresDR(op, b, wl['restart'], wl['tol'], wl['maxit'], wl['ritz'], R1=P)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        res[name] = dict(seconds=round(dt, 3), iterations=int(len(rv) - 1), flag=int(flag), relres=float(relres),
+                         ms_per_iteration=round(dt / max(len(rv) - 1, 1) * 1e3, 3))
+        log(f'  {name}: {len(rv) - 1} iterations, {dt:.2f} s, flag {flag}, relres {relres:.2e}')
+    line = {
+        'metric': 'FP64 solve time with deflated vs plain restarts', 'value': res['pgmresDR']['seconds'],
+        'unit': 's', 'n_gpus': 1, 'steps': 1, 'warmup': 0, 'ms_per_step': res['pgmresDR']['seconds'] * 1e3,
+        'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'c128',
+        'data': 'synthetic (device-assembled Green tensor, generated geometry)',
+        'config': {'workload': args.workload, 'description': wl['desc'], 'unknowns': prob.n,
+                   'restart': wl['restart'], 'ritz': wl['ritz'], 'tol': wl['tol'], 'preconditioner': pdesc,
+                   'parallelism': 'single GPU'},
+        'results': res,
+        'iteration_ratio': round(res['pgmres']['iterations'] / max(res['pgmresDR']['iterations'], 1), 3),
     }
     print(json.dumps(line), flush=True)
 

@@ -296,3 +296,28 @@ def test_batched_ir_alone_equals_in_batch(vp_ctx, monkeypatch, variant):
         print(i, infos[i]['relres'], ia['relres'], rel(X[i], xa), flush=True)
         assert infos[i]['flag'] == 0 and ia['flag'] == 0
         assert rel(X[i], xa) <= 1e-12      # same result alone and in the mix
+
+
+@pytest.mark.parametrize('side', ['none', 'right', 'left'])
+def test_pgmresDR_matches_oracle(vp_ctx, side):
+    # deflated restarts: same residual history and solution as the oracle
+    from voxphoton_amd.vie import NOperator, Preconditioner
+    from voxphoton_amd.krylov_dr import pgmresDR
+    L, M, N = 10, 6, 4
+    G, Mc, Gram, idx, b, dx, ko = _physical(L, M, N, seed=6, frac=0.8)
+    op = NOperator.from_toeplitz(G, None, Mc, Gram, idx, ctx=vp_ctx)
+    P = Preconditioner.level2(G, Mc.flat[0], 1.0, Gram, idx, ctx=vp_ctx)
+    fN = ovie.fN_from_toeplitz(G)
+    ones = np.ones(Mc.shape, complex)
+    Aref = lambda v: ovie.mv_AN_const(v, fN, ones, Mc, Gram, 'notransp', idx)    # noqa: E731
+    Binv = opre.level_2_parallel_func(G, Mc.flat[0], 1.0, L, M, N, Gram)
+    Pref = lambda v: opre.chan_2_mvp_for_parallel_idx(Binv, v, L, M, N, idx)      # noqa: E731
+    kw = dict(R1=P) if side == 'right' else dict(L1=P) if side == 'left' else {}
+    kwo = dict(R1=Pref) if side == 'right' else dict(L1=Pref) if side == 'left' else {}
+    restart, tol, maxit, ritz = 8, 1e-9, 40, 4
+    x, flag, relres, it, rv = pgmresDR(op, b, restart, tol, maxit, ritz, **kw)
+    xo, flago, relreso, ito, rvo = okry.pgmresDR(Aref, b, restart, tol, maxit, ritz, **kwo)
+    print(side, 'device', len(rv), it, relres, 'oracle', len(rvo), ito, relreso, flush=True)
+    assert flag == flago == 0 and it == ito and len(rv) == len(rvo)
+    assert np.allclose(rv, rvo, rtol=1e-6, atol=1e-13)
+    assert rel(x, xo) < 1e-8
