@@ -225,8 +225,13 @@ int vp_gmres(vp_op op, vp_prec M, int side, const void* b, const void* x0, void*
  * need = min(0.5, 0.5 * tol / relres) when need >= inner_tol / 8 (a step
  * that can reach tol goes as deep as it needs instead of leaving a short
  * extra step); stops when ||b - A x|| / ||b|| <= tol or after max_outer
- * steps.  No counterpart
- * in the reference (which solves in FP64 only).
+ * steps.  The inner solves after the first are augmented with the first
+ * one's Krylov space (GCRO: C = (A M^-1) U orthonormal, the residual is
+ * projected on C and Arnoldi runs orthogonal to C), which keeps what the
+ * first solve learnt about the slow modes; VP_IR_RECYCLE=0 restarts every
+ * inner solve afresh.  The recycled space holds up to restart vectors (plus
+ * U) per solve in FP32.  No counterpart in the reference (which solves in
+ * FP64 only).
  */
 int vp_solve_ir(vp_op op, vp_prec M, int side, const void* b, void* x, int mem, double tol, int restart,
                 double inner_tol, int inner_maxit, int max_outer, vp_gmres_info* info);

@@ -304,11 +304,13 @@ struct IrResult {
 // its residual by what this step needs to reach tol (need = tol/relres/2,
 // factor 2 margin for the FP32 residual gap) when that is looser than
 // inner_tol or within a factor 8 below it (one slightly deeper inner solve
-// instead of another refinement step), else by inner_tol.  Problems already
-// done do no work.  Returns true while a problem is still running.
+// instead of another refinement step), else by inner_tol.  With rec, the
+// first inner solve leaves its last cycle's Krylov space as the recycled
+// space of the later ones.  Problems already done do no work.  Returns true
+// while a problem is still running.
 bool ir_step_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2* b, double2* x, double scale32,
                  double tol, int restart, double inner_tol, int inner_maxit, int max_outer, int side,
-                 std::vector<IrProblem>& st) {
+                 std::vector<IrProblem>& st, Recycle* rec) {
   const int np = p64.nprob;
   const int64_t n = p64.n;
   hipStream_t s = p64.stream;
@@ -361,6 +363,11 @@ bool ir_step_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2*
       const double need = std::min(0.5, 0.5 * tol / st[p].relres);
       prm.tol_per[p] = need >= inner_tol / 8 ? need : inner_tol;
     }
+  if (rec) {
+    // the first inner solve leaves its Krylov space as the recycled space of the later ones
+    prm.rec = rec;
+    if (rec->kc == 0) rec->build = true;
+  }
   GmresResult gr;
   gmres_device<float>(p32, r32.as<float2>(), d32.as<float2>(), prm, gr);
   for (int p = 0; p < np; ++p)
@@ -380,7 +387,16 @@ static void solve_ir_sets(ProblemSet<double>& p64, ProblemSet<float>& p32, const
   const int np = p64.nprob;
   VP_HIP(hipMemsetAsync(x, 0, size_t(p64.n) * np * sizeof(double2), p64.stream));
   std::vector<IrProblem> st(np);
-  while (ir_step_set(p64, p32, b, x, scale32, tol, restart, inner_tol, inner_maxit, max_outer, side, st)) {
+  // inner solves after the first run augmented with the first one's Krylov
+  // space (VP_IR_RECYCLE=0: every inner solve starts afresh)
+  const char* er = std::getenv("VP_IR_RECYCLE");
+  Recycle rec;
+  Recycle* prec = nullptr;
+  if (!(er && er[0] == '0')) {
+    rec.alloc(p64.n, np, restart, sizeof(float2));
+    prec = &rec;
+  }
+  while (ir_step_set(p64, p32, b, x, scale32, tol, restart, inner_tol, inner_maxit, max_outer, side, st, prec)) {
   }
   out.flag.resize(np);
   out.steps.resize(np);

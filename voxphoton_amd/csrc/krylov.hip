@@ -10,9 +10,14 @@
 //   k_scale      v_{k+1} = w / ||w||
 // All reductions use fixed-order trees (no atomics), so a problem's result
 // does not depend on which other problems share the launch.
+// An augmented solve (GmresParams::rec) keeps recycled columns C ahead of the
+// basis: the cycle residual is projected on C (a0 = C^H r), every
+// Gram-Schmidt pass runs over [C V] (the C entries form B = C^H Aop v_k), and
+// the update is V y + U (a0 - B y) with Aop U = C (GCRO).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstdlib>
 #include "krylov.hpp"
 
@@ -222,6 +227,9 @@ struct ArnoldiState {
   double* est;      // [p] |g(k+1)|
   int* conv;        // [p] 1 when est < tolabs or breakdown
   int m, maxcols;
+  int kc = 0;               // recycled columns ahead of the basis in h1/h2
+  int kcap = 1;
+  double2* Bst = nullptr;   // [p][m][kcap] C^H Aop v_k (when kc > 0)
 };
 
 __global__ void k_arnoldi(ArnoldiState st, int k, const int* __restrict__ active, int nprob,
@@ -233,7 +241,13 @@ __global__ void k_arnoldi(ArnoldiState st, int k, const int* __restrict__ active
   double2* col = st.H + int64_t(p) * (m + 1) * m + int64_t(k) * (m + 1);
   const double2* h1 = st.h1 + int64_t(p) * st.maxcols;
   const double2* h2 = st.h2 + int64_t(p) * st.maxcols;
-  if (reorth[p])
+  const bool two = reorth[p] != 0;
+  // entries 0..kc-1 are the projections on the recycled columns C
+  for (int i = 0; i < st.kc; ++i)
+    st.Bst[(int64_t(p) * m + k) * st.kcap + i] = two ? make_double2(h1[i].x + h2[i].x, h1[i].y + h2[i].y) : h1[i];
+  h1 += st.kc;
+  h2 += st.kc;
+  if (two)
     for (int i = 0; i <= k; ++i) col[i] = make_double2(h1[i].x + h2[i].x, h1[i].y + h2[i].y);
   else
     for (int i = 0; i <= k; ++i) col[i] = h1[i];
@@ -410,6 +424,82 @@ void dev_norms(const C<T>* x, int64_t n, int nprob, int64_t vstride, double* d_o
 template void dev_norms<float>(const float2*, int64_t, int, int64_t, double*, hipStream_t);
 template void dev_norms<double>(const double2*, int64_t, int, int64_t, double*, hipStream_t);
 
+// coef[p][i] = a0[p][i] - sum_{j < kk[p]} B[p][j][i] y[p][j], i < kc: the
+// recycled part of a cycle's update (z = V y + U coef)
+__global__ void k_aug_coef(const double2* __restrict__ a0, const double2* __restrict__ Bst, int kcap, int m,
+                           const double2* __restrict__ y, const int* __restrict__ kk, int maxcols, int kc,
+                           double2* __restrict__ coef, int nprob, const int* __restrict__ active) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob || (active && !active[p])) return;
+  const int K = kk[p];
+  const double2* yp = y + int64_t(p) * maxcols;
+  for (int i = 0; i < kc; ++i) {
+    double2 s = a0[int64_t(p) * maxcols + i];
+    for (int j = 0; j < K; ++j) {
+      const double2 b = Bst[(int64_t(p) * m + j) * kcap + i];
+      s.x -= b.x * yp[j].x - b.y * yp[j].y;
+      s.y -= b.x * yp[j].y + b.y * yp[j].x;
+    }
+    coef[int64_t(p) * maxcols + i] = s;
+  }
+}
+
+constexpr int JO = 8;   // output columns per tall-skinny product block
+
+// out[p][:, j] = sum_{i < kin[p]} in[p][:, i] S[p][i + ls j] for j < kout
+// (S FP64 with at least ceil(kout / JO) * JO columns, zero past kout).  Rows
+// past n read row n-1 and are not stored.
+template <typename T>
+__global__ void __launch_bounds__(KT) k_tsgemm(const C<T>* __restrict__ in, int64_t iprob,
+                                              const double2* __restrict__ S, int ls, int64_t sprob,
+                                              const int* __restrict__ kin, int kout, C<T>* __restrict__ out,
+                                              int64_t oprob, int64_t n) {
+  const int p = blockIdx.y;
+  const int j0 = blockIdx.z * JO;
+  const int K = kin[p];
+  const C<T>* ip = in + p * iprob;
+  const double2* Sp = S + p * sprob + int64_t(j0) * ls;
+  const int64_t base = int64_t(blockIdx.x) * RB;
+  double2 acc[JO][RPT];
+#pragma unroll
+  for (int jj = 0; jj < JO; ++jj)
+#pragma unroll
+    for (int t = 0; t < RPT; ++t) acc[jj][t] = make_double2(0.0, 0.0);
+  for (int i = 0; i < K; ++i) {
+    double2 v[RPT];
+#pragma unroll
+    for (int t = 0; t < RPT; ++t) v[t] = to_d(ip[int64_t(i) * n + krow(base, t, n)]);
+#pragma unroll
+    for (int jj = 0; jj < JO; ++jj) {
+      const double2 c = Sp[int64_t(jj) * ls + i];
+#pragma unroll
+      for (int t = 0; t < RPT; ++t) {
+        acc[jj][t].x += v[t].x * c.x - v[t].y * c.y;
+        acc[jj][t].y += v[t].x * c.y + v[t].y * c.x;
+      }
+    }
+  }
+  C<T>* op = out + p * oprob;
+#pragma unroll
+  for (int jj = 0; jj < JO; ++jj) {
+    if (j0 + jj >= kout) break;
+#pragma unroll
+    for (int t = 0; t < RPT; ++t) {
+      const int64_t r = base + t * KT + threadIdx.x;
+      if (r < n) op[int64_t(j0 + jj) * n + r] = from_d<T>(acc[jj][t]);
+    }
+  }
+}
+
+void Recycle::alloc(int64_t n, int np, int m_, size_t elem) {
+  m = m_;
+  kcap = m_;
+  kc = 0;
+  build = false;
+  V.alloc(size_t(kcap + m + 1) * n * np * elem);
+  U.alloc(size_t(kcap) * n * np * elem);
+}
+
 // ----- mixed precision helpers -------------------------------------------------
 __global__ void k_resid_d(const double2* __restrict__ b, const double2* __restrict__ ax, double2* __restrict__ r,
                           int64_t total) {
@@ -499,13 +589,18 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   const int np = ps.nprob;
   const int64_t n = ps.n;
   const int m = prm.restart;
-  const int maxcols = m + 2;
+  Recycle* rec = prm.rec;
+  VP_REQUIRE(!rec || rec->m == m, ERR_INVALID, "recycled space of another restart length");
+  const int kcap = rec ? rec->kcap : 0;
+  const int kc = rec ? rec->kc : 0;      // recycled columns C (fixed during this solve)
+  const int maxcols = kcap + m + 2;
   hipStream_t s = ps.stream;
   const int nblk = int((n + RB - 1) / RB);
   const bool left = ps.has_prec && prm.side == 0;
   const bool right = ps.has_prec && prm.side == 1;
 
-  DevBuf V(size_t(m + 1) * n * np * sizeof(C<T>));
+  DevBuf Vown;
+  if (!rec) Vown.alloc(size_t(m + 1) * n * np * sizeof(C<T>));
   DevBuf t1(size_t(n) * np * sizeof(C<T>)), t2(size_t(n) * np * sizeof(C<T>));
   DevBuf part(size_t(np) * maxcols * nblk * sizeof(double2));
   DevBuf H(size_t(np) * (m + 1) * m * sizeof(double2)), cs(size_t(np) * m * sizeof(double)),
@@ -514,11 +609,26 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       nr(size_t(np) * maxcols * sizeof(double2)), yv(size_t(np) * maxcols * sizeof(double2));
   DevBuf tolabs(np * sizeof(double)), invn(np * sizeof(double)), est(np * sizeof(double)), conv(np * sizeof(int)),
       kk(np * sizeof(int)), beta(np * sizeof(double)), bn(np * sizeof(double)), act(np * sizeof(int));
-  C<T>* Vb = V.as<C<T>>();
-  const int64_t vprob = int64_t(m + 1) * n;
+  // recycled part: B = C^H Aop V, a0 = C^H r per cycle, update coefficients of U
+  DevBuf Bst, a0, coef, kcv;
+  if (kc > 0) {
+    Bst.alloc(size_t(np) * m * kcap * sizeof(double2));
+    a0.alloc(size_t(np) * maxcols * sizeof(double2));
+    coef.alloc(size_t(np) * maxcols * sizeof(double2));
+    kcv.alloc(np * sizeof(int));
+    std::vector<int> hk(np, kc);
+    VP_HIP(hipMemcpy(kcv.p, hk.data(), np * sizeof(int), hipMemcpyHostToDevice));
+  }
+  // Krylov storage: [C (kc of kcap columns, right-aligned) | V (m+1 columns)]
+  const int64_t vprob = int64_t(kcap + m + 1) * n;
+  C<T>* Vb = (rec ? rec->V.as<C<T>>() : Vown.as<C<T>>()) + int64_t(kcap) * n;
+  C<T>* Cb = Vb - int64_t(kc) * n;   // Gram-Schmidt runs over columns Cb .. Vb + k
   ArnoldiState st{H.as<double2>(), cs.as<double>(), sn.as<double2>(), g.as<double2>(), h1.as<double2>(),
                   h2.as<double2>(), nr.as<double2>(), tolabs.as<double>(), invn.as<double>(), est.as<double>(),
                   conv.as<int>(), m, maxcols};
+  st.kc = kc;
+  st.kcap = std::max(kcap, 1);
+  st.Bst = kc > 0 ? Bst.as<double2>() : nullptr;
   res.flag.assign(np, 1);
   res.iter_inner.assign(np, 0);
   res.iter_outer.assign(np, 0);
@@ -578,6 +688,16 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       hipLaunchKernelGGL(k_copy2<T>, dim3(grid1(n), np), dim3(256), 0, s, t2.as<C<T>>(), n, v0, vprob, n, 0, dact);
       VP_HIP(hipGetLastError());
     }
+    if (kc > 0) {
+      // GCRO: a0 = C^H r, r -= C a0 (the U part of the update minimises it away)
+      hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Cb, n, vprob, kc, v0, vprob, n,
+                         part.as<double2>(), maxcols, dact, 0);
+      hipLaunchKernelGGL(k_reduce_cols, dim3(kc, np), dim3(KT), 0, s, part.as<double2>(), nblk, kc, maxcols,
+                         a0.as<double2>(), dact);
+      hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Cb, n, vprob, kc, a0.as<double2>(), v0,
+                         vprob, n, part.as<double2>(), maxcols, 1, dact);
+      VP_HIP(hipGetLastError());
+    }
     dev_norms<T>(v0, n, np, vprob, beta.as<double>(), s);
     VP_HIP(hipMemcpy(hbeta.data(), beta.p, np * sizeof(double), hipMemcpyDeviceToHost));
   };
@@ -585,12 +705,124 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     hipLaunchKernelGGL(k_zero<T>, dim3(grid1(n * np)), dim3(256), 0, s, x, n * np);
     VP_HIP(hipGetLastError());
   }
+  // x += M^-1 (V y + U coef) (right) or V y + U coef, y from the rotated
+  // least squares of kk[p] columns, for the problems in mask
+  auto update_x = [&](const int* mask) {
+    hipLaunchKernelGGL(k_backsolve, dim3((np + 63) / 64), dim3(64), 0, s, st, kk.as<int>(), yv.as<double2>(), np,
+                       mask);
+    if (kc > 0)
+      hipLaunchKernelGGL(k_aug_coef, dim3((np + 63) / 64), dim3(64), 0, s, a0.as<double2>(), Bst.as<double2>(), kcap,
+                         m, yv.as<double2>(), kk.as<int>(), maxcols, kc, coef.as<double2>(), np, mask);
+    const C<T>* Ub = kc > 0 ? rec->U.template as<C<T>>() : nullptr;
+    const int64_t uprob = int64_t(kcap) * n;
+    if (right) {
+      hipLaunchKernelGGL(k_lincomb<T>, dim3(grid1(n), np), dim3(KT), 0, s, Vb, n, vprob, yv.as<double2>(),
+                         kk.as<int>(), maxcols, t1.as<C<T>>(), n, n, 0, mask);
+      if (kc > 0)
+        hipLaunchKernelGGL(k_lincomb<T>, dim3(grid1(n), np), dim3(KT), 0, s, Ub, n, uprob, coef.as<double2>(),
+                           kcv.as<int>(), maxcols, t1.as<C<T>>(), n, n, 1, mask);
+      ps_apply_M<T>(ps, t1.as<C<T>>(), n, t2.as<C<T>>(), n, mask);
+      hipLaunchKernelGGL(k_copy2<T>, dim3(grid1(n), np), dim3(256), 0, s, t2.as<C<T>>(), n, x, n, n, 1, mask);
+    } else {
+      hipLaunchKernelGGL(k_lincomb<T>, dim3(grid1(n), np), dim3(KT), 0, s, Vb, n, vprob, yv.as<double2>(),
+                         kk.as<int>(), maxcols, x, n, n, 1, mask);
+      if (kc > 0)
+        hipLaunchKernelGGL(k_lincomb<T>, dim3(grid1(n), np), dim3(KT), 0, s, Ub, n, uprob, coef.as<double2>(),
+                           kcv.as<int>(), maxcols, x, n, n, 1, mask);
+    }
+    VP_HIP(hipGetLastError());
+  };
+  // C = V Q and U = V(:, 0..K-1) R^-1 from each problem's last cycle, where
+  // the Givens rotations G give G H = [R; 0] and Q = G^H (first K columns):
+  // Aop V(:, 0..K-1) = V H = C R, so Aop U = C with C orthonormal
+  auto build_recycled = [&](const std::vector<int>& lk) {
+    int kn = 0;
+    for (int p = 0; p < np; ++p) kn = std::max(kn, std::min(lk[p], kcap));
+    if (kn == 0) return;
+    std::vector<double2> hH(size_t(np) * (m + 1) * m), hsn(size_t(np) * m);
+    std::vector<double> hcs(size_t(np) * m);
+    VP_HIP(hipMemcpy(hH.data(), H.p, hH.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    VP_HIP(hipMemcpy(hsn.data(), sn.p, hsn.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    VP_HIP(hipMemcpy(hcs.data(), cs.p, hcs.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const int lsC = kn + 1, ncol = (kn + JO - 1) / JO * JO;
+    std::vector<double2> SC(size_t(np) * lsC * ncol, make_double2(0.0, 0.0)),
+        SU(size_t(np) * kn * ncol, make_double2(0.0, 0.0));
+    std::vector<int> kinC(np, 0), kinU(np, 0);
+    using cd = std::complex<double>;
+    for (int p = 0; p < np; ++p) {
+      const int K = std::min(lk[p], kn);
+      if (K == 0) continue;
+      std::vector<cd> Q(size_t(K + 1) * (K + 1), cd(0.0));     // row-major
+      for (int i = 0; i <= K; ++i) Q[size_t(i) * (K + 1) + i] = 1.0;
+      for (int i = K - 1; i >= 0; --i) {
+        const double c = hcs[size_t(p) * m + i];
+        const cd sv(hsn[size_t(p) * m + i].x, hsn[size_t(p) * m + i].y);
+        for (int col = 0; col <= K; ++col) {
+          const cd a = Q[size_t(i) * (K + 1) + col], b = Q[size_t(i + 1) * (K + 1) + col];
+          Q[size_t(i) * (K + 1) + col] = c * a - sv * b;
+          Q[size_t(i + 1) * (K + 1) + col] = std::conj(sv) * a + c * b;
+        }
+      }
+      const double2* Hp = hH.data() + size_t(p) * (m + 1) * m;
+      auto R = [&](int i, int j) { return cd(Hp[size_t(j) * (m + 1) + i].x, Hp[size_t(j) * (m + 1) + i].y); };
+      bool ok = true;
+      for (int i = 0; i < K; ++i) ok &= std::abs(R(i, i)) > 0.0;
+      if (!ok) continue;   // singular R: this problem recycles nothing
+      double2* sc = SC.data() + size_t(p) * lsC * ncol;
+      for (int j = 0; j < K; ++j)
+        for (int r = 0; r <= K; ++r) {
+          const cd q = Q[size_t(r) * (K + 1) + j];
+          sc[size_t(j) * lsC + r] = make_double2(q.real(), q.imag());
+        }
+      // R^-1 column by column (back substitution)
+      double2* su = SU.data() + size_t(p) * kn * ncol;
+      for (int j = 0; j < K; ++j) {
+        std::vector<cd> xcol(K, cd(0.0));
+        for (int i = j; i >= 0; --i) {
+          cd sum = (i == j) ? cd(1.0) : cd(0.0);
+          for (int l = i + 1; l <= j; ++l) sum -= R(i, l) * xcol[l];
+          xcol[i] = sum / R(i, i);
+        }
+        for (int i = 0; i <= j; ++i) su[size_t(j) * kn + i] = make_double2(xcol[i].real(), xcol[i].imag());
+      }
+      kinC[p] = K + 1;
+      kinU[p] = K;
+    }
+    DevBuf dSC(SC.size() * sizeof(double2)), dSU(SU.size() * sizeof(double2)), dkC(np * sizeof(int)),
+        dkU(np * sizeof(int));
+    VP_HIP(hipMemcpyAsync(dSC.p, SC.data(), dSC.bytes, hipMemcpyHostToDevice, s));
+    VP_HIP(hipMemcpyAsync(dSU.p, SU.data(), dSU.bytes, hipMemcpyHostToDevice, s));
+    VP_HIP(hipMemcpyAsync(dkC.p, kinC.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    VP_HIP(hipMemcpyAsync(dkU.p, kinU.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    C<T>* Cout = Vb - int64_t(kn) * n;
+    hipLaunchKernelGGL(k_tsgemm<T>, dim3(nblk, np, ncol / JO), dim3(KT), 0, s, Vb, vprob, dSC.as<double2>(), lsC,
+                       int64_t(lsC) * ncol, dkC.as<int>(), kn, Cout, vprob, n);
+    hipLaunchKernelGGL(k_tsgemm<T>, dim3(nblk, np, ncol / JO), dim3(KT), 0, s, Vb, vprob, dSU.as<double2>(), kn,
+                       int64_t(kn) * ncol, dkU.as<int>(), kn, rec->U.template as<C<T>>(), int64_t(kcap) * n, n);
+    VP_HIP(hipGetLastError());
+    VP_HIP(hipStreamSynchronize(s));
+    rec->kc = kn;
+    rec->build = false;
+  };
   residual_into_V0(prm.x0_nonzero);
   std::vector<double> last(np);
+  std::vector<int> proj_only(np, 0), lastk(np, 0);
+  bool any_proj = false;
   for (int p = 0; p < np; ++p) {
     last[p] = hbn[p] > 0 ? hbeta[p] / hbn[p] : 0.0;
     res.resvec[p].push_back(last[p]);
     hact[p] = (joins(p) && last[p] > tol_of(p)) ? 1 : 0;
+    // the projection on C alone met the tolerance: x takes the U part only
+    if (kc > 0 && joins(p) && !hact[p]) proj_only[p] = 1;
+    any_proj |= proj_only[p] != 0;
+  }
+  if (any_proj) {
+    std::vector<int> zero(np, 0);
+    DevBuf pm(np * sizeof(int));
+    VP_HIP(hipMemcpyAsync(kk.p, zero.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    VP_HIP(hipMemcpyAsync(pm.p, proj_only.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    update_x(pm.as<int>());
+    VP_HIP(hipStreamSynchronize(s));
   }
   {
     std::vector<double> ta(np);
@@ -628,24 +860,26 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       } else {
         ps_apply_A<T>(ps, vk, vprob, w, vprob, dia);
       }
-      // first classical Gram-Schmidt pass: h1 = V^H w (and ||w||^2), w' = w - V h1, ||w'||^2
-      hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, w, vprob, n,
+      // first classical Gram-Schmidt pass over [C, V]: h1 = [C V]^H w (and ||w||^2),
+      // w' = w - [C V] h1, ||w'||^2
+      const int nc = kc + k + 1;
+      hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Cb, n, vprob, nc, w, vprob, n,
                          part.as<double2>(), maxcols, dia, 1);
-      hipLaunchKernelGGL(k_reduce_cols, dim3(k + 2, np), dim3(KT), 0, s, part.as<double2>(), nblk, k + 2, maxcols,
+      hipLaunchKernelGGL(k_reduce_cols, dim3(nc + 1, np), dim3(KT), 0, s, part.as<double2>(), nblk, nc + 1, maxcols,
                          h1.as<double2>(), dia);
-      hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, h1.as<double2>(), w,
+      hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Cb, n, vprob, nc, h1.as<double2>(), w,
                          vprob, n, part.as<double2>(), maxcols, 1, dia);
       hipLaunchKernelGGL(k_reduce_cols, dim3(1, np), dim3(KT), 0, s, part.as<double2>(), nblk, 1, maxcols,
                          nr.as<double2>(), dia);
       // optional second pass only where the first cancelled (blocks of other problems exit at once)
       hipLaunchKernelGGL(k_dgks, dim3((np + 63) / 64), dim3(64), 0, s, h1.as<double2>(), nr.as<double2>(), maxcols,
-                         k, dia, ract.as<int>(), np, cgs2_always ? 1 : 0, eta2);
+                         nc - 1, dia, ract.as<int>(), np, cgs2_always ? 1 : 0, eta2);
       if (second_pass) {
-        hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, w, vprob, n,
+        hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Cb, n, vprob, nc, w, vprob, n,
                            part.as<double2>(), maxcols, ract.as<int>(), 0);
-        hipLaunchKernelGGL(k_reduce_cols, dim3(k + 1, np), dim3(KT), 0, s, part.as<double2>(), nblk, k + 1,
+        hipLaunchKernelGGL(k_reduce_cols, dim3(nc, np), dim3(KT), 0, s, part.as<double2>(), nblk, nc,
                            maxcols, h2.as<double2>(), ract.as<int>());
-        hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Vb, n, vprob, k + 1, h2.as<double2>(),
+        hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Cb, n, vprob, nc, h2.as<double2>(),
                            w, vprob, n, part.as<double2>(), maxcols, 1, ract.as<int>());
         hipLaunchKernelGGL(k_reduce_cols, dim3(1, np), dim3(KT), 0, s, part.as<double2>(), nblk, 1, maxcols,
                            nr.as<double2>(), ract.as<int>());
@@ -660,6 +894,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       for (int p = 0; p < np; ++p) {
         if (!inner[p]) continue;
         hk[p] = k + 1;
+        lastk[p] = k + 1;
         res.total[p] += 1;
         res.iter_inner[p] = k + 1;
         last[p] = hbn[p] > 0 ? hest[p] / hbn[p] : 0.0;
@@ -677,19 +912,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     }
     // solve the least squares and update x for every problem active in this cycle
     VP_HIP(hipMemcpyAsync(kk.p, hk.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_backsolve, dim3((np + 63) / 64), dim3(64), 0, s, st, kk.as<int>(), yv.as<double2>(), np,
-                       dact);
-    if (right) {
-      // x += M^-1 (V y)
-      hipLaunchKernelGGL(k_lincomb<T>, dim3(grid1(n), np), dim3(KT), 0, s, Vb, n, vprob, yv.as<double2>(),
-                         kk.as<int>(), maxcols, t1.as<C<T>>(), n, n, 0, dact);
-      ps_apply_M<T>(ps, t1.as<C<T>>(), n, t2.as<C<T>>(), n, dact);
-      hipLaunchKernelGGL(k_copy2<T>, dim3(grid1(n), np), dim3(256), 0, s, t2.as<C<T>>(), n, x, n, n, 1, dact);
-    } else {
-      hipLaunchKernelGGL(k_lincomb<T>, dim3(grid1(n), np), dim3(KT), 0, s, Vb, n, vprob, yv.as<double2>(),
-                         kk.as<int>(), maxcols, x, n, n, 1, dact);
-    }
-    VP_HIP(hipGetLastError());
+    update_x(dact);
     VP_HIP(hipStreamSynchronize(s));   // iact is released at the end of this cycle
     // restart: fresh residual (pgmres_iter.m:150 carries it by recurrence)
     for (int p = 0; p < np; ++p) hact[p] = (hact[p] && last[p] > tol_of(p)) ? 1 : 0;
@@ -699,6 +922,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     upload_act();
     residual_into_V0(true);
   }
+  if (rec && rec->build && kc == 0) build_recycled(lastk);
   if (!prm.final_residual) {
     // caller recomputes the residual itself: report the recurrence estimate
     for (int p = 0; p < np; ++p) {

@@ -40,6 +40,22 @@ void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* 
 template <typename T>
 void ps_apply_M(ProblemSet<T>& ps, const C<T>* x, int64_t xs, C<T>* y, int64_t ys, const int* active);
 
+// Recycled space kept across the inner FP32 solves of one refinement run
+// (GCRO with the whole Krylov space of the first inner solve): C orthonormal
+// and U with Aop U = C, Aop = (A/s) M^-1 (right) or M^-1 (A/s) (left).  C is
+// stored right-aligned in the leading kcap columns of the Krylov buffer V
+// (columns kcap-kc .. kcap-1), so Gram-Schmidt against [C, V] runs over
+// contiguous columns; the Arnoldi basis starts at column kcap.  Problems
+// whose own space is narrower carry zero columns.
+struct Recycle {
+  DevBuf V;            // [p][(kcap + m + 1) * n] precision of the inner solve
+  DevBuf U;            // [p][kcap * n]
+  int kcap = 0, m = 0;
+  int kc = 0;          // recycled columns in use (0: none yet)
+  bool build = false;  // form C and U from the last cycle of the next solve
+  void alloc(int64_t n, int np, int m_, size_t elem);
+};
+
 struct GmresParams {
   int restart = 50;
   double tol = 1e-6;
@@ -50,6 +66,7 @@ struct GmresParams {
   std::vector<double> tol_per;   // per-problem tolerance (empty: tol for all)
   std::vector<int> active_in;    // per-problem participation (empty: all); inactive problems do no work
   bool final_residual = true;    // recompute the true residual after the last cycle
+  Recycle* rec = nullptr;        // augmented solve (restart must equal rec->m)
 };
 
 struct GmresResult {
@@ -72,7 +89,7 @@ struct IrProblem {
 };
 bool ir_step_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2* b, double2* x, double scale32,
                  double tol, int restart, double inner_tol, int inner_maxit, int max_outer, int side,
-                 std::vector<IrProblem>& st);
+                 std::vector<IrProblem>& st, Recycle* rec = nullptr);
 
 // Device vector helpers (FP64 accumulation).
 template <typename T>
