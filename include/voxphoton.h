@@ -250,7 +250,9 @@ int vp_solve_ir_batched(const vp_op* ops, const vp_prec* precs, int nsolve, int 
  * No counterpart in the reference, whose scripts solve one wavelength after
  * another (Bragg_MOR_script.m:88-95, example_Bragg_1500nm_to_1600nm.m:85-114).
  * A server owns one HIP stream on the context's device and a worker thread.
- * Submitted mixed-precision solves (vp_solve_ir semantics) wait in a FIFO
+ * Submitted mixed-precision solves (vp_solve_ir semantics, except that every
+ * inner solve starts afresh: requests regroup between rounds, so no recycled
+ * Krylov space is carried, as with VP_IR_RECYCLE=0) wait in a FIFO
  * queue and are admitted while the server's HBM budget (solver state of the
  * admitted requests: Krylov basis, refinement vectors, batch workspaces) and
  * its slot count allow.  Every round, the worker groups the admitted

@@ -37,9 +37,21 @@ def _requests(vp_ctx):
 
 
 def _solo(vp_ctx, reqs):
+    # the server's refinement rounds start every inner solve afresh (requests
+    # regroup between rounds, so no Krylov space is carried over): the solo
+    # solves run the same way
     import ctypes
+    import os
     from voxphoton_amd import _lib as L, vie
     out = []
+    os.environ['VP_IR_RECYCLE'] = '0'
+    try:
+        return _solo_runs(vp_ctx, reqs, L, vie, ctypes, out)
+    finally:
+        os.environ.pop('VP_IR_RECYCLE', None)
+
+
+def _solo_runs(vp_ctx, reqs, L, vie, ctypes, out):
     for op, P, db, n in reqs:
         dx = vie.DeviceArray(vp_ctx, n * 16)
         info = L.GmresInfo()

@@ -319,9 +319,15 @@ bool ir_step_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2*
   std::vector<int> hact(np);
   for (int p = 0; p < np; ++p) hact[p] = st[p].done ? 0 : 1;
   VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
-  // r = b - A x  (FP64)
-  ps_apply_A<double>(p64, x, n, ax.as<double2>(), n, act.as<int>());
-  dev_residual_d(b, ax.as<double2>(), r.as<double2>(), n, np, s);
+  // r = b - A x  (FP64); before any step x = 0 and r = b
+  bool all_first = true;
+  for (int p = 0; p < np; ++p) all_first &= !hact[p] || st[p].steps == 0;
+  if (all_first) {
+    VP_HIP(hipMemcpyAsync(r.p, b, size_t(n) * np * sizeof(double2), hipMemcpyDeviceToDevice, s));
+  } else {
+    ps_apply_A<double>(p64, x, n, ax.as<double2>(), n, act.as<int>());
+    dev_residual_d(b, ax.as<double2>(), r.as<double2>(), n, np, s);
+  }
   dev_norms<double>(r.as<double2>(), n, np, n, rn.as<double>(), s);
   dev_norms<double>(b, n, np, n, bn.as<double>(), s);
   std::vector<double> hbn(np), hrn(np);

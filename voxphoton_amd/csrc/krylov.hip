@@ -444,11 +444,12 @@ __global__ void k_aug_coef(const double2* __restrict__ a0, const double2* __rest
   }
 }
 
-constexpr int JO = 8;   // output columns per tall-skinny product block
+constexpr int JO = 32;   // output columns per tall-skinny product block
 
 // out[p][:, j] = sum_{i < kin[p]} in[p][:, i] S[p][i + ls j] for j < kout
-// (S FP64 with at least ceil(kout / JO) * JO columns, zero past kout).  Rows
-// past n read row n-1 and are not stored.
+// (S FP64 with at least ceil(kout / JO) * JO columns, zero past kout).  One
+// row per thread and JO outputs per block, so each input column is read
+// ceil(kout / JO) times; S is read uniformly.
 template <typename T>
 __global__ void __launch_bounds__(KT) k_tsgemm(const C<T>* __restrict__ in, int64_t iprob,
                                               const double2* __restrict__ S, int ls, int64_t sprob,
@@ -459,36 +460,25 @@ __global__ void __launch_bounds__(KT) k_tsgemm(const C<T>* __restrict__ in, int6
   const int K = kin[p];
   const C<T>* ip = in + p * iprob;
   const double2* Sp = S + p * sprob + int64_t(j0) * ls;
-  const int64_t base = int64_t(blockIdx.x) * RB;
-  double2 acc[JO][RPT];
+  const int64_t r = int64_t(blockIdx.x) * KT + threadIdx.x;
+  const int64_t rr = min<int64_t>(r, n - 1);
+  double2 acc[JO];
 #pragma unroll
-  for (int jj = 0; jj < JO; ++jj)
-#pragma unroll
-    for (int t = 0; t < RPT; ++t) acc[jj][t] = make_double2(0.0, 0.0);
+  for (int jj = 0; jj < JO; ++jj) acc[jj] = make_double2(0.0, 0.0);
   for (int i = 0; i < K; ++i) {
-    double2 v[RPT];
-#pragma unroll
-    for (int t = 0; t < RPT; ++t) v[t] = to_d(ip[int64_t(i) * n + krow(base, t, n)]);
+    const double2 v = to_d(ip[int64_t(i) * n + rr]);
 #pragma unroll
     for (int jj = 0; jj < JO; ++jj) {
       const double2 c = Sp[int64_t(jj) * ls + i];
-#pragma unroll
-      for (int t = 0; t < RPT; ++t) {
-        acc[jj][t].x += v[t].x * c.x - v[t].y * c.y;
-        acc[jj][t].y += v[t].x * c.y + v[t].y * c.x;
-      }
+      acc[jj].x += v.x * c.x - v.y * c.y;
+      acc[jj].y += v.x * c.y + v.y * c.x;
     }
   }
+  if (r >= n) return;
   C<T>* op = out + p * oprob;
 #pragma unroll
-  for (int jj = 0; jj < JO; ++jj) {
-    if (j0 + jj >= kout) break;
-#pragma unroll
-    for (int t = 0; t < RPT; ++t) {
-      const int64_t r = base + t * KT + threadIdx.x;
-      if (r < n) op[int64_t(j0 + jj) * n + r] = from_d<T>(acc[jj][t]);
-    }
-  }
+  for (int jj = 0; jj < JO; ++jj)
+    if (j0 + jj < kout) op[int64_t(j0 + jj) * n + r] = from_d<T>(acc[jj]);
 }
 
 void Recycle::alloc(int64_t n, int np, int m_, size_t elem) {
@@ -795,9 +785,10 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     VP_HIP(hipMemcpyAsync(dkC.p, kinC.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
     VP_HIP(hipMemcpyAsync(dkU.p, kinU.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
     C<T>* Cout = Vb - int64_t(kn) * n;
-    hipLaunchKernelGGL(k_tsgemm<T>, dim3(nblk, np, ncol / JO), dim3(KT), 0, s, Vb, vprob, dSC.as<double2>(), lsC,
+    const unsigned rblk = unsigned((n + KT - 1) / KT);
+    hipLaunchKernelGGL(k_tsgemm<T>, dim3(rblk, np, ncol / JO), dim3(KT), 0, s, Vb, vprob, dSC.as<double2>(), lsC,
                        int64_t(lsC) * ncol, dkC.as<int>(), kn, Cout, vprob, n);
-    hipLaunchKernelGGL(k_tsgemm<T>, dim3(nblk, np, ncol / JO), dim3(KT), 0, s, Vb, vprob, dSU.as<double2>(), kn,
+    hipLaunchKernelGGL(k_tsgemm<T>, dim3(rblk, np, ncol / JO), dim3(KT), 0, s, Vb, vprob, dSU.as<double2>(), kn,
                        int64_t(kn) * ncol, dkU.as<int>(), kn, rec->U.template as<C<T>>(), int64_t(kcap) * n, n);
     VP_HIP(hipGetLastError());
     VP_HIP(hipStreamSynchronize(s));
