@@ -155,6 +155,30 @@ def test_level2_plane_larger_than_lds(vp_ctx):
     assert rel(P.apply(x, 'fp32'), ref) < 2e-3      # binary16 blocks of the quadrant form
 
 
+@pytest.mark.parametrize('reg', ['1', '0'])
+@pytest.mark.parametrize('N', [6, 12, 20, 26, 30, 40, 50, 60, 64])
+def test_level2_block_inversion_every_shape(vp_ctx, monkeypatch, N, reg):
+    # 3N x 3N blocks of 18..192 rows: the register Gauss-Jordan kernel at every
+    # elements-per-thread shape (1..6 x 1..6 per thread, 256 or 512 threads),
+    # the panel kernel from 133 to 192 rows, the global-memory kernel with
+    # VP_PC_INVERT_REG=0
+    from voxphoton_amd.vie import Preconditioner
+    monkeypatch.setenv('VP_PC_INVERT_REG', reg)
+    L, M = 3, 2
+    rng = np.random.default_rng(N)
+    # unit-scale tensor: blocks far from the identity (condition numbers up to
+    # ~5e2), so partial pivoting moves most rows
+    G = random_toeplitz(L, M, N, rng)
+    idx = ovie.idx_from_mask(np.ones((L, M, N), bool))
+    Binv = opre.level_2_parallel_func(G, 0.5, 1.0, L, M, N, 1.0)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
+    P = Preconditioner.level2(G, 0.5, 1.0, 1.0, idx, precisions=('fp64',), ctx=vp_ctx)
+    e = rel(P.apply(x), ref)
+    print('3N', 3 * N, 'register kernel', reg, 'rel', e, flush=True)
+    assert e < 1e-9
+
+
 def test_level2_without_parity_zeros_keeps_all_blocks(vp_ctx):
     # Odd components that do not vanish on their zero planes break the
     # mirror symmetry of the Chan blocks: every block must be stored.

@@ -13,6 +13,7 @@
 // DFT along y, then batched Gauss-Jordan inversion in FP64.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 #include "precond.hpp"
@@ -692,6 +693,76 @@ __global__ void k_pc_chan_x(const double2* __restrict__ G, double2* __restrict__
   }
 }
 
+// Same coefficients, one workgroup per line (k, n, m): the Chan-weighted line
+// c_i (i < L) is formed once in LDS and its length-L DFT runs in two levels,
+// L = L1*L2, i = i1 + L1*i2, kx = k2 + L2*k1:
+//   T[k2*L1 + i1] = w^{i1 k2} sum_{i2} c_{i1 + L1 i2} w^{L1 i2 k2},
+//   ck[kx]        = sum_{i1} T[k2*L1 + i1] w^{L2 i1 k1},
+// L*(L1 + L2) products instead of L^2 (w = twL[1], indices reduced mod L).
+__global__ void __launch_bounds__(256) k_pc_chan_x2(const double2* __restrict__ G, double2* __restrict__ ck, int L,
+                                                    int M, int N, int L1, double2 mc, double diag,
+                                                    const double2* __restrict__ twL) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double2* c = reinterpret_cast<double2*>(smem_raw);
+  double2* T = c + L;
+  const int L2 = L / L1;
+  int r = blockIdx.x;
+  const int m = r % M; r /= M;
+  const int n = r % N;
+  const int k = r / N;
+  const int64_t V = int64_t(L) * M * N;
+  const int64_t base = int64_t(L) * (m + int64_t(M) * n) + V * k;
+  const double sg = c_pc_even_x[k] ? 1.0 : -1.0;
+  const bool dg = m == 0 && n == 0 && (k == 0 || k == 3 || k == 5);
+  auto A = [&](int i) -> double2 {
+    const double2 g = G[base + i];
+    double2 a = make_double2(-(mc.x * g.x - mc.y * g.y), -(mc.x * g.y + mc.y * g.x));
+    if (i == 0 && dg) a.x += diag;
+    return a;
+  };
+  for (int i = threadIdx.x; i < L; i += blockDim.x) {
+    double2 v = A(i);
+    if (i > 0) {
+      const double2 aj = A(L - i);
+      const double wa = double(L - i) / L, wb = sg * double(i) / L;
+      v = make_double2(wa * v.x + wb * aj.x, wa * v.y + wb * aj.y);
+    }
+    c[i] = v;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < L; e += blockDim.x) {
+    const int k2 = e / L1, i1 = e - k2 * L1;
+    const int step = L1 * k2;   // < L
+    double2 acc = make_double2(0.0, 0.0);
+    int ph = 0;
+    for (int i2 = 0; i2 < L2; ++i2) {
+      const double2 v = c[i1 + L1 * i2], w = twL[ph];
+      acc.x += v.x * w.x - v.y * w.y;
+      acc.y += v.x * w.y + v.y * w.x;
+      ph += step;
+      if (ph >= L) ph -= L;
+    }
+    const double2 w = twL[(i1 * k2) % L];
+    T[e] = make_double2(acc.x * w.x - acc.y * w.y, acc.x * w.y + acc.y * w.x);
+  }
+  __syncthreads();
+  double2* out = ck + (int64_t(k) * N * M + int64_t(n) * M + m) * L;   // layout [k][n][m][kx]
+  for (int kx = threadIdx.x; kx < L; kx += blockDim.x) {
+    const int k1 = kx / L2, k2 = kx - k1 * L2;
+    const int step = L2 * k1;   // < L
+    double2 acc = make_double2(0.0, 0.0);
+    int ph = 0;
+    for (int i1 = 0; i1 < L1; ++i1) {
+      const double2 v = T[k2 * L1 + i1], w = twL[ph];
+      acc.x += v.x * w.x - v.y * w.y;
+      acc.y += v.x * w.y + v.y * w.x;
+      ph += step;
+      if (ph >= L) ph -= L;
+    }
+    out[kx] = acc;
+  }
+}
+
 // Block (kx, ky) entry [(c, nt), (d, ns)] = sum_t chan_y(t) w_M^{t ky}.  The
 // M x M y-Toeplitz block of component k at z-offset (nt, ns) has first
 // column col(t) and first row row(t) built from v(t) = ck[k][|nt-ns|][t][kx]
@@ -876,7 +947,18 @@ void build_level2_blocks(const double2* G, int L, int M, int N, double2 mc, doub
   VP_HIP(hipMemcpyAsync(dL.p, tL.data(), dL.bytes, hipMemcpyHostToDevice, s));
   VP_HIP(hipMemcpyAsync(dM.p, tM.data(), dM.bytes, hipMemcpyHostToDevice, s));
   DevBuf ck(size_t(6) * L * M * N * sizeof(double2));
-  {
+  const char* ed = std::getenv("VP_PC_CHANX_DIRECT");   // 1: the direct O(L^2) kernel
+  const size_t lds2 = size_t(2) * L * sizeof(double2);
+  if (!(ed && std::atoi(ed) != 0) && lds2 <= 160 * 1024) {
+    int L1 = 1;   // largest divisor of L not above sqrt(L)
+    for (int d = 1; int64_t(d) * d <= L; ++d)
+      if (L % d == 0) L1 = d;
+    const void* fn = reinterpret_cast<const void*>(&k_pc_chan_x2);
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2));
+    hipLaunchKernelGGL(k_pc_chan_x2, dim3(6 * M * N), dim3(256), lds2, s, G, ck.as<double2>(), L, M, N, L1, mc, diag,
+                       dL.as<double2>());
+    VP_HIP(hipGetLastError());
+  } else {
     const int64_t total = int64_t(6) * L * M * N;
     const int blocks = int(std::min<int64_t>((total + 255) / 256, 1 << 20));
     hipLaunchKernelGGL(k_pc_chan_x, dim3(blocks), dim3(256), 0, s, G, ck.as<double2>(), L, M, N, mc, diag,
@@ -940,13 +1022,332 @@ bool toeplitz_parity_zero(const double2* G, int L, int M, int N, hipStream_t s) 
   return std::isfinite(a) && z <= 1e-12 * a;
 }
 
+// Register-resident Gauss-Jordan inversion of one n x n block per workgroup
+// (n <= 2*64 + 64 rows for the pivot search; the host keeps ceil(n/E)^2 <= NT).
+// Thread (tr, tc) of a TR x TR grid, TR = ceil(n/E), holds rows tr + TR*a and
+// columns tc + TR*b (a, b < E) in registers.  Pivoting is implicit: step k
+// pivots on (p_k, k), p_k the unused row of largest |re|+|im| in column k
+// (ties to the smaller row), and no rows move.  Each step is the exchange
+//   W[p][k] = 1/piv, W[p][j] = W[p][j]/piv, W[i][k] = -W[i][k]/piv,
+//   W[i][j] -= W[i][k] W[p][j]/piv,
+// written as one rank-one update after zeroing row p and column k with
+// c[p] = -1 and rs[k] = 1/piv.  After n steps inv(A)[pinv[i]][perm[j]] =
+// W[i][j], with perm[k] = p_k and pinv its inverse.  Steps run column-major
+// (k = r + TR*b, b unrolled), so the owners of column k read a compile-time
+// register column; the pivot row index is wave uniform, so the owners of
+// row p branch on it without divergence.
+constexpr int kInvMaxN = 192;
+
+template <int NT, int E>
+__global__ void __launch_bounds__(NT) k_pc_invert_reg(double2* __restrict__ blk, int n, int* __restrict__ singular) {
+  double2* A = blk + int64_t(blockIdx.x) * n * n;
+  __shared__ double2 s_col[2][kInvMaxN];
+  __shared__ double2 s_row[kInvMaxN];
+  __shared__ int s_perm[kInvMaxN], s_pinv[kInvMaxN];
+  const int TR = (n + E - 1) / E;
+  const int tr = int(threadIdx.x) % TR, tc = int(threadIdx.x) / TR;
+  const bool act = tc < TR;
+  const int lane = int(threadIdx.x) & 63;
+  double2 W[E][E];
+#pragma unroll
+  for (int a = 0; a < E; ++a)
+#pragma unroll
+    for (int b = 0; b < E; ++b) {
+      const int i = tr + TR * a, j = tc + TR * b;
+      W[a][b] = (act && i < n && j < n) ? A[int64_t(j) * n + i] : make_double2(0.0, 0.0);
+    }
+  unsigned used = 0;   // bit q: row lane + 64q already a pivot row
+  int k = 0;
+#pragma unroll
+  for (int b = 0; b < E; ++b) {
+    for (int r = 0; r < TR; ++r, ++k) {
+      if (k >= n) break;
+      const int buf = k & 1;
+      // column k to LDS, then zero it (its new entries come from the update)
+      if (act && tc == r) {
+#pragma unroll
+        for (int a = 0; a < E; ++a) {
+          const int i = tr + TR * a;
+          if (i < n) s_col[buf][i] = W[a][b];
+          W[a][b] = make_double2(0.0, 0.0);
+        }
+      }
+      __syncthreads();
+      // every wavefront finds the same pivot
+      double best = -1.0;
+      int bi = 0;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int i = lane + 64 * q;
+        if (i < n && !((used >> q) & 1u)) {
+          const double2 v = s_col[buf][i];
+          const double m = fabs(v.x) + fabs(v.y);
+          if (m > best) { best = m; bi = i; }
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      const int p = __builtin_amdgcn_readfirstlane(bi);
+      if ((p & 63) == lane) used |= 1u << (p >> 6);
+      const double2 piv = s_col[buf][p];
+      const double den = piv.x * piv.x + piv.y * piv.y;
+      const bool ok = best > 0.0 && den > 0.0;
+      const double2 ip = ok ? make_double2(piv.x / den, -piv.y / den) : make_double2(0.0, 0.0);
+      if (threadIdx.x == 0) {
+        s_perm[k] = p;
+        s_pinv[p] = k;
+        if (!ok) singular[0] = 1;
+      }
+      // owners of row p publish rs = row/piv (rs[k] = 1/piv) and zero the row
+      const int pr = p % TR, ap = p / TR;
+#pragma unroll
+      for (int a = 0; a < E; ++a) {
+        if (a == ap) {
+          if (act && tr == pr) {
+#pragma unroll
+            for (int bb = 0; bb < E; ++bb) {
+              const int j = tc + TR * bb;
+              if (j < n) s_row[j] = (j == k) ? ip : cmul(W[a][bb], ip);
+              W[a][bb] = make_double2(0.0, 0.0);
+            }
+          }
+        }
+      }
+      __syncthreads();
+      // rank-one update
+      double2 cv[E], rv[E];
+#pragma unroll
+      for (int a = 0; a < E; ++a) {
+        const int i = tr + TR * a;
+        cv[a] = i < n ? s_col[buf][i] : make_double2(0.0, 0.0);
+        if (i == p) cv[a] = make_double2(-1.0, 0.0);
+      }
+#pragma unroll
+      for (int bb = 0; bb < E; ++bb) {
+        const int j = tc + TR * bb;
+        rv[bb] = (act && j < n) ? s_row[j] : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int a = 0; a < E; ++a)
+#pragma unroll
+        for (int bb = 0; bb < E; ++bb) {
+          W[a][bb].x -= cv[a].x * rv[bb].x - cv[a].y * rv[bb].y;
+          W[a][bb].y -= cv[a].x * rv[bb].y + cv[a].y * rv[bb].x;
+        }
+    }
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int a = 0; a < E; ++a)
+#pragma unroll
+      for (int b = 0; b < E; ++b) {
+        const int i = tr + TR * a, j = tc + TR * b;
+        if (i < n && j < n) A[int64_t(s_perm[j]) * n + s_pinv[i]] = W[a][b];
+      }
+  }
+}
+
+// Blocked Gauss-Jordan for blocks too large for registers (n <= 192), one
+// 256-thread workgroup per block, in place in global memory.  Columns are
+// taken in panels of PB.  A panel (n x pb) is swept in LDS with the same
+// implicit-pivot exchange as k_pc_invert_reg; that leaves D = [inv(W_PK);
+// -W_OK inv(W_PK)] (P the panel's pivot rows, K its columns, O the others),
+// and every other column j is then updated at once:
+//   W[P][j] = D[P] W_old[P][j],  W[O][j] = W_old[O][j] + D[O] W_old[P][j],
+// i.e. rows P cleared and the rank-pb product D * W_old[P][j] added.  Each
+// column is read and written once per panel by one wavefront (lanes along
+// rows, D in registers).  The result is inv(A) up to the pivot permutations,
+// applied at the end by following the cycles of perm with two LDS columns:
+// column perm[j] of the inverse is column j of W with row i moved to pinv[i].
+template <int PB>
+__global__ void __launch_bounds__(256, 2) k_pc_invert_panel(double2* __restrict__ blk, int n, int* __restrict__ singular) {
+  constexpr int NT = 256, MAXN = 192, EP = (MAXN * PB + NT - 1) / NT;
+  double2* A = blk + int64_t(blockIdx.x) * n * n;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double2* s_pan = reinterpret_cast<double2*>(smem_raw);   // [q][i], PB x n
+  double2* s_prow = s_pan + PB * n;                          // [q][j], PB x n
+  __shared__ int s_used[MAXN], s_perm[MAXN], s_pinv[MAXN], s_plist[PB];
+  __shared__ int s_p;
+  __shared__ double2 s_ip;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int i = tid; i < n; i += NT) s_used[i] = 0;
+  for (int c0 = 0; c0 < n; c0 += PB) {
+    const int pb = min(PB, n - c0);
+    for (int e = tid; e < pb * n; e += NT) s_pan[e] = A[int64_t(c0) * n + e];
+    __syncthreads();
+    for (int q = 0; q < pb; ++q) {
+      const int k = c0 + q;
+      if (wv == 0) {
+        double best = -1.0;
+        int bi = 0;
+#pragma unroll
+        for (int a = 0; a < MAXN / 64; ++a) {
+          const int i = lane + 64 * a;
+          if (i < n && !s_used[i]) {
+            const double2 v = s_pan[q * n + i];
+            const double m = fabs(v.x) + fabs(v.y);
+            if (m > best) { best = m; bi = i; }
+          }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ob = __shfl_xor(best, off);
+          const int oi = __shfl_xor(bi, off);
+          if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        if (lane == 0) {
+          const double2 piv = s_pan[q * n + bi];
+          const double den = piv.x * piv.x + piv.y * piv.y;
+          const bool ok = best > 0.0 && den > 0.0;
+          s_ip = ok ? make_double2(piv.x / den, -piv.y / den) : make_double2(0.0, 0.0);
+          if (!ok) singular[0] = 1;
+          s_p = bi;
+          s_used[bi] = 1;
+          s_perm[k] = bi;
+          s_pinv[bi] = k;
+          s_plist[q] = bi;
+        }
+      }
+      __syncthreads();
+      const int p = s_p;
+      const double2 ip = s_ip;
+      double2 nv[EP];
+#pragma unroll
+      for (int r = 0; r < EP; ++r) {
+        const int e = tid + NT * r;
+        if (e < pb * n) {
+          const int qq = e / n, i = e - qq * n;
+          const double2 ci = (i == p) ? make_double2(-1.0, 0.0) : s_pan[q * n + i];
+          const double2 rs = (qq == q) ? ip : cmul(s_pan[qq * n + p], ip);
+          const double2 base = (i == p || qq == q) ? make_double2(0.0, 0.0) : s_pan[e];
+          nv[r] = make_double2(base.x - (ci.x * rs.x - ci.y * rs.y), base.y - (ci.x * rs.y + ci.y * rs.x));
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < EP; ++r) {
+        const int e = tid + NT * r;
+        if (e < pb * n) s_pan[e] = nv[r];
+      }
+      __syncthreads();
+    }
+    // old pivot rows outside the panel, then the panel's own columns back
+    for (int e = tid; e < pb * n; e += NT) {
+      const int q = e / n, j = e - q * n;
+      if (j < c0 || j >= c0 + pb) s_prow[e] = A[int64_t(j) * n + s_plist[q]];
+    }
+    for (int e = tid; e < pb * n; e += NT) A[int64_t(c0) * n + e] = s_pan[e];
+    __syncthreads();
+    // rank-pb update of every other column, one column per wavefront at a time
+    double2 d[MAXN / 64][PB];
+    bool inP[MAXN / 64];
+#pragma unroll
+    for (int a = 0; a < MAXN / 64; ++a) {
+      const int i = lane + 64 * a;
+      inP[a] = i < n && s_used[i] && s_pinv[i] >= c0;
+#pragma unroll
+      for (int q = 0; q < PB; ++q) d[a][q] = (i < n && q < pb) ? s_pan[q * n + i] : make_double2(0.0, 0.0);
+    }
+    for (int j = wv; j < n; j += NT / 64) {
+      if (j >= c0 && j < c0 + pb) continue;
+      double2 acc[MAXN / 64];
+#pragma unroll
+      for (int a = 0; a < MAXN / 64; ++a) {
+        const int i = lane + 64 * a;
+        acc[a] = (i < n && !inP[a]) ? A[int64_t(j) * n + i] : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int q = 0; q < PB; ++q) {
+        if (q < pb) {
+          const double2 pr = s_prow[q * n + j];   // same address in every lane
+#pragma unroll
+          for (int a = 0; a < MAXN / 64; ++a) {
+            acc[a].x += d[a][q].x * pr.x - d[a][q].y * pr.y;
+            acc[a].y += d[a][q].x * pr.y + d[a][q].y * pr.x;
+          }
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < MAXN / 64; ++a) {
+        const int i = lane + 64 * a;
+        if (i < n) A[int64_t(j) * n + i] = acc[a];
+      }
+    }
+    __syncthreads();
+  }
+  // permutations: follow the cycles of perm over whole columns
+  double2* bufA = s_pan;
+  double2* bufB = s_prow;
+  int* s_seen = s_used;   // reused: 0 = column not yet placed
+  for (int i = tid; i < n; i += NT) s_seen[i] = 0;
+  __syncthreads();
+  for (int j0 = 0; j0 < n; ++j0) {
+    if (s_seen[j0]) continue;   // uniform: read after a barrier
+    for (int i = tid; i < n; i += NT) bufA[s_pinv[i]] = A[int64_t(j0) * n + i];
+    __syncthreads();
+    int cur = j0;
+    do {
+      const int nxt = s_perm[cur];
+      if (nxt != j0)
+        for (int i = tid; i < n; i += NT) bufB[s_pinv[i]] = A[int64_t(nxt) * n + i];
+      __syncthreads();
+      for (int i = tid; i < n; i += NT) A[int64_t(nxt) * n + i] = bufA[i];
+      if (tid == 0) s_seen[nxt] = 1;
+      __syncthreads();
+      double2* t = bufA;
+      bufA = bufB;
+      bufB = t;
+      cur = nxt;
+    } while (cur != j0);
+  }
+}
+
+constexpr int kInvPanel = 12;   // panel width of k_pc_invert_panel
+
+// Kernel choice: E elements per thread side with ceil(n/E)^2 <= NT for the
+// register kernel (1..6 with 256 threads, 7 = 6 with 512), 8 = panel kernel.
+static int invert_reg_shape(int n, int& nt) {
+  for (int e = 1; e <= 6; ++e) {
+    const int tr = (n + e - 1) / e;
+    if (tr * tr <= 256) { nt = 256; return e; }
+  }
+  if (n <= 132) { nt = 512; return 6; }   // ceil(132/6)^2 = 484
+  if (n <= 192) return 8;                  // panel kernel
+  return 0;
+}
+
 void invert_blocks(double2* blk, int64_t nblocks, int n, hipStream_t s) {
   VP_REQUIRE(n <= 256, ERR_UNSUPPORTED, "block inversion supports blocks up to 256 x 256");
   DevBuf flag(sizeof(int));
   VP_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+  int nt = 256;
+  const char* ev = std::getenv("VP_PC_INVERT_REG");   // 0: the global-memory kernel for every size
+  const int e = (ev && std::atoi(ev) == 0) ? 0 : invert_reg_shape(n, nt);
   for (int64_t b0 = 0; b0 < nblocks; b0 += 65535) {
     const int nb = int(std::min<int64_t>(65535, nblocks - b0));
-    hipLaunchKernelGGL(k_pc_invert, dim3(nb), dim3(256), 0, s, blk + b0 * n * n, n, flag.as<int>());
+    double2* p = blk + b0 * n * n;
+    switch (nt == 512 && e == 6 ? 7 : e) {
+      case 1: hipLaunchKernelGGL((k_pc_invert_reg<256, 1>), dim3(nb), dim3(256), 0, s, p, n, flag.as<int>()); break;
+      case 2: hipLaunchKernelGGL((k_pc_invert_reg<256, 2>), dim3(nb), dim3(256), 0, s, p, n, flag.as<int>()); break;
+      case 3: hipLaunchKernelGGL((k_pc_invert_reg<256, 3>), dim3(nb), dim3(256), 0, s, p, n, flag.as<int>()); break;
+      case 4: hipLaunchKernelGGL((k_pc_invert_reg<256, 4>), dim3(nb), dim3(256), 0, s, p, n, flag.as<int>()); break;
+      case 5: hipLaunchKernelGGL((k_pc_invert_reg<256, 5>), dim3(nb), dim3(256), 0, s, p, n, flag.as<int>()); break;
+      case 6: hipLaunchKernelGGL((k_pc_invert_reg<256, 6>), dim3(nb), dim3(256), 0, s, p, n, flag.as<int>()); break;
+      case 7: hipLaunchKernelGGL((k_pc_invert_reg<512, 6>), dim3(nb), dim3(512), 0, s, p, n, flag.as<int>()); break;
+      case 8: {
+        const void* fn = reinterpret_cast<const void*>(&k_pc_invert_panel<kInvPanel>);
+        const int lds = 2 * kInvPanel * n * int(sizeof(double2));
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        hipLaunchKernelGGL((k_pc_invert_panel<kInvPanel>), dim3(nb), dim3(256), lds, s, p, n, flag.as<int>());
+        break;
+      }
+      default: hipLaunchKernelGGL(k_pc_invert, dim3(nb), dim3(256), 0, s, p, n, flag.as<int>()); break;
+    }
     VP_HIP(hipGetLastError());
   }
   int h = 0;
