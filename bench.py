@@ -63,7 +63,7 @@ WORKLOADS = {
              'lambda U[1500,1600] nm: 40% straight waveguides of length U[2,8] um, 40% Bragg gratings of '
              'U{20..150} periods, 20% microdisks of radius U[2,5] um (220 nm thick); requests admitted by the '
              'server HBM budget, refinement rounds batched per geometry',
-        tol=1e-6, restart=50, inner_tol=1e-2, inner_maxit=4, max_outer=20, nreq=40, rate=50.0, seed=4,
+        tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=4, max_outer=20, nreq=40, rate=50.0, seed=4,
         budget_gb=96, max_active=32),
     'bragg_20nm': dict(
         desc='Bragg grating (100 periods) at 20 nm voxels, grid 1800x50x22 (FFT 3600x100x44), lambda 1550 nm',
@@ -772,7 +772,7 @@ def run_serve(args, wl, log):
         'data': 'synthetic (seeded request stream; device-assembled Green tensors)',
         'config': {'workload': args.workload, 'description': wl['desc'], 'requests': len(done), 'mix': kinds,
                    'unknowns_min': int(min(r['n'] for r in done)), 'unknowns_max': int(max(r['n'] for r in done)),
-                   'tol': wl['tol'], 'restart': f"{wl['restart']} (disks 100)", 'hbm_budget_gb': wl['budget_gb'],
+                   'tol': wl['tol'], 'restart': wl['restart'], 'hbm_budget_gb': wl['budget_gb'],
                    'max_active': wl['max_active'], 'parallelism': 'single GPU'},
         'relres_worst': worst, 'all_converged': all(r['info']['flag'] == 0 for r in done),
         'latency_ms': {'p50': round(float(np.percentile(lat, 50)), 3), 'p99': round(float(np.percentile(lat, 99)), 3),

@@ -279,6 +279,44 @@ __device__ __forceinline__ C<T> twiddle(const C<T>* __restrict__ tw, int idx) {
   return w;
 }
 
+// Twiddle w^idx, w = exp(-2 pi i / n), from two short FP64 factor tables in
+// LDS, lo[a] = w^a (a < 64) and hi[b] = w^(64 b): w^idx = lo[idx % 64] *
+// hi[idx / 64], formed in FP64 and rounded once to T, so FP32 twiddles equal
+// the correctly rounded table entries.  fill() evaluates the (64 +
+// ceil(n/64)) entries with sincospi; every thread of the workgroup calls it
+// and a barrier must follow.
+template <typename T>
+struct TwSplit {
+  const double2* lo;
+  const double2* hi;
+  __device__ __forceinline__ C<T> operator()(int idx) const {
+    const double2 a = lo[idx & 63], b = hi[idx >> 6];
+    C<T> w;
+    w.x = T(a.x * b.x - a.y * b.y);
+    w.y = T(a.x * b.y + a.y * b.x);
+    return w;
+  }
+  static constexpr int entries(int n) { return 64 + (n + 63) / 64; }   // double2 entries
+  __device__ static TwSplit fill(void* lds, int n) {
+    double2* t = reinterpret_cast<double2*>(lds);
+    const int nhi = (n + 63) / 64;
+    for (int e = threadIdx.x; e < 64 + nhi; e += blockDim.x) {
+      const int k = e < 64 ? e : 64 * (e - 64);
+      double sv, cv;
+      sincospi(2.0 * double(k) / double(n), &sv, &cv);
+      t[e] = make_double2(cv, -sv);
+    }
+    return TwSplit{t, t + 64};
+  }
+};
+
+template <typename T, bool INV>
+__device__ __forceinline__ C<T> twiddle(const TwSplit<T>& tw, int idx) {
+  C<T> w = tw(idx);
+  if (INV) w.y = -w.y;
+  return w;
+}
+
 // One Stockham stage from ``src`` to ``dst`` (distinct LDS buffers with the
 // same layout).  Each thread loops over butterflies, so only R values are
 // live at a time; the caller synchronises between stages.
@@ -509,9 +547,8 @@ __device__ C<T>* lds_fft(C<T>* a, C<T>* b, const FftPlan& pl, int nbatch, const 
 // b-fastest there).  tw[k * twm] = exp(-2 pi i k / NLEN).
 // ---------------------------------------------------------------------------
 template <typename T, bool INV, int NLEN, int NB, int LD, int NT, int NS, int R, bool FIRST, bool LAST, bool BF,
-          class Ld, class St>
-__device__ __forceinline__ void ct_stage(const C<T>* src, C<T>* dst, const C<T>* __restrict__ tw, int twm, Ld& ld,
-                                         St& st) {
+          class Tw, class Ld, class St>
+__device__ __forceinline__ void ct_stage(const C<T>* src, C<T>* dst, const Tw& tw, int twm, Ld& ld, St& st) {
   constexpr int NBF = NLEN / R;
   constexpr int TOTAL = NBF * NB;
 #pragma unroll
@@ -545,8 +582,8 @@ __device__ __forceinline__ void ct_stage(const C<T>* src, C<T>* dst, const C<T>*
 // between stages.  Returns the LDS buffer holding the result when LDSOUT
 // (after a barrier), else nullptr.
 template <typename T, bool INV, int NLEN, int NB, int LD, int NT, int NS, bool FIRST, bool FB, bool LB, bool LDSOUT,
-          int R0, int... Rs, class Ld, class St>
-__device__ __forceinline__ C<T>* ct_fft(C<T>* a, C<T>* b, const C<T>* __restrict__ tw, int twm, Ld& ld, St& st) {
+          int R0, int... Rs, class Tw, class Ld, class St>
+__device__ __forceinline__ C<T>* ct_fft(C<T>* a, C<T>* b, const Tw& tw, int twm, Ld& ld, St& st) {
   constexpr bool END = sizeof...(Rs) == 0;
   constexpr bool BF = FIRST ? FB : (END && !LDSOUT ? LB : false);
   ct_stage<T, INV, NLEN, NB, LD, NT, NS, R0, FIRST, END && !LDSOUT, BF>(a, b, tw, twm, ld, st);

@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(256) k_ypass(const ProbPtrs<T>* __restrict__ p
 // writes the S-point DFT.
 template <typename T, int NT, int... Rs>
 __global__ void __launch_bounds__(NT) k_xfwd_ct(const ProbPtrs<T>* __restrict__ probs, VecArgs<T> va, Geometry g,
-                                                int transp, const C<T>* __restrict__ twx) {
+                                                int transp, const C<T>* __restrict__ twx, int twlds) {
   constexpr int S = (Rs * ...);
   if (va.active && !va.active[blockIdx.y]) return;
   const ProbPtrs<T> P = probs[blockIdx.y];
@@ -268,7 +268,15 @@ __global__ void __launch_bounds__(NT) k_xfwd_ct(const ProbPtrs<T>* __restrict__ 
   };
   const BufRes Wo = buf_res(P.work + int64_t(pencil) * g.x.P + int64_t(px) * S, uint32_t(S) * uint32_t(sizeof(C<T>)));
   auto st = [&](int, int k, C<T> v) { buf_store(Wo, uint32_t(k), v); };
-  ct_fft<T, false, S, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, W1, twx, R, ld, st);
+  if (twlds) {
+    // stage twiddles from two short FP64 factor tables in LDS instead of
+    // the Lp-entry global table
+    const TwSplit<T> tws = TwSplit<T>::fill(W1 + S, S * R);
+    __syncthreads();
+    ct_fft<T, false, S, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, W1, tws, R, ld, st);
+  } else {
+    ct_fft<T, false, S, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, W1, twx, R, ld, st);
+  }
 }
 
 // Inverse: one workgroup per pencil runs the R class transforms, folds
@@ -276,7 +284,7 @@ __global__ void __launch_bounds__(NT) k_xfwd_ct(const ProbPtrs<T>* __restrict__ 
 // the epilogue Gram*Mr.*x - Mc.*N(x) (or its 'transp' form) per unknown.
 template <typename T, int NT, int... Rs>
 __global__ void __launch_bounds__(NT) k_xinv_ct(const ProbPtrs<T>* __restrict__ probs, VecArgs<T> va, Geometry g,
-                                                int transp, const C<T>* __restrict__ twx) {
+                                                int transp, const C<T>* __restrict__ twx, int twlds) {
   constexpr int S = (Rs * ...);
   constexpr int EPT = (S + NT - 1) / NT;
   if (va.active && !va.active[blockIdx.y]) return;
@@ -294,10 +302,16 @@ __global__ void __launch_bounds__(NT) k_xinv_ct(const ProbPtrs<T>* __restrict__ 
 #pragma unroll
   for (int e = 0; e < EPT; ++e) acc[e] = czero<T>();
   auto none = [](int, int, C<T>) {};
+  TwSplit<T> tws{};
+  if (twlds) {
+    tws = TwSplit<T>::fill(W1 + S, S * R);
+    __syncthreads();
+  }
   for (int px = 0; px < R; ++px) {
     const BufRes Wi = buf_res(P.work + int64_t(pencil) * g.x.P + int64_t(px) * S, uint32_t(S) * uint32_t(sizeof(C<T>)));
     auto ld = [&](int, int k) { return buf_load<T>(Wi, uint32_t(k)); };
-    const C<T>* Rb = ct_fft<T, true, S, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, W1, twx, R, ld, none);
+    const C<T>* Rb = twlds ? ct_fft<T, true, S, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, W1, tws, R, ld, none)
+                           : ct_fft<T, true, S, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, W1, twx, R, ld, none);
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int l = int(threadIdx.x) + e * NT;
@@ -923,17 +937,23 @@ static void launch_xpass_ct(const Geometry& g, const MatvecPlan& mp, const ProbP
   const unsigned npen = unsigned(3 * g.N * g.M);
   const dim3 grid(INV ? npen : npen * unsigned(g.x.R), nprob);
   bool done = false;
+  static const int twlds = [] {   // VP_XTW_LDS=0: stage twiddles from the global table
+    const char* e = std::getenv("VP_XTW_LDS");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
 #define VP_XCT_LAUNCH(SL, NTF, FR, NTI, IR)                                                             \
   if (!done && mp.xct == SL) {                                                                          \
-    const size_t lds = 2 * size_t(SL) * sizeof(C<T>);                                                   \
+    const size_t lds = 2 * size_t(SL) * sizeof(C<T>) + TwSplit<T>::entries(SL * g.x.R) * sizeof(double2); \
     if (INV) {                                                                                          \
       const void* fn = reinterpret_cast<const void*>(&k_xinv_ct<T, NTI, VP_UNPAREN IR>);                \
       (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));              \
-      hipLaunchKernelGGL((k_xinv_ct<T, NTI, VP_UNPAREN IR>), grid, dim3(NTI), lds, s, d_probs, va, g, transp, twx); \
+      hipLaunchKernelGGL((k_xinv_ct<T, NTI, VP_UNPAREN IR>), grid, dim3(NTI), lds, s, d_probs, va, g, transp, twx, \
+                         twlds);                                                                        \
     } else {                                                                                            \
       const void* fn = reinterpret_cast<const void*>(&k_xfwd_ct<T, NTF, VP_UNPAREN FR>);                \
       (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));              \
-      hipLaunchKernelGGL((k_xfwd_ct<T, NTF, VP_UNPAREN FR>), grid, dim3(NTF), lds, s, d_probs, va, g, transp, twx); \
+      hipLaunchKernelGGL((k_xfwd_ct<T, NTF, VP_UNPAREN FR>), grid, dim3(NTF), lds, s, d_probs, va, g, transp, twx, \
+                         twlds);                                                                        \
     }                                                                                                   \
     done = true;                                                                                        \
   }

@@ -367,7 +367,7 @@ __device__ __forceinline__ int64_t pc_unknown(const PcArgs<T>& a, const PcGeom& 
 }
 
 template <typename T, int NT, int... Rs>
-__global__ void __launch_bounds__(NT) k_pc_xfwd_ct(PcArgs<T> a, PcGeom g) {
+__global__ void __launch_bounds__(NT) k_pc_xfwd_ct(PcArgs<T> a, PcGeom g, int twlds) {
   constexpr int L = (Rs * ...);
   if (a.active && !a.active[blockIdx.y]) return;
   const C<T>* __restrict__ X = a.x + int64_t(blockIdx.y) * a.xstride;
@@ -385,11 +385,17 @@ __global__ void __launch_bounds__(NT) k_pc_xfwd_ct(PcArgs<T> a, PcGeom g) {
   };
   const BufRes Wo = buf_res(work + int64_t(p) * L, uint32_t(L) * uint32_t(sizeof(C<T>)));
   auto st = [&](int, int k, C<T> v) { buf_store(Wo, uint32_t(k), v); };
-  ct_fft<T, false, L, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, W1, a.twx, 1, ld, st);
+  if (twlds) {   // stage twiddles from two short FP64 factor tables in LDS
+    const TwSplit<T> tws = TwSplit<T>::fill(W1 + L, L);
+    __syncthreads();
+    ct_fft<T, false, L, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, W1, tws, 1, ld, st);
+  } else {
+    ct_fft<T, false, L, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, W1, a.twx, 1, ld, st);
+  }
 }
 
 template <typename T, int NT, int... Rs>
-__global__ void __launch_bounds__(NT) k_pc_xinv_ct(PcArgs<T> a, PcGeom g) {
+__global__ void __launch_bounds__(NT) k_pc_xinv_ct(PcArgs<T> a, PcGeom g, int twlds) {
   constexpr int L = (Rs * ...);
   if (a.active && !a.active[blockIdx.y]) return;
   C<T>* __restrict__ Y = a.y + int64_t(blockIdx.y) * a.ystride;
@@ -404,7 +410,14 @@ __global__ void __launch_bounds__(NT) k_pc_xinv_ct(PcArgs<T> a, PcGeom g) {
   const BufRes Wi = buf_res(work + int64_t(p) * L, uint32_t(L) * uint32_t(sizeof(C<T>)));
   auto ld = [&](int, int k) { return buf_load<T>(Wi, uint32_t(k)); };
   auto none = [](int, int, C<T>) {};
-  const C<T>* R = ct_fft<T, true, L, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, W1, a.twx, 1, ld, none);
+  const C<T>* R;
+  if (twlds) {
+    const TwSplit<T> tws = TwSplit<T>::fill(W1 + L, L);
+    __syncthreads();
+    R = ct_fft<T, true, L, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, W1, tws, 1, ld, none);
+  } else {
+    R = ct_fft<T, true, L, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, W1, a.twx, 1, ld, none);
+  }
   for (int l = threadIdx.x; l < L; l += NT) {
     const int64_t j = pc_unknown(a, g, l + int64_t(L) * (m + int64_t(M) * n), c);
     if (j >= 0) Y[pc_outer(a, j)] = R[l];
@@ -484,7 +497,11 @@ template <typename T>
 static void launch_prec_pct(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) {
   constexpr bool F64 = sizeof(T) == 8;
   const unsigned npen = unsigned(3 * g.N * g.M);
-  const size_t ldsx = 2 * size_t(g.L) * sizeof(C<T>);
+  const size_t ldsx = 2 * size_t(g.L) * sizeof(C<T>) + TwSplit<T>::entries(g.L) * sizeof(double2);
+  static const int twlds = [] {   // VP_XTW_LDS=0: stage twiddles from the global table
+    const char* e = std::getenv("VP_XTW_LDS");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
   auto xpass = [&](bool inv) {
     bool done = false;
 #define VP_PCX(SL, NTF, FR, NTI, IR)                                                                      \
@@ -492,11 +509,11 @@ static void launch_prec_pct(const PcGeom& g, const PcArgs<T>& a, int nprob, hipS
     if (inv) {                                                                                          \
       const void* fn = reinterpret_cast<const void*>(&k_pc_xinv_ct<T, NTI, VP_UNPAREN IR>);             \
       (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsx));             \
-      hipLaunchKernelGGL((k_pc_xinv_ct<T, NTI, VP_UNPAREN IR>), dim3(npen, nprob), dim3(NTI), ldsx, s, a, g); \
+      hipLaunchKernelGGL((k_pc_xinv_ct<T, NTI, VP_UNPAREN IR>), dim3(npen, nprob), dim3(NTI), ldsx, s, a, g, twlds); \
     } else {                                                                                            \
       const void* fn = reinterpret_cast<const void*>(&k_pc_xfwd_ct<T, NTF, VP_UNPAREN FR>);             \
       (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsx));             \
-      hipLaunchKernelGGL((k_pc_xfwd_ct<T, NTF, VP_UNPAREN FR>), dim3(npen, nprob), dim3(NTF), ldsx, s, a, g); \
+      hipLaunchKernelGGL((k_pc_xfwd_ct<T, NTF, VP_UNPAREN FR>), dim3(npen, nprob), dim3(NTF), ldsx, s, a, g, twlds); \
     }                                                                                                   \
     done = true;                                                                                        \
   }
