@@ -160,3 +160,22 @@ def test_matvec_fN_with_and_without_parity_structure(vp_ctx, monkeypatch, varian
             e32 = rel(op.apply(x, flag, 'fp32'), ref)
             print(variant, sym, GG is G, flag, e64, e32, flush=True)
             assert e64 < 1e-12 and e32 < 1e-5
+
+
+@pytest.mark.parametrize('pair', ['1', '0'])
+@pytest.mark.parametrize('shape,fft', [((30, 6, 3), (3600, 0, 0)), ((20, 6, 3), (7200, 0, 0))])
+def test_matvec_long_x_classes(vp_ctx, monkeypatch, shape, fft, pair):
+    # x lines zero padded to 3600 and 7200 points: compile-time class
+    # transforms of 1800 and 3600 points; the FP32 inverse runs both classes
+    # of a pencil as one batch (VP_XINV_PAIR=1) or one after the other
+    from voxphoton_amd.vie import NOperator
+    monkeypatch.setenv('VP_MATVEC_VARIANT', 'split')
+    monkeypatch.setenv('VP_XINV_PAIR', pair)
+    G, Mr, Mc, Gram, idx, x = random_problem(*shape, seed=77, with_mr=True)
+    A = ovie.dense_AN(G, Mr, Mc, Gram, idx)
+    op = NOperator.from_toeplitz(G, Mr, Mc, Gram, idx, fft_shape=fft, ctx=vp_ctx)
+    for flag, ref in (('notransp', A @ x), ('transp', A.conj().T @ x)):
+        e64 = rel(op.apply(x, flag, 'fp64'), ref)
+        e32 = rel(op.apply(x, flag, 'fp32'), ref)
+        print(shape, fft, 'pair', pair, flag, 'rel', e64, e32, flush=True)
+        assert e64 < 1e-12 and e32 < 1e-5

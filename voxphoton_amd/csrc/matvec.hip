@@ -279,6 +279,29 @@ __global__ void __launch_bounds__(NT) k_xfwd_ct(const ProbPtrs<T>* __restrict__ 
   }
 }
 
+// x-inverse epilogue for box voxel (l, m, n), component c: the unknown's
+// Gram*Mr.*x - Mc.*N(x) (or its 'transp' form, conjugated) from N(x) = nx.
+template <typename T>
+__device__ __forceinline__ void xinv_epilogue(const ProbPtrs<T>& P, const Geometry& g, const C<T>* __restrict__ X,
+                                              C<T>* __restrict__ Y, int l, int m, int n, int c, int transp,
+                                              C<T> nx) {
+  const int64_t v = g.voxel(l, m, n);
+  int64_t pv, u;
+  if (g.structured) {
+    pv = P.pos[v];
+    u = pv < 0 ? -1 : pv + int64_t(c) * g.nS;
+  } else {
+    pv = P.pos[int64_t(c) * g.V + v];
+    u = pv;
+  }
+  if (u >= 0) {
+    const C<T> xv = X[u];
+    C<T> id = cscale(xv, P.gram);
+    if (P.mr) id = transp ? cmulc(id, P.mr[pv]) : cmul(id, P.mr[pv]);
+    Y[u] = transp ? csub(id, cconj(nx)) : csub(id, cmul(P.mc[pv], nx));
+  }
+}
+
 // Inverse: one workgroup per pencil runs the R class transforms, folds
 // them back with conj(w_Lp^{l px}) in registers, crops to l < L and applies
 // the epilogue Gram*Mr.*x - Mc.*N(x) (or its 'transp' form) per unknown.
@@ -326,22 +349,41 @@ __global__ void __launch_bounds__(NT) k_xinv_ct(const ProbPtrs<T>* __restrict__ 
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int l = int(threadIdx.x) + e * NT;
+    if (l < L) xinv_epilogue(P, g, X, Y, l, m, n, c, transp, acc[e]);
+  }
+}
+
+// Same for two residue classes (R = 2, FP32): both class transforms run as
+// one batch of two in LDS (element k of class b at 2k + b), so a workgroup
+// takes the stages once with twice the butterflies per stage.
+template <typename T, int NT, int... Rs>
+__global__ void __launch_bounds__(NT) k_xinv_ct2(const ProbPtrs<T>* __restrict__ probs, VecArgs<T> va, Geometry g,
+                                                 int transp, const C<T>* __restrict__ twx) {
+  constexpr int S = (Rs * ...);
+  constexpr int EPT = (S + NT - 1) / NT;
+  if (va.active && !va.active[blockIdx.y]) return;
+  const ProbPtrs<T> P = probs[blockIdx.y];
+  const C<T>* __restrict__ X = va.x + int64_t(blockIdx.y) * va.xstride;
+  C<T>* __restrict__ Y = va.y + int64_t(blockIdx.y) * va.ystride;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
+  C<T>* W1 = W0 + 2 * S;
+  const int L = g.L, M = g.M, N = g.N;
+  const int pencil = xcd_remap(blockIdx.x, gridDim.x);
+  const int cn = pencil / M, m = pencil - cn * M;
+  const int c = cn / N, n = cn - c * N;
+  const TwSplit<T> tws = TwSplit<T>::fill(W1 + 2 * S, 2 * S);
+  __syncthreads();
+  const BufRes Wi = buf_res(P.work + int64_t(pencil) * g.x.P, uint32_t(2 * S) * uint32_t(sizeof(C<T>)));
+  auto ld = [&](int b, int k) { return buf_load<T>(Wi, uint32_t(b * S + k)); };
+  auto none = [](int, int, C<T>) {};
+  const C<T>* Rb = ct_fft<T, true, S, 2, 2, NT, 1, true, false, false, true, Rs...>(W0, W1, tws, 2, ld, none);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int l = int(threadIdx.x) + e * NT;
     if (l < L) {
-      const int64_t v = g.voxel(l, m, n);
-      int64_t pv, u;
-      if (g.structured) {
-        pv = P.pos[v];
-        u = pv < 0 ? -1 : pv + int64_t(c) * g.nS;
-      } else {
-        pv = P.pos[int64_t(c) * g.V + v];
-        u = pv;
-      }
-      if (u >= 0) {
-        const C<T> xv = X[u];
-        C<T> id = cscale(xv, P.gram);
-        if (P.mr) id = transp ? cmulc(id, P.mr[pv]) : cmul(id, P.mr[pv]);
-        Y[u] = transp ? csub(id, cconj(acc[e])) : csub(id, cmul(P.mc[pv], acc[e]));
-      }
+      const C<T> t = cadd(Rb[2 * l], cmulc(Rb[2 * l + 1], twx[l]));
+      xinv_epilogue(P, g, X, Y, l, m, n, c, transp, t);
     }
   }
 }
@@ -937,10 +979,31 @@ static void launch_xpass_ct(const Geometry& g, const MatvecPlan& mp, const ProbP
   const unsigned npen = unsigned(3 * g.N * g.M);
   const dim3 grid(INV ? npen : npen * unsigned(g.x.R), nprob);
   bool done = false;
-  static const int twlds = [] {   // VP_XTW_LDS=0: stage twiddles from the global table
+  const int twlds = [] {   // VP_XTW_LDS=0: stage twiddles from the global table (read per launch)
     const char* e = std::getenv("VP_XTW_LDS");
     return (e && e[0] == '0') ? 0 : 1;
   }();
+  // FP32, two residue classes: both class transforms of a pencil in one batch
+  const int xpair = [] {   // VP_XINV_PAIR=0: one class transform after the other (read per launch)
+    const char* e = std::getenv("VP_XINV_PAIR");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
+  if constexpr (sizeof(T) == 4) {
+    if (INV && xpair && g.x.R == 2) {
+#define VP_XPAIR_LAUNCH(SL, NTP, PR)                                                                    \
+  if (!done && mp.xct == SL &&                                                                          \
+      4 * size_t(SL) * sizeof(C<T>) + TwSplit<T>::entries(2 * SL) * sizeof(double2) <= 160 * 1024) {    \
+    const size_t lds = 4 * size_t(SL) * sizeof(C<T>) + TwSplit<T>::entries(2 * SL) * sizeof(double2);    \
+    const void* fn = reinterpret_cast<const void*>(&k_xinv_ct2<T, NTP, VP_UNPAREN PR>);                 \
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));                \
+    hipLaunchKernelGGL((k_xinv_ct2<T, NTP, VP_UNPAREN PR>), grid, dim3(NTP), lds, s, d_probs, va, g, transp, twx); \
+    done = true;                                                                                        \
+  }
+      VP_XPAIR_LAUNCH(3600, 1024, (16, 9, 5, 5))
+      VP_XPAIR_LAUNCH(1800, 768, (8, 9, 5, 5))
+#undef VP_XPAIR_LAUNCH
+    }
+  }
 #define VP_XCT_LAUNCH(SL, NTF, FR, NTI, IR)                                                             \
   if (!done && mp.xct == SL) {                                                                          \
     const size_t lds = 2 * size_t(SL) * sizeof(C<T>) + TwSplit<T>::entries(SL * g.x.R) * sizeof(double2); \

@@ -498,7 +498,7 @@ static void launch_prec_pct(const PcGeom& g, const PcArgs<T>& a, int nprob, hipS
   constexpr bool F64 = sizeof(T) == 8;
   const unsigned npen = unsigned(3 * g.N * g.M);
   const size_t ldsx = 2 * size_t(g.L) * sizeof(C<T>) + TwSplit<T>::entries(g.L) * sizeof(double2);
-  static const int twlds = [] {   // VP_XTW_LDS=0: stage twiddles from the global table
+  const int twlds = [] {   // VP_XTW_LDS=0: stage twiddles from the global table (read per launch)
     const char* e = std::getenv("VP_XTW_LDS");
     return (e && e[0] == '0') ? 0 : 1;
   }();
