@@ -23,6 +23,15 @@ namespace vp {
 // ----------------------------------------------------------------------------
 // Apply kernels
 // ----------------------------------------------------------------------------
+// acc += (br + i bi) * x as four FMAs
+template <typename T>
+__device__ __forceinline__ void cfma_acc(C<T>& acc, T br, T bi, C<T> x) {
+  acc.x = fma(br, x.x, acc.x);
+  acc.x = fma(-bi, x.y, acc.x);
+  acc.y = fma(br, x.y, acc.y);
+  acc.y = fma(bi, x.x, acc.y);
+}
+
 template <typename T>
 __device__ __forceinline__ int64_t pc_outer(const PcArgs<T>& a, int64_t j) {
   return a.sub ? int64_t(a.sub[j]) : j;
@@ -246,16 +255,29 @@ __global__ void __launch_bounds__(256) k_pc_quadmul(PcArgs<T> a, PcGeom g) {
     C<T> acc0 = czero<T>(), acc1 = czero<T>(), acc2 = czero<T>(), acc3 = czero<T>();
     if constexpr (sizeof(T) == 4) {
       if (Hb) {
-        // binary16 blocks: half the bytes of the block stream
-        for (int q = 0; q < n3; ++q) {
-          const uint32_t hw = Hb[int64_t(q) * n3 + r];
-          const C<T> b = mk<T>(T(__builtin_bit_cast(_Float16, uint16_t(hw & 0xffffu))),
-                               T(__builtin_bit_cast(_Float16, uint16_t(hw >> 16))));
-          acc0 = cadd(acc0, cmul(b, Xs[0][q]));
-          acc1 = cadd(acc1, cmul(b, Xs[1][q]));
-          acc2 = cadd(acc2, cmul(b, Xs[2][q]));
-          acc3 = cadd(acc3, cmul(b, Xs[3][q]));
+        // binary16 blocks: half the bytes of the block stream.  Columns go in
+        // chunks of QU whose block loads are all issued before any is used;
+        // each entry is converted once and enters four complex
+        // multiply-accumulates of two FMAs per part.
+        auto step = [&](uint32_t hw, int q) {
+          T br = T(__builtin_bit_cast(_Float16, uint16_t(hw & 0xffffu)));
+          T bi = T(__builtin_bit_cast(_Float16, uint16_t(hw >> 16)));
+          asm volatile("" : "+v"(br), "+v"(bi));   // keep the conversion out of the FMAs
+          cfma_acc(acc0, br, bi, Xs[0][q]);
+          cfma_acc(acc1, br, bi, Xs[1][q]);
+          cfma_acc(acc2, br, bi, Xs[2][q]);
+          cfma_acc(acc3, br, bi, Xs[3][q]);
+        };
+        constexpr int QU = 8;
+        int q0 = 0;
+        for (; q0 + QU <= n3; q0 += QU) {
+          uint32_t hw[QU];
+#pragma unroll
+          for (int u = 0; u < QU; ++u) hw[u] = Hb[int64_t(q0 + u) * n3 + r];
+#pragma unroll
+          for (int u = 0; u < QU; ++u) step(hw[u], q0 + u);
         }
+        for (; q0 < n3; ++q0) step(Hb[int64_t(q0) * n3 + r], q0);
         const T hs = T(a.hscale[blockIdx.y]);
         acc0 = cscale(acc0, hs);
         acc1 = cscale(acc1, hs);
@@ -264,13 +286,22 @@ __global__ void __launch_bounds__(256) k_pc_quadmul(PcArgs<T> a, PcGeom g) {
       }
     }
     if (!Hb) {
-      for (int q = 0; q < n3; ++q) {
-        const C<T> b = Bb[int64_t(q) * n3 + r];
-        acc0 = cadd(acc0, cmul(b, Xs[0][q]));
-        acc1 = cadd(acc1, cmul(b, Xs[1][q]));
-        acc2 = cadd(acc2, cmul(b, Xs[2][q]));
-        acc3 = cadd(acc3, cmul(b, Xs[3][q]));
+      auto step = [&](C<T> b, int q) {
+        cfma_acc(acc0, b.x, b.y, Xs[0][q]);
+        cfma_acc(acc1, b.x, b.y, Xs[1][q]);
+        cfma_acc(acc2, b.x, b.y, Xs[2][q]);
+        cfma_acc(acc3, b.x, b.y, Xs[3][q]);
+      };
+      constexpr int QU = sizeof(T) == 8 ? 4 : 8;
+      int q0 = 0;
+      for (; q0 + QU <= n3; q0 += QU) {
+        C<T> bv[QU];
+#pragma unroll
+        for (int u = 0; u < QU; ++u) bv[u] = Bb[int64_t(q0 + u) * n3 + r];
+#pragma unroll
+        for (int u = 0; u < QU; ++u) step(bv[u], q0 + u);
       }
+      for (; q0 < n3; ++q0) step(Bb[int64_t(q0) * n3 + r], q0);
     }
     const int cr = r / N;
     const C<T> accs[4] = {acc0, acc1, acc2, acc3};
