@@ -25,6 +25,14 @@ __host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 __host__ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+// f0*a + f1*b + f2*c as one chain of FMAs per part (5 FMAs + 1 product)
+template <typename V>
+__host__ __device__ __forceinline__ V cdot3(V f0, V a, V f1, V b, V f2, V c) {
+  V r;
+  r.x = fma(f0.x, a.x, fma(-f0.y, a.y, fma(f1.x, b.x, fma(-f1.y, b.y, fma(f2.x, c.x, -f2.y * c.y)))));
+  r.y = fma(f0.x, a.y, fma(f0.y, a.x, fma(f1.x, b.y, fma(f1.y, b.x, fma(f2.x, c.y, f2.y * c.x)))));
+  return r;
+}
 // a * conj(b)
 __host__ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
   return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);

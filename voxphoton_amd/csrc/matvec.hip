@@ -645,9 +645,9 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 8 ? 2 : 1)) k_zfused(const 
           f3 = buf_load<T>(F, o + 3); f4 = buf_load<T>(F, o + 4); f5 = buf_load<T>(F, o + 5);
         }
         const C<T> a = x0[r], b = x1[r], cc = x2[r];
-        x0[r] = cadd(cadd(cmul(f0, a), cmul(f1, b)), cmul(f2, cc));
-        x1[r] = cadd(cadd(cmul(f1, a), cmul(f3, b)), cmul(f4, cc));
-        x2[r] = cadd(cadd(cmul(f2, a), cmul(f4, b)), cmul(f5, cc));
+        x0[r] = cdot3(f0, a, f1, b, f2, cc);
+        x1[r] = cdot3(f1, a, f3, b, f4, cc);
+        x2[r] = cdot3(f2, a, f4, b, f5, cc);
         // double precision: keep at most a quarter of the tensor loads in
         // flight so the registers fit two waves per SIMD without spilling
         if constexpr (sizeof(T) == 8 && RB >= 8) {
