@@ -15,6 +15,7 @@
 // Gram-Schmidt pass runs over [C V] (the C entries form B = C^H Aop v_k), and
 // the update is V y + U (a0 - B y) with Aop U = C (GCRO).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -212,6 +213,43 @@ __global__ void __launch_bounds__(KT) k_reduce_cols(const double2* __restrict__ 
   double2 acc[1] = {make_double2(0.0, 0.0)};
   for (int b = threadIdx.x; b < nblk; b += KT) { acc[0].x += src[b].x; acc[0].y += src[b].y; }
   block_reduce_store<1>(acc, out + int64_t(p) * maxcols + j, 1, 1);
+}
+
+// Publishes one Arnoldi step's estimates and convergence flags into coherent
+// pinned host memory, then (after a system-scope fence) the step's sequence
+// number, which the host polls.
+__global__ void k_publish(const double* __restrict__ est, const int* __restrict__ conv, int np, double* hest,
+                          int* hconv, int* hseq, int seq) {
+  for (int p = threadIdx.x; p < np; p += blockDim.x) {
+    reinterpret_cast<volatile double*>(hest)[p] = est[p];
+    reinterpret_cast<volatile int*>(hconv)[p] = conv[p];
+  }
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    reinterpret_cast<volatile int*>(hseq)[0] = seq;
+  }
+}
+
+// Spins until the device has published step ``seq``.  Past 200 ms of
+// spinning it waits on the stream instead and, should the published values
+// still not be visible, copies them from the device.
+static void wait_published(const int* hseq, int seq, hipStream_t s, double* hest, const double* est, int* hconv,
+                           const int* conv, int np) {
+  const volatile int* vs = hseq;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned spins = 1; *vs != seq; ++spins) {
+    if ((spins & 4095u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
+      VP_HIP(hipStreamSynchronize(s));
+      if (*vs != seq) {
+        VP_HIP(hipMemcpy(hest, est, np * sizeof(double), hipMemcpyDeviceToHost));
+        VP_HIP(hipMemcpy(hconv, conv, np * sizeof(int), hipMemcpyDeviceToHost));
+      }
+      break;
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
 }
 
 struct ArnoldiState {
@@ -654,11 +692,20 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     dev_norms<T>(b, n, np, n, bn.as<double>(), s);
   }
   std::vector<double> hbn(np), hbeta(np);
-  // per-step estimates and convergence flags come back through pinned host
-  // memory (a DMA per step instead of a staged copy)
-  PinnedBuf pin(np * (sizeof(double) + sizeof(int)));
+  // per-step estimates and convergence flags: the device publishes them into
+  // coherent pinned host memory, followed by the step's sequence number, and
+  // the host spins on that number (no copy kernels, no blocking wait)
+  PinnedBuf pin(np * (sizeof(double) + sizeof(int)) + sizeof(int), hipHostMallocMapped | hipHostMallocCoherent);
   double* hest = static_cast<double*>(pin.p);
   int* hconv = reinterpret_cast<int*>(hest + np);
+  int* hseq = hconv + np;
+  *reinterpret_cast<volatile int*>(hseq) = -1;
+  void* dpin = nullptr;
+  VP_HIP(hipHostGetDevicePointer(&dpin, pin.p, 0));
+  double* dhest = static_cast<double*>(dpin);
+  int* dhconv = reinterpret_cast<int*>(dhest + np);
+  int* dhseq = dhconv + np;
+  int seq = 0;
   VP_HIP(hipMemcpy(hbn.data(), bn.p, np * sizeof(double), hipMemcpyDeviceToHost));
   // initial residual into V_0 slot of each problem
   auto residual_into_V0 = [&](bool x_nonzero) {
@@ -878,9 +925,11 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       hipLaunchKernelGGL(k_arnoldi, dim3((np + 63) / 64), dim3(64), 0, s, st, k, dia, np, ract.as<int>());
       hipLaunchKernelGGL(k_scale<T>, dim3(grid1(n), np), dim3(256), 0, s, w, vprob, n, invn.as<double>(), 0, dia);
       VP_HIP(hipGetLastError());
-      VP_HIP(hipMemcpyAsync(hest, est.p, np * sizeof(double), hipMemcpyDeviceToHost, s));
-      VP_HIP(hipMemcpyAsync(hconv, conv.p, np * sizeof(int), hipMemcpyDeviceToHost, s));
-      VP_HIP(hipStreamSynchronize(s));
+      ++seq;
+      hipLaunchKernelGGL(k_publish, dim3(1), dim3(256), 0, s, est.as<double>(), conv.as<int>(), np, dhest, dhconv,
+                         dhseq, seq);
+      VP_HIP(hipGetLastError());
+      wait_published(hseq, seq, s, hest, est.as<double>(), hconv, conv.as<int>(), np);
       bool still = false, changed = false;
       for (int p = 0; p < np; ++p) {
         if (!inner[p]) continue;

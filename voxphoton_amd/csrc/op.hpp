@@ -43,7 +43,7 @@ struct DevBuf {
 // RAII pinned host buffer (hipHostMalloc).
 struct PinnedBuf {
   void* p = nullptr;
-  explicit PinnedBuf(size_t b) { VP_HIP(hipHostMalloc(&p, b, hipHostMallocDefault)); }
+  explicit PinnedBuf(size_t b, unsigned flags = hipHostMallocDefault) { VP_HIP(hipHostMalloc(&p, b, flags)); }
   ~PinnedBuf() {
     if (p) (void)hipHostFree(p);
   }
