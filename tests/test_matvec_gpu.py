@@ -72,13 +72,18 @@ def test_matvec_linearity_and_zero(vp_ctx):
     assert np.all(op.apply(np.zeros_like(x)) == 0)
 
 
-def test_assembly_matches_oracle(vp_ctx):
+@pytest.mark.parametrize('near', ['1', '0'])
+@pytest.mark.parametrize('shape', [(7, 6, 5), (3, 8, 2)])
+def test_assembly_matches_oracle(vp_ctx, monkeypatch, shape, near):
+    # near cells (every offset below 5) integrated by one workgroup each
+    # (VP_ASSEMBLE_NEAR=1) or by one thread each
     from voxphoton_amd.vie import assemble_toeplitz
+    monkeypatch.setenv('VP_ASSEMBLE_NEAR', near)
     dx, ko = 20e-9, 2 * np.pi * np.sqrt(2.085) / 1550e-9
-    Gd = assemble_toeplitz((7, 6, 5), dx, ko)
-    Go = ogr.assemble_G(7, 6, 5, dx, ko)
+    Gd = assemble_toeplitz(shape, dx, ko)
+    Go = ogr.assemble_G(*shape, dx, ko)
     err = np.abs(Gd - Go).max() / np.abs(Go).max()
-    print('assembly rel err', err, flush=True)
+    print(shape, 'near', near, 'assembly rel err', err, flush=True)
     assert err < 1e-12
 
 

@@ -2,6 +2,7 @@
 // circulant tensor layout, Toeplitz -> circulant spectrum, and assembly of the
 // Toeplitz Green tensor.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 #include "op.hpp"
@@ -391,7 +392,9 @@ __device__ __forceinline__ double2 kern(double2 kappa, double rho, double jac) {
   return make_double2(cs * f, -sn * f);
 }
 
-__device__ double2 pair_parallel(const int m[3], int a, int sp, int sq, double2 kappa, int rule) {
+// io >= 0 restricts the outermost quadrature index to io (a slice of the
+// sum, for spreading one cell's integrals over threads); io < 0 sums all.
+__device__ double2 pair_parallel(const int m[3], int a, int sp, int sq, double2 kappa, int rule, int io = -1) {
   const int b = (a == 0) ? 1 : 0;
   const int c = (a == 2) ? 1 : 2;
   const double h = double(sq - (m[a] + sp));
@@ -405,7 +408,7 @@ __device__ double2 pair_parallel(const int m[3], int a, int sp, int sq, double2 
         const double sg1 = (mb == al) ? 1.0 : -1.0;
         const double sg2 = (mc == be) ? 1.0 : -1.0;
         const int n = c_rules.n[0];
-        for (int i = 0; i < n; ++i) {
+        for (int i = io < 0 ? 0 : io, ie = io < 0 ? n : min(io + 1, n); i < ie; ++i) {
           const double xi = c_rules.x[0][i];
           for (int j = 0; j < n; ++j) {
             const double eta = c_rules.x[0][j];
@@ -424,7 +427,7 @@ __device__ double2 pair_parallel(const int m[3], int a, int sp, int sq, double2 
         }
       } else {
         const int n = c_rules.n[rule];
-        for (int i = 0; i < n; ++i) {
+        for (int i = io < 0 ? 0 : io, ie = io < 0 ? n : min(io + 1, n); i < ie; ++i) {
           const double s1 = al + c_rules.x[rule][i];
           const double d1 = s1 - mb;
           for (int j = 0; j < n; ++j) {
@@ -443,7 +446,7 @@ __device__ double2 pair_parallel(const int m[3], int a, int sp, int sq, double2 
   return tot;
 }
 
-__device__ double2 pair_perp(const int m[3], int ap, int sp, int aq, int sq, double2 kappa, int rule) {
+__device__ double2 pair_perp(const int m[3], int ap, int sp, int aq, int sq, double2 kappa, int rule, int io = -1) {
   const int c = 3 - ap - aq;
   const int ustar = m[ap] + sp;
   const int vstar = sq - m[aq];
@@ -454,7 +457,7 @@ __device__ double2 pair_perp(const int m[3], int ap, int sp, int aq, int sq, dou
     if (sing) {
       const double sgs = (sstar == ga) ? 1.0 : -1.0;
       const int n = c_rules.n[0];
-      for (int i = 0; i < n; ++i) {
+      for (int i = io < 0 ? 0 : io, ie = io < 0 ? n : min(io + 1, n); i < ie; ++i) {
         const double xi = c_rules.x[0][i];
         for (int j = 0; j < n; ++j) {
           const double eta = c_rules.x[0][j];
@@ -478,7 +481,7 @@ __device__ double2 pair_perp(const int m[3], int ap, int sp, int aq, int sq, dou
       }
     } else {
       const int n = c_rules.n[rule];
-      for (int i = 0; i < n; ++i) {
+      for (int i = io < 0 ? 0 : io, ie = io < 0 ? n : min(io + 1, n); i < ie; ++i) {
         const double d1 = c_rules.x[rule][i] - m[ap] - sp;
         for (int j = 0; j < n; ++j) {
           const double d2 = sq - m[aq] - c_rules.x[rule][j];
@@ -498,7 +501,62 @@ __device__ double2 pair_perp(const int m[3], int ap, int sp, int aq, int sq, dou
   return tot;
 }
 
-__global__ void k_assemble(double2* __restrict__ G, int L, int M, int N, double2 kappa, double scale) {
+// Cells with every offset below kNear use the singular and near rules; their
+// integrals cost orders of magnitude more than a far cell's.
+constexpr int kNear = 5;
+
+// One workgroup per near cell: threads take (face p, face q, outermost
+// quadrature index) items, and a fixed tree sums them.
+__global__ void __launch_bounds__(256) k_assemble_near(double2* __restrict__ G, int L, int M, int N, double2 kappa,
+                                                       double scale) {
+  const int nl = min(L, kNear), nm = min(M, kNear);
+  int b = blockIdx.x;
+  const int i = b % nl;
+  b /= nl;
+  const int j = b % nm;
+  const int l = b / nm;
+  const int m[3] = {i, j, l};
+  const int mx = max(i, max(j, l));
+  const int rule = mx <= 1 ? 1 : 2;
+  double acc[12];
+  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+  for (int it = threadIdx.x; it < 36 * 16; it += blockDim.x) {
+    const int io = it % 16, pq = it / 16;
+    const int p = pq / 6, q = pq - 6 * (pq / 6);
+    const int ap = p >> 1, sp = p & 1, aq = q >> 1, sq = q & 1;
+    bool any = false;
+    for (int k = 0; k < 6; ++k) any |= c_proj[p][q][k] != 0;
+    if (!any) continue;
+    const double2 I = (ap == aq) ? pair_parallel(m, ap, sp, sq, kappa, rule, io)
+                                 : pair_perp(m, ap, sp, aq, sq, kappa, rule, io);
+    for (int k = 0; k < 6; ++k) {
+      const double pr = c_proj[p][q][k];
+      acc[2 * k] += pr * I.x;
+      acc[2 * k + 1] += pr * I.y;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1)
+    for (int k = 0; k < 12; ++k) acc[k] += __shfl_xor(acc[k], off);
+  __shared__ double part[4][12];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 12; ++k) part[wv][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    double2 o = make_double2(0.0, 0.0);
+    for (int w = 0; w < 4; ++w) { o.x += part[w][2 * k]; o.y += part[w][2 * k + 1]; }
+    o = make_double2(o.x * scale, o.y * scale);
+    bool zero = false;
+    for (int ax = 0; ax < 3; ++ax) zero |= (c_parity[k][ax] < 0) && (m[ax] == 0);
+    if (zero) o = make_double2(0.0, 0.0);
+    const int64_t V = int64_t(L) * M * N;
+    G[i + int64_t(L) * (j + int64_t(M) * l) + V * k] = o;
+  }
+}
+
+// skip_near: cells left to k_assemble_near.
+__global__ void k_assemble(double2* __restrict__ G, int L, int M, int N, double2 kappa, double scale, int skip_near) {
   const int64_t V = int64_t(L) * M * N;
   for (int64_t v = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; v < V; v += int64_t(gridDim.x) * blockDim.x) {
     const int i = int(v % L);
@@ -506,7 +564,8 @@ __global__ void k_assemble(double2* __restrict__ G, int L, int M, int N, double2
     const int l = int(v / (int64_t(L) * M));
     const int m[3] = {i, j, l};
     const int mx = max(i, max(j, l));
-    const int rule = mx <= 1 ? 1 : (mx < 5 ? 2 : 3);
+    if (skip_near && mx < kNear) continue;
+    const int rule = mx <= 1 ? 1 : (mx < kNear ? 2 : 3);
     double2 acc[6];
     for (int k = 0; k < 6; ++k) acc[k] = make_double2(0.0, 0.0);
     for (int p = 0; p < 6; ++p) {
@@ -600,8 +659,16 @@ void assemble_toeplitz_dev(int L, int M, int N, double dx, double ko, double2* G
   init_assembly_constants();
   const int64_t V = int64_t(L) * M * N;
   const int blocks = int(std::min<int64_t>((V + 127) / 128, 1 << 20));
-  hipLaunchKernelGGL(k_assemble, dim3(blocks), dim3(128), 0, s, G, L, M, N, make_double2(ko * dx, ko_im * dx),
-                     dx * dx * dx / (4.0 * kPi));
+  const double2 kappa = make_double2(ko * dx, ko_im * dx);
+  const double scale = dx * dx * dx / (4.0 * kPi);
+  const char* e = std::getenv("VP_ASSEMBLE_NEAR");   // 0: near cells integrated by one thread each
+  const int near = (e && e[0] == '0') ? 0 : 1;
+  if (near) {
+    const int nnear = std::min(L, kNear) * std::min(M, kNear) * std::min(N, kNear);
+    hipLaunchKernelGGL(k_assemble_near, dim3(nnear), dim3(256), 0, s, G, L, M, N, kappa, scale);
+    VP_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_assemble, dim3(blocks), dim3(128), 0, s, G, L, M, N, kappa, scale, near);
   VP_HIP(hipGetLastError());
   VP_HIP(hipStreamSynchronize(s));
 }
