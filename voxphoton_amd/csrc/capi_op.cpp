@@ -75,6 +75,7 @@ int vp_ctx_destroy(vp_ctx ctx) {
   (void)hipStreamDestroy(ctx->stream);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   delete ctx;
+  dev_cache_flush();   // released large buffers go back to the device
   VP_API_END
 }
 

@@ -18,6 +18,16 @@ struct vp_ctx_s {
 
 namespace vp {
 
+// Device memory of the library.  Requests of 64 MiB and more go through a
+// per-device cache: a released buffer is kept, after a device
+// synchronisation as hipFree would do, and handed to a later request of
+// between 4/5 of its size and its size (48 GiB at most are kept; the cache is
+// emptied and the allocation retried when hipMalloc runs out of memory).
+// Repeated solves of one size then allocate nothing.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+void dev_cache_flush();
+
 // RAII device buffer.
 struct DevBuf {
   void* p = nullptr;
@@ -26,11 +36,11 @@ struct DevBuf {
   explicit DevBuf(size_t b) { alloc(b); }
   void alloc(size_t b) {
     release();
-    if (b) VP_HIP(hipMalloc(&p, b));
+    if (b) p = dev_alloc(b);
     bytes = b;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) dev_free(p);
     p = nullptr;
     bytes = 0;
   }

@@ -1,7 +1,9 @@
 // Host-side planning: radix factorisation and launch shapes for the matvec.
 #include <algorithm>
+#include <map>
 #include <cstdlib>
 #include <mutex>
+#include <vector>
 #include "vp_internal.hpp"
 
 namespace vp {
@@ -96,6 +98,103 @@ bool pcrow_supported(int n) {
     default: return false;
   }
 #undef VP_PCROW_CASE
+}
+
+namespace {
+constexpr size_t kCacheMin = size_t(64) << 20;
+constexpr size_t kCacheMax = size_t(48) << 30;
+struct CachedBuf {
+  int dev;
+  void* p;
+  size_t bytes;
+};
+std::mutex g_cache_mu;
+std::vector<CachedBuf> g_cache;               // released, oldest first
+std::map<void*, std::pair<int, size_t>> g_big;  // live and cached large allocations: device, size
+size_t g_cache_bytes = 0;
+
+void flush_device(int dev) {   // g_cache_mu held
+  for (size_t i = 0; i < g_cache.size();) {
+    if (dev < 0 || g_cache[i].dev == dev) {
+      (void)hipFree(g_cache[i].p);
+      g_big.erase(g_cache[i].p);
+      g_cache_bytes -= g_cache[i].bytes;
+      g_cache.erase(g_cache.begin() + long(i));
+    } else {
+      ++i;
+    }
+  }
+}
+}  // namespace
+
+void* dev_alloc(size_t b) {
+  int dev = 0;
+  VP_HIP(hipGetDevice(&dev));
+  if (b >= kCacheMin) {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    int best = -1;
+    for (int i = 0; i < int(g_cache.size()); ++i) {
+      const CachedBuf& c = g_cache[size_t(i)];
+      if (c.dev == dev && c.bytes >= b && c.bytes - c.bytes / 5 <= b &&
+          (best < 0 || c.bytes < g_cache[size_t(best)].bytes))
+        best = i;
+    }
+    if (best >= 0) {
+      void* p = g_cache[size_t(best)].p;
+      g_cache_bytes -= g_cache[size_t(best)].bytes;
+      g_cache.erase(g_cache.begin() + best);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, b);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+    (void)hipGetLastError();
+    {
+      std::lock_guard<std::mutex> lk(g_cache_mu);
+      flush_device(dev);
+    }
+    e = hipMalloc(&p, b);
+  }
+  VP_HIP(e);
+  if (b >= kCacheMin) {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_big[p] = {dev, b};
+  }
+  return p;
+}
+
+void dev_free(void* p) {
+  if (!p) return;
+  std::pair<int, size_t> info;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_big.find(p);
+    if (it == g_big.end()) {
+      (void)hipFree(p);
+      return;
+    }
+    info = it->second;
+  }
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  if (cur != info.first) (void)hipSetDevice(info.first);
+  (void)hipDeviceSynchronize();   // work that uses the buffer ends before it can be handed out again
+  if (cur != info.first) (void)hipSetDevice(cur);
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  g_cache.push_back({info.first, p, info.second});
+  g_cache_bytes += info.second;
+  while (g_cache_bytes > kCacheMax && !g_cache.empty()) {
+    (void)hipFree(g_cache.front().p);
+    g_big.erase(g_cache.front().p);
+    g_cache_bytes -= g_cache.front().bytes;
+    g_cache.erase(g_cache.begin());
+  }
+}
+
+void dev_cache_flush() {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  flush_device(-1);
 }
 
 int xct_threads(int S, bool fp64) {
