@@ -41,9 +41,9 @@ WORKLOADS = {
     'sweep_bragg_20nm': dict(
         desc='wavelength sweep of the 20 nm Bragg grating (1800x50x22, FFT 3600x100x44): 101 wavelengths '
              '1500-1600 nm, each with its own Green tensor and preconditioner, 2 right-hand sides per wavelength '
-             '(y- and z-polarised Gaussians), solved as batches of 8 wavelengths x 2 RHS',
+             '(y- and z-polarised Gaussians), solved as batches of 4 wavelengths x 2 RHS',
         tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=4, max_outer=12, prec='face', axis=1, damping=0.0,
-        base='bragg_20nm', nlam=101, lam0=1500e-9, lam1=1600e-9, chunk=8, pols=('y', 'z')),
+        base='bragg_20nm', nlam=101, lam0=1500e-9, lam1=1600e-9, chunk=4, pols=('y', 'z')),
     'mor_bragg_20nm': dict(
         desc='transmission spectrum of the 20 nm Bragg grating (1800x50x22, FFT 3600x100x44) by model order '
              'reduction: 5 Gauss-Legendre sample wavelengths in 1500-1600 nm, order 5 (25 solves per structure), '
@@ -172,6 +172,7 @@ def main():
     ap.add_argument('--no-fields', action='store_true', help='skip the excitation / total-field timing')
     ap.add_argument('--inner-tol', type=float, default=None, help='override the workload inner tolerance')
     ap.add_argument('--inner-maxit', type=int, default=None, help='override the workload inner restart cycles')
+    ap.add_argument('--batch', type=int, default=None, help='sweep: wavelengths per batched solve (override)')
     ap.add_argument('--traffic-probe', action='store_true',
                     help='for rocprofv3 --pmc runs: a 1 GiB device copy (calibration), then 3 FP32 matvecs '
                          'and 3 FP32 preconditioner applies')
@@ -189,6 +190,8 @@ def main():
         wl['inner_tol'] = args.inner_tol
     if args.inner_maxit is not None:
         wl['inner_maxit'] = args.inner_maxit
+    if args.batch is not None and 'chunk' in wl:
+        wl['chunk'] = args.batch
     log = lambda *a: print('[bench]', *a, file=sys.stderr, flush=True)   # noqa: E731
     if 'nlam' in wl:
         run_sweep(args, wl, log)
