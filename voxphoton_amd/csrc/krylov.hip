@@ -25,9 +25,9 @@
 namespace vp {
 
 constexpr int KT = 256;    // threads per Krylov block
-constexpr int RPT = 8;     // rows per thread
+constexpr int RPT = 16;    // rows per thread
 constexpr int RB = KT * RPT;
-constexpr int JC = 4;      // columns per register pass
+constexpr int JC = 2;      // columns per register pass
 
 __device__ __forceinline__ double2 to_d(float2 v) { return make_double2(v.x, v.y); }
 __device__ __forceinline__ double2 to_d(double2 v) { return v; }
