@@ -28,6 +28,16 @@ if ROOT not in sys.path:
 METRIC = 'solves/s at 1e-6 residual (mixed vs FP64); ms/matvec + GB/s; p50/p99 latency'
 HBM_PEAK = 8.0e12        # MI355X HBM3E, bytes/s
 
+
+def _prec_dtype():
+    half = os.environ.get('VP_PC_HALF', '1') != '0'
+    blocks = 'binary16-stored blocks (scaled, FP32 arithmetic)' if half else 'FP32 blocks'
+    return (f'c64 inner (FP32 matvec, FP32 GMRES, FP32 preconditioner FFTs with {blocks}), '
+            'c128 outer residual and refinement')
+
+
+PREC_DTYPE = _prec_dtype()
+
 WORKLOADS = {
     'bragg_10nm': dict(
         desc='Bragg grating, Si 500 nm +/-50 nm corrugation, 220 nm, 320 nm x 100 periods, quadratic '
@@ -319,19 +329,29 @@ def main():
         f'{Bb64 / t_mv64 / 1e9:.0f} GB/s as built), fp32 {t_mv32 * 1e3:.3f} ms ({B32 / t_mv32 / 1e9:.0f} GB/s '
         f'canonical, {Bb32 / t_mv32 / 1e9:.0f} GB/s as built)')
 
-    # ---- plain FP64 GPU solve (serial)
+    # ---- plain FP64 GPU solve (serial), into its own buffer: the mixed
+    # solution stays in dx and is compared with it
     fp64 = None
+    x_err = None
     if not args.no_fp64_solve:
+        dx64 = vie.DeviceArray(ctx, b.nbytes)
         info = L.GmresInfo()
         ctx.synchronize()
         t = time.perf_counter()
-        L.check(L.lib.vp_gmres(op.handle, hp, L.VP_PREC_RIGHT, db.ptr, None, dx.ptr, L.VP_MEM_DEVICE, L.VP_FP64,
+        L.check(L.lib.vp_gmres(op.handle, hp, L.VP_PREC_RIGHT, db.ptr, None, dx64.ptr, L.VP_MEM_DEVICE, L.VP_FP64,
                                wl['restart'], wl['tol'], 400, ctypes.byref(info)))
         ctx.synchronize()
         t64 = time.perf_counter() - t
         fp64 = dict(solves_per_s=1.0 / t64, seconds=t64, iterations=info.total_iters, relres=info.relres,
                     flag=info.flag)
         log(f'plain fp64: {t64:.3f} s/solve, {info.total_iters} its, relres {info.relres:.3e}')
+        xm = dx.download(np.empty(n, dtype=complex))
+        x64s = dx64.download(np.empty(n, dtype=complex))
+        x_err = float(np.linalg.norm(xm - x64s) / np.linalg.norm(x64s))
+        # both solutions have true residuals <= tol, so they differ by at most
+        # about cond(A) * 2 tol; the measured difference is reported beside it
+        log(f'mixed vs plain FP64 solution: relative difference {x_err:.3e}')
+        del xm, x64s, dx64
 
     # ---- roofline of the dominant kernel
     dom = max(stats, key=lambda k: stats[k][1])
@@ -356,10 +376,26 @@ def main():
     if not args.no_fields:
         fields = field_block(prob, vie, L, ctx, dx, log)
 
-    # ---- CPU baseline: the FP64 oracle matvec on the host cores
+    # ---- CPU baseline (FP64 oracle on the host cores); the oracle's matvec
+    # of the seeded vector also checks the device matvecs of the same vector
     cpu = None
+    mv_check = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(prob, vie, fp64, log)
+        cpu, y_ref = cpu_baseline(prob, vie, fp64, log, pc_box=getattr(prec, 'box_shape', None),
+                                  restart=wl['restart'])
+        yd64 = y64.download(np.empty(n, dtype=complex))
+        L.check(L.lib.vp_op_apply(op.handle, 0, x32.ptr, y32.ptr, L.VP_MEM_DEVICE, L.VP_FP32))
+        ctx.synchronize()
+        yd32 = y32.download(np.empty(n, dtype=np.complex64)).astype(complex) * op.fp32_scale
+        nref = np.linalg.norm(y_ref)
+        mv_check = dict(fp64_rel_err=float(np.linalg.norm(yd64 - y_ref) / nref),
+                        fp32_rel_err=float(np.linalg.norm(yd32 - y_ref) / nref),
+                        bound_fp64=1e-12, bound_fp32=1e-5,
+                        vector='default_rng(1550) N(0,1) + i N(0,1), complex128')
+        log(f'device matvec vs FP64 oracle: fp64 {mv_check["fp64_rel_err"]:.2e}, fp32 {mv_check["fp32_rel_err"]:.2e}')
+        del yd64, yd32, y_ref
+        if mv_check['fp64_rel_err'] > 1e-12 or mv_check['fp32_rel_err'] > 1e-5:
+            raise SystemExit(f'device matvec disagrees with the FP64 oracle: {mv_check}')
 
     lat_ms = np.array(lat) * 1e3
     line = {
@@ -373,7 +409,7 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'c64 inner (FP32 matvec/preconditioner/GMRES), c128 outer residual',
+        'dtype': PREC_DTYPE,
         'data': 'synthetic (device-assembled Green tensor, generated geometry)',
         'config': {'workload': args.workload, 'description': wl['desc'], 'unknowns': n, 'tol': wl['tol'],
                    'restart': wl['restart'], 'inner_tol': wl['inner_tol'], 'inner_maxit': wl['inner_maxit'],
@@ -386,6 +422,8 @@ def main():
                        'p99': round(float(np.percentile(lat_ms, 99)), 3)},
         'fp64_plain': fp64,
         'mixed_vs_fp64': None if fp64 is None else round(solves_per_s / fp64['solves_per_s'], 3),
+        'x_err_vs_fp64': x_err,
+        'matvec_vs_oracle': mv_check,
         'matvec': {'fp64_ms': round(t_mv64 * 1e3, 4), 'fp32_ms': round(t_mv32 * 1e3, 4),
                    'model_bytes_fp64': B64, 'model_bytes_fp32': B32,
                    'built_bytes_fp64': Bb64, 'built_bytes_fp32': Bb32,
@@ -486,7 +524,7 @@ def run_sweep(args, wl, log):
         'metric': METRIC, 'value': round(value, 4), 'unit': 'solves/s', 'n_gpus': 1,
         'steps': nsolve, 'warmup': 0, 'ms_per_step': round(t_solve / nsolve * 1e3, 3),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'c64 inner (FP32 matvec/preconditioner/GMRES), c128 outer residual',
+        'dtype': PREC_DTYPE,
         'data': 'synthetic (device-assembled Green tensor per wavelength, generated geometry)',
         'config': {'workload': args.workload, 'description': wl['desc'], 'unknowns': n, 'solves': nsolve,
                    'tol': wl['tol'], 'restart': wl['restart'], 'inner_tol': wl['inner_tol'],
@@ -772,7 +810,7 @@ def run_serve(args, wl, log):
         'metric': METRIC, 'value': round(len(done) / t_total, 4), 'unit': 'solves/s', 'n_gpus': 1,
         'steps': len(done), 'warmup': 0, 'ms_per_step': round(t_total / len(done) * 1e3, 3),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'c64 inner (FP32 matvec/preconditioner/GMRES), c128 outer residual',
+        'dtype': PREC_DTYPE,
         'data': 'synthetic (seeded request stream; device-assembled Green tensors)',
         'config': {'workload': args.workload, 'description': wl['desc'], 'requests': len(done), 'mix': kinds,
                    'unknowns_min': int(min(r['n'] for r in done)), 'unknowns_max': int(max(r['n'] for r in done)),
@@ -907,9 +945,21 @@ def make_preconditioner(prob, wl, vie, configs):
     return P, desc
 
 
-def cpu_baseline(prob, vie, fp64, log):
-    """FP64 oracle matvec (NumPy/SciPy restatement of mv_AN_const) on the host."""
-    from oracle import vie as ovie
+def cpu_baseline(prob, vie, fp64, log, pc_box=None, restart=100):
+    """CPU baseline: the FP64 oracle (NumPy/SciPy restatement of the
+    reference's mv_AN_const, chan_2_mvp_for_parallel_idx and pgmres_iter) on
+    the host cores, on a bounded sample of the workload:
+
+    * one full matvec of the seeded vector (its output is returned, so the
+      caller checks the device matvecs against it);
+    * the 2-level preconditioner apply on a slab of kx frequencies of the
+      preconditioner box with blocks of the box's size, scaled to all kx;
+    * one classical Gram-Schmidt step against 8 basis vectors, scaled to the
+      mean width (restart + 1) / 2 of a GMRES(restart) cycle.
+
+    Extrapolated to one plain FP64 GMRES solve with the GPU-measured
+    iteration count (plus the final residual matvec)."""
+    from oracle import vie as ovie, precond as opre
     cores = min(len(os.sched_getaffinity(0)), 16)
     G = vie.assemble_toeplitz(prob.shape, prob.dx, prob.ko)
     fN = ovie.fN_from_toeplitz(G, workers=cores)
@@ -917,20 +967,47 @@ def cpu_baseline(prob, vie, fp64, log):
     rng = np.random.default_rng(1550)
     x = rng.standard_normal(prob.n) + 1j * rng.standard_normal(prob.n)
     Mr = np.ones(prob.shape, complex)
-    Mc = prob.Mc
-    idx = prob.idx
     t = time.perf_counter()
-    ovie.mv_AN_const(x, fN, Mr, Mc, prob.gram, 'notransp', idx, workers=cores)
+    y_ref = ovie.mv_AN_const(x, fN, Mr, prob.Mc, prob.gram, 'notransp', prob.idx, workers=cores)
     t_mv = time.perf_counter() - t
+    del fN
     log(f'cpu oracle matvec {t_mv:.2f} s on {cores} threads')
-    n_mv = None
-    if fp64 is not None:
-        n_mv = fp64['iterations'] + 2
-    value = None if n_mv is None else 1.0 / (t_mv * n_mv)
-    return dict(value=value, unit='solves/s', cores=cores, kind='port',
-                sample=f'one FP64 oracle matvec (scipy.fft, {cores} threads) = {t_mv:.2f} s, extrapolated to the '
-                       f'{n_mv} matvecs of one plain FP64 GMRES solve; preconditioner and Arnoldi work excluded',
-                matvec_s=t_mv)
+    t_pc = 0.0
+    pc_note = 'no preconditioner'
+    if pc_box is not None:
+        Lc, Mc_, Nc = pc_box
+        Ls = min(Lc, 32)
+        rs = np.random.default_rng(7)
+        nb = 3 * Nc
+        B = (rs.standard_normal((Ls, Mc_, nb, nb)) + 1j * rs.standard_normal((Ls, Mc_, nb, nb))) / nb
+        v = rs.standard_normal(3 * Ls * Mc_ * Nc) + 1j * rs.standard_normal(3 * Ls * Mc_ * Nc)
+        t = time.perf_counter()
+        opre.chan_2_mvp(B, v, Ls, Mc_, Nc)
+        t_pc = (time.perf_counter() - t) * Lc / Ls
+        pc_note = f'2-level apply on {Ls} of {Lc} kx of the {Lc}x{Mc_}x{Nc} box ({nb}x{nb} blocks), scaled'
+        del B, v
+    j_s = 8
+    jm = (restart + 1) / 2
+    Vb = np.ones((j_s, prob.n), dtype=complex)
+    w = x.copy()
+    t = time.perf_counter()
+    h = Vb.conj() @ w
+    w -= Vb.T @ h
+    np.linalg.norm(w)
+    t_cgs = (time.perf_counter() - t) * jm / j_s
+    del Vb, w
+    log(f'cpu oracle preconditioner apply {t_pc:.2f} s (sampled), Gram-Schmidt step {t_cgs:.2f} s (sampled)')
+    n_it = None if fp64 is None else fp64['iterations']
+    per_solve = None if n_it is None else n_it * (t_mv + t_pc + t_cgs) + t_mv
+    value = None if per_solve is None else 1.0 / per_solve
+    cpu = dict(value=value, unit='solves/s', cores=cores, kind='port',
+               sample=f'FP64 oracle on {cores} host threads: one full matvec ({t_mv:.2f} s); {pc_note} '
+                      f'({t_pc:.2f} s); one Gram-Schmidt step against 8 vectors scaled to {jm:.1f} '
+                      f'({t_cgs:.2f} s); extrapolated to the {n_it} iterations of one plain FP64 GMRES solve '
+                      f'plus its final residual matvec',
+               matvec_s=round(t_mv, 3), precond_apply_s=round(t_pc, 3), gram_schmidt_s=round(t_cgs, 3),
+               seconds_per_solve=None if per_solve is None else round(per_solve, 2))
+    return cpu, y_ref
 
 
 if __name__ == '__main__':

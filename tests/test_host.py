@@ -3,6 +3,9 @@ padding, benchmark byte models, argument validation of the C ABI)."""
 import ctypes
 
 import numpy as np
+
+from oracle import vie as ovie
+from tests.helpers import rel
 import pytest
 
 
@@ -109,3 +112,14 @@ def test_serving_problems_plan(kind, size):
     for sl in prob.box:
         assert _prime_factors(sl.stop - sl.start) <= {2, 3, 5, 7, 11}
     assert pc['prec'] in ('face', 'scalar')
+
+
+def test_config_rhs_matches_oracle_rhs():
+    # the configurations' JVIE-II right-hand side equals the oracle's
+    # Gram * j w eps0 Mc .* Einc(idx) (JVIE_Solver_const.m:151-157, :243-251)
+    from voxphoton_amd import configs
+    prob = configs.straight_waveguide()
+    for pol in ('y', 'z'):
+        b = prob.gaussian_rhs(pol)
+        ref = ovie.jvie2_rhs(prob.gaussian_field(pol), prob.Mc, prob.gram, prob.freq, prob.idx)
+        assert rel(b, ref) < 1e-15

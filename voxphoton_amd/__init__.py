@@ -5,4 +5,4 @@ loop run in libvoxphoton.so (hand-written HIP); this package is the host-side
 mirror of the reference's MATLAB call surface.  Importing a submodule that
 needs the library raises ImportError when it has not been built.
 """
-__all__ = ['vie', 'geometry']
+__all__ = ['vie', 'configs', 'mor', 'krylov_dr']

@@ -43,12 +43,28 @@ def _compile(src):
     return obj, r.stderr
 
 
-def build_library(verbose=False):
+def _stale(src):
+    """True when the object of src is older than src or any header."""
+    obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + '.o')
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(d, f) for d in (CSRC, INCLUDE) for f in os.listdir(d)
+                                        if f.endswith(('.hpp', '.h'))]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build_library(verbose=False, incremental=False):
+    """Compiles every source (incremental=True: only those whose object is
+    older than the source or a header) and links the library."""
     os.makedirs(OBJDIR, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        results = list(ex.map(_compile, srcs))
-    objs = [o for o, _ in results]
+    todo = [s for s in srcs if _stale(s)] if incremental else srcs
+    results = []
+    if todo:
+        with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(todo))) as ex:
+            results = list(ex.map(_compile, todo))
+    objs = [os.path.join(OBJDIR, os.path.splitext(s)[0] + '.o') for s in srcs]
     if verbose:
         for _, err in results:
             if err.strip():
@@ -63,4 +79,4 @@ def build_library(verbose=False):
 
 
 if __name__ == '__main__':
-    print(build_library(verbose='-v' in sys.argv))
+    print(build_library(verbose='-v' in sys.argv, incremental='--incremental' in sys.argv))

@@ -67,8 +67,9 @@ std::string group_key(const vp_op_s* o, const vp_prec_s* q, const vp_solve_param
                 o->p64.plan.variant, o->p32.plan.variant, o->p32.plan.fsym, o->scale32);
   std::string k(buf);
   if (q)
-    std::snprintf(buf, sizeof(buf), "P%d %d %d %lld %d %lld %d|", q->g.L, q->g.M, q->g.N,
-                  static_cast<long long>(q->nbox), q->g.level, static_cast<long long>(q->g.n_outer), q->g.bsym);
+    std::snprintf(buf, sizeof(buf), "P%d %d %d %lld %d %lld %d %016llx %016llx|", q->g.L, q->g.M, q->g.N,
+                  static_cast<long long>(q->nbox), q->g.level, static_cast<long long>(q->g.n_outer), q->g.bsym,
+                  static_cast<unsigned long long>(q->pos_hash), static_cast<unsigned long long>(q->sub_hash));
   else
     std::snprintf(buf, sizeof(buf), "P-|");
   k += buf;

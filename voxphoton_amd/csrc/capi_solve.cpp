@@ -100,6 +100,7 @@ static vp_prec_s* prec_begin(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const 
     p->g.n_outer = 0;
     p->pos.alloc(pos.size() * sizeof(int));
     VP_HIP(hipMemcpy(p->pos.p, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice));
+    p->pos_hash = map_hash(pos);
     prec_geometry(p, level);
   } catch (...) {
     delete p;
@@ -233,6 +234,9 @@ void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* 
       VP_REQUIRE(q && q->g.L == q0->g.L && q->g.M == q0->g.M && q->g.N == q0->g.N && q->nbox == q0->nbox &&
                      q->g.level == q0->g.level && q->g.n_outer == q0->g.n_outer && q->g.bsym == q0->g.bsym,
                  ERR_INVALID, "batched preconditioners must share one geometry");
+      // the batched launch reads problem 0's voxel map and embedding for all
+      VP_REQUIRE(q->pos_hash == q0->pos_hash && q->sub_hash == q0->sub_hash && q->sub.bytes == q0->sub.bytes,
+                 ERR_INVALID, "batched preconditioners must share one voxel map and embedding");
       if (std::is_same<T, float>::value && (!q->p32.present || q->scale32 != ops[p]->scale32))
         prec_make_fp32(q, ops[p]->scale32);
       PrecPrec<T>& Q = pprec_of<T>(q);
@@ -338,7 +342,7 @@ bool ir_step_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2*
   for (int p = 0; p < np; ++p) {
     if (!hact[p]) continue;
     IrProblem& q = st[p];
-    q.relres = hbn[p] > 0 ? hrn[p] / hbn[p] : 0.0;
+    q.relres = relres_of(hrn[p], hbn[p]);
     q.hist.push_back(q.relres);
     if (q.relres <= tol || !std::isfinite(q.relres)) {
       q.done = true;
@@ -579,6 +583,7 @@ int vp_prec_set_embedding(vp_prec p, int64_t n_outer, const int64_t* sub_of_op, 
   VP_HIP(hipSetDevice(p->ctx->device));
   p->sub.alloc(h.size() * sizeof(int));
   VP_HIP(hipMemcpy(p->sub.p, h.data(), p->sub.bytes, hipMemcpyHostToDevice));
+  p->sub_hash = map_hash(h);
   p->g.n_outer = n_outer;
   p->p64.xs.alloc(size_t(n_outer) * sizeof(double2));
   p->p64.ys.alloc(size_t(n_outer) * sizeof(double2));

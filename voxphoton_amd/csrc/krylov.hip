@@ -847,7 +847,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   std::vector<int> proj_only(np, 0), lastk(np, 0);
   bool any_proj = false;
   for (int p = 0; p < np; ++p) {
-    last[p] = hbn[p] > 0 ? hbeta[p] / hbn[p] : 0.0;
+    last[p] = relres_of(hbeta[p], hbn[p]);
     res.resvec[p].push_back(last[p]);
     hact[p] = (joins(p) && last[p] > tol_of(p)) ? 1 : 0;
     // the projection on C alone met the tolerance: x takes the U part only
@@ -937,7 +937,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
         lastk[p] = k + 1;
         res.total[p] += 1;
         res.iter_inner[p] = k + 1;
-        last[p] = hbn[p] > 0 ? hest[p] / hbn[p] : 0.0;
+        last[p] = relres_of(hest[p], hbn[p]);
         res.resvec[p].push_back(last[p]);
         if (hconv[p] || k == m - 1) {
           inner[p] = 0;
@@ -983,7 +983,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   VP_HIP(hipMemcpy(hbeta.data(), beta.p, np * sizeof(double), hipMemcpyDeviceToHost));
   VP_HIP(hipMemcpy(hbn.data(), bn.p, np * sizeof(double), hipMemcpyDeviceToHost));
   for (int p = 0; p < np; ++p) {
-    res.relres[p] = hbn[p] > 0 ? hbeta[p] / hbn[p] : 0.0;
+    res.relres[p] = relres_of(hbeta[p], hbn[p]);
     res.flag[p] = res.relres[p] <= tol_of(p) ? 0 : 1;
   }
 }

@@ -2,10 +2,20 @@
 // semantics) with classical Gram-Schmidt, a selective second pass (DGKS
 // criterion) and FP64 Givens least squares, batched over independent problems.
 #pragma once
+#include <cmath>
+#include <limits>
 #include <vector>
 #include "precond.hpp"
 
 namespace vp {
+
+// Relative residual |r| / |b|.  A non-finite norm (a NaN or Inf in b, x or
+// the operator's output) gives NaN, which no tolerance accepts, so the
+// problem stops unconverged (flag 1); b = 0 gives 0 (x = 0 solves it).
+inline double relres_of(double rnorm, double bnorm) {
+  if (!std::isfinite(rnorm) || !std::isfinite(bnorm)) return std::numeric_limits<double>::quiet_NaN();
+  return bnorm > 0 ? rnorm / bnorm : 0.0;
+}
 
 // One set of independent problems of identical geometry solved together.
 // Vector p of a batch sits at base + p * n.

@@ -93,12 +93,24 @@ struct vp_prec_s {
   double scale32 = 1.0;      // FP32 copy inverts A / scale32
   vp::DevBuf pos;
   vp::DevBuf sub;            // embedding (optional)
-  vp::DevBuf blocks64_host_order;  // unused placeholder for symmetry
+  // content hashes of the voxel map and the embedding: batched launches
+  // share one map, so only preconditioners with equal maps batch together
+  uint64_t pos_hash = 0, sub_hash = 0;
   vp::PrecPrec<double> p64;
   vp::PrecPrec<float> p32;
 };
 
 namespace vp {
+// FNV-1a over the entries of an index map.
+inline uint64_t map_hash(const std::vector<int>& v) {
+  uint64_t h = 1469598103934665603ull;
+  for (int e : v) {
+    h ^= uint64_t(uint32_t(e));
+    h *= 1099511628211ull;
+  }
+  return h ^ uint64_t(v.size());
+}
+
 template <typename T> PrecPrec<T>& pprec_of(vp_prec_s* p);
 template <> inline PrecPrec<double>& pprec_of<double>(vp_prec_s* p) { return p->p64; }
 template <> inline PrecPrec<float>& pprec_of<float>(vp_prec_s* p) { return p->p32; }
