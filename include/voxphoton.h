@@ -352,6 +352,21 @@ int vp_field_total(vp_op op, const void* J, const void* Einc, double freq, void*
 int vp_op_create_lincomb(const vp_op* ops, const double* w, int k, vp_op* out);
 int vp_op_lincomb_into(vp_op out, const vp_op* ops, const double* w, int k);
 int vp_vec_dots(vp_ctx ctx, const void* const* X, int k, const void* const* Y, int m, int64_t n, double* out);
+/*
+ * vp_vec_dots_noalloc: the same X^H Y (same kernels, same results) with no
+ * device allocation: its pointer tables, per-block partial sums and results
+ * live in a 16 MiB array the library holds per GPU instead of the context's
+ * scratch.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Limit: the scratch need,
+ * 256-byte aligned 8k + 8m + 16 k m ceil(n / 1024) + 16 k m bytes, must fit
+ * 16 MiB (e.g. k = m = 8 up to n = 16 M), else VP_ERR_UNSUPPORTED.  Every
+ * call on one GPU shares its array: calls from threads of one process take
+ * turns on a per-GPU lock, and the array keeps the last call's contents
+ * (each call writes every entry it reads).
+ */
+int vp_vec_dots_noalloc(vp_ctx ctx, const void* const* X, int k, const void* const* Y, int m, int64_t n,
+                        double* out);
 int vp_vec_lincomb(vp_ctx ctx, const void* const* X, int k, const double* C, int m, int64_t n, void* const* Y);
 int vp_vec_absmax(vp_ctx ctx, const void* const* X, int k, int64_t n, double* out);
 int vp_vec_gather(vp_ctx ctx, const void* X, int64_t nx, const int64_t* idx, const void* w, double alpha_re,

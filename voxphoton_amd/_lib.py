@@ -104,6 +104,7 @@ SIGNATURES = {
     'vp_field_total': (c_int, [c_vp, c_vp, c_vp, c_dbl, c_vp, c_int]),
     'vp_op_create_lincomb': (c_int, [c_vp, c_vp, c_int, P(c_vp)]),
     'vp_vec_dots': (c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_i64, c_vp]),
+    'vp_vec_dots_noalloc': (c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_i64, c_vp]),
     'vp_vec_lincomb': (c_int, [c_vp, c_vp, c_int, c_vp, c_int, c_i64, c_vp]),
     'vp_op_lincomb_into': (c_int, [c_vp, c_vp, c_vp, c_int]),
     'vp_vec_absmax': (c_int, [c_vp, c_vp, c_int, c_i64, c_vp]),

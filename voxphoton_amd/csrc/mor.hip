@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
 #include "op.hpp"
 #include "capi_common.hpp"
@@ -208,23 +209,39 @@ template void tensor_lincomb<float>(float2*, const float2* const*, const double2
 
 }  // namespace vp
 
-using namespace vp;
+// Scratch of the allocation-free entry point (vp_vec_dots_noalloc): a
+// file-scope device array of 16 MiB that every GPU the library runs on holds
+// (its own copy, at its own address) until the process ends.  Host code never
+// passes the array's name to a kernel or a copy: it uses the per-device
+// address from hipGetSymbolAddress, taken once per device with that device
+// current (noalloc_scratch below).
+static __device__ __align__(256) char scratch[size_t(16) << 20];
 
-extern "C" {
+namespace vp {
+namespace {
 
-int vp_vec_dots(vp_ctx ctx, const void* const* X, int k, const void* const* Y, int m, int64_t n, double* out) {
-  VP_API_BEGIN
+void check_dots_args(vp_ctx ctx, const void* const* X, int k, const void* const* Y, int m, int64_t n,
+                     const double* out) {
   VP_REQUIRE(ctx && X && Y && out, ERR_INVALID, "ctx/X/Y/out is NULL");
   VP_REQUIRE(k > 0 && m > 0 && k <= 4096 && m <= 4096 && n > 0, ERR_INVALID, "bad vector counts");
   for (int i = 0; i < k; ++i) VP_REQUIRE(X[i], ERR_INVALID, "X has a NULL vector");
   for (int j = 0; j < m; ++j) VP_REQUIRE(Y[j], ERR_INVALID, "Y has a NULL vector");
-  VP_HIP(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  const int nblk = int((n + int64_t(KT) * RPT - 1) / (int64_t(KT) * RPT));
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  const size_t need = align256(sizeof(void*) * k) + align256(sizeof(void*) * m) +
-                      align256(size_t(k) * m * nblk * sizeof(double2)) + align256(size_t(k) * m * sizeof(double2));
-  Carve cv{static_cast<char*>(ctx_scratch(ctx, need))};
+}
+
+int dots_blocks(int64_t n) { return int((n + int64_t(KT) * RPT - 1) / (int64_t(KT) * RPT)); }
+
+// scratch of one X^H Y: the two pointer tables, the per-block partial sums
+// (every entry written before it is read) and the k x m results
+size_t dots_scratch_bytes(int k, int m, int64_t n) {
+  return align256(sizeof(void*) * k) + align256(sizeof(void*) * m) +
+         align256(size_t(k) * m * dots_blocks(n) * sizeof(double2)) + align256(size_t(k) * m * sizeof(double2));
+}
+
+// X^H Y with its scratch at base; returns after the stream drains
+void run_dots(hipStream_t s, char* base, const void* const* X, int k, const void* const* Y, int m, int64_t n,
+              double* out) {
+  const int nblk = dots_blocks(n);
+  Carve cv{base};
   auto* dx = cv.take<const double2*>(k);
   auto* dy = cv.take<const double2*>(m);
   auto* part = cv.take<double2>(size_t(k) * m * nblk);
@@ -237,6 +254,56 @@ int vp_vec_dots(vp_ctx ctx, const void* const* X, int k, const void* const* Y, i
   VP_HIP(hipGetLastError());
   VP_HIP(hipMemcpyAsync(out, res, size_t(k) * m * sizeof(double2), hipMemcpyDeviceToHost, s));
   VP_HIP(hipStreamSynchronize(s));
+}
+
+constexpr size_t kNoAllocScratchBytes = sizeof(scratch);
+constexpr int kMaxNoAllocDevices = 64;
+std::mutex g_noalloc_init_mu;
+char* g_noalloc_ptr[kMaxNoAllocDevices] = {};
+// calls on one device run one at a time: they share its array
+std::mutex g_noalloc_call_mu[kMaxNoAllocDevices];
+
+char* noalloc_scratch(int device) {   // device is current
+  std::lock_guard<std::mutex> lk(g_noalloc_init_mu);
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    VP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    VP_REQUIRE(p, ERR_DEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+    g_noalloc_ptr[device] = static_cast<char*>(p);
+  }
+  return g_noalloc_ptr[device];
+}
+
+}  // namespace
+}  // namespace vp
+
+using namespace vp;
+
+extern "C" {
+
+int vp_vec_dots(vp_ctx ctx, const void* const* X, int k, const void* const* Y, int m, int64_t n, double* out) {
+  VP_API_BEGIN
+  check_dots_args(ctx, X, k, Y, m, n, out);
+  VP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  run_dots(ctx->stream, static_cast<char*>(ctx_scratch(ctx, dots_scratch_bytes(k, m, n))), X, k, Y, m, n, out);
+  VP_API_END
+}
+
+int vp_vec_dots_noalloc(vp_ctx ctx, const void* const* X, int k, const void* const* Y, int m, int64_t n,
+                        double* out) {
+  VP_API_BEGIN
+  check_dots_args(ctx, X, k, Y, m, n, out);
+  const size_t need = dots_scratch_bytes(k, m, n);
+  VP_REQUIRE(need <= kNoAllocScratchBytes, ERR_UNSUPPORTED,
+             "vp_vec_dots_noalloc: k*m*ceil(n/1024) partial sums (" + std::to_string(need) +
+                 " bytes with the pointer tables) exceed its " + std::to_string(kNoAllocScratchBytes) +
+                 "-byte scratch; use vp_vec_dots");
+  VP_REQUIRE(ctx->device >= 0 && ctx->device < kMaxNoAllocDevices, ERR_UNSUPPORTED, "device index too large");
+  VP_HIP(hipSetDevice(ctx->device));
+  char* base = noalloc_scratch(ctx->device);
+  std::lock_guard<std::mutex> lk(g_noalloc_call_mu[ctx->device]);
+  run_dots(ctx->stream, base, X, k, Y, m, n, out);
   VP_API_END
 }
 
