@@ -101,12 +101,13 @@ def unknown_box(prob):
     return tuple(ext)
 
 
-def kernel_model_bytes(prob, kernel, s, fsym=True, pc_box=None):
+def kernel_model_bytes(prob, kernel, s, fsym=True, pc_box=None, fft=None):
     """Algorithmic bytes per launch of one matvec or preconditioner kernel.
 
     The passes run on the unknowns' bounding box Lb x Mb x Nb inside the
-    Lp x Mp x Np FFT grid: Qb = Lp*Mb*Nb x-spectra and Yb = Lp*Mp*Nb
-    y-spectra per component.  The split z pass reads the parity octant of
+    Lp x Mp x Np FFT grid (``fft``: the sizes the operator's plan runs at,
+    default the reference's 2L x 2M x 2N): Qb = Lp*Mb*Nb x-spectra and
+    Yb = Lp*Mp*Nb y-spectra per component.  The split z pass reads the parity octant of
     fN, Pc = (Lp/2+1)(Mp/2+1)(Np/2+1) entries per component (fsym), the
     fused plane kernel the full P = Lp*Mp*Np.
 
@@ -133,7 +134,7 @@ def kernel_model_bytes(prob, kernel, s, fsym=True, pc_box=None):
             return s * (V3 + 2 * n)
         return None
     L, M, N = prob.shape
-    Lp, Mp, Np = 2 * L, 2 * M, 2 * N
+    Lp, Mp, Np = fft if fft is not None else (2 * L, 2 * M, 2 * N)
     Lb, Mb, Nb = unknown_box(prob)
     P = Lp * Mp * Np
     Pc = (Lp // 2 + 1) * (Mp // 2 + 1) * (Np // 2 + 1) if fsym else P
@@ -156,14 +157,14 @@ def kernel_model_bytes(prob, kernel, s, fsym=True, pc_box=None):
     return None
 
 
-def kernel_table(prob, stats, pc_box):
+def kernel_table(prob, stats, pc_box, fft=None):
     """Per-kernel launches, mean duration (HIP events) and, where the byte
     model covers the kernel, its FP32 algorithmic GB/s."""
     out = {}
     for k, (cnt, ms) in stats.items():
         avg = ms / 1e3 / max(cnt, 1)
         row = dict(launches=cnt, avg_us=round(avg * 1e6, 2))
-        kb = kernel_model_bytes(prob, k, 8, pc_box=pc_box)
+        kb = kernel_model_bytes(prob, k, 8, pc_box=pc_box, fft=fft)
         if kb is not None and cnt:
             row['GBps'] = round(kb / avg / 1e9, 1)
         out[k] = row
@@ -223,7 +224,10 @@ def main():
     t0 = time.perf_counter()
     op = vie.NOperator.assembled(prob.shape, prob.dx, prob.ko, None, prob.Mc, prob.idx)
     t_op = time.perf_counter() - t0
-    log(f'operator assembled + embedded in {t_op:.2f} s')
+    oinfo = op.info()
+    fft32, fft64 = oinfo['fft_fp32'], oinfo['fft_fp64']
+    log(f'operator assembled + embedded in {t_op:.2f} s; unknowns box {oinfo["box"]}, autotuned FFT sizes '
+        f'FP32 {fft32} FP64 {fft64} (reference {tuple(2 * v for v in prob.shape)})')
     if args.traffic_probe:
         P, _ = make_preconditioner(prob, wl, vie, configs)
         traffic_probe(op, P, ctx, n, vie, L, log)
@@ -317,7 +321,8 @@ def main():
         ctx.synchronize()
         ran = read_stats(mv_kernels)
         L.check(L.lib.vp_ctx_set_profiling(ctx.handle, 0))
-        return sum(kernel_model_bytes(prob, k, s) for k, (cnt, _) in ran.items() if cnt)
+        fft = fft64 if s == 16 else fft32
+        return sum(kernel_model_bytes(prob, k, s, fft=fft) for k, (cnt, _) in ran.items() if cnt)
 
     t_mv64 = time_mv(x64, y64, L.VP_FP64)
     t_mv32 = time_mv(x32, y32, L.VP_FP32)
@@ -358,7 +363,7 @@ def main():
     launches, ms = stats[dom]
     avg_s = ms / 1e3 / max(launches, 1)
     pc_box = getattr(prec, 'box_shape', None)
-    kb = kernel_model_bytes(prob, dom, 8, pc_box=pc_box)
+    kb = kernel_model_bytes(prob, dom, 8, pc_box=pc_box, fft=fft32)
     roof = None
     if kb is not None and launches:
         achieved = kb / avg_s / 1e9
@@ -431,7 +436,9 @@ def main():
                    'fp64_GBps': round(B64 / t_mv64 / 1e9, 1), 'fp32_GBps': round(B32 / t_mv32 / 1e9, 1),
                    'fp64_frac_hbm': round(B64 / t_mv64 / HBM_PEAK, 4),
                    'fp32_frac_hbm': round(B32 / t_mv32 / HBM_PEAK, 4)},
-        'kernel_ms': kernel_table(prob, stats, pc_box),
+        'kernel_ms': kernel_table(prob, stats, pc_box, fft=fft32),
+        'fft_sizes': {'fp32': fft32, 'fp64': fft64, 'reference': [2 * v for v in prob.shape],
+                      'unknowns_box': oinfo['box'], 'chosen_by': 'autotuner (timed per box and precision)'},
         'setup_s': {'operator': round(t_op, 3), 'preconditioner': round(t_pc, 3)},
         'roofline': roof,
         'fields': fields,

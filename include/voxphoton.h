@@ -94,7 +94,9 @@ int vp_memcpy_d2h(vp_ctx ctx, void* dst, const void* src, int64_t bytes);
  * (Mr may be NULL for Mr == 1, the JVIE-II case).  gram: the real Gram factor
  * dx^3.  idx: n one-based indices into the 3*L*M*N field (idxS3).
  * precisions: VP_FP64, VP_FP32 or both OR-ed; FP32 copies are stored
- * prescaled by 1/gram (see vp_op_fp32_scale).
+ * prescaled by 1/gram (see vp_op_fp32_scale).  The matvec runs at the
+ * tensor's own sizes Lp x Mp x Np, which must be at least 2B-1 for the
+ * bounding box B of the unknowns' voxels (the reference's 2L x 2M x 2N are).
  */
 int vp_op_create(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_t Mp, int64_t Np,
                  const double* fN, const double* Mr, const double* Mc, double gram, const int64_t* idx,
@@ -103,9 +105,13 @@ int vp_op_create(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_
 /*
  * Same operator from the Toeplitz tensor opToeplitz (getOPERATORS_SAM.m:1
  * second output; L x M x N x 6 complex128).  The circulant embedding and
- * its FFT (circulant_nop_const.m, fft_operator.m) run on the device.
- * Lp, Mp, Np = 0 selects the reference sizes 2L, 2M, 2N; any sizes >= 2D-1
- * give the same operator.
+ * its FFT (circulant_nop_const.m, fft_operator.m) run on the device, on the
+ * offsets inside the bounding box B of the unknowns' voxels: any circulant
+ * of at least 2B-1 points per axis gives the same operator on idx as the
+ * reference's 2L x 2M x 2N.  A size of 0 lets an autotuner choose that
+ * axis per precision (timed on the device, cached per box; environment
+ * VP_FFT_SIZES=reference selects 2L x 2M x 2N); given sizes must be at least
+ * 2B-1.  vp_op_info reports the sizes chosen.
  */
 int vp_op_create_toeplitz(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_t Mp, int64_t Np,
                           const double* G, const double* Mr, const double* Mc, double gram, const int64_t* idx,
@@ -114,7 +120,8 @@ int vp_op_create_toeplitz(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t L
 /*
  * Assembles the Toeplitz tensor on the device for a uniform grid of voxel
  * size dx at wavenumber ko (assembly_nop_const.m:1-168 semantics, own
- * cubature) and creates the operator from it.  gram is dx^3.
+ * cubature; only the offsets inside the unknowns' bounding box) and creates
+ * the operator from it, FFT sizes as vp_op_create_toeplitz.  gram is dx^3.
  */
 int vp_op_create_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_t Mp, int64_t Np,
                            double dx, double ko, const double* Mr, const double* Mc, const int64_t* idx,
@@ -136,6 +143,16 @@ int vp_op_apply(vp_op op, int transp, const void* x, void* y, int mem, int prec)
 int vp_op_apply_batched(const vp_op* ops, int nvec, int transp, const void* x, void* y, int prec);
 int vp_op_fp32_scale(vp_op op, double* scale);
 int vp_op_size(vp_op op, int64_t* n);
+/*
+ * Geometry and plans of an operator, 17 entries: out[0..2] the grid L, M, N;
+ * out[3..5] the origin and out[6..8] the extents of the unknowns' bounding
+ * box; out[9..11] the FFT sizes of the FP64 plan and out[12..14] of the FP32
+ * plan (0 when that precision is absent); out[15], out[16] their matvec
+ * variants (0 fused planes, 1 split passes, -1 absent).  No counterpart in
+ * the reference, whose FFT sizes are always 2L x 2M x 2N
+ * (circulant_nop_const.m:5).
+ */
+int vp_op_info(vp_op op, int64_t* out);
 int vp_op_destroy(vp_op op);
 
 /* ---- preconditioners (pc/src_preconditioning) -------------------------------- */

@@ -87,6 +87,9 @@ def test_byte_models():
     # quadrant blocks of the face-averaged preconditioner, binary16 in FP32 applies
     pb = bench.kernel_model_bytes(prob, 'pc_block', 8, pc_box=(3600, 22, 60))
     assert pb == 4 * 1801 * 12 * 180 ** 2 + 8 * 2 * 3 * 3600 * 22 * 60
+    # the same pass at sizes embedded on the unknowns' box (7200 x 120 x 48)
+    zb = bench.kernel_model_bytes(prob, 'zpass', 8, fft=(7200, 120, 48))
+    assert zb == 8 * (6 * 3601 * 61 * 25 + 6 * 7200 * 120 * 22)
 
 
 def test_server_arguments_fail_cleanly():

@@ -74,6 +74,7 @@ SIGNATURES = {
     'vp_op_apply_batched': (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_int]),
     'vp_op_fp32_scale': (c_int, [c_vp, P(c_dbl)]),
     'vp_op_size': (c_int, [c_vp, P(c_i64)]),
+    'vp_op_info': (c_int, [c_vp, P(c_i64)]),
     'vp_op_destroy': (c_int, [c_vp]),
     'vp_prec_create_chan2': (c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_int, P(c_vp)]),
     'vp_prec_create_chan1': (c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_int, P(c_vp)]),

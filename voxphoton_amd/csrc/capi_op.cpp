@@ -247,7 +247,6 @@ static void init_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const std::vector<do
     VP_HIP(hipMemcpy(P.mr.p, h.data(), cnt * sizeof(C<T>), hipMemcpyHostToDevice));
   }
   P.gram = T(gram_t);
-  P.work.alloc(size_t(3) * op->Lp * op->Mb * op->Nb * sizeof(C<T>));
   P.xs.alloc(size_t(op->n) * sizeof(C<T>));
   P.ys.alloc(size_t(op->n) * sizeof(C<T>));
 }
@@ -272,11 +271,7 @@ static vp_op_s* op_begin(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp
                          int precisions) {
   VP_REQUIRE(ctx, ERR_INVALID, "ctx is NULL");
   VP_REQUIRE(L > 0 && M > 0 && N > 0, ERR_INVALID, "grid extents must be positive");
-  if (Lp == 0) Lp = 2 * L;
-  if (Mp == 0) Mp = 2 * M;
-  if (Np == 0) Np = 2 * N;
-  VP_REQUIRE(Lp >= 2 * L - 1 && Mp >= 2 * M - 1 && Np >= 2 * N - 1, ERR_INVALID,
-             "FFT sizes must be at least 2*D-1 in every dimension");
+  VP_REQUIRE(Lp >= 0 && Mp >= 0 && Np >= 0, ERR_INVALID, "FFT sizes must be >= 0 (0: chosen by the autotuner)");
   VP_REQUIRE(Lp < (1 << 20) && Mp < (1 << 20) && Np < (1 << 20), ERR_UNSUPPORTED, "FFT size too large");
   VP_REQUIRE(Mc, ERR_INVALID, "Mc is NULL");
   VP_REQUIRE(precisions & (VP_FP64 | VP_FP32), ERR_INVALID, "precisions must include VP_FP64 or VP_FP32");
@@ -294,6 +289,13 @@ static vp_op_s* op_begin(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp
     std::vector<int> pos;
     index_setup(op, idx, n, pos);
     unknown_box(op, pos);
+    // only offsets inside the unknowns' box reach the result: any circulant
+    // of at least 2B-1 per axis gives the same operator on idx
+    VP_REQUIRE((Lp == 0 || Lp >= 2 * op->Lb - 1) && (Mp == 0 || Mp >= 2 * op->Mb - 1) &&
+                   (Np == 0 || Np >= 2 * op->Nb - 1),
+               ERR_INVALID,
+               "FFT sizes must be at least 2*B-1 for the unknowns' bounding box B (" + std::to_string(op->Lb) +
+                   " x " + std::to_string(op->Mb) + " x " + std::to_string(op->Nb) + ")");
     op->pos.alloc(pos.size() * sizeof(int));
     VP_HIP(hipMemcpy(op->pos.p, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice));
     auto mc = gather_material(op, Mc, idx, 0.0);
@@ -321,20 +323,35 @@ static int variant_request() {
   throw VpError{ERR_INVALID, "VP_MATVEC_VARIANT must be fused, split or auto"};
 }
 
+// FFT sizes of one precision's plan.
+struct Sizes {
+  int Lp, Mp, Np;
+};
+
 // Geometry, twiddles, tensor layout and workspaces of one precision for one
-// matvec variant.
+// matvec variant at FFT sizes z.  F is the true-frequency FP64 tensor at
+// those sizes, or nullptr for a zero tensor of the right layout (timing).
 template <typename T>
-static void configure_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const double2* F, double scale, int variant) {
+static void configure_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const double2* F, Sizes z, double scale,
+                           int variant, bool fsym) {
   hipStream_t s = op->ctx->stream;
-  build_geometry(P.g, P.plan, *op, fp64, op->ctx->device, variant);
+  build_geometry(P.g, P.plan, *op, z.Lp, z.Mp, z.Np, fp64, op->ctx->device, variant);
   upload_twiddles(P, s);
-  P.plan.fsym = (variant == 1 && op->fsym) ? 1 : 0;
-  if (P.plan.fsym)
-    layout_fN_sym<T>(P.g, F, P.fN, scale, s);
-  else
-    layout_fN<T>(P.g, F, P.fN, scale, s);
+  P.plan.fsym = (variant == 1 && fsym) ? 1 : 0;
+  if (F) {
+    if (P.plan.fsym)
+      layout_fN_sym<T>(P.g, F, P.fN, scale, s);
+    else
+      layout_fN<T>(P.g, F, P.fN, scale, s);
+  } else {
+    const int64_t cnt = P.plan.fsym ? int64_t(z.Lp / 2 + 1) * (z.Mp / 2 + 1) * (z.Np / 2 + 1) * 6
+                                    : int64_t(z.Lp) * z.Mp * z.Np * 6;
+    if (P.fN.bytes != size_t(cnt) * sizeof(C<T>)) P.fN.alloc(size_t(cnt) * sizeof(C<T>));
+    VP_HIP(hipMemsetAsync(P.fN.p, 0, P.fN.bytes, s));
+  }
+  P.work.alloc(size_t(3) * z.Lp * op->Mb * op->Nb * sizeof(C<T>));
   if (variant == 1)
-    P.work2.alloc(size_t(3) * op->Lp * op->Mp * op->Nb * sizeof(C<T>));
+    P.work2.alloc(size_t(3) * z.Lp * z.Mp * op->Nb * sizeof(C<T>));
   else
     P.work2.release();
   upload_probs(op, P);
@@ -378,8 +395,9 @@ static std::mutex g_variant_mu;
 static std::map<VariantKey, int> g_variant;
 
 template <typename T>
-static void finalize_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const double2* F, double scale) {
-  const VariantKey key{op->ctx->device, op->Lb, op->Mb, op->Nb, op->Lp, op->Mp, op->Np, int(fp64)};
+static void finalize_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const double2* F, Sizes z, double scale,
+                          bool fsym) {
+  const VariantKey key{op->ctx->device, op->Lb, op->Mb, op->Nb, z.Lp, z.Mp, z.Np, int(fp64)};
   int v = variant_request();
   if (v < 0) {
     std::lock_guard<std::mutex> lk(g_variant_mu);
@@ -387,7 +405,7 @@ static void finalize_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const double2* F
     if (it != g_variant.end()) v = it->second;
   }
   if (v >= 0) {
-    configure_prec(op, P, fp64, F, scale, v);
+    configure_prec(op, P, fp64, F, z, scale, v, fsym);
     return;
   }
   double best = 0;
@@ -395,7 +413,7 @@ static void finalize_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const double2* F
   std::string why;
   for (int cand : {0, 1}) {
     try {
-      configure_prec(op, P, fp64, F, scale, cand);
+      configure_prec(op, P, fp64, F, z, scale, cand, fsym);
     } catch (const VpError& e) {
       if (e.code != ERR_UNSUPPORTED) throw;
       why = e.msg;
@@ -407,20 +425,154 @@ static void finalize_prec(vp_op_s* op, OpPrec<T>& P, bool fp64, const double2* F
     if (best_v < 0 || t < best) { best = t; best_v = cand; }
   }
   VP_REQUIRE(best_v >= 0, ERR_UNSUPPORTED, why);
-  if (configured != best_v) configure_prec(op, P, fp64, F, scale, best_v);
+  if (configured != best_v) configure_prec(op, P, fp64, F, z, scale, best_v, fsym);
   std::lock_guard<std::mutex> lk(g_variant_mu);
   g_variant.emplace(key, best_v);
 }
 
-// Common tail of every vp_op_create*: F is the true-frequency tensor.
-static void op_finish(vp_op_s* op, const double2* F) {
-  const double P = double(op->Lp) * op->Mp * op->Np;
+// ---------------------------------------------------------------------------
+// FFT-size autotuner.  Any circulant of at least 2B-1 points per axis (B the
+// unknowns' box) gives the same operator, so the sizes are free.  Per axis
+// the candidates are the lengths in [2B-1, 2B-1 + max(8, (2B-1)/4)] whose
+// prime factors the device FFTs handle; lengths with compile-time kernels
+// (x: class length Lp/2, y: the y pass, z: a fused radix pair) come first,
+// two of them at most, else the two shortest.  Every combination is timed
+// with the split matvec on a zero tensor (the kernels' time does not depend
+// on the values), and the fastest is kept per (device, box, precision) for
+// the process, so operators of one shape share their sizes and batch
+// together.  VP_FFT_SIZES=reference selects the reference's 2L x 2M x 2N,
+// VP_FFT_SIZES=min the shortest candidates without timing.
+// ---------------------------------------------------------------------------
+static bool smooth_length(int n, bool fp64) {
+  for (int q : {2, 3, 5, 7, 11, 13}) {
+    if (q == 13 && fp64) break;
+    while (n % q == 0) n /= q;
+  }
+  return n == 1;
+}
+
+static std::vector<int> axis_candidates(int B, int axis, bool fp64, int fixed) {
+  if (fixed > 0) return {fixed};
+  const int lo = std::max(2, 2 * B - 1);
+  const int hi = lo + std::max(8, lo / 4);
+  std::vector<int> fast, slow;
+  for (int n = lo; n <= hi; ++n) {
+    if (!smooth_length(n, fp64)) continue;
+    int ra = 0, rb = 0;
+    const bool f = axis == 0 ? (n % 2 == 0 && xct_threads(n / 2, fp64) > 0)
+                 : axis == 1 ? yct_supported(n)
+                             : zfused_pair(n, fp64, ra, rb);
+    (f ? fast : slow).push_back(n);
+  }
+  std::vector<int> out(fast.begin(), fast.begin() + std::min<size_t>(2, fast.size()));
+  for (size_t i = 0; out.size() < 2 && i < slow.size(); ++i)
+    if (out.empty() || slow[i] < out[0] || fast.empty()) out.push_back(slow[i]);
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+struct SizeKey {
+  int device, Lb, Mb, Nb, fp64, fsym, Lp, Mp, Np;   // Lp, Mp, Np: the caller's fixed sizes (0 free)
+  bool operator<(const SizeKey& o) const {
+    return std::tie(device, Lb, Mb, Nb, fp64, fsym, Lp, Mp, Np) <
+           std::tie(o.device, o.Lb, o.Mb, o.Nb, o.fp64, o.fsym, o.Lp, o.Mp, o.Np);
+  }
+};
+static std::mutex g_sizes_mu;
+static std::map<SizeKey, Sizes> g_sizes;
+
+static int fft_sizes_mode() {   // 0 auto (timed), 1 reference 2D, 2 shortest candidates
+  const char* e = std::getenv("VP_FFT_SIZES");
+  if (!e || !*e || !std::strcmp(e, "auto")) return 0;
+  if (!std::strcmp(e, "reference")) return 1;
+  if (!std::strcmp(e, "min")) return 2;
+  throw VpError{ERR_INVALID, "VP_FFT_SIZES must be auto, reference or min"};
+}
+
+template <typename T>
+static Sizes choose_sizes(vp_op_s* op, OpPrec<T>& P, bool fp64, bool fsym) {
+  const int mode = fft_sizes_mode();
+  if (mode == 1)
+    return Sizes{op->Lp ? op->Lp : 2 * op->L, op->Mp ? op->Mp : 2 * op->M, op->Np ? op->Np : 2 * op->N};
+  const auto cx = axis_candidates(op->Lb, 0, fp64, op->Lp), cy = axis_candidates(op->Mb, 1, fp64, op->Mp),
+             cz = axis_candidates(op->Nb, 2, fp64, op->Np);
+  VP_REQUIRE(!cx.empty() && !cy.empty() && !cz.empty(), ERR_UNSUPPORTED, "no FFT size candidates for the box");
+  if (mode == 2 || (cx.size() == 1 && cy.size() == 1 && cz.size() == 1)) return Sizes{cx[0], cy[0], cz[0]};
+  const SizeKey key{op->ctx->device, op->Lb, op->Mb, op->Nb, int(fp64), int(fsym), op->Lp, op->Mp, op->Np};
+  {
+    std::lock_guard<std::mutex> lk(g_sizes_mu);
+    auto it = g_sizes.find(key);
+    if (it != g_sizes.end()) return it->second;
+  }
+  Sizes best{0, 0, 0};
+  double tbest = 0;
+  std::string why = "no FFT size candidate plans";
+  for (int a : cx)
+    for (int b : cy)
+      for (int c : cz) {
+        const Sizes z{a, b, c};
+        try {
+          configure_prec(op, P, fp64, nullptr, z, 1.0, 1, fsym);
+        } catch (const VpError& e) {
+          if (e.code != ERR_UNSUPPORTED) throw;
+          why = e.msg;
+          continue;
+        }
+        const double t = time_applies(op, P);
+        if (best.Lp == 0 || t < tbest) { best = z; tbest = t; }
+      }
+  if (best.Lp == 0) {   // the split passes do not plan at any candidate: shortest sizes, any variant
+    best = Sizes{cx[0], cy[0], cz[0]};
+  }
+  std::lock_guard<std::mutex> lk(g_sizes_mu);
+  g_sizes.emplace(key, best);
+  return best;
+}
+
+// Source of an operator's tensor: the caller's true-frequency tensor at fixed
+// sizes, or a device Toeplitz tensor on the unknowns' box (Lb x Mb x Nb x 6)
+// embedded at the sizes each precision's plan chooses.
+struct TensorSource {
+  const double2* F = nullptr;
+  Sizes fixed{0, 0, 0};
+  const double2* Gbox = nullptr;
+};
+
+// Common tail of every vp_op_create*.
+static void op_finish(vp_op_s* op, const TensorSource& src) {
+  hipStream_t s = op->ctx->stream;
   // VP_FN_SYMMETRY=0 keeps the full tensor in every variant.
   const char* e = std::getenv("VP_FN_SYMMETRY");
-  op->fsym = !(e && std::strcmp(e, "0") == 0) && fN_parity_symmetric(*op, F, op->ctx->stream);
-  if (op->p64.present) finalize_prec(op, op->p64, true, F, 1.0 / P);
-  if (op->p32.present) finalize_prec(op, op->p32, false, F, 1.0 / (P * op->scale32));
-  VP_HIP(hipStreamSynchronize(op->ctx->stream));
+  const bool sym_allowed = !(e && std::strcmp(e, "0") == 0);
+  DevBuf built;            // tensor embedded from Gbox at built_z
+  Sizes built_z{0, 0, 0};
+  auto tensor_at = [&](Sizes z) -> const double2* {
+    if (src.F) return src.F;
+    if (built.p && built_z.Lp == z.Lp && built_z.Mp == z.Mp && built_z.Np == z.Np) return built.as<double2>();
+    built.release();
+    built.alloc(size_t(z.Lp) * z.Mp * z.Np * 6 * sizeof(double2));
+    toeplitz_to_fN(op->Lb, op->Mb, op->Nb, z.Lp, z.Mp, z.Np, op->ctx->device, src.Gbox, built.as<double2>(), s);
+    built_z = z;
+    return built.as<double2>();
+  };
+  // symmetry: a caller's tensor is checked; an embedded Toeplitz tensor is
+  // parity symmetric by construction (checked all the same)
+  bool fsym = sym_allowed;
+  if (src.F) fsym = sym_allowed && fN_parity_symmetric(src.fixed.Lp, src.fixed.Mp, src.fixed.Np, src.F, s);
+  auto one = [&](auto tag, bool fp64, double scale_div) {
+    using T = decltype(tag);
+    OpPrec<T>& P = prec_of<T>(op);
+    if (!P.present) return;
+    Sizes z = src.F ? src.fixed : choose_sizes<T>(op, P, fp64, fsym);
+    const double2* F = tensor_at(z);
+    if (!src.F && fsym) fsym = fN_parity_symmetric(z.Lp, z.Mp, z.Np, F, s);
+    const double Pn = double(z.Lp) * z.Mp * z.Np;
+    finalize_prec(op, P, fp64, F, z, 1.0 / (Pn * scale_div), fsym);
+  };
+  one(double{}, true, 1.0);
+  one(float{}, false, op->scale32);
+  op->fsym = fsym;
+  VP_HIP(hipStreamSynchronize(s));
 }
 
 template <typename T>
@@ -440,8 +592,12 @@ namespace {
 // every operator of a combination shares the geometry, scaling, precisions
 // and plan of ops[0], so that their stored tensors combine entry by entry
 void require_combinable(const vp_op_s* o0, const vp_op_s* o) {
-  VP_REQUIRE(o && o->ctx == o0->ctx && o->L == o0->L && o->M == o0->M && o->N == o0->N && o->Lp == o0->Lp &&
-                 o->Mp == o0->Mp && o->Np == o0->Np && o->n == o0->n && o->gram == o0->gram &&
+  auto same_sizes = [](const Geometry& a, const Geometry& b) {
+    return a.x.P == b.x.P && a.y.P == b.y.P && a.z.P == b.z.P;
+  };
+  VP_REQUIRE(o && o->ctx == o0->ctx && o->L == o0->L && o->M == o0->M && o->N == o0->N &&
+                 same_sizes(o->p64.g, o0->p64.g) && same_sizes(o->p32.g, o0->p32.g) && o->n == o0->n &&
+                 o->gram == o0->gram &&
                  o->scale32 == o0->scale32 && o->fsym == o0->fsym && o->p64.present == o0->p64.present &&
                  o->p32.present == o0->p32.present && o->p64.fN.bytes == o0->p64.fN.bytes &&
                  o->p32.fN.bytes == o0->p32.fN.bytes && o->p64.plan.variant == o0->p64.plan.variant &&
@@ -479,7 +635,10 @@ int vp_op_create(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_
     const size_t bytes = size_t(op->Lp) * op->Mp * op->Np * 6 * sizeof(double2);
     DevBuf F(bytes);
     VP_HIP(hipMemcpy(F.p, fN, bytes, hipMemcpyHostToDevice));
-    op_finish(op, F.as<double2>());
+    TensorSource src;
+    src.F = F.as<double2>();
+    src.fixed = Sizes{op->Lp, op->Mp, op->Np};
+    op_finish(op, src);
   } catch (...) {
     delete op;
     throw;
@@ -495,12 +654,21 @@ int vp_op_create_toeplitz(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t L
   VP_REQUIRE(out && G, ERR_INVALID, "out/G is NULL");
   vp_op_s* op = op_begin(ctx, L, M, N, Lp, Mp, Np, Mr, Mc, gram, idx, n, precisions);
   try {
-    const size_t gb = size_t(op->V) * 6 * sizeof(double2);
-    DevBuf dG(gb);
-    VP_HIP(hipMemcpy(dG.p, G, gb, hipMemcpyHostToDevice));
-    DevBuf F(size_t(op->Lp) * op->Mp * op->Np * 6 * sizeof(double2));
-    toeplitz_to_fN(*op, dG.as<double2>(), F.as<double2>(), ctx->stream);
-    op_finish(op, F.as<double2>());
+    // the offsets inside the unknowns' box: G(0:Lb, 0:Mb, 0:Nb, :)
+    const int64_t Vb = int64_t(op->Lb) * op->Mb * op->Nb;
+    std::vector<double2> hb(size_t(Vb) * 6);
+    for (int k = 0; k < 6; ++k)
+      for (int c = 0; c < op->Nb; ++c)
+        for (int b = 0; b < op->Mb; ++b) {
+          const double* src = G + 2 * (int64_t(op->L) * (b + int64_t(op->M) * (c + int64_t(op->N) * k)));
+          double2* dst = hb.data() + int64_t(op->Lb) * (b + int64_t(op->Mb) * (c + int64_t(op->Nb) * k));
+          std::memcpy(dst, src, size_t(op->Lb) * sizeof(double2));
+        }
+    DevBuf dG(hb.size() * sizeof(double2));
+    VP_HIP(hipMemcpy(dG.p, hb.data(), dG.bytes, hipMemcpyHostToDevice));
+    TensorSource src;
+    src.Gbox = dG.as<double2>();
+    op_finish(op, src);
   } catch (...) {
     delete op;
     throw;
@@ -517,11 +685,12 @@ int vp_op_create_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t 
   VP_REQUIRE(dx > 0 && std::isfinite(ko) && ko >= 0, ERR_INVALID, "dx must be positive and ko finite");
   vp_op_s* op = op_begin(ctx, L, M, N, Lp, Mp, Np, Mr, Mc, dx * dx * dx, idx, n, precisions);
   try {
-    DevBuf dG(size_t(op->V) * 6 * sizeof(double2));
-    assemble_toeplitz_dev(op->L, op->M, op->N, dx, ko, dG.as<double2>(), ctx->stream);
-    DevBuf F(size_t(op->Lp) * op->Mp * op->Np * 6 * sizeof(double2));
-    toeplitz_to_fN(*op, dG.as<double2>(), F.as<double2>(), ctx->stream);
-    op_finish(op, F.as<double2>());
+    // the Green tensor at the offsets inside the unknowns' box only
+    DevBuf dG(size_t(op->Lb) * op->Mb * op->Nb * 6 * sizeof(double2));
+    assemble_toeplitz_dev(op->Lb, op->Mb, op->Nb, dx, ko, dG.as<double2>(), ctx->stream);
+    TensorSource src;
+    src.Gbox = dG.as<double2>();
+    op_finish(op, src);
   } catch (...) {
     delete op;
     throw;
@@ -590,8 +759,9 @@ int vp_op_apply_batched(const vp_op* ops, int nvec, int transp, const void* x, v
   VP_REQUIRE(o0, ERR_INVALID, "ops[0] is NULL");
   for (int i = 1; i < nvec; ++i) {
     VP_REQUIRE(ops[i] && ops[i]->L == o0->L && ops[i]->M == o0->M && ops[i]->N == o0->N &&
-                   ops[i]->Lp == o0->Lp && ops[i]->Mp == o0->Mp && ops[i]->Np == o0->Np && ops[i]->n == o0->n &&
-                   ops[i]->structured == o0->structured,
+                   ops[i]->n == o0->n && ops[i]->structured == o0->structured && ops[i]->l0 == o0->l0 &&
+                   ops[i]->m0 == o0->m0 && ops[i]->n0 == o0->n0 && ops[i]->Lb == o0->Lb && ops[i]->Mb == o0->Mb &&
+                   ops[i]->Nb == o0->Nb,
                ERR_INVALID, "batched operators must share one geometry");
   }
   VP_HIP(hipSetDevice(o0->ctx->device));
@@ -608,7 +778,9 @@ int vp_op_apply_batched(const vp_op* ops, int nvec, int transp, const void* x, v
     // all problems must have identical class splits; rebuild if a later op chose differently
     for (int i = 1; i < nvec; ++i) {
       OpPrec<T>& P = prec_of<T>(ops[i]);
-      VP_REQUIRE(P.g.y.R == P0.g.y.R && P.g.z.R == P0.g.z.R, ERR_INVALID, "batched operators differ in plan");
+      VP_REQUIRE(P.g.x.P == P0.g.x.P && P.g.y.P == P0.g.y.P && P.g.z.P == P0.g.z.P && P.g.y.R == P0.g.y.R &&
+                     P.g.z.R == P0.g.z.R && P.plan.variant == P0.plan.variant && P.plan.fsym == P0.plan.fsym,
+                 ERR_INVALID, "batched operators differ in plan");
     }
     DevBuf d(sizeof(ProbPtrs<T>) * nvec);
     VP_HIP(hipMemcpyAsync(d.p, h.data(), d.bytes, hipMemcpyHostToDevice, s));
@@ -697,6 +869,18 @@ int vp_op_fp32_scale(vp_op op, double* scale) {
   VP_API_BEGIN
   VP_REQUIRE(op && scale, ERR_INVALID, "op/scale is NULL");
   *scale = op->scale32;
+  VP_API_END
+}
+
+int vp_op_info(vp_op op, int64_t* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(op && out, ERR_INVALID, "op/out is NULL");
+  const int64_t v[17] = {op->L, op->M, op->N, op->l0, op->m0, op->n0, op->Lb, op->Mb, op->Nb,
+                         op->p64.present ? op->p64.g.x.P : 0, op->p64.present ? op->p64.g.y.P : 0,
+                         op->p64.present ? op->p64.g.z.P : 0, op->p32.present ? op->p32.g.x.P : 0,
+                         op->p32.present ? op->p32.g.y.P : 0, op->p32.present ? op->p32.g.z.P : 0,
+                         op->p64.present ? op->p64.plan.variant : -1, op->p32.present ? op->p32.plan.variant : -1};
+  std::memcpy(out, v, sizeof(v));
   VP_API_END
 }
 

@@ -53,8 +53,9 @@ struct ServeReq {
 int64_t request_bytes(const vp_op_s* op, const vp_prec_s* pc, const vp_solve_params& prm) {
   const int64_t n = op->n;
   int64_t b = int64_t(prm.restart + 1 + 6) * n * 8 + int64_t(6) * n * 16;
-  const int64_t w = int64_t(3) * op->Lp * op->Mb * op->Nb;
-  const int64_t w2 = op->p32.plan.variant == 1 ? int64_t(3) * op->Lp * op->Mp * op->Nb : 0;
+  const Geometry& g = op->p32.present ? op->p32.g : op->p64.g;
+  const int64_t w = int64_t(3) * g.x.P * op->Mb * op->Nb;
+  const int64_t w2 = op->p32.plan.variant == 1 ? int64_t(3) * g.x.P * g.y.P * op->Nb : 0;
   b += (w + w2) * (16 + 8);
   if (pc) b += int64_t(3) * pc->g.V * (16 + 8) * (pc->g.level == 1 ? 2 : 1);
   return b;
@@ -62,7 +63,8 @@ int64_t request_bytes(const vp_op_s* op, const vp_prec_s* pc, const vp_solve_par
 
 std::string group_key(const vp_op_s* o, const vp_prec_s* q, const vp_solve_params& p) {
   char buf[512];
-  std::snprintf(buf, sizeof(buf), "%d %d %d|%d %d %d %d %d %d|%lld %d %d %d %d %.17g|", o->Lp, o->Mp, o->Np, o->l0,
+  std::snprintf(buf, sizeof(buf), "%d %d %d %d %d %d|%d %d %d %d %d %d|%lld %d %d %d %d %.17g|", o->p64.g.x.P,
+                o->p64.g.y.P, o->p64.g.z.P, o->p32.g.x.P, o->p32.g.y.P, o->p32.g.z.P, o->l0,
                 o->m0, o->n0, o->Lb, o->Mb, o->Nb, static_cast<long long>(o->n), o->structured,
                 o->p64.plan.variant, o->p32.plan.variant, o->p32.plan.fsym, o->scale32);
   std::string k(buf);

@@ -188,24 +188,24 @@ void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* 
   ps.twy = P0.twy();
   ps.twz = P0.twz();
   std::vector<ProbPtrs<T>> h(np);
-  const size_t wbytes = size_t(3) * o0->Lp * o0->Mb * o0->Nb * sizeof(C<T>);
-  const size_t w2bytes = P0.plan.variant == 1 ? size_t(3) * o0->Lp * o0->Mp * o0->Nb * sizeof(C<T>) : 0;
+  const size_t wbytes = size_t(3) * P0.g.x.P * o0->Mb * o0->Nb * sizeof(C<T>);
+  const size_t w2bytes = P0.plan.variant == 1 ? size_t(3) * P0.g.x.P * P0.g.y.P * o0->Nb * sizeof(C<T>) : 0;
   if (np > 1) {
     ps.works.alloc(wbytes * np);
     ps.works2.alloc(w2bytes * np);
   }
   for (int p = 0; p < np; ++p) {
     vp_op_s* op = ops[p];
-    VP_REQUIRE(op && op->L == o0->L && op->M == o0->M && op->N == o0->N && op->Lp == o0->Lp &&
-                   op->Mp == o0->Mp && op->Np == o0->Np && op->n == o0->n && op->structured == o0->structured &&
+    VP_REQUIRE(op && op->L == o0->L && op->M == o0->M && op->N == o0->N && op->n == o0->n &&
+                   op->structured == o0->structured &&
                    op->l0 == o0->l0 && op->m0 == o0->m0 && op->n0 == o0->n0 && op->Lb == o0->Lb &&
                    op->Mb == o0->Mb && op->Nb == o0->Nb,
                ERR_INVALID, "batched operators must share one geometry");
     OpPrec<T>& P = prec_of<T>(op);
     VP_REQUIRE(P.present, ERR_INVALID, "operator lacks the precision this solve needs");
     // The plan is a function of (grid, FFT sizes, precision, variant, device).
-    VP_REQUIRE(P.plan.variant == P0.plan.variant && P.plan.fsym == P0.plan.fsym && P.g.y.R == P0.g.y.R &&
-                   P.g.z.R == P0.g.z.R,
+    VP_REQUIRE(P.plan.variant == P0.plan.variant && P.plan.fsym == P0.plan.fsym && P.g.x.P == P0.g.x.P &&
+                   P.g.y.P == P0.g.y.P && P.g.z.P == P0.g.z.P && P.g.y.R == P0.g.y.R && P.g.z.R == P0.g.z.R,
                ERR_INVALID, "batched operators differ in plan");
     ProbPtrs<T> pp{};
     pp.work = np > 1 ? reinterpret_cast<C<T>*>(static_cast<char*>(ps.works.p) + wbytes * p)

@@ -84,7 +84,10 @@ struct OpPrec {
 
 struct vp_op_s {
   vp_ctx ctx = nullptr;
-  int L = 0, M = 0, N = 0, Lp = 0, Mp = 0, Np = 0;
+  int L = 0, M = 0, N = 0;
+  // FFT sizes the caller fixed (its fN's, or requested explicitly); 0: each
+  // precision's plan takes the autotuner's sizes (OpPrec::g.{x,y,z}.P)
+  int Lp = 0, Mp = 0, Np = 0;
   int l0 = 0, m0 = 0, n0 = 0, Lb = 0, Mb = 0, Nb = 0;   // bounding box of the unknowns' voxels
   int64_t V = 0, nS = 0;
   int structured = 0;
@@ -107,9 +110,11 @@ template <> inline OpPrec<double>& prec_of<double>(vp_op_s* op) { return op->p64
 template <> inline OpPrec<float>& prec_of<float>(vp_op_s* op) { return op->p32; }
 
 // Build helpers (setup.hip)
-// Chooses residue classes and launch shapes for one precision; throws
+// Chooses residue classes and launch shapes for one precision at FFT sizes
+// Lp x Mp x Np (each at least 2B-1 for the unknowns' box B); throws
 // ERR_UNSUPPORTED when no shape fits.
-void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, int device, int variant);
+void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, int Lp, int Mp, int Np, bool fp64, int device,
+                    int variant);
 template <typename T> void upload_twiddles(OpPrec<T>& P, hipStream_t s);
 // Writes the class-layout tensor of geometry g from a true-frequency device
 // tensor F (Lp x Mp x Np x 6 complex128, column-major), times scale.
@@ -118,9 +123,12 @@ template <typename T> void layout_fN(const Geometry& g, const double2* F, DevBuf
 template <typename T>
 void layout_fN_sym(const Geometry& g, const double2* F, DevBuf& out, double scale, hipStream_t s);
 // True when F(P - k) = parity * F(k) along every axis (to 1e-12 of max |F|).
-bool fN_parity_symmetric(const vp_op_s& op, const double2* F, hipStream_t s);
-// True-frequency tensor from a device Toeplitz tensor (L x M x N x 6).
-void toeplitz_to_fN(const vp_op_s& op, const double2* G, double2* F, hipStream_t s);
+bool fN_parity_symmetric(int Lp, int Mp, int Np, const double2* F, hipStream_t s);
+// True-frequency tensor F (Lp x Mp x Np x 6) of the circulant embedding of a
+// device Toeplitz tensor G (L x M x N x 6, offsets 0..D-1 per axis); sizes at
+// least 2D-1.
+void toeplitz_to_fN(int L, int M, int N, int Lp, int Mp, int Np, int device, const double2* G, double2* F,
+                    hipStream_t s);
 // Device assembly of the Toeplitz tensor.
 // ko_im < 0 assembles a damped kernel e^{-j(ko + j ko_im)R}/R (preconditioners only).
 void assemble_toeplitz_dev(int L, int M, int N, double dx, double ko, double2* G, hipStream_t s, double ko_im = 0.0);

@@ -19,7 +19,8 @@ static std::vector<int> divisors(int n) {
   return d;
 }
 
-void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, int device, int variant) {
+void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, int Lp, int Mp, int Np, bool fp64, int device,
+                    int variant) {
   g = Geometry{};
   g.L = op.Lb; g.M = op.Mb; g.N = op.Nb;
   g.l0 = op.l0; g.m0 = op.m0; g.n0 = op.n0;
@@ -27,12 +28,14 @@ void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, i
   g.V = op.V;
   g.nS = op.nS;
   g.structured = op.structured;
-  g.x.D = op.L; g.x.P = op.Lp;
-  g.y.D = op.M; g.y.P = op.Mp;
-  g.z.D = op.N; g.z.P = op.Np;
-  g.x.R = (op.Lp % 2 == 0) ? 2 : 1;
-  g.x.S = op.Lp / g.x.R;
-  VP_REQUIRE(g.x.S >= op.L, ERR_INVALID, "Lp too small for the grid");
+  g.x.D = op.Lb; g.x.P = Lp;
+  g.y.D = op.Mb; g.y.P = Mp;
+  g.z.D = op.Nb; g.z.P = Np;
+  VP_REQUIRE(Lp >= 2 * op.Lb - 1 && Mp >= 2 * op.Mb - 1 && Np >= 2 * op.Nb - 1, ERR_INVALID,
+             "FFT sizes must be at least 2*B-1 for the unknowns' bounding box B");
+  g.x.R = (Lp % 2 == 0) ? 2 : 1;
+  g.x.S = Lp / g.x.R;
+  VP_REQUIRE(g.x.S >= op.Lb, ERR_INVALID, "Lp too small for the unknowns' box");
   VP_REQUIRE(factorize(g.x.S, fp64, g.x.plan, g.x.R), ERR_UNSUPPORTED,
              "x FFT length " + std::to_string(g.x.S) + " has a prime factor above the supported radices");
   struct Cand { int ry, rz; };
@@ -40,8 +43,8 @@ void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, i
   if (variant == 1) {
     // split passes: full padded lengths, no residue classes
     Geometry t = g;
-    t.y.R = 1; t.y.S = op.Mp;
-    t.z.R = 1; t.z.S = op.Np;
+    t.y.R = 1; t.y.S = Mp;
+    t.z.R = 1; t.z.S = Np;
     std::string why2;
     VP_REQUIRE(factorize(t.y.S, fp64, t.y.plan, 1) && factorize(t.z.S, fp64, t.z.plan, 1), ERR_UNSUPPORTED,
                "padded y/z FFT lengths have a prime factor above the supported radices");
@@ -51,8 +54,8 @@ void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, i
     mp = p;
     return;
   }
-  for (int ry : divisors(op.Mp))
-    for (int rz : divisors(op.Np)) cands.push_back({ry, rz});
+  for (int ry : divisors(Mp))
+    for (int rz : divisors(Np)) cands.push_back({ry, rz});
   std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) {
     if (a.ry * a.rz != b.ry * b.rz) return a.ry * a.rz < b.ry * b.rz;
     return std::abs(a.ry - a.rz) < std::abs(b.ry - b.rz);
@@ -60,8 +63,8 @@ void build_geometry(Geometry& g, MatvecPlan& mp, const vp_op_s& op, bool fp64, i
   std::string why = "no residue-class split of the yz plane fits";
   for (const Cand& c : cands) {
     Geometry t = g;
-    t.y.R = c.ry; t.y.S = op.Mp / c.ry;
-    t.z.R = c.rz; t.z.S = op.Np / c.rz;
+    t.y.R = c.ry; t.y.S = Mp / c.ry;
+    t.z.R = c.rz; t.z.S = Np / c.rz;
     if (!factorize(t.y.S, fp64, t.y.plan, t.y.R)) continue;
     if (!factorize(t.z.S, fp64, t.z.plan, t.z.R)) continue;
     MatvecPlan p{};
@@ -230,14 +233,17 @@ __global__ void __launch_bounds__(256) k_embed_dft_x(const double2* __restrict__
   }
 }
 
-void toeplitz_to_fN(const vp_op_s& op, const double2* G, double2* F, hipStream_t s) {
-  auto twx = twiddle_table(op.Lp), twy = twiddle_table(op.Mp), twz = twiddle_table(op.Np);
+void toeplitz_to_fN(int L, int M, int N, int Lp, int Mp, int Np, int device, const double2* G, double2* F,
+                    hipStream_t s) {
+  VP_REQUIRE(Lp >= 2 * L - 1 && Mp >= 2 * M - 1 && Np >= 2 * N - 1, ERR_INVALID,
+             "circulant sizes must be at least 2*D-1 for the tensor's extents D");
+  auto twx = twiddle_table(Lp), twy = twiddle_table(Mp), twz = twiddle_table(Np);
   DevBuf dtx(twx.size() * sizeof(double2)), dty(twy.size() * sizeof(double2)), dtz(twz.size() * sizeof(double2));
   VP_HIP(hipMemcpyAsync(dtx.p, twx.data(), dtx.bytes, hipMemcpyHostToDevice, s));
   VP_HIP(hipMemcpyAsync(dty.p, twy.data(), dty.bytes, hipMemcpyHostToDevice, s));
   VP_HIP(hipMemcpyAsync(dtz.p, twz.data(), dtz.bytes, hipMemcpyHostToDevice, s));
-  DevBuf t1(size_t(op.Lp) * op.M * op.N * 6 * sizeof(double2));
-  DevBuf t2(size_t(op.Lp) * op.Mp * op.N * 6 * sizeof(double2));
+  DevBuf t1(size_t(Lp) * M * N * 6 * sizeof(double2));
+  DevBuf t2(size_t(Lp) * Mp * N * 6 * sizeof(double2));
   auto launch = [&](const double2* in, double2* out, int ax, int D, int P, int e0, int e1, int e2,
                     const double2* tw) {
     const int64_t total = int64_t(e0) * e1 * e2 * 6;
@@ -248,23 +254,23 @@ void toeplitz_to_fN(const vp_op_s& op, const double2* G, double2* F, hipStream_t
   {
     // x pass by FFT when the residue-class length plans and fits LDS; the
     // direct sum otherwise
-    const int R = (op.Lp % 2 == 0) ? 2 : 1;
-    const int S = op.Lp / R;
+    const int R = (Lp % 2 == 0) ? 2 : 1;
+    const int S = Lp / R;
     FftPlan plan{};
     const size_t lds = size_t(2) * S * sizeof(double2);
-    if (factorize(S, true, plan, R) && lds <= size_t(max_lds_bytes(op.ctx->device)) && S >= op.L) {
-      const int64_t npen = int64_t(op.M) * op.N * 6;
+    if (factorize(S, true, plan, R) && lds <= size_t(max_lds_bytes(device)) && S >= L) {
+      const int64_t npen = int64_t(M) * N * 6;
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_embed_dft_x),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-      hipLaunchKernelGGL(k_embed_dft_x, dim3(unsigned(npen)), dim3(256), lds, s, G, t1.as<double2>(), op.L, op.Lp,
-                         R, S, plan, npen, dtx.as<double2>());
+      hipLaunchKernelGGL(k_embed_dft_x, dim3(unsigned(npen)), dim3(256), lds, s, G, t1.as<double2>(), L, Lp, R, S,
+                         plan, npen, dtx.as<double2>());
       VP_HIP(hipGetLastError());
     } else {
-      launch(G, t1.as<double2>(), 0, op.L, op.Lp, op.Lp, op.M, op.N, dtx.as<double2>());
+      launch(G, t1.as<double2>(), 0, L, Lp, Lp, M, N, dtx.as<double2>());
     }
   }
-  launch(t1.as<double2>(), t2.as<double2>(), 1, op.M, op.Mp, op.Lp, op.Mp, op.N, dty.as<double2>());
-  launch(t2.as<double2>(), F, 2, op.N, op.Np, op.Lp, op.Mp, op.Np, dtz.as<double2>());
+  launch(t1.as<double2>(), t2.as<double2>(), 1, M, Mp, Lp, Mp, N, dty.as<double2>());
+  launch(t2.as<double2>(), F, 2, N, Np, Lp, Mp, Np, dtz.as<double2>());
   VP_HIP(hipStreamSynchronize(s));
 }
 
@@ -314,11 +320,11 @@ __global__ void k_sym_check(const double2* __restrict__ F, int Lp, int Mp, int N
   }
 }
 
-bool fN_parity_symmetric(const vp_op_s& op, const double2* F, hipStream_t s) {
-  const int64_t total = int64_t(op.Lp) * op.Mp * op.Np * 6;
+bool fN_parity_symmetric(int Lp, int Mp, int Np, const double2* F, hipStream_t s) {
+  const int64_t total = int64_t(Lp) * Mp * Np * 6;
   const int blocks = int(std::min<int64_t>((total + 255) / 256, 4096));
   DevBuf part(size_t(2) * blocks * sizeof(double));
-  hipLaunchKernelGGL(k_sym_check, dim3(blocks), dim3(256), 0, s, F, op.Lp, op.Mp, op.Np, part.as<double>());
+  hipLaunchKernelGGL(k_sym_check, dim3(blocks), dim3(256), 0, s, F, Lp, Mp, Np, part.as<double>());
   VP_HIP(hipGetLastError());
   std::vector<double> h(size_t(2) * blocks);
   VP_HIP(hipMemcpyAsync(h.data(), part.p, part.bytes, hipMemcpyDeviceToHost, s));

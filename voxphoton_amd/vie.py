@@ -94,7 +94,9 @@ class NOperator:
 
     @classmethod
     def from_toeplitz(cls, G, Mr, Mc, Gram, idx, precisions=('fp64', 'fp32'), fft_shape=(0, 0, 0), ctx=None):
-        """From opToeplitz (L x M x N x 6); the embedding and FFT run on the GPU."""
+        """From opToeplitz (L x M x N x 6); the embedding and FFT run on the GPU.
+        ``fft_shape`` (0, 0, 0) lets the autotuner choose the FFT sizes (at
+        least 2B-1 for the unknowns' box B); any such sizes give the same operator."""
         ctx = ctx or default_context()
         G = np.asarray(G)
         Lg, Mg, Ng = G.shape[:3]
@@ -109,7 +111,8 @@ class NOperator:
 
     @classmethod
     def assembled(cls, shape, dx, ko, Mr, Mc, idx, precisions=('fp64', 'fp32'), fft_shape=(0, 0, 0), ctx=None):
-        """Green tensor assembled on the GPU for voxel size dx and wavenumber ko."""
+        """Green tensor assembled on the GPU for voxel size dx and wavenumber ko
+        (FFT sizes as ``from_toeplitz``)."""
         ctx = ctx or default_context()
         Lg, Mg, Ng = shape
         mr, mc = cls._mats(Mr, Mc)
@@ -131,6 +134,15 @@ class NOperator:
         check(L.lib.vp_op_apply(self.handle, 1 if transp_flag == 'transp' else 0, ptr(x), ptr(y), L.VP_MEM_HOST,
                                 L.VP_FP64 if prec == 'fp64' else L.VP_FP32))
         return y
+
+    def info(self):
+        """Grid, unknowns' bounding box and the FFT sizes / variant of each
+        precision's plan (vp_op_info)."""
+        v = (ctypes.c_int64 * 17)()
+        check(L.lib.vp_op_info(self.handle, v))
+        v = list(v)
+        return dict(grid=tuple(v[0:3]), box_origin=tuple(v[3:6]), box=tuple(v[6:9]),
+                    fft_fp64=tuple(v[9:12]), fft_fp32=tuple(v[12:15]), variant_fp64=v[15], variant_fp32=v[16])
 
     def apply_device(self, xdev, ydev, prec='fp64', transp=False):
         """Device pointers (ints) in the given precision; no host traffic."""
