@@ -937,6 +937,22 @@ def make_preconditioner(prob, wl, vie, configs):
     wl['axis'] (1 = y, 2 = z); 'scalar': the core contrast on the box.
     """
     n = prob.n
+    if wl['prec'] == 'composite':
+        # block Jacobi over the sub-boxes (prec_fun_disk_idx.m): a 2-level
+        # circulant with the core contrast on each box, embedded on its unknowns
+        parts = []
+        for box in prob.extra['parts']:
+            shape, bidx, sub = prob.box_problem(box)
+            q = vie.Preconditioner.level2_assembled(shape, prob.dx, prob.ko, prob.core_contrast(), 1.0, bidx,
+                                                    damping=wl['damping'])
+            q.set_embedding(n, sub)
+            q.box_shape = tuple(int(v) for v in shape)
+            parts.append(q)
+        P = vie.Preconditioner.composite(parts)
+        P.box_shape = parts[-1].box_shape
+        desc = ('block-Jacobi composite of 2-level circulants (core contrast) on the boxes '
+                f'{[q.box_shape for q in parts]}, damping {wl["damping"]}')
+        return P, desc
     if wl['prec'] == 'face':
         shape, pidx, sub, face = configs.face_box(prob, wl['axis'])
         P = vie.Preconditioner.level2_face(shape, prob.dx, prob.ko, face, pidx, damping=wl['damping'])

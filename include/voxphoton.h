@@ -201,6 +201,18 @@ int vp_prec_create_level2_face(vp_ctx ctx, int64_t L, int64_t M, int64_t N, doub
  * unknown vector of the sub-box unknown i; unknowns not covered pass
  * through unchanged (block-Jacobi, prec_fun_disk_idx.m:1-23). */
 int vp_prec_set_embedding(vp_prec p, int64_t n_outer, const int64_t* sub_of_op, int64_t nsub);
+/*
+ * Block-Jacobi composite (prec_fun_disk_idx.m:1-23): y = x on the unknowns no
+ * part covers, and on each part's unknowns that part's M^{-1} applied to
+ * them.  Every part carries an embedding (vp_prec_set_embedding) into the
+ * same outer vector, and the parts cover disjoint unknowns (VP_ERR_INVALID
+ * otherwise).  The composite refers to the parts without owning them: they
+ * must outlive it.  The reference's resonator setup (resonator_MOR_scripts.m:
+ * 104-109, 165-170, 195-201) pairs a preconditioner on the bus waveguide's
+ * box with a 2-level one on the disk's box; it negates both inverses, which
+ * changes no GMRES iterate, so the parts here keep their own sign.
+ */
+int vp_prec_create_composite(vp_ctx ctx, const vp_prec* parts, int nparts, vp_prec* out);
 /* y = M^{-1} x; same mem/prec conventions as vp_op_apply (the FP32 copy of a
  * preconditioner built for the FP64 operator is scaled by the op's 1/gram so
  * that it inverts the FP32 operator). */

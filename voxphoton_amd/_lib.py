@@ -85,6 +85,7 @@ SIGNATURES = {
     'vp_prec_create_level2_face': (c_int, [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_vp, c_vp, c_i64, c_int,
                                            P(c_vp)]),
     'vp_prec_set_embedding': (c_int, [c_vp, c_i64, c_vp, c_i64]),
+    'vp_prec_create_composite': (c_int, [c_vp, c_vp, c_int, P(c_vp)]),
     'vp_prec_apply': (c_int, [c_vp, c_vp, c_vp, c_int, c_int]),
     'vp_prec_set_fp32_scale': (c_int, [c_vp, c_dbl]),
     'vp_prec_destroy': (c_int, [c_vp]),

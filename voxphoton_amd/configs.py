@@ -97,9 +97,10 @@ class Problem:
         i0 = self.idx - 1
         return self.gram * tau3[i0] * Einc.reshape(-1, order='F')[i0]
 
-    def box_problem(self):
-        """Preconditioner box: (shape, box idx (1-based), sub_of_op (1-based))."""
-        sx, sy, sz = self.box
+    def box_problem(self, box=None):
+        """Preconditioner box (default ``self.box``): (shape, box idx
+        (1-based), sub_of_op (1-based))."""
+        sx, sy, sz = box if box is not None else self.box
         bmask = self.mask[sx, sy, sz]
         bidx = idx_from_mask(bmask)
         # map box unknowns to operator unknowns
@@ -246,6 +247,46 @@ def disk_resonator(radius=2e-6, dx=20e-9, lam=1550e-9, thick=220e-9, margin=2):
     return p
 
 
+def microdisk_bus(radius=10e-6, gap=200e-9, width=500e-9, thick=220e-9, dx=15e-9, lam=1550e-9, L=1400, M=1500,
+                  N=30):
+    """Si microdisk with a straight bus waveguide in SiO2 (the resonator of
+    resonator_MOR_scripts.m).  The disk (``radius``, ``thick``) and the bus
+    (``width`` wide, running along x over the whole grid, ``gap`` from the
+    disk edge in +y) share the z rows of the core; the pair is centred in y
+    and z.  Defaults: radius 10 um, gap 200 nm, 500 nm bus, 220 nm thick,
+    15 nm voxels on 1400 x 1500 x 30.
+
+    ``extra['parts']`` holds the two preconditioner boxes of the composite
+    (prec_fun_disk_idx.m): the bus box (all of x, the bus rows) and the disk's
+    bounding box, which are disjoint."""
+    R = radius / dx
+    wb = int(round(width / dx))
+    gp = max(int(round(gap / dx)), 1)
+    tz = int(round(thick / dx))
+    d = int(round(2 * R))
+    span = d + gp + wb
+    if d + 2 > L or span + 2 > M or tz + 2 > N:
+        raise ValueError('grid too small for the resonator')
+    y0 = (M - span) // 2                  # first disk row
+    yc = y0 + (d - 1) / 2.0
+    xc = (L - 1) / 2.0
+    z0 = (N - tz) // 2
+    ii, jj = np.meshgrid(np.arange(L), np.arange(M), indexing='ij')
+    disk = ((ii - xc) ** 2 + (jj - yc) ** 2) <= R ** 2
+    yb = y0 + d + gp                       # first bus row
+    mask2 = disk.copy()
+    mask2[:, yb:yb + wb] = True
+    mask = np.zeros((L, M, N), bool)
+    mask[:, :, z0:z0 + tz] = mask2[:, :, None]
+    eps = np.where(mask, EPS_SI, 1.0).astype(np.complex128)
+    p = Problem('microdisk_bus', L, M, N, dx, lam, mask, eps)
+    p.box = _bbox(mask)
+    dbox = _bbox(np.where(np.arange(M)[None, :, None] < yb, mask, False))
+    bbox = (slice(0, L), slice(yb, yb + wb), slice(z0, z0 + tz))
+    p.extra = dict(radius=radius, gap=gap, width=width, parts=[bbox, dbox])
+    return p
+
+
 def smooth_length(n, primes=(2, 3, 5, 7, 11)):
     """Smallest m >= n whose prime factors are all in ``primes`` (lengths the
     device FFT plans in every precision)."""
@@ -289,4 +330,6 @@ def config_problem(name):
         return bragg_grating()
     if name == 'bragg_20nm':
         return bragg_grating(dx=20e-9, M=50, N=22)
+    if name == 'microdisk_15nm':
+        return microdisk_bus()
     raise KeyError(name)

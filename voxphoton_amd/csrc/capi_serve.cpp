@@ -57,7 +57,8 @@ int64_t request_bytes(const vp_op_s* op, const vp_prec_s* pc, const vp_solve_par
   const int64_t w = int64_t(3) * g.x.P * op->Mb * op->Nb;
   const int64_t w2 = op->p32.plan.variant == 1 ? int64_t(3) * g.x.P * g.y.P * op->Nb : 0;
   b += (w + w2) * (16 + 8);
-  if (pc) b += int64_t(3) * pc->g.V * (16 + 8) * (pc->g.level == 1 ? 2 : 1);
+  for (const vp_prec_s* q : prec_leaves(const_cast<vp_prec_s*>(pc)))
+    b += int64_t(3) * q->g.V * (16 + 8) * (q->g.level == 1 ? 2 : 1);
   return b;
 }
 
@@ -68,13 +69,13 @@ std::string group_key(const vp_op_s* o, const vp_prec_s* q, const vp_solve_param
                 o->m0, o->n0, o->Lb, o->Mb, o->Nb, static_cast<long long>(o->n), o->structured,
                 o->p64.plan.variant, o->p32.plan.variant, o->p32.plan.fsym, o->scale32);
   std::string k(buf);
-  if (q)
-    std::snprintf(buf, sizeof(buf), "P%d %d %d %lld %d %lld %d %016llx %016llx|", q->g.L, q->g.M, q->g.N,
-                  static_cast<long long>(q->nbox), q->g.level, static_cast<long long>(q->g.n_outer), q->g.bsym,
-                  static_cast<unsigned long long>(q->pos_hash), static_cast<unsigned long long>(q->sub_hash));
-  else
-    std::snprintf(buf, sizeof(buf), "P-|");
-  k += buf;
+  if (!q) k += "P-|";
+  for (const vp_prec_s* l : prec_leaves(const_cast<vp_prec_s*>(q))) {
+    std::snprintf(buf, sizeof(buf), "P%d %d %d %lld %d %lld %d %016llx %016llx|", l->g.L, l->g.M, l->g.N,
+                  static_cast<long long>(l->nbox), l->g.level, static_cast<long long>(l->g.n_outer), l->g.bsym,
+                  static_cast<unsigned long long>(l->pos_hash), static_cast<unsigned long long>(l->sub_hash));
+    k += buf;
+  }
   std::snprintf(buf, sizeof(buf), "%.17g %d %.17g %d %d %d", p.tol, p.restart, p.inner_tol, p.inner_maxit,
                 p.max_outer, p.side);
   return k + buf;

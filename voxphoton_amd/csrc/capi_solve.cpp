@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <memory>
 #include <vector>
 #include "krylov.hpp"
 #include "capi_common.hpp"
@@ -165,10 +166,15 @@ __global__ void k_scale_c128(double2* a, int64_t n, double s) {
 
 template <typename T>
 static void prec_apply_dev(vp_prec_s* p, const C<T>* x, C<T>* y) {
-  PrecPrec<T>& P = pprec_of<T>(p);
-  VP_REQUIRE(P.present, ERR_INVALID, "preconditioner lacks this precision");
-  PcArgs<T> a = prec_args<T>(p, x, y);
-  launch_prec<T>(p->g, a, 1, p->ctx->stream);
+  const auto leaves = prec_leaves(p);
+  for (size_t k = 0; k < leaves.size(); ++k) {
+    vp_prec_s* q = leaves[k];
+    PrecPrec<T>& P = pprec_of<T>(q);
+    VP_REQUIRE(P.present, ERR_INVALID, "preconditioner lacks this precision");
+    PcArgs<T> a = prec_args<T>(q, x, y);
+    a.skip_copy = k > 0 ? 1 : 0;
+    launch_prec<T>(q->g, a, 1, p->ctx->stream);
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -224,54 +230,69 @@ void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* 
   VP_HIP(hipMemcpy(ps.probs.p, h.data(), ps.probs.bytes, hipMemcpyHostToDevice));
   ps.has_prec = precs && precs[0];
   if (ps.has_prec) {
-    vp_prec_s* q0 = precs[0];
-    std::vector<const C<T>*> bps(np);
-    std::vector<const uint32_t*> hps(np, nullptr);
-    std::vector<float> hsc(np, 1.f);
-    bool half = true;
+    // every problem's preconditioner has the same parts (one, or the parts
+    // of a composite); part k of all problems runs as one batched launch
+    const auto leaves0 = prec_leaves(precs[0]);
+    const size_t K = leaves0.size();
+    std::vector<std::vector<vp_prec_s*>> lv(np);
     for (int p = 0; p < np; ++p) {
-      vp_prec_s* q = precs[p];
-      VP_REQUIRE(q && q->g.L == q0->g.L && q->g.M == q0->g.M && q->g.N == q0->g.N && q->nbox == q0->nbox &&
-                     q->g.level == q0->g.level && q->g.n_outer == q0->g.n_outer && q->g.bsym == q0->g.bsym,
-                 ERR_INVALID, "batched preconditioners must share one geometry");
-      // the batched launch reads problem 0's voxel map and embedding for all
-      VP_REQUIRE(q->pos_hash == q0->pos_hash && q->sub_hash == q0->sub_hash && q->sub.bytes == q0->sub.bytes,
-                 ERR_INVALID, "batched preconditioners must share one voxel map and embedding");
-      if (std::is_same<T, float>::value && (!q->p32.present || q->scale32 != ops[p]->scale32))
-        prec_make_fp32(q, ops[p]->scale32);
-      PrecPrec<T>& Q = pprec_of<T>(q);
-      VP_REQUIRE(Q.present, ERR_INVALID, "preconditioner lacks this precision");
-      bps[p] = Q.blocks.template as<C<T>>();
-      half = half && Q.hblocks.p != nullptr;
-      if (Q.hblocks.p) {
-        hps[p] = Q.hblocks.template as<uint32_t>();
-        hsc[p] = Q.hscale;
+      lv[p] = prec_leaves(precs[p]);
+      VP_REQUIRE(lv[p].size() == K, ERR_INVALID, "batched preconditioners must have the same parts");
+    }
+    ps.pparts.clear();
+    for (size_t k = 0; k < K; ++k) {
+      auto part = std::make_unique<typename ProblemSet<T>::PrecPart>();
+      vp_prec_s* q0 = leaves0[k];
+      std::vector<const C<T>*> bps(np);
+      std::vector<const uint32_t*> hps(np, nullptr);
+      std::vector<float> hsc(np, 1.f);
+      bool half = true;
+      for (int p = 0; p < np; ++p) {
+        vp_prec_s* q = lv[p][k];
+        VP_REQUIRE(q && q->g.L == q0->g.L && q->g.M == q0->g.M && q->g.N == q0->g.N && q->nbox == q0->nbox &&
+                       q->g.level == q0->g.level && q->g.n_outer == q0->g.n_outer && q->g.bsym == q0->g.bsym,
+                   ERR_INVALID, "batched preconditioners must share one geometry");
+        // the batched launch reads problem 0's voxel map and embedding for all
+        VP_REQUIRE(q->pos_hash == q0->pos_hash && q->sub_hash == q0->sub_hash && q->sub.bytes == q0->sub.bytes,
+                   ERR_INVALID, "batched preconditioners must share one voxel map and embedding");
+        if (std::is_same<T, float>::value && (!q->p32.present || q->scale32 != ops[p]->scale32))
+          prec_make_fp32(q, ops[p]->scale32);
+        PrecPrec<T>& Q = pprec_of<T>(q);
+        VP_REQUIRE(Q.present, ERR_INVALID, "preconditioner lacks this precision");
+        bps[p] = Q.blocks.template as<C<T>>();
+        half = half && Q.hblocks.p != nullptr;
+        if (Q.hblocks.p) {
+          hps[p] = Q.hblocks.template as<uint32_t>();
+          hsc[p] = Q.hscale;
+        }
       }
+      const int64_t outer = q0->g.n_outer ? q0->g.n_outer : q0->nbox;
+      VP_REQUIRE(outer == ps.n, ERR_INVALID, "preconditioner size does not match the operator");
+      part->pg = q0->g;
+      part->pblocks.alloc(sizeof(const C<T>*) * np);
+      VP_HIP(hipMemcpy(part->pblocks.p, bps.data(), part->pblocks.bytes, hipMemcpyHostToDevice));
+      const int64_t wst = 3 * q0->g.V;
+      part->pwork.alloc(size_t(wst) * np * sizeof(C<T>));
+      if (q0->g.level == 1) part->pwork2.alloc(size_t(wst) * np * sizeof(C<T>));
+      PcArgs<T> a = prec_args<T>(q0, nullptr, nullptr);
+      a.work = part->pwork.template as<C<T>>();
+      a.work2 = q0->g.level == 1 ? part->pwork2.template as<C<T>>() : nullptr;
+      a.wstride = wst;
+      a.blocks = part->pblocks.template as<const C<T>*>();
+      a.hblocks = nullptr;
+      a.hscale = nullptr;
+      a.skip_copy = k > 0 ? 1 : 0;
+      if (half) {   // every problem's preconditioner has the binary16 copy
+        part->phblocks.alloc(sizeof(const uint32_t*) * np);
+        VP_HIP(hipMemcpy(part->phblocks.p, hps.data(), part->phblocks.bytes, hipMemcpyHostToDevice));
+        part->phscale.alloc(sizeof(float) * np);
+        VP_HIP(hipMemcpy(part->phscale.p, hsc.data(), part->phscale.bytes, hipMemcpyHostToDevice));
+        a.hblocks = part->phblocks.template as<const uint32_t*>();
+        a.hscale = part->phscale.template as<float>();
+      }
+      part->pa = a;
+      ps.pparts.push_back(std::move(part));
     }
-    const int64_t outer = q0->g.n_outer ? q0->g.n_outer : q0->nbox;
-    VP_REQUIRE(outer == ps.n, ERR_INVALID, "preconditioner size does not match the operator");
-    ps.pg = q0->g;
-    ps.pblocks.alloc(sizeof(const C<T>*) * np);
-    VP_HIP(hipMemcpy(ps.pblocks.p, bps.data(), ps.pblocks.bytes, hipMemcpyHostToDevice));
-    const int64_t wst = 3 * q0->g.V;
-    ps.pwork.alloc(size_t(wst) * np * sizeof(C<T>));
-    if (q0->g.level == 1) ps.pwork2.alloc(size_t(wst) * np * sizeof(C<T>));
-    PcArgs<T> a = prec_args<T>(q0, nullptr, nullptr);
-    a.work = ps.pwork.template as<C<T>>();
-    a.work2 = q0->g.level == 1 ? ps.pwork2.template as<C<T>>() : nullptr;
-    a.wstride = wst;
-    a.blocks = ps.pblocks.template as<const C<T>*>();
-    a.hblocks = nullptr;
-    a.hscale = nullptr;
-    if (half) {   // every problem's preconditioner has the binary16 copy
-      ps.phblocks.alloc(sizeof(const uint32_t*) * np);
-      VP_HIP(hipMemcpy(ps.phblocks.p, hps.data(), ps.phblocks.bytes, hipMemcpyHostToDevice));
-      ps.phscale.alloc(sizeof(float) * np);
-      VP_HIP(hipMemcpy(ps.phscale.p, hsc.data(), ps.phscale.bytes, hipMemcpyHostToDevice));
-      a.hblocks = ps.phblocks.template as<const uint32_t*>();
-      a.hscale = ps.phscale.template as<float>();
-    }
-    ps.pa = a;
   }
 }
 
@@ -570,6 +591,7 @@ int vp_prec_create_level2_face(vp_ctx ctx, int64_t L, int64_t M, int64_t N, doub
 int vp_prec_set_embedding(vp_prec p, int64_t n_outer, const int64_t* sub_of_op, int64_t nsub) {
   VP_API_BEGIN
   VP_REQUIRE(p && sub_of_op, ERR_INVALID, "p/sub is NULL");
+  VP_REQUIRE(p->parts.empty(), ERR_INVALID, "a composite takes its parts' embeddings");
   VP_REQUIRE(nsub == p->nbox, ERR_INVALID, "embedding length must equal the number of box unknowns");
   VP_REQUIRE(n_outer >= nsub, ERR_INVALID, "outer vector shorter than the box");
   std::vector<int> h(static_cast<size_t>(nsub));
@@ -584,6 +606,7 @@ int vp_prec_set_embedding(vp_prec p, int64_t n_outer, const int64_t* sub_of_op, 
   p->sub.alloc(h.size() * sizeof(int));
   VP_HIP(hipMemcpy(p->sub.p, h.data(), p->sub.bytes, hipMemcpyHostToDevice));
   p->sub_hash = map_hash(h);
+  p->sub_host = h;
   p->g.n_outer = n_outer;
   p->p64.xs.alloc(size_t(n_outer) * sizeof(double2));
   p->p64.ys.alloc(size_t(n_outer) * sizeof(double2));
@@ -598,7 +621,17 @@ int vp_prec_set_fp32_scale(vp_prec p, double scale) {
   VP_API_BEGIN
   VP_REQUIRE(p && std::isfinite(scale) && scale > 0, ERR_INVALID, "bad scale");
   VP_HIP(hipSetDevice(p->ctx->device));
-  prec_make_fp32(p, scale);
+  for (vp_prec_s* q : prec_leaves(p)) {
+    prec_make_fp32(q, scale);
+    if (q->g.n_outer) {
+      q->p32.xs.alloc(size_t(q->g.n_outer) * sizeof(float2));
+      q->p32.ys.alloc(size_t(q->g.n_outer) * sizeof(float2));
+    }
+  }
+  if (!p->parts.empty()) {
+    p->scale32 = scale;
+    p->p32.present = true;
+  }
   if (p->g.n_outer) {
     p->p32.xs.alloc(size_t(p->g.n_outer) * sizeof(float2));
     p->p32.ys.alloc(size_t(p->g.n_outer) * sizeof(float2));
@@ -635,6 +668,64 @@ int vp_prec_apply(vp_prec p, const void* x, void* y, int mem, int prec) {
   } else {
     throw VpError{ERR_INVALID, "mem must be VP_MEM_HOST or VP_MEM_DEVICE"};
   }
+  VP_API_END
+}
+
+int vp_prec_create_composite(vp_ctx ctx, const vp_prec* parts, int nparts, vp_prec* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && parts && out && nparts > 0, ERR_INVALID, "bad composite arguments");
+  int64_t n_outer = -1, nbox = 0;
+  bool has64 = true, has32 = true;
+  for (int k = 0; k < nparts; ++k) {
+    vp_prec_s* q = parts[k];
+    VP_REQUIRE(q && q->ctx == ctx, ERR_INVALID, "composite parts must be preconditioners of this context");
+    VP_REQUIRE(q->parts.empty(), ERR_INVALID, "a composite part must not itself be a composite");
+    VP_REQUIRE(q->g.n_outer > 0 && !q->sub_host.empty(), ERR_INVALID,
+               "every composite part needs an embedding (vp_prec_set_embedding)");
+    VP_REQUIRE(n_outer < 0 || q->g.n_outer == n_outer, ERR_INVALID,
+               "composite parts must embed into one outer vector");
+    n_outer = q->g.n_outer;
+    nbox += q->nbox;
+    has64 = has64 && q->p64.present;
+    has32 = has32 && q->p32.present;
+    for (int j = 0; j < k; ++j) VP_REQUIRE(parts[j] != q, ERR_INVALID, "a part appears twice");
+  }
+  VP_REQUIRE(has64 || has32, ERR_INVALID, "the parts share no precision");
+  // disjoint index sets: block Jacobi
+  std::vector<char> seen(size_t(n_outer), 0);
+  for (int k = 0; k < nparts; ++k)
+    for (int i : parts[k]->sub_host) {
+      VP_REQUIRE(!seen[size_t(i)], ERR_INVALID, "composite parts must cover disjoint unknowns");
+      seen[size_t(i)] = 1;
+    }
+  VP_HIP(hipSetDevice(ctx->device));
+  auto* p = new vp_prec_s();
+  try {
+    p->ctx = ctx;
+    p->parts.assign(parts, parts + nparts);
+    p->g = parts[0]->g;          // outer length and a representative geometry
+    p->g.n_outer = n_outer;
+    p->nbox = nbox;
+    p->scale32 = parts[0]->scale32;
+    p->p64.present = has64;
+    p->p32.present = has32;
+    uint64_t h = 1469598103934665603ull;
+    for (int k = 0; k < nparts; ++k) h = (h ^ parts[k]->pos_hash ^ (parts[k]->sub_hash << 1)) * 1099511628211ull;
+    p->pos_hash = h;
+    p->sub_hash = h;
+    if (has64) {
+      p->p64.xs.alloc(size_t(n_outer) * sizeof(double2));
+      p->p64.ys.alloc(size_t(n_outer) * sizeof(double2));
+    }
+    if (has32) {
+      p->p32.xs.alloc(size_t(n_outer) * sizeof(float2));
+      p->p32.ys.alloc(size_t(n_outer) * sizeof(float2));
+    }
+  } catch (...) {
+    delete p;
+    throw;
+  }
+  *out = p;
   VP_API_END
 }
 

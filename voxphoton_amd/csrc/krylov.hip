@@ -584,13 +584,15 @@ void ps_apply_A(ProblemSet<T>& ps, const C<T>* x, int64_t xs, C<T>* y, int64_t y
 }
 template <typename T>
 void ps_apply_M(ProblemSet<T>& ps, const C<T>* x, int64_t xs, C<T>* y, int64_t ys, const int* active) {
-  PcArgs<T> a = ps.pa;
-  a.x = x;
-  a.y = y;
-  a.xstride = xs;
-  a.ystride = ys;
-  a.active = active;
-  launch_prec<T>(ps.pg, a, ps.nprob, ps.stream);
+  for (auto& part : ps.pparts) {
+    PcArgs<T> a = part->pa;
+    a.x = x;
+    a.y = y;
+    a.xstride = xs;
+    a.ystride = ys;
+    a.active = active;
+    launch_prec<T>(part->pg, a, ps.nprob, ps.stream);
+  }
 }
 template void ps_apply_A<float>(ProblemSet<float>&, const float2*, int64_t, float2*, int64_t, const int*);
 template void ps_apply_A<double>(ProblemSet<double>&, const double2*, int64_t, double2*, int64_t, const int*);

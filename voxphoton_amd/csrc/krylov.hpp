@@ -4,6 +4,7 @@
 #pragma once
 #include <cmath>
 #include <limits>
+#include <memory>
 #include <vector>
 #include "precond.hpp"
 
@@ -32,12 +33,16 @@ struct ProblemSet {
   const C<T>* twx = nullptr;
   const C<T>* twy = nullptr;
   const C<T>* twz = nullptr;
-  // preconditioner (optional)
+  // preconditioner (optional): one part, or the parts of block-Jacobi
+  // composites applied one after the other
+  struct PrecPart {
+    PcGeom pg{};
+    PcArgs<T> pa{};                 // x, y, active, strides filled at call time
+    DevBuf pblocks, pwork, pwork2;  // block pointer array, workspaces
+    DevBuf phblocks, phscale;       // binary16 block pointers and scales (FP32 quadrant form)
+  };
   bool has_prec = false;
-  PcGeom pg{};
-  PcArgs<T> pa{};                 // x, y, active, strides filled at call time
-  DevBuf pblocks, pwork, pwork2;  // block pointer array, workspaces
-  DevBuf phblocks, phscale;       // binary16 block pointers and scales (FP32 quadrant form)
+  std::vector<std::unique_ptr<PrecPart>> pparts;
   DevBuf active;                  // int[nprob]
 };
 

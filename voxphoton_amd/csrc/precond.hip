@@ -600,7 +600,7 @@ static void launch_prec_pct(const PcGeom& g, const PcArgs<T>& a, int nprob, hipS
 
 template <typename T>
 void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) {
-  if (a.sub && g.n_outer > 0) {
+  if (a.sub && g.n_outer > 0 && !a.skip_copy) {
     const int blocks = int(std::min<int64_t>((g.n_outer + 255) / 256, 4096));
     hipLaunchKernelGGL(k_copy_vec<T>, dim3(blocks, nprob), dim3(256), 0, s, a.x, a.y, g.n_outer, a.xstride,
                        a.ystride, a.active);

@@ -48,6 +48,10 @@ struct PcArgs {
   // scaled by 1/hscale (nullptr: the blocks above)
   const uint32_t* const* hblocks;
   const float* hscale;
+  // embedded apply: 0 copies x to y over the outer vector first (unknowns
+  // outside the box pass through); 1 leaves y as it is (a later part of a
+  // block-Jacobi composite, whose earlier parts already wrote y)
+  int skip_copy;
 };
 
 template <typename T>
@@ -96,6 +100,11 @@ struct vp_prec_s {
   // content hashes of the voxel map and the embedding: batched launches
   // share one map, so only preconditioners with equal maps batch together
   uint64_t pos_hash = 0, sub_hash = 0;
+  std::vector<int> sub_host;   // the embedding (0-based outer positions), host copy
+  // block-Jacobi composite (prec_fun_disk_idx): the parts, each embedded in
+  // the same outer vector on disjoint unknowns; not owned (they outlive the
+  // composite).  Empty for a preconditioner with its own blocks.
+  std::vector<vp_prec_s*> parts;
   vp::PrecPrec<double> p64;
   vp::PrecPrec<float> p32;
 };
@@ -109,6 +118,14 @@ inline uint64_t map_hash(const std::vector<int>& v) {
     h *= 1099511628211ull;
   }
   return h ^ uint64_t(v.size());
+}
+
+// The preconditioners whose blocks an apply runs: the parts of a composite,
+// else the preconditioner itself.
+inline std::vector<vp_prec_s*> prec_leaves(vp_prec_s* p) {
+  if (!p) return {};
+  if (!p->parts.empty()) return p->parts;
+  return {p};
 }
 
 template <typename T> PrecPrec<T>& pprec_of(vp_prec_s* p);

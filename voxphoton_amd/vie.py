@@ -275,6 +275,19 @@ class Preconditioner:
                                                ix.size, _prec_mask(precisions), ctypes.byref(h)))
         return cls(h, ctx, ix.size)
 
+    @classmethod
+    def composite(cls, parts, ctx=None):
+        """Block-Jacobi composite of embedded preconditioners on disjoint
+        unknowns (prec_fun_disk_idx.m:1-23); keeps the parts alive."""
+        ctx = ctx or parts[0].ctx
+        hs = (ctypes.c_void_p * len(parts))(*[q.handle.value for q in parts])
+        h = ctypes.c_void_p()
+        check(L.lib.vp_prec_create_composite(ctx.handle, ctypes.cast(hs, ctypes.c_void_p), len(parts),
+                                             ctypes.byref(h)))
+        out = cls(h, ctx, parts[0].n)
+        out.parts = list(parts)
+        return out
+
     def set_embedding(self, n_outer, sub_of_op):
         """Act on an outer vector of length n_outer; box unknown i sits at
         sub_of_op[i] (1-based); other entries pass through (block Jacobi)."""
