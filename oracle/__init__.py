@@ -15,14 +15,21 @@ Pinning (what checks each function against something other than itself):
                                 (closed-form identity of circulant embedding),
                                 adjoint identity, Mc=0 => Gram*Mr*I, linearity.
   * ``vie.circulant_nop_const`` via the dense product above (parity signs).
-  * ``precond.*``               dense F^-1 blkdiag(B) F products; exactly
-                                circulant operators give the identity; the
-                                direct 3Nx3N block construction against the
-                                literal 3MNx3MN restatement; Chan optimality
-                                against least squares.
+  * ``precond.*``               the 1- and 2-level block producers equal the
+                                orthogonal (Frobenius) projection of the dense
+                                operator onto x- / xy-block-circulant matrices;
+                                exactly circulant operators are inverted;
+                                dense F^-1 blkdiag(B) F products for both
+                                applies; chan_x against least squares; the
+                                composite against its two parts.
   * ``krylov.pgmres``           numpy.linalg.solve on small dense systems,
                                 Arnoldi relation, n-step termination.
   * ``greens``                  static closed form of the coincident-square
                                 1/R integral, order-doubling convergence,
-                                brute-force tensor Gauss on far cells.
+                                brute-force tensor Gauss on far cells, and the
+                                reference's own DEMCEM singular values and
+                                tensor cubature (``ref_greens``).
+  * ``ref_greens``              the reference's C++ cubature cores compiled
+                                from the reference sources into ``_ref/`` by
+                                ``build_ref.sh`` (never on the product path).
 """
