@@ -75,6 +75,12 @@ WORKLOADS = {
              'server HBM budget, refinement rounds batched per geometry',
         tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=4, max_outer=20, nreq=40, rate=50.0, seed=4,
         budget_gb=96, max_active=32),
+    'microdisk_15nm': dict(
+        desc='Si microdisk r = 10 um, 220 nm thick, with a 500 nm bus waveguide at a 200 nm gap, SiO2, '
+             'lambda 1550 nm, 15 nm voxels, grid 1400x1500x30 (63 M voxels), y-polarised Gaussian (w0 300 nm) '
+             'centred on the bus; block-Jacobi composite of 2-level circulants on the bus and disk boxes',
+        tol=1e-6, restart=50, inner_tol=1e-2, inner_maxit=40, max_outer=20, prec='composite', damping=0.0,
+        fields=False, cpu_baseline=False),
     'bragg_20nm': dict(
         desc='Bragg grating (100 periods) at 20 nm voxels, grid 1800x50x22 (FFT 3600x100x44), lambda 1550 nm',
         tol=1e-6, restart=100, inner_tol=1e-3, inner_maxit=6, max_outer=12, prec='face', axis=1, damping=0.0),
@@ -378,14 +384,14 @@ def main():
 
     # ---- excitation and post-processing on the same grid
     fields = None
-    if not args.no_fields:
+    if not args.no_fields and wl.get('fields', True):
         fields = field_block(prob, vie, L, ctx, dx, log)
 
     # ---- CPU baseline (FP64 oracle on the host cores); the oracle's matvec
     # of the seeded vector also checks the device matvecs of the same vector
     cpu = None
     mv_check = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and wl.get('cpu_baseline', True):
         cpu, y_ref = cpu_baseline(prob, vie, fp64, log, pc_box=getattr(prec, 'box_shape', None),
                                   restart=wl['restart'])
         yd64 = y64.download(np.empty(n, dtype=complex))

@@ -79,6 +79,9 @@ int vp_ctx_synchronize(vp_ctx ctx);
  * prints tic/toc totals); used by the benchmark's roofline figures. */
 int vp_ctx_set_profiling(vp_ctx ctx, int on);
 int vp_ctx_kernel_stats(vp_ctx ctx, const char* kernel, int64_t* launches, double* total_ms, int reset);
+/* Free and total device memory of the context's GPU (bytes); the free figure
+ * counts the library's cache of released buffers as used. */
+int vp_mem_info(vp_ctx ctx, int64_t* free_bytes, int64_t* total_bytes);
 /* Device memory helpers for callers without their own allocator. */
 int vp_malloc(vp_ctx ctx, int64_t bytes, void** dptr);
 int vp_free(vp_ctx ctx, void* dptr);

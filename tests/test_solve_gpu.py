@@ -345,3 +345,25 @@ def test_pgmresDR_matches_oracle(vp_ctx, side):
     assert flag == flago == 0 and it == ito and len(rv) == len(rvo)
     assert np.allclose(rv, rvo, rtol=1e-6, atol=1e-13)
     assert rel(x, xo) < 1e-8
+
+
+@pytest.mark.parametrize('chany', ['fft', 'direct'])
+@pytest.mark.parametrize('shape', [(6, 10, 2), (5, 12, 3), (4, 49, 2)])
+def test_level2_chan_y_by_fft_matches_oracle(vp_ctx, monkeypatch, shape, chany):
+    # the y-Chan transform of the block builder as one FFT per line (long y
+    # axes) or as direct sums: the same blocks as the oracle's
+    from voxphoton_amd.vie import Preconditioner
+    monkeypatch.setenv('VP_PC_CHANY', chany)
+    L, M, N = shape
+    G = ogr.assemble_G(L, M, N, 20e-9, 5.85e6) if M < 40 else random_toeplitz(L, M, N, np.random.default_rng(3),
+                                                                                 scale=1e-24)
+    mc, gram = 0.8278 + 0.01j, 8e-24
+    idx = ovie.idx_from_mask(np.ones((L, M, N), bool))
+    Binv = opre.level_2_parallel_func(G, mc, 1.0, L, M, N, gram)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
+    P = Preconditioner.level2(G, mc, 1.0, gram, idx, ctx=vp_ctx)
+    e = rel(P.apply(x), ref)
+    print(shape, chany, 'level2 rel', e, flush=True)
+    assert e < 1e-10

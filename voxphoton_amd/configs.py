@@ -81,7 +81,10 @@ class Problem:
         y = (np.arange(self.M) + 0.5) * dx
         z = (np.arange(self.N) + 0.5) * dx
         x = (np.arange(self.L) + 0.5) * dx
+        # centred in the grid, or on the excited guide (extra['rhs_center'], in voxels)
         yc, zc = y.mean(), z.mean()
+        if 'rhs_center' in self.extra:
+            yc, zc = ((np.asarray(self.extra['rhs_center'], float) + 0.5) * dx)
         prof = np.exp(-(((y[None, :, None] - yc) ** 2) + ((z[None, None, :] - zc) ** 2)) / w0 ** 2)
         E = prof * np.exp(-1j * self.ko * x)[:, None, None]
         comp = {'x': 0, 'y': 1, 'z': 2}[pol]
@@ -258,7 +261,8 @@ def microdisk_bus(radius=10e-6, gap=200e-9, width=500e-9, thick=220e-9, dx=15e-9
 
     ``extra['parts']`` holds the two preconditioner boxes of the composite
     (prec_fun_disk_idx.m): the bus box (all of x, the bus rows) and the disk's
-    bounding box, which are disjoint."""
+    bounding box, which are disjoint; ``extra['rhs_center']`` centres the
+    Gaussian excitation on the bus."""
     R = radius / dx
     wb = int(round(width / dx))
     gp = max(int(round(gap / dx)), 1)
@@ -283,8 +287,32 @@ def microdisk_bus(radius=10e-6, gap=200e-9, width=500e-9, thick=220e-9, dx=15e-9
     p.box = _bbox(mask)
     dbox = _bbox(np.where(np.arange(M)[None, :, None] < yb, mask, False))
     bbox = (slice(0, L), slice(yb, yb + wb), slice(z0, z0 + tz))
-    p.extra = dict(radius=radius, gap=gap, width=width, parts=[bbox, dbox])
+    # preconditioner boxes of lengths the device FFTs plan, widened inside
+    # the grid without overlapping each other
+    bbox = _smooth_box(bbox, p.shape, [dbox])
+    dbox = _smooth_box(dbox, p.shape, [bbox])
+    p.extra = dict(radius=radius, gap=gap, width=width, parts=[bbox, dbox],
+                   rhs_center=(yb + (wb - 1) / 2.0, z0 + (tz - 1) / 2.0))
     return p
+
+
+def _overlap(a, b):
+    return all(x.start < y.stop and y.start < x.stop for x, y in zip(a, b))
+
+
+def _smooth_box(box, shape, avoid):
+    """``box`` widened per axis to a length with small prime factors (up or
+    down), inside the grid and disjoint from every box in ``avoid``."""
+    out = list(box)
+    for ax, (sl, ext) in enumerate(zip(box, shape)):
+        w = min(smooth_length(sl.stop - sl.start), ext)
+        for start in (sl.start, sl.stop - w, ext - w, 0):
+            cand = list(out)
+            cand[ax] = slice(max(0, min(start, ext - w)), max(0, min(start, ext - w)) + w)
+            if not any(_overlap(cand, a) for a in avoid):
+                out = cand
+                break
+    return tuple(out)
 
 
 def smooth_length(n, primes=(2, 3, 5, 7, 11)):

@@ -111,6 +111,17 @@ int vp_ctx_kernel_stats(vp_ctx ctx, const char* kernel, int64_t* launches, doubl
   VP_API_END
 }
 
+int vp_mem_info(vp_ctx ctx, int64_t* free_bytes, int64_t* total_bytes) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && free_bytes && total_bytes, ERR_INVALID, "bad arguments");
+  VP_HIP(hipSetDevice(ctx->device));
+  size_t f = 0, t = 0;
+  VP_HIP(hipMemGetInfo(&f, &t));
+  *free_bytes = int64_t(f);
+  *total_bytes = int64_t(t);
+  VP_API_END
+}
+
 int vp_malloc(vp_ctx ctx, int64_t bytes, void** dptr) {
   VP_API_BEGIN
   VP_REQUIRE(ctx && dptr && bytes >= 0, ERR_INVALID, "vp_malloc: bad argument");

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <memory>
@@ -376,6 +377,12 @@ bool ir_step_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2*
     } else {
       any = true;
     }
+  }
+  if (progress_enabled()) {
+    for (int p = 0; p < np; ++p)
+      std::fprintf(stderr, "[vp] refinement: problem %d step %d, FP64 relres %.3e, inner iterations %d%s\n", p,
+                   st[p].steps, st[p].relres, st[p].inner, st[p].done ? " (done)" : "");
+    std::fflush(stderr);
   }
   if (!any) return false;
   VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));

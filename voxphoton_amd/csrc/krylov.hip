@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstdio>
 #include <cstdlib>
 #include "krylov.hpp"
 
@@ -576,6 +577,14 @@ void dev_update_from_f(double2* x, const float2* d, const double* d_norm, int64_
 }
 
 // ----- operator / preconditioner application over a problem set ------------------
+bool progress_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("VP_PROGRESS");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
 template <typename T>
 void ps_apply_A(ProblemSet<T>& ps, const C<T>* x, int64_t xs, C<T>* y, int64_t ys, const int* active) {
   VecArgs<T> va{x, y, xs, ys, active};
@@ -956,6 +965,14 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     VP_HIP(hipMemcpyAsync(kk.p, hk.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
     update_x(dact);
     VP_HIP(hipStreamSynchronize(s));   // iact is released at the end of this cycle
+    if (progress_enabled()) {
+      double worst = 0.0;
+      int nact = 0;
+      for (int p = 0; p < np; ++p)
+        if (hact[p]) { worst = std::max(worst, last[p]); ++nact; }
+      std::fprintf(stderr, "[vp] gmres cycle %d: %d problem(s), worst estimated relres %.3e\n", outit, nact, worst);
+      std::fflush(stderr);
+    }
     // restart: fresh residual (pgmres_iter.m:150 carries it by recurrence)
     for (int p = 0; p < np; ++p) hact[p] = (hact[p] && last[p] > tol_of(p)) ? 1 : 0;
     bool more = false;

@@ -18,6 +18,10 @@ inline double relres_of(double rnorm, double bnorm) {
   return bnorm > 0 ? rnorm / bnorm : 0.0;
 }
 
+// Progress lines on stderr (environment VP_PROGRESS=1): one per GMRES cycle
+// and per refinement step, for long solves whose callers want to see them move.
+bool progress_enabled();
+
 // One set of independent problems of identical geometry solved together.
 // Vector p of a batch sits at base + p * n.
 template <typename T>

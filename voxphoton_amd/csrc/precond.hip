@@ -838,6 +838,72 @@ __device__ __forceinline__ double2 pc_row(int k, int nt, int ns, int t, double2 
   return v;
 }
 
+// The y-Chan transform of the 2-level blocks by FFT, for long y axes.  On
+// one line (k, dz, kx) of the x-Chan coefficients v(t) = ck[k][dz][t][kx],
+// the Chan-weighted sum of k_pc_blocks is, for t >= 1, a column term with
+// weight (M-t)/M on v(t) and a row term with weight t/M on v(M-t); both are
+// the transform of a(t) = (M-t)/M v(t) (a(0) = 0):
+//   A(ky) = sum_{t>=1} a(t) w^{t ky},  sum_{t>=1} t/M v(M-t) w^{t ky} = A(-ky),
+// so an entry is s0 v(0) + s_col A(ky) + s_row A(-ky) with the component's
+// signs away from t = 0 (pc_col, pc_row).  One workgroup per line; the line
+// is read and written with stride L (setup only).
+__global__ void __launch_bounds__(256) k_pc_chan_y_fft(const double2* __restrict__ ck, double2* __restrict__ out,
+                                                       int L, int M, int kxn, FftPlan plan,
+                                                       const double2* __restrict__ twM) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double2* W0 = reinterpret_cast<double2*>(smem_raw);
+  double2* W1 = W0 + M;
+  const int64_t line = blockIdx.x;
+  const int kx = int(line % kxn);
+  const int64_t kd = line / kxn;   // k * N + dz
+  const double2* v = ck + kd * int64_t(M) * L + kx;
+  double2* o = out + kd * int64_t(M) * L + kx;
+  for (int t = threadIdx.x; t < M; t += blockDim.x) {
+    const double w = t == 0 ? 0.0 : double(M - t) / M;
+    const double2 a = v[int64_t(t) * L];
+    W0[t] = make_double2(w * a.x, w * a.y);
+  }
+  __syncthreads();
+  const LdsLayout lay{1, M, 0, 1};
+  const double2* R = lds_fft<double, false>(W0, W1, plan, 1, lay, twM);
+  for (int t = threadIdx.x; t < M; t += blockDim.x) o[int64_t(t) * L] = R[t];
+}
+
+// Block entries from the y-transformed lines (see k_pc_chan_y_fft); same
+// output and face handling as k_pc_blocks.
+__global__ void k_pc_blocks_fft(const double2* __restrict__ ck, const double2* __restrict__ ay,
+                                double2* __restrict__ blk, int L, int M, int N, int kxn, int kyn,
+                                const double2* __restrict__ face, double gram) {
+  const int n3 = 3 * N;
+  const int64_t total = int64_t(kxn) * kyn * n3 * n3;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = e;
+    const int row = int(r % n3); r /= n3;
+    const int colq = int(r % n3); r /= n3;
+    const int ky = int(r % kyn); r /= kyn;
+    const int kx = int(r);
+    const int c = row / N, nt = row - c * N;
+    const int d = colq / N, ns = colq - d * N;
+    const int k = c_pc_comp[c][d];
+    const int dz = nt > ns ? nt - ns : ns - nt;
+    const int64_t base = (int64_t(k) * N + dz) * int64_t(M) * L + kx;
+    double2 acc = pc_col(k, nt, ns, 0, ck[base]);
+    if (M > 1) {
+      const double2 a1 = pc_col(k, nt, ns, 1, ay[base + int64_t(ky) * L]);
+      const double2 a2 = pc_row(k, nt, ns, 1, ay[base + int64_t(ky ? M - ky : 0) * L]);
+      acc.x += a1.x + a2.x;
+      acc.y += a1.y + a2.y;
+    }
+    if (face) {
+      const double2 f = face[nt];
+      acc = make_double2(-(f.x * acc.x - f.y * acc.y), -(f.x * acc.y + f.y * acc.x));
+      if (row == colq) acc.x += gram;
+    }
+    blk[e] = acc;
+  }
+}
+
 // With ``face`` set, the entry becomes Gram*delta - face(n_t) * N2 entry
 // (the face-averaged form inv(Gram*I - diag(face3)*circ_N) of
 // Bragg_MOR_script.m:64-72, here with the exact axis last).
@@ -1013,9 +1079,31 @@ void build_level2_blocks(const double2* G, int L, int M, int N, double2 mc, doub
                        dL.as<double2>());
     VP_HIP(hipGetLastError());
   }
-  {
-    const int64_t total = int64_t(kxn) * kyn * 9 * N * N;
-    const int blocks = int(std::min<int64_t>((total + 255) / 256, 1 << 20));
+  // y-Chan transform: direct sums (O(M) per entry) for short y axes, one FFT
+  // per (component, dz, kx) line for long ones (VP_PC_CHANY=direct|fft|auto)
+  int dev = 0;
+  VP_HIP(hipGetDevice(&dev));
+  const char* ey = std::getenv("VP_PC_CHANY");
+  bool use_fft = M > 48;
+  if (ey && !std::strcmp(ey, "direct")) use_fft = false;
+  if (ey && !std::strcmp(ey, "fft")) use_fft = true;
+  FftPlan yplan{};
+  const size_t ldsy = size_t(2) * M * sizeof(double2);
+  if (use_fft && (M < 2 || !factorize(M, true, yplan, 1) || ldsy > size_t(max_lds_bytes(dev)))) use_fft = false;
+  const int64_t total = int64_t(kxn) * kyn * 9 * N * N;
+  const int blocks = int(std::min<int64_t>((total + 255) / 256, 1 << 20));
+  if (use_fft) {
+    DevBuf ay(size_t(6) * L * M * N * sizeof(double2));
+    const void* fy = reinterpret_cast<const void*>(&k_pc_chan_y_fft);
+    (void)hipFuncSetAttribute(fy, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsy));
+    hipLaunchKernelGGL(k_pc_chan_y_fft, dim3(unsigned(int64_t(6) * N * kxn)), dim3(256), ldsy, s, ck.as<double2>(),
+                       ay.as<double2>(), L, M, kxn, yplan, dM.as<double2>());
+    VP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pc_blocks_fft, dim3(blocks), dim3(256), 0, s, ck.as<double2>(), ay.as<double2>(), blk, L,
+                       M, N, kxn, kyn, face_dev, gram);
+    VP_HIP(hipGetLastError());
+    VP_HIP(hipStreamSynchronize(s));
+  } else {
     hipLaunchKernelGGL(k_pc_blocks, dim3(blocks), dim3(256), 0, s, ck.as<double2>(), blk, L, M, N, kxn, kyn,
                        dM.as<double2>(), face_dev, gram);
     VP_HIP(hipGetLastError());

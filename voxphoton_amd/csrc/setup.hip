@@ -233,6 +233,43 @@ __global__ void __launch_bounds__(256) k_embed_dft_x(const double2* __restrict__
   }
 }
 
+// The y and z passes as FFTs over strided lines: line (i, o) of the input
+// (extent D along the axis, stride `inner`, base i + inner*D*o) is embedded
+// as in k_embed_dft_x (C(0) = G(0), zero for components odd along the axis;
+// C(i) = G(i), C(P-i) = s G(i)) and transformed at full length P in LDS;
+// the output line has stride `inner` and base i + inner*P*o.  o runs over
+// the outer extent times the six components (o / nouter6 is the component).
+__global__ void __launch_bounds__(256) k_embed_dft_line(const double2* __restrict__ G, double2* __restrict__ F,
+                                                        int ax, int D, int P, FftPlan plan, int64_t inner,
+                                                        int64_t nouter6, const double2* __restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double2* W0 = reinterpret_cast<double2*>(smem_raw);
+  double2* W1 = W0 + P;
+  const int64_t line = blockIdx.x;
+  const int64_t i = line % inner, o = line / inner;
+  const int k = int(o / nouter6);
+  const bool odd = c_parity[k][ax] < 0;
+  const double sg = odd ? -1.0 : 1.0;
+  const double2* g = G + i + inner * D * o;
+  double2* f = F + i + inner * P * o;
+  for (int j = threadIdx.x; j < P; j += blockDim.x) {
+    double2 v = make_double2(0.0, 0.0);
+    if (j == 0) {
+      if (!odd) v = g[0];
+    } else if (j < D) {
+      v = g[int64_t(j) * inner];
+    } else if (j > P - D) {
+      const double2 u = g[int64_t(P - j) * inner];
+      v = make_double2(sg * u.x, sg * u.y);
+    }
+    W0[j] = v;
+  }
+  __syncthreads();
+  const LdsLayout lay{1, P, 0, 1};
+  const double2* R = lds_fft<double, false>(W0, W1, plan, 1, lay, tw);
+  for (int j = threadIdx.x; j < P; j += blockDim.x) f[int64_t(j) * inner] = R[j];
+}
+
 void toeplitz_to_fN(int L, int M, int N, int Lp, int Mp, int Np, int device, const double2* G, double2* F,
                     hipStream_t s) {
   VP_REQUIRE(Lp >= 2 * L - 1 && Mp >= 2 * M - 1 && Np >= 2 * N - 1, ERR_INVALID,
@@ -269,8 +306,26 @@ void toeplitz_to_fN(int L, int M, int N, int Lp, int Mp, int Np, int device, con
       launch(G, t1.as<double2>(), 0, L, Lp, Lp, M, N, dtx.as<double2>());
     }
   }
-  launch(t1.as<double2>(), t2.as<double2>(), 1, M, Mp, Lp, Mp, N, dty.as<double2>());
-  launch(t2.as<double2>(), F, 2, N, Np, Lp, Mp, Np, dtz.as<double2>());
+  // y and z passes: FFTs of the embedded lines where the length plans and
+  // fits LDS and the direct sums would be long (D > 16), else direct sums
+  auto pass = [&](const double2* in, double2* out, int ax, int D, int P, int e0, int e1, int e2, const double2* tw) {
+    FftPlan plan{};
+    const size_t lds = size_t(2) * P * sizeof(double2);
+    if (D > 16 && factorize(P, true, plan, 1) && lds <= size_t(max_lds_bytes(device))) {
+      const int64_t inner = ax == 1 ? int64_t(e0) : int64_t(e0) * e1;
+      const int64_t nouter6 = ax == 1 ? int64_t(e2) : 1;
+      const int64_t lines = inner * nouter6 * 6;
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_embed_dft_line),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+      hipLaunchKernelGGL(k_embed_dft_line, dim3(unsigned(lines)), dim3(256), lds, s, in, out, ax, D, P, plan, inner,
+                         nouter6, tw);
+      VP_HIP(hipGetLastError());
+    } else {
+      launch(in, out, ax, D, P, e0, e1, e2, tw);
+    }
+  };
+  pass(t1.as<double2>(), t2.as<double2>(), 1, M, Mp, Lp, Mp, N, dty.as<double2>());
+  pass(t2.as<double2>(), F, 2, N, Np, Lp, Mp, Np, dtz.as<double2>());
   VP_HIP(hipStreamSynchronize(s));
 }
 

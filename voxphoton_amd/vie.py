@@ -32,6 +32,12 @@ class Context:
     def synchronize(self):
         check(L.lib.vp_ctx_synchronize(self.handle))
 
+    def mem_info(self):
+        """(free, total) device memory in bytes (vp_mem_info)."""
+        f, t = ctypes.c_int64(), ctypes.c_int64()
+        check(L.lib.vp_mem_info(self.handle, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
+
     def close(self):
         self._fin()
 
