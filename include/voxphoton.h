@@ -251,6 +251,22 @@ int vp_gmres(vp_op op, vp_prec M, int side, const void* b, const void* x0, void*
              double tol, int maxit, vp_gmres_info* info);
 
 /*
+ * Preconditioned BiCGSTAB: the 'B' solver of the reference's harness
+ * (JVIE_Solver_const.m:469-494, MATLAB's bicgstab(A,b,tol,maxit,M)), in the
+ * published form of van der Vorst (1992) with M (nullable) applied to the
+ * search directions.  Stops when ||b - A x|| <= tol ||b|| (a candidate from
+ * the recurrence is confirmed on the true residual) or after maxit
+ * iterations.  info: flag 0 converged, 1 not converged, 4 breakdown
+ * (<r0, r> or omega zero); relres the final ||b - A x|| / ||b||; resvec the
+ * relative residual at the start and after every half iteration;
+ * total_iters the half iterations, iter_inner the whole iterations and
+ * iter_outer 1 when it stopped at a half iteration (MATLAB's iter = 3.5).
+ * b, x0 (nullable), x, mem and prec as vp_gmres.
+ */
+int vp_bicgstab(vp_op op, vp_prec M, const void* b, const void* x0, void* x, int mem, int prec, double tol,
+                int maxit, vp_gmres_info* info);
+
+/*
  * Mixed precision: FP64 iterative refinement around FP32 GMRES.  Outer
  * residual r = b - A x in FP64 on the device; inner FP32 GMRES(restart)
  * with the FP32 preconditioner solves A d = r to inner_tol, or to

@@ -413,3 +413,33 @@ def test_lagrange_int_reproduces_polynomials():
     F = [np.full((2, 2), np.polyval(coef, si)) for si in s]
     for S in (1.1, 1.5, 2.0):
         assert np.abs(omor.lagrange_int(F, S, s) - np.polyval(coef, S)).max() < 1e-12
+
+
+def test_bicgstab_matches_direct_solve_and_true_residuals():
+    rng = np.random.default_rng(12)
+    n = 40
+    A = np.eye(n) * 8 + rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    seen = []
+
+    def Af(v):
+        return A @ v
+    x, flag, relres, it, rv = okry.bicgstab(Af, b, 1e-12, 200)
+    assert flag == 0 and relres <= 1e-12
+    assert rel(x, np.linalg.solve(A, b)) < 1e-10
+    assert len(rv) == int(2 * it) + 1
+    # preconditioned with the exact inverse: done after the first half step
+    Ainv = np.linalg.inv(A)
+    x, flag, relres, it, rv = okry.bicgstab(Af, b, 1e-12, 5, M=lambda v: Ainv @ v)
+    assert flag == 0 and it == 0.5 and rel(x, np.linalg.solve(A, b)) < 1e-12
+
+
+def test_bicgstab_recurrence_residual_is_the_true_residual():
+    # in exact arithmetic r_k = b - A x_k: recorded norms equal the true ones
+    rng = np.random.default_rng(13)
+    n = 30
+    A = np.eye(n) * 6 + rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for k in range(1, 8):
+        x, flag, relres, it, rv = okry.bicgstab(lambda v: A @ v, b, 1e-30, k)
+        assert abs(rv[-1] - np.linalg.norm(b - A @ x) / np.linalg.norm(b)) <= 1e-12 * max(rv[-1], 1e-3)

@@ -91,6 +91,7 @@ SIGNATURES = {
     'vp_prec_set_fp32_scale': (c_int, [c_vp, c_dbl]),
     'vp_prec_destroy': (c_int, [c_vp]),
     'vp_gmres': (c_int, [c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_dbl, c_int, P(GmresInfo)]),
+    'vp_bicgstab': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_dbl, c_int, P(GmresInfo)]),
     'vp_solve_ir': (c_int, [c_vp, c_vp, c_int, c_vp, c_vp, c_int, c_dbl, c_int, c_dbl, c_int, c_int,
                             P(GmresInfo)]),
     'vp_solve_ir_batched': (c_int, [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_dbl, c_int, c_dbl, c_int, c_int,

@@ -15,7 +15,7 @@ OUT = os.path.join(HERE, 'libvoxphoton.so')
 OBJDIR = os.path.join(HERE, '_build')
 
 SOURCES = ['plan.cpp', 'setup.hip', 'matvec.hip', 'capi_op.cpp', 'precond.hip', 'krylov.hip',
-           'capi_solve.cpp', 'capi_serve.cpp', 'prof.cpp', 'fields.hip', 'mor.hip']
+           'capi_solve.cpp', 'capi_serve.cpp', 'prof.cpp', 'fields.hip', 'mor.hip', 'bicgstab.hip']
 
 ARCH = os.environ.get('VP_OFFLOAD_ARCH', 'gfx950')
 

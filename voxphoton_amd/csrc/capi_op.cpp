@@ -24,11 +24,11 @@ namespace vp {
 void* ctx_scratch(vp_ctx_s* ctx, size_t bytes) {
   if (bytes > ctx->scratch_bytes) {
     if (ctx->scratch) VP_HIP(hipStreamSynchronize(ctx->stream));
-    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->scratch) dev_free(ctx->scratch);
     ctx->scratch = nullptr;
     ctx->scratch_bytes = 0;
     const size_t b = std::max<size_t>(bytes, size_t(1) << 20);
-    VP_HIP(hipMalloc(&ctx->scratch, b));
+    ctx->scratch = dev_alloc(b);
     ctx->scratch_bytes = b;
   }
   return ctx->scratch;
@@ -63,6 +63,7 @@ int vp_ctx_create(int device, vp_ctx* out) {
     delete ctx;
     VP_HIP(e);
   }
+  register_stream(device, ctx->stream);
   *out = ctx;
   VP_API_END
 }
@@ -72,8 +73,9 @@ int vp_ctx_destroy(vp_ctx ctx) {
   if (!ctx) return VP_OK;
   VP_HIP(hipSetDevice(ctx->device));
   (void)hipStreamSynchronize(ctx->stream);
+  unregister_stream(ctx->stream);
   (void)hipStreamDestroy(ctx->stream);
-  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->scratch) dev_free(ctx->scratch);
   delete ctx;
   dev_cache_flush();   // released large buffers go back to the device
   VP_API_END
@@ -127,7 +129,7 @@ int vp_malloc(vp_ctx ctx, int64_t bytes, void** dptr) {
   VP_REQUIRE(ctx && dptr && bytes >= 0, ERR_INVALID, "vp_malloc: bad argument");
   VP_HIP(hipSetDevice(ctx->device));
   *dptr = nullptr;
-  if (bytes) VP_HIP(hipMalloc(dptr, size_t(bytes)));
+  if (bytes) *dptr = dev_alloc(size_t(bytes));
   VP_API_END
 }
 
@@ -135,7 +137,7 @@ int vp_free(vp_ctx ctx, void* dptr) {
   VP_API_BEGIN
   VP_REQUIRE(ctx, ERR_INVALID, "ctx is NULL");
   VP_HIP(hipSetDevice(ctx->device));
-  if (dptr) VP_HIP(hipFree(dptr));
+  if (dptr) dev_free(dptr);
   VP_API_END
 }
 

@@ -1,20 +1,25 @@
 // C ABI: request server with continuous batching of mixed-precision solves.
 //
-// A worker thread owns one HIP stream.  Requests wait in a FIFO queue and are
-// admitted while the HBM budget (their solver state) and the slot count
-// allow; an oversize request is admitted alone.  Each round the admitted
-// requests are grouped by operator geometry and solve parameters, and every
-// group runs one refinement step (ir_step_set: FP64 residual, inner FP32
-// GMRES, update) as one batched launch sequence on gathered b/x vectors.
-// Requests admitted later join their group at the next round; finished
-// requests leave and free their budget at once.  Per-problem arithmetic in
-// the batched kernels never depends on the other problems of a launch, so a
-// request's result is the same alone and in any mix.
+// Requests wait in a FIFO queue and are admitted while the HBM budget (their
+// solver state) and the slot count allow; an oversize request is admitted
+// alone.  Admitted requests are grouped by operator geometry and solve
+// parameters (a group key).  A few executor threads, each with its own HIP
+// stream (VP_SERVER_STREAMS, default 3), take turns on the keys: an executor
+// takes the key that has waited longest and is not being run by another,
+// admits what the queue allows, and runs one refinement step (ir_step_set:
+// FP64 residual, inner FP32 GMRES, update) for every active request of that
+// key as one batched launch sequence.  Groups of different keys therefore
+// run at the same time on the device; requests admitted later join their
+// group at its next step, and finished requests leave and free their budget
+// at once.  Per-problem arithmetic in the batched kernels never depends on
+// the other problems of a launch, so a request's result is the same alone
+// and in any mix.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <deque>
 #include <map>
 #include <memory>
@@ -84,7 +89,7 @@ std::string group_key(const vp_op_s* o, const vp_prec_s* q, const vp_solve_param
 
 struct vp_server_s {
   int device = 0;
-  hipStream_t stream = nullptr;
+  std::vector<hipStream_t> streams;     // one per executor
   int64_t budget = 0;
   int max_active = 1;
   std::mutex mu;
@@ -92,15 +97,17 @@ struct vp_server_s {
   std::deque<ServeReq*> pending;
   std::vector<ServeReq*> active;
   std::map<int64_t, std::unique_ptr<ServeReq>> reqs;
+  std::map<std::string, int64_t> busy;  // key -> executor step number while it runs
+  std::map<std::string, int64_t> last;  // key -> step number of its last start
   int64_t used = 0, next_ticket = 1, rounds = 0;
   bool stop = false;
-  std::thread worker;
+  std::vector<std::thread> workers;
 };
 
 namespace {
 
-// One refinement step for a group of admitted requests of one geometry.
-void run_round(vp_server_s* S, const std::vector<ServeReq*>& g) {
+// One refinement step for a group of admitted requests of one key, on stream st.
+void run_round(const std::vector<ServeReq*>& g, hipStream_t st) {
   const int np = int(g.size());
   vp_op_s* o0 = g[0]->op;
   const int64_t n = o0->n;
@@ -111,94 +118,120 @@ void run_round(vp_server_s* S, const std::vector<ServeReq*>& g) {
     ops[p] = g[p]->op;
     precs[p] = g[p]->prec;
   }
+  // workspaces of their own: another executor may run the same operator
   ProblemSet<double> p64;
-  make_problem_set<double>(p64, ops.data(), nullptr, np);
-  p64.stream = S->stream;
+  make_problem_set<double>(p64, ops.data(), nullptr, np, true);
+  p64.stream = st;
   ProblemSet<float> p32;
-  make_problem_set<float>(p32, ops.data(), precs[0] ? precs.data() : nullptr, np);
-  p32.stream = S->stream;
+  make_problem_set<float>(p32, ops.data(), precs[0] ? precs.data() : nullptr, np, true);
+  p32.stream = st;
   DevBuf B(size_t(n) * np * sizeof(double2)), X(size_t(n) * np * sizeof(double2));
   for (int p = 0; p < np; ++p) {
-    VP_HIP(hipMemcpyAsync(B.as<double2>() + p * n, g[p]->b, n * sizeof(double2), hipMemcpyDeviceToDevice,
-                          S->stream));
-    VP_HIP(hipMemcpyAsync(X.as<double2>() + p * n, g[p]->x, n * sizeof(double2), hipMemcpyDeviceToDevice,
-                          S->stream));
+    VP_HIP(hipMemcpyAsync(B.as<double2>() + p * n, g[p]->b, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    if (g[p]->st.steps == 0 && g[p]->st.hist.empty())
+      VP_HIP(hipMemsetAsync(X.as<double2>() + p * n, 0, n * sizeof(double2), st));
+    else
+      VP_HIP(hipMemcpyAsync(X.as<double2>() + p * n, g[p]->x, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
   }
-  std::vector<IrProblem> st(np);
-  for (int p = 0; p < np; ++p) st[p] = g[p]->st;
+  std::vector<IrProblem> stt(np);
+  for (int p = 0; p < np; ++p) stt[p] = g[p]->st;
   ir_step_set(p64, p32, B.as<double2>(), X.as<double2>(), o0->scale32, prm.tol, prm.restart, prm.inner_tol,
-              prm.inner_maxit, prm.max_outer, prm.side, st);
+              prm.inner_maxit, prm.max_outer, prm.side, stt);
   for (int p = 0; p < np; ++p) {
-    VP_HIP(hipMemcpyAsync(g[p]->x, X.as<double2>() + p * n, n * sizeof(double2), hipMemcpyDeviceToDevice,
-                          S->stream));
-    g[p]->st = st[p];
+    VP_HIP(hipMemcpyAsync(g[p]->x, X.as<double2>() + p * n, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    g[p]->st = stt[p];
   }
-  VP_HIP(hipStreamSynchronize(S->stream));
+  VP_HIP(hipStreamSynchronize(st));
 }
 
-void serve_loop(vp_server_s* S) {
+// Admission (S->mu held): FIFO while the budget and the slots allow; a
+// request is admitted alone when nothing else is active.
+void admit(vp_server_s* S) {
+  while (!S->pending.empty() && int(S->active.size()) < S->max_active) {
+    ServeReq* r = S->pending.front();
+    if (!S->active.empty() && S->used + r->bytes > S->budget) break;
+    S->pending.pop_front();
+    S->used += r->bytes;
+    S->active.push_back(r);
+  }
+}
+
+// The runnable key that started longest ago (S->mu held); "" if none.
+std::string pick_key(vp_server_s* S) {
+  std::string best;
+  int64_t best_last = 0;
+  for (ServeReq* r : S->active) {
+    if (r->st.done || S->busy.count(r->key)) continue;
+    auto it = S->last.find(r->key);
+    const int64_t l = it == S->last.end() ? -1 : it->second;
+    if (best.empty() || l < best_last) {
+      best = r->key;
+      best_last = l;
+    }
+  }
+  return best;
+}
+
+void executor(vp_server_s* S, int k) {
   (void)hipSetDevice(S->device);
+  hipStream_t st = S->streams[size_t(k)];
   for (;;) {
-    std::vector<ServeReq*> run, fresh;
+    std::vector<ServeReq*> batch;
+    std::string key;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->active.empty(); });
-      if (S->pending.empty() && S->active.empty()) return;   // stop requested and drained
-      while (!S->pending.empty() && int(S->active.size()) < S->max_active) {
-        ServeReq* r = S->pending.front();
-        if (!S->active.empty() && S->used + r->bytes > S->budget) break;
-        S->pending.pop_front();
-        S->used += r->bytes;
-        S->active.push_back(r);
-        fresh.push_back(r);
+      for (;;) {
+        admit(S);
+        key = pick_key(S);
+        if (!key.empty()) break;
+        if (S->stop && S->pending.empty() && S->active.empty()) return;
+        S->cv_work.wait(lk);
       }
-      run = S->active;
+      for (ServeReq* r : S->active)
+        if (r->key == key && !r->st.done) batch.push_back(r);
       S->rounds += 1;
+      S->busy[key] = S->rounds;
+      S->last[key] = S->rounds;
     }
-    for (ServeReq* r : fresh) {
-      const hipError_t e = hipMemsetAsync(r->x, 0, size_t(r->op->n) * sizeof(double2), S->stream);
-      if (e != hipSuccess) {
-        r->error = VP_ERR_DEVICE;
-        r->errmsg = hipGetErrorString(e);
+    try {
+      run_round(batch, st);
+    } catch (const VpError& e) {
+      for (ServeReq* r : batch) {
+        r->error = e.code;
+        r->errmsg = e.msg;
         r->st.done = true;
       }
-    }
-    std::map<std::string, std::vector<ServeReq*>> groups;
-    for (ServeReq* r : run)
-      if (!r->st.done) groups[r->key].push_back(r);
-    for (auto& kv : groups) {
-      try {
-        run_round(S, kv.second);
-      } catch (const VpError& e) {
-        for (ServeReq* r : kv.second) {
-          r->error = e.code;
-          r->errmsg = e.msg;
-          r->st.done = true;
-        }
-      } catch (const std::exception& e) {
-        for (ServeReq* r : kv.second) {
-          r->error = VP_ERR_DEVICE;
-          r->errmsg = e.what();
-          r->st.done = true;
-        }
+    } catch (const std::exception& e) {
+      for (ServeReq* r : batch) {
+        r->error = VP_ERR_DEVICE;
+        r->errmsg = e.what();
+        r->st.done = true;
       }
     }
     {
       std::lock_guard<std::mutex> lk(S->mu);
+      S->busy.erase(key);
       const auto now = Clock::now();
       std::vector<ServeReq*> keep;
+      bool any_done = false;
       for (ServeReq* r : S->active) {
         if (r->st.done) {
           r->finished = true;
           r->t_done = now;
           S->used -= r->bytes;
+          any_done = true;
         } else {
           keep.push_back(r);
         }
       }
       S->active.swap(keep);
+      bool key_left = false;
+      for (ServeReq* r : S->active) key_left |= r->key == key;
+      if (!key_left) S->last.erase(key);
+      (void)any_done;
     }
     S->cv_done.notify_all();
+    S->cv_work.notify_all();   // freed budget, a released key or a finished run
   }
 }
 
@@ -236,12 +269,36 @@ int vp_server_create(vp_ctx ctx, int64_t hbm_budget_bytes, int max_active, vp_se
   S->device = ctx->device;
   S->budget = hbm_budget_bytes;
   S->max_active = max_active;
-  VP_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+  int ns = 3;
+  if (const char* e = std::getenv("VP_SERVER_STREAMS")) ns = std::max(1, std::min(8, std::atoi(e)));
+  for (int k = 0; k < ns; ++k) {
+    hipStream_t st = nullptr;
+    const hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      for (hipStream_t o : S->streams) {
+        unregister_stream(o);
+        (void)hipStreamDestroy(o);
+      }
+      VP_HIP(e);
+    }
+    register_stream(S->device, st);
+    S->streams.push_back(st);
+  }
   vp_server_s* raw = S.get();
   try {
-    S->worker = std::thread(serve_loop, raw);
+    for (int k = 0; k < ns; ++k) S->workers.emplace_back(executor, raw, k);
   } catch (...) {
-    (void)hipStreamDestroy(S->stream);
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->stop = true;
+    }
+    S->cv_work.notify_all();
+    for (auto& w : S->workers)
+      if (w.joinable()) w.join();
+    for (hipStream_t o : S->streams) {
+      unregister_stream(o);
+      (void)hipStreamDestroy(o);
+    }
     throw;
   }
   *out = S.release();
@@ -329,10 +386,14 @@ int vp_server_destroy(vp_server s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
-  if (s->worker.joinable()) s->worker.join();
+  for (auto& w : s->workers)
+    if (w.joinable()) w.join();
   (void)hipSetDevice(s->device);
-  (void)hipStreamSynchronize(s->stream);
-  (void)hipStreamDestroy(s->stream);
+  for (hipStream_t st : s->streams) {
+    (void)hipStreamSynchronize(st);
+    unregister_stream(st);
+    (void)hipStreamDestroy(st);
+  }
   delete s;
   VP_API_END
 }

@@ -182,7 +182,7 @@ static void prec_apply_dev(vp_prec_s* p, const C<T>* x, C<T>* y) {
 // Problem sets over operators (and preconditioners)
 // ----------------------------------------------------------------------------
 template <typename T>
-void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* precs, int np) {
+void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* precs, int np, bool private_work) {
   vp_op_s* o0 = ops[0];
   OpPrec<T>& P0 = prec_of<T>(o0);
   VP_REQUIRE(P0.present, ERR_INVALID, "operator lacks the precision this solve needs");
@@ -197,7 +197,8 @@ void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* 
   std::vector<ProbPtrs<T>> h(np);
   const size_t wbytes = size_t(3) * P0.g.x.P * o0->Mb * o0->Nb * sizeof(C<T>);
   const size_t w2bytes = P0.plan.variant == 1 ? size_t(3) * P0.g.x.P * P0.g.y.P * o0->Nb * sizeof(C<T>) : 0;
-  if (np > 1) {
+  const bool own = np > 1 || private_work;
+  if (own) {
     ps.works.alloc(wbytes * np);
     ps.works2.alloc(w2bytes * np);
   }
@@ -215,11 +216,11 @@ void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* 
                    P.g.y.P == P0.g.y.P && P.g.z.P == P0.g.z.P && P.g.y.R == P0.g.y.R && P.g.z.R == P0.g.z.R,
                ERR_INVALID, "batched operators differ in plan");
     ProbPtrs<T> pp{};
-    pp.work = np > 1 ? reinterpret_cast<C<T>*>(static_cast<char*>(ps.works.p) + wbytes * p)
-                     : P.work.template as<C<T>>();
+    pp.work = own ? reinterpret_cast<C<T>*>(static_cast<char*>(ps.works.p) + wbytes * p)
+                  : P.work.template as<C<T>>();
     if (w2bytes)
-      pp.work2 = np > 1 ? reinterpret_cast<C<T>*>(static_cast<char*>(ps.works2.p) + w2bytes * p)
-                        : P.work2.template as<C<T>>();
+      pp.work2 = own ? reinterpret_cast<C<T>*>(static_cast<char*>(ps.works2.p) + w2bytes * p)
+                     : P.work2.template as<C<T>>();
     pp.fN = P.fN.template as<C<T>>();
     pp.pos = op->pos.as<int>();
     pp.mc = P.mc.template as<C<T>>();
@@ -297,8 +298,8 @@ void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* 
   }
 }
 
-template void make_problem_set<double>(ProblemSet<double>&, vp_op_s* const*, vp_prec_s* const*, int);
-template void make_problem_set<float>(ProblemSet<float>&, vp_op_s* const*, vp_prec_s* const*, int);
+template void make_problem_set<double>(ProblemSet<double>&, vp_op_s* const*, vp_prec_s* const*, int, bool);
+template void make_problem_set<float>(ProblemSet<float>&, vp_op_s* const*, vp_prec_s* const*, int, bool);
 
 static void fill_info(vp_gmres_info* info, const GmresResult& r, int p) {
   if (!info) return;

@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include "krylov.hpp"
@@ -236,6 +237,42 @@ __global__ void k_publish(const double* __restrict__ est, const int* __restrict_
 // Spins until the device has published step ``seq``.  Past 200 ms of
 // spinning it waits on the stream instead and, should the published values
 // still not be visible, copies them from the device.
+// Coherent pinned status buffers of the per-step publication, kept in a
+// process-wide pool: a solve takes one at its start and returns it at its
+// end, so repeated solves allocate no pinned memory.  The buffers live until
+// the process ends.
+namespace {
+std::mutex g_status_mu;
+std::vector<std::pair<void*, size_t>> g_status_free;
+
+struct StatusLease {
+  void* p = nullptr;
+  size_t bytes = 0;
+  explicit StatusLease(size_t b) {
+    {
+      std::lock_guard<std::mutex> lk(g_status_mu);
+      for (size_t i = 0; i < g_status_free.size(); ++i)
+        if (g_status_free[i].second >= b) {
+          p = g_status_free[i].first;
+          bytes = g_status_free[i].second;
+          g_status_free.erase(g_status_free.begin() + long(i));
+          return;
+        }
+    }
+    const size_t sz = std::max<size_t>(b, 4096);
+    VP_HIP(hipHostMalloc(&p, sz, hipHostMallocMapped | hipHostMallocCoherent));
+    bytes = sz;
+  }
+  ~StatusLease() {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_status_mu);
+    g_status_free.emplace_back(p, bytes);
+  }
+  StatusLease(const StatusLease&) = delete;
+  StatusLease& operator=(const StatusLease&) = delete;
+};
+}  // namespace
+
 static void wait_published(const int* hseq, int seq, hipStream_t s, double* hest, const double* est, int* hconv,
                            const int* conv, int np) {
   const volatile int* vs = hseq;
@@ -706,7 +743,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   // per-step estimates and convergence flags: the device publishes them into
   // coherent pinned host memory, followed by the step's sequence number, and
   // the host spins on that number (no copy kernels, no blocking wait)
-  PinnedBuf pin(np * (sizeof(double) + sizeof(int)) + sizeof(int), hipHostMallocMapped | hipHostMallocCoherent);
+  StatusLease pin(np * (sizeof(double) + sizeof(int)) + sizeof(int));
   double* hest = static_cast<double*>(pin.p);
   int* hconv = reinterpret_cast<int*>(hest + np);
   int* hseq = hconv + np;

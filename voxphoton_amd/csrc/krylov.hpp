@@ -53,9 +53,12 @@ struct ProblemSet {
 template <typename T>
 void ps_apply_A(ProblemSet<T>& ps, const C<T>* x, int64_t xs, C<T>* y, int64_t ys, const int* active);
 // Problem set over operators (and preconditioners, may be NULL) of one
-// geometry; throws ERR_INVALID when they differ.
+// geometry; throws ERR_INVALID when they differ.  A single problem uses its
+// operator's own spectrum workspace unless private_work asks for new ones
+// (callers that may run one operator on two streams at once).
 template <typename T>
-void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* precs, int np);
+void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* precs, int np,
+                      bool private_work = false);
 template <typename T>
 void ps_apply_M(ProblemSet<T>& ps, const C<T>* x, int64_t xs, C<T>* y, int64_t ys, const int* active);
 

@@ -19,14 +19,19 @@ struct vp_ctx_s {
 namespace vp {
 
 // Device memory of the library.  Requests of 64 MiB and more go through a
-// per-device cache: a released buffer is kept, after a device
-// synchronisation as hipFree would do, and handed to a later request of
-// between 4/5 of its size and its size (48 GiB at most are kept; the cache is
-// emptied and the allocation retried when hipMalloc runs out of memory).
-// Repeated solves of one size then allocate nothing.
+// per-device cache: a released buffer is kept and handed to a later request
+// of between 4/5 of its size and its size once the work enqueued before its
+// release on every stream of the library has finished (events, no device
+// synchronisation); 48 GiB at most are kept, and the cache is emptied and the
+// allocation retried when hipMalloc runs out of memory.  Repeated solves of
+// one size then allocate nothing.
 void* dev_alloc(size_t bytes);
 void dev_free(void* p);
 void dev_cache_flush();
+// The library's streams: a released cached buffer is reused only after the
+// work enqueued on each of them before the release has finished.
+void register_stream(int device, hipStream_t s);
+void unregister_stream(hipStream_t s);
 
 // RAII device buffer.
 struct DevBuf {

@@ -103,22 +103,43 @@ bool pcrow_supported(int n) {
 namespace {
 constexpr size_t kCacheMin = size_t(64) << 20;
 constexpr size_t kCacheMax = size_t(48) << 30;
+// A released buffer is handed out again once the events recorded at its
+// release on every stream of the library (contexts, servers) on its device
+// have completed: all work enqueued before the release, which is all work
+// that can use it, has then finished.  No device-wide synchronisation.
 struct CachedBuf {
   int dev;
   void* p;
   size_t bytes;
+  std::vector<hipEvent_t> ev;
 };
 std::mutex g_cache_mu;
 std::vector<CachedBuf> g_cache;               // released, oldest first
 std::map<void*, std::pair<int, size_t>> g_big;  // live and cached large allocations: device, size
 size_t g_cache_bytes = 0;
+std::vector<std::pair<int, hipStream_t>> g_streams;   // the library's streams (under g_cache_mu)
+
+bool events_done(const CachedBuf& c) {
+  for (hipEvent_t e : c.ev)
+    if (hipEventQuery(e) != hipSuccess) return false;
+  return true;
+}
+
+void drop(CachedBuf& c) {   // g_cache_mu held
+  for (hipEvent_t e : c.ev) {
+    (void)hipEventSynchronize(e);
+    (void)hipEventDestroy(e);
+  }
+  c.ev.clear();
+  (void)hipFree(c.p);
+  g_big.erase(c.p);
+  g_cache_bytes -= c.bytes;
+}
 
 void flush_device(int dev) {   // g_cache_mu held
   for (size_t i = 0; i < g_cache.size();) {
     if (dev < 0 || g_cache[i].dev == dev) {
-      (void)hipFree(g_cache[i].p);
-      g_big.erase(g_cache[i].p);
-      g_cache_bytes -= g_cache[i].bytes;
+      drop(g_cache[i]);
       g_cache.erase(g_cache.begin() + long(i));
     } else {
       ++i;
@@ -126,6 +147,20 @@ void flush_device(int dev) {   // g_cache_mu held
   }
 }
 }  // namespace
+
+void register_stream(int device, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  g_streams.emplace_back(device, s);
+}
+
+void unregister_stream(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  for (size_t i = 0; i < g_streams.size(); ++i)
+    if (g_streams[i].second == s) {
+      g_streams.erase(g_streams.begin() + long(i));
+      break;
+    }
+}
 
 void* dev_alloc(size_t b) {
   int dev = 0;
@@ -136,14 +171,15 @@ void* dev_alloc(size_t b) {
     for (int i = 0; i < int(g_cache.size()); ++i) {
       const CachedBuf& c = g_cache[size_t(i)];
       if (c.dev == dev && c.bytes >= b && c.bytes - c.bytes / 5 <= b &&
-          (best < 0 || c.bytes < g_cache[size_t(best)].bytes))
+          (best < 0 || c.bytes < g_cache[size_t(best)].bytes) && events_done(c))
         best = i;
     }
     if (best >= 0) {
-      void* p = g_cache[size_t(best)].p;
-      g_cache_bytes -= g_cache[size_t(best)].bytes;
+      CachedBuf c = g_cache[size_t(best)];
+      for (hipEvent_t e : c.ev) (void)hipEventDestroy(e);
+      g_cache_bytes -= c.bytes;
       g_cache.erase(g_cache.begin() + best);
-      return p;
+      return c.p;
     }
   }
   void* p = nullptr;
@@ -166,28 +202,37 @@ void* dev_alloc(size_t b) {
 
 void dev_free(void* p) {
   if (!p) return;
-  std::pair<int, size_t> info;
-  {
-    std::lock_guard<std::mutex> lk(g_cache_mu);
-    auto it = g_big.find(p);
-    if (it == g_big.end()) {
-      (void)hipFree(p);
-      return;
-    }
-    info = it->second;
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto it = g_big.find(p);
+  if (it == g_big.end()) {
+    (void)hipFree(p);
+    return;
   }
+  CachedBuf c{it->second.first, p, it->second.second, {}};
   int cur = 0;
   (void)hipGetDevice(&cur);
-  if (cur != info.first) (void)hipSetDevice(info.first);
-  (void)hipDeviceSynchronize();   // work that uses the buffer ends before it can be handed out again
-  if (cur != info.first) (void)hipSetDevice(cur);
-  std::lock_guard<std::mutex> lk(g_cache_mu);
-  g_cache.push_back({info.first, p, info.second});
-  g_cache_bytes += info.second;
+  if (cur != c.dev) (void)hipSetDevice(c.dev);
+  bool ok = true;
+  for (const auto& ds : g_streams) {
+    if (ds.first != c.dev) continue;
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess || hipEventRecord(ev, ds.second) != hipSuccess) {
+      if (ev) (void)hipEventDestroy(ev);
+      ok = false;
+      break;
+    }
+    c.ev.push_back(ev);
+  }
+  if (!ok) {   // no events: fall back to waiting for the device
+    for (hipEvent_t ev : c.ev) (void)hipEventDestroy(ev);
+    c.ev.clear();
+    (void)hipDeviceSynchronize();
+  }
+  if (cur != c.dev) (void)hipSetDevice(cur);
+  g_cache.push_back(c);
+  g_cache_bytes += c.bytes;
   while (g_cache_bytes > kCacheMax && !g_cache.empty()) {
-    (void)hipFree(g_cache.front().p);
-    g_big.erase(g_cache.front().p);
-    g_cache_bytes -= g_cache.front().bytes;
+    drop(g_cache.front());
     g_cache.erase(g_cache.begin());
   }
 }

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 #include "precond.hpp"
 
@@ -315,6 +316,152 @@ __global__ void __launch_bounds__(256) k_pc_quadmul(PcArgs<T> a, PcGeom g) {
   }
 }
 
+// FP32 quadrant block product with binary16 blocks on the packed dot
+// instruction v_dot2_f32_f16 (two fp16 products summed into an f32
+// accumulator): a stored 32-bit block word (re, im) enters
+//   Re(b x) = dot2((b.re, b.im), (x.re, -x.im)),  Im(b x) = dot2((b.re, b.im), (x.im, x.re)),
+// so an entry costs two instructions per column instead of a conversion and
+// four FMAs.  The four mirrored columns are staged once per block as fp16
+// pairs scaled per column by a power of two that puts max |x| at 2^14
+// (exact scaling; fp16 keeps 11 significant bits, as the blocks do); the
+// accumulators are rescaled by hscale / s_c.  Same layout, mirrors and sign
+// flips as k_pc_quadmul.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+__global__ void __launch_bounds__(256) k_pc_quadmul_dot2(PcArgs<float> a, PcGeom g) {
+  if (a.active && !a.active[blockIdx.y]) return;
+  float2* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
+  const int L = g.L, M = g.M, N = g.N, n3 = 3 * N, Hy = g.Hy;
+  // XX[q][2c], XX[q][2c+1]: the two fp16 pairs of column c at entry q
+  __shared__ uint32_t XX[3 * 85][8];
+  __shared__ float cmax[4][4];
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int Hx = g.Hx;
+  const int kxc = g.pct ? blk % Hx : blk / Hy;
+  const int kyc = g.pct ? blk / Hx : blk - kxc * Hy;
+  int kxs[4], kys[4];
+  bool mxs[4], mys[4], use[4];
+  const bool twox = kxc != 0 && 2 * kxc != L, twoy = kyc != 0 && 2 * kyc != M;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    mxs[c] = (c & 1) != 0;
+    mys[c] = (c & 2) != 0;
+    use[c] = (!mxs[c] || twox) && (!mys[c] || twoy);
+    kxs[c] = mxs[c] ? L - kxc : kxc;
+    kys[c] = mys[c] ? M - kyc : kyc;
+  }
+  // gather the columns (at most 4 entries per thread: 4 * n3 <= 4 * 255 and
+  // blockDim.x >= n3 rounded up to a wavefront) and their maxima
+  float2 vals[4];
+  float m4[4] = {0.f, 0.f, 0.f, 0.f};
+  const int nel = 4 * n3;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = threadIdx.x + u * blockDim.x;
+    float2 v = make_float2(0.f, 0.f);
+    if (e < nel) {
+      const int c = e / n3, q = e - c * n3;
+      if (use[c]) {
+        v = work[g.pct ? (int64_t(q) * M + kys[c]) * L + kxs[c] : (int64_t(kxs[c]) * n3 + q) * M + kys[c]];
+        const int cq = q / N;
+        if ((cq == 0 && mxs[c]) || (cq == 1 && mys[c])) v = make_float2(-v.x, -v.y);
+      }
+      const float am = fmaxf(fabsf(v.x), fabsf(v.y));
+#pragma unroll
+      for (int c2 = 0; c2 < 4; ++c2)
+        if (c2 == c) m4[c2] = fmaxf(m4[c2], am);
+    }
+    vals[u] = v;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    for (int off = 32; off > 0; off >>= 1) m4[c] = fmaxf(m4[c], __shfl_xor(m4[c], off));
+    if (lane == 0) cmax[wave][c] = m4[c];
+  }
+  __syncthreads();
+  float sc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float m = 0.f;
+    for (int w = 0; w < int(blockDim.x >> 6); ++w) m = fmaxf(m, cmax[w][c]);
+    int ex = 0;
+    (void)frexpf(m, &ex);                     // m < 2^ex
+    sc[c] = m > 0.f ? ldexpf(1.f, 14 - ex) : 1.f;
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = threadIdx.x + u * blockDim.x;
+    if (e < nel) {
+      const int c = e / n3, q = e - c * n3;
+      const float xr = vals[u].x * sc[c], xi = vals[u].y * sc[c];
+      half2_t pa, pb;
+      pa.x = _Float16(xr);
+      pa.y = _Float16(-xi);
+      pb.x = _Float16(xi);
+      pb.y = _Float16(xr);
+      XX[q][2 * c] = __builtin_bit_cast(uint32_t, pa);
+      XX[q][2 * c + 1] = __builtin_bit_cast(uint32_t, pb);
+    }
+  }
+  __syncthreads();
+  const int64_t boff = (int64_t(kxc) * Hy + kyc) * n3 * n3;
+  const uint32_t* Hb = a.hblocks[blockIdx.y] + boff;
+  const float hs = a.hscale[blockIdx.y];
+  for (int r = threadIdx.x; r < n3; r += blockDim.x) {
+    float ar[4] = {0.f, 0.f, 0.f, 0.f}, ai[4] = {0.f, 0.f, 0.f, 0.f};
+    auto step = [&](uint32_t hw, int q) {
+      const half2_t b = __builtin_bit_cast(half2_t, hw);
+      const uint4 x01 = *reinterpret_cast<const uint4*>(&XX[q][0]);
+      const uint4 x23 = *reinterpret_cast<const uint4*>(&XX[q][4]);
+      ar[0] = __builtin_amdgcn_fdot2(b, __builtin_bit_cast(half2_t, x01.x), ar[0], false);
+      ai[0] = __builtin_amdgcn_fdot2(b, __builtin_bit_cast(half2_t, x01.y), ai[0], false);
+      ar[1] = __builtin_amdgcn_fdot2(b, __builtin_bit_cast(half2_t, x01.z), ar[1], false);
+      ai[1] = __builtin_amdgcn_fdot2(b, __builtin_bit_cast(half2_t, x01.w), ai[1], false);
+      ar[2] = __builtin_amdgcn_fdot2(b, __builtin_bit_cast(half2_t, x23.x), ar[2], false);
+      ai[2] = __builtin_amdgcn_fdot2(b, __builtin_bit_cast(half2_t, x23.y), ai[2], false);
+      ar[3] = __builtin_amdgcn_fdot2(b, __builtin_bit_cast(half2_t, x23.z), ar[3], false);
+      ai[3] = __builtin_amdgcn_fdot2(b, __builtin_bit_cast(half2_t, x23.w), ai[3], false);
+    };
+    constexpr int QU = 8;
+    int q0 = 0;
+    for (; q0 + QU <= n3; q0 += QU) {
+      uint32_t hw[QU];
+#pragma unroll
+      for (int u = 0; u < QU; ++u) hw[u] = Hb[int64_t(q0 + u) * n3 + r];
+#pragma unroll
+      for (int u = 0; u < QU; ++u) step(hw[u], q0 + u);
+    }
+    for (; q0 < n3; ++q0) step(Hb[int64_t(q0) * n3 + r], q0);
+    const int cr = r / N;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (!use[c]) continue;
+      const float f = hs / sc[c];
+      float2 v = make_float2(ar[c] * f, ai[c] * f);
+      if ((cr == 0 && mxs[c]) || (cr == 1 && mys[c])) v = make_float2(-v.x, -v.y);
+      work[g.pct ? (int64_t(r) * M + kys[c]) * L + kxs[c] : (int64_t(kxs[c]) * n3 + r) * M + kys[c]] = v;
+    }
+  }
+}
+
+// Launches the quadrant block product: the dot2 kernel for FP32 applies
+// with binary16 blocks (VP_PC_DOT2=0 keeps the FMA kernel), else k_pc_quadmul.
+template <typename T>
+static void launch_quadmul(const PcGeom& g, const PcArgs<T>& a, dim3 grid, int threads, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    static const bool dot2 = [] {
+      const char* e = std::getenv("VP_PC_DOT2");
+      return !(e && e[0] == '0');
+    }();
+    if (dot2 && a.hblocks && 3 * g.N <= 255) {
+      hipLaunchKernelGGL(k_pc_quadmul_dot2, grid, dim3(threads), 0, s, a, g);
+      return;
+    }
+  }
+  hipLaunchKernelGGL(k_pc_quadmul<T>, grid, dim3(threads), 0, s, a, g);
+}
+
 // x pass inverse + gather into the output vector.
 template <typename T>
 __global__ void __launch_bounds__(512) k_pc_xinv(PcArgs<T> a, PcGeom g) {
@@ -585,7 +732,7 @@ static void launch_prec_pct(const PcGeom& g, const PcArgs<T>& a, int nprob, hipS
   }
   {
     ProfScope ps(s, KID_PC_BLOCK);
-    hipLaunchKernelGGL(k_pc_quadmul<T>, gq, dim3(tq), 0, s, a, g);
+    launch_quadmul<T>(g, a, gq, tq, s);
   }
   {
     ProfScope ps(s, KID_PC_YROWS);
@@ -641,7 +788,7 @@ void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) 
       }
       {
         ProfScope ps(s, KID_PC_BLOCK);
-        hipLaunchKernelGGL(k_pc_quadmul<T>, gq, dim3(tq), 0, s, a, g);
+        launch_quadmul<T>(g, a, gq, tq, s);
       }
       ProfScope ps(s, KID_PC_YROWS);
       hipLaunchKernelGGL((k_pc_yrows<T, true>), gr, dim3(256), ldr, s, a, g);

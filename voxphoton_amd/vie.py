@@ -345,6 +345,23 @@ def pgmres(A, b, restart=50, tol=1e-3, maxit=10, L1=None, L2=None, R1=None, R2=N
     return x, info.flag, info.relres, [info.iter_inner, info.iter_outer], buf[:info.resvec_len].copy()
 
 
+def bicgstab(A, b, tol=1e-6, maxit=20, M=None, x0=None, prec='fp64'):
+    """[x, flag, relres, iter, resvec] = bicgstab(A, b, tol, maxit, M, [], x0).
+
+    MATLAB's call as the harness makes it (JVIE_Solver_const.m:469-494);
+    ``M`` a Preconditioner applied as M^-1 (or None).  ``iter`` counts whole
+    iterations, plus 0.5 when it stopped at a half iteration; ``resvec`` the
+    relative residual at the start and after every half iteration."""
+    b = as_c128(np.asarray(b).reshape(-1))
+    x = np.empty_like(b)
+    x0a = None if x0 is None or not np.any(x0) else as_c128(np.asarray(x0).reshape(-1))
+    info, buf = _info(2 * int(maxit) + 2)
+    check(L.lib.vp_bicgstab(A.handle, M.handle if M is not None else None, ptr(b), ptr(x0a), ptr(x), L.VP_MEM_HOST,
+                            L.VP_FP64 if prec == 'fp64' else L.VP_FP32, float(tol), int(maxit), ctypes.byref(info)))
+    it = info.iter_inner + 0.5 * info.iter_outer
+    return x, info.flag, info.relres, it, buf[:info.resvec_len].copy()
+
+
 def solve_ir(A, b, tol=1e-6, restart=50, inner_tol=1e-3, inner_maxit=4, max_outer=20, M=None, side='left'):
     """Mixed precision: FP64 refinement around FP32 GMRES(restart) on the GPU.
 
@@ -536,3 +553,82 @@ def power_through(E, x_thru, m_edge):
     M = E.shape[1]
     s = E[x_thru - 1, m_edge:M - m_edge, :, :]
     return float(np.sum((s * np.conj(s)).real))
+
+
+def JVIE_Solver_const(Einc, e_r, r, freq, fN=None, tol=1e-3, form=2, solver='B', precond='', max_it=1000,
+                      inner_it=50, outer_it=200, ritz=5, ctx=None):
+    """[Je, flag, relres, iter, resvec, solvert, vsol] = JVIE_Solver_const(Einc, e_r, r, freq, fN, ...).
+
+    The reference's one-call harness (pc/src_vie/src_solver/JVIE_Solver_const.m:1-521)
+    over the device solve path:
+
+    * Mr = e_r, Mc = e_r - 1, unknowns where |Mc| > 1e-12 (:133-149);
+    * Vrhs = Gram tau3(idxS3) .* Einc(idxS3), tau = j w eps0 Mc, Gram = dx^3
+      with dx the z spacing of ``r`` (:137-157);
+    * form 2 (JVIE-II, default): Mc = Mc ./ Mr, Vrhs ./= Mr, Mr = 1 (:243-251);
+      form 1 keeps Mr (JVIE-I);
+    * solver 'G' pgmres(inner_it, tol, outer_it), 'D' pgmresDR(inner_it, tol,
+      outer_it, ritz), 'B' bicgstab(tol, max_it) (:312-508);
+    * Je = zeros(L,M,N,3), Je(idxS3) = vsol (:520-521).
+
+    ``fN`` is the reference's circulant tensor; None assembles the Green
+    tensor on the device for the grid spacing of ``r`` and the wavenumber
+    of ``freq``.  The reference's "preconditioner" ('L'/'R') is the operator
+    mv_AN_const with Mr' = 1/Mr and Mc' = 1/Mr - 1 (:301-303), which under
+    JVIE-II is Gram I: a scalar, which leaves every GMRES and BiCGSTAB
+    iterate unchanged, so it is accepted and not applied; under JVIE-I it is
+    not available here (ValueError).  'F' (the FFTW BiCGSTAB) and 'M'
+    (automatic) run 'B'.
+    """
+    import time
+    from .krylov_dr import pgmresDR
+    ctx = ctx or default_context()
+    Einc = np.asarray(Einc, dtype=complex)
+    e_r = np.asarray(e_r, dtype=complex)
+    r = np.asarray(r, dtype=float)
+    Lg, Mg, Ng = e_r.shape
+    omega = 2 * np.pi * freq
+    c0 = 299792458.0
+    mu0 = 4e-7 * np.pi
+    eps0 = 1.0 / (c0 ** 2 * mu0)
+    ko = omega / c0
+    dx = r[0, 0, 1, 2] - r[0, 0, 0, 2] if Ng > 1 else (r[1, 0, 0, 0] - r[0, 0, 0, 0] if Lg > 1 else
+                                                       r[0, 1, 0, 1] - r[0, 0, 0, 1])
+    Gram = dx ** 3
+    Mr = e_r.copy()
+    Mc = e_r - 1.0
+    nD = Lg * Mg * Ng
+    idxS = np.flatnonzero(np.abs(Mc.reshape(-1, order='F')) > 1e-12) + 1
+    idxS3 = np.concatenate([idxS, nD + idxS, 2 * nD + idxS]).astype(np.int64)
+    tau3 = np.tile((1j * omega * eps0 * Mc).reshape(-1, order='F'), 3)
+    i0 = idxS3 - 1
+    Vrhs = Gram * tau3[i0] * Einc.reshape(-1, order='F')[i0]
+    if form == 2:
+        mr = Mr.reshape(-1, order='F')[idxS - 1]
+        Mc = Mc / Mr
+        Vrhs = Vrhs / np.concatenate([mr, mr, mr])
+        Mr_op = None
+    elif form == 1:
+        if precond in ('L', 'R'):
+            raise ValueError('the JVIE-I preconditioner operator is not available')
+        Mr_op = Mr
+    else:
+        raise ValueError('form must be 1 or 2')
+    if fN is not None:
+        op = NOperator.from_fN(fN, Mr_op, Mc, Gram, idxS3, precisions=('fp64',), ctx=ctx)
+    else:
+        # the harness's Green tensor is in the exterior medium of the grid (ko = w / c0)
+        op = NOperator.assembled((Lg, Mg, Ng), dx, ko, Mr_op, Mc, idxS3, precisions=('fp64',), ctx=ctx)
+    t0 = time.perf_counter()
+    if solver == 'G':
+        vsol, flag, relres, it, resvec = pgmres(op, Vrhs, inner_it, tol, outer_it)
+    elif solver == 'D':
+        vsol, flag, relres, it, resvec = pgmresDR(op, Vrhs, inner_it, tol, outer_it, ritz)
+    elif solver in ('B', 'F', 'M'):
+        vsol, flag, relres, it, resvec = bicgstab(op, Vrhs, tol, max_it)
+    else:
+        raise ValueError("solver must be 'G', 'D', 'B', 'F' or 'M'")
+    solvert = time.perf_counter() - t0
+    Je = np.zeros(3 * nD, dtype=complex)
+    Je[i0] = vsol
+    return Je.reshape(Lg, Mg, Ng, 3, order='F'), flag, relres, it, resvec, solvert, vsol
