@@ -257,3 +257,91 @@ def pgmresDR(A, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, L2=None, 
     relres = np.linalg.norm(b - _apply(A, x)) / np.linalg.norm(b)    # :276
     flag = 0 if relres <= tol else 1
     return x, flag, relres, [p, outit], np.array(resvec)
+
+
+# -- BiCGSTAB ------------------------------------------------------------------
+
+def bicgstab(A, b, tol=1e-6, maxit=20, M=None, x0=None):
+    """Preconditioned BiCGSTAB, the harness's 'B' solver
+    (JVIE_Solver_const.m:469-494 calls MATLAB's bicgstab(A,b,tol,maxit,M),
+    a closed builtin).  Restates the published iteration (van der Vorst,
+    SIAM J. Sci. Stat. Comput. 13 (1992) 631-644) with M applied to the
+    search directions, the stopping rule ||b - A x|| <= tol ||b|| confirmed
+    on the true residual (the recurrence restarts from it when the
+    confirmation fails), and the half-iteration residual history.
+
+    Returns ``(x, flag, relres, iter, resvec)``: flag 0 converged, 1 not,
+    4 breakdown; iter whole iterations (+0.5 at a half iteration); resvec
+    the relative residual at the start and after every half iteration.
+    """
+    b = np.asarray(b, dtype=complex)
+    Mf = (lambda v: v) if M is None else (lambda v: _apply(M, v))
+    x = np.zeros_like(b) if x0 is None else np.asarray(x0, dtype=complex).copy()
+    bnorm = np.linalg.norm(b)
+    if bnorm == 0:
+        return np.zeros_like(b), 0, 0.0, 0.0, np.array([0.0])
+    r = b - _apply(A, x)
+    rt = r.copy()
+    resvec = [np.linalg.norm(r) / bnorm]
+    flag = 0 if resvec[0] <= tol else 1
+    half = 0
+    rho = np.vdot(rt, r)
+    rho_old = alpha = omega = 1.0
+    p = v = None
+    first = True
+    it = 0
+    while flag != 0 and it < maxit:
+        it += 1
+        if not first:
+            rho = np.vdot(rt, r)
+        if rho == 0:
+            flag = 4
+            break
+        p = r.copy() if first else r + (rho / rho_old) * (alpha / omega) * (p - omega * v)
+        first = False
+        ph = Mf(p)
+        v = _apply(A, ph)
+        rtv = np.vdot(rt, v)
+        if rtv == 0:
+            flag = 4
+            break
+        alpha = rho / rtv
+        x = x + alpha * ph
+        s = r - alpha * v
+        half += 1
+        resvec.append(np.linalg.norm(s) / bnorm)
+        if resvec[-1] <= tol:
+            r = b - _apply(A, x)
+            if np.linalg.norm(r) <= tol * bnorm:
+                flag = 0
+                break
+            rho = np.vdot(rt, r)
+            first = True
+            continue
+        sh = Mf(s)
+        t = _apply(A, sh)
+        tt = np.vdot(t, t)
+        if tt == 0:
+            flag = 4
+            break
+        omega = np.vdot(t, s) / tt
+        if omega == 0:
+            flag = 4
+            break
+        x = x + omega * sh
+        r = s - omega * t
+        half += 1
+        resvec.append(np.linalg.norm(r) / bnorm)
+        rho_old = rho
+        rho = np.vdot(rt, r)
+        if resvec[-1] <= tol:
+            r = b - _apply(A, x)
+            if np.linalg.norm(r) <= tol * bnorm:
+                flag = 0
+                break
+            rho = np.vdot(rt, r)
+            first = True
+    relres = np.linalg.norm(b - _apply(A, x)) / bnorm
+    if flag == 0 and relres > tol:
+        flag = 1
+    return x, flag, relres, half // 2 + 0.5 * (half % 2), np.array(resvec)

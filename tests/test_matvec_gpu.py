@@ -25,6 +25,45 @@ def test_matvec_fp64_from_fN(vp_ctx, shape):
     assert rel(y32, ref) < 1e-5
 
 
+@pytest.mark.parametrize('shape,box', [((20, 12, 10), (5, 4, 3)), ((24, 16, 8), (11, 3, 8)), ((9, 30, 6), (9, 7, 2))])
+def test_caller_fN_resampled_to_the_box(vp_ctx, monkeypatch, shape, box):
+    # a caller's fN at the reference's sizes (2L, 2M, 2N) on a grid whose
+    # unknowns fill a smaller box: the tensor is resampled to circulants of
+    # the box (the shortest smooth lengths here) and gives the same operator
+    from voxphoton_amd.vie import NOperator
+    L, M, N = shape
+    rng = np.random.default_rng(L * M + N)
+    G, Mr, Mc, Gram, _, _ = random_problem(L, M, N, seed=L + M + N, with_mr=True)
+    mask = np.zeros(shape, bool)
+    o = [rng.integers(0, s - b + 1) for s, b in zip(shape, box)]
+    sub = tuple(slice(a, a + b) for a, b in zip(o, box))
+    mask[sub] = rng.random(box) < 0.7
+    mask[tuple(a for a in o)] = True
+    mask[tuple(a + b - 1 for a, b in zip(o, box))] = True
+    idx = ovie.idx_from_mask(mask)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    fN = ovie.fN_from_toeplitz(G)
+    monkeypatch.setenv('VP_FFT_SIZES', 'min')
+    op = NOperator.from_fN(fN, Mr, Mc, Gram, idx, ctx=vp_ctx)
+    info = op.info()
+    assert info['box'] == box
+    for f in (info['fft_fp64'], info['fft_fp32']):
+        assert all(2 * b - 1 <= p <= 2 * s for p, b, s in zip(f, box, shape)), (f, box)
+        assert any(p < 2 * s for p, s in zip(f, shape)), f
+    monkeypatch.setenv('VP_FN_RESAMPLE', '0')
+    op_kept = NOperator.from_fN(fN, Mr, Mc, Gram, idx, ctx=vp_ctx)
+    assert op_kept.info()['fft_fp64'] == (2 * L, 2 * M, 2 * N)
+    for flag in ('notransp', 'transp'):
+        ref = ovie.mv_AN_const(x, fN, Mr, Mc, Gram, flag, idx)
+        y = op.apply(x, flag, 'fp64')
+        print(shape, box, info['fft_fp64'], flag, 'fp64 rel', rel(y, ref), flush=True)
+        assert rel(y, ref) < 1e-12
+        assert rel(op.apply(x, flag, 'fp32'), ref) < 1e-5
+        assert rel(op_kept.apply(x, flag, 'fp64'), ref) < 1e-12
+    op.close()
+    op_kept.close()
+
+
 @pytest.mark.parametrize('shape', [(6, 5, 4), (10, 8, 6)])
 def test_matvec_transp_and_mr(vp_ctx, shape):
     from voxphoton_amd.vie import NOperator

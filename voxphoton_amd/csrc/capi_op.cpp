@@ -485,10 +485,12 @@ static std::vector<int> axis_candidates(int B, int axis, bool fp64, int fixed) {
 }
 
 struct SizeKey {
-  int device, Lb, Mb, Nb, fp64, fsym, Lp, Mp, Np;   // Lp, Mp, Np: the caller's fixed sizes (0 free)
+  int device, Lb, Mb, Nb, fp64, fsym;
+  int Lp, Mp, Np;   // the caller's fixed sizes (0 free)
+  int La, Ma, Na;   // the sizes of a caller's tensor, candidates besides the smooth lengths (0 none)
   bool operator<(const SizeKey& o) const {
-    return std::tie(device, Lb, Mb, Nb, fp64, fsym, Lp, Mp, Np) <
-           std::tie(o.device, o.Lb, o.Mb, o.Nb, o.fp64, o.fsym, o.Lp, o.Mp, o.Np);
+    return std::tie(device, Lb, Mb, Nb, fp64, fsym, Lp, Mp, Np, La, Ma, Na) <
+           std::tie(o.device, o.Lb, o.Mb, o.Nb, o.fp64, o.fsym, o.Lp, o.Mp, o.Np, o.La, o.Ma, o.Na);
   }
 };
 static std::mutex g_sizes_mu;
@@ -502,16 +504,30 @@ static int fft_sizes_mode() {   // 0 auto (timed), 1 reference 2D, 2 shortest ca
   throw VpError{ERR_INVALID, "VP_FFT_SIZES must be auto, reference or min"};
 }
 
+// Circulant sizes for one precision: fixed axes (> 0) are kept; each free
+// axis takes the faster of its smooth candidates (and of the caller's
+// tensor size `also`, when given), timed on the device.
 template <typename T>
-static Sizes choose_sizes(vp_op_s* op, OpPrec<T>& P, bool fp64, bool fsym) {
+static Sizes choose_sizes(vp_op_s* op, OpPrec<T>& P, bool fp64, bool fsym, Sizes fixed, Sizes also) {
   const int mode = fft_sizes_mode();
-  if (mode == 1)
-    return Sizes{op->Lp ? op->Lp : 2 * op->L, op->Mp ? op->Mp : 2 * op->M, op->Np ? op->Np : 2 * op->N};
-  const auto cx = axis_candidates(op->Lb, 0, fp64, op->Lp), cy = axis_candidates(op->Mb, 1, fp64, op->Mp),
-             cz = axis_candidates(op->Nb, 2, fp64, op->Np);
+  if (mode == 1) {
+    auto pick = [](int f, int a, int D) { return f ? f : a ? a : 2 * D; };
+    return Sizes{pick(fixed.Lp, also.Lp, op->L), pick(fixed.Mp, also.Mp, op->M), pick(fixed.Np, also.Np, op->N)};
+  }
+  auto cands = [&](int B, int axis, int f, int a) {
+    auto c = axis_candidates(B, axis, fp64, f);
+    if (!f && a > 0 && std::find(c.begin(), c.end(), a) == c.end()) {
+      c.push_back(a);
+      std::sort(c.begin(), c.end());
+    }
+    return c;
+  };
+  const auto cx = cands(op->Lb, 0, fixed.Lp, also.Lp), cy = cands(op->Mb, 1, fixed.Mp, also.Mp),
+             cz = cands(op->Nb, 2, fixed.Np, also.Np);
   VP_REQUIRE(!cx.empty() && !cy.empty() && !cz.empty(), ERR_UNSUPPORTED, "no FFT size candidates for the box");
   if (mode == 2 || (cx.size() == 1 && cy.size() == 1 && cz.size() == 1)) return Sizes{cx[0], cy[0], cz[0]};
-  const SizeKey key{op->ctx->device, op->Lb, op->Mb, op->Nb, int(fp64), int(fsym), op->Lp, op->Mp, op->Np};
+  const SizeKey key{op->ctx->device, op->Lb, op->Mb, op->Nb, int(fp64), int(fsym), fixed.Lp, fixed.Mp, fixed.Np,
+                    also.Lp, also.Mp, also.Np};
   {
     std::lock_guard<std::mutex> lk(g_sizes_mu);
     auto it = g_sizes.find(key);
@@ -542,14 +558,21 @@ static Sizes choose_sizes(vp_op_s* op, OpPrec<T>& P, bool fp64, bool fsym) {
   return best;
 }
 
-// Source of an operator's tensor: the caller's true-frequency tensor at fixed
-// sizes, or a device Toeplitz tensor on the unknowns' box (Lb x Mb x Nb x 6)
-// embedded at the sizes each precision's plan chooses.
+// Source of an operator's tensor: the caller's true-frequency tensor F at
+// its sizes `given` (kept, or resampled to the sizes a precision's plan
+// chooses), or a device Toeplitz tensor on the unknowns' box
+// (Lb x Mb x Nb x 6) embedded at the chosen sizes.
 struct TensorSource {
   const double2* F = nullptr;
-  Sizes fixed{0, 0, 0};
+  Sizes given{0, 0, 0};
   const double2* Gbox = nullptr;
 };
+
+// VP_FN_RESAMPLE=0 keeps a caller's tensor at its own sizes.
+static bool resample_allowed() {
+  const char* e = std::getenv("VP_FN_RESAMPLE");
+  return !(e && std::strcmp(e, "0") == 0);
+}
 
 // Common tail of every vp_op_create*.
 static void op_finish(vp_op_s* op, const TensorSource& src) {
@@ -557,28 +580,45 @@ static void op_finish(vp_op_s* op, const TensorSource& src) {
   // VP_FN_SYMMETRY=0 keeps the full tensor in every variant.
   const char* e = std::getenv("VP_FN_SYMMETRY");
   const bool sym_allowed = !(e && std::strcmp(e, "0") == 0);
-  DevBuf built;            // tensor embedded from Gbox at built_z
+  auto same = [](Sizes a, Sizes b) { return a.Lp == b.Lp && a.Mp == b.Mp && a.Np == b.Np; };
+  DevBuf built;            // tensor embedded from Gbox, or resampled from F, at built_z
   Sizes built_z{0, 0, 0};
-  auto tensor_at = [&](Sizes z) -> const double2* {
-    if (src.F) return src.F;
-    if (built.p && built_z.Lp == z.Lp && built_z.Mp == z.Mp && built_z.Np == z.Np) return built.as<double2>();
-    built.release();
-    built.alloc(size_t(z.Lp) * z.Mp * z.Np * 6 * sizeof(double2));
-    toeplitz_to_fN(op->Lb, op->Mb, op->Nb, z.Lp, z.Mp, z.Np, op->ctx->device, src.Gbox, built.as<double2>(), s);
+  // the tensor at sizes z; for a caller's tensor z may fall back to its own
+  // sizes when a length does not resample
+  auto tensor_at = [&](Sizes& z) -> const double2* {
+    if (src.F && same(z, src.given)) return src.F;
+    if (built.p && same(built_z, z)) return built.as<double2>();
+    if (src.F) {
+      const int P0[3] = {src.given.Lp, src.given.Mp, src.given.Np}, B[3] = {op->Lb, op->Mb, op->Nb},
+                P1[3] = {z.Lp, z.Mp, z.Np};
+      DevBuf r;
+      if (!resample_fN(src.F, P0, B, P1, op->ctx->device, r, s)) {
+        z = src.given;
+        return src.F;
+      }
+      built = std::move(r);
+    } else {
+      built.release();
+      built.alloc(size_t(z.Lp) * z.Mp * z.Np * 6 * sizeof(double2));
+      toeplitz_to_fN(op->Lb, op->Mb, op->Nb, z.Lp, z.Mp, z.Np, op->ctx->device, src.Gbox, built.as<double2>(), s);
+    }
     built_z = z;
     return built.as<double2>();
   };
-  // symmetry: a caller's tensor is checked; an embedded Toeplitz tensor is
-  // parity symmetric by construction (checked all the same)
+  // symmetry: a caller's tensor is checked; an embedded or resampled tensor
+  // is parity symmetric by construction (checked all the same)
   bool fsym = sym_allowed;
-  if (src.F) fsym = sym_allowed && fN_parity_symmetric(src.fixed.Lp, src.fixed.Mp, src.fixed.Np, src.F, s);
+  if (src.F) fsym = sym_allowed && fN_parity_symmetric(src.given.Lp, src.given.Mp, src.given.Np, src.F, s);
+  const bool resample = src.F && resample_allowed();
+  const Sizes fixed = src.F && !resample ? src.given : Sizes{op->Lp, op->Mp, op->Np};
+  const Sizes also = resample ? src.given : Sizes{0, 0, 0};
   auto one = [&](auto tag, bool fp64, double scale_div) {
     using T = decltype(tag);
     OpPrec<T>& P = prec_of<T>(op);
     if (!P.present) return;
-    Sizes z = src.F ? src.fixed : choose_sizes<T>(op, P, fp64, fsym);
+    Sizes z = choose_sizes<T>(op, P, fp64, fsym, fixed, also);
     const double2* F = tensor_at(z);
-    if (!src.F && fsym) fsym = fN_parity_symmetric(z.Lp, z.Mp, z.Np, F, s);
+    if (F != src.F && fsym) fsym = fN_parity_symmetric(z.Lp, z.Mp, z.Np, F, s);
     const double Pn = double(z.Lp) * z.Mp * z.Np;
     finalize_prec(op, P, fp64, F, z, 1.0 / (Pn * scale_div), fsym);
   };
@@ -650,7 +690,8 @@ int vp_op_create(vp_ctx ctx, int64_t L, int64_t M, int64_t N, int64_t Lp, int64_
     VP_HIP(hipMemcpy(F.p, fN, bytes, hipMemcpyHostToDevice));
     TensorSource src;
     src.F = F.as<double2>();
-    src.fixed = Sizes{op->Lp, op->Mp, op->Np};
+    src.given = Sizes{op->Lp, op->Mp, op->Np};
+    op->Lp = op->Mp = op->Np = 0;   // the sizes are free: the tensor resamples
     op_finish(op, src);
   } catch (...) {
     delete op;

@@ -52,6 +52,17 @@ struct DevBuf {
   ~DevBuf() { release(); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   template <typename X> X* as() const { return reinterpret_cast<X*>(p); }
 };
 
@@ -134,6 +145,12 @@ bool fN_parity_symmetric(int Lp, int Mp, int Np, const double2* F, hipStream_t s
 // least 2D-1.
 void toeplitz_to_fN(int L, int M, int N, int Lp, int Mp, int Np, int device, const double2* G, double2* F,
                     hipStream_t s);
+// A caller's true-frequency tensor F (P0[0] x P0[1] x P0[2] x 6) resampled
+// to the circulant sizes P1 (each at least 2B-1 for the unknowns' box
+// extents B): the same operator on the box.  Returns false (out untouched)
+// when no size changes or a changed length does not plan or fit LDS.
+bool resample_fN(const double2* F, const int P0[3], const int B[3], const int P1[3], int device, DevBuf& out,
+                 hipStream_t s);
 // Device assembly of the Toeplitz tensor.
 // ko_im < 0 assembles a damped kernel e^{-j(ko + j ko_im)R}/R (preconditioners only).
 void assemble_toeplitz_dev(int L, int M, int N, double dx, double ko, double2* G, hipStream_t s, double ko_im = 0.0);

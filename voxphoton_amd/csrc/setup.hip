@@ -2,6 +2,7 @@
 // circulant tensor layout, Toeplitz -> circulant spectrum, and assembly of the
 // Toeplitz Green tensor.
 #include <algorithm>
+#include <memory>
 #include <cstdlib>
 #include <cmath>
 #include <vector>
@@ -268,6 +269,93 @@ __global__ void __launch_bounds__(256) k_embed_dft_line(const double2* __restric
   const LdsLayout lay{1, P, 0, 1};
   const double2* R = lds_fft<double, false>(W0, W1, plan, 1, lay, tw);
   for (int j = threadIdx.x; j < P; j += blockDim.x) f[int64_t(j) * inner] = R[j];
+}
+
+// Resampling of a circulant spectrum along one axis: line (i, o) of the
+// input (length P0, stride inner, base i + inner*P0*o) is inverse
+// transformed (the circulant's entries C(d), d < P0), the offsets
+// -(B-1)..(B-1) are kept at their places in a circulant of length P1
+// (negative offsets at P1 - |d|, zeros between), and the result transformed
+// forward at length P1 (stride inner, base i + inner*P1*o).  Any circulant of
+// at least 2B-1 points with the same offsets -(B-1)..(B-1) gives the same
+// product on a box of extent B.
+__global__ void __launch_bounds__(256) k_resample_line(const double2* __restrict__ in, double2* __restrict__ out,
+                                                       int B, int P0, int P1, FftPlan pl0, FftPlan pl1,
+                                                       int64_t inner, const double2* __restrict__ tw0,
+                                                       const double2* __restrict__ tw1) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int Pm = P0 > P1 ? P0 : P1;
+  double2* W0 = reinterpret_cast<double2*>(smem_raw);
+  double2* W1 = W0 + Pm;
+  const int64_t line = blockIdx.x;
+  const int64_t i = line % inner, o = line / inner;
+  const double2* f = in + i + inner * P0 * o;
+  double2* g = out + i + inner * P1 * o;
+  for (int j = threadIdx.x; j < P0; j += blockDim.x) W0[j] = f[int64_t(j) * inner];
+  __syncthreads();
+  const LdsLayout lay0{1, P0, 0, 1};
+  double2* R = lds_fft<double, true>(W0, W1, pl0, 1, lay0, tw0);
+  double2* Q = R == W0 ? W1 : W0;
+  const double inv = 1.0 / P0;
+  for (int j = threadIdx.x; j < P1; j += blockDim.x) {
+    double2 v = make_double2(0.0, 0.0);
+    if (j < B) v = R[j];
+    else if (j > P1 - B) v = R[P0 - (P1 - j)];
+    Q[j] = make_double2(v.x * inv, v.y * inv);
+  }
+  __syncthreads();
+  const LdsLayout lay1{1, P1, 0, 1};
+  const double2* R2 = lds_fft<double, false>(Q, R, pl1, 1, lay1, tw1);
+  for (int j = threadIdx.x; j < P1; j += blockDim.x) g[int64_t(j) * inner] = R2[j];
+}
+
+bool resample_fN(const double2* F, const int P0[3], const int B[3], const int P1[3], int device, DevBuf& out,
+                 hipStream_t s) {
+  FftPlan pl0[3]{}, pl1[3]{};
+  const size_t lmax = size_t(max_lds_bytes(device));
+  for (int a = 0; a < 3; ++a) {
+    if (P0[a] == P1[a]) continue;
+    if (P1[a] < 2 * B[a] - 1 || P0[a] < 2 * B[a] - 1) return false;
+    if (!factorize(P0[a], true, pl0[a], 1) || !factorize(P1[a], true, pl1[a], 1) ||
+        size_t(2) * std::max(P0[a], P1[a]) * sizeof(double2) > lmax)
+      return false;
+  }
+  auto table = [&](int P) {
+    auto t = twiddle_table(P);
+    std::unique_ptr<DevBuf> d(new DevBuf(t.size() * sizeof(double2)));
+    VP_HIP(hipMemcpyAsync(d->p, t.data(), d->bytes, hipMemcpyHostToDevice, s));
+    VP_HIP(hipStreamSynchronize(s));   // t is freed on return
+    return d;
+  };
+  const void* fk = reinterpret_cast<const void*>(&k_resample_line);
+  int cur[3] = {P0[0], P0[1], P0[2]};
+  const double2* in = F;
+  DevBuf bufs[2];
+  int nb = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (P0[a] == P1[a]) continue;
+    int nxt[3] = {cur[0], cur[1], cur[2]};
+    nxt[a] = P1[a];
+    DevBuf& o = bufs[nb++ & 1];
+    o.release();
+    o.alloc(size_t(nxt[0]) * nxt[1] * nxt[2] * 6 * sizeof(double2));
+    auto tw0 = table(P0[a]), tw1 = table(P1[a]);
+    // lines along axis a: stride = product of the extents before a; the
+    // outer index runs over the extents after a and the 6 components
+    const int64_t inner = a == 0 ? 1 : a == 1 ? int64_t(nxt[0]) : int64_t(nxt[0]) * nxt[1];
+    const int64_t outer = a == 0 ? int64_t(nxt[1]) * nxt[2] * 6 : a == 1 ? int64_t(nxt[2]) * 6 : 6;
+    const size_t lds = size_t(2) * std::max(P0[a], P1[a]) * sizeof(double2);
+    (void)hipFuncSetAttribute(fk, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    hipLaunchKernelGGL(k_resample_line, dim3(unsigned(inner * outer)), dim3(256), lds, s, in, o.as<double2>(), B[a],
+                       P0[a], P1[a], pl0[a], pl1[a], inner, tw0->as<double2>(), tw1->as<double2>());
+    VP_HIP(hipGetLastError());
+    VP_HIP(hipStreamSynchronize(s));   // the twiddle tables and the previous buffer are freed next
+    in = o.as<double2>();
+    cur[a] = P1[a];
+  }
+  if (nb == 0) return false;
+  out = std::move(bufs[(nb - 1) & 1]);
+  return true;
 }
 
 void toeplitz_to_fN(int L, int M, int N, int Lp, int Mp, int Np, int device, const double2* G, double2* F,
