@@ -180,6 +180,51 @@ def dense_AN(G, Mr, Mc, Gram, idx):
     return A[np.ix_(i0, i0)]
 
 
+def an_rows(G, Mr, Mc, Gram, idx, x, rows, shape):
+    """Selected entries of y = A x, A = Gram*Mr - Mc.*N on idx, by direct
+    Toeplitz sums (the product mv_AN_const.m:102-146 evaluates by FFT).
+
+    ``G`` holds the Toeplitz tensor at offsets 0 .. D-1 per axis with D at
+    least the extent of the unknowns' voxels, ``Mr``/``Mc`` are per-entry of
+    idx (length n) or per voxel (``shape``), ``rows`` 0-based positions in
+    idx.  Cost O(n) per row: usable at full size for a handful of rows.
+    Negative offsets take the parity signs of the dense product above.
+    """
+    L, M, N = shape
+    V = L * M * N
+    i0 = np.asarray(idx, dtype=np.int64) - 1
+    comp, vox = np.divmod(i0, V)
+    vi = vox % L
+    vj = (vox // L) % M
+    vk = vox // (L * M)
+    x = np.asarray(x, dtype=complex)
+
+    def per_entry(a, r):
+        a = np.asarray(a)
+        if a.ndim == 0:
+            return complex(a)
+        if a.size == i0.size and a.ndim == 1:
+            return complex(a[r])
+        return complex(a.reshape(-1, order='F')[vox[r]])
+
+    out = np.zeros(len(rows), dtype=complex)
+    for t, r in enumerate(rows):
+        c = int(comp[r])
+        dx = vi[r] - vi
+        dy = vj[r] - vj
+        dz = vk[r] - vk
+        acc = 0j
+        for d in range(3):
+            sel = comp == d
+            k = COMP[c, d]
+            sx, sy, sz = dx[sel], dy[sel], dz[sel]
+            sign = (np.where(sx < 0, PARITY[k, 0], 1) * np.where(sy < 0, PARITY[k, 1], 1) *
+                    np.where(sz < 0, PARITY[k, 2], 1))
+            acc += np.sum(sign * G[np.abs(sx), np.abs(sy), np.abs(sz), k] * x[sel])
+        out[t] = Gram * per_entry(Mr, r) * x[r] - per_entry(Mc, r) * acc
+    return out
+
+
 def idx_from_mask(mask):
     """idxS3 construction (pc/src_vie/src_solver/JVIE_Solver_const.m:148-149).
 

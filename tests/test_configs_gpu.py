@@ -14,6 +14,10 @@ against the FP64 oracle.
   the request server, oracle residuals, each result equal to the same
   request served alone, and a request with a NaN right-hand side that fails
   without changing the others.
+* the microdisk with a bus at 15 nm, 1400x1500x30 (64.9 M unknowns): the
+  matvec against direct Toeplitz sums at sampled rows, FP32 against FP64,
+  and against the reference's 2L x 2M x 2N embedding; refinement steps of
+  the mixed solve whose reported residual is the true FP64 residual.
 
 The oracle's tensor is the device-assembled Toeplitz tensor (its cubature is
 checked against the oracle's at small grids); everything after it (the
@@ -286,3 +290,86 @@ def test_serving_stream_against_oracle_and_alone(vp_ctx):
         assert true <= r['prm']['tol'] * (1 + 1e-6)
         assert rel(x, xa) <= 1e-12
         assert info['relres'] == ia['relres']
+
+
+# ---------------------------------------------------------------------------
+# microdisk with a bus waveguide, 1400 x 1500 x 30 at 15 nm (64.9 M unknowns)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.timeout(1100)
+def test_microdisk15_full_size(vp_ctx, monkeypatch):
+    # At this size the oracle's FFT product is out of reach on the host, so
+    # the matvec is checked (a) entry by entry against direct Toeplitz sums
+    # over all unknowns at sampled rows, (b) FP32 against FP64 and (c) against
+    # the same operator embedded at the reference's circulant sizes
+    # 2L x 2M x 2N (the product on idx does not depend on the embedding).
+    # The solve runs a few refinement steps of the mixed GMRES(50) with the
+    # block-Jacobi composite; its reported residual must be the true FP64
+    # residual of the checked matvec.
+    from voxphoton_amd import configs, vie
+    import bench
+    free0, total = vp_ctx.mem_info()
+    t = time.perf_counter()
+    prob = configs.config_problem('microdisk_15nm')
+    assert prob.shape == (1400, 1500, 30)
+    n = prob.n
+    _say(f'microdisk 15 nm: n = {n} unknowns, geometry in {time.perf_counter() - t:.1f} s')
+    t = time.perf_counter()
+    op = vie.NOperator.assembled(prob.shape, prob.dx, prob.ko, None, prob.Mc, prob.idx, ctx=vp_ctx)
+    info = op.info()
+    _say(f'operator in {time.perf_counter() - t:.1f} s: box {info["box"]} at {info["box_origin"]}, FFT sizes '
+         f'fp64 {info["fft_fp64"]} fp32 {info["fft_fp32"]}')
+    x = _seeded(n)
+    y64 = op.apply(x)
+    y32 = op.apply(x, prec='fp32')
+    e32 = rel(y32, y64)
+    rms = np.linalg.norm(y64) / np.sqrt(n)
+    _say(f'fp32 vs fp64 matvec {e32:.2e}')
+    assert e32 < 1e-5
+
+    # (a) direct sums at rows in the bus, the disk and on the disk's rim
+    G = vie.assemble_toeplitz(info['box'], prob.dx, prob.ko, ctx=vp_ctx)
+    rng = np.random.default_rng(15)
+    rows = list(rng.choice(n, 5, replace=False)) + [0, n // 3 + 7, n - 1]
+    t = time.perf_counter()
+    ref = ovie.an_rows(G, 1.0, prob.Mc, prob.gram, prob.idx, x, rows, prob.shape)
+    del G
+    err64 = np.abs(y64[rows] - ref).max() / rms
+    err32 = np.abs(y32[rows] - ref).max() / rms
+    _say(f'direct sums at {len(rows)} rows ({time.perf_counter() - t:.1f} s): max error / rms(y) '
+         f'fp64 {err64:.2e} fp32 {err32:.2e}')
+    assert err64 < 1e-11
+    assert err32 < 1e-4
+
+    # (c) the reference's embedding
+    monkeypatch.setenv('VP_FFT_SIZES', 'reference')
+    t = time.perf_counter()
+    op_ref = vie.NOperator.assembled(prob.shape, prob.dx, prob.ko, None, prob.Mc, prob.idx, precisions=('fp64',),
+                                     ctx=vp_ctx)
+    monkeypatch.delenv('VP_FFT_SIZES')
+    assert op_ref.info()['fft_fp64'] == (2800, 3000, 60)
+    e_emb = rel(op_ref.apply(x), y64)
+    _say(f'embedding at 2800x3000x60 ({time.perf_counter() - t:.1f} s): relative difference {e_emb:.2e}')
+    assert e_emb < 1e-12
+    op_ref.close()
+    del op_ref, y32
+
+    # mixed solve: a few refinement steps; reported residual = true residual
+    wl = dict(bench.WORKLOADS['microdisk_15nm'])
+    t = time.perf_counter()
+    P, desc = bench.make_preconditioner(prob, wl, vie, configs)
+    _say(f'{desc}: setup {time.perf_counter() - t:.1f} s')
+    b = prob.gaussian_rhs('y')
+    t = time.perf_counter()
+    xm, sinfo = vie.solve_ir(op, b, tol=wl['tol'], restart=wl['restart'], inner_tol=wl['inner_tol'],
+                             inner_maxit=wl['inner_maxit'], max_outer=3, M=P, side='right')
+    dt = time.perf_counter() - t
+    free1, _ = vp_ctx.mem_info()
+    true = float(np.linalg.norm(b - op.apply(xm)) / np.linalg.norm(b))
+    _say(f'mixed GMRES({wl["restart"]}) x {sinfo["ir_steps"]} refinement steps ({sinfo["inner_iters"]} inner its, '
+         f'{dt:.1f} s): relres {sinfo["relres"]:.4e}, true FP64 relres {true:.4e}, history '
+         f'{[float("%.3g" % v) for v in sinfo["history"]]}; device memory in use {(total - free1) / 2**30:.1f} GiB '
+         f'of {total / 2**30:.0f} GiB ({(total - free0) / 2**30:.1f} GiB before)')
+    assert abs(true - sinfo['relres']) <= 1e-9 * true
+    assert sinfo['history'][-1] < sinfo['history'][0]
+    assert sinfo['relres'] < 0.5

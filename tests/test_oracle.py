@@ -443,3 +443,23 @@ def test_bicgstab_recurrence_residual_is_the_true_residual():
     for k in range(1, 8):
         x, flag, relres, it, rv = okry.bicgstab(lambda v: A @ v, b, 1e-30, k)
         assert abs(rv[-1] - np.linalg.norm(b - A @ x) / np.linalg.norm(b)) <= 1e-12 * max(rv[-1], 1e-3)
+
+
+def test_an_rows_equal_dense_product_rows():
+    # the sampled-row product (direct Toeplitz sums) against the dense
+    # operator, with Mr/Mc per voxel and per entry, and a tensor that holds
+    # more offsets than the unknowns span
+    from tests.helpers import random_problem
+    G, Mr, Mc, Gram, idx, x = random_problem(6, 5, 4, seed=21, with_mr=True)
+    A = ovie.dense_AN(G, Mr, Mc, Gram, idx)
+    rows = [0, 3, idx.size // 2, idx.size - 1]
+    y = A @ x
+    assert np.abs(ovie.an_rows(G, Mr, Mc, Gram, idx, x, rows, (6, 5, 4)) - y[rows]).max() <= 1e-13 * np.abs(y).max()
+    i0 = idx - 1
+    mr_e = np.tile(Mr.reshape(-1, order='F'), 3)[i0]
+    mc_e = np.tile(Mc.reshape(-1, order='F'), 3)[i0]
+    assert np.abs(ovie.an_rows(G, mr_e, mc_e, Gram, idx, x, rows, (6, 5, 4)) - y[rows]).max() <= \
+        1e-13 * np.abs(y).max()
+    Gbig = np.concatenate([G, np.ones((3, 5, 4, 6), complex)], axis=0)
+    assert np.abs(ovie.an_rows(Gbig, Mr, Mc, Gram, idx, x, rows, (6, 5, 4)) - y[rows]).max() <= \
+        1e-13 * np.abs(y).max()
