@@ -74,7 +74,7 @@ WORKLOADS = {
              'U{20..150} periods, 20% microdisks of radius U[2,5] um (220 nm thick); requests admitted by the '
              'server HBM budget, refinement rounds batched per geometry',
         tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=4, max_outer=20, nreq=40, rate=50.0, seed=4,
-        budget_gb=96, max_active=32),
+        budget_gb=96, max_active=32, setup_workers=4),
     'microdisk_15nm': dict(
         desc='Si microdisk r = 10 um, 220 nm thick, with a 500 nm bus waveguide at a 200 nm gap, SiO2, '
              'lambda 1550 nm, 15 nm voxels, grid 1400x1500x30 (63 M voxels), y-polarised Gaussian (w0 300 nm) '
@@ -744,10 +744,10 @@ def serve_requests(wl):
 def serve_problem(kind, size, lam, configs):
     dx = 20e-9
     # Preconditioners measured per kind: waveguides and gratings converge
-    # fastest with the face-averaged form exact along y; disks need the
-    # core-contrast form with a complex-shifted kernel (face-averaged forms
-    # stagnate on the circular boundary) and GMRES(200): with GMRES(100) a
-    # disk near a resonance stalled at 1.8e-6 after 40 refinement steps.
+    # fastest with the face-averaged form exact along y; disks with the
+    # core-contrast 2-level form on the disk's box, undamped (152-189 inner
+    # iterations for radii 2.2-4.8 um, against 468-1071 with a damped kernel
+    # and GMRES(200)).
     if kind == 'waveguide':
         prob = configs.straight_waveguide(L=int(round(size / dx)), M=48, N=24, dx=dx, lam=lam)
         pc = dict(prec='face', axis=1, damping=0.0)
@@ -756,63 +756,87 @@ def serve_problem(kind, size, lam, configs):
         pc = dict(prec='face', axis=1, damping=0.0)
     else:
         prob = configs.disk_resonator(radius=size, dx=dx, lam=lam)
-        pc = dict(prec='scalar', damping=0.2, restart=200, inner_maxit=2, max_outer=40)
+        pc = dict(prec='scalar', damping=0.0)
     return configs.smoothen(prob), pc
 
 
 def run_serve(args, wl, log):
     """Serving stream through the request server (continuous batching).
 
-    The client thread builds each request's operator and preconditioner on
-    its own stream when the request arrives (setup overlaps the server's
-    solves on the device), submits it and collects finished requests.
+    Client setup (geometry, operator, preconditioner, right-hand side) runs
+    on a pool of setup threads, each with its own context and stream, from
+    each request's arrival; the main thread submits requests as their setup
+    finishes and collects finished requests, so setup overlaps both other
+    setups and the server's solves on the device.
     Latency: submit -> finish (queue + solve) and arrival -> finish (with
     setup and any client backlog)."""
+    import concurrent.futures
+    import threading
     from voxphoton_amd import vie, configs
-    from voxphoton_amd import _lib as L
     reqs = serve_requests(wl)
     ctx = vie.default_context()
     srv = vie.Server(int(wl['budget_gb'] * (1 << 30)), wl['max_active'], ctx=ctx)
     live = {}
     done = []
     t_setup = []
+    tls = threading.local()
+    contexts = []
+    ctx_mu = threading.Lock()
 
-    def collect(block=False):
+    def setup(i, t_arr, kind, size, lam):
+        c = getattr(tls, 'ctx', None)
+        if c is None:
+            c = tls.ctx = vie.Context(ctx.device)
+            with ctx_mu:
+                contexts.append(c)
+        t0 = time.perf_counter()
+        prob, pc = serve_problem(kind, size, lam, configs)
+        op = vie.NOperator.assembled(prob.shape, prob.dx, prob.ko, None, prob.Mc, prob.idx, ctx=c)
+        P, _ = make_preconditioner(prob, pc, vie, configs, ctx=c)
+        b = prob.gaussian_rhs('y')
+        db = vie.DeviceArray(c, b.nbytes)
+        dxv = vie.DeviceArray(c, b.nbytes)
+        db.upload(b)
+        c.synchronize()
+        return dict(i=i, kind=kind, size=size, n=prob.n, t_arr=t_arr, pc=pc, op=op, P=P, db=db, dxv=dxv,
+                    setup=time.perf_counter() - t0)
+
+    def collect():
         for tk in list(live):
             fin, info = srv.poll(tk)
-            if fin or block:
+            if fin:
                 info = srv.wait(tk)
                 rec = live.pop(tk)
                 rec['info'] = info
                 rec['t_done'] = time.perf_counter() - t_start
+                rec.pop('keep', None)
                 done.append(rec)
 
+    pool = concurrent.futures.ThreadPoolExecutor(max_workers=wl['setup_workers'])
+    pending = []
+    nxt = 0
     t_start = time.perf_counter()
-    for i, (t_arr, kind, size, lam) in enumerate(reqs):
-        while time.perf_counter() - t_start < t_arr:
-            collect()
-            time.sleep(0.001)
-        t0 = time.perf_counter()
-        prob, pc = serve_problem(kind, size, lam, configs)
-        op = vie.NOperator.assembled(prob.shape, prob.dx, prob.ko, None, prob.Mc, prob.idx, ctx=ctx)
-        P, _ = make_preconditioner(prob, pc, vie, configs)
-        b = prob.gaussian_rhs('y')
-        db = vie.DeviceArray(ctx, b.nbytes)
-        dxv = vie.DeviceArray(ctx, b.nbytes)
-        db.upload(b)
-        ctx.synchronize()
-        t_setup.append(time.perf_counter() - t0)
-        t_sub = time.perf_counter() - t_start
-        tk = srv.submit(op, P, db, dxv, tol=wl['tol'], restart=pc.get('restart', wl['restart']),
-                        inner_tol=wl['inner_tol'], inner_maxit=pc.get('inner_maxit', wl['inner_maxit']),
-                        max_outer=pc.get('max_outer', wl['max_outer']), side='right')
-        live[tk] = dict(i=i, kind=kind, size=size, n=prob.n, t_arr=t_arr, t_sub=t_sub, keep=(op, P, db, dxv))
-        log(f'  request {i} {kind} {size:.3g} n={prob.n} arrival {t_arr:.3f} s submitted {t_sub:.3f} s '
-            f'(setup {t_setup[-1]:.2f} s), in flight {len(live)}, server {srv.stats()}')
+    while nxt < len(reqs) or pending or live:
+        now = time.perf_counter() - t_start
+        while nxt < len(reqs) and reqs[nxt][0] <= now:
+            pending.append(pool.submit(setup, nxt, *reqs[nxt]))
+            nxt += 1
+        for f in [f for f in pending if f.done()]:
+            pending.remove(f)
+            r = f.result()
+            pc = r['pc']
+            t_setup.append(r['setup'])
+            t_sub = time.perf_counter() - t_start
+            tk = srv.submit(r['op'], r['P'], r['db'], r['dxv'], tol=wl['tol'], restart=pc.get('restart', wl['restart']),
+                            inner_tol=wl['inner_tol'], inner_maxit=pc.get('inner_maxit', wl['inner_maxit']),
+                            max_outer=pc.get('max_outer', wl['max_outer']), side='right')
+            live[tk] = dict(i=r['i'], kind=r['kind'], size=r['size'], n=r['n'], t_arr=r['t_arr'], t_sub=t_sub,
+                            keep=(r['op'], r['P'], r['db'], r['dxv']))
+            log(f'  request {r["i"]} {r["kind"]} {r["size"]:.3g} n={r["n"]} arrival {r["t_arr"]:.3f} s submitted '
+                f'{t_sub:.3f} s (setup {r["setup"]:.2f} s), in flight {len(live)}, server {srv.stats()}')
         collect()
-    while live:
-        collect()
-        time.sleep(0.002)
+        time.sleep(0.001)
+    pool.shutdown()
     t_total = max(r['t_done'] for r in done) - reqs[0][0]
     srv.close()
     lat = np.array([r['info']['seconds'] for r in done]) * 1e3
@@ -827,7 +851,7 @@ def run_serve(args, wl, log):
         'data': 'synthetic (seeded request stream; device-assembled Green tensors)',
         'config': {'workload': args.workload, 'description': wl['desc'], 'requests': len(done), 'mix': kinds,
                    'unknowns_min': int(min(r['n'] for r in done)), 'unknowns_max': int(max(r['n'] for r in done)),
-                   'tol': wl['tol'], 'restart': f"{wl['restart']} (disks 200)", 'hbm_budget_gb': wl['budget_gb'],
+                   'tol': wl['tol'], 'restart': wl['restart'], 'hbm_budget_gb': wl['budget_gb'],
                    'max_active': wl['max_active'], 'parallelism': 'single GPU'},
         'relres_worst': worst, 'all_converged': all(r['info']['flag'] == 0 for r in done),
         'latency_ms': {'p50': round(float(np.percentile(lat, 50)), 3), 'p99': round(float(np.percentile(lat, 99)), 3),
@@ -936,7 +960,7 @@ def field_block(prob, vie, L, ctx, xsol, log):
                 x_thru=x_thru)
 
 
-def make_preconditioner(prob, wl, vie, configs):
+def make_preconditioner(prob, wl, vie, configs, ctx=None):
     """2-level circulant preconditioner of the workload on its core box.
 
     'face': contrast averaged over the two circulant axes, exact along
@@ -950,24 +974,24 @@ def make_preconditioner(prob, wl, vie, configs):
         for box in prob.extra['parts']:
             shape, bidx, sub = prob.box_problem(box)
             q = vie.Preconditioner.level2_assembled(shape, prob.dx, prob.ko, prob.core_contrast(), 1.0, bidx,
-                                                    damping=wl['damping'])
+                                                    damping=wl['damping'], ctx=ctx)
             q.set_embedding(n, sub)
             q.box_shape = tuple(int(v) for v in shape)
             parts.append(q)
-        P = vie.Preconditioner.composite(parts)
+        P = vie.Preconditioner.composite(parts, ctx=ctx)
         P.box_shape = parts[-1].box_shape
         desc = ('block-Jacobi composite of 2-level circulants (core contrast) on the boxes '
                 f'{[q.box_shape for q in parts]}, damping {wl["damping"]}')
         return P, desc
     if wl['prec'] == 'face':
         shape, pidx, sub, face = configs.face_box(prob, wl['axis'])
-        P = vie.Preconditioner.level2_face(shape, prob.dx, prob.ko, face, pidx, damping=wl['damping'])
+        P = vie.Preconditioner.level2_face(shape, prob.dx, prob.ko, face, pidx, damping=wl['damping'], ctx=ctx)
         desc = (f'2-level circulant, face-averaged contrast, exact along {"yz"[wl["axis"] - 1]}, box {shape}, '
                 f'damping {wl["damping"]}')
     else:
         shape, bidx, sub = prob.box_problem()
         P = vie.Preconditioner.level2_assembled(shape, prob.dx, prob.ko, prob.core_contrast(), 1.0, bidx,
-                                                damping=wl['damping'])
+                                                damping=wl['damping'], ctx=ctx)
         desc = f'2-level circulant, core contrast, box {shape}, damping {wl["damping"]}'
     P.set_embedding(n, sub)
     P.box_shape = tuple(int(v) for v in shape)

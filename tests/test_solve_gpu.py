@@ -71,13 +71,15 @@ def test_level2_device_setup_matches_oracle(vp_ctx, monkeypatch, shape, sym, mod
     assert rel(y, ref) < 1e-10
 
 
-@pytest.mark.parametrize('half', ['1', '0'])
+@pytest.mark.parametrize('half', ['1', '0', 'dot2'])
 def test_level2_blocks_wider_than_a_wavefront(vp_ctx, monkeypatch, half):
     # 3N = 66 > 64: block inversion spans several wavefronts.  The FP32 apply
     # of the quadrant form streams binary16 blocks by default (11-bit
-    # significands: ~5e-4 per entry) or FP32 blocks with VP_PC_HALF=0.
+    # significands: ~5e-4 per entry) or FP32 blocks with VP_PC_HALF=0; with
+    # VP_PC_DOT2=1 the block product also rounds the vector to binary16.
     from voxphoton_amd.vie import Preconditioner
-    monkeypatch.setenv('VP_PC_HALF', half)
+    monkeypatch.setenv('VP_PC_HALF', '0' if half == '0' else '1')
+    monkeypatch.setenv('VP_PC_DOT2', '1' if half == 'dot2' else '0')
     L, M, N = 4, 3, 22
     dx, ko = 20e-9, 5.85e6
     G = ogr.assemble_G(L, M, N, dx, ko)
@@ -89,10 +91,14 @@ def test_level2_blocks_wider_than_a_wavefront(vp_ctx, monkeypatch, half):
     P = Preconditioner.level2(G, 0.8278, 1.0, dx ** 3, idx, ctx=vp_ctx)
     assert rel(P.apply(x), ref) < 1e-10
     e32 = rel(P.apply(x, 'fp32'), ref)
-    print('fp32 apply, binary16 blocks' if half == '1' else 'fp32 apply', e32, flush=True)
-    assert e32 < (2e-3 if half == '1' else 1e-4)
-    if half == '1':
+    print({'1': 'fp32 apply, binary16 blocks', '0': 'fp32 apply', 'dot2': 'fp32 apply, dot2'}[half], e32, flush=True)
+    assert e32 < (1e-4 if half == '0' else 2e-3)
+    if half != '0':
         assert e32 > 1e-6      # the binary16 copy is what ran
+    if half == 'dot2':         # the dot2 product is what ran
+        y_dot = P.apply(x, 'fp32')
+        monkeypatch.setenv('VP_PC_DOT2', '0')
+        assert not np.array_equal(y_dot, P.apply(x, 'fp32'))
 
 
 @pytest.mark.parametrize('quad', ['1', '0'])
@@ -265,6 +271,36 @@ def test_gmres_preconditioned(vp_ctx, side):
     assert flag == 0
     assert np.linalg.norm(b - Ad @ x) / np.linalg.norm(b) <= 1e-9 * 1.01
     assert len(rv) < len(rv0)
+
+
+def test_gmres_step_chunks_give_identical_results(vp_ctx, monkeypatch):
+    # the Arnoldi steps run in chunks gated on the device; the chunk length
+    # (host round trips) changes nothing: same iterates, histories, flags,
+    # for single and batched solves in which problems stop at different steps
+    from voxphoton_amd.vie import NOperator, Preconditioner, pgmres, solve_ir_batched
+    G, Mc, Gram, idx, b, dx, ko = _physical(8, 5, 4, seed=6)
+    op = NOperator.from_toeplitz(G, None, Mc, Gram, idx, ctx=vp_ctx)
+    P = Preconditioner.level2(G, Mc.flat[0], 1.0, Gram, idx, ctx=vp_ctx)
+    rng = np.random.default_rng(6)
+    B = np.array([b, rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size), 0.5 * b])
+    out = {}
+    for chunk in ('1', '3', '64'):
+        monkeypatch.setenv('VP_GMRES_CHUNK', chunk)
+        single = pgmres(op, b, 17, 1e-10, 6, R1=P)
+        X, infos = solve_ir_batched([op] * 3, [P] * 3, B, tol=1e-9, restart=13, inner_tol=1e-4, inner_maxit=3,
+                                    max_outer=10, side='right')
+        out[chunk] = (single, X, infos)
+    s1, X1, i1 = out['1']
+    for chunk in ('3', '64'):
+        s, X, infos = out[chunk]
+        assert np.array_equal(s[0], s1[0]) and s[1] == s1[1] and s[2] == s1[2]
+        assert np.array_equal(s[4], s1[4])
+        assert np.array_equal(X, X1)
+        for a, c in zip(infos, i1):
+            assert a['inner_iters'] == c['inner_iters'] and a['relres'] == c['relres']
+            assert np.array_equal(a['history'], c['history'])
+    print('chunked GMRES: single', len(s1[4]) - 1, 'its; batched inner its', [c['inner_iters'] for c in i1],
+          flush=True)
 
 
 def test_gmres_maxit_flag_and_breakdown(vp_ctx):

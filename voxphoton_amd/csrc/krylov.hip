@@ -217,14 +217,35 @@ __global__ void __launch_bounds__(KT) k_reduce_cols(const double2* __restrict__ 
   block_reduce_store<1>(acc, out + int64_t(p) * maxcols + j, 1, 1);
 }
 
-// Publishes one Arnoldi step's estimates and convergence flags into coherent
-// pinned host memory, then (after a system-scope fence) the step's sequence
-// number, which the host polls.
-__global__ void k_publish(const double* __restrict__ est, const int* __restrict__ conv, int np, double* hest,
-                          int* hconv, int* hseq, int seq) {
+// Closes Arnoldi step k on the device: for each problem still stepping,
+// records the step's residual estimate in hist[p][k] and the steps done,
+// and clears its stepping flag once it converged or filled the basis, so
+// that every later step of the cycle is a no-op for it.  The host then only
+// needs the status at the end of a chunk of steps.
+__global__ void k_step_end(const double* __restrict__ est, const int* __restrict__ conv, int* __restrict__ iact,
+                           double* __restrict__ hist, int* __restrict__ kdone, int k, int m, int np) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np || !iact[p]) return;
+  hist[int64_t(p) * m + k] = est[p];
+  kdone[p] = k + 1;
+  if (conv[p] || k == m - 1) iact[p] = 0;
+}
+
+// Publishes a chunk's status (the estimates of steps k0..k1-1, the steps
+// done and the stepping flags) into coherent pinned host memory, then
+// (after a system-scope fence) the chunk's sequence number, which the host
+// polls.
+__global__ void k_publish(const double* __restrict__ hist, const int* __restrict__ kdone,
+                          const int* __restrict__ iact, int np, int m, int k0, int k1, double* hhist, int* hkdone,
+                          int* hiact, int* hseq, int seq) {
+  const int w = k1 - k0;
+  for (int e = threadIdx.x; e < np * w; e += blockDim.x) {
+    const int p = e / w, k = k0 + e - p * w;
+    reinterpret_cast<volatile double*>(hhist)[int64_t(p) * m + k] = hist[int64_t(p) * m + k];
+  }
   for (int p = threadIdx.x; p < np; p += blockDim.x) {
-    reinterpret_cast<volatile double*>(hest)[p] = est[p];
-    reinterpret_cast<volatile int*>(hconv)[p] = conv[p];
+    reinterpret_cast<volatile int*>(hkdone)[p] = kdone[p];
+    reinterpret_cast<volatile int*>(hiact)[p] = iact[p];
   }
   __threadfence_system();
   __syncthreads();
@@ -234,9 +255,6 @@ __global__ void k_publish(const double* __restrict__ est, const int* __restrict_
   }
 }
 
-// Spins until the device has published step ``seq``.  Past 200 ms of
-// spinning it waits on the stream instead and, should the published values
-// still not be visible, copies them from the device.
 // Coherent pinned status buffers of the per-step publication, kept in a
 // process-wide pool: a solve takes one at its start and returns it at its
 // end, so repeated solves allocate no pinned memory.  The buffers live until
@@ -273,21 +291,31 @@ struct StatusLease {
 };
 }  // namespace
 
-static void wait_published(const int* hseq, int seq, hipStream_t s, double* hest, const double* est, int* hconv,
-                           const int* conv, int np) {
+// Spins until the device has published chunk ``seq``.  Past 200 ms of
+// spinning it waits on the stream instead and, should the published values
+// still not be visible, has them copied from the device (fallback).
+template <typename F>
+static void wait_published(const int* hseq, int seq, hipStream_t s, F fallback) {
   const volatile int* vs = hseq;
   const auto t0 = std::chrono::steady_clock::now();
   for (unsigned spins = 1; *vs != seq; ++spins) {
     if ((spins & 4095u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
       VP_HIP(hipStreamSynchronize(s));
-      if (*vs != seq) {
-        VP_HIP(hipMemcpy(hest, est, np * sizeof(double), hipMemcpyDeviceToHost));
-        VP_HIP(hipMemcpy(hconv, conv, np * sizeof(int), hipMemcpyDeviceToHost));
-      }
+      if (*vs != seq) fallback();
       break;
     }
   }
   std::atomic_thread_fence(std::memory_order_acquire);
+}
+
+// Arnoldi steps per host round trip (VP_GMRES_CHUNK, 1..64): the steps of a
+// chunk are enqueued back to back and gated on the device, so a problem
+// that converges inside a chunk costs at most chunk-1 steps of empty
+// launches and its results are the same at every chunk length.
+static int gmres_chunk() {
+  const char* e = std::getenv("VP_GMRES_CHUNK");
+  const int v = e ? std::atoi(e) : 4;
+  return std::min(64, std::max(1, v));
 }
 
 struct ArnoldiState {
@@ -743,17 +771,22 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   // per-step estimates and convergence flags: the device publishes them into
   // coherent pinned host memory, followed by the step's sequence number, and
   // the host spins on that number (no copy kernels, no blocking wait)
-  StatusLease pin(np * (sizeof(double) + sizeof(int)) + sizeof(int));
-  double* hest = static_cast<double*>(pin.p);
-  int* hconv = reinterpret_cast<int*>(hest + np);
-  int* hseq = hconv + np;
+  // chunk status: step estimates hist[p][k], steps done, stepping flags
+  StatusLease pin(size_t(np) * m * sizeof(double) + size_t(2 * np + 1) * sizeof(int));
+  double* hhist = static_cast<double*>(pin.p);
+  int* hkdone = reinterpret_cast<int*>(hhist + size_t(np) * m);
+  int* hiact = hkdone + np;
+  int* hseq = hiact + np;
   *reinterpret_cast<volatile int*>(hseq) = -1;
   void* dpin = nullptr;
   VP_HIP(hipHostGetDevicePointer(&dpin, pin.p, 0));
-  double* dhest = static_cast<double*>(dpin);
-  int* dhconv = reinterpret_cast<int*>(dhest + np);
-  int* dhseq = dhconv + np;
+  double* dhhist = static_cast<double*>(dpin);
+  int* dhkdone = reinterpret_cast<int*>(dhhist + size_t(np) * m);
+  int* dhiact = dhkdone + np;
+  int* dhseq = dhiact + np;
+  DevBuf hist(size_t(np) * m * sizeof(double)), kdone(np * sizeof(int));
   int seq = 0;
+  const int chunk = gmres_chunk();
   VP_HIP(hipMemcpy(hbn.data(), bn.p, np * sizeof(double), hipMemcpyDeviceToHost));
   // initial residual into V_0 slot of each problem
   auto residual_into_V0 = [&](bool x_nonzero) {
@@ -930,11 +963,13 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     }
     DevBuf iact(np * sizeof(int));
     VP_HIP(hipMemcpyAsync(iact.p, inner.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
-    const int* dia = iact.as<int>();
+    int* dia = iact.as<int>();
     hipLaunchKernelGGL(k_scale<T>, dim3(grid1(n), np), dim3(256), 0, s, Vb, vprob, n, beta.as<double>(), 1, dia);
     hipLaunchKernelGGL(k_gmres_init, dim3(np), dim3(64), 0, s, st, beta.as<double>(), np, dia);
     VP_HIP(hipGetLastError());
-    for (int k = 0; k < m; ++k) {
+    // Arnoldi steps in chunks: each step's kernels are gated on the device
+    // stepping flags dia, which k_step_end clears as problems stop
+    auto step = [&](int k) {
       C<T>* vk = Vb + int64_t(k) * n;
       C<T>* w = Vb + int64_t(k + 1) * n;
       if (right) {          // w = A M^-1 v_k
@@ -973,30 +1008,39 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       hipLaunchKernelGGL(k_arnoldi, dim3((np + 63) / 64), dim3(64), 0, s, st, k, dia, np, ract.as<int>());
       hipLaunchKernelGGL(k_scale<T>, dim3(grid1(n), np), dim3(256), 0, s, w, vprob, n, invn.as<double>(), 0, dia);
       VP_HIP(hipGetLastError());
-      ++seq;
-      hipLaunchKernelGGL(k_publish, dim3(1), dim3(256), 0, s, est.as<double>(), conv.as<int>(), np, dhest, dhconv,
-                         dhseq, seq);
+      hipLaunchKernelGGL(k_step_end, dim3((np + 63) / 64), dim3(64), 0, s, est.as<double>(), conv.as<int>(), dia,
+                         hist.as<double>(), kdone.as<int>(), k, m, np);
       VP_HIP(hipGetLastError());
-      wait_published(hseq, seq, s, hest, est.as<double>(), hconv, conv.as<int>(), np);
-      bool still = false, changed = false;
+    };
+    for (int k0 = 0; k0 < m;) {
+      const int k1 = std::min(m, k0 + chunk);
+      for (int k = k0; k < k1; ++k) step(k);
+      ++seq;
+      hipLaunchKernelGGL(k_publish, dim3(1), dim3(256), 0, s, hist.as<double>(), kdone.as<int>(), dia, np, m, k0, k1,
+                         dhhist, dhkdone, dhiact, dhseq, seq);
+      VP_HIP(hipGetLastError());
+      wait_published(hseq, seq, s, [&] {
+        VP_HIP(hipMemcpy(hhist, hist.p, size_t(np) * m * sizeof(double), hipMemcpyDeviceToHost));
+        VP_HIP(hipMemcpy(hkdone, kdone.p, np * sizeof(int), hipMemcpyDeviceToHost));
+        VP_HIP(hipMemcpy(hiact, dia, np * sizeof(int), hipMemcpyDeviceToHost));
+      });
+      bool still = false;
       for (int p = 0; p < np; ++p) {
         if (!inner[p]) continue;
-        hk[p] = k + 1;
-        lastk[p] = k + 1;
-        res.total[p] += 1;
-        res.iter_inner[p] = k + 1;
-        last[p] = relres_of(hest[p], hbn[p]);
-        res.resvec[p].push_back(last[p]);
-        if (hconv[p] || k == m - 1) {
-          inner[p] = 0;
-          changed = true;
-        } else {
-          still = true;
+        const int kd = hkdone[p];
+        for (int j = k0; j < kd; ++j) {
+          res.total[p] += 1;
+          last[p] = relres_of(hhist[size_t(p) * m + j], hbn[p]);
+          res.resvec[p].push_back(last[p]);
         }
+        hk[p] = kd;
+        lastk[p] = kd;
+        res.iter_inner[p] = kd;
+        if (hiact[p]) still = true;
+        else inner[p] = 0;
       }
       if (!still) break;
-      // the device copy of the active flags changes only when a problem stops
-      if (changed) VP_HIP(hipMemcpyAsync(iact.p, inner.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+      k0 = k1;
     }
     // solve the least squares and update x for every problem active in this cycle
     VP_HIP(hipMemcpyAsync(kk.p, hk.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
