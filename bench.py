@@ -74,7 +74,7 @@ WORKLOADS = {
              'U{20..150} periods, 20% microdisks of radius U[2,5] um (220 nm thick); requests admitted by the '
              'server HBM budget, refinement rounds batched per geometry',
         tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=4, max_outer=20, nreq=40, rate=50.0, seed=4,
-        budget_gb=96, max_active=32, setup_workers=4),
+        budget_gb=96, max_active=32, setup_workers=8, fft_sizes='min'),
     'microdisk_15nm': dict(
         desc='Si microdisk r = 10 um, 220 nm thick, with a 500 nm bus waveguide at a 200 nm gap, SiO2, '
              'lambda 1550 nm, 15 nm voxels, grid 1400x1500x30 (63 M voxels), y-polarised Gaussian (w0 300 nm) '
@@ -774,6 +774,10 @@ def run_serve(args, wl, log):
     import threading
     from voxphoton_amd import vie, configs
     reqs = serve_requests(wl)
+    if wl.get('fft_sizes'):
+        # every request has its own box: the shortest FFT sizes instead of
+        # timing candidates on a device that is busy with other solves
+        os.environ['VP_FFT_SIZES'] = wl['fft_sizes']
     ctx = vie.default_context()
     srv = vie.Server(int(wl['budget_gb'] * (1 << 30)), wl['max_active'], ctx=ctx)
     live = {}

@@ -5,7 +5,8 @@
 // alone.  Admitted requests are grouped by operator geometry and solve
 // parameters (a group key).  A few executor threads, each with its own HIP
 // stream (VP_SERVER_STREAMS, default 3), take turns on the keys: an executor
-// takes the key that has waited longest and is not being run by another,
+// takes the key of the smallest requests, aged by the rounds it has waited
+// (or the key that has waited longest), that no other executor is running,
 // admits what the queue allows, and runs one refinement step (ir_step_set:
 // FP64 residual, inner FP32 GMRES, update) for every active request of that
 // key as one batched launch sequence.  Groups of different keys therefore
@@ -20,6 +21,7 @@
 #include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
@@ -50,6 +52,7 @@ struct ServeReq {
   int error = 0;
   std::string errmsg;
   Clock::time_point t_submit, t_done;
+  int64_t admitted_round = 0;
 };
 
 // Solver-state bytes of one request when it runs in a batch: Krylov basis
@@ -101,6 +104,7 @@ struct vp_server_s {
   std::map<std::string, int64_t> last;  // key -> step number of its last start
   int64_t used = 0, next_ticket = 1, rounds = 0;
   bool stop = false;
+  bool small_first = true;              // false: VP_SERVER_POLICY=rr
   std::vector<std::thread> workers;
 };
 
@@ -151,22 +155,31 @@ void admit(vp_server_s* S) {
     ServeReq* r = S->pending.front();
     if (!S->active.empty() && S->used + r->bytes > S->budget) break;
     S->pending.pop_front();
+    r->admitted_round = S->rounds;
     S->used += r->bytes;
     S->active.push_back(r);
   }
 }
 
-// The runnable key that started longest ago (S->mu held); "" if none.
+// The next key to run (S->mu held); "" if none runnable.  Default: the key
+// with the fewest unknowns per request, its size halved for every 16 rounds
+// it has waited (shortest work first, with ageing so that no key starves).
+// VP_SERVER_POLICY=rr: the key that started longest ago (round robin).
 std::string pick_key(vp_server_s* S) {
   std::string best;
-  int64_t best_last = 0;
+  double best_score = 0.0;
   for (ServeReq* r : S->active) {
     if (r->st.done || S->busy.count(r->key)) continue;
     auto it = S->last.find(r->key);
     const int64_t l = it == S->last.end() ? -1 : it->second;
-    if (best.empty() || l < best_last) {
+    double score = double(l);
+    if (S->small_first) {
+      const double waited = double(S->rounds - std::max<int64_t>(l, r->admitted_round));
+      score = double(r->op->n) * std::exp2(-waited / 16.0);
+    }
+    if (best.empty() || score < best_score) {
       best = r->key;
-      best_last = l;
+      best_score = score;
     }
   }
   return best;
@@ -271,6 +284,7 @@ int vp_server_create(vp_ctx ctx, int64_t hbm_budget_bytes, int max_active, vp_se
   S->max_active = max_active;
   int ns = 3;
   if (const char* e = std::getenv("VP_SERVER_STREAMS")) ns = std::max(1, std::min(8, std::atoi(e)));
+  if (const char* e = std::getenv("VP_SERVER_POLICY")) S->small_first = std::strcmp(e, "rr") != 0;
   for (int k = 0; k < ns; ++k) {
     hipStream_t st = nullptr;
     const hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
