@@ -977,15 +977,18 @@ def make_preconditioner(prob, wl, vie, configs, ctx=None):
         parts = []
         for box in prob.extra['parts']:
             shape, bidx, sub = prob.box_problem(box)
-            q = vie.Preconditioner.level2_assembled(shape, prob.dx, prob.ko, prob.core_contrast(), 1.0, bidx,
+            # 'field': the box's own contrast array folded into the Toeplitz
+            # tensor, as resonator_MOR_scripts.m:165-170 builds the disk square
+            mc = prob.Mc[box] if wl.get('contrast') == 'field' else prob.core_contrast()
+            q = vie.Preconditioner.level2_assembled(shape, prob.dx, prob.ko, mc, 1.0, bidx,
                                                     damping=wl['damping'], ctx=ctx)
             q.set_embedding(n, sub)
             q.box_shape = tuple(int(v) for v in shape)
             parts.append(q)
         P = vie.Preconditioner.composite(parts, ctx=ctx)
         P.box_shape = parts[-1].box_shape
-        desc = ('block-Jacobi composite of 2-level circulants (core contrast) on the boxes '
-                f'{[q.box_shape for q in parts]}, damping {wl["damping"]}')
+        desc = (f'block-Jacobi composite of 2-level circulants ({wl.get("contrast", "core")} contrast) on the '
+                f'boxes {[q.box_shape for q in parts]}, damping {wl["damping"]}')
         return P, desc
     if wl['prec'] == 'face':
         shape, pidx, sub, face = configs.face_box(prob, wl['axis'])

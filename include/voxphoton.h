@@ -187,6 +187,19 @@ int vp_prec_create_level2_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N,
                                     double mc_im, double mr0, const int64_t* idx, int64_t n, int precisions,
                                     vp_prec* out);
 /*
+ * 2-level setup with a contrast field, as level_2_parallel_func.m:5-75 takes
+ * an L x M x N array Mc (resonator_MOR_scripts.m:165-170 passes the disk
+ * square's (e_r - 1)/e_r): the Toeplitz entry at offset (i, j, k) is scaled by
+ * Mc(i, j, k) before the blocks are formed.  Mc is interleaved complex128 in
+ * column-major order; Mr(1) = mr0.  The _assembled form assembles G on the
+ * device (damping as for vp_prec_create_level2_assembled).
+ */
+int vp_prec_create_level2_field(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double* G, const double* Mc,
+                                double mr0, double gram, const int64_t* idx, int64_t n, int precisions, vp_prec* out);
+int vp_prec_create_level2_assembled_field(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko,
+                                          double damping, const double* Mc, double mr0, const int64_t* idx, int64_t n,
+                                          int precisions, vp_prec* out);
+/*
  * Face-averaged 2-level preconditioner (the form Bragg_MOR_script.m:61-72
  * builds with level_1_parallel_func_N: inv(Gram*I - diag(face3)*circ_N)),
  * with the circulant approximation taken along the first two axes of the box

@@ -71,6 +71,35 @@ def test_level2_device_setup_matches_oracle(vp_ctx, monkeypatch, shape, sym, mod
     assert rel(y, ref) < 1e-10
 
 
+@pytest.mark.parametrize('shape', [(6, 4, 3), (9, 8, 4), (12, 10, 2)])
+def test_level2_contrast_field_matches_oracle(vp_ctx, shape):
+    # level_2_parallel_func with an L x M x N contrast array (the resonator
+    # script's disk square): Toeplitz entries scaled by Mc at their offsets
+    from voxphoton_amd.vie import Preconditioner
+    L, M, N = shape
+    dx, ko = 20e-9, 5.85e6
+    G = ogr.assemble_G(L, M, N, dx, ko)
+    rng = np.random.default_rng(L + M)
+    Mc = np.where(rng.random(shape) < 0.6, 0.83 - 0.05j, 0.0) + 0.02 * rng.standard_normal(shape)
+    mask = rng.random(shape) < 0.8
+    mask.flat[0] = True
+    idx = ovie.idx_from_mask(mask)
+    Binv = opre.level_2_parallel_func(G, Mc, 1.0, L, M, N, dx ** 3)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
+    P = Preconditioner.level2(G, Mc, 1.0, dx ** 3, idx, ctx=vp_ctx)
+    e64 = rel(P.apply(x), ref)
+    Pa = Preconditioner.level2_assembled(shape, dx, ko, Mc, 1.0, idx, ctx=vp_ctx)
+    ea = rel(Pa.apply(x), ref)
+    e32 = rel(Pa.apply(x, 'fp32'), ref)
+    print(shape, 'contrast field: host G', e64, 'assembled', ea, 'fp32', e32, flush=True)
+    assert e64 < 1e-10 and ea < 1e-10 and e32 < 2e-3
+    # a uniform field is the scalar form
+    Pu = Preconditioner.level2(G, np.full(shape, 0.83 - 0.05j), 1.0, dx ** 3, idx, ctx=vp_ctx)
+    Ps = Preconditioner.level2(G, 0.83 - 0.05j, 1.0, dx ** 3, idx, ctx=vp_ctx)
+    assert rel(Pu.apply(x), Ps.apply(x)) < 1e-13
+
+
 @pytest.mark.parametrize('half', ['1', '0', 'dot2'])
 def test_level2_blocks_wider_than_a_wavefront(vp_ctx, monkeypatch, half):
     # 3N = 66 > 64: block inversion spans several wavefronts.  The FP32 apply

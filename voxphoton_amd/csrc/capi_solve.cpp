@@ -544,6 +544,25 @@ static int prec_level2_common(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const
   return VP_OK;
 }
 
+// G_k(d) *= Mc(d) over the L x M x N offsets of each of the 6 components: the
+// contrast field folded into the Toeplitz tensor, level_2_parallel_func.m:5,
+// 24, 36, 47, 61, 74 (-Mc.*opToeplitz(:,:,:,k) with an array Mc).
+__global__ void k_fold_contrast(double2* __restrict__ G, const double2* __restrict__ mc, int64_t V) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < 6 * V; e += int64_t(gridDim.x) * blockDim.x) {
+    const double2 a = G[e], b = mc[e % V];
+    G[e] = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  }
+}
+
+static void fold_contrast(double2* dG, const double* Mc, int64_t V, hipStream_t s) {
+  DevBuf dm(size_t(V) * sizeof(double2));
+  VP_HIP(hipMemcpyAsync(dm.p, Mc, dm.bytes, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_fold_contrast, dim3(unsigned(std::min<int64_t>((6 * V + 255) / 256, 8192))), dim3(256), 0, s,
+                     dG, dm.as<double2>(), V);
+  VP_HIP(hipGetLastError());
+  VP_HIP(hipStreamSynchronize(s));
+}
+
 int vp_prec_create_level2(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double* G, double mc_re, double mc_im,
                           double mr0, double gram, const int64_t* idx, int64_t n, int precisions, vp_prec* out) {
   VP_API_BEGIN
@@ -565,6 +584,35 @@ int vp_prec_create_level2_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N,
   DevBuf dG(size_t(L * M * N) * 6 * sizeof(double2));
   assemble_toeplitz_dev(int(L), int(M), int(N), dx, ko, dG.as<double2>(), ctx->stream, -damping * ko);
   prec_level2_common(ctx, L, M, N, dG.as<double2>(), mc_re, mc_im, mr0, dx * dx * dx, idx, n, precisions, out);
+  VP_API_END
+}
+
+int vp_prec_create_level2_field(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double* G, const double* Mc,
+                                double mr0, double gram, const int64_t* idx, int64_t n, int precisions, vp_prec* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && out && G && Mc, ERR_INVALID, "ctx/out/G/Mc is NULL");
+  VP_REQUIRE(L > 0 && M > 0 && N > 0, ERR_INVALID, "bad box");
+  VP_HIP(hipSetDevice(ctx->device));
+  const int64_t V = L * M * N;
+  DevBuf dG(size_t(V) * 6 * sizeof(double2));
+  VP_HIP(hipMemcpy(dG.p, G, dG.bytes, hipMemcpyHostToDevice));
+  fold_contrast(dG.as<double2>(), Mc, V, ctx->stream);
+  prec_level2_common(ctx, L, M, N, dG.as<double2>(), 1.0, 0.0, mr0, gram, idx, n, precisions, out);
+  VP_API_END
+}
+
+int vp_prec_create_level2_assembled_field(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko,
+                                          double damping, const double* Mc, double mr0, const int64_t* idx, int64_t n,
+                                          int precisions, vp_prec* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && out && Mc && dx > 0 && damping >= 0, ERR_INVALID, "bad arguments");
+  VP_REQUIRE(L > 0 && M > 0 && N > 0, ERR_INVALID, "bad box");
+  VP_HIP(hipSetDevice(ctx->device));
+  const int64_t V = L * M * N;
+  DevBuf dG(size_t(V) * 6 * sizeof(double2));
+  assemble_toeplitz_dev(int(L), int(M), int(N), dx, ko, dG.as<double2>(), ctx->stream, -damping * ko);
+  fold_contrast(dG.as<double2>(), Mc, V, ctx->stream);
+  prec_level2_common(ctx, L, M, N, dG.as<double2>(), 1.0, 0.0, mr0, dx * dx * dx, idx, n, precisions, out);
   VP_API_END
 }
 

@@ -240,13 +240,19 @@ class Preconditioner:
 
     @classmethod
     def level2(cls, G, Mc, Mr0, Gram, idx, precisions=('fp64', 'fp32'), ctx=None):
-        """level_2_parallel_func(opToeplitz, Mc, Mr, L, M, N, Gram) with scalar Mc, built on the GPU."""
+        """level_2_parallel_func(opToeplitz, Mc, Mr, L, M, N, Gram), built on the GPU; Mc a scalar or an
+        L x M x N array (vp_prec_create_level2 / vp_prec_create_level2_field)."""
         ctx = ctx or default_context()
         G = np.asarray(G)
         Lg, Mg, Ng = G.shape[:3]
         g = as_c128(G.reshape(-1, order='F'))
         ix = as_idx(idx)
         h = ctypes.c_void_p()
+        if np.ndim(Mc):
+            mcf = _contrast_field(Mc, (Lg, Mg, Ng))
+            check(L.lib.vp_prec_create_level2_field(ctx.handle, Lg, Mg, Ng, ptr(g), ptr(mcf), float(Mr0), float(Gram),
+                                                    ptr(ix), ix.size, _prec_mask(precisions), ctypes.byref(h)))
+            return cls(h, ctx, ix.size)
         mc = complex(Mc)
         check(L.lib.vp_prec_create_level2(ctx.handle, Lg, Mg, Ng, ptr(g), mc.real, mc.imag, float(Mr0),
                                           float(Gram), ptr(ix), ix.size, _prec_mask(precisions),
@@ -260,6 +266,12 @@ class Preconditioner:
         Lg, Mg, Ng = shape
         ix = as_idx(idx)
         h = ctypes.c_void_p()
+        if np.ndim(Mc):
+            mcf = _contrast_field(Mc, (Lg, Mg, Ng))
+            check(L.lib.vp_prec_create_level2_assembled_field(ctx.handle, Lg, Mg, Ng, float(dx), float(ko),
+                                                              float(damping), ptr(mcf), float(Mr0), ptr(ix), ix.size,
+                                                              _prec_mask(precisions), ctypes.byref(h)))
+            return cls(h, ctx, ix.size)
         mc = complex(Mc)
         check(L.lib.vp_prec_create_level2_assembled(ctx.handle, Lg, Mg, Ng, float(dx), float(ko), float(damping), mc.real,
                                                     mc.imag, float(Mr0), ptr(ix), ix.size,
@@ -312,6 +324,14 @@ class Preconditioner:
         return y
 
     __call__ = apply
+
+
+def _contrast_field(Mc, shape):
+    """An L x M x N contrast array as interleaved complex128, column-major."""
+    Mc = np.asarray(Mc)
+    if Mc.shape != tuple(shape):
+        raise ValueError(f'contrast field of shape {Mc.shape}, expected {tuple(shape)}')
+    return as_c128(Mc.reshape(-1, order='F'))
 
 
 def _info(cap):
