@@ -193,8 +193,11 @@ __global__ void __launch_bounds__((acc_bound<T, EPT>())) k_plane(const ProbPtrs<
 // kx plane (zero pad N -> Np, DFT, tensor multiply, inverse, crop), and the
 // inverse y sweep crops back to M into X.
 // ----------------------------------------------------------------------------
+// Generic y pass (runtime radix plan): By pencils per workgroup in two LDS
+// buffers; the twiddles are read from the global table (cached), so a long
+// line (Mp in the thousands) takes only its two buffers of LDS.
 template <typename T, bool INV>
-__global__ void __launch_bounds__(256) k_ypass(const ProbPtrs<T>* __restrict__ probs, const int* __restrict__ active,
+__global__ void __launch_bounds__(512) k_ypass(const ProbPtrs<T>* __restrict__ probs, const int* __restrict__ active,
                                                Geometry g, MatvecPlan mp, const C<T>* __restrict__ twy) {
   if (active && !active[blockIdx.y]) return;
   const ProbPtrs<T> P = probs[blockIdx.y];
@@ -202,8 +205,7 @@ __global__ void __launch_bounds__(256) k_ypass(const ProbPtrs<T>* __restrict__ p
   C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
   const int M = g.M, Mp = g.y.P, B = mp.By, ld = mp.ldy;
   C<T>* W1 = W0 + B * ld;
-  C<T>* tw = W1 + B * ld;
-  for (int e = threadIdx.x; e < Mp; e += blockDim.x) tw[e] = twy[e];
+  const C<T>* tw = twy;
   const int64_t npen = int64_t(g.x.P) * 3 * g.N;
   const int64_t p0 = int64_t(xcd_remap(blockIdx.x, gridDim.x)) * B;
   const int nb = int(min<int64_t>(B, npen - p0));

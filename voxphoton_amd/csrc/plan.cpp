@@ -286,7 +286,7 @@ bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out,
     out.By = 0;
     for (int B : {16, 8, 4, 2, 1}) {
       const int ld = Mp + (B > 1 ? 1 : 0);
-      const size_t lds = (2 * size_t(B) * ld + Mp) * s;
+      const size_t lds = 2 * size_t(B) * ld * s;
       if (lds <= budget || B == 1) { out.By = B; out.ldy = ld; out.lds_y = lds; break; }
     }
     out.Tz = 0;
@@ -298,7 +298,8 @@ bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out,
       if (lds <= budget || Tz == 1) { out.Tz = Tz; out.lds_z = lds; break; }
     }
     if (out.lds_y > size_t(lds_max) || out.lds_z > size_t(lds_max)) { why = "split passes exceed LDS"; return false; }
-    out.ty = 256;
+    // long lines (one per workgroup): more threads per stage
+    out.ty = (out.By == 1 && Mp >= 1024) ? 512 : 256;
     out.tz = 256;
     // z transforms of the split pass: forward with the odd radices first
     // (pruned zero-padded input), inverse with them last (pruned outputs)
