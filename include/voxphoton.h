@@ -187,6 +187,30 @@ int vp_prec_create_level2_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N,
                                     double mc_im, double mr0, const int64_t* idx, int64_t n, int precisions,
                                     vp_prec* out);
 /*
+ * 1-level setup on the device (level_1_parallel_func.m:1-169): for every kx
+ * the 3MN x 3MN block of the x-Chan circulant of -Mc.*opToeplitz (Gram*Mr(1)
+ * at the origin), exact in y and z, inverted (Gauss-Jordan on the device up
+ * to 256 rows, rocSOLVER's batched LU above).  Mc is the scalar (mc_re,
+ * mc_im), or, when the Mc pointer is not NULL, an L x M x N contrast field
+ * (interleaved complex128, column-major) scaling the tensor at each offset.
+ * The blocks are applied as vp_prec_create_chan1's (chan_mvp_idx.m).  All L
+ * blocks of (3MN)^2 complex128 are kept: sized for boxes with 3MN in the
+ * hundreds.  The _assembled form assembles G on the device (damping as for
+ * vp_prec_create_level2_assembled, gram = dx^3).
+ */
+int vp_prec_create_level1(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double* G, double mc_re, double mc_im,
+                          const double* Mc, double mr0, double gram, const int64_t* idx, int64_t n, int precisions,
+                          vp_prec* out);
+int vp_prec_create_level1_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko,
+                                    double damping, double mc_re, double mc_im, const double* Mc, double mr0,
+                                    const int64_t* idx, int64_t n, int precisions, vp_prec* out);
+/* Face-averaged 1-level form of the reference's Bragg script
+ * (level_1_parallel_func_N.m:1-192 and Bragg_MOR_script.m:61-72): blocks
+ * inv(Gram*I - diag(face3) * circ_N{kx}) of the bare assembled tensor, with
+ * face[m + M*n] (complex128, M x N) and face3 = [face; face; face]. */
+int vp_prec_create_level1_face(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko, double damping,
+                               const double* face, const int64_t* idx, int64_t n, int precisions, vp_prec* out);
+/*
  * 2-level setup with a contrast field, as level_2_parallel_func.m:5-75 takes
  * an L x M x N array Mc (resonator_MOR_scripts.m:165-170 passes the disk
  * square's (e_r - 1)/e_r): the Toeplitz entry at offset (i, j, k) is scaled by

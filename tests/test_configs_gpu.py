@@ -3,8 +3,8 @@ against the FP64 oracle.
 
 * straight waveguide 200x48x24 at 20 nm: FP64 / FP32 matvec of the seeded
   N(0,1) + iN(0,1) vector against the oracle's mv_AN_const (1e-12 / 1e-5),
-  and a plain FP64 GMRES(50) solve with the 2-level preconditioner to 1e-6
-  whose true residual the oracle recomputes;
+  and plain FP64 GMRES(50) solves with the 2-level and with the 1-level
+  preconditioner to 1e-6 whose true residual the oracle recomputes;
 * Bragg grating 3600x100x44 at 10 nm: the same matvec checks, the
   mixed-precision solve's true residual by the oracle, and its difference
   from the plain FP64 solve;
@@ -116,6 +116,29 @@ def test_straight_wg_fp64_gmres50_true_residual(wg20, vp_ctx):
     assert flag == 0
     assert true <= 1e-6 * (1 + 1e-6)
     assert abs(true - relres) <= 1e-9 * max(true, 1e-300) + 1e-15
+
+
+@pytest.mark.timeout(600)
+def test_straight_wg_fp64_gmres50_level1_true_residual(wg20, vp_ctx):
+    # the reference's 1-level form (level_1_parallel_func, blocks of
+    # 3MN = 825 on the core box, inverted by rocSOLVER) as the right
+    # preconditioner of FP64 GMRES(50)
+    from voxphoton_amd import vie
+    prob, op, oracle = wg20
+    shape, bidx, sub = prob.box_problem()
+    t = time.perf_counter()
+    P = vie.Preconditioner.level1_assembled(shape, prob.dx, prob.ko, prob.core_contrast(), 1.0, bidx,
+                                            ctx=vp_ctx).set_embedding(prob.n, sub)
+    t_setup = time.perf_counter() - t
+    b = prob.gaussian_rhs('y')
+    t = time.perf_counter()
+    x, flag, relres, it, rv = vie.pgmres(op, b, 50, 1e-6, 40, R1=P)
+    dt = time.perf_counter() - t
+    true = oracle.true_relres(x, b)
+    _say(f'straight WG FP64 GMRES(50), 1-level on box {shape} (setup {t_setup:.2f} s): flag {flag}, '
+         f'{len(rv) - 1} iterations, {dt:.2f} s, relres {relres:.3e}, oracle relres {true:.3e}')
+    assert flag == 0
+    assert true <= 1e-6 * (1 + 1e-6)
 
 
 # ---------------------------------------------------------------------------

@@ -71,6 +71,52 @@ def test_level2_device_setup_matches_oracle(vp_ctx, monkeypatch, shape, sym, mod
     assert rel(y, ref) < 1e-10
 
 
+@pytest.mark.parametrize('shape', [(6, 4, 3), (7, 5, 4), (5, 10, 9)])
+def test_level1_setup_matches_oracle(vp_ctx, shape):
+    # level_1_parallel_func built and inverted on the device (3MN = 36, 60:
+    # Gauss-Jordan kernels; 270: rocSOLVER's batched LU), scalar contrast and
+    # contrast field, applied as chan_mvp_idx
+    from voxphoton_amd.vie import Preconditioner
+    L, M, N = shape
+    dx, ko = 20e-9, 5.85e6
+    G = ogr.assemble_G(L, M, N, dx, ko)
+    rng = np.random.default_rng(L * M * N)
+    mask = rng.random(shape) < 0.8
+    mask.flat[0] = True
+    idx = ovie.idx_from_mask(mask)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    mc = 0.8278 - 0.02j
+    ref = opre.chan_mvp_idx(opre.level_1_parallel_func(G, mc, 1.0, L, M, N, dx ** 3), x, L, M, N, idx)
+    P = Preconditioner.level1(G, mc, 1.0, dx ** 3, idx, ctx=vp_ctx)
+    e64, e32 = rel(P.apply(x), ref), rel(P.apply(x, 'fp32'), ref)
+    Pa = Preconditioner.level1_assembled(shape, dx, ko, mc, 1.0, idx, ctx=vp_ctx)
+    ea = rel(Pa.apply(x), ref)
+    Mc = np.where(rng.random(shape) < 0.6, 0.83, 0.0) + 0.01 * rng.standard_normal(shape)
+    reff = opre.chan_mvp_idx(opre.level_1_parallel_func(G, Mc, 1.0, L, M, N, dx ** 3), x, L, M, N, idx)
+    ef = rel(Preconditioner.level1(G, Mc, 1.0, dx ** 3, idx, ctx=vp_ctx).apply(x), reff)
+    print(shape, '3MN', 3 * M * N, 'level1', e64, e32, 'assembled', ea, 'field', ef, flush=True)
+    assert e64 < 1e-10 and ea < 1e-10 and ef < 1e-10 and e32 < 1e-4
+
+
+def test_level1_face_matches_oracle(vp_ctx):
+    # the face-averaged 1-level form of Bragg_MOR_script.m:61-72
+    from voxphoton_amd.vie import Preconditioner
+    L, M, N = 8, 5, 4
+    dx, ko = 20e-9, 5.85e6
+    G = ogr.assemble_G(L, M, N, dx, ko)
+    rng = np.random.default_rng(12)
+    Mr_mid = 1.0 + 4.8 * (rng.random((L, M, N)) < 0.5) + 0.05j * rng.random((L, M, N))
+    hey = np.real(np.sum(Mr_mid, axis=0) / L)
+    face = (hey - 1.0) / hey
+    idx = ovie.idx_from_mask(np.ones((L, M, N), bool))
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    ref = opre.chan_mvp_idx(opre.level_1_parallel_func_N_face(G, Mr_mid, dx ** 3), x, L, M, N, idx)
+    P = Preconditioner.level1_face((L, M, N), dx, ko, face, idx, ctx=vp_ctx)
+    e64 = rel(P.apply(x), ref)
+    print('level1 face', e64, flush=True)
+    assert e64 < 1e-10
+
+
 @pytest.mark.parametrize('shape', [(6, 4, 3), (9, 8, 4), (12, 10, 2)])
 def test_level2_contrast_field_matches_oracle(vp_ctx, shape):
     # level_2_parallel_func with an L x M x N contrast array (the resonator

@@ -83,6 +83,12 @@ SIGNATURES = {
                                       c_int, P(c_vp)]),
     'vp_prec_create_level2_assembled': (c_int, [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl,
                                                 c_vp, c_i64, c_int, P(c_vp)]),
+    'vp_prec_create_level1': (c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_dbl, c_dbl, c_vp, c_dbl, c_dbl, c_vp, c_i64,
+                                      c_int, c_vp]),
+    'vp_prec_create_level1_assembled': (c_int, [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_vp,
+                                                c_dbl, c_vp, c_i64, c_int, c_vp]),
+    'vp_prec_create_level1_face': (c_int, [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_vp, c_vp, c_i64, c_int,
+                                           c_vp]),
     'vp_prec_create_level2_field': (c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_dbl, c_dbl, c_vp, c_i64, c_int,
                                             c_vp]),
     'vp_prec_create_level2_assembled_field': (c_int, [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_vp, c_dbl,

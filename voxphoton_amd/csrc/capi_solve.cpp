@@ -616,6 +616,88 @@ int vp_prec_create_level2_assembled_field(vp_ctx ctx, int64_t L, int64_t M, int6
   VP_API_END
 }
 
+// 1-level setup (level_1_parallel_func.m): blocks of the folded tensor
+// (-mc G, Gram*Mr(1) at the origin) or, face != nullptr, Gram*I -
+// diag(face3) circ_N of the bare tensor (level_1_parallel_func_N.m with
+// Bragg_MOR_script.m:61-72); all L blocks, inverted on the device.
+static int prec_level1_common(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double2* dG, double2 mc,
+                              double diag, const double* face, double gram, const int64_t* idx, int64_t n,
+                              int precisions, vp_prec* out) {
+  VP_REQUIRE(L > 0 && M > 0 && N > 0, ERR_INVALID, "bad box");
+  const int64_t n3 = 3 * M * N;
+  VP_REQUIRE(n3 <= 16384, ERR_UNSUPPORTED, "1-level blocks wider than 16384 rows");
+  vp_prec_s* p = prec_begin(ctx, L, M, N, idx, n, 1);
+  try {
+    p->p64.blocks.alloc(size_t(L) * n3 * n3 * sizeof(double2));
+    DevBuf dface;
+    if (face) {
+      dface.alloc(size_t(M * N) * sizeof(double2));
+      VP_HIP(hipMemcpy(dface.p, face, dface.bytes, hipMemcpyHostToDevice));
+    }
+    build_level1_blocks(dG, int(L), int(M), int(N), mc, diag, p->p64.blocks.as<double2>(), ctx->stream,
+                        face ? dface.as<double2>() : nullptr, gram);
+    invert_blocks_any(p->p64.blocks.as<double2>(), L, int(n3), ctx->stream);
+    prec_finish(p, precisions);
+  } catch (...) {
+    delete p;
+    throw;
+  }
+  *out = p;
+  return VP_OK;
+}
+
+int vp_prec_create_level1(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double* G, double mc_re, double mc_im,
+                          const double* Mc, double mr0, double gram, const int64_t* idx, int64_t n, int precisions,
+                          vp_prec* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && out && G, ERR_INVALID, "ctx/out/G is NULL");
+  VP_REQUIRE(L > 0 && M > 0 && N > 0, ERR_INVALID, "bad box");
+  VP_HIP(hipSetDevice(ctx->device));
+  const int64_t V = L * M * N;
+  DevBuf dG(size_t(V) * 6 * sizeof(double2));
+  VP_HIP(hipMemcpy(dG.p, G, dG.bytes, hipMemcpyHostToDevice));
+  double2 mc = make_double2(mc_re, mc_im);
+  if (Mc) {
+    fold_contrast(dG.as<double2>(), Mc, V, ctx->stream);
+    mc = make_double2(1.0, 0.0);
+  }
+  prec_level1_common(ctx, L, M, N, dG.as<double2>(), mc, gram * mr0, nullptr, 0.0, idx, n, precisions, out);
+  VP_API_END
+}
+
+int vp_prec_create_level1_assembled(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko,
+                                    double damping, double mc_re, double mc_im, const double* Mc, double mr0,
+                                    const int64_t* idx, int64_t n, int precisions, vp_prec* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && out && dx > 0 && damping >= 0, ERR_INVALID, "bad arguments");
+  VP_REQUIRE(L > 0 && M > 0 && N > 0, ERR_INVALID, "bad box");
+  VP_HIP(hipSetDevice(ctx->device));
+  const int64_t V = L * M * N;
+  DevBuf dG(size_t(V) * 6 * sizeof(double2));
+  assemble_toeplitz_dev(int(L), int(M), int(N), dx, ko, dG.as<double2>(), ctx->stream, -damping * ko);
+  double2 mc = make_double2(mc_re, mc_im);
+  if (Mc) {
+    fold_contrast(dG.as<double2>(), Mc, V, ctx->stream);
+    mc = make_double2(1.0, 0.0);
+  }
+  prec_level1_common(ctx, L, M, N, dG.as<double2>(), mc, dx * dx * dx * mr0, nullptr, 0.0, idx, n, precisions,
+                     out);
+  VP_API_END
+}
+
+int vp_prec_create_level1_face(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko, double damping,
+                               const double* face, const int64_t* idx, int64_t n, int precisions, vp_prec* out) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx && out && face && dx > 0 && damping >= 0, ERR_INVALID, "bad arguments");
+  VP_REQUIRE(L > 0 && M > 0 && N > 0, ERR_INVALID, "bad box");
+  VP_HIP(hipSetDevice(ctx->device));
+  DevBuf dG(size_t(L * M * N) * 6 * sizeof(double2));
+  assemble_toeplitz_dev(int(L), int(M), int(N), dx, ko, dG.as<double2>(), ctx->stream, -damping * ko);
+  prec_level1_common(ctx, L, M, N, dG.as<double2>(), make_double2(-1.0, 0.0), 0.0, face, dx * dx * dx, idx, n,
+                     precisions, out);
+  VP_API_END
+}
+
 int vp_prec_create_level2_face(vp_ctx ctx, int64_t L, int64_t M, int64_t N, double dx, double ko, double damping,
                                const double* face, const int64_t* idx, int64_t n, int precisions, vp_prec* out) {
   VP_API_BEGIN

@@ -11,10 +11,15 @@
 // block directly from the Toeplitz tensor: Chan circulant + DFT along x,
 // signed y-Toeplitz first row/column per (c, n_t, d, n_s), Chan circulant +
 // DFT along y, then batched Gauss-Jordan inversion in FP64.
+#include <dlfcn.h>
+#include <rocsolver/rocsolver.h>
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <type_traits>
 #include <vector>
 #include "precond.hpp"
@@ -1195,6 +1200,73 @@ __global__ void __launch_bounds__(256) k_pc_invert(double2* __restrict__ blk, in
   }
 }
 
+// 1-level blocks (level_1_parallel_func.m:5-169, level_1_parallel_func_N.m):
+// for each kx the 3MN x 3MN matrix whose (component c, z index nt, y index m)
+// x (d, ns, m') entry is the y-Toeplitz entry of component k = comp(c, d) at
+// z offset |nt - ns|: the first column for m >= m' (t = m - m'), the first
+// row for m < m' (t = m' - m), with the sign conventions of the 2-level blocks
+// (pc_col, pc_row), v(t) = ck[k][|nt - ns|][t][kx].  Column-major per block.
+// face == nullptr: the blocks of the folded tensor (mc, diag in ck);
+// face = device complex[M*N] (index m + M n): Gram*I - diag(face3) * blocks
+// of the bare tensor.
+__global__ void k_pc_l1_blocks(const double2* __restrict__ ck, double2* __restrict__ blk, int L, int M, int N,
+                               const double2* __restrict__ face, double gram) {
+  const int MN = M * N, n3 = 3 * MN;
+  const int64_t per = int64_t(n3) * n3;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < int64_t(L) * per;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int kx = int(e / per);
+    const int64_t rc = e - int64_t(kx) * per;
+    const int col = int(rc / n3), row = int(rc - int64_t(col) * n3);
+    const int c = row / MN, rr = row - c * MN, nt = rr / M, m = rr - nt * M;
+    const int d = col / MN, cr = col - d * MN, ns = cr / M, mp = cr - ns * M;
+    const int k = c_pc_comp[c][d];
+    const int dz = nt > ns ? nt - ns : ns - nt;
+    const int t = m >= mp ? m - mp : mp - m;
+    const double2 v = ck[((int64_t(k) * N + dz) * M + t) * L + kx];
+    double2 a = m >= mp ? pc_col(k, nt, ns, t, v) : pc_row(k, nt, ns, t, v);
+    if (face) {
+      const double2 f = face[rr];
+      a = make_double2(-(f.x * a.x - f.y * a.y), -(f.x * a.y + f.y * a.x));
+      if (row == col) a.x += gram;
+    }
+    blk[e] = a;
+  }
+}
+
+void build_level1_blocks(const double2* G, int L, int M, int N, double2 mc, double diag, double2* blk,
+                         hipStream_t s, const double2* face_dev, double gram) {
+  std::vector<double2> tL(L);
+  for (int k = 0; k < L; ++k) {
+    SinCos v = sincos_turn(k, L);
+    tL[k] = make_double2(v.c, -v.s);
+  }
+  DevBuf dL(tL.size() * sizeof(double2));
+  VP_HIP(hipMemcpyAsync(dL.p, tL.data(), dL.bytes, hipMemcpyHostToDevice, s));
+  DevBuf ck(size_t(6) * L * M * N * sizeof(double2));
+  const size_t lds2 = size_t(2) * L * sizeof(double2);
+  if (lds2 <= 160 * 1024) {
+    int L1 = 1;
+    for (int d = 1; int64_t(d) * d <= L; ++d)
+      if (L % d == 0) L1 = d;
+    const void* fn = reinterpret_cast<const void*>(&k_pc_chan_x2);
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds2));
+    hipLaunchKernelGGL(k_pc_chan_x2, dim3(6 * M * N), dim3(256), lds2, s, G, ck.as<double2>(), L, M, N, L1, mc, diag,
+                       dL.as<double2>());
+  } else {
+    const int64_t total = int64_t(6) * L * M * N;
+    hipLaunchKernelGGL(k_pc_chan_x, dim3(int(std::min<int64_t>((total + 255) / 256, 1 << 20))), dim3(256), 0, s, G,
+                       ck.as<double2>(), L, M, N, mc, diag, dL.as<double2>());
+  }
+  VP_HIP(hipGetLastError());
+  const int64_t n3 = 3 * int64_t(M) * N;
+  const int64_t total = int64_t(L) * n3 * n3;
+  hipLaunchKernelGGL(k_pc_l1_blocks, dim3(int(std::min<int64_t>((total + 255) / 256, 1 << 20))), dim3(256), 0, s,
+                     ck.as<double2>(), blk, L, M, N, face_dev, gram);
+  VP_HIP(hipGetLastError());
+  VP_HIP(hipStreamSynchronize(s));
+}
+
 void build_level2_blocks(const double2* G, int L, int M, int N, double2 mc, double diag, double2* blk,
                          int kxn, int kyn, hipStream_t s, const double2* face_dev, double gram) {
   auto table = [](int P) {
@@ -1639,6 +1711,86 @@ void invert_blocks(double2* blk, int64_t nblocks, int n, hipStream_t s) {
   VP_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
   VP_HIP(hipStreamSynchronize(s));
   VP_REQUIRE(h == 0, ERR_INVALID, "preconditioner block is singular");
+}
+
+// rocSOLVER's strided batched complex LU and inverse, resolved from the
+// shared libraries on first use (only 1-level blocks wider than 256 need
+// them, so the library does not load rocBLAS otherwise).
+namespace {
+struct RocSolver {
+  using create_t = rocblas_status (*)(rocblas_handle*);
+  using destroy_t = rocblas_status (*)(rocblas_handle);
+  using stream_t = rocblas_status (*)(rocblas_handle, hipStream_t);
+  using getrf_t = decltype(&rocsolver_zgetrf_strided_batched);
+  using getri_t = decltype(&rocsolver_zgetri_strided_batched);
+  create_t create = nullptr;
+  destroy_t destroy = nullptr;
+  stream_t set_stream = nullptr;
+  getrf_t getrf = nullptr;
+  getri_t getri = nullptr;
+  std::string error;
+};
+
+const RocSolver& rocsolver() {
+  static const RocSolver r = [] {
+    RocSolver o;
+    void* hb = dlopen("librocblas.so", RTLD_NOW | RTLD_GLOBAL);
+    if (!hb) hb = dlopen("/opt/rocm/lib/librocblas.so", RTLD_NOW | RTLD_GLOBAL);
+    void* hs = dlopen("librocsolver.so", RTLD_NOW | RTLD_GLOBAL);
+    if (!hs) hs = dlopen("/opt/rocm/lib/librocsolver.so", RTLD_NOW | RTLD_GLOBAL);
+    if (!hb || !hs) {
+      o.error = "rocBLAS / rocSOLVER libraries not found";
+      return o;
+    }
+    o.create = reinterpret_cast<RocSolver::create_t>(dlsym(hb, "rocblas_create_handle"));
+    o.destroy = reinterpret_cast<RocSolver::destroy_t>(dlsym(hb, "rocblas_destroy_handle"));
+    o.set_stream = reinterpret_cast<RocSolver::stream_t>(dlsym(hb, "rocblas_set_stream"));
+    o.getrf = reinterpret_cast<RocSolver::getrf_t>(dlsym(hs, "rocsolver_zgetrf_strided_batched"));
+    o.getri = reinterpret_cast<RocSolver::getri_t>(dlsym(hs, "rocsolver_zgetri_strided_batched"));
+    if (!o.create || !o.destroy || !o.set_stream || !o.getrf || !o.getri) o.error = "rocSOLVER symbols not found";
+    return o;
+  }();
+  return r;
+}
+}  // namespace
+
+void invert_blocks_any(double2* blk, int64_t nblocks, int n, hipStream_t s) {
+  if (n <= 256) {
+    invert_blocks(blk, nblocks, n, s);
+    return;
+  }
+  const RocSolver& rs = rocsolver();
+  VP_REQUIRE(rs.error.empty(), ERR_UNSUPPORTED, rs.error);
+  VP_REQUIRE(nblocks <= INT32_MAX, ERR_UNSUPPORTED, "too many blocks");
+  rocblas_handle h = nullptr;
+  VP_REQUIRE(rs.create(&h) == rocblas_status_success, ERR_DEVICE, "rocblas_create_handle failed");
+  int bad = 0;
+  rocblas_status st1 = rocblas_status_success, st2 = rocblas_status_success;
+  try {
+    VP_REQUIRE(rs.set_stream(h, s) == rocblas_status_success, ERR_DEVICE, "rocblas_set_stream failed");
+    DevBuf ipiv(size_t(nblocks) * n * sizeof(rocblas_int)), info(size_t(nblocks) * sizeof(rocblas_int));
+    auto* A = reinterpret_cast<rocblas_double_complex*>(blk);
+    const rocblas_stride sa = rocblas_stride(n) * n, sp = n;
+    st1 = rs.getrf(h, n, n, A, n, sa, ipiv.as<rocblas_int>(), sp, info.as<rocblas_int>(), rocblas_int(nblocks));
+    if (st1 == rocblas_status_success) {
+      std::vector<rocblas_int> hi(nblocks);
+      VP_HIP(hipMemcpyAsync(hi.data(), info.p, info.bytes, hipMemcpyDeviceToHost, s));
+      VP_HIP(hipStreamSynchronize(s));
+      for (rocblas_int v : hi) bad |= v != 0;
+      if (!bad) {
+        st2 = rs.getri(h, n, A, n, sa, ipiv.as<rocblas_int>(), sp, info.as<rocblas_int>(), rocblas_int(nblocks));
+        VP_HIP(hipMemcpyAsync(hi.data(), info.p, info.bytes, hipMemcpyDeviceToHost, s));
+        VP_HIP(hipStreamSynchronize(s));
+        for (rocblas_int v : hi) bad |= v != 0;
+      }
+    }
+  } catch (...) {
+    rs.destroy(h);
+    throw;
+  }
+  rs.destroy(h);
+  VP_REQUIRE(st1 == rocblas_status_success && st2 == rocblas_status_success, ERR_DEVICE, "rocSOLVER call failed");
+  VP_REQUIRE(!bad, ERR_INVALID, "preconditioner block is singular");
 }
 
 }  // namespace vp

@@ -80,6 +80,13 @@ void launch_prec(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s);
 // Blocks are built for kx < kxn, ky < kyn at (kx * kyn + ky).
 void build_level2_blocks(const double2* G, int L, int M, int N, double2 mc, double diag, double2* blk,
                          int kxn, int kyn, hipStream_t s, const double2* face_dev = nullptr, double gram = 0.0);
+// 1-level (3MN x 3MN per kx, all L) blocks, the same conventions; face_dev =
+// device complex[M*N] for the face-averaged form Gram*I - diag(face3)*circ_N.
+void build_level1_blocks(const double2* G, int L, int M, int N, double2 mc, double diag, double2* blk,
+                         hipStream_t s, const double2* face_dev = nullptr, double gram = 0.0);
+// Inverts blocks of any size: the device Gauss-Jordan kernels up to 256 rows,
+// rocSOLVER's batched LU (getrf / getri, loaded on first use) above.
+void invert_blocks_any(double2* blk, int64_t nblocks, int n, hipStream_t s);
 // True when the odd components of a device Toeplitz tensor (xy, xz, yz)
 // vanish on their zero-offset planes (to 1e-12 of max |G|), which makes the
 // Chan circulants parity symmetric.

@@ -239,6 +239,49 @@ class Preconditioner:
         return cls(h, ctx, ix.size)
 
     @classmethod
+    def level1(cls, G, Mc, Mr0, Gram, idx, precisions=('fp64', 'fp32'), ctx=None):
+        """level_1_parallel_func(opToeplitz, Mc, Mr, L, M, N, Gram), built and inverted on the GPU; Mc a
+        scalar or an L x M x N array (vp_prec_create_level1)."""
+        ctx = ctx or default_context()
+        G = np.asarray(G)
+        Lg, Mg, Ng = G.shape[:3]
+        g = as_c128(G.reshape(-1, order='F'))
+        ix = as_idx(idx)
+        h = ctypes.c_void_p()
+        mcf, mc = (_contrast_field(Mc, (Lg, Mg, Ng)), 0j) if np.ndim(Mc) else (None, complex(Mc))
+        check(L.lib.vp_prec_create_level1(ctx.handle, Lg, Mg, Ng, ptr(g), mc.real, mc.imag,
+                                          ptr(mcf) if mcf is not None else None, float(Mr0), float(Gram), ptr(ix),
+                                          ix.size, _prec_mask(precisions), ctypes.byref(h)))
+        return cls(h, ctx, ix.size)
+
+    @classmethod
+    def level1_assembled(cls, shape, dx, ko, Mc, Mr0, idx, precisions=('fp64', 'fp32'), ctx=None, damping=0.0):
+        """1-level preconditioner from a Green tensor assembled on the GPU (vp_prec_create_level1_assembled)."""
+        ctx = ctx or default_context()
+        Lg, Mg, Ng = shape
+        ix = as_idx(idx)
+        h = ctypes.c_void_p()
+        mcf, mc = (_contrast_field(Mc, (Lg, Mg, Ng)), 0j) if np.ndim(Mc) else (None, complex(Mc))
+        check(L.lib.vp_prec_create_level1_assembled(ctx.handle, Lg, Mg, Ng, float(dx), float(ko), float(damping),
+                                                    mc.real, mc.imag, ptr(mcf) if mcf is not None else None,
+                                                    float(Mr0), ptr(ix), ix.size, _prec_mask(precisions),
+                                                    ctypes.byref(h)))
+        return cls(h, ctx, ix.size)
+
+    @classmethod
+    def level1_face(cls, shape, dx, ko, face, idx, precisions=('fp64', 'fp32'), ctx=None, damping=0.0):
+        """Face-averaged 1-level form of Bragg_MOR_script.m:61-72 (level_1_parallel_func_N): blocks
+        inv(Gram I - diag(face3) circ_N); face is M x N (vp_prec_create_level1_face)."""
+        ctx = ctx or default_context()
+        Lg, Mg, Ng = shape
+        fc = _contrast_field(face, (Mg, Ng))
+        ix = as_idx(idx)
+        h = ctypes.c_void_p()
+        check(L.lib.vp_prec_create_level1_face(ctx.handle, Lg, Mg, Ng, float(dx), float(ko), float(damping), ptr(fc),
+                                               ptr(ix), ix.size, _prec_mask(precisions), ctypes.byref(h)))
+        return cls(h, ctx, ix.size)
+
+    @classmethod
     def level2(cls, G, Mc, Mr0, Gram, idx, precisions=('fp64', 'fp32'), ctx=None):
         """level_2_parallel_func(opToeplitz, Mc, Mr, L, M, N, Gram), built on the GPU; Mc a scalar or an
         L x M x N array (vp_prec_create_level2 / vp_prec_create_level2_field)."""
@@ -327,7 +370,7 @@ class Preconditioner:
 
 
 def _contrast_field(Mc, shape):
-    """An L x M x N contrast array as interleaved complex128, column-major."""
+    """A contrast (or face) array of the given shape as interleaved complex128, column-major."""
     Mc = np.asarray(Mc)
     if Mc.shape != tuple(shape):
         raise ValueError(f'contrast field of shape {Mc.shape}, expected {tuple(shape)}')
