@@ -79,7 +79,7 @@ WORKLOADS = {
         desc='Si microdisk r = 10 um, 220 nm thick, with a 500 nm bus waveguide at a 200 nm gap, SiO2, '
              'lambda 1550 nm, 15 nm voxels, grid 1400x1500x30 (63 M voxels), y-polarised Gaussian (w0 300 nm) '
              'centred on the bus; block-Jacobi composite of 2-level circulants on the bus and disk boxes',
-        tol=1e-6, restart=50, inner_tol=1e-2, inner_maxit=2, max_outer=40, prec='composite', damping=0.0,
+        tol=1e-6, restart=50, inner_tol=1e-2, inner_maxit=2, max_outer=10, prec='composite', damping=0.0,
         fields=False, cpu_baseline=False),
     'bragg_20nm': dict(
         desc='Bragg grating (100 periods) at 20 nm voxels, grid 1800x50x22 (FFT 3600x100x44), lambda 1550 nm',
