@@ -10,6 +10,8 @@ Hessenberg matrices, harmonic Ritz eigenproblems, QR of H P) runs on the host
 in NumPy/SciPy, as the reference does in MATLAB.
 """
 import ctypes
+import os
+import sys
 
 import numpy as np
 
@@ -167,9 +169,17 @@ def pgmresDR(A, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, L2=None, 
                 return Hk.copy(), B[:, :k].copy(), k, rv[:k]
         return H, B, m, rv
 
+    progress = os.environ.get('VP_PROGRESS') == '1'
+
+    def report(outit):
+        if progress:
+            print(f'[vp] pgmresDR cycle {outit}: {len(resvec) - 1} iterations, estimated relres {resvec[-1]:.3e}',
+                  file=sys.stderr, flush=True)
+
     # first cycle: GMRES(restart) (pgmresDR.m:165-203)
     H, _, p, rin = cycle(addr(w.r[ri]), [])
     resvec.extend(list(rin / bnorm))
+    report(0)
     Vc = w.cols(w.V, 0, p + 1)
     y = np.linalg.lstsq(H, vec_dots(ctx, Vc, [w.r[ri]], n)[:, 0], rcond=None)[0]
     xi = 0
@@ -191,6 +201,7 @@ def pgmresDR(A, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, L2=None, 
         Uc = w.cols(w.U[ui], 0, kc)
         H, B, p, rin = cycle(addr(w.r[ri]), Cc)
         resvec.extend(list(rin / bnorm))
+        report(outit)
         Vc = w.cols(w.V, 0, p + 1)
         d = np.sqrt(np.abs(np.diag(vec_dots(ctx, Uc, Uc, n)))) if kc else np.zeros(0)
         D = np.diag(1.0 / d)
