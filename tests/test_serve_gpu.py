@@ -136,3 +136,53 @@ def test_server_rejects_mismatched_preconditioner(vp_ctx):
             srv.submit(op_a, P_b, reqs[0][2], x, **PARAMS)
     finally:
         srv.close()
+
+
+def _notched(vp_ctx, x_notch):
+    """A waveguide with one cross-section column of the core removed at
+    x_notch: equal grids, boxes and unknown counts, different voxel maps."""
+    import dataclasses
+    from voxphoton_amd import configs, vie
+    base = configs.straight_waveguide(L=24, M=14, N=10, dx=40e-9)
+    mask = base.mask.copy()
+    ys, zs = np.nonzero(mask[x_notch])
+    mask[x_notch, ys[0], zs[0]] = False
+    prob = dataclasses.replace(base, mask=mask)
+    op = vie.NOperator.assembled(prob.shape, prob.dx, prob.ko, None, prob.Mc, prob.idx, ctx=vp_ctx)
+    shape, bidx, sub = prob.box_problem()
+    P = vie.Preconditioner.level2_assembled(shape, prob.dx, prob.ko, prob.core_contrast(), 1.0, bidx,
+                                            ctx=vp_ctx).set_embedding(prob.n, sub)
+    b = prob.gaussian_rhs('y')
+    db = vie.DeviceArray(vp_ctx, b.nbytes)
+    db.upload(b)
+    return op, P, db, b.size, prob
+
+
+def test_equal_size_different_masks_are_never_batched_together(vp_ctx):
+    # two geometries with the same grid, box and number of unknowns but
+    # different voxel maps: the batched solve refuses to share one map, and
+    # the server keeps them in separate groups, so each equals its solo solve
+    from voxphoton_amd import vie
+    from voxphoton_amd._lib import VoxPhotonError
+    a = _notched(vp_ctx, 5)
+    b = _notched(vp_ctx, 17)
+    assert a[3] == b[3] and a[4].box == b[4].box and not np.array_equal(a[4].mask, b[4].mask)
+    B = np.array([a[4].gaussian_rhs('y'), b[4].gaussian_rhs('y')])
+    with pytest.raises(VoxPhotonError):
+        vie.solve_ir_batched([a[0], b[0]], [a[1], b[1]], B, tol=PARAMS['tol'], restart=PARAMS['restart'],
+                             inner_tol=PARAMS['inner_tol'], inner_maxit=PARAMS['inner_maxit'],
+                             max_outer=PARAMS['max_outer'], side='right')
+    reqs = [a[:4], b[:4]]
+    solo = _solo(vp_ctx, reqs)
+    srv = vie.Server(hbm_budget_bytes=8 << 30, max_active=16, ctx=vp_ctx)
+    try:
+        xs = [vie.DeviceArray(vp_ctx, n * 16) for (_, _, _, n) in reqs]
+        tickets = [srv.submit(op, P, db, xs[i], **PARAMS) for i, (op, P, db, n) in enumerate(reqs)]
+        for i, t in enumerate(tickets):
+            info = srv.wait(t)
+            x = np.empty(reqs[i][3], complex)
+            xs[i].download(x)
+            print('notched', i, info['relres'], solo[i][2], rel(x, solo[i][0]), flush=True)
+            assert info['flag'] == 0 and rel(x, solo[i][0]) <= 1e-12 and info['relres'] == solo[i][2]
+    finally:
+        srv.close()
