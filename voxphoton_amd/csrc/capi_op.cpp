@@ -464,6 +464,15 @@ static bool smooth_length(int n, bool fp64) {
   return n == 1;
 }
 
+// A length with a compile-time kernel on its axis (x: the class length
+// n/2, y: the y pass, z: a fused radix pair).
+static bool fast_length(int n, int axis, bool fp64) {
+  int ra = 0, rb = 0;
+  return axis == 0 ? (n % 2 == 0 && xct_threads(n / 2, fp64) > 0)
+       : axis == 1 ? yct_supported(n)
+                   : zfused_pair(n, fp64, ra, rb);
+}
+
 static std::vector<int> axis_candidates(int B, int axis, bool fp64, int fixed) {
   if (fixed > 0) return {fixed};
   const int lo = std::max(2, 2 * B - 1);
@@ -471,11 +480,7 @@ static std::vector<int> axis_candidates(int B, int axis, bool fp64, int fixed) {
   std::vector<int> fast, slow;
   for (int n = lo; n <= hi; ++n) {
     if (!smooth_length(n, fp64)) continue;
-    int ra = 0, rb = 0;
-    const bool f = axis == 0 ? (n % 2 == 0 && xct_threads(n / 2, fp64) > 0)
-                 : axis == 1 ? yct_supported(n)
-                             : zfused_pair(n, fp64, ra, rb);
-    (f ? fast : slow).push_back(n);
+    (fast_length(n, axis, fp64) ? fast : slow).push_back(n);
   }
   std::vector<int> out(fast.begin(), fast.begin() + std::min<size_t>(2, fast.size()));
   for (size_t i = 0; out.size() < 2 && i < slow.size(); ++i)
@@ -496,12 +501,15 @@ struct SizeKey {
 static std::mutex g_sizes_mu;
 static std::map<SizeKey, Sizes> g_sizes;
 
-static int fft_sizes_mode() {   // 0 auto (timed), 1 reference 2D, 2 shortest candidates
+// 0 auto (timed), 1 reference 2D, 2 shortest candidates, 3 per axis the
+// shortest candidate with a compile-time kernel, else the shortest (no timing)
+static int fft_sizes_mode() {
   const char* e = std::getenv("VP_FFT_SIZES");
   if (!e || !*e || !std::strcmp(e, "auto")) return 0;
   if (!std::strcmp(e, "reference")) return 1;
   if (!std::strcmp(e, "min")) return 2;
-  throw VpError{ERR_INVALID, "VP_FFT_SIZES must be auto, reference or min"};
+  if (!std::strcmp(e, "kernels")) return 3;
+  throw VpError{ERR_INVALID, "VP_FFT_SIZES must be auto, reference, min or kernels"};
 }
 
 // Circulant sizes for one precision: fixed axes (> 0) are kept; each free
@@ -526,6 +534,14 @@ static Sizes choose_sizes(vp_op_s* op, OpPrec<T>& P, bool fp64, bool fsym, Sizes
              cz = cands(op->Nb, 2, fixed.Np, also.Np);
   VP_REQUIRE(!cx.empty() && !cy.empty() && !cz.empty(), ERR_UNSUPPORTED, "no FFT size candidates for the box");
   if (mode == 2 || (cx.size() == 1 && cy.size() == 1 && cz.size() == 1)) return Sizes{cx[0], cy[0], cz[0]};
+  if (mode == 3) {
+    auto pick = [&](const std::vector<int>& c, int axis) {
+      for (int n : c)
+        if (fast_length(n, axis, fp64)) return n;
+      return c[0];
+    };
+    return Sizes{pick(cx, 0), pick(cy, 1), pick(cz, 2)};
+  }
   const SizeKey key{op->ctx->device, op->Lb, op->Mb, op->Nb, int(fp64), int(fsym), fixed.Lp, fixed.Mp, fixed.Np,
                     also.Lp, also.Mp, also.Np};
   {
