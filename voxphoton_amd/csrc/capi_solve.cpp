@@ -518,7 +518,7 @@ static void level2_block_range(vp_prec_s* p, const double2* dG) {
   const char* ec = std::getenv("VP_PC_CT");
   const int lds_max = max_lds_bytes(p->ctx->device);
   g.pct = (g.pquad && !(ec && ec[0] == '0') && xct_threads(g.L, false) > 0 && xct_threads(g.L, true) > 0 &&
-           pcrow_supported(g.M) && (2 * size_t(g.L) + 64 + (size_t(g.L) + 63) / 64) * 16 <= size_t(lds_max))
+           pcrow_supported(g.M) && (size_t(g.L) + 64 + (size_t(g.L) + 63) / 64) * 16 <= size_t(lds_max))
               ? 1
               : 0;
 }

@@ -598,4 +598,67 @@ __device__ __forceinline__ C<T>* ct_fft(C<T>* a, C<T>* b, const Tw& tw, int twm,
   }
 }
 
+// In-place form of ct_stage / ct_fft: one LDS buffer.  A stage first loads
+// every butterfly input of the thread (ITER x R values in registers), then,
+// after a barrier, writes its outputs into the same buffer, so a transform
+// needs half the LDS of the ping-pong form (the first stage reads through
+// ld and needs no barrier before its writes).
+template <typename T, bool INV, int NLEN, int NB, int LD, int NT, int NS, int R, bool FIRST, bool LAST, bool BF,
+          class Tw, class Ld, class St>
+__device__ __forceinline__ void ct_stage_ip(C<T>* buf, const Tw& tw, int twm, Ld& ld, St& st) {
+  constexpr int NBF = NLEN / R;
+  constexpr int TOTAL = NBF * NB;
+  constexpr int ITER = (TOTAL + NT - 1) / NT;
+  C<T> v[ITER][R];
+#pragma unroll
+  for (int i = 0; i < ITER; ++i) {
+    const int it = i * NT + int(threadIdx.x);
+    if (TOTAL % NT == 0 || it < TOTAL) {
+      const int b = BF ? it % NB : it / NBF;
+      const int j = BF ? it / NB : it - b * NBF;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = FIRST ? ld(b, j + NBF * r) : buf[(j + NBF * r) * LD + b];
+    }
+  }
+  if constexpr (!FIRST) __syncthreads();
+#pragma unroll
+  for (int i = 0; i < ITER; ++i) {
+    const int it = i * NT + int(threadIdx.x);
+    if (TOTAL % NT == 0 || it < TOTAL) {
+      const int b = BF ? it % NB : it / NBF;
+      const int j = BF ? it / NB : it - b * NBF;
+      const int jm = NS > 1 ? j % NS : 0;
+      if constexpr (NS > 1) {
+        constexpr int TM = NLEN / (NS * R);
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], twiddle<T, INV>(tw, r * jm * TM * twm));
+      }
+      dft<T, R, INV>(v[i]);
+      const int base = (j - jm) * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (LAST) st(b, base + r * NS, v[i][r]);
+        else buf[(base + r * NS) * LD + b] = v[i][r];
+      }
+    }
+  }
+}
+
+template <typename T, bool INV, int NLEN, int NB, int LD, int NT, int NS, bool FIRST, bool FB, bool LB, bool LDSOUT,
+          int R0, int... Rs, class Tw, class Ld, class St>
+__device__ __forceinline__ C<T>* ct_fft_ip(C<T>* buf, const Tw& tw, int twm, Ld& ld, St& st) {
+  constexpr bool END = sizeof...(Rs) == 0;
+  constexpr bool BF = FIRST ? FB : (END && !LDSOUT ? LB : false);
+  ct_stage_ip<T, INV, NLEN, NB, LD, NT, NS, R0, FIRST, END && !LDSOUT, BF>(buf, tw, twm, ld, st);
+  if constexpr (!END) {
+    __syncthreads();
+    return ct_fft_ip<T, INV, NLEN, NB, LD, NT, NS * R0, false, FB, LB, LDSOUT, Rs...>(buf, tw, twm, ld, st);
+  } else if constexpr (LDSOUT) {
+    __syncthreads();
+    return buf;
+  } else {
+    return nullptr;
+  }
+}
+
 }  // namespace vp

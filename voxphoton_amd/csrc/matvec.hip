@@ -252,8 +252,7 @@ __global__ void __launch_bounds__(NT) k_xfwd_ct(const ProbPtrs<T>* __restrict__ 
   const ProbPtrs<T> P = probs[blockIdx.y];
   const C<T>* __restrict__ X = va.x + int64_t(blockIdx.y) * va.xstride;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
-  C<T>* W1 = W0 + S;
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);     // one buffer: the stages run in place
   const int R = g.x.R, L = g.L, M = g.M, N = g.N;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int pencil = blk / R, px = blk - pencil * R;
@@ -273,11 +272,11 @@ __global__ void __launch_bounds__(NT) k_xfwd_ct(const ProbPtrs<T>* __restrict__ 
   if (twlds) {
     // stage twiddles from two short FP64 factor tables in LDS instead of
     // the Lp-entry global table
-    const TwSplit<T> tws = TwSplit<T>::fill(W1 + S, S * R);
+    const TwSplit<T> tws = TwSplit<T>::fill(W0 + S, S * R);
     __syncthreads();
-    ct_fft<T, false, S, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, W1, tws, R, ld, st);
+    ct_fft_ip<T, false, S, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, tws, R, ld, st);
   } else {
-    ct_fft<T, false, S, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, W1, twx, R, ld, st);
+    ct_fft_ip<T, false, S, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, twx, R, ld, st);
   }
 }
 
@@ -317,8 +316,7 @@ __global__ void __launch_bounds__(NT) k_xinv_ct(const ProbPtrs<T>* __restrict__ 
   const C<T>* __restrict__ X = va.x + int64_t(blockIdx.y) * va.xstride;
   C<T>* __restrict__ Y = va.y + int64_t(blockIdx.y) * va.ystride;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
-  C<T>* W1 = W0 + S;
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);     // one buffer: the stages run in place
   const int R = g.x.R, L = g.L, M = g.M, N = g.N;
   const int pencil = xcd_remap(blockIdx.x, gridDim.x);
   const int cn = pencil / M, m = pencil - cn * M;
@@ -329,14 +327,14 @@ __global__ void __launch_bounds__(NT) k_xinv_ct(const ProbPtrs<T>* __restrict__ 
   auto none = [](int, int, C<T>) {};
   TwSplit<T> tws{};
   if (twlds) {
-    tws = TwSplit<T>::fill(W1 + S, S * R);
+    tws = TwSplit<T>::fill(W0 + S, S * R);
     __syncthreads();
   }
   for (int px = 0; px < R; ++px) {
     const BufRes Wi = buf_res(P.work + int64_t(pencil) * g.x.P + int64_t(px) * S, uint32_t(S) * uint32_t(sizeof(C<T>)));
     auto ld = [&](int, int k) { return buf_load<T>(Wi, uint32_t(k)); };
-    const C<T>* Rb = twlds ? ct_fft<T, true, S, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, W1, tws, R, ld, none)
-                           : ct_fft<T, true, S, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, W1, twx, R, ld, none);
+    const C<T>* Rb = twlds ? ct_fft_ip<T, true, S, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, tws, R, ld, none)
+                           : ct_fft_ip<T, true, S, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, twx, R, ld, none);
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int l = int(threadIdx.x) + e * NT;
@@ -357,7 +355,8 @@ __global__ void __launch_bounds__(NT) k_xinv_ct(const ProbPtrs<T>* __restrict__ 
 
 // Same for two residue classes (R = 2, FP32): both class transforms run as
 // one batch of two in LDS (element k of class b at 2k + b), so a workgroup
-// takes the stages once with twice the butterflies per stage.
+// takes the stages once with twice the butterflies per stage.  The stages run
+// in place in one buffer (2S elements), so two workgroups fit a CU.
 template <typename T, int NT, int... Rs>
 __global__ void __launch_bounds__(NT) k_xinv_ct2(const ProbPtrs<T>* __restrict__ probs, VecArgs<T> va, Geometry g,
                                                  int transp, const C<T>* __restrict__ twx) {
@@ -368,18 +367,17 @@ __global__ void __launch_bounds__(NT) k_xinv_ct2(const ProbPtrs<T>* __restrict__
   const C<T>* __restrict__ X = va.x + int64_t(blockIdx.y) * va.xstride;
   C<T>* __restrict__ Y = va.y + int64_t(blockIdx.y) * va.ystride;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
-  C<T>* W1 = W0 + 2 * S;
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);     // one buffer: the stages run in place
   const int L = g.L, M = g.M, N = g.N;
   const int pencil = xcd_remap(blockIdx.x, gridDim.x);
   const int cn = pencil / M, m = pencil - cn * M;
   const int c = cn / N, n = cn - c * N;
-  const TwSplit<T> tws = TwSplit<T>::fill(W1 + 2 * S, 2 * S);
+  const TwSplit<T> tws = TwSplit<T>::fill(W0 + 2 * S, 2 * S);
   __syncthreads();
   const BufRes Wi = buf_res(P.work + int64_t(pencil) * g.x.P, uint32_t(2 * S) * uint32_t(sizeof(C<T>)));
   auto ld = [&](int b, int k) { return buf_load<T>(Wi, uint32_t(b * S + k)); };
   auto none = [](int, int, C<T>) {};
-  const C<T>* Rb = ct_fft<T, true, S, 2, 2, NT, 1, true, false, false, true, Rs...>(W0, W1, tws, 2, ld, none);
+  const C<T>* Rb = ct_fft_ip<T, true, S, 2, 2, NT, 1, true, false, false, true, Rs...>(W0, tws, 2, ld, none);
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int l = int(threadIdx.x) + e * NT;
@@ -994,8 +992,8 @@ static void launch_xpass_ct(const Geometry& g, const MatvecPlan& mp, const ProbP
     if (INV && xpair && g.x.R == 2) {
 #define VP_XPAIR_LAUNCH(SL, NTP, PR)                                                                    \
   if (!done && mp.xct == SL &&                                                                          \
-      4 * size_t(SL) * sizeof(C<T>) + TwSplit<T>::entries(2 * SL) * sizeof(double2) <= 160 * 1024) {    \
-    const size_t lds = 4 * size_t(SL) * sizeof(C<T>) + TwSplit<T>::entries(2 * SL) * sizeof(double2);    \
+      2 * size_t(SL) * sizeof(C<T>) + TwSplit<T>::entries(2 * SL) * sizeof(double2) <= 160 * 1024) {    \
+    const size_t lds = 2 * size_t(SL) * sizeof(C<T>) + TwSplit<T>::entries(2 * SL) * sizeof(double2);    \
     const void* fn = reinterpret_cast<const void*>(&k_xinv_ct2<T, NTP, VP_UNPAREN PR>);                 \
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));                \
     hipLaunchKernelGGL((k_xinv_ct2<T, NTP, VP_UNPAREN PR>), grid, dim3(NTP), lds, s, d_probs, va, g, transp, twx); \
@@ -1008,7 +1006,7 @@ static void launch_xpass_ct(const Geometry& g, const MatvecPlan& mp, const ProbP
   }
 #define VP_XCT_LAUNCH(SL, NTF, FR, NTI, IR)                                                             \
   if (!done && mp.xct == SL) {                                                                          \
-    const size_t lds = 2 * size_t(SL) * sizeof(C<T>) + TwSplit<T>::entries(SL * g.x.R) * sizeof(double2); \
+    const size_t lds = size_t(SL) * sizeof(C<T>) + TwSplit<T>::entries(SL * g.x.R) * sizeof(double2);   \
     if (INV) {                                                                                          \
       const void* fn = reinterpret_cast<const void*>(&k_xinv_ct<T, NTI, VP_UNPAREN IR>);                \
       (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));              \

@@ -327,8 +327,9 @@ bool make_matvec_plan(const Geometry& g, bool fp64, int device, MatvecPlan& out,
     // the generic x passes and plane-major x spectra)
     const char* ex = std::getenv("VP_XCT");
     out.xct = 0;
-    // LDS: two class buffers plus the FP64 split twiddle tables (64 + ceil(Lp/64) entries)
-    const size_t xlds = 2 * size_t(g.x.S) * s + (64 + (size_t(g.x.P) + 63) / 64) * 16;
+    // LDS: one class buffer (the stages run in place) plus the FP64 split
+    // twiddle tables (64 + ceil(Lp/64) entries)
+    const size_t xlds = size_t(g.x.S) * s + (64 + (size_t(g.x.P) + 63) / 64) * 16;
     if (out.yct && !(ex && ex[0] == '0') && xct_threads(g.x.S, fp64) > 0 && xlds <= size_t(lds_max))
       out.xct = g.x.S;
     // fused two-stage z kernel (default; VP_ZFUSED=0 keeps the LDS z pass)

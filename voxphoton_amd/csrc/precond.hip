@@ -558,8 +558,7 @@ __global__ void __launch_bounds__(NT) k_pc_xfwd_ct(PcArgs<T> a, PcGeom g, int tw
   const C<T>* __restrict__ X = a.x + int64_t(blockIdx.y) * a.xstride;
   C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
-  C<T>* W1 = W0 + L;
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);     // one buffer: the stages run in place
   const int M = g.M, N = g.N;
   const int p = xcd_remap(blockIdx.x, gridDim.x);   // pencil (c, n, m), m fastest
   const int cn = p / M, m = p - cn * M;
@@ -571,11 +570,11 @@ __global__ void __launch_bounds__(NT) k_pc_xfwd_ct(PcArgs<T> a, PcGeom g, int tw
   const BufRes Wo = buf_res(work + int64_t(p) * L, uint32_t(L) * uint32_t(sizeof(C<T>)));
   auto st = [&](int, int k, C<T> v) { buf_store(Wo, uint32_t(k), v); };
   if (twlds) {   // stage twiddles from two short FP64 factor tables in LDS
-    const TwSplit<T> tws = TwSplit<T>::fill(W1 + L, L);
+    const TwSplit<T> tws = TwSplit<T>::fill(W0 + L, L);
     __syncthreads();
-    ct_fft<T, false, L, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, W1, tws, 1, ld, st);
+    ct_fft_ip<T, false, L, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, tws, 1, ld, st);
   } else {
-    ct_fft<T, false, L, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, W1, a.twx, 1, ld, st);
+    ct_fft_ip<T, false, L, 1, 1, NT, 1, true, false, false, false, Rs...>(W0, a.twx, 1, ld, st);
   }
 }
 
@@ -586,8 +585,7 @@ __global__ void __launch_bounds__(NT) k_pc_xinv_ct(PcArgs<T> a, PcGeom g, int tw
   C<T>* __restrict__ Y = a.y + int64_t(blockIdx.y) * a.ystride;
   const C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);
-  C<T>* W1 = W0 + L;
+  C<T>* W0 = reinterpret_cast<C<T>*>(smem_raw);     // one buffer: the stages run in place
   const int M = g.M, N = g.N;
   const int p = xcd_remap(blockIdx.x, gridDim.x);
   const int cn = p / M, m = p - cn * M;
@@ -597,11 +595,11 @@ __global__ void __launch_bounds__(NT) k_pc_xinv_ct(PcArgs<T> a, PcGeom g, int tw
   auto none = [](int, int, C<T>) {};
   const C<T>* R;
   if (twlds) {
-    const TwSplit<T> tws = TwSplit<T>::fill(W1 + L, L);
+    const TwSplit<T> tws = TwSplit<T>::fill(W0 + L, L);
     __syncthreads();
-    R = ct_fft<T, true, L, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, W1, tws, 1, ld, none);
+    R = ct_fft_ip<T, true, L, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, tws, 1, ld, none);
   } else {
-    R = ct_fft<T, true, L, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, W1, a.twx, 1, ld, none);
+    R = ct_fft_ip<T, true, L, 1, 1, NT, 1, true, false, false, true, Rs...>(W0, a.twx, 1, ld, none);
   }
   for (int l = threadIdx.x; l < L; l += NT) {
     const int64_t j = pc_unknown(a, g, l + int64_t(L) * (m + int64_t(M) * n), c);
@@ -682,7 +680,7 @@ template <typename T>
 static void launch_prec_pct(const PcGeom& g, const PcArgs<T>& a, int nprob, hipStream_t s) {
   constexpr bool F64 = sizeof(T) == 8;
   const unsigned npen = unsigned(3 * g.N * g.M);
-  const size_t ldsx = 2 * size_t(g.L) * sizeof(C<T>) + TwSplit<T>::entries(g.L) * sizeof(double2);
+  const size_t ldsx = size_t(g.L) * sizeof(C<T>) + TwSplit<T>::entries(g.L) * sizeof(double2);
   const int twlds = [] {   // VP_XTW_LDS=0: stage twiddles from the global table (read per launch)
     const char* e = std::getenv("VP_XTW_LDS");
     return (e && e[0] == '0') ? 0 : 1;
