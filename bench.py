@@ -76,10 +76,12 @@ WORKLOADS = {
         tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=4, max_outer=20, nreq=40, rate=50.0, seed=4,
         budget_gb=96, max_active=32, setup_workers=8, fft_sizes='min'),
     'microdisk_15nm': dict(
-        desc='Si microdisk r = 10 um, 220 nm thick, with a 500 nm bus waveguide at a 200 nm gap, SiO2, '
-             'lambda 1550 nm, 15 nm voxels, grid 1400x1500x30 (63 M voxels), y-polarised Gaussian (w0 300 nm) '
-             'centred on the bus; block-Jacobi composite of 2-level circulants on the bus and disk boxes',
-        tol=1e-6, restart=50, inner_tol=1e-2, inner_maxit=2, max_outer=10, prec='composite', damping=0.0,
+        desc='Si microdisk r = 10 um, 220 nm thick, with a 500 nm bus waveguide at a 200 nm gap (quadratic '
+             'absorbers, Im eps 0->1 over 2 um, at both bus ends), SiO2, lambda 1550 nm, 15 nm voxels, grid '
+             '1400x1500x30 (63 M voxels), y-polarised Gaussian (w0 300 nm) centred on the bus; block-Jacobi '
+             'composite of 2-level circulants: face-averaged and exact across the bus on the bus box, core '
+             'contrast on the disk box',
+        tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=1, max_outer=10, prec='composite', damping=0.0,
         fields=False, cpu_baseline=False),
     'bragg_20nm': dict(
         desc='Bragg grating (100 periods) at 20 nm voxels, grid 1800x50x22 (FFT 3600x100x44), lambda 1550 nm',
@@ -190,6 +192,7 @@ def main():
     ap.add_argument('--inner-tol', type=float, default=None, help='override the workload inner tolerance')
     ap.add_argument('--inner-maxit', type=int, default=None, help='override the workload inner restart cycles')
     ap.add_argument('--batch', type=int, default=None, help='sweep: wavelengths per batched solve (override)')
+    ap.add_argument('--max-outer', type=int, default=None, help='override the workload refinement-step limit')
     ap.add_argument('--traffic-probe', action='store_true',
                     help='for rocprofv3 --pmc runs: a 1 GiB device copy (calibration), then 3 FP32 matvecs '
                          'and 3 FP32 preconditioner applies')
@@ -207,6 +210,8 @@ def main():
         wl['inner_tol'] = args.inner_tol
     if args.inner_maxit is not None:
         wl['inner_maxit'] = args.inner_maxit
+    if args.max_outer is not None:
+        wl['max_outer'] = args.max_outer
     if args.batch is not None and 'chunk' in wl:
         wl['chunk'] = args.batch
     log = lambda *a: print('[bench]', *a, file=sys.stderr, flush=True)   # noqa: E731
@@ -973,22 +978,30 @@ def make_preconditioner(prob, wl, vie, configs, ctx=None):
     n = prob.n
     if wl['prec'] == 'composite':
         # block Jacobi over the sub-boxes (prec_fun_disk_idx.m): a 2-level
-        # circulant with the core contrast on each box, embedded on its unknowns
+        # circulant on each box, embedded on its unknowns.  Part forms:
+        # 'face_y' face-averaged contrast, exact across the box's y extent (the
+        # waveguide form of Bragg_MOR_script.m:61-72); 'core' the core contrast,
+        # exact along z; 'field' the box's own contrast array folded into the
+        # Toeplitz tensor, as resonator_MOR_scripts.m:165-170 builds the disk square
+        forms = wl.get('part_forms') or prob.extra.get('part_forms') or ('core',) * len(prob.extra['parts'])
         parts = []
-        for box in prob.extra['parts']:
-            shape, bidx, sub = prob.box_problem(box)
-            # 'field': the box's own contrast array folded into the Toeplitz
-            # tensor, as resonator_MOR_scripts.m:165-170 builds the disk square
-            mc = prob.Mc[box] if wl.get('contrast') == 'field' else prob.core_contrast()
-            q = vie.Preconditioner.level2_assembled(shape, prob.dx, prob.ko, mc, 1.0, bidx,
-                                                    damping=wl['damping'], ctx=ctx)
+        for box, form in zip(prob.extra['parts'], forms):
+            if form == 'face_y':
+                shape, bidx, sub, face = configs.face_box(prob, 1, box=box)
+                q = vie.Preconditioner.level2_face(shape, prob.dx, prob.ko, face, bidx, damping=wl['damping'],
+                                                   ctx=ctx)
+            else:
+                shape, bidx, sub = prob.box_problem(box)
+                mc = prob.Mc[box] if form == 'field' else prob.core_contrast()
+                q = vie.Preconditioner.level2_assembled(shape, prob.dx, prob.ko, mc, 1.0, bidx,
+                                                        damping=wl['damping'], ctx=ctx)
             q.set_embedding(n, sub)
             q.box_shape = tuple(int(v) for v in shape)
             parts.append(q)
         P = vie.Preconditioner.composite(parts, ctx=ctx)
         P.box_shape = parts[-1].box_shape
-        desc = (f'block-Jacobi composite of 2-level circulants ({wl.get("contrast", "core")} contrast) on the '
-                f'boxes {[q.box_shape for q in parts]}, damping {wl["damping"]}')
+        desc = (f'block-Jacobi composite of 2-level circulants on the boxes '
+                f'{[f"{q.box_shape} {f}" for q, f in zip(parts, forms)]}, damping {wl["damping"]}')
         return P, desc
     if wl['prec'] == 'face':
         shape, pidx, sub, face = configs.face_box(prob, wl['axis'])

@@ -378,6 +378,52 @@ def test_gmres_step_chunks_give_identical_results(vp_ctx, monkeypatch):
           flush=True)
 
 
+def test_gmres_graph_replays_equal_stream_launches(vp_ctx, monkeypatch):
+    # VP_GMRES_GRAPH=1 captures each chunk of Arnoldi steps into a hipGraph on
+    # first use and replays it in later restart cycles and later inner solves
+    # of a refinement run.  Every replay must give what the stream launches
+    # give: same iterates, histories and flags (bit for bit), for right, left
+    # and no preconditioning, single and batched, with eager work and
+    # synchronisations between the solves that replay the graphs.
+    from voxphoton_amd.vie import NOperator, Preconditioner, pgmres, solve_ir_batched
+    G, Mc, Gram, idx, b, dx, ko = _physical(8, 5, 4, seed=6)
+    op = NOperator.from_toeplitz(G, None, Mc, Gram, idx, ctx=vp_ctx)
+    P = Preconditioner.level2(G, Mc.flat[0], 1.0, Gram, idx, ctx=vp_ctx)
+    rng = np.random.default_rng(7)
+    B = np.array([b, rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size), 0.5 * b])
+
+    def run():
+        out = [pgmres(op, b, 9, 1e-11, 8, R1=P), pgmres(op, b, 9, 1e-11, 8, L1=P),
+               pgmres(op, b, 9, 1e-11, 8, prec='fp32')]
+        vp_ctx.synchronize()
+        X, infos = solve_ir_batched([op] * 3, [P] * 3, B, tol=1e-10, restart=11, inner_tol=1e-4, inner_maxit=3,
+                                    max_outer=12, side='right')
+        return out, X, infos
+
+    monkeypatch.setenv('VP_GMRES_CHUNK', '3')
+    monkeypatch.setenv('VP_GMRES_GRAPH', '0')
+    ref = run()
+    monkeypatch.setenv('VP_GMRES_GRAPH', '1')
+    for rep in range(3):
+        got = run()
+        # eager work between the graph runs
+        y = op.apply(b)
+        z = P.apply(y, prec='fp32')
+        assert np.isfinite(z).all()
+        vp_ctx.synchronize()
+        for (s, s0) in zip(got[0], ref[0]):
+            assert np.array_equal(s[0], s0[0]) and s[1] == s0[1] and s[2] == s0[2]
+            assert np.array_equal(s[4], s0[4])
+        assert np.array_equal(got[1], ref[1])
+        for a, c in zip(got[2], ref[2]):
+            assert a['inner_iters'] == c['inner_iters'] and a['relres'] == c['relres']
+            assert np.array_equal(a['history'], c['history'])
+    cycles = [len(s0[4]) - 1 for s0 in ref[0]]
+    print('graph GMRES equals stream launches: iterations (right, left, fp32 none)', cycles,
+          'batched inner its', [c['inner_iters'] for c in ref[2]], flush=True)
+    assert max(cycles) > 18     # several restart cycles: the chunk graphs were replayed
+
+
 def test_gmres_maxit_flag_and_breakdown(vp_ctx):
     from voxphoton_amd.vie import NOperator, pgmres
     G, Mc, Gram, idx, b, dx, ko = _physical(5, 4, 3, seed=4)

@@ -118,8 +118,9 @@ class Problem:
         return (Lb, Mb, Nb), bidx, sub.astype(np.int64)
 
 
-def face_box(prob, exact_axis=1):
-    """Face-averaged preconditioner box with ``exact_axis`` (1 = y, 2 = z) last.
+def face_box(prob, exact_axis=1, box=None):
+    """Face-averaged preconditioner box (default ``prob.box``) with
+    ``exact_axis`` (1 = y, 2 = z) last.
 
     Returns (shape, idx, sub_of_op, face): the box permuted so that the
     exact axis is the third one, its unknown indices, the 1-based positions
@@ -127,7 +128,7 @@ def face_box(prob, exact_axis=1):
     axes), and face = (e - 1)/e of the permittivity averaged over the two
     circulant axes (real part, as Bragg_MOR_script.m:64-65 takes it).
     """
-    sx, sy, sz = prob.box
+    sx, sy, sz = box if box is not None else prob.box
     bmask = prob.mask[sx, sy, sz]
     beps = (prob.eps / EPS_SIO2)[sx, sy, sz]
     beps = np.where(bmask, beps, 1.0)
@@ -251,7 +252,7 @@ def disk_resonator(radius=2e-6, dx=20e-9, lam=1550e-9, thick=220e-9, margin=2):
 
 
 def microdisk_bus(radius=10e-6, gap=200e-9, width=500e-9, thick=220e-9, dx=15e-9, lam=1550e-9, L=1400, M=1500,
-                  N=30):
+                  N=30, len_abs=2e-6, abs_im=1.0):
     """Si microdisk with a straight bus waveguide in SiO2 (the resonator of
     resonator_MOR_scripts.m).  The disk (``radius``, ``thick``) and the bus
     (``width`` wide, running along x over the whole grid, ``gap`` from the
@@ -259,9 +260,20 @@ def microdisk_bus(radius=10e-6, gap=200e-9, width=500e-9, thick=220e-9, dx=15e-9
     and z.  Defaults: radius 10 um, gap 200 nm, 500 nm bus, 220 nm thick,
     15 nm voxels on 1400 x 1500 x 30.
 
+    The bus carries quadratic-ramp absorbers at both ends, as the resonator
+    script gives its waveguide (resonator_MOR_scripts.m:66-77,
+    sigma0*fun_absorb on the waveguide voxels only): the bus permittivity is
+    eps_Si - j*abs_im*(d/len_abs)^2 at depth d < len_abs from either end, the
+    form of the grating's absorbers.  Without them the cut bus ends reflect
+    and, with the lossless disk, form a coupled cavity.  ``len_abs = 0``
+    leaves the bus lossless.
+
     ``extra['parts']`` holds the two preconditioner boxes of the composite
     (prec_fun_disk_idx.m): the bus box (all of x, the bus rows) and the disk's
-    bounding box, which are disjoint; ``extra['rhs_center']`` centres the
+    bounding box, which are disjoint; ``extra['part_forms']`` the form of
+    each part's 2-level circulant: face-averaged and exact across the bus
+    ('face_y', the waveguide form of Bragg_MOR_script.m:61-72) and the core
+    contrast on the disk box ('core'); ``extra['rhs_center']`` centres the
     Gaussian excitation on the bus."""
     R = radius / dx
     wb = int(round(width / dx))
@@ -283,6 +295,13 @@ def microdisk_bus(radius=10e-6, gap=200e-9, width=500e-9, thick=220e-9, dx=15e-9
     mask = np.zeros((L, M, N), bool)
     mask[:, :, z0:z0 + tz] = mask2[:, :, None]
     eps = np.where(mask, EPS_SI, 1.0).astype(np.complex128)
+    if len_abs > 0:
+        x = (np.arange(L) + 0.5) * dx
+        depth = np.maximum(np.maximum(len_abs - x, x - (L * dx - len_abs)), 0.0)
+        ramp = abs_im * (depth / len_abs) ** 2
+        bus = np.zeros((L, M, N), bool)
+        bus[:, yb:yb + wb, z0:z0 + tz] = True
+        eps = np.where(bus, EPS_SI - 1j * ramp[:, None, None], eps)
     p = Problem('microdisk_bus', L, M, N, dx, lam, mask, eps)
     p.box = _bbox(mask)
     dbox = _bbox(np.where(np.arange(M)[None, :, None] < yb, mask, False))
@@ -291,7 +310,8 @@ def microdisk_bus(radius=10e-6, gap=200e-9, width=500e-9, thick=220e-9, dx=15e-9
     # the grid without overlapping each other
     bbox = _smooth_box(bbox, p.shape, [dbox])
     dbox = _smooth_box(dbox, p.shape, [bbox])
-    p.extra = dict(radius=radius, gap=gap, width=width, parts=[bbox, dbox],
+    p.extra = dict(radius=radius, gap=gap, width=width, len_abs=len_abs, parts=[bbox, dbox],
+                   part_forms=('face_y', 'core'),
                    rhs_center=(yb + (wb - 1) / 2.0, z0 + (tz - 1) / 2.0))
     return p
 

@@ -18,7 +18,9 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <array>
 #include <complex>
+#include <map>
 #include <mutex>
 #include <cstdio>
 #include <cstdlib>
@@ -234,10 +236,12 @@ __global__ void k_step_end(const double* __restrict__ est, const int* __restrict
 // Publishes a chunk's status (the estimates of steps k0..k1-1, the steps
 // done and the stepping flags) into coherent pinned host memory, then
 // (after a system-scope fence) the chunk's sequence number, which the host
-// polls.
+// polls.  The number is the device counter *dseq plus one, stored back, so a
+// replayed graph of the chunk publishes a new number with the same
+// arguments; the host counts chunks alike.
 __global__ void k_publish(const double* __restrict__ hist, const int* __restrict__ kdone,
                           const int* __restrict__ iact, int np, int m, int k0, int k1, double* hhist, int* hkdone,
-                          int* hiact, int* hseq, int seq) {
+                          int* hiact, int* hseq, int* __restrict__ dseq) {
   const int w = k1 - k0;
   for (int e = threadIdx.x; e < np * w; e += blockDim.x) {
     const int p = e / w, k = k0 + e - p * w;
@@ -250,9 +254,15 @@ __global__ void k_publish(const double* __restrict__ hist, const int* __restrict
   __threadfence_system();
   __syncthreads();
   if (threadIdx.x == 0) {
+    const int seq = dseq[0] + 1;
+    dseq[0] = seq;
     __threadfence_system();
     reinterpret_cast<volatile int*>(hseq)[0] = seq;
   }
+}
+
+__global__ void k_set_int(int* __restrict__ a, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) a[0] = v;
 }
 
 // Coherent pinned status buffers of the per-step publication, kept in a
@@ -290,6 +300,39 @@ struct StatusLease {
   StatusLease& operator=(const StatusLease&) = delete;
 };
 }  // namespace
+
+// Workspaces of gmres_device for one shape (problems, length, restart,
+// Gram-Schmidt width, recycled capacity), kept by the problem set between
+// calls, and the executable graphs of Arnoldi step chunks captured over them.
+// Every kernel argument of a chunk is a pointer into these buffers, the
+// problem set's operator and preconditioner data, or a value fixed by the
+// graph key, so a graph replays the same launches the stream path issues.
+struct GmresWork {
+  int64_t n = 0;
+  int np = 0, m = 0, maxcols = 0, kcap = 0;
+  size_t elem = 0;
+  bool own_v = false;
+  DevBuf Vown, t1, t2, part, H, cs, sn, g, h1, h2, nr, yv;
+  DevBuf tolabs, invn, est, conv, kk, beta, bn, act, iact, ract, hist, kdone, dseq;
+  DevBuf Bst, a0, coef, kcv;
+  std::unique_ptr<StatusLease> pin;
+  int seq = 0;                          // chunks published (host count of *dseq)
+  // (k0, k1, kc, mode, second pass, recycled base column) -> graph of steps k0..k1-1 + publish
+  std::map<std::array<int64_t, 6>, hipGraphExec_t> graphs;
+  ~GmresWork() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+  }
+};
+void destroy_gmres_work(GmresWork* w) { delete w; }
+
+// VP_GMRES_GRAPH=1 captures each chunk of Arnoldi steps into a hipGraph the
+// first time the problem set runs it and replays the graph afterwards (later
+// restart cycles and later inner solves of a refinement run); 0 launches the
+// kernels on the stream every time.
+static bool gmres_graph_enabled() {
+  const char* e = std::getenv("VP_GMRES_GRAPH");
+  return e && std::atoi(e) != 0;
+}
 
 // Spins until the device has published chunk ``seq``.  Past 200 ms of
 // spinning it waits on the stream instead and, should the published values
@@ -703,29 +746,59 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   const bool left = ps.has_prec && prm.side == 0;
   const bool right = ps.has_prec && prm.side == 1;
 
-  DevBuf Vown;
-  if (!rec) Vown.alloc(size_t(m + 1) * n * np * sizeof(C<T>));
-  DevBuf t1(size_t(n) * np * sizeof(C<T>)), t2(size_t(n) * np * sizeof(C<T>));
-  DevBuf part(size_t(np) * maxcols * nblk * sizeof(double2));
-  DevBuf H(size_t(np) * (m + 1) * m * sizeof(double2)), cs(size_t(np) * m * sizeof(double)),
-      sn(size_t(np) * m * sizeof(double2)), g(size_t(np) * (m + 1) * sizeof(double2));
-  DevBuf h1(size_t(np) * maxcols * sizeof(double2)), h2(size_t(np) * maxcols * sizeof(double2)),
-      nr(size_t(np) * maxcols * sizeof(double2)), yv(size_t(np) * maxcols * sizeof(double2));
-  DevBuf tolabs(np * sizeof(double)), invn(np * sizeof(double)), est(np * sizeof(double)), conv(np * sizeof(int)),
-      kk(np * sizeof(int)), beta(np * sizeof(double)), bn(np * sizeof(double)), act(np * sizeof(int));
-  // recycled part: B = C^H Aop V, a0 = C^H r per cycle, update coefficients of U
-  DevBuf Bst, a0, coef, kcv;
+  // workspaces: the problem set's from an earlier call of the same shape, else new ones
+  GmresWork* W = ps.gw.get();
+  if (!W || W->n != n || W->np != np || W->m != m || W->maxcols != maxcols || W->kcap != kcap ||
+      W->elem != sizeof(C<T>) || W->own_v != !rec) {
+    ps.gw.reset();
+    ps.gw.reset(new GmresWork);
+    W = ps.gw.get();
+    W->n = n;
+    W->np = np;
+    W->m = m;
+    W->maxcols = maxcols;
+    W->kcap = kcap;
+    W->elem = sizeof(C<T>);
+    W->own_v = !rec;
+    if (!rec) W->Vown.alloc(size_t(m + 1) * n * np * sizeof(C<T>));
+    W->t1.alloc(size_t(n) * np * sizeof(C<T>));
+    W->t2.alloc(size_t(n) * np * sizeof(C<T>));
+    W->part.alloc(size_t(np) * maxcols * nblk * sizeof(double2));
+    W->H.alloc(size_t(np) * (m + 1) * m * sizeof(double2));
+    W->cs.alloc(size_t(np) * m * sizeof(double));
+    W->sn.alloc(size_t(np) * m * sizeof(double2));
+    W->g.alloc(size_t(np) * (m + 1) * sizeof(double2));
+    for (DevBuf* d : {&W->h1, &W->h2, &W->nr, &W->yv}) d->alloc(size_t(np) * maxcols * sizeof(double2));
+    for (DevBuf* d : {&W->tolabs, &W->invn, &W->est, &W->beta, &W->bn}) d->alloc(np * sizeof(double));
+    for (DevBuf* d : {&W->conv, &W->kk, &W->act, &W->iact, &W->ract, &W->kdone}) d->alloc(np * sizeof(int));
+    W->hist.alloc(size_t(np) * m * sizeof(double));
+    W->dseq.alloc(sizeof(int));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, W->dseq.as<int>(), 0);
+    VP_HIP(hipGetLastError());
+    // recycled part: B = C^H Aop V, a0 = C^H r per cycle, update coefficients of U
+    if (kcap > 0) {
+      W->Bst.alloc(size_t(np) * m * kcap * sizeof(double2));
+      W->a0.alloc(size_t(np) * maxcols * sizeof(double2));
+      W->coef.alloc(size_t(np) * maxcols * sizeof(double2));
+      W->kcv.alloc(np * sizeof(int));
+    }
+    // step status: estimates hist[p][k], steps done, stepping flags, sequence number
+    W->pin.reset(new StatusLease(size_t(np) * m * sizeof(double) + size_t(2 * np + 1) * sizeof(int)));
+    *reinterpret_cast<volatile int*>(static_cast<int*>(W->pin->p) + (size_t(np) * m * 2 + 2 * np)) = -1;
+    W->seq = 0;
+  }
+  DevBuf &t1 = W->t1, &t2 = W->t2, &part = W->part, &H = W->H, &cs = W->cs, &sn = W->sn, &g = W->g;
+  DevBuf &h1 = W->h1, &h2 = W->h2, &nr = W->nr, &yv = W->yv;
+  DevBuf &tolabs = W->tolabs, &invn = W->invn, &est = W->est, &conv = W->conv, &kk = W->kk, &beta = W->beta,
+         &bn = W->bn, &act = W->act;
+  DevBuf &Bst = W->Bst, &a0 = W->a0, &coef = W->coef, &kcv = W->kcv;
   if (kc > 0) {
-    Bst.alloc(size_t(np) * m * kcap * sizeof(double2));
-    a0.alloc(size_t(np) * maxcols * sizeof(double2));
-    coef.alloc(size_t(np) * maxcols * sizeof(double2));
-    kcv.alloc(np * sizeof(int));
     std::vector<int> hk(np, kc);
     VP_HIP(hipMemcpy(kcv.p, hk.data(), np * sizeof(int), hipMemcpyHostToDevice));
   }
   // Krylov storage: [C (kc of kcap columns, right-aligned) | V (m+1 columns)]
   const int64_t vprob = int64_t(kcap + m + 1) * n;
-  C<T>* Vb = (rec ? rec->V.as<C<T>>() : Vown.as<C<T>>()) + int64_t(kcap) * n;
+  C<T>* Vb = (rec ? rec->V.as<C<T>>() : W->Vown.as<C<T>>()) + int64_t(kcap) * n;
   C<T>* Cb = Vb - int64_t(kc) * n;   // Gram-Schmidt runs over columns Cb .. Vb + k
   ArnoldiState st{H.as<double2>(), cs.as<double>(), sn.as<double2>(), g.as<double2>(), h1.as<double2>(),
                   h2.as<double2>(), nr.as<double2>(), tolabs.as<double>(), invn.as<double>(), est.as<double>(),
@@ -746,7 +819,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
 
   std::vector<int> hact(np, 1);
   for (int p = 0; p < np; ++p) hact[p] = joins(p) ? 1 : 0;
-  DevBuf ract(np * sizeof(int));                 // per-step second-pass flags
+  DevBuf& ract = W->ract;                        // per-step second-pass flags
   const char* ecg = std::getenv("VP_GMRES_CGS2");  // 1: second pass at every step
   const bool cgs2_always = ecg && ecg[0] == '1';
   // One classical Gram-Schmidt pass per step, as pgmres_iter.m:106-107.
@@ -769,24 +842,22 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   }
   std::vector<double> hbn(np), hbeta(np);
   // per-step estimates and convergence flags: the device publishes them into
-  // coherent pinned host memory, followed by the step's sequence number, and
+  // coherent pinned host memory, followed by the chunk's sequence number, and
   // the host spins on that number (no copy kernels, no blocking wait)
   // chunk status: step estimates hist[p][k], steps done, stepping flags
-  StatusLease pin(size_t(np) * m * sizeof(double) + size_t(2 * np + 1) * sizeof(int));
-  double* hhist = static_cast<double*>(pin.p);
+  double* hhist = static_cast<double*>(W->pin->p);
   int* hkdone = reinterpret_cast<int*>(hhist + size_t(np) * m);
   int* hiact = hkdone + np;
   int* hseq = hiact + np;
-  *reinterpret_cast<volatile int*>(hseq) = -1;
   void* dpin = nullptr;
-  VP_HIP(hipHostGetDevicePointer(&dpin, pin.p, 0));
+  VP_HIP(hipHostGetDevicePointer(&dpin, W->pin->p, 0));
   double* dhhist = static_cast<double*>(dpin);
   int* dhkdone = reinterpret_cast<int*>(dhhist + size_t(np) * m);
   int* dhiact = dhkdone + np;
   int* dhseq = dhiact + np;
-  DevBuf hist(size_t(np) * m * sizeof(double)), kdone(np * sizeof(int));
-  int seq = 0;
+  DevBuf &hist = W->hist, &kdone = W->kdone;
   const int chunk = gmres_chunk();
+  const bool use_graph = gmres_graph_enabled() && s != nullptr && !prof_enabled_on(s);
   VP_HIP(hipMemcpy(hbn.data(), bn.p, np * sizeof(double), hipMemcpyDeviceToHost));
   // initial residual into V_0 slot of each problem
   auto residual_into_V0 = [&](bool x_nonzero) {
@@ -961,9 +1032,8 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       if (hact[p]) res.iter_outer[p] = outit;
       hk[p] = 0;
     }
-    DevBuf iact(np * sizeof(int));
-    VP_HIP(hipMemcpyAsync(iact.p, inner.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
-    int* dia = iact.as<int>();
+    VP_HIP(hipMemcpyAsync(W->iact.p, inner.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    int* dia = W->iact.as<int>();
     hipLaunchKernelGGL(k_scale<T>, dim3(grid1(n), np), dim3(256), 0, s, Vb, vprob, n, beta.as<double>(), 1, dia);
     hipLaunchKernelGGL(k_gmres_init, dim3(np), dim3(64), 0, s, st, beta.as<double>(), np, dia);
     VP_HIP(hipGetLastError());
@@ -1012,13 +1082,45 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
                          hist.as<double>(), kdone.as<int>(), k, m, np);
       VP_HIP(hipGetLastError());
     };
+    // steps k0..k1-1 and the chunk's publication, on the stream or as the
+    // problem set's captured graph of that chunk
+    auto issue_chunk = [&](int k0, int k1) {
+      for (int k = k0; k < k1; ++k) step(k);
+      hipLaunchKernelGGL(k_publish, dim3(1), dim3(256), 0, s, hist.as<double>(), kdone.as<int>(), dia, np, m, k0, k1,
+                         dhhist, dhkdone, dhiact, dhseq, W->dseq.as<int>());
+      VP_HIP(hipGetLastError());
+    };
+    auto run_chunk = [&](int k0, int k1) {
+      if (!use_graph) {
+        issue_chunk(k0, k1);
+        return;
+      }
+      const std::array<int64_t, 6> key{k0, k1, kc, right ? 2 : (left ? 1 : 0), second_pass ? 1 : 0,
+                                       static_cast<int64_t>(reinterpret_cast<intptr_t>(Vb))};
+      auto it = W->graphs.find(key);
+      if (it == W->graphs.end()) {
+        VP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipGraph_t gr = nullptr;
+        try {
+          issue_chunk(k0, k1);
+        } catch (...) {
+          (void)hipStreamEndCapture(s, &gr);
+          if (gr) (void)hipGraphDestroy(gr);
+          throw;
+        }
+        VP_HIP(hipStreamEndCapture(s, &gr));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        VP_HIP(e);
+        it = W->graphs.emplace(key, ex).first;
+      }
+      VP_HIP(hipGraphLaunch(it->second, s));
+    };
     for (int k0 = 0; k0 < m;) {
       const int k1 = std::min(m, k0 + chunk);
-      for (int k = k0; k < k1; ++k) step(k);
-      ++seq;
-      hipLaunchKernelGGL(k_publish, dim3(1), dim3(256), 0, s, hist.as<double>(), kdone.as<int>(), dia, np, m, k0, k1,
-                         dhhist, dhkdone, dhiact, dhseq, seq);
-      VP_HIP(hipGetLastError());
+      run_chunk(k0, k1);
+      const int seq = ++W->seq;
       wait_published(hseq, seq, s, [&] {
         VP_HIP(hipMemcpy(hhist, hist.p, size_t(np) * m * sizeof(double), hipMemcpyDeviceToHost));
         VP_HIP(hipMemcpy(hkdone, kdone.p, np * sizeof(int), hipMemcpyDeviceToHost));
@@ -1045,7 +1147,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     // solve the least squares and update x for every problem active in this cycle
     VP_HIP(hipMemcpyAsync(kk.p, hk.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
     update_x(dact);
-    VP_HIP(hipStreamSynchronize(s));   // iact is released at the end of this cycle
+    VP_HIP(hipStreamSynchronize(s));   // the update is done before the host reuses the cycle's buffers
     if (progress_enabled()) {
       double worst = 0.0;
       int nact = 0;

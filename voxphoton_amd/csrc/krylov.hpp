@@ -22,6 +22,15 @@ inline double relres_of(double rnorm, double bnorm) {
 // and per refinement step, for long solves whose callers want to see them move.
 bool progress_enabled();
 
+// Device workspaces of gmres_device kept by a problem set between calls (the
+// inner solves of one refinement run) with the hipGraphs of Arnoldi step
+// chunks captured over them (krylov.hip).
+struct GmresWork;
+void destroy_gmres_work(GmresWork* w);
+struct GmresWorkDeleter {
+  void operator()(GmresWork* w) const { destroy_gmres_work(w); }
+};
+
 // One set of independent problems of identical geometry solved together.
 // Vector p of a batch sits at base + p * n.
 template <typename T>
@@ -48,6 +57,7 @@ struct ProblemSet {
   bool has_prec = false;
   std::vector<std::unique_ptr<PrecPart>> pparts;
   DevBuf active;                  // int[nprob]
+  std::unique_ptr<GmresWork, GmresWorkDeleter> gw;   // gmres_device workspaces and step graphs
 };
 
 template <typename T>

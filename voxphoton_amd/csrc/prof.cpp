@@ -92,6 +92,12 @@ void prof_enable(hipStream_t s, bool on) {
   st.on = on;
 }
 
+bool prof_enabled_on(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_prof.find(s);
+  return it != g_prof.end() && it->second.on;
+}
+
 void prof_stats(hipStream_t s, int kid, long long* launches, double* ms, bool reset) {
   (void)hipStreamSynchronize(s);
   std::lock_guard<std::mutex> lk(g_mu);

@@ -115,6 +115,7 @@ const char* kernel_id_name(int kid);
 void prof_begin(hipStream_t s, int kid, void** token);
 void prof_end(hipStream_t s, void* token);
 void prof_enable(hipStream_t s, bool on);
+bool prof_enabled_on(hipStream_t s);
 void prof_stats(hipStream_t s, int kid, long long* launches, double* ms, bool reset);
 struct ProfScope {
   hipStream_t s;
