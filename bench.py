@@ -81,8 +81,8 @@ WORKLOADS = {
              '1400x1500x30 (63 M voxels), y-polarised Gaussian (w0 300 nm) centred on the bus; block-Jacobi '
              'composite of 2-level circulants: face-averaged and exact across the bus on the bus box, core '
              'contrast on the disk box',
-        tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=1, max_outer=10, prec='composite', damping=0.0,
-        fields=False, cpu_baseline=False),
+        tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=1, max_outer=40, outer_restart=30, prec='composite',
+        damping=0.0, fields=False, cpu_baseline=False),
     'bragg_20nm': dict(
         desc='Bragg grating (100 periods) at 20 nm voxels, grid 1800x50x22 (FFT 3600x100x44), lambda 1550 nm',
         tol=1e-6, restart=100, inner_tol=1e-3, inner_maxit=6, max_outer=12, prec='face', axis=1, damping=0.0),
@@ -260,9 +260,13 @@ def main():
 
     def solve_mixed():
         info = L.GmresInfo()
-        L.check(L.lib.vp_solve_ir(op.handle, hp, L.VP_PREC_RIGHT, db.ptr, dx.ptr, L.VP_MEM_DEVICE, wl['tol'],
-                                  wl['restart'], wl['inner_tol'], wl['inner_maxit'], wl['max_outer'],
-                                  ctypes.byref(info)))
+        args = [op.handle, hp, L.VP_PREC_RIGHT, db.ptr, dx.ptr, L.VP_MEM_DEVICE, wl['tol'], wl['restart'],
+                wl['inner_tol'], wl['inner_maxit'], wl['max_outer']]
+        if wl.get('outer_restart'):
+            # flexible refinement: FGMRES in FP64 over the inner solves
+            L.check(L.lib.vp_solve_fgmres_ir(*args, wl['outer_restart'], ctypes.byref(info)))
+        else:
+            L.check(L.lib.vp_solve_ir(*args, ctypes.byref(info)))
         return info
 
     for _ in range(args.warmup):
@@ -430,6 +434,8 @@ def main():
         'config': {'workload': args.workload, 'description': wl['desc'], 'unknowns': n, 'tol': wl['tol'],
                    'restart': wl['restart'], 'inner_tol': wl['inner_tol'], 'inner_maxit': wl['inner_maxit'],
                    'preconditioner': None if prec is None else pdesc, 'preconditioning': 'right',
+                   'refinement': (f'flexible (FGMRES in FP64 over the inner solves, cycles of {wl["outer_restart"]})'
+                                  if wl.get('outer_restart') else 'one correction per step'),
                    'parallelism': 'single GPU'},
         'relres_worst': worst,
         'ir_steps': [i.ir_steps for i in infos],

@@ -331,6 +331,31 @@ int vp_solve_ir_batched(const vp_op* ops, const vp_prec* precs, int nsolve, int 
                         double tol, int restart, double inner_tol, int inner_maxit, int max_outer,
                         vp_gmres_info* infos);
 
+/*
+ * Flexible refinement: vp_solve_ir's inner FP32 solves (same restart,
+ * inner_tol, need rule and recycled Krylov space) used as the variable
+ * preconditioner of an FP64 flexible GMRES (FGMRES).  Outer step j solves
+ * (A/s) z_j = v_j in FP32, forms w = A z_j / s in FP64, orthogonalises it
+ * against the outer basis with two classical Gram-Schmidt passes and
+ * minimises ||b - A x|| over every correction of the cycle (x += Z y), where
+ * plain refinement takes each correction with weight one.  Cycles of
+ * outer_restart steps restart from the true FP64 residual; a solve stops at
+ * ||b - A x|| / ||b|| <= tol (recomputed in FP64) or after max_outer inner
+ * solves.  The history (info->resvec) holds the relative residual after
+ * every inner solve: the FGMRES estimate inside a cycle, the recomputed
+ * residual at its end.  Costs one FP64 matvec more per cycle than
+ * vp_solve_ir, and (outer_restart + 1) complex128 plus outer_restart
+ * complex64 vectors per solve.  No counterpart in the reference (which
+ * solves in FP64 only).
+ */
+int vp_solve_fgmres_ir(vp_op op, vp_prec M, int side, const void* b, void* x, int mem, double tol, int restart,
+                       double inner_tol, int inner_maxit, int max_outer, int outer_restart, vp_gmres_info* info);
+
+/* Batched flexible refinement (vp_solve_ir_batched's arguments and layout). */
+int vp_solve_fgmres_ir_batched(const vp_op* ops, const vp_prec* precs, int nsolve, int side, const void* b,
+                               void* x, double tol, int restart, double inner_tol, int inner_maxit, int max_outer,
+                               int outer_restart, vp_gmres_info* infos);
+
 /* ---- request server (continuous batching) ----------------------------------
  * No counterpart in the reference, whose scripts solve one wavelength after
  * another (Bragg_MOR_script.m:88-95, example_Bragg_1500nm_to_1600nm.m:85-114).

@@ -16,8 +16,9 @@ against the FP64 oracle.
   without changing the others.
 * the microdisk with a bus at 15 nm, 1400x1500x30 (64.9 M unknowns): the
   matvec against direct Toeplitz sums at sampled rows, FP32 against FP64,
-  and against the reference's 2L x 2M x 2N embedding; refinement steps of
-  the mixed solve whose reported residual is the true FP64 residual.
+  and against the reference's 2L x 2M x 2N embedding; the mixed solve to
+  1e-6 (flexible refinement), whose reported residual is the true FP64
+  residual.
 
 The oracle's tensor is the device-assembled Toeplitz tensor (its cubature is
 checked against the oracle's at small grids); everything after it (the
@@ -377,7 +378,9 @@ def test_microdisk15_full_size(vp_ctx, monkeypatch):
     op_ref.close()
     del op_ref, y32
 
-    # mixed solve: a few refinement steps; reported residual = true residual
+    # mixed solve to 1e-6 as the workload runs it: GMRES(100) inner solves with
+    # the block-Jacobi composite inside flexible FP64 refinement; the
+    # reported residual must be the true FP64 residual of the checked matvec
     wl = dict(bench.WORKLOADS['microdisk_15nm'])
     t = time.perf_counter()
     P, desc = bench.make_preconditioner(prob, wl, vie, configs)
@@ -385,14 +388,14 @@ def test_microdisk15_full_size(vp_ctx, monkeypatch):
     b = prob.gaussian_rhs('y')
     t = time.perf_counter()
     xm, sinfo = vie.solve_ir(op, b, tol=wl['tol'], restart=wl['restart'], inner_tol=wl['inner_tol'],
-                             inner_maxit=wl['inner_maxit'], max_outer=3, M=P, side='right')
+                             inner_maxit=wl['inner_maxit'], max_outer=wl['max_outer'], M=P, side='right',
+                             outer_restart=wl['outer_restart'])
     dt = time.perf_counter() - t
     free1, _ = vp_ctx.mem_info()
     true = float(np.linalg.norm(b - op.apply(xm)) / np.linalg.norm(b))
-    _say(f'mixed GMRES({wl["restart"]}) x {sinfo["ir_steps"]} refinement steps ({sinfo["inner_iters"]} inner its, '
-         f'{dt:.1f} s): relres {sinfo["relres"]:.4e}, true FP64 relres {true:.4e}, history '
-         f'{[float("%.3g" % v) for v in sinfo["history"]]}; device memory in use {(total - free1) / 2**30:.1f} GiB '
-         f'of {total / 2**30:.0f} GiB ({(total - free0) / 2**30:.1f} GiB before)')
+    _say(f'mixed GMRES({wl["restart"]}), flexible refinement: {sinfo["ir_steps"]} inner solves '
+         f'({sinfo["inner_iters"]} inner its, {dt:.1f} s): relres {sinfo["relres"]:.4e}, true FP64 relres '
+         f'{true:.4e}, history {[float("%.3g" % v) for v in sinfo["history"]]}; device memory in use '
+         f'{(total - free1) / 2**30:.1f} GiB of {total / 2**30:.0f} GiB ({(total - free0) / 2**30:.1f} GiB before)')
     assert abs(true - sinfo['relres']) <= 1e-9 * true
-    assert sinfo['history'][-1] < sinfo['history'][0]
-    assert sinfo['relres'] < 0.5
+    assert sinfo['flag'] == 0 and true <= wl['tol']

@@ -450,6 +450,49 @@ def test_mixed_precision_ir_reaches_fp64_tolerance(vp_ctx):
     assert abs(true_rel - info['relres']) <= 1e-3 * true_rel + 1e-15
 
 
+@pytest.mark.parametrize('outer_m', [30, 3])
+def test_flexible_refinement_reaches_fp64_tolerance(vp_ctx, outer_m):
+    # vp_solve_fgmres_ir: FGMRES in FP64 over the inner FP32 solves.  The true
+    # residual (recomputed by the oracle) meets the tolerance and equals the
+    # reported one; the history falls monotonically inside a cycle (the outer
+    # step minimises the residual over all corrections of the cycle); cycles
+    # of 3 outer steps restart from the true residual; with weak inner solves
+    # it needs no more steps than plain refinement; batched = solo bit for bit.
+    from voxphoton_amd.vie import NOperator, Preconditioner, solve_ir, solve_ir_batched
+    L, M, N = 12, 6, 4
+    G, Mc, Gram, idx, b, dx, ko = _physical(L, M, N, seed=3)
+    op = NOperator.from_toeplitz(G, None, Mc, Gram, idx, ctx=vp_ctx)
+    P = Preconditioner.level2(G, Mc.flat[0], 1.0, Gram, idx, ctx=vp_ctx)
+    fN = ovie.fN_from_toeplitz(G)
+    ones = np.ones(Mc.shape, complex)
+    # weak inner solves (restart 6, one cycle) so that many outer steps are needed
+    prm = dict(tol=1e-10, restart=6, inner_tol=1e-2, inner_maxit=1, max_outer=60, M=P, side='right')
+    x, info = solve_ir(op, b, outer_restart=outer_m, **prm)
+    r = b - ovie.mv_AN_const(x, fN, ones, Mc, Gram, 'notransp', idx)
+    true_rel = np.linalg.norm(r) / np.linalg.norm(b)
+    x_ir, info_ir = solve_ir(op, b, **prm)
+    print(f'flexible refinement (cycles of {outer_m}): {info["ir_steps"]} steps, {info["inner_iters"]} inner its, '
+          f'relres {info["relres"]:.3e}, oracle {true_rel:.3e}; plain refinement {info_ir["ir_steps"]} steps '
+          f'({info_ir["relres"]:.3e})', flush=True)
+    assert info['flag'] == 0 and true_rel <= 1e-10 * 1.01
+    assert abs(true_rel - info['relres']) <= 1e-3 * true_rel
+    h = np.asarray(info['history'])
+    assert h[0] == 1.0 and len(h) == info['ir_steps'] + 1
+    if outer_m == 30:
+        # estimates fall monotonically; the last entry is the recomputed residual
+        assert np.all(np.diff(h[:-1]) <= 1e-12 * h[:-2])
+        assert info['ir_steps'] <= info_ir['ir_steps']
+    rng = np.random.default_rng(4)
+    B = np.array([b, (rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size)) * np.abs(b).max()])
+    bprm = dict(prm)
+    bprm.pop('M')
+    X, infos = solve_ir_batched([op, op], [P, P], B, outer_restart=outer_m, **bprm)
+    for i in range(2):
+        xs, inf_s = solve_ir(op, B[i], outer_restart=outer_m, **prm)
+        assert infos[i]['flag'] == 0 and inf_s['flag'] == 0
+        assert np.array_equal(X[i], xs) and infos[i]['ir_steps'] == inf_s['ir_steps']
+
+
 @pytest.mark.parametrize('variant', ['fused', 'split'])
 def test_batched_ir_alone_equals_in_batch(vp_ctx, monkeypatch, variant):
     from voxphoton_amd.vie import NOperator, Preconditioner, solve_ir, solve_ir_batched

@@ -106,6 +106,10 @@ SIGNATURES = {
                             P(GmresInfo)]),
     'vp_solve_ir_batched': (c_int, [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_dbl, c_int, c_dbl, c_int, c_int,
                                     P(GmresInfo)]),
+    'vp_solve_fgmres_ir': (c_int, [c_vp, c_vp, c_int, c_vp, c_vp, c_int, c_dbl, c_int, c_dbl, c_int, c_int, c_int,
+                                   P(GmresInfo)]),
+    'vp_solve_fgmres_ir_batched': (c_int, [c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_dbl, c_int, c_dbl, c_int, c_int,
+                                           c_int, P(GmresInfo)]),
     'vp_server_create': (c_int, [c_vp, c_i64, c_int, P(c_vp)]),
     'vp_server_submit': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, P(SolveParams), P(c_i64)]),
     'vp_server_poll': (c_int, [c_vp, c_i64, P(c_int), P(GmresInfo)]),

@@ -420,9 +420,11 @@ bool ir_step_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2*
   return true;
 }
 
+// outer_m = 0: one correction per refinement step (ir_step_set); > 0:
+// flexible refinement in cycles of outer_m steps (fgmres_ir_set)
 static void solve_ir_sets(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2* b, double2* x,
                           double scale32, double tol, int restart, double inner_tol, int inner_maxit,
-                          int max_outer, int side, IrResult& out) {
+                          int max_outer, int side, int outer_m, IrResult& out) {
   const int np = p64.nprob;
   VP_HIP(hipMemsetAsync(x, 0, size_t(p64.n) * np * sizeof(double2), p64.stream));
   std::vector<IrProblem> st(np);
@@ -435,7 +437,12 @@ static void solve_ir_sets(ProblemSet<double>& p64, ProblemSet<float>& p32, const
     rec.alloc(p64.n, np, restart, sizeof(float2));
     prec = &rec;
   }
-  while (ir_step_set(p64, p32, b, x, scale32, tol, restart, inner_tol, inner_maxit, max_outer, side, st, prec)) {
+  if (outer_m > 0) {
+    fgmres_ir_set(p64, p32, b, x, scale32, tol, restart, inner_tol, inner_maxit, max_outer, outer_m, side, st,
+                  prec);
+  } else {
+    while (ir_step_set(p64, p32, b, x, scale32, tol, restart, inner_tol, inner_maxit, max_outer, side, st, prec)) {
+    }
   }
   out.flag.resize(np);
   out.steps.resize(np);
@@ -936,9 +943,10 @@ int vp_gmres(vp_op op, vp_prec M, int side, const void* b, const void* x0, void*
 
 static void ir_common(vp_op_s* const* ops, vp_prec_s* const* precs, int np, int side, const double2* db,
                       double2* dx, double tol, int restart, double inner_tol, int inner_maxit, int max_outer,
-                      vp_gmres_info* infos) {
+                      int outer_m, vp_gmres_info* infos) {
   VP_REQUIRE(restart > 0 && restart <= 1000 && inner_maxit > 0 && max_outer >= 0 && tol > 0 && inner_tol > 0,
              ERR_INVALID, "bad solve parameters");
+  VP_REQUIRE(outer_m >= 0 && outer_m <= 1000, ERR_INVALID, "outer restart must be 0..1000");
   for (int p = 0; p < np; ++p)
     VP_REQUIRE(ops[p]->p64.present && ops[p]->p32.present, ERR_INVALID,
                "mixed precision needs operators created with VP_FP64|VP_FP32");
@@ -948,7 +956,8 @@ static void ir_common(vp_op_s* const* ops, vp_prec_s* const* precs, int np, int 
   ProblemSet<float> p32;
   make_problem_set<float>(p32, ops, precs, np);
   IrResult r;
-  solve_ir_sets(p64, p32, db, dx, ops[0]->scale32, tol, restart, inner_tol, inner_maxit, max_outer, side, r);
+  solve_ir_sets(p64, p32, db, dx, ops[0]->scale32, tol, restart, inner_tol, inner_maxit, max_outer, side, outer_m,
+                r);
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (infos)
     for (int p = 0; p < np; ++p) {
@@ -969,9 +978,8 @@ static void ir_common(vp_op_s* const* ops, vp_prec_s* const* precs, int np, int 
     }
 }
 
-int vp_solve_ir(vp_op op, vp_prec M, int side, const void* b, void* x, int mem, double tol, int restart,
-                double inner_tol, int inner_maxit, int max_outer, vp_gmres_info* info) {
-  VP_API_BEGIN
+static void solve_one(vp_op op, vp_prec M, int side, const void* b, void* x, int mem, double tol, int restart,
+                      double inner_tol, int inner_maxit, int max_outer, int outer_m, vp_gmres_info* info) {
   VP_REQUIRE(op && b && x, ERR_INVALID, "op/b/x is NULL");
   VP_REQUIRE(side == VP_PREC_LEFT || side == VP_PREC_RIGHT, ERR_INVALID, "side must be LEFT or RIGHT");
   VP_HIP(hipSetDevice(op->ctx->device));
@@ -981,22 +989,20 @@ int vp_solve_ir(vp_op op, vp_prec M, int side, const void* b, void* x, int mem, 
   vp_prec_s* precs[1] = {M};
   if (mem == VP_MEM_DEVICE) {
     ir_common(ops, M ? precs : nullptr, 1, side, static_cast<const double2*>(b), static_cast<double2*>(x), tol,
-              restart, inner_tol, inner_maxit, max_outer, info);
+              restart, inner_tol, inner_maxit, max_outer, outer_m, info);
   } else {
     DevBuf db(n * sizeof(double2)), dx(n * sizeof(double2));
     VP_HIP(hipMemcpyAsync(db.p, b, n * sizeof(double2), hipMemcpyHostToDevice, s));
     ir_common(ops, M ? precs : nullptr, 1, side, db.as<double2>(), dx.as<double2>(), tol, restart, inner_tol,
-              inner_maxit, max_outer, info);
+              inner_maxit, max_outer, outer_m, info);
     VP_HIP(hipMemcpyAsync(x, dx.p, n * sizeof(double2), hipMemcpyDeviceToHost, s));
     VP_HIP(hipStreamSynchronize(s));
   }
-  VP_API_END
 }
 
-int vp_solve_ir_batched(const vp_op* ops, const vp_prec* precs, int nsolve, int side, const void* b, void* x,
-                        double tol, int restart, double inner_tol, int inner_maxit, int max_outer,
-                        vp_gmres_info* infos) {
-  VP_API_BEGIN
+static void solve_batched(const vp_op* ops, const vp_prec* precs, int nsolve, int side, const void* b, void* x,
+                          double tol, int restart, double inner_tol, int inner_maxit, int max_outer, int outer_m,
+                          vp_gmres_info* infos) {
   VP_REQUIRE(ops && nsolve > 0 && b && x, ERR_INVALID, "bad batched arguments");
   VP_REQUIRE(side == VP_PREC_LEFT || side == VP_PREC_RIGHT, ERR_INVALID, "side must be LEFT or RIGHT");
   VP_REQUIRE(ops[0], ERR_INVALID, "ops[0] is NULL");
@@ -1012,7 +1018,39 @@ int vp_solve_ir_batched(const vp_op* ops, const vp_prec* precs, int nsolve, int 
   std::vector<vp_prec_s*> q;
   if (anyp) q.assign(precs, precs + nsolve);
   ir_common(o.data(), anyp ? q.data() : nullptr, nsolve, side, static_cast<const double2*>(b),
-            static_cast<double2*>(x), tol, restart, inner_tol, inner_maxit, max_outer, infos);
+            static_cast<double2*>(x), tol, restart, inner_tol, inner_maxit, max_outer, outer_m, infos);
+}
+
+int vp_solve_ir(vp_op op, vp_prec M, int side, const void* b, void* x, int mem, double tol, int restart,
+                double inner_tol, int inner_maxit, int max_outer, vp_gmres_info* info) {
+  VP_API_BEGIN
+  solve_one(op, M, side, b, x, mem, tol, restart, inner_tol, inner_maxit, max_outer, 0, info);
+  VP_API_END
+}
+
+int vp_solve_ir_batched(const vp_op* ops, const vp_prec* precs, int nsolve, int side, const void* b, void* x,
+                        double tol, int restart, double inner_tol, int inner_maxit, int max_outer,
+                        vp_gmres_info* infos) {
+  VP_API_BEGIN
+  solve_batched(ops, precs, nsolve, side, b, x, tol, restart, inner_tol, inner_maxit, max_outer, 0, infos);
+  VP_API_END
+}
+
+int vp_solve_fgmres_ir(vp_op op, vp_prec M, int side, const void* b, void* x, int mem, double tol, int restart,
+                       double inner_tol, int inner_maxit, int max_outer, int outer_restart, vp_gmres_info* info) {
+  VP_API_BEGIN
+  VP_REQUIRE(outer_restart > 0, ERR_INVALID, "outer_restart must be positive");
+  solve_one(op, M, side, b, x, mem, tol, restart, inner_tol, inner_maxit, max_outer, outer_restart, info);
+  VP_API_END
+}
+
+int vp_solve_fgmres_ir_batched(const vp_op* ops, const vp_prec* precs, int nsolve, int side, const void* b,
+                               void* x, double tol, int restart, double inner_tol, int inner_maxit, int max_outer,
+                               int outer_restart, vp_gmres_info* infos) {
+  VP_API_BEGIN
+  VP_REQUIRE(outer_restart > 0, ERR_INVALID, "outer_restart must be positive");
+  solve_batched(ops, precs, nsolve, side, b, x, tol, restart, inner_tol, inner_maxit, max_outer, outer_restart,
+                infos);
   VP_API_END
 }
 

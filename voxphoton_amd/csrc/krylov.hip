@@ -22,6 +22,7 @@
 #include <complex>
 #include <map>
 #include <mutex>
+#include <utility>
 #include <cstdio>
 #include <cstdlib>
 #include "krylov.hpp"
@@ -71,11 +72,12 @@ __device__ __forceinline__ int64_t krow(int64_t base, int t, int64_t n) {
   return min<int64_t>(base + t * KT + threadIdx.x, n - 1);
 }
 
-// partial dots of w (registers) with columns 0..k-1 of V
+// partial dots of w (registers) with columns c0..k-1 of V (c0 a multiple of
+// JC); each column's partial is the same whichever range it is computed in
 template <typename T>
 __device__ __forceinline__ void dots_core(const C<T>* __restrict__ Vp, int64_t vcol, int k, const double2 (&wv)[RPT],
-                                          int64_t base, int64_t n, double2* part, int nblk) {
-  for (int j0 = 0; j0 < k; j0 += JC) {
+                                          int64_t base, int64_t n, double2* part, int nblk, int c0 = 0) {
+  for (int j0 = c0; j0 < k; j0 += JC) {
     C<T> v[JC][RPT];
 #pragma unroll
     for (int jj = 0; jj < JC; ++jj) {
@@ -99,14 +101,19 @@ __device__ __forceinline__ void dots_core(const C<T>* __restrict__ Vp, int64_t v
   }
 }
 
-// with_norm: column k of the partials also gets ||w||^2 (in .x).
+// with_norm: column k of the partials also gets ||w||^2 (in .x).  Block z
+// takes columns [z*cpg, (z+1)*cpg) (cpg a multiple of JC; one group of all
+// columns when gridDim.z = 1), so short vectors still fill the device; the
+// partials do not depend on the split.
 template <typename T>
 __global__ void __launch_bounds__(KT) k_dots(const C<T>* __restrict__ V, int64_t vcol, int64_t vprob, int k,
                                             const C<T>* __restrict__ w, int64_t wprob, int64_t n,
                                             double2* __restrict__ partials, int maxcols,
-                                            const int* __restrict__ active, int with_norm) {
+                                            const int* __restrict__ active, int with_norm, int cpg = 0) {
   const int p = blockIdx.y;
   if (active && !active[p]) return;
+  const int c0 = cpg > 0 ? int(blockIdx.z) * cpg : 0;
+  const int c1 = cpg > 0 ? min(k, c0 + cpg) : k;
   const C<T>* Vp = V + p * vprob;
   const C<T>* wp = w + p * wprob;
   const int nblk = gridDim.x;
@@ -118,13 +125,23 @@ __global__ void __launch_bounds__(KT) k_dots(const C<T>* __restrict__ V, int64_t
     wv[t] = base + t * KT + threadIdx.x < n ? to_d(x) : make_double2(0.0, 0.0);
   }
   double2* part = partials + (int64_t(p) * maxcols) * nblk + blockIdx.x;
-  dots_core<T>(Vp, vcol, k, wv, base, n, part, nblk);
-  if (with_norm) {
+  dots_core<T>(Vp, vcol, c1, wv, base, n, part, nblk, c0);
+  if (with_norm && blockIdx.z == 0) {
     double2 acc[1] = {make_double2(0.0, 0.0)};
 #pragma unroll
     for (int t = 0; t < RPT; ++t) acc[0].x += wv[t].x * wv[t].x + wv[t].y * wv[t].y;
     block_reduce_store<1>(acc, part + int64_t(k) * nblk, nblk, 1);
   }
+}
+
+// Column groups of k_dots: about 1024 workgroups in all, never more groups
+// than column pairs.  Returns (groups, columns per group).
+static std::pair<int, int> dots_split(int ncols, int nblk, int np) {
+  const int pairs = std::max(1, (ncols + JC - 1) / JC);
+  const int want = std::max(1, 1024 / std::max(1, nblk * np));
+  const int g = std::min(pairs, want);
+  const int cpg = ((pairs + g - 1) / g) * JC;
+  return {std::max(1, (ncols + cpg - 1) / cpg), cpg};
 }
 
 // Selective reorthogonalisation (Daniel-Gragg-Kaufman-Stewart): a second
@@ -879,8 +896,9 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     }
     if (kc > 0) {
       // GCRO: a0 = C^H r, r -= C a0 (the U part of the update minimises it away)
-      hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Cb, n, vprob, kc, v0, vprob, n,
-                         part.as<double2>(), maxcols, dact, 0);
+      const auto sp = dots_split(kc, nblk, np);
+      hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np, sp.first), dim3(KT), 0, s, Cb, n, vprob, kc, v0, vprob, n,
+                         part.as<double2>(), maxcols, dact, 0, sp.second);
       hipLaunchKernelGGL(k_reduce_cols, dim3(kc, np), dim3(KT), 0, s, part.as<double2>(), nblk, kc, maxcols,
                          a0.as<double2>(), dact);
       hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Cb, n, vprob, kc, a0.as<double2>(), v0,
@@ -1054,8 +1072,9 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       // first classical Gram-Schmidt pass over [C, V]: h1 = [C V]^H w (and ||w||^2),
       // w' = w - [C V] h1, ||w'||^2
       const int nc = kc + k + 1;
-      hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Cb, n, vprob, nc, w, vprob, n,
-                         part.as<double2>(), maxcols, dia, 1);
+      const auto sp = dots_split(nc, nblk, np);
+      hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np, sp.first), dim3(KT), 0, s, Cb, n, vprob, nc, w, vprob, n,
+                         part.as<double2>(), maxcols, dia, 1, sp.second);
       hipLaunchKernelGGL(k_reduce_cols, dim3(nc + 1, np), dim3(KT), 0, s, part.as<double2>(), nblk, nc + 1, maxcols,
                          h1.as<double2>(), dia);
       hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Cb, n, vprob, nc, h1.as<double2>(), w,
@@ -1066,8 +1085,8 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       hipLaunchKernelGGL(k_dgks, dim3((np + 63) / 64), dim3(64), 0, s, h1.as<double2>(), nr.as<double2>(), maxcols,
                          nc - 1, dia, ract.as<int>(), np, cgs2_always ? 1 : 0, eta2);
       if (second_pass) {
-        hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Cb, n, vprob, nc, w, vprob, n,
-                           part.as<double2>(), maxcols, ract.as<int>(), 0);
+        hipLaunchKernelGGL(k_dots<T>, dim3(nblk, np, sp.first), dim3(KT), 0, s, Cb, n, vprob, nc, w, vprob, n,
+                           part.as<double2>(), maxcols, ract.as<int>(), 0, sp.second);
         hipLaunchKernelGGL(k_reduce_cols, dim3(nc, np), dim3(KT), 0, s, part.as<double2>(), nblk, nc,
                            maxcols, h2.as<double2>(), ract.as<int>());
         hipLaunchKernelGGL(k_axpy_dots<T>, dim3(nblk, np), dim3(KT), 0, s, Cb, n, vprob, nc, h2.as<double2>(),
@@ -1191,5 +1210,290 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
 }
 template void gmres_device<float>(ProblemSet<float>&, const float2*, float2*, const GmresParams&, GmresResult&);
 template void gmres_device<double>(ProblemSet<double>&, const double2*, double2*, const GmresParams&, GmresResult&);
+
+// ----- flexible outer refinement ---------------------------------------------
+// o[p] = v[p] rounded to complex64
+__global__ void k_cvt_d2f(const double2* __restrict__ v, int64_t vprob, float2* __restrict__ o, int64_t oprob,
+                          int64_t n, const int* __restrict__ active) {
+  const int p = blockIdx.y;
+  if (active && !active[p]) return;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double2 a = v[p * vprob + i];
+    o[p * oprob + i] = make_float2(float(a.x), float(a.y));
+  }
+}
+
+// o[p] = scale * z[p] in complex128
+__global__ void k_cvt_f2d(const float2* __restrict__ z, int64_t zprob, double2* __restrict__ o, int64_t oprob,
+                          int64_t n, double scale, const int* __restrict__ active) {
+  const int p = blockIdx.y;
+  if (active && !active[p]) return;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const float2 a = z[p * zprob + i];
+    o[p * oprob + i] = make_double2(scale * double(a.x), scale * double(a.y));
+  }
+}
+
+// x[p] += scale * sum_{j < kk[p]} y[p][j] Z[p][j]: complex64 columns, FP64
+// coefficients and accumulation, columns in order for every row
+__global__ void __launch_bounds__(KT) k_lincomb_zx(const float2* __restrict__ Z, int64_t zcol, int64_t zprob,
+                                                  const double2* __restrict__ y, const int* __restrict__ kk,
+                                                  int maxcols, double scale, double2* __restrict__ x, int64_t n,
+                                                  const int* __restrict__ active) {
+  const int p = blockIdx.y;
+  if (active && !active[p]) return;
+  const float2* Zp = Z + p * zprob;
+  const double2* yp = y + int64_t(p) * maxcols;
+  double2* xp = x + p * n;
+  const int K = kk[p];
+  for (int64_t i = blockIdx.x * int64_t(KT) + threadIdx.x; i < n; i += int64_t(gridDim.x) * KT) {
+    double2 s = make_double2(0.0, 0.0);
+    for (int j = 0; j < K; ++j) {
+      const float2 zf = Zp[int64_t(j) * zcol + i];
+      s.x += double(zf.x) * yp[j].x - double(zf.y) * yp[j].y;
+      s.y += double(zf.x) * yp[j].y + double(zf.y) * yp[j].x;
+    }
+    xp[i] = make_double2(xp[i].x + scale * s.x, xp[i].y + scale * s.y);
+  }
+}
+
+// r[p] = b[p] - ax[p] into column 0 of the outer basis (stride vprob)
+__global__ void k_resid_col(const double2* __restrict__ b, const double2* __restrict__ ax, double2* __restrict__ r,
+                            int64_t vprob, int64_t n, const int* __restrict__ active) {
+  const int p = blockIdx.y;
+  if (active && !active[p]) return;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double2 bb = b[p * n + i], a = ax[p * n + i];
+    r[p * vprob + i] = make_double2(bb.x - a.x, bb.y - a.y);
+  }
+}
+
+void fgmres_ir_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2* b, double2* x, double scale32,
+                   double tol, int restart, double inner_tol, int inner_maxit, int max_outer, int outer_m, int side,
+                   std::vector<IrProblem>& st, Recycle* rec) {
+  using cd = std::complex<double>;
+  const int np = p64.nprob;
+  const int64_t n = p64.n;
+  hipStream_t s = p64.stream;
+  const int mo = std::max(1, std::min(outer_m, std::max(1, max_outer)));
+  const int maxc = mo + 2;
+  const int nblk = int((n + RB - 1) / RB);
+  const int64_t vprob = int64_t(mo + 1) * n, zprob = int64_t(mo) * n;
+  DevBuf Vo(size_t(vprob) * np * sizeof(double2)), Zo(size_t(zprob) * np * sizeof(float2));
+  DevBuf r32(size_t(n) * np * sizeof(float2)), d32(size_t(n) * np * sizeof(float2)),
+      t64(size_t(n) * np * sizeof(double2));
+  DevBuf part(size_t(np) * maxc * nblk * sizeof(double2)), h1(size_t(np) * maxc * sizeof(double2)),
+      h2(size_t(np) * maxc * sizeof(double2)), nr(size_t(np) * maxc * sizeof(double2)),
+      yv(size_t(np) * maxc * sizeof(double2));
+  DevBuf kk(np * sizeof(int)), act(np * sizeof(int)), cact(np * sizeof(int)), beta(np * sizeof(double)),
+      bnd(np * sizeof(double)), invn(np * sizeof(double));
+  double2* V = Vo.as<double2>();
+  float2* Z = Zo.as<float2>();
+  const dim3 gv(unsigned(grid1(n)), unsigned(np));
+  dev_norms<double>(b, n, np, n, bnd.as<double>(), s);
+  std::vector<double> hbn(np), hbeta(np), hinv(np);
+  VP_HIP(hipMemcpy(hbn.data(), bnd.p, np * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<int> hact(np);
+  for (int p = 0; p < np; ++p) hact[p] = st[p].done ? 0 : 1;
+  // host least squares per problem: H column-major (mo+1) x mo, Givens, g
+  std::vector<cd> H(size_t(np) * (mo + 1) * mo), sn(size_t(np) * mo), g(size_t(np) * (mo + 1));
+  std::vector<double> cs(size_t(np) * mo);
+  std::vector<double2> hh1(size_t(np) * maxc), hh2(size_t(np) * maxc), hnr(size_t(np) * maxc),
+      hy(size_t(np) * maxc);
+  bool first = true;
+  for (;;) {
+    // true residual of the current iterate: r = b - A x (before any step x = 0 and r = b)
+    VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    bool all_first = first;
+    for (int p = 0; p < np; ++p) all_first &= !hact[p] || st[p].steps == 0;
+    if (all_first) {
+      hipLaunchKernelGGL(k_copy2<double>, gv, dim3(256), 0, s, b, n, V, vprob, n, 0, act.as<int>());
+    } else {
+      ps_apply_A<double>(p64, x, n, t64.as<double2>(), n, act.as<int>());
+      hipLaunchKernelGGL(k_resid_col, gv, dim3(256), 0, s, b, t64.as<double2>(), V, vprob, n, act.as<int>());
+    }
+    VP_HIP(hipGetLastError());
+    dev_norms<double>(V, n, np, vprob, beta.as<double>(), s);
+    VP_HIP(hipMemcpy(hbeta.data(), beta.p, np * sizeof(double), hipMemcpyDeviceToHost));
+    bool any = false;
+    for (int p = 0; p < np; ++p) {
+      if (!hact[p]) continue;
+      IrProblem& q = st[p];
+      q.relres = relres_of(hbeta[p], hbn[p]);
+      // inside a cycle the history holds the FGMRES estimate of each step; the
+      // true residual of the same iterate replaces it at the cycle's end
+      if (first || q.hist.empty()) q.hist.push_back(q.relres);
+      else q.hist.back() = q.relres;
+      if (q.relres <= tol || !std::isfinite(q.relres)) {
+        q.done = true;
+        q.flag = q.relres <= tol ? 0 : 1;
+        hact[p] = 0;
+      } else if (q.steps >= max_outer) {
+        q.done = true;
+        q.flag = 1;
+        hact[p] = 0;
+      } else {
+        any = true;
+      }
+    }
+    first = false;
+    if (progress_enabled()) {
+      for (int p = 0; p < np; ++p)
+        std::fprintf(stderr, "[vp] flexible refinement: problem %d step %d, FP64 relres %.3e, inner iterations %d%s\n",
+                     p, st[p].steps, st[p].relres, st[p].inner, st[p].done ? " (done)" : "");
+      std::fflush(stderr);
+    }
+    if (!any) break;
+    // v_0 = r / ||r||
+    VP_HIP(hipMemcpyAsync(act.p, hact.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_scale<double>, gv, dim3(256), 0, s, V, vprob, n, beta.as<double>(), 1, act.as<int>());
+    VP_HIP(hipGetLastError());
+    std::fill(H.begin(), H.end(), cd(0.0));
+    std::fill(g.begin(), g.end(), cd(0.0));
+    for (int p = 0; p < np; ++p) g[size_t(p) * (mo + 1)] = hbeta[p];
+    std::vector<int> cyc(hact), kdone(np, 0);   // in this cycle; steps taken
+    std::vector<int> step_act(hact);            // still stepping
+    for (int j = 0; j < mo; ++j) {
+      bool stepping = false;
+      for (int p = 0; p < np; ++p) stepping |= step_act[p] != 0;
+      if (!stepping) break;
+      VP_HIP(hipMemcpyAsync(act.p, step_act.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+      const int* da = act.as<int>();
+      // z_j = inner FP32 solve of (A/s) d = v_j
+      hipLaunchKernelGGL(k_cvt_d2f, gv, dim3(256), 0, s, V + int64_t(j) * n, vprob, r32.as<float2>(), n, n, da);
+      VP_HIP(hipGetLastError());
+      GmresParams prm;
+      prm.restart = restart;
+      prm.tol = inner_tol;
+      prm.maxit = inner_maxit;
+      prm.side = side;
+      prm.final_residual = false;
+      prm.active_in = step_act;
+      prm.tol_per.assign(np, inner_tol);
+      for (int p = 0; p < np; ++p)
+        if (step_act[p]) {
+          // the outer residual estimate before this step decides how deep the inner solve goes
+          const double cur = std::abs(g[size_t(p) * (mo + 1) + j]) / (hbn[p] > 0 ? hbn[p] : 1.0);
+          if (cur > 0) {
+            const double need = std::min(0.5, 0.5 * tol / cur);
+            prm.tol_per[p] = need >= inner_tol / 8 ? need : inner_tol;
+          }
+        }
+      if (rec) {
+        prm.rec = rec;
+        if (rec->kc == 0) rec->build = true;
+      }
+      GmresResult gr;
+      gmres_device<float>(p32, r32.as<float2>(), d32.as<float2>(), prm, gr);
+      for (int p = 0; p < np; ++p)
+        if (step_act[p]) {
+          st[p].inner += gr.total[p];
+          st[p].steps += 1;
+        }
+      // Z_j = d (complex64), w = A (d / s) into v_{j+1}
+      double2* w = V + int64_t(j + 1) * n;
+      hipLaunchKernelGGL(k_copy2<float>, gv, dim3(256), 0, s, d32.as<float2>(), n, Z + int64_t(j) * n, zprob, n, 0,
+                         da);
+      hipLaunchKernelGGL(k_cvt_f2d, gv, dim3(256), 0, s, d32.as<float2>(), n, t64.as<double2>(), n, n, 1.0 / scale32,
+                         da);
+      VP_HIP(hipGetLastError());
+      ps_apply_A<double>(p64, t64.as<double2>(), n, w, vprob, da);
+      // classical Gram-Schmidt, two passes, against v_0..v_j
+      const int nc = j + 1;
+      const auto sp = dots_split(nc, nblk, np);
+      hipLaunchKernelGGL(k_dots<double>, dim3(nblk, np, sp.first), dim3(KT), 0, s, V, n, vprob, nc, w, vprob, n,
+                         part.as<double2>(), maxc, da, 0, sp.second);
+      hipLaunchKernelGGL(k_reduce_cols, dim3(nc, np), dim3(KT), 0, s, part.as<double2>(), nblk, nc, maxc,
+                         h1.as<double2>(), da);
+      hipLaunchKernelGGL(k_axpy_dots<double>, dim3(nblk, np), dim3(KT), 0, s, V, n, vprob, nc, h1.as<double2>(), w,
+                         vprob, n, part.as<double2>(), maxc, 0, da);
+      hipLaunchKernelGGL(k_reduce_cols, dim3(nc, np), dim3(KT), 0, s, part.as<double2>(), nblk, nc, maxc,
+                         h2.as<double2>(), da);
+      hipLaunchKernelGGL(k_axpy_dots<double>, dim3(nblk, np), dim3(KT), 0, s, V, n, vprob, nc, h2.as<double2>(), w,
+                         vprob, n, part.as<double2>(), maxc, 1, da);
+      hipLaunchKernelGGL(k_reduce_cols, dim3(1, np), dim3(KT), 0, s, part.as<double2>(), nblk, 1, maxc,
+                         nr.as<double2>(), da);
+      VP_HIP(hipGetLastError());
+      VP_HIP(hipMemcpyAsync(hh1.data(), h1.p, hh1.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+      VP_HIP(hipMemcpyAsync(hh2.data(), h2.p, hh2.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+      VP_HIP(hipMemcpyAsync(hnr.data(), nr.p, hnr.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+      VP_HIP(hipStreamSynchronize(s));
+      for (int p = 0; p < np; ++p) {
+        hinv[p] = 0.0;
+        if (!step_act[p]) continue;
+        cd* Hc = H.data() + (size_t(p) * mo + j) * (mo + 1);   // column j of problem p
+        for (int i = 0; i <= j; ++i) {
+          const double2 a = hh1[size_t(p) * maxc + i], c = hh2[size_t(p) * maxc + i];
+          Hc[i] = cd(a.x + c.x, a.y + c.y);
+        }
+        const double hn = std::sqrt(std::max(0.0, hnr[size_t(p) * maxc].x));
+        Hc[j + 1] = hn;
+        // earlier rotations, then the one that zeroes H(j+1, j)
+        for (int i = 0; i < j; ++i) {
+          const double c = cs[size_t(p) * mo + i];
+          const cd sv = sn[size_t(p) * mo + i];
+          const cd a = Hc[i], bb = Hc[i + 1];
+          Hc[i] = c * a + sv * bb;
+          Hc[i + 1] = -std::conj(sv) * a + c * bb;
+        }
+        const cd a = Hc[j];
+        const double den = std::sqrt(std::norm(a) + hn * hn);
+        double c = 1.0;
+        cd sv(0.0);
+        if (den > 0) {
+          const double aa = std::abs(a);
+          if (aa > 0) {
+            c = aa / den;
+            sv = (a / aa) * hn / den;
+          } else {
+            c = 0.0;
+            sv = 1.0;
+          }
+        }
+        // [c s; -conj(s) c] maps (a, hn) to (den * a/|a|, 0)
+        cs[size_t(p) * mo + j] = c;
+        sn[size_t(p) * mo + j] = sv;
+        Hc[j] = c * a + sv * cd(hn);
+        Hc[j + 1] = 0.0;
+        cd* gp = g.data() + size_t(p) * (mo + 1);
+        const cd gj = gp[j];
+        gp[j] = c * gj;
+        gp[j + 1] = -std::conj(sv) * gj;
+        kdone[p] = j + 1;
+        const double est = relres_of(std::abs(gp[j + 1]), hbn[p]);
+        st[p].hist.push_back(est);
+        if (!(est > tol) || hn == 0.0 || st[p].steps >= max_outer || !std::isfinite(est)) step_act[p] = 0;
+        hinv[p] = hn > 0 ? 1.0 / hn : 0.0;
+      }
+      // v_{j+1} = w / ||w||
+      VP_HIP(hipMemcpyAsync(invn.p, hinv.data(), np * sizeof(double), hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_scale<double>, gv, dim3(256), 0, s, w, vprob, n, invn.as<double>(), 0, da);
+      VP_HIP(hipGetLastError());
+    }
+    // x += Z y / s, y = R^-1 g (back substitution over each problem's steps)
+    for (int p = 0; p < np; ++p) {
+      const int K = kdone[p];
+      const cd* Hp = H.data() + size_t(p) * mo * (mo + 1);
+      const cd* gp = g.data() + size_t(p) * (mo + 1);
+      std::vector<cd> yy(std::max(K, 1), cd(0.0));
+      for (int i = K - 1; i >= 0; --i) {
+        cd sum = gp[i];
+        for (int l = i + 1; l < K; ++l) sum -= Hp[size_t(l) * (mo + 1) + i] * yy[l];
+        const cd d = Hp[size_t(i) * (mo + 1) + i];
+        yy[i] = std::abs(d) > 0 ? sum / d : cd(0.0);
+      }
+      for (int i = 0; i < maxc; ++i)
+        hy[size_t(p) * maxc + i] = i < K ? make_double2(yy[i].real(), yy[i].imag()) : make_double2(0.0, 0.0);
+    }
+    VP_HIP(hipMemcpyAsync(yv.p, hy.data(), hy.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+    VP_HIP(hipMemcpyAsync(kk.p, kdone.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    VP_HIP(hipMemcpyAsync(cact.p, cyc.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_lincomb_zx, gv, dim3(KT), 0, s, Z, n, zprob, yv.as<double2>(), kk.as<int>(), maxc,
+                       1.0 / scale32, x, n, cact.as<int>());
+    VP_HIP(hipGetLastError());
+    VP_HIP(hipStreamSynchronize(s));
+    for (int p = 0; p < np; ++p) hact[p] = cyc[p];
+  }
+}
 
 }  // namespace vp

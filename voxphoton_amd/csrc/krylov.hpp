@@ -123,6 +123,19 @@ bool ir_step_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2*
                  double tol, int restart, double inner_tol, int inner_maxit, int max_outer, int side,
                  std::vector<IrProblem>& st, Recycle* rec = nullptr);
 
+// Flexible refinement (FGMRES in FP64 whose preconditioner is the inner FP32
+// solve): outer step j solves (A/s) z_j = v_j with the FP32 GMRES and the
+// preconditioner, forms w = A z_j / s in FP64, orthogonalises it against the
+// outer basis (two classical Gram-Schmidt passes) and minimises the FP64
+// residual over all corrections of the cycle, x += Z y.  Cycles of outer_m
+// steps restart from the true residual; every problem stops at its true
+// FP64 relative residual <= tol or after max_outer inner solves.
+// st[p].hist holds the residual after every inner solve (the FGMRES
+// estimate inside a cycle, the true residual at its end).
+void fgmres_ir_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2* b, double2* x, double scale32,
+                   double tol, int restart, double inner_tol, int inner_maxit, int max_outer, int outer_m, int side,
+                   std::vector<IrProblem>& st, Recycle* rec = nullptr);
+
 // Device vector helpers (FP64 accumulation).
 template <typename T>
 void dev_norms(const C<T>* x, int64_t n, int nprob, int64_t vstride, double* d_out, hipStream_t s);

@@ -425,18 +425,25 @@ def bicgstab(A, b, tol=1e-6, maxit=20, M=None, x0=None, prec='fp64'):
     return x, info.flag, info.relres, it, buf[:info.resvec_len].copy()
 
 
-def solve_ir(A, b, tol=1e-6, restart=50, inner_tol=1e-3, inner_maxit=4, max_outer=20, M=None, side='left'):
+def solve_ir(A, b, tol=1e-6, restart=50, inner_tol=1e-3, inner_maxit=4, max_outer=20, M=None, side='left',
+             outer_restart=0):
     """Mixed precision: FP64 refinement around FP32 GMRES(restart) on the GPU.
 
-    Returns (x, info dict) with the true FP64 relative residual ``relres``.
+    ``outer_restart`` > 0 runs the flexible form (vp_solve_fgmres_ir: FGMRES
+    in FP64 over the inner solves, cycles of that many steps); 0 the plain
+    refinement (vp_solve_ir).  Returns (x, info dict) with the true FP64
+    relative residual ``relres``.
     """
     b = as_c128(np.asarray(b).reshape(-1))
     x = np.empty_like(b)
     info, buf = _info(max_outer + 2)
-    check(L.lib.vp_solve_ir(A.handle, M.handle if M is not None else None,
-                            L.VP_PREC_RIGHT if side == 'right' else L.VP_PREC_LEFT, ptr(b), ptr(x), L.VP_MEM_HOST,
-                            float(tol), int(restart), float(inner_tol), int(inner_maxit), int(max_outer),
-                            ctypes.byref(info)))
+    args = [A.handle, M.handle if M is not None else None, L.VP_PREC_RIGHT if side == 'right' else L.VP_PREC_LEFT,
+            ptr(b), ptr(x), L.VP_MEM_HOST, float(tol), int(restart), float(inner_tol), int(inner_maxit),
+            int(max_outer)]
+    if outer_restart:
+        check(L.lib.vp_solve_fgmres_ir(*args, int(outer_restart), ctypes.byref(info)))
+    else:
+        check(L.lib.vp_solve_ir(*args, ctypes.byref(info)))
     return x, dict(flag=info.flag, relres=info.relres, ir_steps=info.ir_steps, inner_iters=info.total_iters,
                    seconds=info.seconds, history=buf[:info.resvec_len].copy())
 
@@ -464,8 +471,9 @@ class DeviceArray:
 
 
 def solve_ir_batched(ops, precs, B, tol=1e-6, restart=50, inner_tol=1e-3, inner_maxit=4, max_outer=20,
-                     side='left'):
-    """Batched mixed-precision solves: row i of B through ops[i] (and precs[i])."""
+                     side='left', outer_restart=0):
+    """Batched mixed-precision solves: row i of B through ops[i] (and
+    precs[i]); ``outer_restart`` as in solve_ir."""
     ctx = ops[0].ctx
     B = as_c128(np.asarray(B))
     ns, n = B.shape
@@ -483,11 +491,14 @@ def solve_ir_batched(ops, precs, B, tol=1e-6, restart=50, inner_tol=1e-3, inner_
         bufs.append(buf)
         infos[i].resvec = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         infos[i].resvec_cap = buf.size
-    check(L.lib.vp_solve_ir_batched(ctypes.cast(hops, ctypes.c_void_p),
-                                    ctypes.cast(hpre, ctypes.c_void_p) if hpre is not None else None, ns,
-                                    L.VP_PREC_RIGHT if side == 'right' else L.VP_PREC_LEFT, db.ptr, dx.ptr,
-                                    float(tol), int(restart), float(inner_tol), int(inner_maxit), int(max_outer),
-                                    ctypes.cast(infos, ctypes.POINTER(L.GmresInfo))))
+    args = [ctypes.cast(hops, ctypes.c_void_p), ctypes.cast(hpre, ctypes.c_void_p) if hpre is not None else None, ns,
+            L.VP_PREC_RIGHT if side == 'right' else L.VP_PREC_LEFT, db.ptr, dx.ptr, float(tol), int(restart),
+            float(inner_tol), int(inner_maxit), int(max_outer)]
+    if outer_restart:
+        check(L.lib.vp_solve_fgmres_ir_batched(*args, int(outer_restart),
+                                               ctypes.cast(infos, ctypes.POINTER(L.GmresInfo))))
+    else:
+        check(L.lib.vp_solve_ir_batched(*args, ctypes.cast(infos, ctypes.POINTER(L.GmresInfo))))
     X = np.empty_like(B)
     dx.download(X)
     out = [dict(flag=infos[i].flag, relres=infos[i].relres, ir_steps=infos[i].ir_steps,
