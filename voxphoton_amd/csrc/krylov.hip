@@ -25,6 +25,7 @@
 #include <utility>
 #include <cstdio>
 #include <cstdlib>
+#include "eig.hpp"
 #include "krylov.hpp"
 
 namespace vp {
@@ -745,6 +746,14 @@ __global__ void k_copy2(const C<T>* __restrict__ a, int64_t as, C<T>* __restrict
     bp[i] = accumulate ? cadd(bp[i], ap[i]) : ap[i];
 }
 
+// Width of the recycled space kept after the first inner solve of a
+// refinement run (VP_IR_RECYCLE_K): 0 keeps the whole last cycle; k > 0 the
+// k harmonic Ritz directions of smallest modulus (harmonic_pair).
+static int recycle_k() {
+  const char* e = std::getenv("VP_IR_RECYCLE_K");
+  return e ? std::max(0, std::atoi(e)) : 0;
+}
+
 // ----- GMRES driver ---------------------------------------------------------------
 static int grid1(int64_t total) { return int(std::min<int64_t>((total + 255) / 256, 8192)); }
 
@@ -943,21 +952,21 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   // the Givens rotations G give G H = [R; 0] and Q = G^H (first K columns):
   // Aop V(:, 0..K-1) = V H = C R, so Aop U = C with C orthonormal
   auto build_recycled = [&](const std::vector<int>& lk) {
-    int kn = 0;
-    for (int p = 0; p < np; ++p) kn = std::max(kn, std::min(lk[p], kcap));
-    if (kn == 0) return;
+    int kmax = 0;   // Arnoldi columns of the last cycle per problem, at most kcap
+    for (int p = 0; p < np; ++p) kmax = std::max(kmax, std::min(lk[p], kcap));
+    if (kmax == 0) return;
     std::vector<double2> hH(size_t(np) * (m + 1) * m), hsn(size_t(np) * m);
     std::vector<double> hcs(size_t(np) * m);
     VP_HIP(hipMemcpy(hH.data(), H.p, hH.size() * sizeof(double2), hipMemcpyDeviceToHost));
     VP_HIP(hipMemcpy(hsn.data(), sn.p, hsn.size() * sizeof(double2), hipMemcpyDeviceToHost));
     VP_HIP(hipMemcpy(hcs.data(), cs.p, hcs.size() * sizeof(double), hipMemcpyDeviceToHost));
-    const int lsC = kn + 1, ncol = (kn + JO - 1) / JO * JO;
-    std::vector<double2> SC(size_t(np) * lsC * ncol, make_double2(0.0, 0.0)),
-        SU(size_t(np) * kn * ncol, make_double2(0.0, 0.0));
-    std::vector<int> kinC(np, 0), kinU(np, 0);
     using cd = std::complex<double>;
+    const int kdef = recycle_k();
+    // per problem: SC ((K+1) x ko) and SU (K x ko), column-major
+    std::vector<std::vector<cd>> SCp(np), SUp(np);
+    std::vector<int> Kp(np, 0), kop(np, 0);
     for (int p = 0; p < np; ++p) {
-      const int K = std::min(lk[p], kn);
+      const int K = std::min(lk[p], kmax);
       if (K == 0) continue;
       std::vector<cd> Q(size_t(K + 1) * (K + 1), cd(0.0));     // row-major
       for (int i = 0; i <= K; ++i) Q[size_t(i) * (K + 1) + i] = 1.0;
@@ -975,14 +984,27 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       bool ok = true;
       for (int i = 0; i < K; ++i) ok &= std::abs(R(i, i)) > 0.0;
       if (!ok) continue;   // singular R: this problem recycles nothing
-      double2* sc = SC.data() + size_t(p) * lsC * ncol;
-      for (int j = 0; j < K; ++j)
-        for (int r = 0; r <= K; ++r) {
-          const cd q = Q[size_t(r) * (K + 1) + j];
-          sc[size_t(j) * lsC + r] = make_double2(q.real(), q.imag());
+      if (kdef > 0 && kdef < K) {
+        // Hb = Q(:, 0..K-1) R, the cycle's unrotated Hessenberg matrix
+        std::vector<cd> Hb(size_t(K + 1) * K, cd(0.0));
+        for (int j = 0; j < K; ++j)
+          for (int i = 0; i <= K; ++i) {
+            cd t = 0.0;
+            for (int l = 0; l <= j; ++l) t += Q[size_t(i) * (K + 1) + l] * R(l, j);
+            Hb[size_t(j) * (K + 1) + i] = t;
+          }
+        const int kk2 = harmonic_pair(K, Hb, kdef, SCp[size_t(p)], SUp[size_t(p)]);
+        if (kk2 > 0) {
+          Kp[p] = K;
+          kop[p] = kk2;
+          continue;
         }
-      // R^-1 column by column (back substitution)
-      double2* su = SU.data() + size_t(p) * kn * ncol;
+      }
+      // the whole cycle: C = V Q(:, 0..K-1), U = V R^-1
+      SCp[size_t(p)].assign(size_t(K + 1) * K, cd(0.0));
+      SUp[size_t(p)].assign(size_t(K) * K, cd(0.0));
+      for (int j = 0; j < K; ++j)
+        for (int r = 0; r <= K; ++r) SCp[size_t(p)][size_t(j) * (K + 1) + r] = Q[size_t(r) * (K + 1) + j];
       for (int j = 0; j < K; ++j) {
         std::vector<cd> xcol(K, cd(0.0));
         for (int i = j; i >= 0; --i) {
@@ -990,7 +1012,30 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
           for (int l = i + 1; l <= j; ++l) sum -= R(i, l) * xcol[l];
           xcol[i] = sum / R(i, i);
         }
-        for (int i = 0; i <= j; ++i) su[size_t(j) * kn + i] = make_double2(xcol[i].real(), xcol[i].imag());
+        for (int i = 0; i <= j; ++i) SUp[size_t(p)][size_t(j) * K + i] = xcol[i];
+      }
+      Kp[p] = K;
+      kop[p] = K;
+    }
+    int kn = 0;
+    for (int p = 0; p < np; ++p) kn = std::max(kn, kop[p]);
+    if (kn == 0) return;
+    const int lsC = kmax + 1, lsU = kmax, ncol = (kn + JO - 1) / JO * JO;
+    std::vector<double2> SC(size_t(np) * lsC * ncol, make_double2(0.0, 0.0)),
+        SU(size_t(np) * lsU * ncol, make_double2(0.0, 0.0));
+    std::vector<int> kinC(np, 0), kinU(np, 0);
+    for (int p = 0; p < np; ++p) {
+      const int K = Kp[p], ko = kop[p];
+      if (ko == 0) continue;
+      for (int j = 0; j < ko; ++j) {
+        for (int r = 0; r <= K; ++r) {
+          const cd v = SCp[size_t(p)][size_t(j) * (K + 1) + r];
+          SC[size_t(p) * lsC * ncol + size_t(j) * lsC + r] = make_double2(v.real(), v.imag());
+        }
+        for (int r = 0; r < K; ++r) {
+          const cd v = SUp[size_t(p)][size_t(j) * K + r];
+          SU[size_t(p) * lsU * ncol + size_t(j) * lsU + r] = make_double2(v.real(), v.imag());
+        }
       }
       kinC[p] = K + 1;
       kinU[p] = K;
@@ -1005,8 +1050,8 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     const unsigned rblk = unsigned((n + KT - 1) / KT);
     hipLaunchKernelGGL(k_tsgemm<T>, dim3(rblk, np, ncol / JO), dim3(KT), 0, s, Vb, vprob, dSC.as<double2>(), lsC,
                        int64_t(lsC) * ncol, dkC.as<int>(), kn, Cout, vprob, n);
-    hipLaunchKernelGGL(k_tsgemm<T>, dim3(rblk, np, ncol / JO), dim3(KT), 0, s, Vb, vprob, dSU.as<double2>(), kn,
-                       int64_t(kn) * ncol, dkU.as<int>(), kn, rec->U.template as<C<T>>(), int64_t(kcap) * n, n);
+    hipLaunchKernelGGL(k_tsgemm<T>, dim3(rblk, np, ncol / JO), dim3(KT), 0, s, Vb, vprob, dSU.as<double2>(), lsU,
+                       int64_t(lsU) * ncol, dkU.as<int>(), kn, rec->U.template as<C<T>>(), int64_t(kcap) * n, n);
     VP_HIP(hipGetLastError());
     VP_HIP(hipStreamSynchronize(s));
     rec->kc = kn;
