@@ -193,6 +193,9 @@ def main():
     ap.add_argument('--inner-maxit', type=int, default=None, help='override the workload inner restart cycles')
     ap.add_argument('--batch', type=int, default=None, help='sweep: wavelengths per batched solve (override)')
     ap.add_argument('--max-outer', type=int, default=None, help='override the workload refinement-step limit')
+    ap.add_argument('--reference-path', action='store_true',
+                    help="also time one solve on the reference's FP64 GPU path (pgmres on host arrays, "
+                         'transfers per operator application, operator at 2L x 2M x 2N)')
     ap.add_argument('--traffic-probe', action='store_true',
                     help='for rocprofv3 --pmc runs: a 1 GiB device copy (calibration), then 3 FP32 matvecs '
                          'and 3 FP32 preconditioner applies')
@@ -373,6 +376,14 @@ def main():
         log(f'mixed vs plain FP64 solution: relative difference {x_err:.3e}')
         del xm, x64s, dx64
 
+    # ---- the reference's own FP64 GPU path (opt-in: host Arnoldi, transfers per application)
+    refp = None
+    if args.reference_path and prec is not None:
+        from voxphoton_amd import refpath
+        _, ri = refpath.solve_reference_path(vie, prob, prec, b, wl['restart'], wl['tol'], 40, ctx=ctx)
+        refp = dict(ri, fft=list(ri['fft']), solves_per_s=1.0 / ri['seconds'])
+        log(f'reference FP64 GPU path: {ri["seconds"]:.2f} s, {ri["iterations"]} its, relres {ri["relres"]:.2e}')
+
     # ---- roofline of the dominant kernel
     dom = max(stats, key=lambda k: stats[k][1])
     launches, ms = stats[dom]
@@ -444,6 +455,8 @@ def main():
                        'p99': round(float(np.percentile(lat_ms, 99)), 3)},
         'fp64_plain': fp64,
         'mixed_vs_fp64': None if fp64 is None else round(solves_per_s / fp64['solves_per_s'], 3),
+        'reference_fp64_gpu_path': refp,
+        'mixed_vs_reference_fp64_gpu_path': None if refp is None else round(solves_per_s * refp['seconds'], 2),
         'x_err_vs_fp64': x_err,
         'matvec_vs_oracle': mv_check,
         'matvec': {'fp64_ms': round(t_mv64 * 1e3, 4), 'fp32_ms': round(t_mv32 * 1e3, 4),
@@ -542,6 +555,20 @@ def run_sweep(args, wl, log):
         its64.append(info.total_iters)
         log(f'  plain FP64 at {lam * 1e9:.0f} nm: {t64[-1]:.3f} s, {info.total_iters} its, relres {info.relres:.2e}')
     fp64_serial = nsolve * float(np.mean(t64))
+    # the reference's own FP64 GPU path (JVIE_Solver_const GPU_flag = 1):
+    # pgmres on host arrays, every operator and preconditioner application
+    # uploads, runs on the device (operator at 2L x 2M x 2N) and downloads;
+    # timed on the same first and last wavelength (first right-hand side)
+    from voxphoton_amd import refpath
+    rinfos = []
+    for lam in (lams[0], lams[-1]):
+        prob, op, P, rhs = build(lam)
+        _, ri = refpath.solve_reference_path(vie, prob, P, rhs[0], wl['restart'], wl['tol'], 40, ctx=ctx)
+        rinfos.append(ri)
+        log(f'  reference FP64 GPU path at {lam * 1e9:.0f} nm: {ri["seconds"]:.2f} s, {ri["iterations"]} its, '
+            f'relres {ri["relres"]:.2e}, FFT {ri["fft"]}')
+        del op, P
+    rsec = float(np.mean([ri['seconds'] for ri in rinfos]))
     value = nsolve / t_solve
     worst = max(i['relres'] for i in infos_all)
     line = {
@@ -560,6 +587,17 @@ def run_sweep(args, wl, log):
         'fp64_plain_serial': {'seconds_per_solve': round(float(np.mean(t64)), 4), 'iterations': its64,
                               'solves_per_s': round(nsolve / fp64_serial, 4)},
         'mixed_batched_vs_fp64_serial': round(value / (nsolve / fp64_serial), 3),
+        'reference_fp64_gpu_path': {
+            'description': "the reference's FP64 GPU solve path (JVIE_Solver_const GPU_flag=1): pgmres "
+                           'on host arrays (numpy, the host cores), every operator and preconditioner '
+                           'application uploaded, run in FP64 on the device (operator at 2L x 2M x 2N on the '
+                           'full grid) and downloaded; same preconditioner, restart and tolerance; the '
+                           'first and last wavelength (first right-hand side), serial',
+            'seconds_per_solve': round(rsec, 3), 'seconds': [round(ri['seconds'], 3) for ri in rinfos],
+            'iterations': [ri['iterations'] for ri in rinfos], 'relres': [ri['relres'] for ri in rinfos],
+            'flags': [ri['flag'] for ri in rinfos], 'fft': list(rinfos[0]['fft']),
+            'solves_per_s': round(1.0 / rsec, 4)},
+        'mixed_batched_vs_reference_fp64_gpu_path': round(value * rsec, 2),
     }
     print(json.dumps(line), flush=True)
 
