@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(256) k_pc_blockmul(PcArgs<T> a, PcGeom g) {
 // block entry is read once and feeds four products.  The mirrored blocks
 // are S B S (S negates the x or y component): the sign of the column's
 // component is folded into the staged vector, the row sign into the result.
-template <typename T>
+template <typename T, int QH = 8>
 __global__ void __launch_bounds__(256) k_pc_quadmul(PcArgs<T> a, PcGeom g) {
   if (a.active && !a.active[blockIdx.y]) return;
   C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(256) k_pc_quadmul(PcArgs<T> a, PcGeom g) {
           cfma_acc(acc2, br, bi, Xs[2][q]);
           cfma_acc(acc3, br, bi, Xs[3][q]);
         };
-        constexpr int QU = 8;
+        constexpr int QU = QH;
         int q0 = 0;
         for (; q0 + QU <= n3; q0 += QU) {
           uint32_t hw[QU];
@@ -466,7 +466,14 @@ static void launch_quadmul(const PcGeom& g, const PcArgs<T>& a, dim3 grid, int t
       return;
     }
   }
-  hipLaunchKernelGGL(k_pc_quadmul<T>, grid, dim3(threads), 0, s, a, g);
+  // binary16 blocks: columns of block loads in flight per chunk (VP_PC_QU;
+  // 16 measured 672 us at Bragg 10 nm against 692 / 696 / 707 us for 8 / 24 / 32)
+  const char* eq = std::getenv("VP_PC_QU");
+  const int qu = eq ? std::atoi(eq) : 16;
+  if (qu == 16) hipLaunchKernelGGL((k_pc_quadmul<T, 16>), grid, dim3(threads), 0, s, a, g);
+  else if (qu == 32) hipLaunchKernelGGL((k_pc_quadmul<T, 32>), grid, dim3(threads), 0, s, a, g);
+  else if (qu == 24) hipLaunchKernelGGL((k_pc_quadmul<T, 24>), grid, dim3(threads), 0, s, a, g);
+  else hipLaunchKernelGGL((k_pc_quadmul<T, 8>), grid, dim3(threads), 0, s, a, g);
 }
 
 // x pass inverse + gather into the output vector.
