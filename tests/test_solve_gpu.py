@@ -176,6 +176,34 @@ def test_level2_blocks_wider_than_a_wavefront(vp_ctx, monkeypatch, half):
         assert not np.array_equal(y_dot, P.apply(x, 'fp32'))
 
 
+def test_block_product_window_changes_no_result(vp_ctx, monkeypatch):
+    # the binary16 block product keeps 8, 16 or 32 columns of block loads in
+    # flight (tuned when the FP32 preconditioner is made, VP_PC_QU overrides);
+    # every row still adds its columns in order, so the applies are equal bit
+    # for bit, and the tuned default is one of them
+    from voxphoton_amd.vie import Preconditioner
+    monkeypatch.setenv('VP_PC_HALF', '1')
+    monkeypatch.setenv('VP_PC_DOT2', '0')
+    for (L, M, N) in [(6, 4, 22), (8, 6, 15), (5, 4, 60)]:
+        dx, ko = 20e-9, 5.85e6
+        G = ogr.assemble_G(L, M, N, dx, ko)
+        idx = ovie.idx_from_mask(np.ones((L, M, N), bool))
+        rng = np.random.default_rng(L + N)
+        x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+        monkeypatch.delenv('VP_PC_QU', raising=False)
+        P = Preconditioner.level2(G, 0.8278, 1.0, dx ** 3, idx, ctx=vp_ctx)
+        y_tuned = P.apply(x, 'fp32')
+        ys = {}
+        for qu in ('8', '16', '32'):
+            monkeypatch.setenv('VP_PC_QU', qu)
+            ys[qu] = P.apply(x, 'fp32')
+        assert np.array_equal(ys['8'], ys['16']) and np.array_equal(ys['8'], ys['32'])
+        assert np.array_equal(y_tuned, ys['16'])
+        ref = opre.chan_2_mvp_for_parallel_idx(opre.level_2_parallel_func(G, 0.8278, 1.0, L, M, N, dx ** 3), x,
+                                               L, M, N, idx)
+        assert rel(y_tuned, ref) < 2e-3
+
+
 @pytest.mark.parametrize('quad', ['1', '0'])
 @pytest.mark.parametrize('sym', ['1', '0'])
 def test_level2_face_matches_oracle(vp_ctx, monkeypatch, sym, quad):

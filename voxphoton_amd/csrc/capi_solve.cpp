@@ -7,7 +7,10 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 #include <vector>
 #include "krylov.hpp"
 #include "capi_common.hpp"
@@ -134,6 +137,32 @@ static void prec_alloc_prec(vp_prec_s* p, PrecPrec<T>& P) {
 }
 
 // Builds the FP32 block copy from the FP64 blocks, scaled by scale32.
+// Load window of the binary16 block product, timed once per (device, block
+// geometry) in the process and shared by every preconditioner of that shape.
+struct QuKey {
+  int device, L, M, N, Hx, Hy, pct;
+  bool operator<(const QuKey& o) const {
+    return std::tie(device, L, M, N, Hx, Hy, pct) < std::tie(o.device, o.L, o.M, o.N, o.Hx, o.Hy, o.pct);
+  }
+};
+static std::mutex g_qu_mu;
+static std::map<QuKey, int> g_qu;
+
+static int quadmul_window(vp_prec_s* p) {
+  const PcGeom& g = p->g;
+  const QuKey key{p->ctx->device, g.L, g.M, g.N, g.Hx, g.Hy, g.pct};
+  {
+    std::lock_guard<std::mutex> lk(g_qu_mu);
+    auto it = g_qu.find(key);
+    if (it != g_qu.end()) return it->second;
+  }
+  const PcArgs<float> a = prec_args<float>(p, nullptr, nullptr);
+  const int qu = tune_quadmul_window(g, a, p->ctx->stream);
+  std::lock_guard<std::mutex> lk(g_qu_mu);
+  g_qu.emplace(key, qu);
+  return qu;
+}
+
 static void prec_make_fp32(vp_prec_s* p, double scale32) {
   const int64_t nb = int64_t(p->p64.blocks.bytes / sizeof(double2));
   p->p32.blocks.alloc(size_t(nb) * sizeof(float2));
@@ -144,7 +173,10 @@ static void prec_make_fp32(vp_prec_s* p, double scale32) {
   // quadrant form: the FP32 apply streams binary16 blocks (VP_PC_HALF=0 keeps FP32)
   const char* eh = std::getenv("VP_PC_HALF");
   p->p32.hblocks.release();
-  if (p->g.level == 2 && p->g.pquad && !(eh && eh[0] == '0')) make_half_blocks(p->p32, p->ctx->stream);
+  if (p->g.level == 2 && p->g.pquad && !(eh && eh[0] == '0')) {
+    make_half_blocks(p->p32, p->ctx->stream);
+    p->g.qu = quadmul_window(p);
+  }
 }
 
 // blocks_dev: FP64 blocks in (kx, ky)-major order, already inverted; scaled here by 1/(L*M)

@@ -457,6 +457,15 @@ __global__ void __launch_bounds__(256) k_pc_quadmul_dot2(PcArgs<float> a, PcGeom
 // rounding of the vector makes the preconditioner a nonlinear map, and
 // right-preconditioned GMRES there needs 87 inner iterations instead of 67.
 template <typename T>
+static void launch_quadmul_window(const PcGeom& g, const PcArgs<T>& a, dim3 grid, int threads, int qu,
+                                  hipStream_t s) {
+  if (qu == 32) hipLaunchKernelGGL((k_pc_quadmul<T, 32>), grid, dim3(threads), 0, s, a, g);
+  else if (qu == 8) hipLaunchKernelGGL((k_pc_quadmul<T, 8>), grid, dim3(threads), 0, s, a, g);
+  else hipLaunchKernelGGL((k_pc_quadmul<T, 16>), grid, dim3(threads), 0, s, a, g);
+  VP_HIP(hipGetLastError());
+}
+
+template <typename T>
 static void launch_quadmul(const PcGeom& g, const PcArgs<T>& a, dim3 grid, int threads, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     const char* e = std::getenv("VP_PC_DOT2");
@@ -466,14 +475,35 @@ static void launch_quadmul(const PcGeom& g, const PcArgs<T>& a, dim3 grid, int t
       return;
     }
   }
-  // binary16 blocks: columns of block loads in flight per chunk (VP_PC_QU;
-  // 16 measured 672 us at Bragg 10 nm against 692 / 696 / 707 us for 8 / 24 / 32)
+  // binary16 blocks: columns of block loads in flight per chunk, the
+  // preconditioner's tuned window g.qu (VP_PC_QU overrides)
   const char* eq = std::getenv("VP_PC_QU");
-  const int qu = eq ? std::atoi(eq) : 16;
-  if (qu == 16) hipLaunchKernelGGL((k_pc_quadmul<T, 16>), grid, dim3(threads), 0, s, a, g);
-  else if (qu == 32) hipLaunchKernelGGL((k_pc_quadmul<T, 32>), grid, dim3(threads), 0, s, a, g);
-  else if (qu == 24) hipLaunchKernelGGL((k_pc_quadmul<T, 24>), grid, dim3(threads), 0, s, a, g);
-  else hipLaunchKernelGGL((k_pc_quadmul<T, 8>), grid, dim3(threads), 0, s, a, g);
+  const int qu = eq ? std::atoi(eq) : (g.qu > 0 ? g.qu : 16);
+  launch_quadmul_window<T>(g, a, grid, threads, qu, s);
+}
+
+int tune_quadmul_window(const PcGeom& g, const PcArgs<float>& a, hipStream_t s) {
+  if (!a.hblocks) return 16;
+  const dim3 grid(unsigned(int64_t(g.Hx) * g.Hy), 1);
+  const int threads = std::min(256, ((3 * g.N + 63) / 64) * 64);
+  hipEvent_t e0, e1;
+  VP_HIP(hipEventCreate(&e0));
+  VP_HIP(hipEventCreate(&e1));
+  int best = 16;
+  float tbest = 0.f;
+  for (int qu : {8, 16, 32}) {
+    launch_quadmul_window<float>(g, a, grid, threads, qu, s);   // warm-up
+    VP_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < 3; ++r) launch_quadmul_window<float>(g, a, grid, threads, qu, s);
+    VP_HIP(hipEventRecord(e1, s));
+    VP_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    VP_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (qu == 8 || ms < tbest) { tbest = ms; best = qu; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
 }
 
 // x pass inverse + gather into the output vector.

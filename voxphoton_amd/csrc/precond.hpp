@@ -25,6 +25,9 @@ struct PcGeom {
   // quadrant form over pencil-major spectra X[c][n][m][kx] with compile-time
   // x passes (length L) and row passes (length M); 0: plane-major X[kx][c][n][m]
   int pct;
+  // columns of block loads in flight per chunk of the binary16 block product
+  // (8, 16 or 32; 0: 16), timed when the FP32 preconditioner is made
+  int qu = 0;
   size_t lds_row, lds_tile;
   size_t lds1, lds2;
   FftPlan px, py;       // unpadded DFTs of length L and M
@@ -53,6 +56,11 @@ struct PcArgs {
   // block-Jacobi composite, whose earlier parts already wrote y)
   int skip_copy;
 };
+
+// Times the binary16 block product of g's quadrant form with 8, 16 and 32
+// columns of loads in flight (a few launches each on the preconditioner's own
+// workspace) and returns the fastest window.
+int tune_quadmul_window(const PcGeom& g, const PcArgs<float>& a, hipStream_t s);
 
 template <typename T>
 struct PrecPrec {
