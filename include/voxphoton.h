@@ -360,11 +360,12 @@ int vp_solve_fgmres_ir_batched(const vp_op* ops, const vp_prec* precs, int nsolv
  * No counterpart in the reference, whose scripts solve one wavelength after
  * another (Bragg_MOR_script.m:88-95, example_Bragg_1500nm_to_1600nm.m:85-114).
  * A server owns one HIP stream on the context's device and a worker thread.
- * Submitted mixed-precision solves (vp_solve_ir semantics, except that every
- * inner solve starts afresh: requests regroup between rounds, so no recycled
- * Krylov space is carried, as with VP_IR_RECYCLE=0) wait in a FIFO
- * queue and are admitted while the server's HBM budget (solver state of the
- * admitted requests: Krylov basis, refinement vectors, batch workspaces) and
+ * Submitted mixed-precision solves (vp_solve_ir semantics: each request
+ * keeps the recycled Krylov space of its first inner solve on the device
+ * between rounds, VP_SERVER_RECYCLE=0 restarts every inner solve afresh) wait
+ * in a FIFO queue and are admitted while the server's HBM budget (solver state
+ * of the admitted requests: Krylov basis, recycled space, refinement vectors,
+ * batch workspaces) and
  * its slot count allow.  Every round, the worker groups the admitted
  * requests by operator geometry and solve parameters and runs one
  * refinement step for each group as one batched launch sequence; requests

@@ -37,18 +37,14 @@ def _requests(vp_ctx):
 
 
 def _solo(vp_ctx, reqs):
-    # the server's refinement rounds start every inner solve afresh (requests
-    # regroup between rounds, so no Krylov space is carried over): the solo
-    # solves run the same way
+    # a served request keeps the recycled Krylov space of its first inner
+    # solve between rounds, as vp_solve_ir keeps it between refinement steps;
+    # requests that share a round carry zero columns for each other's
+    # narrower spaces, which changes no result, so the solo solve is vp_solve_ir
     import ctypes
-    import os
     from voxphoton_amd import _lib as L, vie
     out = []
-    os.environ['VP_IR_RECYCLE'] = '0'
-    try:
-        return _solo_runs(vp_ctx, reqs, L, vie, ctypes, out)
-    finally:
-        os.environ.pop('VP_IR_RECYCLE', None)
+    return _solo_runs(vp_ctx, reqs, L, vie, ctypes, out)
 
 
 def _solo_runs(vp_ctx, reqs, L, vie, ctypes, out):

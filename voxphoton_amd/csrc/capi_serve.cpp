@@ -9,7 +9,9 @@
 // (or the key that has waited longest), that no other executor is running,
 // admits what the queue allows, and runs one refinement step (ir_step_set:
 // FP64 residual, inner FP32 GMRES, update) for every active request of that
-// key as one batched launch sequence.  Groups of different keys therefore
+// key as one batched launch sequence.  Each request keeps the recycled Krylov
+// space of its first inner solve between rounds (VP_SERVER_RECYCLE=0: none).
+// Groups of different keys therefore
 // run at the same time on the device; requests admitted later join their
 // group at its next step, and finished requests leave and free their budget
 // at once.  Per-problem arithmetic in the batched kernels never depends on
@@ -53,7 +55,17 @@ struct ServeReq {
   std::string errmsg;
   Clock::time_point t_submit, t_done;
   int64_t admitted_round = 0;
+  // recycled Krylov space of the request's first inner solve (C and U,
+  // right-aligned in kcap = restart columns of complex64), kept between rounds
+  DevBuf rC, rU;
+  int rkc = 0;
 };
+
+// VP_SERVER_RECYCLE=0: every inner solve of a served request starts afresh.
+bool server_recycle() {
+  const char* e = std::getenv("VP_SERVER_RECYCLE");
+  return !(e && e[0] == '0');
+}
 
 // Solver-state bytes of one request when it runs in a batch: Krylov basis
 // and GMRES vectors (FP32), refinement and gathered vectors (FP64), and the
@@ -61,6 +73,8 @@ struct ServeReq {
 int64_t request_bytes(const vp_op_s* op, const vp_prec_s* pc, const vp_solve_params& prm) {
   const int64_t n = op->n;
   int64_t b = int64_t(prm.restart + 1 + 6) * n * 8 + int64_t(6) * n * 16;
+  // recycled space: its columns ahead of the basis in a round, and C and U kept between rounds
+  if (server_recycle()) b += int64_t(3) * prm.restart * n * 8;
   const Geometry& g = op->p32.present ? op->p32.g : op->p64.g;
   const int64_t w = int64_t(3) * g.x.P * op->Mb * op->Nb;
   const int64_t w2 = op->p32.plan.variant == 1 ? int64_t(3) * g.x.P * g.y.P * op->Nb : 0;
@@ -139,13 +153,53 @@ void run_round(const std::vector<ServeReq*>& g, hipStream_t st) {
   }
   std::vector<IrProblem> stt(np);
   for (int p = 0; p < np; ++p) stt[p] = g[p]->st;
+  // each request's recycled space enters the round's problem set (zero where
+  // a request has none yet: that request builds its own in this round) and
+  // leaves it again, so its inner solves depend on its own history only
+  Recycle rec;
+  const bool recycle = server_recycle();
+  const int kcap = prm.restart;
+  const int64_t vprob = int64_t(2 * kcap + 1) * n, colb = int64_t(kcap) * n * int64_t(sizeof(float2));
+  if (recycle) {
+    rec.alloc(n, np, kcap, sizeof(float2));
+    rec.kcp.assign(np, 0);
+    rec.want.assign(np, 0);
+    for (int p = 0; p < np; ++p) {
+      float2* cdst = rec.V.as<float2>() + p * vprob;
+      float2* udst = rec.U.as<float2>() + p * int64_t(kcap) * n;
+      if (g[p]->rkc > 0) {
+        VP_HIP(hipMemcpyAsync(cdst, g[p]->rC.p, colb, hipMemcpyDeviceToDevice, st));
+        VP_HIP(hipMemcpyAsync(udst, g[p]->rU.p, colb, hipMemcpyDeviceToDevice, st));
+      } else {
+        VP_HIP(hipMemsetAsync(cdst, 0, colb, st));
+        VP_HIP(hipMemsetAsync(udst, 0, colb, st));
+      }
+      rec.kcp[p] = g[p]->rkc;
+      rec.want[p] = g[p]->rkc == 0 ? 1 : 0;
+      rec.kc = std::max(rec.kc, g[p]->rkc);
+    }
+  }
   ir_step_set(p64, p32, B.as<double2>(), X.as<double2>(), o0->scale32, prm.tol, prm.restart, prm.inner_tol,
-              prm.inner_maxit, prm.max_outer, prm.side, stt);
+              prm.inner_maxit, prm.max_outer, prm.side, stt, recycle ? &rec : nullptr);
   for (int p = 0; p < np; ++p) {
     VP_HIP(hipMemcpyAsync(g[p]->x, X.as<double2>() + p * n, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
     g[p]->st = stt[p];
+    if (recycle && rec.kcp[p] > 0 && g[p]->rkc == 0 && !stt[p].done) {
+      g[p]->rC.alloc(size_t(colb));
+      g[p]->rU.alloc(size_t(colb));
+      VP_HIP(hipMemcpyAsync(g[p]->rC.p, rec.V.as<float2>() + p * vprob, colb, hipMemcpyDeviceToDevice, st));
+      VP_HIP(hipMemcpyAsync(g[p]->rU.p, rec.U.as<float2>() + p * int64_t(kcap) * n, colb, hipMemcpyDeviceToDevice,
+                            st));
+      g[p]->rkc = rec.kcp[p];
+    }
   }
   VP_HIP(hipStreamSynchronize(st));
+  for (int p = 0; p < np; ++p)
+    if (stt[p].done) {   // a finished request's space is freed at once
+      g[p]->rC.release();
+      g[p]->rU.release();
+      g[p]->rkc = 0;
+    }
 }
 
 // Admission (S->mu held): FIFO while the budget and the slots allow; a

@@ -435,9 +435,12 @@ bool ir_step_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double2*
       prm.tol_per[p] = need >= inner_tol / 8 ? need : inner_tol;
     }
   if (rec) {
-    // the first inner solve leaves its Krylov space as the recycled space of the later ones
+    // the first inner solve leaves its Krylov space as the recycled space of
+    // the later ones (per problem where the members bring their own spaces)
     prm.rec = rec;
-    if (rec->kc == 0) rec->build = true;
+    if (rec->want.empty() ? rec->kc == 0
+                          : std::any_of(rec->want.begin(), rec->want.end(), [](int v) { return v != 0; }))
+      rec->build = true;
   }
   GmresResult gr;
   gmres_device<float>(p32, r32.as<float2>(), d32.as<float2>(), prm, gr);

@@ -619,8 +619,9 @@ template <typename T>
 __global__ void __launch_bounds__(KT) k_tsgemm(const C<T>* __restrict__ in, int64_t iprob,
                                               const double2* __restrict__ S, int ls, int64_t sprob,
                                               const int* __restrict__ kin, int kout, C<T>* __restrict__ out,
-                                              int64_t oprob, int64_t n) {
+                                              int64_t oprob, int64_t n, const int* __restrict__ write = nullptr) {
   const int p = blockIdx.y;
+  if (write && !write[p]) return;   // this problem keeps what its output columns hold
   const int j0 = blockIdx.z * JO;
   const int K = kin[p];
   const C<T>* ip = in + p * iprob;
@@ -929,7 +930,8 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     if (kc > 0)
       hipLaunchKernelGGL(k_aug_coef, dim3((np + 63) / 64), dim3(64), 0, s, a0.as<double2>(), Bst.as<double2>(), kcap,
                          m, yv.as<double2>(), kk.as<int>(), maxcols, kc, coef.as<double2>(), np, mask);
-    const C<T>* Ub = kc > 0 ? rec->U.template as<C<T>>() : nullptr;
+    // U right-aligned in the kcap columns, as C is ahead of the basis
+    const C<T>* Ub = kc > 0 ? rec->U.template as<C<T>>() + int64_t(kcap - kc) * n : nullptr;
     const int64_t uprob = int64_t(kcap) * n;
     if (right) {
       hipLaunchKernelGGL(k_lincomb<T>, dim3(grid1(n), np), dim3(KT), 0, s, Vb, n, vprob, yv.as<double2>(),
@@ -952,8 +954,12 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
   // the Givens rotations G give G H = [R; 0] and Q = G^H (first K columns):
   // Aop V(:, 0..K-1) = V H = C R, so Aop U = C with C orthonormal
   auto build_recycled = [&](const std::vector<int>& lk) {
+    // problems that build a space now (the others keep theirs)
+    std::vector<int> wants(np, 1);
+    if (int(rec->want.size()) == np) wants = rec->want;
     int kmax = 0;   // Arnoldi columns of the last cycle per problem, at most kcap
-    for (int p = 0; p < np; ++p) kmax = std::max(kmax, std::min(lk[p], kcap));
+    for (int p = 0; p < np; ++p)
+      if (wants[p]) kmax = std::max(kmax, std::min(lk[p], kcap));
     if (kmax == 0) return;
     std::vector<double2> hH(size_t(np) * (m + 1) * m), hsn(size_t(np) * m);
     std::vector<double> hcs(size_t(np) * m);
@@ -966,7 +972,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     std::vector<std::vector<cd>> SCp(np), SUp(np);
     std::vector<int> Kp(np, 0), kop(np, 0);
     for (int p = 0; p < np; ++p) {
-      const int K = std::min(lk[p], kmax);
+      const int K = wants[p] ? std::min(lk[p], kmax) : 0;
       if (K == 0) continue;
       std::vector<cd> Q(size_t(K + 1) * (K + 1), cd(0.0));     // row-major
       for (int i = 0; i <= K; ++i) Q[size_t(i) * (K + 1) + i] = 1.0;
@@ -1041,21 +1047,28 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
       kinU[p] = K;
     }
     DevBuf dSC(SC.size() * sizeof(double2)), dSU(SU.size() * sizeof(double2)), dkC(np * sizeof(int)),
-        dkU(np * sizeof(int));
+        dkU(np * sizeof(int)), dwr(np * sizeof(int));
     VP_HIP(hipMemcpyAsync(dSC.p, SC.data(), dSC.bytes, hipMemcpyHostToDevice, s));
     VP_HIP(hipMemcpyAsync(dSU.p, SU.data(), dSU.bytes, hipMemcpyHostToDevice, s));
     VP_HIP(hipMemcpyAsync(dkC.p, kinC.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
     VP_HIP(hipMemcpyAsync(dkU.p, kinU.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    VP_HIP(hipMemcpyAsync(dwr.p, wants.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    // C and U right-aligned in the kcap columns; problems not building keep theirs
     C<T>* Cout = Vb - int64_t(kn) * n;
+    C<T>* Uout = rec->U.template as<C<T>>() + int64_t(kcap - kn) * n;
     const unsigned rblk = unsigned((n + KT - 1) / KT);
     hipLaunchKernelGGL(k_tsgemm<T>, dim3(rblk, np, ncol / JO), dim3(KT), 0, s, Vb, vprob, dSC.as<double2>(), lsC,
-                       int64_t(lsC) * ncol, dkC.as<int>(), kn, Cout, vprob, n);
+                       int64_t(lsC) * ncol, dkC.as<int>(), kn, Cout, vprob, n, dwr.as<int>());
     hipLaunchKernelGGL(k_tsgemm<T>, dim3(rblk, np, ncol / JO), dim3(KT), 0, s, Vb, vprob, dSU.as<double2>(), lsU,
-                       int64_t(lsU) * ncol, dkU.as<int>(), kn, rec->U.template as<C<T>>(), int64_t(kcap) * n, n);
+                       int64_t(lsU) * ncol, dkU.as<int>(), kn, Uout, int64_t(kcap) * n, n, dwr.as<int>());
     VP_HIP(hipGetLastError());
     VP_HIP(hipStreamSynchronize(s));
-    rec->kc = kn;
+    if (int(rec->kcp.size()) == np)
+      for (int p = 0; p < np; ++p)
+        if (wants[p] && kop[p] > 0) rec->kcp[p] = kn;
+    rec->kc = std::max(rec->kc, kn);
     rec->build = false;
+    if (int(rec->want.size()) == np) std::fill(rec->want.begin(), rec->want.end(), 0);
   };
   residual_into_V0(prm.x0_nonzero);
   std::vector<double> last(np);
@@ -1228,7 +1241,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     upload_act();
     residual_into_V0(true);
   }
-  if (rec && rec->build && kc == 0) build_recycled(lastk);
+  if (rec && rec->build && (kc == 0 || int(rec->want.size()) == np)) build_recycled(lastk);
   if (!prm.final_residual) {
     // caller recomputes the residual itself: report the recurrence estimate
     for (int p = 0; p < np; ++p) {
@@ -1426,7 +1439,8 @@ void fgmres_ir_set(ProblemSet<double>& p64, ProblemSet<float>& p32, const double
         }
       if (rec) {
         prm.rec = rec;
-        if (rec->kc == 0) rec->build = true;
+        if (rec->want.empty() ? rec->kc == 0 : std::any_of(rec->want.begin(), rec->want.end(), [](int v) { return v != 0; }))
+          rec->build = true;
       }
       GmresResult gr;
       gmres_device<float>(p32, r32.as<float2>(), d32.as<float2>(), prm, gr);

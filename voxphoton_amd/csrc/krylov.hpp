@@ -85,6 +85,13 @@ struct Recycle {
   int kcap = 0, m = 0;
   int kc = 0;          // recycled columns in use (0: none yet)
   bool build = false;  // form C and U from the last cycle of the next solve
+  // Per-problem spaces (problem sets whose members bring recycled spaces of
+  // their own, the request server): C and U are right-aligned in the kcap
+  // columns, member p's own columns are the last kcp[p] (columns left of them
+  // are zero, which changes no result), and want[p] asks the next solve to
+  // build p's space from its last cycle while the others keep theirs.  Empty:
+  // every problem has kc columns and builds together when kc == 0.
+  std::vector<int> kcp, want;
   void alloc(int64_t n, int np, int m_, size_t elem);
 };
 
