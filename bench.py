@@ -193,6 +193,7 @@ def main():
     ap.add_argument('--inner-maxit', type=int, default=None, help='override the workload inner restart cycles')
     ap.add_argument('--batch', type=int, default=None, help='sweep: wavelengths per batched solve (override)')
     ap.add_argument('--max-outer', type=int, default=None, help='override the workload refinement-step limit')
+    ap.add_argument('--nlam', type=int, default=None, help='sweep: number of wavelengths (override)')
     ap.add_argument('--reference-path', action='store_true',
                     help="also time one solve on the reference's FP64 GPU path (pgmres on host arrays, "
                          'transfers per operator application, operator at 2L x 2M x 2N)')
@@ -217,6 +218,8 @@ def main():
         wl['max_outer'] = args.max_outer
     if args.batch is not None and 'chunk' in wl:
         wl['chunk'] = args.batch
+    if args.nlam is not None and 'nlam' in wl:
+        wl['nlam'] = args.nlam
     log = lambda *a: print('[bench]', *a, file=sys.stderr, flush=True)   # noqa: E731
     if 'nlam' in wl:
         run_sweep(args, wl, log)
@@ -507,14 +510,8 @@ def run_sweep(args, wl, log):
     infos_all = []
     chunk = wl['chunk']
     lat = []
-    for c0 in range(0, len(lams), chunk):
-        t0 = time.perf_counter()
-        items = [build(lam) for lam in lams[c0:c0 + chunk]]
-        ctx.synchronize()
-        t_setup += time.perf_counter() - t0
-        ops = [it[1] for it in items for _ in pols]
-        precs = [it[2] for it in items for _ in pols]
-        B = np.array([r for it in items for r in it[3]])
+
+    def solve_batch(ops, precs, B):
         ns = B.shape[0]
         db = vie.DeviceArray(ctx, B.nbytes)
         dxv = vie.DeviceArray(ctx, B.nbytes)
@@ -522,21 +519,47 @@ def run_sweep(args, wl, log):
         hops = (ctypes.c_void_p * ns)(*[o.handle.value for o in ops])
         hpre = (ctypes.c_void_p * ns)(*[p.handle.value for p in precs])
         infos = (L.GmresInfo * ns)()
+        args = [ctypes.cast(hops, ctypes.c_void_p), ctypes.cast(hpre, ctypes.c_void_p), ns, L.VP_PREC_RIGHT, db.ptr,
+                dxv.ptr, wl['tol'], wl['restart'], wl['inner_tol'], wl['inner_maxit'], wl['max_outer']]
         ctx.synchronize()
         t0 = time.perf_counter()
-        L.check(L.lib.vp_solve_ir_batched(ctypes.cast(hops, ctypes.c_void_p), ctypes.cast(hpre, ctypes.c_void_p),
-                                          ns, L.VP_PREC_RIGHT, db.ptr, dxv.ptr, wl['tol'], wl['restart'],
-                                          wl['inner_tol'], wl['inner_maxit'], wl['max_outer'],
-                                          ctypes.cast(infos, ctypes.POINTER(L.GmresInfo))))
+        L.check(L.lib.vp_solve_ir_batched(*args, ctypes.cast(infos, ctypes.POINTER(L.GmresInfo))))
         ctx.synchronize()
-        dt = time.perf_counter() - t0
+        return time.perf_counter() - t0, [infos[i] for i in range(ns)]
+
+    for c0 in range(0, len(lams), chunk):
+        t0 = time.perf_counter()
+        items = [build(lam) for lam in lams[c0:c0 + chunk]]
+        ctx.synchronize()
+        t_setup += time.perf_counter() - t0
+        dt, infos = solve_batch([it[1] for it in items for _ in pols], [it[2] for it in items for _ in pols],
+                                np.array([r for it in items for r in it[3]]))
         t_solve += dt
         lat.append(dt)
-        infos_all += [dict(flag=infos[i].flag, relres=infos[i].relres, ir=infos[i].ir_steps,
-                           its=infos[i].total_iters) for i in range(ns)]
-        log(f'  wavelengths {c0}..{c0 + len(items) - 1}: {ns} solves in {dt:.2f} s '
-            f'(worst relres {max(i.relres for i in infos):.2e}), setup {t_setup:.1f} s so far')
-        del items, ops, precs, db, dxv
+        infos_all += [dict(flag=i.flag, relres=i.relres, ir=i.ir_steps, its=i.total_iters) for i in infos]
+        log(f'  wavelengths {c0}..{c0 + len(items) - 1}: {len(infos)} solves in {dt:.2f} s '
+            f'(worst relres {max(i.relres for i in infos):.2e}, inner its '
+            f'{[i.total_iters for i in infos]}), setup {t_setup:.1f} s so far')
+        del items
+    value = nsolve / t_solve
+    worst = max(i['relres'] for i in infos_all)
+    line = {
+        'metric': METRIC, 'value': round(value, 4), 'unit': 'solves/s', 'n_gpus': 1,
+        'steps': nsolve, 'warmup': 0, 'ms_per_step': round(t_solve / nsolve * 1e3, 3),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': PREC_DTYPE,
+        'data': 'synthetic (device-assembled Green tensor per wavelength, generated geometry)',
+        'config': {'workload': args.workload, 'description': wl['desc'], 'unknowns': n, 'solves': nsolve,
+                   'tol': wl['tol'], 'restart': wl['restart'], 'inner_tol': wl['inner_tol'],
+                   'batch': chunk * len(pols), 'preconditioning': 'right', 'parallelism': 'single GPU'},
+        'relres_worst': worst, 'all_converged': all(i['flag'] == 0 for i in infos_all),
+        'inner_iterations_mean': float(np.mean([i['its'] for i in infos_all])),
+        'batch_seconds': [round(v, 3) for v in lat],
+        'setup_s_per_wavelength': round(t_setup / len(lams), 3),
+    }
+    if args.no_fp64_solve:
+        print(json.dumps(line), flush=True)
+        return
     # plain FP64 serial baseline on the first and last wavelength
     t64 = []
     its64 = []
@@ -569,21 +592,7 @@ def run_sweep(args, wl, log):
             f'relres {ri["relres"]:.2e}, FFT {ri["fft"]}')
         del op, P
     rsec = float(np.mean([ri['seconds'] for ri in rinfos]))
-    value = nsolve / t_solve
-    worst = max(i['relres'] for i in infos_all)
-    line = {
-        'metric': METRIC, 'value': round(value, 4), 'unit': 'solves/s', 'n_gpus': 1,
-        'steps': nsolve, 'warmup': 0, 'ms_per_step': round(t_solve / nsolve * 1e3, 3),
-        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': PREC_DTYPE,
-        'data': 'synthetic (device-assembled Green tensor per wavelength, generated geometry)',
-        'config': {'workload': args.workload, 'description': wl['desc'], 'unknowns': n, 'solves': nsolve,
-                   'tol': wl['tol'], 'restart': wl['restart'], 'inner_tol': wl['inner_tol'],
-                   'batch': chunk * len(pols), 'preconditioning': 'right', 'parallelism': 'single GPU'},
-        'relres_worst': worst, 'all_converged': all(i['flag'] == 0 for i in infos_all),
-        'inner_iterations_mean': float(np.mean([i['its'] for i in infos_all])),
-        'batch_seconds': [round(v, 3) for v in lat],
-        'setup_s_per_wavelength': round(t_setup / len(lams), 3),
+    line.update({
         'fp64_plain_serial': {'seconds_per_solve': round(float(np.mean(t64)), 4), 'iterations': its64,
                               'solves_per_s': round(nsolve / fp64_serial, 4)},
         'mixed_batched_vs_fp64_serial': round(value / (nsolve / fp64_serial), 3),
@@ -598,7 +607,7 @@ def run_sweep(args, wl, log):
             'flags': [ri['flag'] for ri in rinfos], 'fft': list(rinfos[0]['fft']),
             'solves_per_s': round(1.0 / rsec, 4)},
         'mixed_batched_vs_reference_fp64_gpu_path': round(value * rsec, 2),
-    }
+    })
     print(json.dumps(line), flush=True)
 
 
