@@ -51,9 +51,10 @@ WORKLOADS = {
     'sweep_bragg_20nm': dict(
         desc='wavelength sweep of the 20 nm Bragg grating (1800x50x22, FFT 3600x100x44): 101 wavelengths '
              '1500-1600 nm, each with its own Green tensor and preconditioner, 2 right-hand sides per wavelength '
-             '(y- and z-polarised Gaussians), solved as batches of 4 wavelengths x 2 RHS',
-        tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=4, max_outer=12, prec='face', axis=1, damping=0.0,
-        base='bragg_20nm', nlam=101, lam0=1500e-9, lam1=1600e-9, chunk=4, pols=('y', 'z')),
+             '(y- and z-polarised Gaussians), solved as batches of 4 wavelengths x 2 RHS; GMRES(150), recycled '
+             'space of 30 harmonic Ritz directions',
+        tol=1e-6, restart=150, inner_tol=1e-2, inner_maxit=4, max_outer=12, prec='face', axis=1, damping=0.0,
+        base='bragg_20nm', nlam=101, lam0=1500e-9, lam1=1600e-9, chunk=4, pols=('y', 'z'), recycle_width=30),
     'mor_bragg_20nm': dict(
         desc='transmission spectrum of the 20 nm Bragg grating (1800x50x22, FFT 3600x100x44) by model order '
              'reduction: 5 Gauss-Legendre sample wavelengths in 1500-1600 nm, order 5 (25 solves per structure), '
@@ -194,6 +195,9 @@ def main():
     ap.add_argument('--batch', type=int, default=None, help='sweep: wavelengths per batched solve (override)')
     ap.add_argument('--max-outer', type=int, default=None, help='override the workload refinement-step limit')
     ap.add_argument('--nlam', type=int, default=None, help='sweep: number of wavelengths (override)')
+    ap.add_argument('--restart', type=int, default=None, help='override the workload GMRES restart')
+    ap.add_argument('--recycle-width', type=int, default=None,
+                    help='recycled Krylov space: harmonic Ritz directions kept (0: the whole cycle)')
     ap.add_argument('--reference-path', action='store_true',
                     help="also time one solve on the reference's FP64 GPU path (pgmres on host arrays, "
                          'transfers per operator application, operator at 2L x 2M x 2N)')
@@ -220,6 +224,10 @@ def main():
         wl['chunk'] = args.batch
     if args.nlam is not None and 'nlam' in wl:
         wl['nlam'] = args.nlam
+    if args.restart is not None:
+        wl['restart'] = args.restart
+    if args.recycle_width is not None:
+        wl['recycle_width'] = args.recycle_width
     log = lambda *a: print('[bench]', *a, file=sys.stderr, flush=True)   # noqa: E731
     if 'nlam' in wl:
         run_sweep(args, wl, log)
@@ -494,6 +502,7 @@ def run_sweep(args, wl, log):
     lams = np.linspace(wl['lam0'], wl['lam1'], wl['nlam'])
     pols = wl['pols']
     ctx = vie.default_context()
+    ctx.set_recycle_width(wl.get('recycle_width', -1))
     n = base.n
     nsolve = len(lams) * len(pols)
     log(f'sweep: {len(lams)} wavelengths x {len(pols)} RHS on grid {base.shape}, unknowns {n}')
@@ -551,6 +560,7 @@ def run_sweep(args, wl, log):
         'data': 'synthetic (device-assembled Green tensor per wavelength, generated geometry)',
         'config': {'workload': args.workload, 'description': wl['desc'], 'unknowns': n, 'solves': nsolve,
                    'tol': wl['tol'], 'restart': wl['restart'], 'inner_tol': wl['inner_tol'],
+                   'recycle_width': wl.get('recycle_width', 0),
                    'batch': chunk * len(pols), 'preconditioning': 'right', 'parallelism': 'single GPU'},
         'relres_worst': worst, 'all_converged': all(i['flag'] == 0 for i in infos_all),
         'inner_iterations_mean': float(np.mean([i['its'] for i in infos_all])),
@@ -878,27 +888,34 @@ def run_serve(args, wl, log):
     pending = []
     nxt = 0
     t_start = time.perf_counter()
-    while nxt < len(reqs) or pending or live:
-        now = time.perf_counter() - t_start
-        while nxt < len(reqs) and reqs[nxt][0] <= now:
-            pending.append(pool.submit(setup, nxt, *reqs[nxt]))
-            nxt += 1
-        for f in [f for f in pending if f.done()]:
-            pending.remove(f)
-            r = f.result()
-            pc = r['pc']
-            t_setup.append(r['setup'])
-            t_sub = time.perf_counter() - t_start
-            tk = srv.submit(r['op'], r['P'], r['db'], r['dxv'], tol=wl['tol'], restart=pc.get('restart', wl['restart']),
-                            inner_tol=wl['inner_tol'], inner_maxit=pc.get('inner_maxit', wl['inner_maxit']),
-                            max_outer=pc.get('max_outer', wl['max_outer']), side='right')
-            live[tk] = dict(i=r['i'], kind=r['kind'], size=r['size'], n=r['n'], t_arr=r['t_arr'], t_sub=t_sub,
-                            keep=(r['op'], r['P'], r['db'], r['dxv']))
-            log(f'  request {r["i"]} {r["kind"]} {r["size"]:.3g} n={r["n"]} arrival {r["t_arr"]:.3f} s submitted '
-                f'{t_sub:.3f} s (setup {r["setup"]:.2f} s), in flight {len(live)}, server {srv.stats()}')
-        collect()
-        time.sleep(0.001)
-    pool.shutdown()
+    try:
+        while nxt < len(reqs) or pending or live:
+            now = time.perf_counter() - t_start
+            while nxt < len(reqs) and reqs[nxt][0] <= now:
+                pending.append(pool.submit(setup, nxt, *reqs[nxt]))
+                nxt += 1
+            for f in [f for f in pending if f.done()]:
+                pending.remove(f)
+                r = f.result()
+                pc = r['pc']
+                t_setup.append(r['setup'])
+                t_sub = time.perf_counter() - t_start
+                tk = srv.submit(r['op'], r['P'], r['db'], r['dxv'], tol=wl['tol'], restart=pc.get('restart', wl['restart']),
+                                inner_tol=wl['inner_tol'], inner_maxit=pc.get('inner_maxit', wl['inner_maxit']),
+                                max_outer=pc.get('max_outer', wl['max_outer']), side='right')
+                live[tk] = dict(i=r['i'], kind=r['kind'], size=r['size'], n=r['n'], t_arr=r['t_arr'], t_sub=t_sub,
+                                keep=(r['op'], r['P'], r['db'], r['dxv']))
+                log(f'  request {r["i"]} {r["kind"]} {r["size"]:.3g} n={r["n"]} arrival {r["t_arr"]:.3f} s submitted '
+                    f'{t_sub:.3f} s (setup {r["setup"]:.2f} s), in flight {len(live)}, server {srv.stats()}')
+            collect()
+            time.sleep(0.001)
+        pool.shutdown()
+    except BaseException:
+        # a client that leaves on an exception first stops the server, whose
+        # executors may still be solving with the requests' operators and vectors
+        pool.shutdown(wait=True)
+        srv.close()
+        raise
     t_total = max(r['t_done'] for r in done) - reqs[0][0]
     srv.close()
     lat = np.array([r['info']['seconds'] for r in done]) * 1e3

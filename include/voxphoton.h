@@ -78,6 +78,18 @@ int vp_ctx_synchronize(vp_ctx ctx);
  * (reset != 0 clears them).  No counterpart in the reference (which only
  * prints tic/toc totals); used by the benchmark's roofline figures. */
 int vp_ctx_set_profiling(vp_ctx ctx, int on);
+/*
+ * Recycled Krylov space of the mixed-precision solves on this context
+ * (vp_solve_ir*, vp_solve_fgmres_ir*, the request server): the first inner
+ * solve's last cycle becomes the space the later inner solves are augmented
+ * with.  width 0 keeps the whole cycle; width k > 0 keeps the k harmonic Ritz
+ * directions of smallest modulus when the cycle is longer (GCRO-DR's
+ * selection); -1 (the default) takes VP_IR_RECYCLE_K from the environment,
+ * else 0.  The request server's rounds always keep the whole cycle.  A long cycle kept whole makes every later Arnoldi step
+ * orthogonalise against all of it; a narrow space saves that work when few
+ * inner solves follow.  No counterpart in the reference.
+ */
+int vp_ctx_set_recycle_width(vp_ctx ctx, int width);
 int vp_ctx_kernel_stats(vp_ctx ctx, const char* kernel, int64_t* launches, double* total_ms, int reset);
 /* Free and total device memory of the context's GPU (bytes); the free figure
  * counts the library's cache of released buffers as used. */

@@ -478,6 +478,51 @@ def test_mixed_precision_ir_reaches_fp64_tolerance(vp_ctx):
     assert abs(true_rel - info['relres']) <= 1e-3 * true_rel + 1e-15
 
 
+def test_recycle_width_of_the_context(vp_ctx, monkeypatch):
+    # vp_ctx_set_recycle_width: width 0 keeps the whole last cycle (equal bit
+    # for bit to the default without VP_IR_RECYCLE_K), width k the k harmonic
+    # Ritz directions; with a narrow space the solve still reaches the
+    # tolerance (the oracle recomputes the residual) and batched = solo
+    from voxphoton_amd.vie import NOperator, Preconditioner, solve_ir, solve_ir_batched
+    monkeypatch.delenv('VP_IR_RECYCLE_K', raising=False)
+    L, M, N = 12, 6, 4
+    G, Mc, Gram, idx, b, dx, ko = _physical(L, M, N, seed=3)
+    op = NOperator.from_toeplitz(G, None, Mc, Gram, idx, ctx=vp_ctx)
+    P = Preconditioner.level2(G, Mc.flat[0], 1.0, Gram, idx, ctx=vp_ctx)
+    fN = ovie.fN_from_toeplitz(G)
+    ones = np.ones(Mc.shape, complex)
+    # short inner solves that run their whole cycle, so the cycle is wider
+    # than the narrow space
+    prm = dict(tol=1e-10, restart=12, inner_tol=1e-4, inner_maxit=1, max_outer=60, M=P, side='right')
+    try:
+        x_def, i_def = solve_ir(op, b, **prm)
+        vp_ctx.set_recycle_width(0)
+        x0, i0 = solve_ir(op, b, **prm)
+        assert np.array_equal(x_def, x0) and i_def['inner_iters'] == i0['inner_iters']
+        vp_ctx.set_recycle_width(4)
+        x4, i4 = solve_ir(op, b, **prm)
+        r = b - ovie.mv_AN_const(x4, fN, ones, Mc, Gram, 'notransp', idx)
+        true_rel = np.linalg.norm(r) / np.linalg.norm(b)
+        print(f'recycled width 4: {i4["ir_steps"]} steps {i4["inner_iters"]} inner its relres {i4["relres"]:.3e} '
+              f'(oracle {true_rel:.3e}); whole cycle {i0["ir_steps"]} steps {i0["inner_iters"]} its', flush=True)
+        assert i4['flag'] == 0 and true_rel <= 1e-10 * 1.01
+        assert abs(true_rel - i4['relres']) <= 1e-3 * true_rel + 1e-15
+        assert not np.array_equal(x4, x0)
+        rng = np.random.default_rng(6)
+        B = np.array([b, (rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size)) * np.abs(b).max()])
+        bprm = dict(prm)
+        bprm.pop('M')
+        X, infos = solve_ir_batched([op, op], [P, P], B, **bprm)
+        for i in range(2):
+            xs, inf_s = solve_ir(op, B[i], **prm)
+            assert np.array_equal(X[i], xs) and infos[i]['inner_iters'] == inf_s['inner_iters']
+        from voxphoton_amd._lib import VoxPhotonError
+        with pytest.raises(VoxPhotonError):
+            vp_ctx.set_recycle_width(-2)
+    finally:
+        vp_ctx.set_recycle_width(-1)
+
+
 @pytest.mark.parametrize('outer_m', [30, 3])
 def test_flexible_refinement_reaches_fp64_tolerance(vp_ctx, outer_m):
     # vp_solve_fgmres_ir: FGMRES in FP64 over the inner FP32 solves.  The true

@@ -58,6 +58,7 @@ SIGNATURES = {
     'vp_ctx_destroy': (c_int, [c_vp]),
     'vp_ctx_synchronize': (c_int, [c_vp]),
     'vp_ctx_set_profiling': (c_int, [c_vp, c_int]),
+    'vp_ctx_set_recycle_width': (c_int, [c_vp, c_int]),
     'vp_ctx_kernel_stats': (c_int, [c_vp, ctypes.c_char_p, P(c_i64), P(c_dbl), c_int]),
     'vp_mem_info': (c_int, [c_vp, P(c_i64), P(c_i64)]),
     'vp_malloc': (c_int, [c_vp, c_i64, P(c_vp)]),

@@ -89,6 +89,14 @@ int vp_ctx_synchronize(vp_ctx ctx) {
   VP_API_END
 }
 
+int vp_ctx_set_recycle_width(vp_ctx ctx, int width) {
+  VP_API_BEGIN
+  VP_REQUIRE(ctx, ERR_INVALID, "ctx is NULL");
+  VP_REQUIRE(width >= -1 && width <= 1000, ERR_INVALID, "recycle width must be -1..1000");
+  ctx->recycle_width = width;
+  VP_API_END
+}
+
 int vp_ctx_set_profiling(vp_ctx ctx, int on) {
   VP_API_BEGIN
   VP_REQUIRE(ctx, ERR_INVALID, "ctx is NULL");

@@ -143,6 +143,9 @@ void run_round(const std::vector<ServeReq*>& g, hipStream_t st) {
   ProblemSet<float> p32;
   make_problem_set<float>(p32, ops.data(), precs[0] ? precs.data() : nullptr, np, true);
   p32.stream = st;
+  // server rounds keep the whole last cycle as a request's recycled space
+  // (the narrower harmonic selection of the context setting is not used here)
+  p32.recycle_width = 0;
   DevBuf B(size_t(n) * np * sizeof(double2)), X(size_t(n) * np * sizeof(double2));
   for (int p = 0; p < np; ++p) {
     VP_HIP(hipMemcpyAsync(B.as<double2>() + p * n, g[p]->b, n * sizeof(double2), hipMemcpyDeviceToDevice, st));

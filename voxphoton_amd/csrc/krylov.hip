@@ -614,9 +614,7 @@ constexpr int JO = 32;   // output columns per tall-skinny product block
 // out[p][:, j] = sum_{i < kin[p]} in[p][:, i] S[p][i + ls j] for j < kout
 // (S FP64 with at least ceil(kout / JO) * JO columns, zero past kout).  One
 // row per thread and JO outputs per block, so each input column is read
-// ceil(kout / JO) times.  The coefficients of IC inputs at a time sit in LDS
-// (every lane reads the same word: a broadcast); each complex product is
-// accumulated as four FP64 FMAs, the inputs in increasing order.
+// ceil(kout / JO) times; S is read uniformly.
 template <typename T>
 __global__ void __launch_bounds__(KT) k_tsgemm(const C<T>* __restrict__ in, int64_t iprob,
                                               const double2* __restrict__ S, int ls, int64_t sprob,
@@ -633,35 +631,13 @@ __global__ void __launch_bounds__(KT) k_tsgemm(const C<T>* __restrict__ in, int6
   double2 acc[JO];
 #pragma unroll
   for (int jj = 0; jj < JO; ++jj) acc[jj] = make_double2(0.0, 0.0);
-  constexpr int IC = 32, IU = 8;
-  __shared__ double2 sS[JO * IC];
-  for (int i0 = 0; i0 < K; i0 += IC) {
-    const int ic = min(IC, K - i0);
-    __syncthreads();   // the previous chunk's coefficients are no longer read
-    for (int e = threadIdx.x; e < JO * IC; e += KT) {
-      const int jj = e / IC, ii = e - jj * IC;
-      sS[e] = ii < ic ? Sp[int64_t(jj) * ls + i0 + ii] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    for (int iu = 0; iu < ic; iu += IU) {
-      C<T> vv[IU];
+  for (int i = 0; i < K; ++i) {
+    const double2 v = to_d(ip[int64_t(i) * n + rr]);
 #pragma unroll
-      for (int u = 0; u < IU; ++u) vv[u] = ip[int64_t(i0 + min(iu + u, ic - 1)) * n + rr];
-#pragma unroll
-      for (int u = 0; u < IU; ++u) {
-        if (iu + u < ic) {
-          const double2 v = to_d(vv[u]);
-          const double2* cs = sS + iu + u;
-#pragma unroll
-          for (int jj = 0; jj < JO; ++jj) {
-            const double2 c = cs[jj * IC];
-            acc[jj].x = fma(v.x, c.x, acc[jj].x);
-            acc[jj].x = fma(-v.y, c.y, acc[jj].x);
-            acc[jj].y = fma(v.x, c.y, acc[jj].y);
-            acc[jj].y = fma(v.y, c.x, acc[jj].y);
-          }
-        }
-      }
+    for (int jj = 0; jj < JO; ++jj) {
+      const double2 c = Sp[int64_t(jj) * ls + i];
+      acc[jj].x += v.x * c.x - v.y * c.y;
+      acc[jj].y += v.x * c.y + v.y * c.x;
     }
   }
   if (r >= n) return;
@@ -772,9 +748,11 @@ __global__ void k_copy2(const C<T>* __restrict__ a, int64_t as, C<T>* __restrict
 }
 
 // Width of the recycled space kept after the first inner solve of a
-// refinement run (VP_IR_RECYCLE_K): 0 keeps the whole last cycle; k > 0 the
-// k harmonic Ritz directions of smallest modulus (harmonic_pair).
-static int recycle_k() {
+// refinement run: the context's setting (vp_ctx_set_recycle_width), else
+// VP_IR_RECYCLE_K, else 0.  0 keeps the whole last cycle; k > 0 the k
+// harmonic Ritz directions of smallest modulus (harmonic_pair).
+static int recycle_k(int ctx_width) {
+  if (ctx_width >= 0) return ctx_width;
   const char* e = std::getenv("VP_IR_RECYCLE_K");
   return e ? std::max(0, std::atoi(e)) : 0;
 }
@@ -991,7 +969,7 @@ void gmres_device(ProblemSet<T>& ps, const C<T>* b, C<T>* x, const GmresParams& 
     VP_HIP(hipMemcpy(hsn.data(), sn.p, hsn.size() * sizeof(double2), hipMemcpyDeviceToHost));
     VP_HIP(hipMemcpy(hcs.data(), cs.p, hcs.size() * sizeof(double), hipMemcpyDeviceToHost));
     using cd = std::complex<double>;
-    const int kdef = recycle_k();
+    const int kdef = recycle_k(ps.recycle_width);
     // per problem: SC ((K+1) x ko) and SU (K x ko), column-major
     std::vector<std::vector<cd>> SCp(np), SUp(np);
     std::vector<int> Kp(np, 0), kop(np, 0);

@@ -58,6 +58,7 @@ struct ProblemSet {
   std::vector<std::unique_ptr<PrecPart>> pparts;
   DevBuf active;                  // int[nprob]
   std::unique_ptr<GmresWork, GmresWorkDeleter> gw;   // gmres_device workspaces and step graphs
+  int recycle_width = -1;         // the context's recycled-space width (vp_ctx_s::recycle_width)
 };
 
 template <typename T>

@@ -14,6 +14,10 @@ struct vp_ctx_s {
   std::mutex mu;
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  // recycled Krylov space of mixed-precision solves: at most this many
+  // harmonic Ritz directions of the first inner solve's last cycle (0: the
+  // whole cycle; -1: VP_IR_RECYCLE_K or the whole cycle)
+  int recycle_width = -1;
 };
 
 namespace vp {

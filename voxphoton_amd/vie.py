@@ -32,6 +32,13 @@ class Context:
     def synchronize(self):
         check(L.lib.vp_ctx_synchronize(self.handle))
 
+    def set_recycle_width(self, width):
+        """Recycled Krylov space of the mixed-precision solves on this
+        context (vp_ctx_set_recycle_width): 0 keeps the first inner solve's
+        whole last cycle, k > 0 its k harmonic Ritz directions of smallest
+        modulus, -1 the environment's VP_IR_RECYCLE_K (default)."""
+        check(L.lib.vp_ctx_set_recycle_width(self.handle, int(width)))
+
     def mem_info(self):
         """(free, total) device memory in bytes (vp_mem_info)."""
         f, t = ctypes.c_int64(), ctypes.c_int64()
@@ -523,11 +530,22 @@ class Server:
         check(L.lib.vp_server_create(self.ctx.handle, int(hbm_budget_bytes), int(max_active), ctypes.byref(h)))
         self.handle = h
         self._keep = {}
+        self._fin = weakref.finalize(self, L.lib.vp_server_destroy, h)
+
+    def _renew_finalizer(self):
+        # At interpreter exit finalizers run newest first.  The server's own
+        # is made newer than everything a request uses (all created before
+        # its submit), so the server stops and drains its executors before
+        # those operators and vectors are freed, also when the caller leaves
+        # on an exception with requests in flight.
+        if self._fin.detach() is not None:
+            self._fin = weakref.finalize(self, L.lib.vp_server_destroy, self.handle)
 
     def submit(self, op, M, b, x, tol=1e-6, restart=50, inner_tol=1e-3, inner_maxit=4, max_outer=20, side='right'):
         prm = L.SolveParams(float(tol), int(restart), float(inner_tol), int(inner_maxit), int(max_outer),
                             L.VP_PREC_RIGHT if side == 'right' else L.VP_PREC_LEFT)
         t = ctypes.c_int64()
+        self._renew_finalizer()
         check(L.lib.vp_server_submit(self.handle, op.handle, M.handle if M is not None else None, b.ptr, x.ptr,
                                      ctypes.byref(prm), ctypes.byref(t)))
         self._keep[t.value] = (op, M, b, x, max_outer)
@@ -557,9 +575,12 @@ class Server:
 
     def close(self):
         if self.handle:
-            check(L.lib.vp_server_destroy(self.handle))
-            self.handle = None
-            self._keep.clear()
+            self._fin.detach()
+            h, self.handle = self.handle, None
+            try:
+                check(L.lib.vp_server_destroy(h))
+            finally:
+                self._keep.clear()
 
 
 def E_field_DGF_mollified(J, Icoord, Ocoord, ko, cutoff, ctx=None):
