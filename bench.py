@@ -634,6 +634,7 @@ def run_mor(args, wl, log):
     from voxphoton_amd import vie, configs, mor
     import dataclasses
     ctx = vie.default_context()
+    ctx.set_recycle_width(wl.get('recycle_width', -1))
     base = configs.config_problem(wl['base'])
     straight = configs.bragg_grating(dx=base.dx, corr=0.0, M=base.M, N=base.N)
     xg, _ = np.polynomial.legendre.leggauss(wl['nG'])
@@ -732,7 +733,8 @@ def run_mor(args, wl, log):
         'data': 'synthetic (device-assembled Green tensors, generated geometry)',
         'config': {'workload': args.workload, 'description': wl['desc'], 'unknowns': base.n,
                    'samples_nm': [round(v * 1e9, 3) for v in s], 'order': wl['order'], 'tol': wl['tol'],
-                   'check_tol': wl['check_tol'],
+                   'check_tol': wl['check_tol'], 'restart': wl['restart'],
+                   'recycle_width': wl.get('recycle_width', 0),
                    'wavelengths': int(lams.size), 'parallelism': 'single GPU'},
         'total_s': round(t_total, 3), 'stages': out, 'checks': checks,
         'direct_full_solves_estimate_s': round(direct, 1),
