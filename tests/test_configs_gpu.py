@@ -269,8 +269,6 @@ def _serve_setup(vp_ctx):
     return reqs
 
 
-@pytest.mark.skip(reason="the request server left the GPU faulted in a run of this test and of the mixed "
-                         "serving stream; its cause is not found yet, so the server stays off the GPU until it is")
 @pytest.mark.timeout(900)
 def test_serving_stream_against_oracle_and_alone(vp_ctx):
     from voxphoton_amd import vie

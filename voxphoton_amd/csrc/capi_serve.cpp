@@ -186,7 +186,6 @@ void run_round(const std::vector<ServeReq*>& g, hipStream_t st) {
               prm.inner_maxit, prm.max_outer, prm.side, stt, recycle ? &rec : nullptr);
   for (int p = 0; p < np; ++p) {
     VP_HIP(hipMemcpyAsync(g[p]->x, X.as<double2>() + p * n, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
-    g[p]->st = stt[p];
     if (recycle && rec.kcp[p] > 0 && g[p]->rkc == 0 && !stt[p].done) {
       g[p]->rC.alloc(size_t(colb));
       g[p]->rU.alloc(size_t(colb));
@@ -203,6 +202,9 @@ void run_round(const std::vector<ServeReq*>& g, hipStream_t st) {
       g[p]->rU.release();
       g[p]->rkc = 0;
     }
+  // the new states last, once the round's device work has finished: a
+  // request seen as done may be collected (and deleted) by its client
+  for (int p = 0; p < np; ++p) g[p]->st = stt[p];
 }
 
 // Admission (S->mu held): FIFO while the budget and the slots allow; a
@@ -226,7 +228,8 @@ std::string pick_key(vp_server_s* S) {
   std::string best;
   double best_score = 0.0;
   for (ServeReq* r : S->active) {
-    if (r->st.done || S->busy.count(r->key)) continue;
+    // a busy key's requests belong to the executor running their round
+    if (S->busy.count(r->key) || r->st.done) continue;
     auto it = S->last.find(r->key);
     const int64_t l = it == S->last.end() ? -1 : it->second;
     double score = double(l);
@@ -266,12 +269,14 @@ void executor(vp_server_s* S, int k) {
     try {
       run_round(batch, st);
     } catch (const VpError& e) {
+      (void)hipStreamSynchronize(st);   // the failed round's queued work ends before its requests do
       for (ServeReq* r : batch) {
         r->error = e.code;
         r->errmsg = e.msg;
         r->st.done = true;
       }
     } catch (const std::exception& e) {
+      (void)hipStreamSynchronize(st);
       for (ServeReq* r : batch) {
         r->error = VP_ERR_DEVICE;
         r->errmsg = e.what();
@@ -282,10 +287,12 @@ void executor(vp_server_s* S, int k) {
       std::lock_guard<std::mutex> lk(S->mu);
       S->busy.erase(key);
       const auto now = Clock::now();
+      // only this round's requests: another executor may be running a round
+      // of other requests, whose states are its own until that round ends
       std::vector<ServeReq*> keep;
       bool any_done = false;
       for (ServeReq* r : S->active) {
-        if (r->st.done) {
+        if (std::find(batch.begin(), batch.end(), r) != batch.end() && r->st.done) {
           r->finished = true;
           r->t_done = now;
           S->used -= r->bytes;
