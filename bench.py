@@ -73,9 +73,10 @@ WORKLOADS = {
         desc='serving stream: seeded Poisson arrivals (mean 50 req/s) of mixed solves at 20 nm voxels, '
              'lambda U[1500,1600] nm: 40% straight waveguides of length U[2,8] um, 40% Bragg gratings of '
              'U{20..150} periods, 20% microdisks of radius U[2,5] um (220 nm thick); requests admitted by the '
-             'server HBM budget, refinement rounds batched per geometry',
-        tol=1e-6, restart=100, inner_tol=1e-2, inner_maxit=4, max_outer=20, nreq=40, rate=50.0, seed=4,
-        budget_gb=96, max_active=32, setup_workers=8, fft_sizes='min'),
+             'server HBM budget, refinement rounds batched per geometry; GMRES(150), recycled space of 30 '
+             'harmonic Ritz directions per request',
+        tol=1e-6, restart=150, inner_tol=1e-2, inner_maxit=4, max_outer=20, nreq=40, rate=50.0, seed=4,
+        budget_gb=96, max_active=32, setup_workers=8, fft_sizes='min', recycle_width=30),
     'microdisk_15nm': dict(
         desc='Si microdisk r = 10 um, 220 nm thick, with a 500 nm bus waveguide at a 200 nm gap (quadratic '
              'absorbers, Im eps 0->1 over 2 um, at both bus ends), SiO2, lambda 1550 nm, 15 nm voxels, grid '
@@ -849,6 +850,7 @@ def run_serve(args, wl, log):
         # timing candidates on a device that is busy with other solves
         os.environ['VP_FFT_SIZES'] = wl['fft_sizes']
     ctx = vie.default_context()
+    ctx.set_recycle_width(wl.get('recycle_width', -1))
     srv = vie.Server(int(wl['budget_gb'] * (1 << 30)), wl['max_active'], ctx=ctx)
     live = {}
     done = []
@@ -932,7 +934,8 @@ def run_serve(args, wl, log):
         'data': 'synthetic (seeded request stream; device-assembled Green tensors)',
         'config': {'workload': args.workload, 'description': wl['desc'], 'requests': len(done), 'mix': kinds,
                    'unknowns_min': int(min(r['n'] for r in done)), 'unknowns_max': int(max(r['n'] for r in done)),
-                   'tol': wl['tol'], 'restart': wl['restart'], 'hbm_budget_gb': wl['budget_gb'],
+                   'tol': wl['tol'], 'restart': wl['restart'], 'recycle_width': wl.get('recycle_width', 0),
+                   'hbm_budget_gb': wl['budget_gb'],
                    'max_active': wl['max_active'], 'parallelism': 'single GPU'},
         'relres_worst': worst, 'all_converged': all(r['info']['flag'] == 0 for r in done),
         'latency_ms': {'p50': round(float(np.percentile(lat, 50)), 3), 'p99': round(float(np.percentile(lat, 99)), 3),

@@ -85,7 +85,8 @@ int vp_ctx_set_profiling(vp_ctx ctx, int on);
  * with.  width 0 keeps the whole cycle; width k > 0 keeps the k harmonic Ritz
  * directions of smallest modulus when the cycle is longer (GCRO-DR's
  * selection); -1 (the default) takes VP_IR_RECYCLE_K from the environment,
- * else 0.  The request server's rounds always keep the whole cycle.  A long cycle kept whole makes every later Arnoldi step
+ * else 0.  The request server's rounds use the setting of the context the
+ * server was created on.  A long cycle kept whole makes every later Arnoldi step
  * orthogonalise against all of it; a narrow space saves that work when few
  * inner solves follow.  No counterpart in the reference.
  */
@@ -386,7 +387,9 @@ int vp_solve_fgmres_ir_batched(const vp_op* ops, const vp_prec* precs, int nsolv
  * not depend on its group: results are identical alone and in the mix.
  * b and x are device complex128 vectors of the operator's length; the caller
  * keeps b, x, the operator and the preconditioner alive and does not use
- * them until the request is done.
+ * them until the request is done.  Rounds keep the recycled space as the
+ * creating context's vp_ctx_set_recycle_width says; that context outlives
+ * the server.
  */
 typedef struct vp_server_s* vp_server;
 typedef struct {

@@ -60,8 +60,20 @@ def _solo_runs(vp_ctx, reqs, L, vie, ctypes, out):
     return out
 
 
-def test_server_results_equal_solo_solves(vp_ctx):
+@pytest.mark.parametrize('width', [-1, 4])
+def test_server_results_equal_solo_solves(vp_ctx, monkeypatch, width):
+    # width: the context's recycled-space width, which the server's rounds and
+    # the solo solves both take (4: harmonic directions, narrower than a cycle)
     from voxphoton_amd import vie
+    monkeypatch.delenv('VP_IR_RECYCLE_K', raising=False)
+    vp_ctx.set_recycle_width(width)
+    try:
+        _server_equals_solo(vp_ctx, vie)
+    finally:
+        vp_ctx.set_recycle_width(-1)
+
+
+def _server_equals_solo(vp_ctx, vie):
     reqs = _requests(vp_ctx)
     solo = _solo(vp_ctx, reqs)
     srv = vie.Server(hbm_budget_bytes=8 << 30, max_active=16, ctx=vp_ctx)

@@ -106,6 +106,7 @@ std::string group_key(const vp_op_s* o, const vp_prec_s* q, const vp_solve_param
 
 struct vp_server_s {
   int device = 0;
+  vp_ctx ctx = nullptr;                 // the creating context (its recycle width applies to rounds)
   std::vector<hipStream_t> streams;     // one per executor
   int64_t budget = 0;
   int max_active = 1;
@@ -125,7 +126,7 @@ struct vp_server_s {
 namespace {
 
 // One refinement step for a group of admitted requests of one key, on stream st.
-void run_round(const std::vector<ServeReq*>& g, hipStream_t st) {
+void run_round(const std::vector<ServeReq*>& g, hipStream_t st, int recycle_width) {
   const int np = int(g.size());
   vp_op_s* o0 = g[0]->op;
   const int64_t n = o0->n;
@@ -143,9 +144,7 @@ void run_round(const std::vector<ServeReq*>& g, hipStream_t st) {
   ProblemSet<float> p32;
   make_problem_set<float>(p32, ops.data(), precs[0] ? precs.data() : nullptr, np, true);
   p32.stream = st;
-  // server rounds keep the whole last cycle as a request's recycled space
-  // (the narrower harmonic selection of the context setting is not used here)
-  p32.recycle_width = 0;
+  p32.recycle_width = recycle_width;
   DevBuf B(size_t(n) * np * sizeof(double2)), X(size_t(n) * np * sizeof(double2));
   for (int p = 0; p < np; ++p) {
     VP_HIP(hipMemcpyAsync(B.as<double2>() + p * n, g[p]->b, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
@@ -267,7 +266,7 @@ void executor(vp_server_s* S, int k) {
       S->last[key] = S->rounds;
     }
     try {
-      run_round(batch, st);
+      run_round(batch, st, S->ctx->recycle_width);
     } catch (const VpError& e) {
       (void)hipStreamSynchronize(st);   // the failed round's queued work ends before its requests do
       for (ServeReq* r : batch) {
@@ -344,6 +343,7 @@ int vp_server_create(vp_ctx ctx, int64_t hbm_budget_bytes, int max_active, vp_se
   VP_HIP(hipSetDevice(ctx->device));
   auto S = std::make_unique<vp_server_s>();
   S->device = ctx->device;
+  S->ctx = ctx;
   S->budget = hbm_budget_bytes;
   S->max_active = max_active;
   int ns = 3;

@@ -197,6 +197,14 @@ __global__ void k_scale_c128(double2* a, int64_t n, double s) {
     a[i] = make_double2(a[i].x * s, a[i].y * s);
 }
 
+// An embedded apply first copies x to y (unknowns outside the box pass
+// through), except for the later parts of a composite (the first part made
+// the copy) and for a box holding every unknown of the outer vector: the
+// embedding is then a permutation and the x inverse pass writes every entry.
+static int skip_outer_copy(const vp_prec_s* q, size_t part) {
+  return (part > 0 || (q->g.n_outer > 0 && q->nbox == q->g.n_outer)) ? 1 : 0;
+}
+
 template <typename T>
 static void prec_apply_dev(vp_prec_s* p, const C<T>* x, C<T>* y) {
   const auto leaves = prec_leaves(p);
@@ -205,7 +213,7 @@ static void prec_apply_dev(vp_prec_s* p, const C<T>* x, C<T>* y) {
     PrecPrec<T>& P = pprec_of<T>(q);
     VP_REQUIRE(P.present, ERR_INVALID, "preconditioner lacks this precision");
     PcArgs<T> a = prec_args<T>(q, x, y);
-    a.skip_copy = k > 0 ? 1 : 0;
+    a.skip_copy = skip_outer_copy(q, k);
     launch_prec<T>(q->g, a, 1, p->ctx->stream);
   }
 }
@@ -316,7 +324,7 @@ void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* 
       a.blocks = part->pblocks.template as<const C<T>*>();
       a.hblocks = nullptr;
       a.hscale = nullptr;
-      a.skip_copy = k > 0 ? 1 : 0;
+      a.skip_copy = skip_outer_copy(q0, k);
       if (half) {   // every problem's preconditioner has the binary16 copy
         part->phblocks.alloc(sizeof(const uint32_t*) * np);
         VP_HIP(hipMemcpy(part->phblocks.p, hps.data(), part->phblocks.bytes, hipMemcpyHostToDevice));

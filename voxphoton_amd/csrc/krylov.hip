@@ -614,7 +614,9 @@ constexpr int JO = 32;   // output columns per tall-skinny product block
 // out[p][:, j] = sum_{i < kin[p]} in[p][:, i] S[p][i + ls j] for j < kout
 // (S FP64 with at least ceil(kout / JO) * JO columns, zero past kout).  One
 // row per thread and JO outputs per block, so each input column is read
-// ceil(kout / JO) times; S is read uniformly.
+// ceil(kout / JO) times.  The coefficients of IC inputs at a time sit in LDS
+// (every lane reads the same word: a broadcast); each complex product is
+// accumulated as four FP64 FMAs, the inputs in increasing order.
 template <typename T>
 __global__ void __launch_bounds__(KT) k_tsgemm(const C<T>* __restrict__ in, int64_t iprob,
                                               const double2* __restrict__ S, int ls, int64_t sprob,
@@ -631,13 +633,35 @@ __global__ void __launch_bounds__(KT) k_tsgemm(const C<T>* __restrict__ in, int6
   double2 acc[JO];
 #pragma unroll
   for (int jj = 0; jj < JO; ++jj) acc[jj] = make_double2(0.0, 0.0);
-  for (int i = 0; i < K; ++i) {
-    const double2 v = to_d(ip[int64_t(i) * n + rr]);
+  constexpr int IC = 32, IU = 8;
+  __shared__ double2 sS[JO * IC];
+  for (int i0 = 0; i0 < K; i0 += IC) {
+    const int ic = min(IC, K - i0);
+    __syncthreads();   // the previous chunk's coefficients are no longer read
+    for (int e = threadIdx.x; e < JO * IC; e += KT) {
+      const int jj = e / IC, ii = e - jj * IC;
+      sS[e] = ii < ic ? Sp[int64_t(jj) * ls + i0 + ii] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int iu = 0; iu < ic; iu += IU) {
+      C<T> vv[IU];
 #pragma unroll
-    for (int jj = 0; jj < JO; ++jj) {
-      const double2 c = Sp[int64_t(jj) * ls + i];
-      acc[jj].x += v.x * c.x - v.y * c.y;
-      acc[jj].y += v.x * c.y + v.y * c.x;
+      for (int u = 0; u < IU; ++u) vv[u] = ip[int64_t(i0 + min(iu + u, ic - 1)) * n + rr];
+#pragma unroll
+      for (int u = 0; u < IU; ++u) {
+        if (iu + u < ic) {
+          const double2 v = to_d(vv[u]);
+          const double2* cs = sS + iu + u;
+#pragma unroll
+          for (int jj = 0; jj < JO; ++jj) {
+            const double2 c = cs[jj * IC];
+            acc[jj].x = fma(v.x, c.x, acc[jj].x);
+            acc[jj].x = fma(-v.y, c.y, acc[jj].x);
+            acc[jj].y = fma(v.x, c.y, acc[jj].y);
+            acc[jj].y = fma(v.y, c.x, acc[jj].y);
+          }
+        }
+      }
     }
   }
   if (r >= n) return;

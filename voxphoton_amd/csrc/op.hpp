@@ -1,5 +1,6 @@
 // Operator / context objects behind the opaque C handles.
 #pragma once
+#include <atomic>
 #include <memory>
 #include <mutex>
 #include <vector>
@@ -17,7 +18,7 @@ struct vp_ctx_s {
   // recycled Krylov space of mixed-precision solves: at most this many
   // harmonic Ritz directions of the first inner solve's last cycle (0: the
   // whole cycle; -1: VP_IR_RECYCLE_K or the whole cycle)
-  int recycle_width = -1;
+  std::atomic<int> recycle_width{-1};
 };
 
 namespace vp {
