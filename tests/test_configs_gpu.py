@@ -212,12 +212,18 @@ def test_bragg20_batched_sweep_slice(vp_ctx):
             rhs.append(prob.gaussian_rhs(pol))
             probs.append(prob)
     B = np.array(rhs)
-    X, infos = vie.solve_ir_batched(ops, precs, B, tol=wl['tol'], restart=wl['restart'], inner_tol=wl['inner_tol'],
-                                    inner_maxit=wl['inner_maxit'], max_outer=wl['max_outer'], side='right')
-    # solo solve of the last problem: identical to its batched result
-    Xs, infos_s = vie.solve_ir_batched(ops[-1:], precs[-1:], B[-1:], tol=wl['tol'], restart=wl['restart'],
-                                       inner_tol=wl['inner_tol'], inner_maxit=wl['inner_maxit'],
-                                       max_outer=wl['max_outer'], side='right')
+    # the workload's recycled-space width (harmonic Ritz directions) on the operators' context
+    vp_ctx.set_recycle_width(wl.get('recycle_width', -1))
+    try:
+        X, infos = vie.solve_ir_batched(ops, precs, B, tol=wl['tol'], restart=wl['restart'],
+                                        inner_tol=wl['inner_tol'], inner_maxit=wl['inner_maxit'],
+                                        max_outer=wl['max_outer'], side='right')
+        # solo solve of the last problem: identical to its batched result
+        Xs, infos_s = vie.solve_ir_batched(ops[-1:], precs[-1:], B[-1:], tol=wl['tol'], restart=wl['restart'],
+                                           inner_tol=wl['inner_tol'], inner_maxit=wl['inner_maxit'],
+                                           max_outer=wl['max_outer'], side='right')
+    finally:
+        vp_ctx.set_recycle_width(-1)
     oracles = {}
     for i, prob in enumerate(probs):
         if prob.lam not in oracles:
