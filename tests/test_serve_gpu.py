@@ -194,3 +194,41 @@ def test_equal_size_different_masks_are_never_batched_together(vp_ctx):
             assert info['flag'] == 0 and rel(x, solo[i][0]) <= 1e-12 and info['relres'] == solo[i][2]
     finally:
         srv.close()
+
+
+def test_concurrent_groups_retire_only_their_own_requests(vp_ctx):
+    # six geometries (six groups) at once, so the executors run rounds side by
+    # side; the client collects each request as soon as it is done and drops
+    # its vectors at once, as a serving client does.  A request is reported
+    # done only after its own round has finished with it: every result equals
+    # its solo solve, in both waves of submissions.
+    from voxphoton_amd import vie
+    geos = [_notched(vp_ctx, x) for x in (3, 7, 11, 15, 19, 21)]
+    reqs = [g[:4] for g in geos]
+    solo = _solo(vp_ctx, reqs)
+    srv = vie.Server(hbm_budget_bytes=8 << 30, max_active=16, ctx=vp_ctx)
+    try:
+        for wave in range(2):
+            live = {}
+            for i, (op, P, db, n) in enumerate(reqs):
+                x = vie.DeviceArray(vp_ctx, n * 16)
+                live[srv.submit(op, P, db, x, **PARAMS)] = (i, x)
+            t0 = time.time()
+            while live and time.time() - t0 < 120:
+                for t in list(live):
+                    done, _ = srv.poll(t)
+                    if not done:
+                        continue
+                    info = srv.wait(t)
+                    i, x = live.pop(t)
+                    h = np.empty(reqs[i][3], complex)
+                    x.download(h)
+                    del x                      # the client drops the vector at once
+                    assert info['flag'] == 0 and info['relres'] == solo[i][2], (wave, i)
+                    assert rel(h, solo[i][0]) <= 1e-12, (wave, i)
+                time.sleep(0.001)
+            assert not live
+            st = srv.stats()
+            assert st['active'] == 0 and st['bytes_in_use'] == 0
+    finally:
+        srv.close()
