@@ -35,3 +35,9 @@ def test_no_device_calls_fail_cleanly():
         rc = _lib.lib.vp_ctx_create(0, ctypes.byref(h))
         assert rc != 0 and _lib.lib.vp_last_error()
     assert _lib.lib.vp_op_apply(None, 0, None, None, 0, 1) == _lib.VP_ERR_INVALID
+
+
+def test_recycle_width_rejects_a_missing_context():
+    from voxphoton_amd import _lib
+    assert _lib.lib.vp_ctx_set_recycle_width(None, 30) == _lib.VP_ERR_INVALID
+    assert b'ctx' in _lib.lib.vp_last_error()
