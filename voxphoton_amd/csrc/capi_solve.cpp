@@ -128,7 +128,7 @@ static void prec_alloc_prec(vp_prec_s* p, PrecPrec<T>& P) {
   P.tw.alloc(tw.size() * sizeof(C<T>));
   VP_HIP(hipMemcpy(P.tw.p, tw.data(), P.tw.bytes, hipMemcpyHostToDevice));
   P.work.alloc(size_t(3) * g.V * sizeof(C<T>));
-  if (g.level == 1) P.work2.alloc(size_t(3) * g.V * sizeof(C<T>));
+  if (g.level == 1 || g.qlay) P.work2.alloc(size_t(3) * g.V * sizeof(C<T>));
   P.xs.alloc(size_t(std::max<int64_t>(p->nbox, 1)) * sizeof(C<T>));
   P.ys.alloc(size_t(std::max<int64_t>(p->nbox, 1)) * sizeof(C<T>));
   const C<T>* bp = P.blocks.template as<C<T>>();
@@ -140,9 +140,9 @@ static void prec_alloc_prec(vp_prec_s* p, PrecPrec<T>& P) {
 // Load window of the binary16 block product, timed once per (device, block
 // geometry) in the process and shared by every preconditioner of that shape.
 struct QuKey {
-  int device, L, M, N, Hx, Hy, pct;
+  int device, L, M, N, Hx, Hy, pct, qlay;
   bool operator<(const QuKey& o) const {
-    return std::tie(device, L, M, N, Hx, Hy, pct) < std::tie(o.device, o.L, o.M, o.N, o.Hx, o.Hy, o.pct);
+    return std::tie(device, L, M, N, Hx, Hy, pct, qlay) < std::tie(o.device, o.L, o.M, o.N, o.Hx, o.Hy, o.pct, o.qlay);
   }
 };
 static std::mutex g_qu_mu;
@@ -150,7 +150,7 @@ static std::map<QuKey, int> g_qu;
 
 static int quadmul_window(vp_prec_s* p) {
   const PcGeom& g = p->g;
-  const QuKey key{p->ctx->device, g.L, g.M, g.N, g.Hx, g.Hy, g.pct};
+  const QuKey key{p->ctx->device, g.L, g.M, g.N, g.Hx, g.Hy, g.pct, g.qlay};
   {
     std::lock_guard<std::mutex> lk(g_qu_mu);
     auto it = g_qu.find(key);
@@ -316,10 +316,10 @@ void make_problem_set(ProblemSet<T>& ps, vp_op_s* const* ops, vp_prec_s* const* 
       VP_HIP(hipMemcpy(part->pblocks.p, bps.data(), part->pblocks.bytes, hipMemcpyHostToDevice));
       const int64_t wst = 3 * q0->g.V;
       part->pwork.alloc(size_t(wst) * np * sizeof(C<T>));
-      if (q0->g.level == 1) part->pwork2.alloc(size_t(wst) * np * sizeof(C<T>));
+      if (q0->g.level == 1 || q0->g.qlay) part->pwork2.alloc(size_t(wst) * np * sizeof(C<T>));
       PcArgs<T> a = prec_args<T>(q0, nullptr, nullptr);
       a.work = part->pwork.template as<C<T>>();
-      a.work2 = q0->g.level == 1 ? part->pwork2.template as<C<T>>() : nullptr;
+      a.work2 = (q0->g.level == 1 || q0->g.qlay) ? part->pwork2.template as<C<T>>() : nullptr;
       a.wstride = wst;
       a.blocks = part->pblocks.template as<const C<T>*>();
       a.hblocks = nullptr;
@@ -572,6 +572,10 @@ static void level2_block_range(vp_prec_s* p, const double2* dG) {
            pcrow_supported(g.M) && (size_t(g.L) + 64 + (size_t(g.L) + 63) / 64) * 16 <= size_t(lds_max))
               ? 1
               : 0;
+  // q-contiguous spectra for the block product (VP_PC_QLAY=0: gather the
+  // columns from the pencil-major spectra)
+  const char* eqc = std::getenv("VP_PC_QLAY");
+  g.qlay = (g.pct && !(eqc && eqc[0] == '0')) ? 1 : 0;
 }
 
 static int prec_level2_common(vp_ctx ctx, int64_t L, int64_t M, int64_t N, const double2* dG, double mc_re,

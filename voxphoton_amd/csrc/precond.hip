@@ -212,6 +212,15 @@ __global__ void __launch_bounds__(256) k_pc_blockmul(PcArgs<T> a, PcGeom g) {
   }
 }
 
+// Index of the spectrum entry of row q = c*N + n at (kx, ky) in the layout
+// the block products read: q-contiguous Y[kx][ky][q] (qlay, in work2),
+// pencil-major X[q][ky][kx] (pct) or plane-major X[kx][q][ky].
+__device__ __forceinline__ int64_t pc_spec_index(const PcGeom& g, int q, int ky, int kx, int n3) {
+  if (g.qlay) return (int64_t(kx) * g.M + ky) * n3 + q;
+  if (g.pct) return (int64_t(q) * g.M + ky) * g.L + kx;
+  return (int64_t(kx) * n3 + q) * g.M + ky;
+}
+
 // Quadrant block products (blocks stored for kx <= L/2, ky <= M/2): one
 // workgroup per stored block B(kxc, kyc) applies it to the up to four
 // columns (kx, ky) in {kxc, L-kxc} x {kyc, M-kyc} that share it, so every
@@ -221,7 +230,7 @@ __global__ void __launch_bounds__(256) k_pc_blockmul(PcArgs<T> a, PcGeom g) {
 template <typename T, int QH = 8>
 __global__ void __launch_bounds__(256) k_pc_quadmul(PcArgs<T> a, PcGeom g) {
   if (a.active && !a.active[blockIdx.y]) return;
-  C<T>* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
+  C<T>* __restrict__ work = (g.qlay ? a.work2 : a.work) + int64_t(blockIdx.y) * a.wstride;
   const C<T>* __restrict__ blocks = a.blocks[blockIdx.y];
   const int L = g.L, M = g.M, N = g.N, n3 = 3 * N, Hy = g.Hy;
   __shared__ C<T> Xs[4][3 * 85];
@@ -247,7 +256,7 @@ __global__ void __launch_bounds__(256) k_pc_quadmul(PcArgs<T> a, PcGeom g) {
     const int c = e / n3, q = e - c * n3;
     C<T> v = czero<T>();
     if (use[c]) {
-      v = work[g.pct ? (int64_t(q) * M + kys[c]) * L + kxs[c] : (int64_t(kxs[c]) * n3 + q) * M + kys[c]];
+      v = work[pc_spec_index(g, q, kys[c], kxs[c], n3)];
       const int cq = q / N;
       if ((cq == 0 && mxs[c]) || (cq == 1 && mys[c])) v = cneg(v);
     }
@@ -316,7 +325,7 @@ __global__ void __launch_bounds__(256) k_pc_quadmul(PcArgs<T> a, PcGeom g) {
       if (!use[c]) continue;
       C<T> v = accs[c];
       if ((cr == 0 && mxs[c]) || (cr == 1 && mys[c])) v = cneg(v);
-      work[g.pct ? (int64_t(r) * M + kys[c]) * L + kxs[c] : (int64_t(kxs[c]) * n3 + r) * M + kys[c]] = v;
+      work[pc_spec_index(g, r, kys[c], kxs[c], n3)] = v;
     }
   }
 }
@@ -335,7 +344,7 @@ typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
 __global__ void __launch_bounds__(256) k_pc_quadmul_dot2(PcArgs<float> a, PcGeom g) {
   if (a.active && !a.active[blockIdx.y]) return;
-  float2* __restrict__ work = a.work + int64_t(blockIdx.y) * a.wstride;
+  float2* __restrict__ work = (g.qlay ? a.work2 : a.work) + int64_t(blockIdx.y) * a.wstride;
   const int L = g.L, M = g.M, N = g.N, n3 = 3 * N, Hy = g.Hy;
   // XX[q][2c], XX[q][2c+1]: the two fp16 pairs of column c at entry q
   __shared__ uint32_t XX[3 * 85][8];
@@ -367,7 +376,7 @@ __global__ void __launch_bounds__(256) k_pc_quadmul_dot2(PcArgs<float> a, PcGeom
     if (e < nel) {
       const int c = e / n3, q = e - c * n3;
       if (use[c]) {
-        v = work[g.pct ? (int64_t(q) * M + kys[c]) * L + kxs[c] : (int64_t(kxs[c]) * n3 + q) * M + kys[c]];
+        v = work[pc_spec_index(g, q, kys[c], kxs[c], n3)];
         const int cq = q / N;
         if ((cq == 0 && mxs[c]) || (cq == 1 && mys[c])) v = make_float2(-v.x, -v.y);
       }
@@ -445,7 +454,7 @@ __global__ void __launch_bounds__(256) k_pc_quadmul_dot2(PcArgs<float> a, PcGeom
       const float f = hs / sc[c];
       float2 v = make_float2(ar[c] * f, ai[c] * f);
       if ((cr == 0 && mxs[c]) || (cr == 1 && mys[c])) v = make_float2(-v.x, -v.y);
-      work[g.pct ? (int64_t(r) * M + kys[c]) * L + kxs[c] : (int64_t(kxs[c]) * n3 + r) * M + kys[c]] = v;
+      work[pc_spec_index(g, r, kys[c], kxs[c], n3)] = v;
     }
   }
 }
@@ -671,6 +680,135 @@ __global__ void __launch_bounds__(256) k_pc_rows_ct(PcArgs<T> a, PcGeom g) {
   ct_fft<T, INV, MM, NB, LD, 256, 1, true, true, true, false, Rs...>(W0, W1, tw, 1, ld, st);
 }
 
+// Pencil-major spectra X[q][m][kx] (work) <-> q-contiguous Y[kx][m][q]
+// (work2), q = c*N + n < 3N: 32 x 32 tiles of one m through LDS, read and
+// written as 256-byte runs.  Grid: (kx tiles * q tiles, M, problems).
+template <typename T, bool TO_Q>
+__global__ void __launch_bounds__(256) k_pc_qlay(PcArgs<T> a, PcGeom g) {
+  const int p = blockIdx.z;
+  if (a.active && !a.active[p]) return;
+  const int L = g.L, M = g.M, n3 = 3 * g.N;
+  const int xt = (L + 31) / 32;
+  const int kx0 = int(blockIdx.x % unsigned(xt)) * 32, q0 = int(blockIdx.x / unsigned(xt)) * 32;
+  const int m = blockIdx.y;
+  C<T>* __restrict__ X = a.work + int64_t(p) * a.wstride;
+  C<T>* __restrict__ Y = a.work2 + int64_t(p) * a.wstride;
+  __shared__ C<T> tile[32][33];
+  const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;   // 32 x 8 threads
+  if (TO_Q) {
+    for (int r = ly; r < 32; r += 8) {
+      const int q = q0 + r, kx = kx0 + lx;
+      if (q < n3 && kx < L) tile[r][lx] = X[(int64_t(q) * M + m) * L + kx];
+    }
+    __syncthreads();
+    for (int r = ly; r < 32; r += 8) {
+      const int kx = kx0 + r, q = q0 + lx;
+      if (q < n3 && kx < L) Y[(int64_t(kx) * M + m) * n3 + q] = tile[lx][r];
+    }
+  } else {
+    for (int r = ly; r < 32; r += 8) {
+      const int kx = kx0 + r, q = q0 + lx;
+      if (q < n3 && kx < L) tile[lx][r] = Y[(int64_t(kx) * M + m) * n3 + q];
+    }
+    __syncthreads();
+    for (int r = ly; r < 32; r += 8) {
+      const int q = q0 + r, kx = kx0 + lx;
+      if (q < n3 && kx < L) X[(int64_t(q) * M + m) * L + kx] = tile[r][lx];
+    }
+  }
+}
+
+// One line of length R1*R2 (R2 = 1: a single radix) in registers:
+// x[n], n = R2*n1 + n2, becomes X[k1 + R1*k2] (R1-point DFTs over n1, twiddles
+// w^(n2*k1), R2-point DFTs over n2).
+template <typename T, bool INV, int R1, int R2>
+__device__ __forceinline__ void dft_line(C<T>* v) {
+  if constexpr (R2 == 1) {
+    dft<T, R1, INV>(v);
+  } else {
+    constexpr int MM = R1 * R2;
+    constexpr RootTable<MM> rt{};
+    C<T> y[MM];   // y[n2 * R1 + k1]
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      C<T> u[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) u[n1] = v[R2 * n1 + n2];
+      dft<T, R1, INV>(u);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const int e = (n2 * k1) % MM;
+        C<T> w;
+        w.x = T(rt.c[e]);
+        w.y = INV ? T(rt.s[e]) : T(-rt.s[e]);
+        y[n2 * R1 + k1] = e == 0 ? u[k1] : cmul(u[k1], w);
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      C<T> b[R2];
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) b[n2] = y[n2 * R1 + k1];
+      dft<T, R2, INV>(b);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+    }
+  }
+}
+
+// Row transforms (length M = R1*R2) fused with the change of layout around
+// the block product: forward X[q][m][kx] (work) -> Y[kx][m][q] (work2),
+// inverse Y -> X.  A workgroup takes QB rows q by KB columns kx, one line
+// per thread.  The pencil-major side is read or written straight from the
+// registers (a wavefront touches runs of KB consecutive kx); the q-contiguous
+// side goes through an LDS tile of the workgroup's lines, read or written as
+// runs of QB consecutive q.
+template <typename T, bool INV, int R1, int R2>
+__global__ void __launch_bounds__(256) k_pc_rows_q(PcArgs<T> a, PcGeom g) {
+  constexpr int MM = R1 * R2, QB = 16, KB = 16, LDK = KB + 1;
+  if (a.active && !a.active[blockIdx.y]) return;
+  C<T>* __restrict__ X = a.work + int64_t(blockIdx.y) * a.wstride;
+  C<T>* __restrict__ Y = a.work2 + int64_t(blockIdx.y) * a.wstride;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C<T>* S = reinterpret_cast<C<T>*>(smem_raw);   // S[(m * QB + ql) * LDK + kl]
+  const int L = g.L, n3 = 3 * g.N;
+  const int nkt = (L + KB - 1) / KB;
+  const int kx0 = int(blockIdx.x % unsigned(nkt)) * KB, q0 = int(blockIdx.x / unsigned(nkt)) * QB;
+  const int t = threadIdx.x;
+  const int ql = t / KB, kl = t % KB;
+  const int q = q0 + ql, kx = kx0 + kl;
+  const bool mine = q < n3 && kx < L;
+  C<T> v[MM];
+  if (!INV) {
+#pragma unroll
+    for (int m = 0; m < MM; ++m) v[m] = mine ? X[(int64_t(q) * MM + m) * L + kx] : czero<T>();
+    dft_line<T, false, R1, R2>(v);
+#pragma unroll
+    for (int m = 0; m < MM; ++m) S[(m * QB + ql) * LDK + kl] = v[m];
+    __syncthreads();
+    for (int e = t; e < MM * QB * KB; e += 256) {   // q fastest
+      const int eq = e % QB, r = e / QB, m = r % MM, ek = r / MM;
+      if (q0 + eq < n3 && kx0 + ek < L)
+        Y[(int64_t(kx0 + ek) * MM + m) * n3 + q0 + eq] = S[(m * QB + eq) * LDK + ek];
+    }
+  } else {
+    for (int e = t; e < MM * QB * KB; e += 256) {   // q fastest
+      const int eq = e % QB, r = e / QB, m = r % MM, ek = r / MM;
+      S[(m * QB + eq) * LDK + ek] = (q0 + eq < n3 && kx0 + ek < L)
+                                         ? Y[(int64_t(kx0 + ek) * MM + m) * n3 + q0 + eq]
+                                         : czero<T>();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < MM; ++m) v[m] = S[(m * QB + ql) * LDK + kl];
+    dft_line<T, true, R1, R2>(v);
+    if (mine) {
+#pragma unroll
+      for (int m = 0; m < MM; ++m) X[(int64_t(q) * MM + m) * L + kx] = v[m];
+    }
+  }
+}
+
 // max over entries of max(|re|, |im|) (non-negative floats order as their bits)
 __global__ void k_absmax_c64(const float2* __restrict__ a, int64_t n, unsigned* __restrict__ out) {
   float m = 0.f;
@@ -768,9 +906,39 @@ static void launch_prec_pct(const PcGeom& g, const PcArgs<T>& a, int nprob, hipS
     ProfScope ps(s, KID_PC_XFWD);
     xpass(false);
   }
+  // q-contiguous spectra for the block product: the row passes write and read
+  // them directly where the fused kernel fits (length with at most two
+  // radices, LDS within 64 KiB), else separate LDS transposes around the
+  // pencil-major row passes
+  const dim3 gt(unsigned(((g.L + 31) / 32) * ((3 * g.N + 31) / 32)), unsigned(g.M), unsigned(nprob));
+  const dim3 gr(unsigned(((g.L + 15) / 16) * ((3 * g.N + 15) / 16)), unsigned(nprob));
+  const size_t ldsq = size_t(g.M) * 16 * 17 * sizeof(C<T>);
+  auto rows_q = [&](bool inv) {
+    if (!g.qlay || ldsq > 65536) return false;
+    bool done = false;
+#define VP_PCQ(NL, R1, R2)                                                                              \
+  if (!done && g.M == NL) {                                                                             \
+    static_assert(NL == R1 * R2, "row length is the product of its radices");                           \
+    const void* fn = inv ? reinterpret_cast<const void*>(&k_pc_rows_q<T, true, R1, R2>)                 \
+                         : reinterpret_cast<const void*>(&k_pc_rows_q<T, false, R1, R2>);               \
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsq));               \
+    if (inv)                                                                                            \
+      hipLaunchKernelGGL((k_pc_rows_q<T, true, R1, R2>), gr, dim3(256), ldsq, s, a, g);                 \
+    else                                                                                                \
+      hipLaunchKernelGGL((k_pc_rows_q<T, false, R1, R2>), gr, dim3(256), ldsq, s, a, g);                \
+    done = true;                                                                                        \
+  }
+    VP_PCQ(11, 11, 1) VP_PCQ(22, 2, 11) VP_PCQ(10, 2, 5) VP_PCQ(12, 3, 4) VP_PCQ(20, 4, 5) VP_PCQ(24, 3, 8)
+    VP_PCQ(44, 4, 11) VP_PCQ(16, 4, 4) VP_PCQ(32, 4, 8)
+#undef VP_PCQ
+    return done;
+  };
   {
     ProfScope ps(s, KID_PC_YROWS);
-    rows(false);
+    if (!rows_q(false)) {
+      rows(false);
+      if (g.qlay) hipLaunchKernelGGL((k_pc_qlay<T, true>), gt, dim3(256), 0, s, a, g);
+    }
   }
   {
     ProfScope ps(s, KID_PC_BLOCK);
@@ -778,7 +946,10 @@ static void launch_prec_pct(const PcGeom& g, const PcArgs<T>& a, int nprob, hipS
   }
   {
     ProfScope ps(s, KID_PC_YROWS);
-    rows(true);
+    if (!rows_q(true)) {
+      if (g.qlay) hipLaunchKernelGGL((k_pc_qlay<T, false>), gt, dim3(256), 0, s, a, g);
+      rows(true);
+    }
   }
   {
     ProfScope ps(s, KID_PC_XINV);

@@ -28,6 +28,10 @@ struct PcGeom {
   // columns of block loads in flight per chunk of the binary16 block product
   // (8, 16 or 32; 0: 16), timed when the FP32 preconditioner is made
   int qu = 0;
+  // pct only: the block product runs over q-contiguous spectra Y[kx][m][q]
+  // (q = c*N + n; each mirrored column one contiguous run) in work2, filled
+  // from and returned to X by LDS-tiled transposes around it
+  int qlay = 0;
   size_t lds_row, lds_tile;
   size_t lds1, lds2;
   FftPlan px, py;       // unpadded DFTs of length L and M
@@ -39,7 +43,7 @@ struct PcArgs {
   C<T>* y;
   int64_t xstride, ystride;
   C<T>* work;           // 3 * L * M * N per problem
-  C<T>* work2;          // 1-level only: block product output
+  C<T>* work2;          // 1-level: block product output; 2-level qlay: q-contiguous spectra
   int64_t wstride;
   const C<T>* const* blocks;   // device array: blocks of problem p
   const int* pos;
