@@ -249,6 +249,37 @@ def test_level2_pencil_major_quadrant(vp_ctx, monkeypatch, shape, ct):
     assert e64 < 1e-10 and e32 < 2e-3
 
 
+@pytest.mark.parametrize('shape', [(100, 22, 3), (100, 48, 2), (200, 12, 2)])
+def test_q_contiguous_block_product_matches_gathers(vp_ctx, monkeypatch, shape):
+    # the block product over q-contiguous spectra (row transforms that change
+    # the layout: fused for M = 22 and 12 in FP32 and M = 12 in FP64, the
+    # pencil-major row pass with separate transposes for M = 22 in FP64, whose
+    # tile exceeds 64 KiB, and for M = 48, a length of three radices) equals
+    # the same preconditioner gathering its columns from the pencil-major
+    # spectra, and the oracle
+    from voxphoton_amd.vie import Preconditioner, assemble_toeplitz
+    L, M, N = shape
+    dx, ko = 20e-9, 5.85e6
+    G = assemble_toeplitz(shape, dx, ko, ctx=vp_ctx)
+    mc, gram = 0.8278, dx ** 3
+    mask = np.ones(shape, bool)
+    mask[::7, 1, :] = False
+    idx = ovie.idx_from_mask(mask)
+    rng = np.random.default_rng(L + 3 * M)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    monkeypatch.setenv('VP_PC_QLAY', '1')
+    Pq = Preconditioner.level2(G, mc, 1.0, gram, idx, ctx=vp_ctx)
+    monkeypatch.setenv('VP_PC_QLAY', '0')
+    Pg = Preconditioner.level2(G, mc, 1.0, gram, idx, ctx=vp_ctx)
+    Binv = opre.level_2_parallel_func(G, mc, 1.0, L, M, N, gram)
+    ref = opre.chan_2_mvp_for_parallel_idx(Binv, x, L, M, N, idx)
+    q64, g64 = Pq.apply(x), Pg.apply(x)
+    q32, g32 = Pq.apply(x, 'fp32'), Pg.apply(x, 'fp32')
+    print(shape, 'q vs gathers', rel(q64, g64), rel(q32, g32), 'oracle', rel(q64, ref), rel(q32, ref), flush=True)
+    assert rel(q64, g64) < 1e-13 and rel(q32, g32) < 1e-5
+    assert rel(q64, ref) < 1e-10 and rel(q32, ref) < 2e-3
+
+
 def test_level2_plane_larger_than_lds(vp_ctx):
     # 3N x M plane of 90 x 60 complex128 exceeds LDS: the split passes run
     from voxphoton_amd.vie import Preconditioner
