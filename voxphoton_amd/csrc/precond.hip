@@ -792,19 +792,20 @@ __global__ void __launch_bounds__(256) k_pc_rows_q(PcArgs<T> a, PcGeom g) {
         Y[(int64_t(kx0 + ek) * MM + m) * n3 + q0 + eq] = S[(m * QB + eq) * LDK + ek];
     }
   } else {
-    for (int e = t; e < MM * QB * KB; e += 256) {   // q fastest
-      const int eq = e % QB, r = e / QB, m = r % MM, ek = r / MM;
-      S[(m * QB + eq) * LDK + ek] = (q0 + eq < n3 && kx0 + ek < L)
-                                         ? Y[(int64_t(kx0 + ek) * MM + m) * n3 + q0 + eq]
-                                         : czero<T>();
-    }
-    __syncthreads();
+    // lines taken q-fastest, so a wavefront reads runs of QB consecutive q
+    const int iq = t % QB, ik = t / QB;
+    const bool mine_q = q0 + iq < n3 && kx0 + ik < L;
 #pragma unroll
-    for (int m = 0; m < MM; ++m) v[m] = S[(m * QB + ql) * LDK + kl];
+    for (int m = 0; m < MM; ++m)
+      v[m] = mine_q ? Y[(int64_t(kx0 + ik) * MM + m) * n3 + q0 + iq] : czero<T>();
     dft_line<T, true, R1, R2>(v);
-    if (mine) {
 #pragma unroll
-      for (int m = 0; m < MM; ++m) X[(int64_t(q) * MM + m) * L + kx] = v[m];
+    for (int m = 0; m < MM; ++m) S[(m * QB + iq) * LDK + ik] = v[m];
+    __syncthreads();
+    for (int e = t; e < MM * QB * KB; e += 256) {   // kx fastest
+      const int ek = e % KB, r = e / KB, eq = r % QB, m = r / QB;
+      if (q0 + eq < n3 && kx0 + ek < L)
+        X[(int64_t(q0 + eq) * MM + m) * L + kx0 + ek] = S[(m * QB + eq) * LDK + ek];
     }
   }
 }
